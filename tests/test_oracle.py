@@ -1,0 +1,247 @@
+"""The CPU oracle pinned against independent evidence (no GPU).
+
+BLAKE3 and every stream built on it against libLLVM's implementation (tests/golden); the field
+products against their bit-by-bit definitions (pcs/tests/fields.rs:33-72) and algebraic identities;
+eq expansions and the switch's opened value against brute force; the challenge norm against the
+naive Horner evaluation (pcs/src/challenge.rs:627-645); the proof against the checker's identities,
+the published constants and tampering.
+"""
+import json
+import os
+import random
+import struct
+
+import numpy as np
+import pytest
+
+import oracle_ctypes as O
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "blake3_vectors.json")))
+KEY = b"whats the Elvish word for friend"
+CTX = b"BLAKE3 2019-12-27 16:29:52 test vectors context"
+TEST_INPUT = bytes(i % 251 for i in range(102400))
+
+
+def test_blake3_against_libllvm_vectors():
+    for h in GOLD["hash"]:
+        k = {0: b"", 1: KEY, 2: CTX}[h["mode"]]
+        assert O.blake3(h["mode"], k, TEST_INPUT[: h["len"]], 0, 131).hex() == h["out0"], h
+        assert O.blake3(h["mode"], k, TEST_INPUT[: h["len"]], 1000, 77).hex() == h["out1000"], h
+
+
+def test_published_blake3_empty_digests():
+    # The BLAKE3 specification's test vectors for the empty input.
+    assert O.blake3(0, b"", b"", 0, 32).hex().startswith("af1349b9f5f9a1a6a0404dea36dcc949")
+    assert O.blake3(1, KEY, b"", 0, 32).hex().startswith("92b2b75604ed3c761f9d6f62392c8a92")
+
+
+def test_table_prefix_and_tail():
+    for t in GOLD["table"]:
+        seed = bytes.fromhex(t["seed"])
+        tab = O.table(seed, 20)
+        assert tab[:4096].tobytes().hex() == t["prefix"]
+        assert tab[131008:131072].tobytes().hex() == t["at_131008"]
+        ctx = b"labinius-fast/binary-table/v1"
+        assert O.blake3(1, seed, ctx, (1 << 28) - 64, 64).hex() == t["at_2pow28_minus_64"]
+
+
+def test_transcript_derivations():
+    for t in GOLD["transcript"]:
+        n = t["n"]
+        b, f = O.transcript_probe(bytes.fromhex(t["domain"]), bytes.fromhex(t["msg"]), bytes.fromhex(t["label"]), n)
+        raw = bytes.fromhex(t["b128_second"])
+        want = np.frombuffer(raw, dtype="<u8").reshape(n, 2)
+        assert (b == want).all()
+        raw = np.frombuffer(bytes.fromhex(t["f162_raw"]), dtype="<u8").reshape(n, 3).copy()
+        raw[:, 2] &= (1 << 34) - 1
+        assert (f == raw).all()
+
+
+def _draw(stream: bytes, q: int):
+    """The matrix draw rule (verifier/proof_verify/src/rq.rs:114-144) over a given byte stream."""
+    bits = (q - 1).bit_length()
+    width = (bits + 7) // 8
+    out, at = [], 0
+    while len(out) < 648 and at + width <= len(stream):
+        v = int.from_bytes(stream[at: at + width], "little") & ((1 << bits) - 1)
+        at += width
+        if v < q:
+            out.append(v)
+    return out
+
+
+def test_matrix_expansion_stream():
+    for m in GOLD["matrix_stream"]:
+        q = int(m["q"])
+        want = _draw(bytes.fromhex(m["stream"]), q)
+        got = O.ajtai_element(q, m["elements"], m["index"])
+        assert got[: len(want)] == want
+        assert all(0 <= x < q for x in got)
+
+
+def test_sign_words():
+    lib = O.lib()
+    for s in GOLD["signs"]:
+        c = O.Challenge()
+        c.weight = len(s["positions"])
+        for i, p in enumerate(s["positions"]):
+            c.positions[i] = p
+        assert lib.orc_sign_bits(O.ctypes.byref(c)) == s["word"] & ((1 << c.weight) - 1)
+
+
+def _rand(rng, bits):
+    return [rng.getrandbits(64) for _ in range(bits // 64)] + ([rng.getrandbits(bits % 64)] if bits % 64 else [])
+
+
+def test_field_products_match_bitwise_definitions():
+    rng = random.Random(5)
+    for _ in range(200):
+        a, b = _rand(rng, 128), _rand(rng, 128)
+        assert (O.b128_mul(a, b) == O.b128_mul(a, b, bitwise=True)).all()
+        a, b = _rand(rng, 162), _rand(rng, 162)
+        assert (O.f162_mul(a, b) == O.f162_mul(a, b, bitwise=True)).all()
+    # edge operands: zero, one, all ones
+    for a in ([0, 0], [1, 0], [2**64 - 1, 2**64 - 1]):
+        for b in ([0, 0], [1, 0], [2**64 - 1, 2**64 - 1]):
+            assert (O.b128_mul(a, b) == O.b128_mul(a, b, bitwise=True)).all()
+
+
+def _pow(mul, x, e, one):
+    r = np.array(one, dtype=np.uint64)
+    b = np.array(x, dtype=np.uint64)
+    while e:
+        if e & 1:
+            r = mul(r, b)
+        b = mul(b, b)
+        e >>= 1
+    return r
+
+
+def test_field_identities():
+    # x has order 243 in F162^* (pcs/tests/fields.rs:115-127)
+    x = [2, 0, 0]
+    acc = np.array([1, 0, 0], dtype=np.uint64)
+    hits = []
+    for k in range(1, 244):
+        acc = O.f162_mul(acc, x)
+        if list(acc) == [1, 0, 0]:
+            hits.append(k)
+    assert hits == [243]
+    # x^128 = x^7 + x^2 + x + 1 in the GHASH basis
+    x64 = [0, 1]
+    assert list(O.b128_mul(x64, x64)) == [0x87, 0]
+    # Frobenius: a^(2^128) = a, a^(2^162) = a
+    rng = random.Random(9)
+    for _ in range(3):
+        a = _rand(rng, 128)
+        r = np.array(a, dtype=np.uint64)
+        for _ in range(128):
+            r = O.b128_mul(r, r)
+        assert list(r) == a
+        a = _rand(rng, 162)
+        r = np.array(a, dtype=np.uint64)
+        for _ in range(162):
+            r = O.f162_mul(r, r)
+        assert list(r) == a
+
+
+def test_fold_cap_constants():
+    # verifier/proof_verify/src/main.rs:476-480
+    cap = O.lib().orc_fold_cap(31)
+    assert cap == 293_534_171_136
+    assert int(cap ** 0.5) == 541_787
+
+
+def test_proof_length_at_challenge_shape():
+    # solution/proof_generate/src/main.rs:2335
+    assert O.proof_len(31, O.DEFAULT_Q) == 13_958_612
+
+
+def test_challenge_norms_and_signs():
+    ch = O.challenges_probe(b"labinius/test-challenges", b"prefix", 40)
+    lib = O.lib()
+    for c in ch:
+        pos = list(c.positions[: c.weight])
+        assert c.weight == 28 and pos == sorted(set(pos)) and max(pos) < 162
+        fast = lib.orc_canonical_inf_norm_sq(O.ctypes.byref(c))
+        naive = lib.orc_canonical_inf_norm_sq_naive(O.ctypes.byref(c))
+        assert fast <= 144 + 1e-12
+        assert abs(fast - naive) <= 1e-9 * max(1.0, naive)
+        assert c.signs == lib.orc_sign_bits(O.ctypes.byref(c))
+
+
+@pytest.fixture(scope="module")
+def proof20():
+    seed = bytes([42] * 32)
+    return seed, O.prove(seed, 20)
+
+
+def test_prove_verify_k20(proof20):
+    seed, p = proof20
+    assert len(p) == O.proof_len(20)
+    assert O.verify(seed, 20, p) is None
+    assert O.verify(bytes([43] * 32), 20, p) is not None
+
+
+@pytest.mark.parametrize("offset_from", ["commitment", "partial", "round", "opened", "row", "fold_lo"])
+def test_tampering_is_rejected(proof20, offset_from):
+    seed, p = proof20
+    w = O.coefficient_bytes(O.DEFAULT_Q)
+    clen = 1024 * 648 * w
+    at = {"commitment": 28 + 1000, "partial": 28 + clen + 8 + 16 * 5, "round": 28 + clen + 8 + 2048 + 8 + 24,
+          "opened": 28 + clen + 8 + 2048 + 8 + 13 * 48, "row": 28 + clen + 8 + 2048 + 8 + 13 * 48 + 24 + 8 + 7,
+          "fold_lo": len(p) - 4 * 100}[offset_from]
+    bad = bytearray(p)
+    bad[at] ^= 1
+    assert O.verify(seed, 20, bytes(bad)) is not None
+
+
+def test_opened_is_the_lifted_evaluation(proof20):
+    """opened = sum_j eq_expand_f162(coords)[j] lift(w_j) (crossfield.rs:691-696)."""
+    seed, p = proof20
+    r_hi, batch, coords, ch = O.replay(seed, 20, p)
+    tab = O.table(seed, 20)
+    eq = np.zeros((1 << 13, 3), dtype=np.uint64)
+    O.lib().orc_eq_expand_f162(O._p(np.ascontiguousarray(coords)), 13, O._p(eq))
+    words = tab.view("<u8").reshape(-1, 2)
+    acc = np.zeros(3, dtype=np.uint64)
+    for j in range(1 << 13):
+        acc ^= O.f162_mul(eq[j], [words[j, 0], words[j, 1], 0])
+    end = len(p) - (8 + 4 * 648 * 2) - (8 + 20736)
+    opened = np.frombuffer(p[end - 24: end], dtype="<u8")
+    assert (acc == opened).all()
+
+
+def test_eq_expansion_brute_force():
+    rng = random.Random(3)
+    r = np.array([_rand(rng, 128) for _ in range(5)], dtype=np.uint64)
+    eq = np.zeros((32, 2), dtype=np.uint64)
+    O.lib().orc_eq_expand_b128(O._p(r), 5, O._p(eq))
+    for j in range(32):
+        acc = np.array([1, 0], dtype=np.uint64)
+        for i in range(5):
+            bit = (j >> (4 - i)) & 1  # r[0] is the most significant bit of the index
+            f = r[i] if bit else r[i] ^ np.array([1, 0], dtype=np.uint64)
+            acc = O.b128_mul(acc, f)
+        assert (acc == eq[j]).all()
+
+
+def test_commitment_of_one_bit_is_a_shifted_row():
+    """A table with one set bit commits to X^i A_m mod (X^648 - X^324 + 1) in its column."""
+    q = O.DEFAULT_Q
+    tab = np.zeros(1 << 17, dtype=np.uint8)
+    column, m, k, p = 3, 1, 2, 100          # bit p of word 4m+k of column 3
+    word = column * 8 + 4 * m + k
+    tab[16 * word + p // 8] |= 1 << (p % 8)
+    got = O.commit_columns(tab, 20, q, column, 1)
+    a = O.ajtai_element(q, 2, m)
+    shift = 4 * p + k
+    raw = [0] * 1295
+    for t in range(648):
+        raw[t + shift] += a[t]
+    for t in range(1294, 647, -1):
+        raw[t - 324] += raw[t]
+        raw[t - 648] -= raw[t]
+        raw[t] = 0
+    want = b"".join((x % q).to_bytes(5, "little") for x in raw[:648])
+    assert got.tobytes() == want

@@ -1,0 +1,123 @@
+/* labinius_gpu.h — C ABI of the MI355X Labinius prover (liblabinius_gpu.so).
+ *
+ * The entry points replace the kernel calls the reference prover's `prove()` makes
+ * (solution/proof_generate/src/main.rs:1692-2009) and the whole-proof executable contract
+ * (`labinius-fast --prove P --table T [--modulus Q]`, main.rs:2050-2090). Plain pointers and
+ * sizes only; every buffer is caller-owned host memory unless a name says "device". Every call
+ * returns 0 on success or a negative LBN_E_* code; lbn_last_error() describes the last failure
+ * on that context. No call aborts the process.
+ *
+ * Threading: one context per host thread; calls on one context are serial (the Fiat-Shamir
+ * order forces it). Contexts are independent and may run concurrently.
+ *
+ * Shape: log_vars = k in [19, 31]; k = 31 is the challenge shape (2^24 GHASH words, 1024
+ * columns of 16,384 words, 4096 ring elements per column). Smaller k keep 1024 columns and
+ * shrink the column length; they are this library's generalisation for tests and batching.
+ */
+#ifndef LABINIUS_GPU_H
+#define LABINIUS_GPU_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LBN_OK 0
+#define LBN_E_ARG (-1)     /* invalid argument or shape */
+#define LBN_E_STATE (-2)   /* call out of order (no table, no modulus, no partials, ...) */
+#define LBN_E_DEVICE (-3)  /* HIP runtime error */
+#define LBN_E_MODULUS (-4) /* modulus outside [34,816,965,049, 2^96) */
+#define LBN_E_RANGE (-5)   /* a fold coefficient left i16 (cannot happen for honest input) */
+#define LBN_E_IO (-6)      /* file I/O (CLI helpers) */
+
+typedef struct lbn_ctx lbn_ctx;
+
+/* A weight-w signed challenge of R_162: coefficient positions[i] is -1 when bit i of signs is
+ * set, +1 otherwise; positions sorted and distinct (pcs/src/challenge.rs:396-406). */
+typedef struct {
+    uint8_t positions[32];
+    uint32_t signs;
+    uint32_t weight;
+} lbn_challenge;
+
+/* Per-phase wall times of the last lbn_prove, microseconds (the reference prints the same split
+ * as its `labinius-profile` line, main.rs:1999-2008). */
+typedef struct {
+    double commit_us, transcript_us, partials_us, switch_us, row_us, challenges_us, fold_us,
+        encode_us, total_us;
+} lbn_profile;
+
+/* Context on HIP device `device` for shape `log_vars`: allocates the instance's HBM. */
+int lbn_create(int device, int log_vars, lbn_ctx **out);
+void lbn_destroy(lbn_ctx *ctx);
+const char *lbn_last_error(const lbn_ctx *ctx);
+
+/* The witness table: 2^log_vars / 8 bytes, word j = u128 LE of bytes 16j..16j+16
+ * (Table::load, main.rs:387-429; decode_table, verifier/proof_verify/src/main.rs:143-153).
+ * lbn_load_table copies host memory to HBM; lbn_load_table_device copies from device memory. */
+int lbn_load_table(lbn_ctx *ctx, const void *host_bytes, size_t bytes);
+int lbn_load_table_device(lbn_ctx *ctx, const void *device_bytes, size_t bytes);
+/* Fill the table on the device from a 32-byte seed: the BLAKE3 XOF the trusted table writer
+ * produces (verifier/table_writer/src/main.rs:14-22). */
+int lbn_generate_table(lbn_ctx *ctx, const uint8_t seed[32]);
+
+/* Commitment modulus Q (any value in [34,816,965,049, 2^96)); builds the public matrix
+ * rq::ajtai_matrix(Q, ring_elements) (verifier/proof_verify/src/rq.rs:114-144) and its
+ * transform into HBM. Public data: independent of the seed. */
+int lbn_set_modulus(lbn_ctx *ctx, uint64_t q_lo, uint64_t q_hi);
+
+/* Commitment A*W mod Q of the loaded table, packed ceil(bits(Q-1)/8) bytes LE per coefficient,
+ * column-major (column c coefficient t at c*648+t). Replaces commit_columns_rows
+ * (pcs/src/simd/sms_commit.rs:1473) + commitment_bytes_sms (main.rs:1080-1127). */
+int lbn_commit(lbn_ctx *ctx, uint8_t *out, size_t out_len);
+
+/* The 128 GHASH partial evaluations at r_hi (k-7 B128, as u64 pairs), and the eq expansion kept
+ * on the device for the switch. Replaces eq_expand_and_partials_factored_blocks +
+ * partials_from_sets (pcs/src/fields/crossfield.rs:1136, 1451). */
+int lbn_partials(lbn_ctx *ctx, const uint64_t *r_hi, uint64_t *partials);
+
+/* The GHASH->F162 switch sumcheck over the loaded table (crossfield.rs LeanSwitch 2168-2549,
+ * semantics SwitchProver 563-623; driven as main.rs:948-992): begin with the 128 batch weights
+ * (F162 as 3 u64), then per round msg -> [e0, einf] and fold(r); final -> opened. */
+int lbn_switch_begin(lbn_ctx *ctx, const uint64_t *batch);
+int lbn_switch_msg(lbn_ctx *ctx, uint64_t *msg);
+int lbn_switch_fold(lbn_ctx *ctx, const uint64_t *r);
+int lbn_switch_final(lbn_ctx *ctx, uint64_t *opened);
+
+/* Row evaluation row[c] = sum_i eq(p0,i) lift(w[c*len+i]) for the 1024 columns, p0 with
+ * k-17 F162 coordinates (eval.rs row_evaluate_words 426-480). row: 1024 x 3 u64. */
+int lbn_row_eval(lbn_ctx *ctx, const uint64_t *p0, int n_p0, uint64_t *row);
+
+/* The exact integer fold v_m = sum_j c_j(-X^4) W_{j,m} over the ring elements, as i32 LE
+ * (fold_witness_bitsliced_into, pcs/src/fold.rs:1370; reference.rs:78-115). n_challenges = 1024,
+ * out holds ring_elements*648 values. */
+int lbn_fold(lbn_ctx *ctx, const lbn_challenge *challenges, int n_challenges, int32_t *out,
+             size_t n_out);
+
+/* The whole prover (main.rs prove 1692-2009; reference.rs:137-177): commit, transcript,
+ * partials, switch, row, folding challenges, fold, bincode proof. The table and modulus must be
+ * loaded. proof_len is the exact proof size; prof may be NULL. */
+size_t lbn_proof_len(int log_vars, uint64_t q_lo, uint64_t q_hi);
+int lbn_prove(lbn_ctx *ctx, const uint8_t seed[32], uint8_t *proof, size_t cap, size_t *len,
+              lbn_profile *prof);
+
+/* Number of CRT primes the commitment uses for the current modulus (diagnostics). */
+int lbn_commit_primes(const lbn_ctx *ctx);
+
+/* Kernel timing (diagnostics): when enabled, every pass records HIP events on the context's own
+ * stream around its kernels; lbn_get_timing returns the milliseconds accumulated since the last
+ * lbn_reset_timing. No counterpart in the reference (it prints wall-clock phases only). */
+typedef struct {
+    double table_ms, eq_ms, partials_ms, switch0_ms, switch_rounds_ms, row_ms, fold_ms, commit_mac_ms,
+        commit_finish_ms;
+    int switch_rounds;
+} lbn_timing;
+int lbn_set_timing(lbn_ctx *ctx, int enable);
+int lbn_reset_timing(lbn_ctx *ctx);
+int lbn_get_timing(const lbn_ctx *ctx, lbn_timing *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,205 @@
+"""Python view of liblabinius_gpu, the MI355X Labinius prover (C ABI in include/labinius_gpu.h).
+
+The methods mirror the reference prover's kernel entry points one to one (commit, partials,
+switch rounds, row evaluation, fold; solution/proof_generate/src/main.rs:1692-2009) and the whole
+prover. Every call goes to the HIP library; there is no CPU fallback: if the library is missing
+the import-time loader raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "build", "liblabinius_gpu.so")
+CLI_PATH = os.path.join(HERE, "build", "labinius-fast")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "labinius_gpu.h")
+DEFAULT_Q = 35_039_150_080  # solution/proof_generate/src/parameters.rs:14
+
+ERRORS = {-1: "argument", -2: "state", -3: "device", -4: "modulus", -5: "range", -6: "io"}
+
+
+class LbnError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)} error: {msg}")
+        self.code = code
+
+
+class Challenge(ctypes.Structure):
+    _fields_ = [("positions", ctypes.c_uint8 * 32), ("signs", ctypes.c_uint32), ("weight", ctypes.c_uint32)]
+
+
+class Profile(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("commit_us", "transcript_us", "partials_us", "switch_us", "row_us",
+                                              "challenges_us", "fold_us", "encode_us", "total_us")]
+
+
+class Timing(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("table_ms", "eq_ms", "partials_ms", "switch0_ms", "switch_rounds_ms",
+                                              "row_ms", "fold_ms", "commit_mac_ms", "commit_finish_ms")] + \
+               [("switch_rounds", ctypes.c_int)]
+
+
+_lib = None
+
+
+def lib():
+    """The loaded HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.c_void_p
+        L.lbn_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(P)]
+        L.lbn_destroy.argtypes = [P]
+        L.lbn_last_error.argtypes = [P]
+        L.lbn_last_error.restype = ctypes.c_char_p
+        L.lbn_load_table.argtypes = [P, P, ctypes.c_size_t]
+        L.lbn_load_table_device.argtypes = [P, P, ctypes.c_size_t]
+        L.lbn_generate_table.argtypes = [P, P]
+        L.lbn_set_modulus.argtypes = [P, ctypes.c_uint64, ctypes.c_uint64]
+        L.lbn_commit.argtypes = [P, P, ctypes.c_size_t]
+        L.lbn_partials.argtypes = [P, P, P]
+        L.lbn_switch_begin.argtypes = [P, P]
+        L.lbn_switch_msg.argtypes = [P, P]
+        L.lbn_switch_fold.argtypes = [P, P]
+        L.lbn_switch_final.argtypes = [P, P]
+        L.lbn_row_eval.argtypes = [P, P, ctypes.c_int, P]
+        L.lbn_fold.argtypes = [P, P, ctypes.c_int, P, ctypes.c_size_t]
+        L.lbn_proof_len.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64]
+        L.lbn_proof_len.restype = ctypes.c_size_t
+        L.lbn_prove.argtypes = [P, P, P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t), P]
+        L.lbn_commit_primes.argtypes = [P]
+        L.lbn_set_timing.argtypes = [P, ctypes.c_int]
+        L.lbn_reset_timing.argtypes = [P]
+        L.lbn_get_timing.argtypes = [P, P]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _b(b: bytes):
+    return ctypes.cast(ctypes.c_char_p(b), ctypes.c_void_p)
+
+
+def proof_len(log_vars: int, q: int = DEFAULT_Q) -> int:
+    return lib().lbn_proof_len(log_vars, q & (2**64 - 1), q >> 64)
+
+
+def coefficient_bytes(q: int) -> int:
+    return ((q - 1).bit_length() + 7) // 8
+
+
+class Prover:
+    """One lbn_ctx: one table resident in HBM on one GPU."""
+
+    def __init__(self, log_vars: int = 31, device: int = 0):
+        self.log_vars = log_vars
+        self.log_packed = log_vars - 7
+        self.row_log = self.log_packed - 10
+        self.ring_elements = (1 << self.row_log) // 4
+        self.table_bytes = (1 << log_vars) // 8
+        self.q = None
+        h = ctypes.c_void_p()
+        rc = lib().lbn_create(device, log_vars, ctypes.byref(h))
+        if rc != 0:
+            raise LbnError(rc, "lbn_create failed")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().lbn_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != 0:
+            raise LbnError(rc, lib().lbn_last_error(self.h).decode())
+
+    def load_table(self, table: np.ndarray):
+        t = np.ascontiguousarray(table, dtype=np.uint8)
+        self._check(lib().lbn_load_table(self.h, _p(t), t.size))
+
+    def generate_table(self, seed: bytes):
+        self._check(lib().lbn_generate_table(self.h, _b(seed)))
+
+    def set_modulus(self, q: int = DEFAULT_Q):
+        self._check(lib().lbn_set_modulus(self.h, q & (2**64 - 1), q >> 64))
+        self.q = q
+
+    def commit(self) -> bytes:
+        n = 1024 * 648 * coefficient_bytes(self.q)
+        out = np.zeros(n, dtype=np.uint8)
+        self._check(lib().lbn_commit(self.h, _p(out), n))
+        return out.tobytes()
+
+    def partials(self, r_hi: np.ndarray) -> np.ndarray:
+        r = np.ascontiguousarray(r_hi, dtype=np.uint64)
+        out = np.zeros((128, 2), dtype=np.uint64)
+        self._check(lib().lbn_partials(self.h, _p(r), _p(out)))
+        return out
+
+    def switch_begin(self, batch: np.ndarray):
+        self._batch = np.ascontiguousarray(batch, dtype=np.uint64)
+        self._check(lib().lbn_switch_begin(self.h, _p(self._batch)))
+
+    def switch_msg(self) -> np.ndarray:
+        out = np.zeros((2, 3), dtype=np.uint64)
+        self._check(lib().lbn_switch_msg(self.h, _p(out)))
+        return out
+
+    def switch_fold(self, r: np.ndarray):
+        rr = np.ascontiguousarray(r, dtype=np.uint64)
+        self._check(lib().lbn_switch_fold(self.h, _p(rr)))
+
+    def switch_final(self) -> np.ndarray:
+        out = np.zeros(3, dtype=np.uint64)
+        self._check(lib().lbn_switch_final(self.h, _p(out)))
+        return out
+
+    def row_eval(self, p0: np.ndarray) -> np.ndarray:
+        p = np.ascontiguousarray(p0, dtype=np.uint64)
+        out = np.zeros((1024, 3), dtype=np.uint64)
+        self._check(lib().lbn_row_eval(self.h, _p(p), p.shape[0], _p(out)))
+        return out
+
+    def fold(self, challenges) -> np.ndarray:
+        out = np.zeros((self.ring_elements, 648), dtype=np.int32)
+        self._check(lib().lbn_fold(self.h, ctypes.cast(challenges, ctypes.c_void_p), 1024, _p(out), out.size))
+        return out
+
+    def prove(self, seed: bytes):
+        cap = proof_len(self.log_vars, self.q)
+        out = np.zeros(cap, dtype=np.uint8)
+        n = ctypes.c_size_t(0)
+        prof = Profile()
+        self._check(lib().lbn_prove(self.h, _b(seed), _p(out), cap, ctypes.byref(n), ctypes.addressof(prof)))
+        return out[: n.value].tobytes(), {f: getattr(prof, f) for f, _ in Profile._fields_}
+
+    def commit_primes(self) -> int:
+        return lib().lbn_commit_primes(self.h)
+
+    def set_timing(self, enable=True):
+        self._check(lib().lbn_set_timing(self.h, 1 if enable else 0))
+        self._check(lib().lbn_reset_timing(self.h))
+
+    def timing(self) -> dict:
+        t = Timing()
+        self._check(lib().lbn_get_timing(self.h, ctypes.addressof(t)))
+        return {f: getattr(t, f) for f, _ in Timing._fields_}
+
+
+def exported_symbols():
+    """Names of the functions include/labinius_gpu.h declares."""
+    import re
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(lbn_[a-z_0-9]+)\s*\(", text)))

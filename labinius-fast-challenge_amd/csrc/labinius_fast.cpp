@@ -1,0 +1,97 @@
+// labinius-fast: the prover executable, a drop-in for the reference's
+//   labinius-fast --prove PROOF --table TABLE [--modulus Q]        (main.rs:2050-2090)
+// with the seed in LABINIUS_SEED (64 hex characters, main.rs:76-80). It reads the witness table
+// (exactly 2^k/8 bytes), proves on the GPU through liblabinius_gpu and writes the bincode proof.
+// Extensions: LABINIUS_LOG_VARS (default 31) selects a smaller shape; LABINIUS_DEVICE the GPU.
+// Any failure exits non-zero with a message, as the reference's aborts do.
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/labinius_gpu.h"
+
+typedef unsigned __int128 u128;
+
+static void die(const char *msg) {
+    fprintf(stderr, "labinius-fast: %s\n", msg);
+    exit(101);
+}
+
+static bool parse_seed(const char *s, uint8_t out[32]) {
+    if (!s || strlen(s) != 64) return false;
+    for (int i = 0; i < 32; i++) {
+        unsigned v;
+        if (sscanf(s + 2 * i, "%2x", &v) != 1) return false;
+        out[i] = (uint8_t)v;
+    }
+    return true;
+}
+
+static bool parse_u128(const char *s, u128 *out) {
+    while (*s == ' ' || *s == '\t' || *s == '\n') s++;
+    u128 v = 0;
+    int digits = 0;
+    for (; *s >= '0' && *s <= '9'; s++, digits++) {
+        u128 n = v * 10 + (u128)(*s - '0');
+        if (n / 10 != v) return false;
+        v = n;
+    }
+    while (*s == ' ' || *s == '\t' || *s == '\n') s++;
+    if (!digits || *s) return false;
+    *out = v;
+    return true;
+}
+
+int main(int argc, char **argv) {
+    const auto start = std::chrono::steady_clock::now();
+    if (!(argc == 5 || argc == 7)) die("usage: labinius-fast --prove FILE --table FILE [--modulus Q]");
+    if (strcmp(argv[1], "--prove")) die("unknown command");
+    if (strcmp(argv[3], "--table")) die("witness table path is required");
+    uint8_t seed[32];
+    if (!parse_seed(getenv("LABINIUS_SEED"), seed)) die("LABINIUS_SEED must be 64 hex characters");
+    u128 q = (u128)35039150080ull;  // solution/proof_generate/src/parameters.rs:14
+    if (argc == 7) {
+        if (strcmp(argv[5], "--modulus")) die("unknown prover option");
+        if (!parse_u128(argv[6], &q)) die("invalid commitment modulus");
+    }
+    const int k = getenv("LABINIUS_LOG_VARS") ? atoi(getenv("LABINIUS_LOG_VARS")) : 31;
+    const int device = getenv("LABINIUS_DEVICE") ? atoi(getenv("LABINIUS_DEVICE")) : 0;
+    if (k < 19 || k > 31) die("LABINIUS_LOG_VARS must be in [19, 31]");
+    const size_t table_bytes = ((size_t)1 << k) / 8;
+
+    FILE *f = fopen(argv[4], "rb");
+    if (!f) die("cannot open the witness table");
+    std::vector<uint8_t> table(table_bytes + 1);
+    size_t got = fread(table.data(), 1, table_bytes + 1, f);
+    fclose(f);
+    if (got != table_bytes) die("witness table has the wrong length");
+    const auto loaded = std::chrono::steady_clock::now();
+
+    lbn_ctx *ctx = nullptr;
+    if (lbn_create(device, k, &ctx) != LBN_OK) die("cannot create the GPU context");
+    if (lbn_load_table(ctx, table.data(), table_bytes) != LBN_OK) die(lbn_last_error(ctx));
+    if (lbn_set_modulus(ctx, (uint64_t)q, (uint64_t)(q >> 64)) != LBN_OK) die(lbn_last_error(ctx));
+    const auto setup = std::chrono::steady_clock::now();
+    const size_t cap = lbn_proof_len(k, (uint64_t)q, (uint64_t)(q >> 64));
+    std::vector<uint8_t> proof(cap);
+    size_t len = 0;
+    lbn_profile prof{};
+    if (lbn_prove(ctx, seed, proof.data(), cap, &len, &prof) != LBN_OK) die(lbn_last_error(ctx));
+    const auto proved = std::chrono::steady_clock::now();
+    FILE *o = fopen(argv[2], "wb");
+    if (!o || fwrite(proof.data(), 1, len, o) != len || fclose(o) != 0) die("cannot write the proof");
+    const auto written = std::chrono::steady_clock::now();
+    auto us = [](auto a, auto b) { return (long long)std::chrono::duration_cast<std::chrono::microseconds>(b - a).count(); };
+    fprintf(stderr,
+            "labinius-profile load_us=%lld setup_us=%lld commit_us=%.0f transcript_us=%.0f partials_us=%.0f "
+            "switch_us=%.0f row_us=%.0f challenges_us=%.0f fold_us=%.0f encode_us=%.0f prove_us=%.0f write_us=%lld "
+            "total_us=%lld commit_primes=%d\n",
+            us(start, loaded), us(loaded, setup), prof.commit_us, prof.transcript_us, prof.partials_us, prof.switch_us,
+            prof.row_us, prof.challenges_us, prof.fold_us, prof.encode_us, prof.total_us, us(proved, written),
+            us(start, written), lbn_commit_primes(ctx));
+    lbn_destroy(ctx);
+    return 0;
+}

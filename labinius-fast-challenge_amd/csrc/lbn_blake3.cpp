@@ -1,0 +1,248 @@
+// BLAKE3 (host side of the prover); see lbn_blake3.h.
+#include "lbn_blake3.h"
+
+#include <algorithm>
+#include <thread>
+#include <vector>
+
+namespace lbn {
+
+const uint32_t Blake3::IV_WORDS[8] = {0x6A09E667u, 0xBB67AE85u, 0x3C6EF372u, 0xA54FF53Au,
+                                      0x510E527Fu, 0x9B05688Cu, 0x1F83D9ABu, 0x5BE0CD19u};
+
+static inline uint32_t rd32(const uint8_t *p) {
+    return (uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) | ((uint32_t)p[3] << 24);
+}
+static inline uint32_t ror(uint32_t x, int n) { return (x >> n) | (x << (32 - n)); }
+
+#define B3G(a, b, c, d, x, y)          \
+    do {                               \
+        s[a] += s[b] + (x);            \
+        s[d] = ror(s[d] ^ s[a], 16);   \
+        s[c] += s[d];                  \
+        s[b] = ror(s[b] ^ s[c], 12);   \
+        s[a] += s[b] + (y);            \
+        s[d] = ror(s[d] ^ s[a], 8);    \
+        s[c] += s[d];                  \
+        s[b] = ror(s[b] ^ s[c], 7);    \
+    } while (0)
+
+void Blake3::compress(const uint32_t cv[8], const uint32_t m_in[16], uint64_t counter, uint32_t len,
+                      uint32_t flags, uint32_t out[16]) {
+    static const uint8_t SCHEDULE[7][16] = {
+        {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15}, {2, 6, 3, 10, 7, 0, 4, 13, 1, 11, 12, 5, 9, 14, 15, 8},
+        {3, 4, 10, 12, 13, 2, 7, 14, 6, 5, 9, 0, 11, 15, 8, 1}, {10, 7, 12, 9, 14, 3, 13, 15, 4, 0, 11, 2, 5, 8, 1, 6},
+        {12, 13, 9, 11, 15, 10, 14, 8, 7, 2, 5, 3, 0, 1, 6, 4}, {9, 14, 11, 5, 8, 12, 15, 1, 13, 3, 0, 10, 2, 6, 4, 7},
+        {11, 15, 5, 0, 1, 9, 8, 6, 14, 10, 2, 12, 3, 4, 7, 13}};
+    uint32_t s[16] = {cv[0],   cv[1],   cv[2],   cv[3],   cv[4],   cv[5],   cv[6],   cv[7],
+                      IV_WORDS[0], IV_WORDS[1], IV_WORDS[2], IV_WORDS[3], (uint32_t)counter,
+                      (uint32_t)(counter >> 32), len, flags};
+    for (int r = 0; r < 7; r++) {
+        const uint8_t *q = SCHEDULE[r];
+        B3G(0, 4, 8, 12, m_in[q[0]], m_in[q[1]]);
+        B3G(1, 5, 9, 13, m_in[q[2]], m_in[q[3]]);
+        B3G(2, 6, 10, 14, m_in[q[4]], m_in[q[5]]);
+        B3G(3, 7, 11, 15, m_in[q[6]], m_in[q[7]]);
+        B3G(0, 5, 10, 15, m_in[q[8]], m_in[q[9]]);
+        B3G(1, 6, 11, 12, m_in[q[10]], m_in[q[11]]);
+        B3G(2, 7, 8, 13, m_in[q[12]], m_in[q[13]]);
+        B3G(3, 4, 9, 14, m_in[q[14]], m_in[q[15]]);
+    }
+    for (int i = 0; i < 8; i++) {
+        out[i] = s[i] ^ s[i + 8];
+        out[i + 8] = s[i + 8] ^ cv[i];
+    }
+}
+
+void Blake3::init(const uint32_t key[8], uint32_t flags) {
+    memcpy(key_, key, 32);
+    memcpy(cv_, key, 32);
+    flags_ = flags;
+    depth_ = 0;
+    chunk_ = 0;
+    blocks_ = 0;
+    buf_len_ = 0;
+}
+
+Blake3 Blake3::keyed(const uint8_t key[32]) {
+    Blake3 h;
+    uint32_t k[8];
+    for (int i = 0; i < 8; i++) k[i] = rd32(key + 4 * i);
+    h.init(k, KEYED);
+    return h;
+}
+
+Blake3 Blake3::derive_key(const char *context) {
+    Blake3 c;
+    c.init(IV_WORDS, DERIVE_CONTEXT);
+    c.update(context, strlen(context));
+    uint8_t ck[32];
+    c.xof(0, ck, 32);
+    Blake3 h;
+    uint32_t k[8];
+    for (int i = 0; i < 8; i++) k[i] = rd32(ck + 4 * i);
+    h.init(k, DERIVE_MATERIAL);
+    return h;
+}
+
+static void words16(const uint8_t *block, size_t len, uint32_t m[16]) {
+    uint8_t b[64] = {0};
+    memcpy(b, block, len);
+    for (int i = 0; i < 16; i++) m[i] = rd32(b + 4 * i);
+}
+
+void Blake3::chunk_cv(uint32_t out[8]) const {
+    uint32_t m[16], o[16];
+    words16(buf_, buf_len_, m);
+    compress(cv_, m, chunk_, (uint32_t)buf_len_, flags_ | CHUNK_END | (blocks_ == 0 ? CHUNK_START : 0), o);
+    memcpy(out, o, 32);
+}
+
+void Blake3::add_chunk_cv(const uint32_t cv_in[8], uint64_t total) {
+    uint32_t cv[8];
+    memcpy(cv, cv_in, 32);
+    while ((total & 1) == 0) {
+        uint32_t m[16], o[16];
+        depth_--;
+        memcpy(m, stack_[depth_], 32);
+        memcpy(m + 8, cv, 32);
+        compress(key_, m, 0, 64, flags_ | PARENT, o);
+        memcpy(cv, o, 32);
+        total >>= 1;
+    }
+    memcpy(stack_[depth_++], cv, 32);
+}
+
+void Blake3::absorb_block() {
+    uint32_t m[16], o[16];
+    words16(buf_, 64, m);
+    compress(cv_, m, chunk_, 64, flags_ | (blocks_ == 0 ? CHUNK_START : 0), o);
+    memcpy(cv_, o, 32);
+    blocks_++;
+    buf_len_ = 0;
+}
+
+void Blake3::update(const void *data, size_t len) {
+    const uint8_t *p = (const uint8_t *)data;
+    while (len) {
+        if ((size_t)blocks_ * 64 + buf_len_ == CHUNK) {  // full chunk and more input: close it
+            uint32_t cv[8];
+            chunk_cv(cv);
+            add_chunk_cv(cv, chunk_ + 1);
+            chunk_++;
+            blocks_ = 0;
+            buf_len_ = 0;
+            memcpy(cv_, key_, 32);
+        }
+        if (buf_len_ == 64) absorb_block();
+        size_t take = std::min(len, (size_t)64 - buf_len_);
+        memcpy(buf_ + buf_len_, p, take);
+        buf_len_ += take;
+        p += take;
+        len -= take;
+    }
+}
+
+// A whole chunk's chaining value at chunk index `index` (not the root).
+static void chunk_cv_of(const uint32_t key[8], uint32_t flags, const uint8_t *chunk, uint64_t index, uint32_t out[8]) {
+    uint32_t cv[8], m[16], o[16];
+    memcpy(cv, key, 32);
+    for (int b = 0; b < 16; b++) {
+        for (int i = 0; i < 16; i++) m[i] = rd32(chunk + 64 * b + 4 * i);
+        uint32_t f = flags | (b == 0 ? Blake3::CHUNK_START : 0) | (b == 15 ? Blake3::CHUNK_END : 0);
+        Blake3::compress(cv, m, index, 64, f, o);
+        memcpy(cv, o, 32);
+    }
+    memcpy(out, cv, 32);
+}
+
+void Blake3::update_parallel(const void *data, size_t len, int threads) {
+    const uint8_t *p = (const uint8_t *)data;
+    // Finish the current chunk sequentially.
+    size_t in_chunk = (size_t)blocks_ * 64 + buf_len_;
+    if (in_chunk < CHUNK) {
+        size_t take = std::min(len, CHUNK - in_chunk);
+        update(p, take);
+        p += take;
+        len -= take;
+    }
+    // Whole chunks strictly before the last byte go to the threads; the rest stays sequential.
+    size_t whole = len ? (len - 1) / CHUNK : 0;
+    if (whole < 8 || threads <= 1) {
+        update(p, len);
+        return;
+    }
+    // The current (full) chunk is closed by the first byte of more input.
+    {
+        uint32_t cv[8];
+        chunk_cv(cv);
+        add_chunk_cv(cv, chunk_ + 1);
+        chunk_++;
+        blocks_ = 0;
+        buf_len_ = 0;
+        memcpy(cv_, key_, 32);
+    }
+    std::vector<uint32_t> cvs(whole * 8);
+    const uint64_t first = chunk_;
+    auto work = [&](size_t lo, size_t hi) {
+        for (size_t c = lo; c < hi; c++) chunk_cv_of(key_, flags_, p + c * CHUNK, first + c, &cvs[8 * c]);
+    };
+    int nt = (int)std::min<size_t>((size_t)threads, whole / 4);
+    std::vector<std::thread> pool;
+    size_t per = (whole + nt - 1) / nt;
+    for (int t = 1; t < nt; t++) {
+        size_t lo = t * per, hi = std::min(whole, lo + per);
+        if (lo < hi) pool.emplace_back(work, lo, hi);
+    }
+    work(0, std::min(whole, per));
+    for (auto &th : pool) th.join();
+    for (size_t c = 0; c + 1 < whole; c++) add_chunk_cv(&cvs[8 * c], first + c + 1);
+    // The last whole chunk is pushed too, since more input (at least one byte) follows it.
+    add_chunk_cv(&cvs[8 * (whole - 1)], first + whole);
+    chunk_ = first + whole;
+    blocks_ = 0;
+    buf_len_ = 0;
+    memcpy(cv_, key_, 32);
+    update(p + whole * CHUNK, len - whole * CHUNK);
+}
+
+void Blake3::xof(uint64_t offset, uint8_t *out, size_t len) const {
+    uint32_t cv[8], m[16], blen, flags;
+    if (depth_ == 0) {
+        memcpy(cv, cv_, 32);
+        words16(buf_, buf_len_, m);
+        blen = (uint32_t)buf_len_;
+        flags = flags_ | CHUNK_END | (blocks_ == 0 ? CHUNK_START : 0);
+    } else {
+        uint32_t right[8];
+        chunk_cv(right);
+        for (int i = depth_ - 1; i > 0; i--) {
+            uint32_t mm[16], o[16];
+            memcpy(mm, stack_[i], 32);
+            memcpy(mm + 8, right, 32);
+            compress(key_, mm, 0, 64, flags_ | PARENT, o);
+            memcpy(right, o, 32);
+        }
+        memcpy(cv, key_, 32);
+        memcpy(m, stack_[0], 32);
+        memcpy(m + 8, right, 32);
+        blen = 64;
+        flags = flags_ | PARENT;
+    }
+    uint64_t counter = offset / 64;
+    size_t skip = offset % 64;
+    while (len) {
+        uint32_t o[16];
+        compress(cv, m, counter++, blen, flags | ROOT, o);
+        uint8_t bytes[64];
+        for (int i = 0; i < 16; i++)
+            for (int b = 0; b < 4; b++) bytes[4 * i + b] = (uint8_t)(o[i] >> (8 * b));
+        size_t take = std::min(len, (size_t)64 - skip);
+        memcpy(out, bytes + skip, take);
+        out += take;
+        len -= take;
+        skip = 0;
+    }
+}
+
+}  // namespace lbn

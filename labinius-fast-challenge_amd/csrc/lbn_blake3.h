@@ -1,0 +1,77 @@
+// BLAKE3 for the prover's host side: the Fiat-Shamir transcript, the public matrix expansion and
+// the challenge sign key. A portable implementation of the published specification, with a
+// multi-threaded absorb that hashes whole chunks on several threads and pushes their chaining
+// values in order, so the state equals a sequential absorb of the same bytes.
+#pragma once
+#include <cstddef>
+#include <cstdint>
+#include <cstring>
+
+namespace lbn {
+
+class Blake3 {
+   public:
+    static constexpr uint32_t CHUNK_START = 1, CHUNK_END = 2, PARENT = 4, ROOT = 8, KEYED = 16,
+                              DERIVE_CONTEXT = 32, DERIVE_MATERIAL = 64;
+    static constexpr size_t CHUNK = 1024;
+
+    Blake3() { init(IV_WORDS, 0); }
+    static Blake3 keyed(const uint8_t key[32]);
+    static Blake3 derive_key(const char *context);
+
+    void update(const void *data, size_t len);
+    // The same state as update(), with whole chunks hashed on `threads` threads.
+    void update_parallel(const void *data, size_t len, int threads);
+    void update_u64(uint64_t x) {
+        uint8_t b[8];
+        for (int i = 0; i < 8; i++) b[i] = (uint8_t)(x >> (8 * i));
+        update(b, 8);
+    }
+    // Extendable output from byte `offset`; the state is unchanged.
+    void xof(uint64_t offset, uint8_t *out, size_t len) const;
+
+    static void compress(const uint32_t cv[8], const uint32_t m[16], uint64_t counter, uint32_t len,
+                         uint32_t flags, uint32_t out[16]);
+    static const uint32_t IV_WORDS[8];
+
+   private:
+    void init(const uint32_t key[8], uint32_t flags);
+    void chunk_cv(uint32_t out[8]) const;  // current chunk as a non-root chunk
+    void add_chunk_cv(const uint32_t cv[8], uint64_t total);
+    void absorb_block();
+    uint32_t key_[8];
+    uint32_t flags_;
+    uint32_t stack_[54][8];
+    int depth_;
+    uint32_t cv_[8];
+    uint64_t chunk_;
+    int blocks_;
+    uint8_t buf_[64];
+    size_t buf_len_;
+};
+
+// An XOF stream read strictly in order (the transcript's readers and the matrix expansion).
+class XofReader {
+   public:
+    explicit XofReader(const Blake3 &h) : h_(h) {}
+    uint8_t byte() {
+        if (pos_ == sizeof buf_) refill();
+        return buf_[pos_++];
+    }
+    void read(uint8_t *out, size_t n) {
+        for (size_t i = 0; i < n; i++) out[i] = byte();
+    }
+
+   private:
+    void refill() {
+        h_.xof(off_, buf_, sizeof buf_);
+        off_ += sizeof buf_;
+        pos_ = 0;
+    }
+    Blake3 h_;
+    uint64_t off_ = 0;
+    uint8_t buf_[1024];
+    size_t pos_ = sizeof buf_;
+};
+
+}  // namespace lbn

@@ -1,0 +1,1250 @@
+// liblabinius_gpu: the MI355X kernels of the Labinius prover and the device half of the C ABI.
+//
+// Passes over the 2^k-bit table (k = 31: 2^24 words, 256 MiB resident in HBM):
+//   table  : BLAKE3 XOF of the seed, one root compression per 64-byte block (table writer)
+//   eq     : eq_hi[j] = EH[j >> 12] * EL[j & 4095], a per-block "multiply by EH" byte table
+//   partials: v[k] = sum_j bit_k(w_j) eq_hi[j]                       (M4RM, 4-word values)
+//   switch : round 0 from the words, then fold+message fused rounds (M4RM, 6-word values)
+//   row    : row[c] = sum_i eq(p0,i) lift(w_{c,i})                   (M4RM + x^b recombination)
+//   fold   : v_m = sum_j c_j(-X^4) W_{j,m}, bit-sliced counters per challenge position
+//   commit : per CRT prime, binary split/twist -> mixed-radix 2/3 DIF NTT -> MAC with the
+//            transformed matrix -> inverse NTT -> Garner -> mod Q -> packed bytes
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "lbn_internal.h"
+#include "lbn_kernels.h"
+
+using namespace lbn;
+using namespace lbn::dev;
+
+namespace lbn {
+void ajtai_matrix_host(u128 q, size_t elements, u128 *out);  // host BLAKE3 expansion
+bool prime_tables(const PrimeInfo &pi, std::vector<uint32_t> &tabs);
+}  // namespace lbn
+
+// Layout of one prime's NTT constant tables (u32 offsets), built by prime_tables().
+namespace lbn {
+namespace tab {
+constexpr int TWB = 0;                 // [2][324][3] binary split+twist values (normal form)
+constexpr int TW = TWB + 2 * 324 * 3;  // [2][324] beta_h^i (Montgomery)
+constexpr int ETA = TW + 2 * 324;      // [2] eta_h (Montgomery)
+constexpr int FWD = ETA + 2;           // [323] DIF twiddles (Montgomery)
+constexpr int INV = FWD + 323;         // [323] DIT inverse twiddles (Montgomery)
+constexpr int W3 = INV + 323;          // [2] cube root of unity and its inverse (Montgomery)
+constexpr int UNTW = W3 + 2;           // [2][324] beta_h^-i / 324 (Montgomery)
+constexpr int ETA0 = UNTW + 2 * 324;   // eta_0 (Montgomery)
+constexpr int DINV = ETA0 + 1;         // 1 / (eta_0 - eta_1) (Montgomery)
+constexpr int SIZE = DINV + 1;
+// stage twiddle bases inside FWD/INV for spans 324, 162, 81, 27, 9, 3
+constexpr int ST0 = 0, ST1 = 162, ST2 = 243, ST3 = 297, ST4 = 315, ST5 = 321;
+}  // namespace tab
+}  // namespace lbn
+
+struct lbn::DeviceState {
+    hipStream_t stream = nullptr;
+    uint4 *table = nullptr;
+    uint4 *E[2] = {nullptr, nullptr};
+    uint32_t *P[2] = {nullptr, nullptr};
+    uint32_t *T = nullptr;        // [0,768): T0, [768,1536): Tinf, [1536, 2048): partials
+    uint4 *EH = nullptr, *EL = nullptr;
+    uint32_t *eqrow = nullptr, *rowU = nullptr, *rowOut = nullptr, *eqlo = nullptr;
+    uint32_t *fold_flag = nullptr;
+    int *fold_off = nullptr;
+    uint16_t *fold_cols = nullptr;
+    int32_t *fold_out = nullptr;
+    uint32_t *ntt_tabs = nullptr;   // MAX_PRIMES * tab::SIZE
+    uint32_t *ahat = nullptr;       // nprimes * RE * 648
+    unsigned long long *cacc = nullptr;  // nprimes * 1024 * 648
+    uint8_t *cbytes = nullptr;      // 1024 * 648 * width
+    uint64_t *amat = nullptr;       // RE * 648 * 2 (staging for the matrix)
+    size_t ahat_elems = 0;
+    hipEvent_t ev[2 * 9] = {};      // start/stop per timed pass
+};
+
+// Timed passes (indices into lbn_timing's fields, in order).
+enum Pass { T_TABLE = 0, T_EQ, T_PARTIALS, T_SWITCH0, T_ROUNDS, T_ROW, T_FOLD, T_MAC, T_FINISH, T_COUNT };
+
+static void tstart(lbn_ctx *ctx, int pass) {
+    if (ctx->timing) hipEventRecord(ctx->dev->ev[2 * pass], ctx->dev->stream);
+}
+static void tstop(lbn_ctx *ctx, int pass) {
+    if (ctx->timing) hipEventRecord(ctx->dev->ev[2 * pass + 1], ctx->dev->stream);
+}
+// After the stream has been synchronised: add the pass's elapsed time.
+static void tcollect(lbn_ctx *ctx, int pass) {
+    if (!ctx->timing) return;
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, ctx->dev->ev[2 * pass], ctx->dev->ev[2 * pass + 1]) == hipSuccess)
+        (&ctx->times.table_ms)[pass] += ms;
+    if (pass == T_ROUNDS) ctx->times.switch_rounds++;
+}
+
+#define HIPCHK(ctx, x)                                                              \
+    do {                                                                            \
+        hipError_t e_ = (x);                                                        \
+        if (e_ != hipSuccess)                                                       \
+            return (ctx)->fail(LBN_E_DEVICE, std::string(#x ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+// =============================================================================================
+// small kernels
+// =============================================================================================
+
+__global__ void k_fill_u32(uint32_t *p, size_t n, uint32_t v) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
+static void fill_u32(hipStream_t s, void *p, size_t n, uint32_t v = 0) {
+    if (!n) return;
+    int blocks = (int)std::min<size_t>((n + 255) / 256, 4096);
+    k_fill_u32<<<blocks, 256, 0, s>>>((uint32_t *)p, n, v);
+}
+
+// ---------------------------------------------------------------------------------------------
+// The witness table on the device: BLAKE3 keyed by the seed, updated with the 29-byte context
+// "labinius-fast/binary-table/v1", read as XOF (verifier/table_writer/src/main.rs:14-22). The
+// input is one block of one chunk, so output block i is the root compression with counter i.
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t rotr32(uint32_t x, int n) { return (x >> n) | (x << (32 - n)); }
+
+__device__ __forceinline__ void b3_g(uint32_t &a, uint32_t &b, uint32_t &c, uint32_t &d, uint32_t x,
+                                     uint32_t y) {
+    a = a + b + x;
+    d = rotr32(d ^ a, 16);
+    c = c + d;
+    b = rotr32(b ^ c, 12);
+    a = a + b + y;
+    d = rotr32(d ^ a, 8);
+    c = c + d;
+    b = rotr32(b ^ c, 7);
+}
+
+struct B3Root {
+    uint32_t cv[8];
+    uint32_t m[16];
+    uint32_t len, flags;
+};
+
+__global__ void k_table_xof(B3Root root, uint32_t *out, size_t blocks) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= blocks) return;
+    const uint32_t IV0 = 0x6A09E667u, IV1 = 0xBB67AE85u, IV2 = 0x3C6EF372u, IV3 = 0xA54FF53Au;
+    uint32_t s[16] = {root.cv[0], root.cv[1], root.cv[2], root.cv[3], root.cv[4], root.cv[5], root.cv[6],
+                      root.cv[7], IV0, IV1, IV2, IV3, (uint32_t)i, (uint32_t)(i >> 32), root.len, root.flags};
+    uint32_t m[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) m[k] = root.m[k];
+    const int perm[16] = {2, 6, 3, 10, 7, 0, 4, 13, 1, 11, 12, 5, 9, 14, 15, 8};
+#pragma unroll
+    for (int r = 0; r < 7; r++) {
+        b3_g(s[0], s[4], s[8], s[12], m[0], m[1]);
+        b3_g(s[1], s[5], s[9], s[13], m[2], m[3]);
+        b3_g(s[2], s[6], s[10], s[14], m[4], m[5]);
+        b3_g(s[3], s[7], s[11], s[15], m[6], m[7]);
+        b3_g(s[0], s[5], s[10], s[15], m[8], m[9]);
+        b3_g(s[1], s[6], s[11], s[12], m[10], m[11]);
+        b3_g(s[2], s[7], s[8], s[13], m[12], m[13]);
+        b3_g(s[3], s[4], s[9], s[14], m[14], m[15]);
+        uint32_t t[16];
+#pragma unroll
+        for (int k = 0; k < 16; k++) t[k] = m[perm[k]];
+#pragma unroll
+        for (int k = 0; k < 16; k++) m[k] = t[k];
+    }
+    uint32_t *o = out + 16 * i;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        o[k] = s[k] ^ s[k + 8];
+        o[k + 8] = s[k + 8] ^ root.cv[k];
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// eq_hi[jh * 4096 + jl] = EH[jh] * EL[jl] in GF(2^128): EH[jh] times the 12-coordinate
+// expansion EL, through a 16 x 256 byte-window table of y -> EH[jh] * y.
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_eq_expand(const uint4 *EH, const uint4 *EL, uint4 *E0) {
+    __shared__ uint4 basis[128];
+    __shared__ uint4 tab[16 * 256];
+    const int t = threadIdx.x;
+    const uint32_t jh = blockIdx.x;
+    if (t < 128) {
+        uint4 e = EH[jh];
+        uint64_t lo = (uint64_t)e.x | ((uint64_t)e.y << 32), hi = (uint64_t)e.z | ((uint64_t)e.w << 32);
+        for (int s = 0; s < t; s++) b128_mulx(lo, hi);
+        basis[t] = make_uint4((uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32));
+    }
+    __syncthreads();
+    {
+        const int w = t >> 4, bh = t & 15;
+        uint4 base = make_uint4(0, 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+            if ((bh >> i) & 1) {
+                uint4 b = basis[8 * w + 4 + i];
+                base.x ^= b.x, base.y ^= b.y, base.z ^= b.z, base.w ^= b.w;
+            }
+        uint4 b0 = basis[8 * w], b1 = basis[8 * w + 1], b2 = basis[8 * w + 2], b3 = basis[8 * w + 3];
+#pragma unroll
+        for (int lo = 0; lo < 16; lo++) {
+            uint4 v = base;
+            if (lo & 1) v.x ^= b0.x, v.y ^= b0.y, v.z ^= b0.z, v.w ^= b0.w;
+            if (lo & 2) v.x ^= b1.x, v.y ^= b1.y, v.z ^= b1.z, v.w ^= b1.w;
+            if (lo & 4) v.x ^= b2.x, v.y ^= b2.y, v.z ^= b2.z, v.w ^= b2.w;
+            if (lo & 8) v.x ^= b3.x, v.y ^= b3.y, v.z ^= b3.z, v.w ^= b3.w;
+            tab[w * 256 + bh * 16 + lo] = v;
+        }
+    }
+    __syncthreads();
+    for (int jl = t; jl < 4096; jl += 256) {
+        uint4 y = EL[jl];
+        uint32_t yw[4] = {y.x, y.y, y.z, y.w};
+        uint4 acc = make_uint4(0, 0, 0, 0);
+#pragma unroll
+        for (int w = 0; w < 16; w++) {
+            uint4 v = tab[w * 256 + ((yw[w >> 2] >> (8 * (w & 3))) & 255)];
+            acc.x ^= v.x, acc.y ^= v.y, acc.z ^= v.z, acc.w ^= v.w;
+        }
+        E0[(size_t)jh * 4096 + jl] = acc;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// partials v[k] = sum_j bit_k(w_j) eq_hi[j] (crossfield.rs:152-165).
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_partials(const uint4 *table, const uint4 *E0, size_t n, uint32_t *out) {
+    __shared__ M4Lds<4> L;
+    __shared__ uint32_t red[4 * 128 * 4];
+    uint32_t acc[8][4] = {};
+    for (size_t base = (size_t)blockIdx.x * 256; base < n; base += (size_t)gridDim.x * 256) {
+        const size_t j = base + threadIdx.x;
+        uint4 w = table[j], e = E0[j];
+        uint32_t sel[4] = {w.x, w.y, w.z, w.w};
+        uint32_t val[4] = {e.x, e.y, e.z, e.w};
+        m4rm_step<4>(acc, sel, val, L);
+    }
+    m4rm_flush<4>(acc, red, out);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Switch round 0 (sumcheck.rs:89-117 with a_j = L(E0_j), p_j = lift(w_j)):
+//   T0_k   = sum_{j<h} bit_k(E0_j) w_j
+//   Tinf_k = sum_{j<h} bit_k(E1_j) (w_j + w_{j+h}),  E1_j = E0_j + E0_{j+h} (written)
+// The host turns T into e0 = sum_k L[k] T0_k, einf = sum_k L[k] Tinf_k.
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_switch_msg0(const uint4 *table, const uint4 *E0, size_t h, uint4 *E1,
+                                                     uint32_t *T0, uint32_t *Tinf) {
+    __shared__ M4Lds<4> L;
+    __shared__ uint32_t red[4 * 128 * 4];
+    uint32_t acc0[8][4] = {}, acci[8][4] = {};
+    for (size_t base = (size_t)blockIdx.x * 256; base < h; base += (size_t)gridDim.x * 256) {
+        const size_t j = base + threadIdx.x;
+        uint4 ea = E0[j], eb = E0[j + h], wa = table[j], wb = table[j + h];
+        uint4 e1 = make_uint4(ea.x ^ eb.x, ea.y ^ eb.y, ea.z ^ eb.z, ea.w ^ eb.w);
+        E1[j] = e1;
+        {
+            uint32_t sel[4] = {ea.x, ea.y, ea.z, ea.w};
+            uint32_t val[4] = {wa.x, wa.y, wa.z, wa.w};
+            m4rm_step<4>(acc0, sel, val, L);
+        }
+        {
+            uint32_t sel[4] = {e1.x, e1.y, e1.z, e1.w};
+            uint32_t val[4] = {wa.x ^ wb.x, wa.y ^ wb.y, wa.z ^ wb.z, wa.w ^ wb.w};
+            m4rm_step<4>(acci, sel, val, L);
+        }
+    }
+    m4rm_flush<4>(acc0, red, T0);
+    __syncthreads();
+    m4rm_flush<4>(acci, red, Tinf);
+}
+
+// Per-block nibble-window table of y -> rho * y in F162: 41 windows x 16 entries x 6 words.
+struct RhoArg {
+    uint64_t r[3];
+};
+constexpr int RHO_WINDOWS = 41;
+
+__device__ void build_rho_table(RhoArg rho, uint32_t *basis /*162*6*/, uint32_t *tab /*41*16*6*/) {
+    const int t = threadIdx.x;
+    if (t < 162) {
+        uint64_t a0 = rho.r[0], a1 = rho.r[1], a2 = rho.r[2];
+        for (int s = 0; s < t; s++) f162_mulx(a0, a1, a2);
+        uint32_t *b = basis + 6 * t;
+        b[0] = (uint32_t)a0, b[1] = (uint32_t)(a0 >> 32), b[2] = (uint32_t)a1, b[3] = (uint32_t)(a1 >> 32);
+        b[4] = (uint32_t)a2, b[5] = (uint32_t)(a2 >> 32);
+    }
+    __syncthreads();
+    for (int e = t; e < RHO_WINDOWS * 16; e += blockDim.x) {
+        const int w = e >> 4, nib = e & 15;
+        uint32_t v[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            const int i = 4 * w + b;
+            if (((nib >> b) & 1) && i < 162) {
+#pragma unroll
+                for (int k = 0; k < 6; k++) v[k] ^= basis[6 * i + k];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 6; k++) tab[6 * e + k] = v[k];
+    }
+    __syncthreads();
+}
+
+template <int NW>  // number of nibble windows of the input (32 for a lifted word, 41 for F162)
+__device__ __forceinline__ void rho_mul(const uint32_t *tab, const uint32_t (&y)[6], uint32_t (&out)[6]) {
+#pragma unroll
+    for (int k = 0; k < 6; k++) out[k] = 0;
+#pragma unroll
+    for (int w = 0; w < NW; w++) {
+        const uint32_t nib = (y[w >> 3] >> (4 * (w & 7))) & 15;
+        const uint32_t *e = tab + 6 * (w * 16 + nib);
+#pragma unroll
+        for (int k = 0; k < 6; k++) out[k] ^= e[k];
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Fused switch round: fold round r with rho (sumcheck.rs:119-139) and the message of round r+1.
+//   p'_j = p_j + rho (p_j + p_{j+h}) for j < h, with h = 2 h2 the current half;
+//   T0_k   += bit_k(E_j) p'_j                 (E = E^(r+1), j < h2)
+//   Tinf_k += bit_k(E_j + E_{j+h2}) (p'_j + p'_{j+h2}),   E^(r+2)_j = E_j + E_{j+h2} (written)
+// WORDS: p is the table (lifted words) in round 0, F162 (6 words) afterwards.
+// ---------------------------------------------------------------------------------------------
+template <bool WORDS>
+__global__ __launch_bounds__(256) void k_switch_round(const uint32_t *pin, size_t h2, RhoArg rho, const uint4 *Ein,
+                                                      uint4 *Eout, uint32_t *pout, uint32_t *T0, uint32_t *Tinf) {
+    __shared__ uint32_t rbasis[162 * 6];
+    __shared__ uint32_t rtab[RHO_WINDOWS * 16 * 6];
+    __shared__ M4Lds<6> L;
+    __shared__ uint32_t red[4 * 128 * 6];
+    build_rho_table(rho, rbasis, rtab);
+    uint32_t acc0[8][6] = {}, acci[8][6] = {};
+    const size_t h = 2 * h2;
+    const size_t nthreads = (h2 + 255) / 256 * 256;
+    for (size_t base = (size_t)blockIdx.x * 256; base < nthreads; base += (size_t)gridDim.x * 256) {
+        const size_t j = base + threadIdx.x;
+        const bool live = j < h2;
+        uint32_t q0[6] = {0, 0, 0, 0, 0, 0}, q1[6] = {0, 0, 0, 0, 0, 0};
+        uint32_t e[4] = {0, 0, 0, 0}, e2[4] = {0, 0, 0, 0};
+        if (live) {
+            uint32_t a0[6], a1[6], b0[6], b1[6];
+            const size_t idx[4] = {j, j + h2, j + h, j + h + h2};
+            uint32_t *dst[4] = {a0, a1, b0, b1};
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                if (WORDS) {
+                    uint4 w = ((const uint4 *)pin)[idx[u]];
+                    dst[u][0] = w.x, dst[u][1] = w.y, dst[u][2] = w.z, dst[u][3] = w.w, dst[u][4] = 0, dst[u][5] = 0;
+                } else {
+                    const uint2 *s = (const uint2 *)(pin + 6 * idx[u]);
+                    uint2 x0 = s[0], x1 = s[1], x2 = s[2];
+                    dst[u][0] = x0.x, dst[u][1] = x0.y, dst[u][2] = x1.x, dst[u][3] = x1.y, dst[u][4] = x2.x,
+                    dst[u][5] = x2.y;
+                }
+            }
+            uint32_t d0[6], d1[6], m0[6], m1[6];
+#pragma unroll
+            for (int k = 0; k < 6; k++) d0[k] = a0[k] ^ b0[k], d1[k] = a1[k] ^ b1[k];
+            if (WORDS) {
+                rho_mul<32>(rtab, d0, m0);
+                rho_mul<32>(rtab, d1, m1);
+            } else {
+                rho_mul<RHO_WINDOWS>(rtab, d0, m0);
+                rho_mul<RHO_WINDOWS>(rtab, d1, m1);
+            }
+#pragma unroll
+            for (int k = 0; k < 6; k++) q0[k] = a0[k] ^ m0[k], q1[k] = a1[k] ^ m1[k];
+            uint2 *o0 = (uint2 *)(pout + 6 * j), *o1 = (uint2 *)(pout + 6 * (j + h2));
+            o0[0] = make_uint2(q0[0], q0[1]), o0[1] = make_uint2(q0[2], q0[3]), o0[2] = make_uint2(q0[4], q0[5]);
+            o1[0] = make_uint2(q1[0], q1[1]), o1[1] = make_uint2(q1[2], q1[3]), o1[2] = make_uint2(q1[4], q1[5]);
+            uint4 ea = Ein[j], eb = Ein[j + h2];
+            e[0] = ea.x, e[1] = ea.y, e[2] = ea.z, e[3] = ea.w;
+            e2[0] = ea.x ^ eb.x, e2[1] = ea.y ^ eb.y, e2[2] = ea.z ^ eb.z, e2[3] = ea.w ^ eb.w;
+            Eout[j] = make_uint4(e2[0], e2[1], e2[2], e2[3]);
+        }
+        m4rm_step<6>(acc0, e, q0, L);
+        uint32_t dq[6];
+#pragma unroll
+        for (int k = 0; k < 6; k++) dq[k] = q0[k] ^ q1[k];
+        m4rm_step<6>(acci, e2, dq, L);
+    }
+    m4rm_flush<6>(acc0, red, T0);
+    __syncthreads();
+    m4rm_flush<6>(acci, red, Tinf);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Row evaluation (eval.rs:173-184, 426-480): eq table of p0, then per column
+//   U_b = sum_i bit_b(w_{c,i}) eq_i   (M4RM)   and   row[c] = sum_b x^b U_b  (mod x^162+x^81+1).
+// ---------------------------------------------------------------------------------------------
+__global__ void k_eq_row(const uint64_t *lo, const uint64_t *hi, int lo_bits, size_t clen, uint32_t *eqrow) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= clen) return;
+    const uint64_t *a = lo + 3 * (i & ((1u << lo_bits) - 1));
+    const uint64_t *b = hi + 3 * (i >> lo_bits);
+    uint64_t r[3];
+    f162_mul_slow(a, b, r);
+    uint32_t *o = eqrow + 6 * i;
+    o[0] = (uint32_t)r[0], o[1] = (uint32_t)(r[0] >> 32), o[2] = (uint32_t)r[1], o[3] = (uint32_t)(r[1] >> 32);
+    o[4] = (uint32_t)r[2], o[5] = (uint32_t)(r[2] >> 32);
+}
+
+__global__ __launch_bounds__(256) void k_row(const uint4 *table, const uint32_t *eqrow, size_t clen, uint32_t *U) {
+    __shared__ M4Lds<6> L;
+    __shared__ uint32_t red[4 * 128 * 6];
+    uint32_t acc[8][6] = {};
+    const size_t c = blockIdx.x;
+    for (size_t base = 0; base < clen; base += 256) {
+        const size_t i = base + threadIdx.x;
+        uint32_t sel[4] = {0, 0, 0, 0}, val[6] = {0, 0, 0, 0, 0, 0};
+        if (i < clen) {
+            uint4 w = table[c * clen + i];
+            sel[0] = w.x, sel[1] = w.y, sel[2] = w.z, sel[3] = w.w;
+#pragma unroll
+            for (int k = 0; k < 6; k++) val[k] = eqrow[6 * i + k];
+        }
+        m4rm_step<6>(acc, sel, val, L);
+    }
+    m4rm_flush_store<6>(acc, red, U + c * 128 * 6);
+}
+
+// row[c] = sum_b x^b U_b: 128 threads per column, one b each, then an XOR reduction.
+__global__ __launch_bounds__(128) void k_row_finish(const uint32_t *U, uint64_t *row) {
+    __shared__ uint64_t part[128 * 3];
+    const int b = threadIdx.x;
+    const uint32_t *u = U + ((size_t)blockIdx.x * 128 + b) * 6;
+    uint64_t x[3] = {(uint64_t)u[0] | ((uint64_t)u[1] << 32), (uint64_t)u[2] | ((uint64_t)u[3] << 32),
+                     (uint64_t)u[4] | ((uint64_t)u[5] << 32)};
+    // x << b into five words (b < 128)
+    uint64_t v[5] = {0, 0, 0, 0, 0};
+    const int wsh = b >> 6, bsh = b & 63;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        v[k + wsh] ^= x[k] << bsh;
+        if (bsh) v[k + wsh + 1] ^= x[k] >> (64 - bsh);
+    }
+    uint64_t r[3];
+    f162_reduce5(v, r);
+    part[3 * b] = r[0], part[3 * b + 1] = r[1], part[3 * b + 2] = r[2];
+    __syncthreads();
+    for (int s = 64; s > 0; s >>= 1) {
+        if (b < s)
+            for (int k = 0; k < 3; k++) part[3 * b + k] ^= part[3 * (b + s) + k];
+        __syncthreads();
+    }
+    if (b < 3) row[(size_t)blockIdx.x * 3 + b] = part[b];
+}
+
+// ---------------------------------------------------------------------------------------------
+// The fold (reference.rs:78-115): for ring element m and residue class k = t mod 4,
+//   raw_k[q] = sum_pos S_{pos,k}[q - pos],  S_{pos,k}[p] = sum_{j: pos in c_j} +-bit_p(w_{j,4m+k}),
+// with the sign of (j, pos) = sign_t XOR (pos odd). The columns holding each position are listed
+// once per fold (positives then negatives); each (pos, k, 32-bit word) task counts its 32 bit
+// positions with NB-bit bit-sliced counters. Then X^648 = X^324 - 1 folds raw into 648 values:
+//   v[t] = R[t] - R[t+648] - R[t+972] (t < 324),  v[t] = R[t] + R[t+324] (t >= 324).
+// ---------------------------------------------------------------------------------------------
+template <int NB>
+__device__ __forceinline__ void ctr_add(uint32_t (&c)[NB], uint32_t x) {
+#pragma unroll
+    for (int i = 0; i < NB; i++) {
+        uint32_t t = c[i] & x;
+        c[i] ^= x;
+        x = t;
+    }
+}
+
+template <int NB>
+__global__ __launch_bounds__(256) void k_fold(const uint32_t *table32, size_t clen, const int *off,
+                                              const uint16_t *cols, int32_t *out, uint32_t *flag) {
+    __shared__ uint32_t s_w[16 * 1024];
+    __shared__ int32_t s_raw[4 * 292];
+    const size_t m = blockIdx.x;
+    const int tid = threadIdx.x;
+    for (int idx = tid; idx < 16 * 1024; idx += 256) {
+        const int j = idx >> 4, kw = idx & 15, k = kw >> 2, wd = kw & 3;
+        s_w[kw * 1024 + j] = table32[(((size_t)j * clen + 4 * m + k) << 2) + wd];
+    }
+    for (int i = tid; i < 4 * 292; i += 256) s_raw[i] = 0;
+    __syncthreads();
+    for (int task = tid; task < 162 * 16; task += 256) {
+        const int pos = task >> 4, kw = task & 15;
+        const uint32_t *W = s_w + kw * 1024;
+        uint32_t P[NB], Q[NB];
+#pragma unroll
+        for (int i = 0; i < NB; i++) P[i] = 0, Q[i] = 0;
+        const int e0 = off[2 * pos], e1 = off[2 * pos + 1], e2 = off[2 * pos + 2];
+        for (int e = e0; e < e1; e++) ctr_add<NB>(P, W[cols[e]]);
+        for (int e = e1; e < e2; e++) ctr_add<NB>(Q, W[cols[e]]);
+        const int k = kw >> 2, q0 = pos + 32 * (kw & 3);
+        for (int p = 0; p < 32; p++) {
+            int cnt = 0;
+#pragma unroll
+            for (int i = 0; i < NB; i++) cnt += (int)(((P[i] >> p) & 1) << i) - (int)(((Q[i] >> p) & 1) << i);
+            if (cnt) atomicAdd(&s_raw[k * 292 + q0 + p], cnt);
+        }
+    }
+    __syncthreads();
+    for (int t = tid; t < 648; t += 256) {
+        auto R = [&](int tp) -> int { return tp <= 1159 ? s_raw[(tp & 3) * 292 + (tp >> 2)] : 0; };
+        int v = t < 324 ? R(t) - R(t + 648) - R(t + 972) : R(t) + R(t + 324);
+        out[m * 648 + t] = v;
+        if (v < -32768 || v > 32767) atomicOr(flag, 1u);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Commitment: exact A*W over Z via CRT over 31-bit primes p = 1 (mod 1944).
+//
+// Per prime, R_p = Z_p[X]/(X^648 - X^324 + 1) splits as X^324 = eta_h (eta_0 = w^324,
+// eta_1 = w^1620, w a primitive 1944-th root); half h is twisted by beta_h^i (beta_0 = w,
+// beta_1 = w^5) into a cyclic length-324 DFT with root rho = w^6, done as a DIF with radices
+// 2,2,3,3,3,3 in LDS (output in digit-reversed slot order; the inverse is the matching DIT).
+// ---------------------------------------------------------------------------------------------
+template <int SPAN, int RAD>
+__device__ __forceinline__ void dif_stage(uint32_t *X, int nhalves, const uint32_t *tw, uint32_t w3, PC pc) {
+    constexpr int M = SPAN / RAD, NB = 324 / RAD;
+    const int total = nhalves * NB;
+    for (int bf = threadIdx.x; bf < total; bf += blockDim.x) {
+        const int hh = bf / NB, idx = bf - hh * NB, blk = idx / M, j = idx - blk * M;
+        uint32_t *b = X + hh * 324 + blk * SPAN + j;
+        if (RAD == 2) {
+            uint32_t y0 = b[0], y1 = b[M];
+            uint32_t x0 = addm(y0, y1, pc.p), x1 = subm(y0, y1, pc.p);
+            if (j) x1 = mont(x1, tw[j], pc);
+            b[0] = x0, b[M] = x1;
+        } else {
+            uint32_t y0 = b[0], y1 = b[M], y2 = b[2 * M];
+            uint32_t t = mont(subm(y1, y2, pc.p), w3, pc);
+            uint32_t x0 = addm(addm(y0, y1, pc.p), y2, pc.p);
+            uint32_t x1 = addm(subm(y0, y2, pc.p), t, pc.p);
+            uint32_t x2 = subm(subm(y0, y1, pc.p), t, pc.p);
+            if (j) x1 = mont(x1, tw[2 * j], pc), x2 = mont(x2, tw[2 * j + 1], pc);
+            b[0] = x0, b[M] = x1, b[2 * M] = x2;
+        }
+    }
+    __syncthreads();
+}
+
+template <int SPAN, int RAD>
+__device__ __forceinline__ void dit_stage(uint32_t *X, int nhalves, const uint32_t *itw, uint32_t w3i, PC pc) {
+    constexpr int M = SPAN / RAD, NB = 324 / RAD;
+    const int total = nhalves * NB;
+    for (int bf = threadIdx.x; bf < total; bf += blockDim.x) {
+        const int hh = bf / NB, idx = bf - hh * NB, blk = idx / M, j = idx - blk * M;
+        uint32_t *b = X + hh * 324 + blk * SPAN + j;
+        if (RAD == 2) {
+            uint32_t y0 = b[0], y1 = b[M];
+            if (j) y1 = mont(y1, itw[j], pc);
+            b[0] = addm(y0, y1, pc.p), b[M] = subm(y0, y1, pc.p);
+        } else {
+            uint32_t y0 = b[0], y1 = b[M], y2 = b[2 * M];
+            if (j) y1 = mont(y1, itw[2 * j], pc), y2 = mont(y2, itw[2 * j + 1], pc);
+            uint32_t t = mont(subm(y1, y2, pc.p), w3i, pc);
+            b[0] = addm(addm(y0, y1, pc.p), y2, pc.p);
+            b[M] = addm(subm(y0, y2, pc.p), t, pc.p);
+            b[2 * M] = subm(subm(y0, y1, pc.p), t, pc.p);
+        }
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ void dif_all(uint32_t *X, int nhalves, const uint32_t *fwd, uint32_t w3, PC pc) {
+    dif_stage<324, 2>(X, nhalves, fwd + tab::ST0, w3, pc);
+    dif_stage<162, 2>(X, nhalves, fwd + tab::ST1, w3, pc);
+    dif_stage<81, 3>(X, nhalves, fwd + tab::ST2, w3, pc);
+    dif_stage<27, 3>(X, nhalves, fwd + tab::ST3, w3, pc);
+    dif_stage<9, 3>(X, nhalves, fwd + tab::ST4, w3, pc);
+    dif_stage<3, 3>(X, nhalves, fwd + tab::ST5, w3, pc);
+}
+
+__device__ __forceinline__ void dit_all(uint32_t *X, int nhalves, const uint32_t *inv, uint32_t w3i, PC pc) {
+    dit_stage<3, 3>(X, nhalves, inv + tab::ST5, w3i, pc);
+    dit_stage<9, 3>(X, nhalves, inv + tab::ST4, w3i, pc);
+    dit_stage<27, 3>(X, nhalves, inv + tab::ST3, w3i, pc);
+    dit_stage<81, 3>(X, nhalves, inv + tab::ST2, w3i, pc);
+    dit_stage<162, 2>(X, nhalves, inv + tab::ST1, w3i, pc);
+    dit_stage<324, 2>(X, nhalves, inv + tab::ST0, w3i, pc);
+}
+
+// Transformed matrix: ahat[m][s] = NTT(A_m mod p)[s] * R mod p (Montgomery form for the MAC).
+__global__ __launch_bounds__(256) void k_ahat(const uint64_t *amat, const uint32_t *tabs, PC pc, uint32_t r2,
+                                              uint32_t *ahat) {
+    __shared__ uint32_t X[648];
+    __shared__ uint32_t a[648];
+    const size_t m = blockIdx.x;
+    for (int t = threadIdx.x; t < 648; t += 256) {
+        uint64_t lo = amat[(m * 648 + t) * 2], hi = amat[(m * 648 + t) * 2 + 1];
+        // (hi * 2^64 + lo) mod p
+        uint64_t two64 = ((uint64_t)0 - pc.p) % pc.p;  // 2^64 mod p
+        uint64_t x = ((hi % pc.p) * two64 + lo % pc.p) % pc.p;
+        a[t] = (uint32_t)x;
+    }
+    __syncthreads();
+    for (int v = threadIdx.x; v < 648; v += 256) {
+        const int h = v / 324, i = v - h * 324;
+        uint32_t u = addm(a[i], mont(a[i + 324], tabs[tab::ETA + h], pc), pc.p);
+        X[v] = mont(u, tabs[tab::TW + h * 324 + i], pc);
+    }
+    __syncthreads();
+    dif_all(X, 2, tabs + tab::FWD, tabs[tab::W3], pc);
+    for (int s = threadIdx.x; s < 648; s += 256) ahat[m * 648 + s] = mont(X[s], r2, pc);
+}
+
+// Forward transform of the binary elements of TC columns and MAC against ahat, over the ring
+// elements [m0, m1) of this block; partial sums are added into cacc (u64 atomics).
+constexpr int TC = 8;
+constexpr int CACC = (TC * 648 + 255) / 256;
+
+__global__ __launch_bounds__(256) void k_commit_mac(const uint32_t *table32, size_t clen, int m_per, int re,
+                                                    const uint32_t *ahat, const uint32_t *tabs, PC pc,
+                                                    unsigned long long *cacc) {
+    __shared__ uint32_t X[TC * 648];
+    __shared__ uint32_t s_twb[2 * 324 * 3];
+    __shared__ uint32_t s_fwd[323];
+    __shared__ uint32_t s_bits[TC * 16];
+    const int tid = threadIdx.x;
+    const int c0 = blockIdx.x * TC;
+    const int m0 = blockIdx.y * m_per, m1 = min(re, m0 + m_per);
+    for (int i = tid; i < 2 * 324 * 3; i += 256) s_twb[i] = tabs[tab::TWB + i];
+    for (int i = tid; i < 323; i += 256) s_fwd[i] = tabs[tab::FWD + i];
+    const uint32_t w3 = tabs[tab::W3];
+    unsigned long long acc[CACC];
+#pragma unroll
+    for (int u = 0; u < CACC; u++) acc[u] = 0;
+    __syncthreads();
+    for (int m = m0; m < m1; m++) {
+        if (tid < TC * 16) {
+            const int col = tid >> 4, wd = tid & 15;  // word k = wd >> 2, u32 wd & 3
+            s_bits[tid] = table32[(((size_t)(c0 + col) * clen + 4 * (size_t)m + (wd >> 2)) << 2) + (wd & 3)];
+        }
+        __syncthreads();
+        for (int v = tid; v < TC * 648; v += 256) {
+            const int col = v / 648, s = v - col * 648, h = s / 324, i = s - h * 324;
+            const uint32_t *bits = s_bits + col * 16;
+            // coefficient c of the element is bit (c >> 2) of word (c & 3)
+            const int cl = i, ch = i + 324;
+            uint32_t lo = (bits[(cl & 3) * 4 + ((cl >> 2) >> 5)] >> ((cl >> 2) & 31)) & 1;
+            uint32_t hi = ch < 512 ? (bits[(ch & 3) * 4 + ((ch >> 2) >> 5)] >> ((ch >> 2) & 31)) & 1 : 0;
+            const uint32_t sel = lo | (hi << 1);
+            X[v] = sel ? s_twb[(h * 324 + i) * 3 + sel - 1] : 0u;
+        }
+        __syncthreads();
+        dif_all(X, 2 * TC, s_fwd, w3, pc);
+        const uint32_t *am = ahat + (size_t)m * 648;
+#pragma unroll
+        for (int u = 0; u < CACC; u++) {
+            const int v = tid + 256 * u;
+            if (v < TC * 648) {
+                const int s = v % 648;
+                acc[u] += mont_lazy(X[v], am[s], pc);
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int u = 0; u < CACC; u++) {
+        const int v = tid + 256 * u;
+        if (v < TC * 648) atomicAdd(cacc + (size_t)c0 * 648 + v, acc[u]);
+    }
+}
+
+// Inverse transform per column and prime, Garner over the primes, reduction mod Q, packing.
+struct CrtArg {
+    uint32_t p[MAX_PRIMES];
+    uint32_t inv[MAX_PRIMES][MAX_PRIMES];  // inv[i][j] = p_j^-1 mod p_i (j < i)
+    uint64_t q_lo, q_hi;
+    uint64_t m_lo, m_hi;  // product of the primes
+    int nprimes, width;
+};
+
+__device__ __forceinline__ uint32_t mulmod_u64(uint32_t a, uint32_t b, uint32_t p) {
+    return (uint32_t)(((uint64_t)a * b) % p);
+}
+
+__global__ __launch_bounds__(256) void k_commit_finish(const unsigned long long *cacc, const uint32_t *tabs_all,
+                                                       CrtArg crt, PC pc0, PC pc1, PC pc2, PC pc3, uint8_t *outb) {
+    __shared__ uint32_t X[648];
+    __shared__ uint32_t res[MAX_PRIMES][648];
+    const int c = blockIdx.x;
+    const PC pcs[4] = {pc0, pc1, pc2, pc3};
+    for (int k = 0; k < crt.nprimes; k++) {
+        const PC pc = pcs[k];
+        const uint32_t *tabs = tabs_all + k * tab::SIZE;
+        const unsigned long long *src = cacc + ((size_t)k * 1024 + c) * 648;
+        for (int s = threadIdx.x; s < 648; s += 256) X[s] = (uint32_t)(src[s] % pc.p);
+        __syncthreads();
+        dit_all(X, 2, tabs + tab::INV, tabs[tab::W3 + 1], pc);
+        for (int i = threadIdx.x; i < 324; i += 256) {
+            uint32_t u0 = mont(X[i], tabs[tab::UNTW + i], pc);
+            uint32_t u1 = mont(X[324 + i], tabs[tab::UNTW + 324 + i], pc);
+            uint32_t ahi = mont(subm(u0, u1, pc.p), tabs[tab::DINV], pc);
+            uint32_t alo = subm(u0, mont(ahi, tabs[tab::ETA0], pc), pc.p);
+            res[k][i] = alo;
+            res[k][i + 324] = ahi;
+        }
+        __syncthreads();
+    }
+    const u128 Q = ((u128)crt.q_hi << 64) | crt.q_lo;
+    const u128 M = ((u128)crt.m_hi << 64) | crt.m_lo;
+    for (int t = threadIdx.x; t < 648; t += 256) {
+        // Garner: x = v0 + p0 (v1 + p1 (v2 + p2 v3))
+        uint32_t v[MAX_PRIMES];
+        for (int i = 0; i < crt.nprimes; i++) {
+            uint32_t x = res[i][t];
+            for (int j = 0; j < i; j++) {
+                uint32_t vj = v[j] % crt.p[i];
+                x = mulmod_u64(x >= vj ? x - vj : x + crt.p[i] - vj, crt.inv[i][j], crt.p[i]);
+            }
+            v[i] = x;
+        }
+        u128 x = v[crt.nprimes - 1];
+        for (int i = crt.nprimes - 2; i >= 0; i--) x = x * crt.p[i] + v[i];
+        // centred lift: the exact coefficient is x or x - M
+        bool neg = x > (M >> 1);
+        u128 mag = neg ? M - x : x;
+        // mag mod Q by shift-subtract
+        u128 r = 0;
+        for (int b = 127; b >= 0; b--) {
+            r = (r << 1) | ((mag >> b) & 1);
+            if (r >= Q) r -= Q;
+        }
+        if (neg && r) r = Q - r;
+        uint8_t *o = outb + ((size_t)c * 648 + t) * crt.width;
+        for (int b = 0; b < crt.width; b++) o[b] = (uint8_t)(r >> (8 * b));
+    }
+}
+
+// =============================================================================================
+// Host side of the device passes
+// =============================================================================================
+
+bool lbn::Shape::of(int log_vars, Shape &s) {
+    if (log_vars < 19 || log_vars > 31) return false;
+    s.log_vars = log_vars;
+    s.log_packed = log_vars - 7;
+    s.row_log = s.log_packed - COLUMN_LOG;
+    s.n_words = (size_t)1 << s.log_packed;
+    s.column_len = (size_t)1 << s.row_log;
+    s.ring_elements = s.column_len / 4;
+    s.table_bytes = ((size_t)1 << log_vars) / 8;
+    return true;
+}
+
+int lbn::dev_init(lbn_ctx *ctx) {
+    auto *d = new DeviceState();
+    ctx->dev = d;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
+    const Shape &s = ctx->shape;
+    HIPCHK(ctx, hipMalloc(&d->table, s.n_words * 16));
+    HIPCHK(ctx, hipMalloc(&d->E[0], s.n_words * 16));
+    HIPCHK(ctx, hipMalloc(&d->E[1], s.n_words / 2 * 16));
+    HIPCHK(ctx, hipMalloc(&d->P[0], s.n_words / 2 * 24));
+    HIPCHK(ctx, hipMalloc(&d->P[1], s.n_words / 4 * 24));
+    HIPCHK(ctx, hipMalloc(&d->T, 2048 * 4));
+    HIPCHK(ctx, hipMalloc(&d->EH, ((size_t)1 << (s.log_packed - 12)) * 16));
+    HIPCHK(ctx, hipMalloc(&d->EL, 4096 * 16));
+    HIPCHK(ctx, hipMalloc(&d->eqrow, s.column_len * 24));
+    HIPCHK(ctx, hipMalloc(&d->eqlo, 256 * 24));
+    HIPCHK(ctx, hipMalloc(&d->rowU, (size_t)COLUMNS * 128 * 24));
+    HIPCHK(ctx, hipMalloc(&d->rowOut, (size_t)COLUMNS * 24));
+    HIPCHK(ctx, hipMalloc(&d->fold_flag, 4));
+    HIPCHK(ctx, hipMalloc(&d->fold_off, (2 * 162 + 1) * sizeof(int)));
+    HIPCHK(ctx, hipMalloc(&d->fold_cols, (size_t)COLUMNS * 32 * sizeof(uint16_t)));
+    HIPCHK(ctx, hipMalloc(&d->fold_out, s.ring_elements * N * sizeof(int32_t)));
+    HIPCHK(ctx, hipMalloc(&d->ntt_tabs, (size_t)MAX_PRIMES * tab::SIZE * 4));
+    HIPCHK(ctx, hipMalloc(&d->cacc, (size_t)MAX_PRIMES * COLUMNS * N * 8));
+    HIPCHK(ctx, hipMalloc(&d->cbytes, (size_t)COLUMNS * N * 16));
+    for (auto &e : d->ev) HIPCHK(ctx, hipEventCreate(&e));
+    return LBN_OK;
+}
+
+void lbn::dev_free(lbn_ctx *ctx) {
+    DeviceState *d = ctx->dev;
+    if (!d) return;
+    hipSetDevice(ctx->device);
+    void *ptrs[] = {d->table, d->E[0], d->E[1], d->P[0], d->P[1], d->T, d->EH, d->EL, d->eqrow, d->eqlo,
+                    d->rowU, d->rowOut, d->fold_flag, d->fold_off, d->fold_cols, d->fold_out, d->ntt_tabs,
+                    d->ahat, d->cacc, d->cbytes, d->amat};
+    for (void *p : ptrs)
+        if (p) hipFree(p);
+    for (auto &e : d->ev)
+        if (e) hipEventDestroy(e);
+    if (d->stream) hipStreamDestroy(d->stream);
+    delete d;
+    ctx->dev = nullptr;
+}
+
+extern "C" int lbn_load_table(lbn_ctx *ctx, const void *host_bytes, size_t bytes) {
+    if (!ctx || !host_bytes) return LBN_E_ARG;
+    if (bytes != ctx->shape.table_bytes)
+        return ctx->fail(LBN_E_ARG, "witness table has the wrong length");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->dev->table, host_bytes, bytes, hipMemcpyHostToDevice, ctx->dev->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->dev->stream));
+    ctx->table_loaded = true;
+    ctx->partials_done = false;
+    ctx->round = -1;
+    return LBN_OK;
+}
+
+extern "C" int lbn_load_table_device(lbn_ctx *ctx, const void *device_bytes, size_t bytes) {
+    if (!ctx || !device_bytes) return LBN_E_ARG;
+    if (bytes != ctx->shape.table_bytes)
+        return ctx->fail(LBN_E_ARG, "witness table has the wrong length");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->dev->table, device_bytes, bytes, hipMemcpyDeviceToDevice, ctx->dev->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->dev->stream));
+    ctx->table_loaded = true;
+    ctx->partials_done = false;
+    ctx->round = -1;
+    return LBN_OK;
+}
+
+extern "C" int lbn_generate_table(lbn_ctx *ctx, const uint8_t seed[32]) {
+    if (!ctx || !seed) return LBN_E_ARG;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    static const char CTXT[] = "labinius-fast/binary-table/v1";
+    B3Root root{};
+    for (int i = 0; i < 8; i++)
+        root.cv[i] = (uint32_t)seed[4 * i] | ((uint32_t)seed[4 * i + 1] << 8) | ((uint32_t)seed[4 * i + 2] << 16) |
+                     ((uint32_t)seed[4 * i + 3] << 24);
+    uint8_t block[64] = {0};
+    memcpy(block, CTXT, sizeof CTXT - 1);
+    for (int i = 0; i < 16; i++)
+        root.m[i] = (uint32_t)block[4 * i] | ((uint32_t)block[4 * i + 1] << 8) | ((uint32_t)block[4 * i + 2] << 16) |
+                    ((uint32_t)block[4 * i + 3] << 24);
+    root.len = sizeof CTXT - 1;
+    root.flags = 16u /*KEYED_HASH*/ | 1u /*CHUNK_START*/ | 2u /*CHUNK_END*/ | 8u /*ROOT*/;
+    const size_t blocks = ctx->shape.table_bytes / 64;
+    tstart(ctx, T_TABLE);
+    k_table_xof<<<(unsigned)((blocks + 255) / 256), 256, 0, ctx->dev->stream>>>(root, (uint32_t *)ctx->dev->table,
+                                                                              blocks);
+    tstop(ctx, T_TABLE);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipStreamSynchronize(ctx->dev->stream));
+    tcollect(ctx, T_TABLE);
+    ctx->table_loaded = true;
+    ctx->partials_done = false;
+    ctx->round = -1;
+    return LBN_OK;
+}
+
+// ----------------------------------------------------------------------------- modulus, commit
+
+static u128 mul_mod_small(u128 a, uint32_t b, u128 m) { return (a * b) % m; }
+
+extern "C" int lbn_set_modulus(lbn_ctx *ctx, uint64_t q_lo, uint64_t q_hi) {
+    if (!ctx) return LBN_E_ARG;
+    const u128 q = ((u128)q_hi << 64) | q_lo;
+    if (q < (u128)34816965049ull || q >= ((u128)1 << 96))
+        return ctx->fail(LBN_E_MODULUS, "commitment modulus outside [34816965049, 2^96)");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    DeviceState *d = ctx->dev;
+    const Shape &s = ctx->shape;
+    // Primes: the largest 31-bit primes = 1 (mod 1944), with a primitive 1944-th root each.
+    static const PrimeInfo PR[MAX_PRIMES] = {
+        {2147478481u, 0, 0, 1158804115u}, {2147449321u, 0, 0, 738766195u},
+        {2147429881u, 0, 0, 878989323u},  {2147425993u, 0, 0, 1104246913u}};
+    // |coefficient| <= RE * 1024 * (Q - 1): CRT range must exceed twice that.
+    u128 bound = (u128)s.ring_elements * 1024 * (q - 1);
+    int np = 0;
+    u128 M = 1;
+    bool big = false;
+    while (np < MAX_PRIMES) {
+        M *= PR[np].p;
+        np++;
+        if (M / 2 > bound) {
+            big = true;
+            break;
+        }
+    }
+    if (!big) return ctx->fail(LBN_E_MODULUS, "modulus too large for the CRT range");
+    ctx->nprimes = np;
+    std::vector<uint32_t> all((size_t)MAX_PRIMES * tab::SIZE, 0);
+    for (int k = 0; k < np; k++) {
+        PrimeInfo pi = PR[k];
+        uint32_t inv = 1;  // p^-1 mod 2^32 by Newton
+        for (int it = 0; it < 5; it++) inv *= 2 - pi.p * inv;
+        pi.pinv_neg = (uint32_t)(0u - inv);
+        pi.r2 = (uint32_t)(((u128)1 << 64) % pi.p);
+        ctx->primes[k] = pi;
+        std::vector<uint32_t> t;
+        if (!prime_tables(pi, t)) return ctx->fail(LBN_E_ARG, "NTT table construction failed");
+        std::copy(t.begin(), t.end(), all.begin() + (size_t)k * tab::SIZE);
+    }
+    HIPCHK(ctx, hipMemcpyAsync(d->ntt_tabs, all.data(), all.size() * 4, hipMemcpyHostToDevice, d->stream));
+    // The public matrix (rq.rs:114-144), expanded on the host, reduced and transformed here.
+    std::vector<u128> A(s.ring_elements * N);
+    ajtai_matrix_host(q, s.ring_elements, A.data());
+    if (!d->amat) HIPCHK(ctx, hipMalloc(&d->amat, s.ring_elements * N * 16));
+    HIPCHK(ctx, hipMemcpyAsync(d->amat, A.data(), A.size() * 16, hipMemcpyHostToDevice, d->stream));
+    const size_t need = (size_t)np * s.ring_elements * N;
+    if (d->ahat_elems < need) {
+        if (d->ahat) HIPCHK(ctx, hipFree(d->ahat));
+        HIPCHK(ctx, hipMalloc(&d->ahat, need * 4));
+        d->ahat_elems = need;
+    }
+    for (int k = 0; k < np; k++) {
+        PC pc{ctx->primes[k].p, ctx->primes[k].pinv_neg};
+        k_ahat<<<(unsigned)s.ring_elements, 256, 0, d->stream>>>(
+            d->amat, d->ntt_tabs + (size_t)k * tab::SIZE, pc, ctx->primes[k].r2,
+            d->ahat + (size_t)k * s.ring_elements * N);
+        HIPCHK(ctx, hipGetLastError());
+    }
+    HIPCHK(ctx, hipStreamSynchronize(d->stream));
+    ctx->q = q;
+    int bits = 0;
+    for (u128 x = q - 1; x; x >>= 1) bits++;
+    ctx->width = (bits + 7) / 8;
+    ctx->modulus_set = true;
+    (void)mul_mod_small;
+    return LBN_OK;
+}
+
+extern "C" int lbn_commit_primes(const lbn_ctx *ctx) { return ctx ? ctx->nprimes : 0; }
+
+static uint32_t powmod32(uint64_t a, uint64_t e, uint32_t p) {
+    uint64_t r = 1;
+    a %= p;
+    while (e) {
+        if (e & 1) r = r * a % p;
+        a = a * a % p;
+        e >>= 1;
+    }
+    return (uint32_t)r;
+}
+
+extern "C" int lbn_commit(lbn_ctx *ctx, uint8_t *out, size_t out_len) {
+    if (!ctx || !out) return LBN_E_ARG;
+    if (!ctx->table_loaded || !ctx->modulus_set) return ctx->fail(LBN_E_STATE, "table and modulus must be loaded");
+    const Shape &s = ctx->shape;
+    const size_t clen_bytes = (size_t)COLUMNS * N * ctx->width;
+    if (out_len != clen_bytes) return ctx->fail(LBN_E_ARG, "commitment buffer has the wrong length");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    DeviceState *d = ctx->dev;
+    const int np = ctx->nprimes;
+    fill_u32(d->stream, d->cacc, (size_t)np * COLUMNS * N * 2);
+    const int re = (int)s.ring_elements;
+    int msplit = std::max(1, std::min(re, 2048 / (COLUMNS / TC)));  // ~2048 blocks per prime
+    int m_per = (re + msplit - 1) / msplit;
+    msplit = (re + m_per - 1) / m_per;
+    tstart(ctx, T_MAC);
+    for (int k = 0; k < np; k++) {
+        PC pc{ctx->primes[k].p, ctx->primes[k].pinv_neg};
+        dim3 grid(COLUMNS / TC, msplit);
+        k_commit_mac<<<grid, 256, 0, d->stream>>>((const uint32_t *)d->table, s.column_len, m_per, re,
+                                                   d->ahat + (size_t)k * re * N, d->ntt_tabs + (size_t)k * tab::SIZE,
+                                                   pc, d->cacc + (size_t)k * COLUMNS * N);
+        HIPCHK(ctx, hipGetLastError());
+    }
+    tstop(ctx, T_MAC);
+    CrtArg crt{};
+    crt.nprimes = np;
+    crt.width = ctx->width;
+    crt.q_lo = (uint64_t)ctx->q;
+    crt.q_hi = (uint64_t)(ctx->q >> 64);
+    u128 M = 1;
+    for (int i = 0; i < np; i++) {
+        crt.p[i] = ctx->primes[i].p;
+        M *= ctx->primes[i].p;
+        for (int j = 0; j < i; j++) crt.inv[i][j] = powmod32(ctx->primes[j].p % ctx->primes[i].p, ctx->primes[i].p - 2,
+                                                            ctx->primes[i].p);
+    }
+    crt.m_lo = (uint64_t)M;
+    crt.m_hi = (uint64_t)(M >> 64);
+    PC pcs[MAX_PRIMES];
+    for (int i = 0; i < MAX_PRIMES; i++) pcs[i] = PC{i < np ? ctx->primes[i].p : 1u, i < np ? ctx->primes[i].pinv_neg : 0u};
+    tstart(ctx, T_FINISH);
+    k_commit_finish<<<COLUMNS, 256, 0, d->stream>>>(d->cacc, d->ntt_tabs, crt, pcs[0], pcs[1], pcs[2], pcs[3],
+                                                      d->cbytes);
+    tstop(ctx, T_FINISH);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipMemcpyAsync(out, d->cbytes, clen_bytes, hipMemcpyDeviceToHost, d->stream));
+    HIPCHK(ctx, hipStreamSynchronize(d->stream));
+    tcollect(ctx, T_MAC);
+    tcollect(ctx, T_FINISH);
+    return LBN_OK;
+}
+
+// ----------------------------------------------------------------------------- partials, switch
+
+extern "C" int lbn_partials(lbn_ctx *ctx, const uint64_t *r_hi, uint64_t *partials) {
+    if (!ctx || !r_hi || !partials) return LBN_E_ARG;
+    if (!ctx->table_loaded) return ctx->fail(LBN_E_STATE, "no table loaded");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    DeviceState *d = ctx->dev;
+    const Shape &s = ctx->shape;
+    const int lp = s.log_packed, nh = lp - 12;
+    ctx->r_hi.assign((const B128 *)r_hi, (const B128 *)r_hi + lp);
+    std::vector<B128> EH((size_t)1 << nh), EL(4096);
+    eq_expand_b128(ctx->r_hi.data(), nh, EH.data());
+    eq_expand_b128(ctx->r_hi.data() + nh, 12, EL.data());
+    HIPCHK(ctx, hipMemcpyAsync(d->EH, EH.data(), EH.size() * 16, hipMemcpyHostToDevice, d->stream));
+    HIPCHK(ctx, hipMemcpyAsync(d->EL, EL.data(), EL.size() * 16, hipMemcpyHostToDevice, d->stream));
+    tstart(ctx, T_EQ);
+    k_eq_expand<<<(unsigned)EH.size(), 256, 0, d->stream>>>(d->EH, d->EL, d->E[0]);
+    tstop(ctx, T_EQ);
+    HIPCHK(ctx, hipGetLastError());
+    fill_u32(d->stream, d->T + 1536, 512);
+    const unsigned blocks = (unsigned)std::min<size_t>(s.n_words / 256, 1024);
+    tstart(ctx, T_PARTIALS);
+    k_partials<<<blocks, 256, 0, d->stream>>>(d->table, d->E[0], s.n_words, d->T + 1536);
+    tstop(ctx, T_PARTIALS);
+    HIPCHK(ctx, hipGetLastError());
+    uint32_t out[512];
+    HIPCHK(ctx, hipMemcpyAsync(out, d->T + 1536, sizeof out, hipMemcpyDeviceToHost, d->stream));
+    HIPCHK(ctx, hipStreamSynchronize(d->stream));
+    tcollect(ctx, T_EQ);
+    tcollect(ctx, T_PARTIALS);
+    for (int k = 0; k < 128; k++) {
+        partials[2 * k] = (uint64_t)out[4 * k] | ((uint64_t)out[4 * k + 1] << 32);
+        partials[2 * k + 1] = (uint64_t)out[4 * k + 2] | ((uint64_t)out[4 * k + 3] << 32);
+    }
+    ctx->partials_done = true;
+    ctx->round = -1;
+    return LBN_OK;
+}
+
+// T holds 128 values of V words: e = sum_k L[k] * T_k (F162 products on the host).
+static F162 combine(const std::vector<F162> &L, const uint32_t *T, int V) {
+    F162 acc{{0, 0, 0}};
+    for (int k = 0; k < 128; k++) {
+        const uint32_t *t = T + V * k;
+        F162 x{{(uint64_t)t[0] | ((uint64_t)t[1] << 32), (uint64_t)t[2] | ((uint64_t)t[3] << 32),
+                V == 6 ? ((uint64_t)t[4] | ((uint64_t)t[5] << 32)) : 0}};
+        if (x.w[0] | x.w[1] | x.w[2]) acc = f162_add(acc, f162_mul(L[k], x));
+    }
+    return acc;
+}
+
+static int launch_blocks(size_t items) { return (int)std::max<size_t>(1, std::min<size_t>((items + 255) / 256, 1024)); }
+
+extern "C" int lbn_switch_begin(lbn_ctx *ctx, const uint64_t *batch) {
+    if (!ctx || !batch) return LBN_E_ARG;
+    if (!ctx->partials_done) return ctx->fail(LBN_E_STATE, "lbn_partials must run first");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    DeviceState *d = ctx->dev;
+    ctx->lmap.assign((const F162 *)batch, (const F162 *)batch + 128);
+    ctx->round = 0;
+    const size_t h = ctx->shape.n_words / 2;
+    fill_u32(d->stream, d->T, 1536);
+    tstart(ctx, T_SWITCH0);
+    k_switch_msg0<<<launch_blocks(h), 256, 0, d->stream>>>(d->table, d->E[0], h, d->E[1], d->T, d->T + 768);
+    tstop(ctx, T_SWITCH0);
+    HIPCHK(ctx, hipGetLastError());
+    return LBN_OK;
+}
+
+extern "C" int lbn_switch_msg(lbn_ctx *ctx, uint64_t *msg) {
+    if (!ctx || !msg) return LBN_E_ARG;
+    if (ctx->round < 0 || ctx->round >= ctx->shape.log_packed) return ctx->fail(LBN_E_STATE, "no switch round pending");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    DeviceState *d = ctx->dev;
+    uint32_t T[1536];
+    HIPCHK(ctx, hipMemcpyAsync(T, d->T, sizeof T, hipMemcpyDeviceToHost, d->stream));
+    HIPCHK(ctx, hipStreamSynchronize(d->stream));
+    tcollect(ctx, ctx->round == 0 ? T_SWITCH0 : T_ROUNDS);
+    const int V = ctx->round == 0 ? 4 : 6;
+    F162 e0 = combine(ctx->lmap, T, V), einf = combine(ctx->lmap, T + 768, V);
+    memcpy(msg, e0.w, 24);
+    memcpy(msg + 3, einf.w, 24);
+    return LBN_OK;
+}
+
+extern "C" int lbn_switch_fold(lbn_ctx *ctx, const uint64_t *rr) {
+    if (!ctx || !rr) return LBN_E_ARG;
+    const int lp = ctx->shape.log_packed;
+    if (ctx->round < 0 || ctx->round >= lp) return ctx->fail(LBN_E_STATE, "no switch round pending");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    DeviceState *d = ctx->dev;
+    const int r = ctx->round;
+    F162 rho{{rr[0], rr[1], rr[2]}};
+    // L_{r+1}(y) = L_r((1 + c) y) + rho L_r(y), c = r_hi[r].
+    {
+        const B128 one_c = b128_add(ctx->r_hi[r], B128{{1, 0}});
+        std::vector<F162> next(128);
+        B128 z = one_c;  // (1 + c) x^k
+        for (int k = 0; k < 128; k++) {
+            F162 acc{{0, 0, 0}};
+            for (int m = 0; m < 128; m++)
+                if ((z.w[m >> 6] >> (m & 63)) & 1) acc = f162_add(acc, ctx->lmap[m]);
+            next[k] = f162_add(acc, f162_mul(rho, ctx->lmap[k]));
+            uint64_t carry = z.w[1] >> 63;  // z *= x
+            z.w[1] = (z.w[1] << 1) | (z.w[0] >> 63);
+            z.w[0] = (z.w[0] << 1) ^ (carry ? 0x87ull : 0ull);
+        }
+        ctx->lmap.swap(next);
+    }
+    const size_t n = ctx->shape.n_words;
+    const size_t h = n >> (r + 1);  // current half
+    if (r == lp - 1) {
+        // p^(lp-1) has two elements: opened = p0 + rho (p0 + p1).
+        uint32_t two[12];
+        const uint32_t *src = (r == 0) ? (const uint32_t *)d->table : d->P[(r - 1) & 1];
+        if (r == 0) {
+            uint32_t w[8];
+            HIPCHK(ctx, hipMemcpyAsync(w, src, 32, hipMemcpyDeviceToHost, d->stream));
+            HIPCHK(ctx, hipStreamSynchronize(d->stream));
+            for (int u = 0; u < 2; u++)
+                for (int k = 0; k < 6; k++) two[6 * u + k] = k < 4 ? w[4 * u + k] : 0;
+        } else {
+            HIPCHK(ctx, hipMemcpyAsync(two, src, 48, hipMemcpyDeviceToHost, d->stream));
+            HIPCHK(ctx, hipStreamSynchronize(d->stream));
+        }
+        F162 p0{{(uint64_t)two[0] | ((uint64_t)two[1] << 32), (uint64_t)two[2] | ((uint64_t)two[3] << 32),
+                 (uint64_t)two[4] | ((uint64_t)two[5] << 32)}};
+        F162 p1{{(uint64_t)two[6] | ((uint64_t)two[7] << 32), (uint64_t)two[8] | ((uint64_t)two[9] << 32),
+                 (uint64_t)two[10] | ((uint64_t)two[11] << 32)}};
+        ctx->opened = f162_add(p0, f162_mul(rho, f162_add(p0, p1)));
+        ctx->round = lp;
+        return LBN_OK;
+    }
+    const size_t h2 = h / 2;
+    RhoArg ra{{rr[0], rr[1], rr[2]}};
+    const uint4 *Ein = d->E[(r + 1) & 1];
+    uint4 *Eout = d->E[r & 1];
+    uint32_t *pout = d->P[r & 1];
+    fill_u32(d->stream, d->T, 1536);
+    tstart(ctx, T_ROUNDS);
+    if (r == 0)
+        k_switch_round<true><<<launch_blocks(h2), 256, 0, d->stream>>>((const uint32_t *)d->table, h2, ra, Ein, Eout,
+                                                                        pout, d->T, d->T + 768);
+    else
+        k_switch_round<false><<<launch_blocks(h2), 256, 0, d->stream>>>(d->P[(r - 1) & 1], h2, ra, Ein, Eout, pout,
+                                                                         d->T, d->T + 768);
+    tstop(ctx, T_ROUNDS);
+    HIPCHK(ctx, hipGetLastError());
+    ctx->round = r + 1;
+    return LBN_OK;
+}
+
+extern "C" int lbn_switch_final(lbn_ctx *ctx, uint64_t *opened) {
+    if (!ctx || !opened) return LBN_E_ARG;
+    if (ctx->round != ctx->shape.log_packed) return ctx->fail(LBN_E_STATE, "switch rounds not finished");
+    memcpy(opened, ctx->opened.w, 24);
+    return LBN_OK;
+}
+
+// ----------------------------------------------------------------------------- row evaluation
+
+extern "C" int lbn_row_eval(lbn_ctx *ctx, const uint64_t *p0, int n_p0, uint64_t *row) {
+    if (!ctx || !p0 || !row) return LBN_E_ARG;
+    const Shape &s = ctx->shape;
+    if (n_p0 != s.row_log) return ctx->fail(LBN_E_ARG, "p0 must have log_vars - 17 coordinates");
+    if (!ctx->table_loaded) return ctx->fail(LBN_E_STATE, "no table loaded");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    DeviceState *d = ctx->dev;
+    const int lo_bits = std::min(s.row_log, 7), hi_bits = s.row_log - lo_bits;
+    std::vector<F162> lo((size_t)1 << lo_bits), hi((size_t)1 << hi_bits);
+    eq_table_f162((const F162 *)p0, lo_bits, lo.data());
+    eq_table_f162((const F162 *)p0 + lo_bits, hi_bits, hi.data());
+    HIPCHK(ctx, hipMemcpyAsync(d->eqlo, lo.data(), lo.size() * 24, hipMemcpyHostToDevice, d->stream));
+    HIPCHK(ctx, hipMemcpyAsync((uint8_t *)d->eqlo + 128 * 24, hi.data(), hi.size() * 24, hipMemcpyHostToDevice,
+                               d->stream));
+    tstart(ctx, T_ROW);
+    k_eq_row<<<(unsigned)((s.column_len + 255) / 256), 256, 0, d->stream>>>(
+        (const uint64_t *)d->eqlo, (const uint64_t *)((uint8_t *)d->eqlo + 128 * 24), lo_bits, s.column_len, d->eqrow);
+    HIPCHK(ctx, hipGetLastError());
+    k_row<<<COLUMNS, 256, 0, d->stream>>>(d->table, d->eqrow, s.column_len, d->rowU);
+    HIPCHK(ctx, hipGetLastError());
+    k_row_finish<<<COLUMNS, 128, 0, d->stream>>>(d->rowU, (uint64_t *)d->rowOut);
+    tstop(ctx, T_ROW);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipMemcpyAsync(row, d->rowOut, (size_t)COLUMNS * 24, hipMemcpyDeviceToHost, d->stream));
+    HIPCHK(ctx, hipStreamSynchronize(d->stream));
+    tcollect(ctx, T_ROW);
+    return LBN_OK;
+}
+
+// ----------------------------------------------------------------------------- fold
+
+extern "C" int lbn_fold(lbn_ctx *ctx, const lbn_challenge *ch, int n_challenges, int32_t *out, size_t n_out) {
+    if (!ctx || !ch || !out) return LBN_E_ARG;
+    const Shape &s = ctx->shape;
+    if (n_challenges != COLUMNS) return ctx->fail(LBN_E_ARG, "one challenge per column (1024)");
+    if (n_out != s.ring_elements * N) return ctx->fail(LBN_E_ARG, "fold output has the wrong length");
+    if (!ctx->table_loaded) return ctx->fail(LBN_E_STATE, "no table loaded");
+    // Columns per challenge position, positives then negatives (sign = sign_t xor (pos odd)).
+    std::vector<int> cnt(2 * 162, 0);
+    for (int j = 0; j < COLUMNS; j++) {
+        if (ch[j].weight > 32) return ctx->fail(LBN_E_ARG, "challenge weight above 32");
+        for (uint32_t t = 0; t < ch[j].weight; t++) {
+            const int pos = ch[j].positions[t];
+            if (pos >= 162) return ctx->fail(LBN_E_ARG, "challenge position above 161");
+            const int neg = (int)((ch[j].signs >> t) & 1) ^ (pos & 1);
+            cnt[2 * pos + neg]++;
+        }
+    }
+    std::vector<int> off(2 * 162 + 1, 0);
+    for (int i = 0; i < 2 * 162; i++) off[i + 1] = off[i] + cnt[i];
+    std::vector<uint16_t> cols(std::max(1, off[2 * 162]));
+    std::vector<int> fillp(off.begin(), off.end() - 1);
+    for (int j = 0; j < COLUMNS; j++)
+        for (uint32_t t = 0; t < ch[j].weight; t++) {
+            const int pos = ch[j].positions[t];
+            const int neg = (int)((ch[j].signs >> t) & 1) ^ (pos & 1);
+            cols[fillp[2 * pos + neg]++] = (uint16_t)j;
+        }
+    const int maxlen = *std::max_element(cnt.begin(), cnt.end());
+    int nb = 1;
+    while ((1 << nb) <= maxlen) nb++;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    DeviceState *d = ctx->dev;
+    HIPCHK(ctx, hipMemcpyAsync(d->fold_off, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice, d->stream));
+    HIPCHK(ctx, hipMemcpyAsync(d->fold_cols, cols.data(), cols.size() * 2, hipMemcpyHostToDevice, d->stream));
+    fill_u32(d->stream, d->fold_flag, 1);
+    const unsigned grid = (unsigned)s.ring_elements;
+    const uint32_t *t32 = (const uint32_t *)d->table;
+    tstart(ctx, T_FOLD);
+#define FOLD_CASE(B)                                                                                   \
+    case B:                                                                                            \
+        k_fold<B><<<grid, 256, 0, d->stream>>>(t32, s.column_len, d->fold_off, d->fold_cols, d->fold_out, \
+                                               d->fold_flag);                                          \
+        break;
+    switch (std::max(nb, 6)) {
+        FOLD_CASE(6)
+        FOLD_CASE(7)
+        FOLD_CASE(8)
+        FOLD_CASE(9)
+        FOLD_CASE(10)
+        FOLD_CASE(11)
+        default:
+            return ctx->fail(LBN_E_ARG, "challenge lists too long");
+    }
+#undef FOLD_CASE
+    tstop(ctx, T_FOLD);
+    HIPCHK(ctx, hipGetLastError());
+    uint32_t flag = 0;
+    HIPCHK(ctx, hipMemcpyAsync(out, d->fold_out, n_out * 4, hipMemcpyDeviceToHost, d->stream));
+    HIPCHK(ctx, hipMemcpyAsync(&flag, d->fold_flag, 4, hipMemcpyDeviceToHost, d->stream));
+    HIPCHK(ctx, hipStreamSynchronize(d->stream));
+    tcollect(ctx, T_FOLD);
+    if (flag) return ctx->fail(LBN_E_RANGE, "a fold coefficient does not fit the wire codec");
+    return LBN_OK;
+}
+
+// ----------------------------------------------------------------------------- timing
+
+extern "C" int lbn_set_timing(lbn_ctx *ctx, int enable) {
+    if (!ctx) return LBN_E_ARG;
+    ctx->timing = enable != 0;
+    return LBN_OK;
+}
+extern "C" int lbn_reset_timing(lbn_ctx *ctx) {
+    if (!ctx) return LBN_E_ARG;
+    ctx->times = lbn_timing{};
+    return LBN_OK;
+}
+extern "C" int lbn_get_timing(const lbn_ctx *ctx, lbn_timing *out) {
+    if (!ctx || !out) return LBN_E_ARG;
+    *out = ctx->times;
+    return LBN_OK;
+}

@@ -1,0 +1,552 @@
+// Host half of liblabinius_gpu: binary-field products (PCLMULQDQ), the Fiat-Shamir transcript,
+// the short-challenge sampler, the public matrix expansion, the NTT constant tables, and the
+// whole-proof orchestration lbn_prove (the order of solution/proof_generate/src/main.rs:1692-2009
+// and verifier/proof_verify/src/reference.rs:137-177). The table-sized passes run on the GPU.
+#include <immintrin.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "lbn_blake3.h"
+#include "lbn_internal.h"
+
+namespace lbn {
+
+// ============================================================================= fields
+
+static inline __m128i clmul(uint64_t a, uint64_t b) {
+    return _mm_clmulepi64_si128(_mm_set_epi64x(0, (long long)a), _mm_set_epi64x(0, (long long)b), 0x00);
+}
+static inline u128 to_u128(__m128i x) {
+    return (u128)(uint64_t)_mm_cvtsi128_si64(x) | ((u128)(uint64_t)_mm_extract_epi64(x, 1) << 64);
+}
+
+// GHASH basis: x^128 = x^7 + x^2 + x + 1 (pcs/src/fields/scalar.rs:4, 46-68).
+B128 b128_mul(B128 a, B128 b) {
+    u128 lo = to_u128(clmul(a.w[0], b.w[0]));
+    u128 hi = to_u128(clmul(a.w[1], b.w[1]));
+    u128 mid = to_u128(clmul(a.w[0], b.w[1])) ^ to_u128(clmul(a.w[1], b.w[0]));
+    lo ^= mid << 64;
+    hi ^= mid >> 64;
+    u128 t = hi ^ (hi << 1) ^ (hi << 2) ^ (hi << 7);
+    u128 o = (hi >> 127) ^ (hi >> 126) ^ (hi >> 121);
+    lo ^= t ^ o ^ (o << 1) ^ (o << 2) ^ (o << 7);
+    return B128{{(uint64_t)lo, (uint64_t)(lo >> 64)}};
+}
+
+// x^162 = x^81 + 1 (pcs/src/fields/scalar.rs:97-121).
+F162 f162_mul(F162 a, F162 b) {
+    uint64_t v[6] = {0, 0, 0, 0, 0, 0};
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            u128 t = to_u128(clmul(a.w[i], b.w[j]));
+            v[i + j] ^= (uint64_t)t;
+            v[i + j + 1] ^= (uint64_t)(t >> 64);
+        }
+    for (int it = 0; it < 2; it++) {
+        uint64_t h0 = (v[2] >> 34) | (v[3] << 30), h1 = (v[3] >> 34) | (v[4] << 30), h2 = (v[4] >> 34) | (v[5] << 30);
+        v[2] &= (1ull << 34) - 1;
+        v[3] = v[4] = v[5] = 0;
+        v[0] ^= h0, v[1] ^= h1, v[2] ^= h2;
+        v[1] ^= h0 << 17;
+        v[2] ^= (h0 >> 47) | (h1 << 17);
+        v[3] ^= (h1 >> 47) | (h2 << 17);
+        v[4] ^= h2 >> 47;
+    }
+    return F162{{v[0], v[1], v[2]}};
+}
+
+void eq_expand_b128(const B128 *r, int n, B128 *t) {
+    t[0] = B128{{1, 0}};
+    for (int round = 0; round < n; round++)
+        for (long long i = (1ll << round) - 1; i >= 0; i--) {
+            B128 hi = b128_mul(t[i], r[round]);
+            t[2 * i + 1] = hi;
+            t[2 * i] = b128_add(t[i], hi);
+        }
+}
+
+void eq_expand_f162(const F162 *r, int n, F162 *t) {
+    t[0] = F162{{1, 0, 0}};
+    for (int round = 0; round < n; round++)
+        for (long long i = (1ll << round) - 1; i >= 0; i--) {
+            F162 hi = f162_mul(t[i], r[round]);
+            t[2 * i + 1] = hi;
+            t[2 * i] = f162_add(t[i], hi);
+        }
+}
+
+void eq_table_f162(const F162 *p, int n, F162 *t) {
+    t[0] = F162{{1, 0, 0}};
+    size_t len = 1;
+    for (int k = 0; k < n; k++, len *= 2)
+        for (size_t b = 0; b < len; b++) {
+            F162 y = f162_mul(t[b], p[k]);
+            t[b + len] = y;
+            t[b] = f162_add(t[b], y);
+        }
+}
+
+static int host_threads() {
+    unsigned n = std::thread::hardware_concurrency();
+    return (int)std::max(1u, std::min(n ? n : 1u, 16u));
+}
+
+template <class F>
+static void parallel_for(size_t n, F f) {
+    int nt = (int)std::min<size_t>((size_t)host_threads(), n);
+    if (nt <= 1) {
+        for (size_t i = 0; i < n; i++) f(i);
+        return;
+    }
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nt; t++)
+        pool.emplace_back([&, t] {
+            for (size_t i = t; i < n; i += nt) f(i);
+        });
+    for (auto &th : pool) th.join();
+}
+
+// ============================================================================= transcript
+// pcs/src/challenge.rs:226-326 and 724-749: one BLAKE3 over the absorbed stream; a derivation
+// hashes stream || u64le(counter) || u64le(|label|) || label and advances the counter.
+
+class Transcript {
+   public:
+    explicit Transcript(const char *domain) {
+        size_t n = strlen(domain);
+        h_.update_u64(n);
+        h_.update(domain, n);
+    }
+    void absorb_bytes(const void *b, size_t n, int threads = 1) {
+        h_.update_u64(n);
+        if (threads > 1)
+            h_.update_parallel(b, n, threads);
+        else
+            h_.update(b, n);
+    }
+    void absorb_u64(uint64_t x) { h_.update_u64(x); }
+    void absorb_b128(const B128 *xs, size_t n) {
+        h_.update_u64(n);
+        for (size_t i = 0; i < n; i++) h_.update_u64(xs[i].w[0]), h_.update_u64(xs[i].w[1]);
+    }
+    void absorb_f162(const F162 *xs, size_t n) {
+        h_.update_u64(n);
+        for (size_t i = 0; i < n; i++)
+            for (int k = 0; k < 3; k++) h_.update_u64(xs[i].w[k]);
+    }
+    Blake3 reader_at(const char *label, uint64_t offset) const {
+        Blake3 r = h_;
+        size_t n = strlen(label);
+        r.update_u64(counter_ + offset);
+        r.update_u64(n);
+        r.update(label, n);
+        return r;
+    }
+    Blake3 reader(const char *label) {
+        Blake3 r = reader_at(label, 0);
+        counter_++;
+        return r;
+    }
+    void skip(uint64_t n) { counter_ += n; }
+    std::vector<B128> sample_b128(const char *label, int n) {
+        std::vector<uint8_t> b((size_t)16 * n);
+        reader(label).xof(0, b.data(), b.size());
+        std::vector<B128> out(n);
+        memcpy(out.data(), b.data(), b.size());
+        return out;
+    }
+    std::vector<F162> sample_f162(const char *label, int n) {
+        std::vector<uint8_t> b((size_t)24 * n);
+        reader(label).xof(0, b.data(), b.size());
+        std::vector<F162> out(n);
+        memcpy(out.data(), b.data(), b.size());
+        for (auto &x : out) x.w[2] &= (1ull << 34) - 1;
+        return out;
+    }
+
+   private:
+    Blake3 h_;
+    uint64_t counter_ = 0;
+};
+
+// ============================================================================= short challenges
+// pcs/src/challenge.rs: Fisher-Yates attempt 662-688 over one XOF stream, keyed sign word
+// 429-441, rejection by max_u |c(zeta^u)|^2 <= 144 + 1e-12 in f64 with the reference's order of
+// operations (565-623): per root, terms added in position order from 0.0, then re*re + im*im.
+
+struct ChallengeSampler {
+    double cosv[243], sinv[243];
+    int units[81];
+    Blake3 sign_key;
+    ChallengeSampler() : sign_key(Blake3::derive_key("labinius 2026 challenge signs v1")) {
+        for (int m = 0; m < 243; m++) {
+            volatile double angle = 2.0 * M_PI * (double)m / 243.0;
+            cosv[m] = std::cos(angle);
+            sinv[m] = std::sin(angle);
+        }
+        int n = 0;
+        for (int x = 1; x < 243 && n < 81; x++)
+            if (x % 3) units[n++] = x;
+    }
+    static uint32_t below(XofReader &x, uint32_t n) {
+        const uint32_t limit = (65536u / n) * n;
+        for (;;) {
+            uint32_t lo = x.byte(), hi = x.byte();
+            uint32_t r = lo | (hi << 8);
+            if (r < limit) return r % n;
+        }
+    }
+    void attempt(XofReader &x, lbn_challenge &c) const {
+        uint8_t perm[162];
+        uint64_t seen[3] = {0, 0, 0};
+        for (int i = 0; i < 162; i++) perm[i] = (uint8_t)i;
+        for (int i = 0; i < WEIGHT; i++) {
+            int j = i + (int)below(x, (uint32_t)(162 - i));
+            uint8_t p = perm[j];
+            perm[j] = perm[i];
+            seen[p >> 6] |= 1ull << (p & 63);
+        }
+        memset(&c, 0, sizeof c);
+        c.weight = WEIGHT;
+        int n = 0;
+        for (int w = 0; w < 3; w++)
+            for (uint64_t b = seen[w]; b; b &= b - 1) c.positions[n++] = (uint8_t)(64 * w + __builtin_ctzll(b));
+        Blake3 h = sign_key;
+        h.update_u64(WEIGHT);
+        h.update(c.positions, WEIGHT);
+        uint8_t o[4];
+        h.xof(0, o, 4);
+        uint32_t word = (uint32_t)o[0] | ((uint32_t)o[1] << 8) | ((uint32_t)o[2] << 16) | ((uint32_t)o[3] << 24);
+        c.signs = word & ((1u << WEIGHT) - 1);
+    }
+    bool within(const lbn_challenge &c, double bound_sq) const {
+        for (int k = 0; k < 81; k++) {
+            double re = 0.0, im = 0.0;
+            for (uint32_t t = 0; t < c.weight; t++) {
+                const int m = (c.positions[t] * units[k]) % 243;
+                const bool neg = (c.signs >> t) & 1;
+                re = re + (neg ? -cosv[m] : cosv[m]);
+                im = im + (neg ? -sinv[m] : sinv[m]);
+            }
+            const double a = re * re, b = im * im;
+            if (a + b > bound_sq) return false;
+        }
+        return true;
+    }
+    lbn_challenge sample(const Blake3 &reader) const {
+        const double bound = 12.0, bound_sq = bound * bound + 1e-12;
+        XofReader x(reader);
+        lbn_challenge c;
+        for (;;) {
+            attempt(x, c);
+            if (within(c, bound_sq)) return c;
+        }
+    }
+};
+
+// ============================================================================= public matrix
+// rq::ajtai_matrix (verifier/proof_verify/src/rq.rs:53-144): element i from BLAKE3 keyed
+// [0x5a; 32] over domain || q (16 LE) || u64(elements) || u64(i); width-byte LE draws masked to
+// bits(q-1), rejected while >= q.
+
+void ajtai_matrix_host(u128 q, size_t elements, u128 *out) {
+    int bits = 0;
+    for (u128 x = q - 1; x; x >>= 1) bits++;
+    const int width = (bits + 7) / 8;
+    const u128 mask = bits == 128 ? ~(u128)0 : (((u128)1 << bits) - 1);
+    uint8_t seed[32];
+    memset(seed, 0x5a, 32);
+    Blake3 base = Blake3::keyed(seed);
+    static const char DOMAIN[] = "labinius-fast/ajtai-matrix/v1";
+    base.update(DOMAIN, sizeof DOMAIN - 1);
+    uint8_t qb[16];
+    for (int i = 0; i < 16; i++) qb[i] = (uint8_t)(q >> (8 * i));
+    base.update(qb, 16);
+    base.update_u64(elements);
+    parallel_for(elements, [&](size_t i) {
+        Blake3 h = base;
+        h.update_u64(i);
+        XofReader x(h);
+        for (int t = 0; t < N; t++) {
+            for (;;) {
+                uint8_t b[16];
+                x.read(b, width);
+                u128 v = 0;
+                for (int k = 0; k < width; k++) v |= (u128)b[k] << (8 * k);
+                v &= mask;
+                if (v < q) {
+                    out[i * N + t] = v;
+                    break;
+                }
+            }
+        }
+    });
+}
+
+// ============================================================================= NTT tables
+
+static uint32_t pw(uint64_t a, uint64_t e, uint32_t p) {
+    uint64_t r = 1;
+    a %= p;
+    while (e) {
+        if (e & 1) r = r * a % p;
+        a = a * a % p;
+        e >>= 1;
+    }
+    return (uint32_t)r;
+}
+
+bool prime_tables(const PrimeInfo &pi, std::vector<uint32_t> &t) {
+    const uint32_t p = pi.p;
+    auto mf = [&](uint32_t x) { return (uint32_t)(((uint64_t)x << 32) % p); };  // Montgomery form
+    auto inv = [&](uint32_t x) { return pw(x, p - 2, p); };
+    const uint32_t w = pi.omega;
+    if (pw(w, 1944, p) != 1 || pw(w, 972, p) == 1 || pw(w, 648, p) == 1) return false;
+    const uint32_t eta[2] = {pw(w, 324, p), pw(w, 1620, p)};
+    const uint32_t beta[2] = {w, pw(w, 5, p)};
+    const uint32_t rho = pw(w, 6, p);
+    // layout must match lbn_device.hip namespace tab
+    const int TWB = 0, TW = TWB + 2 * 324 * 3, ETA = TW + 2 * 324, FWD = ETA + 2, INV = FWD + 323, W3 = INV + 323,
+              UNTW = W3 + 2, ETA0 = UNTW + 2 * 324, DINV = ETA0 + 1, SIZE = DINV + 1;
+    t.assign(SIZE, 0);
+    for (int h = 0; h < 2; h++)
+        for (int i = 0; i < 324; i++) {
+            uint32_t b = pw(beta[h], i, p);
+            t[TWB + (h * 324 + i) * 3 + 0] = b;
+            t[TWB + (h * 324 + i) * 3 + 1] = (uint32_t)((uint64_t)eta[h] * b % p);
+            t[TWB + (h * 324 + i) * 3 + 2] = (uint32_t)((uint64_t)(1 + eta[h]) % p * b % p);
+            t[TW + h * 324 + i] = mf(b);
+            uint32_t ib = pw(inv(beta[h]), i, p);
+            t[UNTW + h * 324 + i] = mf((uint32_t)((uint64_t)ib * inv(324) % p));
+        }
+    t[ETA] = mf(eta[0]);
+    t[ETA + 1] = mf(eta[1]);
+    const int spans[6] = {324, 162, 81, 27, 9, 3}, rads[6] = {2, 2, 3, 3, 3, 3};
+    int at = 0;
+    for (int s = 0; s < 6; s++) {
+        const int span = spans[s], r = rads[s], m = span / r;
+        const uint32_t ws = pw(rho, 324 / span, p), wsi = inv(ws);
+        for (int j = 0; j < m; j++)
+            for (int k = 1; k < r; k++) {
+                t[FWD + at] = mf(pw(ws, (uint64_t)j * k, p));
+                t[INV + at] = mf(pw(wsi, (uint64_t)j * k, p));
+                at++;
+            }
+    }
+    if (at != 323) return false;
+    t[W3] = mf(pw(rho, 108, p));
+    t[W3 + 1] = mf(inv(pw(rho, 108, p)));
+    t[ETA0] = mf(eta[0]);
+    t[DINV] = mf(inv((eta[0] + p - eta[1]) % p));
+    return true;
+}
+
+}  // namespace lbn
+
+using namespace lbn;
+
+// ============================================================================= C ABI (host)
+
+extern "C" int lbn_create(int device, int log_vars, lbn_ctx **out) {
+    if (!out) return LBN_E_ARG;
+    *out = nullptr;
+    auto *ctx = new lbn_ctx();
+    ctx->device = device;
+    if (!Shape::of(log_vars, ctx->shape)) {
+        delete ctx;
+        return LBN_E_ARG;
+    }
+    int rc = dev_init(ctx);
+    if (rc != LBN_OK) {
+        dev_free(ctx);
+        delete ctx;
+        return rc;
+    }
+    *out = ctx;
+    return LBN_OK;
+}
+
+extern "C" void lbn_destroy(lbn_ctx *ctx) {
+    if (!ctx) return;
+    dev_free(ctx);
+    delete ctx;
+}
+
+extern "C" const char *lbn_last_error(const lbn_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+static size_t f162_bytes(size_t n) { return (n * 162 + 7) / 8; }
+
+extern "C" size_t lbn_proof_len(int log_vars, uint64_t q_lo, uint64_t q_hi) {
+    Shape s;
+    if (!Shape::of(log_vars, s)) return 0;
+    u128 q = ((u128)q_hi << 64) | q_lo;
+    if (q < 2) return 0;
+    int bits = 0;
+    for (u128 x = q - 1; x; x >>= 1) bits++;
+    size_t w = (size_t)(bits + 7) / 8;
+    return 4 + 16 + 8 + (size_t)COLUMNS * N * w + 8 + 16 * 128 + 8 + 48 * (size_t)s.log_packed + 24 + 8 +
+           f162_bytes(COLUMNS) + 8 + 4 * (size_t)N * s.ring_elements;
+}
+
+namespace {
+struct Writer {
+    uint8_t *p;
+    size_t at = 0;
+    void bytes(const void *b, size_t n) {
+        memcpy(p + at, b, n);
+        at += n;
+    }
+    void u64(uint64_t x) {
+        for (int i = 0; i < 8; i++) p[at++] = (uint8_t)(x >> (8 * i));
+    }
+};
+double now_us() {
+    return std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+// wire::pack_f162 (pcs/src/wire/mod.rs:445-453): 162 bits per element, least significant first.
+void pack_f162(const F162 *v, size_t n, uint8_t *out) {
+    memset(out, 0, f162_bytes(n));
+    size_t bit = 0;
+    for (size_t i = 0; i < n; i++) {
+        for (int l = 0; l < 3; l++) {
+            const int nb = l < 2 ? 64 : 34;
+            uint64_t x = v[i].w[l];
+            for (int b = 0; b < nb; b++, bit++)
+                if ((x >> b) & 1) out[bit >> 3] |= (uint8_t)(1u << (bit & 7));
+        }
+    }
+}
+}  // namespace
+
+#define TRY(x)                        \
+    do {                              \
+        int rc_ = (x);                \
+        if (rc_ != LBN_OK) return rc_; \
+    } while (0)
+
+extern "C" int lbn_prove(lbn_ctx *ctx, const uint8_t seed[32], uint8_t *proof, size_t cap, size_t *len,
+                         lbn_profile *prof) {
+    if (!ctx || !seed || !proof || !len) return LBN_E_ARG;
+    if (!ctx->table_loaded || !ctx->modulus_set) return ctx->fail(LBN_E_STATE, "table and modulus must be loaded");
+    const Shape &s = ctx->shape;
+    const u128 q = ctx->q;
+    const size_t plen = lbn_proof_len(s.log_vars, (uint64_t)q, (uint64_t)(q >> 64));
+    if (cap < plen) return ctx->fail(LBN_E_ARG, "proof buffer too small");
+    const int lp = s.log_packed;
+    const size_t clen = (size_t)COLUMNS * N * ctx->width;
+    const size_t row_bytes = f162_bytes(COLUMNS);
+    const size_t fold_bytes = 4 * (size_t)N * s.ring_elements;
+    // Field offsets of the bincode layout (verifier/proof_verify/src/main.rs:98-124).
+    const size_t off_commit = 4 + 16 + 8;
+    const size_t off_partials = off_commit + clen;  // u64 len + 128 x u128
+    const size_t off_rounds = off_partials + 8 + 16 * 128;
+    const size_t off_opened = off_rounds + 8 + 48 * (size_t)lp;
+    const size_t off_row = off_opened + 24;
+    const size_t off_fold = off_row + 8 + row_bytes;
+    lbn_profile pf{};
+    const double t0 = now_us();
+
+    TRY(lbn_commit(ctx, proof + off_commit, clen));
+    const double t1 = now_us();
+
+    // verifier/proof_verify/src/main.rs:166-182 (absorb k and 10, the shape of this instance)
+    Transcript t("labinius-fast/v1");
+    t.absorb_bytes(seed, 32);
+    t.absorb_u64((uint64_t)s.log_vars);
+    t.absorb_u64(COLUMN_LOG);
+    uint8_t qb[16];
+    for (int i = 0; i < 16; i++) qb[i] = (uint8_t)(q >> (8 * i));
+    t.absorb_bytes(qb, 16);
+    t.absorb_bytes(proof + off_commit, clen, host_threads());
+    std::vector<B128> r_lo = t.sample_b128("r-lo", 7);
+    std::vector<B128> r_hi = t.sample_b128("r-hi", lp);
+    const double t2 = now_us();
+
+    std::vector<B128> partials(128);
+    TRY(lbn_partials(ctx, (const uint64_t *)r_hi.data(), (uint64_t *)partials.data()));
+    {
+        B128 eq_lo[128];
+        eq_expand_b128(r_lo.data(), 7, eq_lo);
+        B128 claim{{0, 0}};
+        for (int b = 0; b < 128; b++) claim = b128_add(claim, b128_mul(partials[b], eq_lo[b]));
+        t.absorb_b128(&claim, 1);
+        t.absorb_b128(partials.data(), 128);
+    }
+    std::vector<F162> rb = t.sample_f162("switch-batch", 7);
+    std::vector<F162> batch(128);
+    eq_expand_f162(rb.data(), 7, batch.data());
+    const double t3 = now_us();
+
+    std::vector<F162> rounds(2 * (size_t)lp), coords(lp);
+    TRY(lbn_switch_begin(ctx, (const uint64_t *)batch.data()));
+    for (int r = 0; r < lp; r++) {
+        TRY(lbn_switch_msg(ctx, (uint64_t *)&rounds[2 * r]));
+        t.absorb_f162(&rounds[2 * r], 2);
+        coords[r] = t.sample_f162("switch-round", 1)[0];
+        TRY(lbn_switch_fold(ctx, coords[r].w));
+    }
+    F162 opened;
+    TRY(lbn_switch_final(ctx, opened.w));
+    t.absorb_f162(&opened, 1);
+    const double t4 = now_us();
+
+    // EvaluationPoint::msb_first (scheme/commitment.rs:163-174): p0 = reverse(coords[10..]).
+    std::vector<F162> p0(s.row_log), row(COLUMNS);
+    for (int i = 0; i < s.row_log; i++) p0[i] = coords[lp - 1 - i];
+    TRY(lbn_row_eval(ctx, (const uint64_t *)p0.data(), s.row_log, (uint64_t *)row.data()));
+    const double t5 = now_us();
+
+    // RowEvaluation::absorb (scheme/commitment.rs:320-331), then 1024 challenges in parallel,
+    // each one derivation at its own counter (challenge.rs:778-802).
+    {
+        static const char L[] = "labinius/row-evaluation";
+        t.absorb_bytes(L, sizeof L - 1);
+        std::vector<uint8_t> rb2((size_t)24 * COLUMNS);
+        memcpy(rb2.data(), row.data(), rb2.size());
+        t.absorb_bytes(rb2.data(), rb2.size());
+    }
+    static const ChallengeSampler sampler;
+    std::vector<lbn_challenge> ch(COLUMNS);
+    parallel_for(COLUMNS, [&](size_t i) { ch[i] = sampler.sample(t.reader_at("short-challenge", i)); });
+    t.skip(COLUMNS);
+    const double t6 = now_us();
+
+    TRY(lbn_fold(ctx, ch.data(), COLUMNS, (int32_t *)(proof + off_fold + 8), (size_t)N * s.ring_elements));
+    const double t7 = now_us();
+
+    Writer w{proof};
+    const uint32_t version = 1;
+    w.bytes(&version, 4);
+    w.u64((uint64_t)q);
+    w.u64((uint64_t)(q >> 64));
+    w.u64(clen);
+    w.at = off_partials;
+    w.u64(128);
+    for (auto &x : partials) w.u64(x.w[0]), w.u64(x.w[1]);
+    w.u64((uint64_t)lp);
+    for (auto &x : rounds)
+        for (int k = 0; k < 3; k++) w.u64(x.w[k]);
+    for (int k = 0; k < 3; k++) w.u64(opened.w[k]);
+    w.u64(row_bytes);
+    pack_f162(row.data(), COLUMNS, proof + w.at);
+    w.at += row_bytes;
+    w.u64(fold_bytes);
+    *len = plen;
+    const double t8 = now_us();
+    pf.commit_us = t1 - t0;
+    pf.transcript_us = t2 - t1;
+    pf.partials_us = t3 - t2;
+    pf.switch_us = t4 - t3;
+    pf.row_us = t5 - t4;
+    pf.challenges_us = t6 - t5;
+    pf.fold_us = t7 - t6;
+    pf.encode_us = t8 - t7;
+    pf.total_us = t8 - t0;
+    if (prof) *prof = pf;
+    return LBN_OK;
+}

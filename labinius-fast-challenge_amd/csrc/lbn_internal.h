@@ -1,0 +1,87 @@
+// Internal declarations shared by the device (.hip) and host (.cpp) halves of liblabinius_gpu.
+#pragma once
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/labinius_gpu.h"
+
+typedef unsigned __int128 u128;
+
+namespace lbn {
+
+constexpr int N = 648;           // ring degree of R_Q = Z_Q[X]/(X^648 - X^324 + 1)
+constexpr int HALF_N = 324;      // the middle exponent X^324
+constexpr int PACK = 128;        // bits per GHASH word
+constexpr int COLUMN_LOG = 10;   // columns = 1024 (verifier/proof_verify/src/main.rs:37)
+constexpr int COLUMNS = 1 << COLUMN_LOG;
+constexpr int WEIGHT = 28;       // challenge weight (pcs/src/challenge.rs:56)
+constexpr int MAX_PRIMES = 4;    // 31-bit CRT primes for the exact commitment
+
+struct Shape {
+    int log_vars = 0, log_packed = 0, row_log = 0;
+    size_t n_words = 0, column_len = 0, ring_elements = 0, table_bytes = 0;
+    static bool of(int log_vars, Shape &s);
+};
+
+// GF(2^128) (GHASH basis) and F162 = GF(2)[x]/(x^162 + x^81 + 1) on the host, PCLMUL based.
+struct B128 {
+    uint64_t w[2];
+};
+struct F162 {
+    uint64_t w[3];
+};
+B128 b128_mul(B128 a, B128 b);
+F162 f162_mul(F162 a, F162 b);
+inline B128 b128_add(B128 a, B128 b) { return B128{{a.w[0] ^ b.w[0], a.w[1] ^ b.w[1]}}; }
+inline F162 f162_add(F162 a, F162 b) { return F162{{a.w[0] ^ b.w[0], a.w[1] ^ b.w[1], a.w[2] ^ b.w[2]}}; }
+inline F162 f162_lift(B128 a) { return F162{{a.w[0], a.w[1], 0}}; }
+inline bool f162_eq(F162 a, F162 b) { return a.w[0] == b.w[0] && a.w[1] == b.w[1] && a.w[2] == b.w[2]; }
+// eq expansions (crossfield.rs:66-97: first coordinate = most significant index bit) and the
+// LSB-first eq table (eval.rs:51-65).
+void eq_expand_b128(const B128 *r, int n, B128 *out);
+void eq_expand_f162(const F162 *r, int n, F162 *out);
+void eq_table_f162(const F162 *p, int n, F162 *out);
+
+// One CRT prime of the commitment transform.
+struct PrimeInfo {
+    uint32_t p, pinv_neg, r2;  // -p^-1 mod 2^32, R^2 mod p (R = 2^32)
+    uint32_t omega;            // primitive 1944-th root of unity
+};
+
+// Device-side state (owned by lbn_ctx; defined in lbn_device.hip).
+struct DeviceState;
+
+}  // namespace lbn
+
+struct lbn_ctx {
+    int device = 0;
+    lbn::Shape shape;
+    std::string err;
+    bool table_loaded = false;
+    bool modulus_set = false;
+    u128 q = 0;
+    int width = 0;
+    int nprimes = 0;
+    lbn::PrimeInfo primes[lbn::MAX_PRIMES];
+    // switch state kept on the host: r_hi, the linear map a = L(E) and the round index.
+    std::vector<lbn::B128> r_hi;
+    std::vector<lbn::F162> lmap;  // 128 entries
+    int round = -1;
+    bool partials_done = false;
+    lbn::F162 opened{};
+    lbn::DeviceState *dev = nullptr;
+    bool timing = false;
+    lbn_timing times{};
+    int fail(int code, const std::string &msg) {
+        err = msg;
+        return code;
+    }
+};
+
+namespace lbn {
+// Device entry points used by the host prover (lbn_device.hip).
+int dev_init(lbn_ctx *ctx);
+void dev_free(lbn_ctx *ctx);
+}  // namespace lbn

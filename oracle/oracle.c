@@ -584,10 +584,11 @@ void orc_commit_columns_packed(const uint8_t *table, int log_vars, uint64_t q_lo
 }
 
 /* rq::a_times_v (rq.rs:188-217). */
-static void a_times_v(const u128 *a, const int16_t *v, size_t elements, u128 q, u128 *out) {
-    i128 *raw = (i128 *)calloc(2 * N - 1, sizeof(i128));
+/* Each thread accumulates its own range of elements as rq.rs does (reducing every 64 elements);
+ * the per-thread accumulators are then summed modulo q, which is the same residue. */
+static void a_times_v_range(const u128 *a, const int16_t *v, size_t e0, size_t e1, u128 q, i128 *raw) {
     i128 m = (i128)q;
-    for (size_t e = 0; e < elements; e++) {
+    for (size_t e = e0; e < e1; e++) {
         const u128 *el = a + N * e;
         const int16_t *fo = v + N * e;
         for (int j = 0; j < N; j++) {
@@ -603,8 +604,23 @@ static void a_times_v(const u128 *a, const int16_t *v, size_t elements, u128 q, 
         if ((e + 1) % 64 == 0)
             for (int t = 0; t < 2 * N - 1; t++) raw[t] = rem_euclid(raw[t], m);
     }
+    for (int t = 0; t < 2 * N - 1; t++) raw[t] = rem_euclid(raw[t], m);
+}
+
+static void a_times_v(const u128 *a, const int16_t *v, size_t elements, u128 q, u128 *out) {
+    const int parts = 64;
+    i128 *raws = (i128 *)calloc((size_t)parts * (2 * N - 1), sizeof(i128));
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int p = 0; p < parts; p++) {
+        size_t e0 = elements * p / parts, e1 = elements * (p + 1) / parts;
+        a_times_v_range(a, v, e0, e1, q, raws + (size_t)p * (2 * N - 1));
+    }
+    i128 *raw = (i128 *)calloc(2 * N - 1, sizeof(i128));
+    for (int p = 0; p < parts; p++)
+        for (int t = 0; t < 2 * N - 1; t++) raw[t] = rem_euclid(raw[t] + raws[(size_t)p * (2 * N - 1) + t], (i128)q);
     fold_into_ring(raw, q, out);
     free(raw);
+    free(raws);
 }
 
 /* rq::challenge_combination (rq.rs:226-256): c_j sits in R_648 as c(-X^4). */
@@ -636,12 +652,28 @@ static void challenge_combination(const u128 *columns, const orc_challenge *ch, 
 void orc_eq_expand_b128(const orc_b128 *r, int n, orc_b128 *t) {
     t[0].w[0] = 1;
     t[0].w[1] = 0;
-    for (int round = 0; round < n; round++)
-        for (long long i = (1ll << round) - 1; i >= 0; i--) {
-            orc_b128 hi = bmul(t[i], r[round]);
-            t[2 * i + 1] = hi;
-            t[2 * i] = badd(t[i], hi);
+    orc_b128 *tmp = NULL;
+    for (int round = 0; round < n; round++) {
+        long long len = 1ll << round;
+        if (len < 4096) {
+            for (long long i = len - 1; i >= 0; i--) {
+                orc_b128 hi = bmul(t[i], r[round]);
+                t[2 * i + 1] = hi;
+                t[2 * i] = badd(t[i], hi);
+            }
+            continue;
         }
+        /* large rounds: the same doubling from a copy of the previous round, in parallel */
+        if (!tmp) tmp = (orc_b128 *)malloc(sizeof(orc_b128) << (n - 1));
+        memcpy(tmp, t, sizeof(orc_b128) * len);
+#pragma omp parallel for schedule(static)
+        for (long long i = 0; i < len; i++) {
+            orc_b128 hi = bmul(tmp[i], r[round]);
+            t[2 * i + 1] = hi;
+            t[2 * i] = badd(tmp[i], hi);
+        }
+    }
+    free(tmp);
 }
 
 /* eq_expand_f162 (crossfield.rs:66-78): the same doubling over F162. */
@@ -671,10 +703,18 @@ void orc_eq_table(const orc_f162 *p, int n, orc_f162 *t) {
 
 static void partials_from_eq(const uint8_t *table, size_t n, const orc_b128 *eq, orc_b128 *v) {
     memset(v, 0, sizeof(orc_b128) * PACK);
-    for (size_t j = 0; j < n; j++) {
-        orc_b128 w = word_at(table, j);
-        for (int k = 0; k < PACK; k++)
-            if (bit_of(w.w, k)) v[k] = badd(v[k], eq[j]);
+#pragma omp parallel
+    {
+        orc_b128 loc[PACK];
+        memset(loc, 0, sizeof loc);
+#pragma omp for schedule(static)
+        for (long long j = 0; j < (long long)n; j++) {
+            orc_b128 w = word_at(table, (size_t)j);
+            for (int k = 0; k < PACK; k++)
+                if (bit_of(w.w, k)) loc[k] = badd(loc[k], eq[j]);
+        }
+#pragma omp critical
+        for (int k = 0; k < PACK; k++) v[k] = badd(v[k], loc[k]);
     }
 }
 

@@ -1,0 +1,218 @@
+"""Parity of the HIP prover with the CPU oracle, through the C ABI (needs an MI355X).
+
+Bit-exact comparisons throughout: every value on this path is exact (integers mod Q, GF(2^128),
+F162, i16 folds), so the proof bytes of the GPU prover must equal the oracle's for the same seed,
+shape and modulus. At the challenge shape (k = 31) the oracle's full prover is too slow, so the GPU
+proof is checked by the oracle's port of the trusted checker and by per-pass spot checks replayed
+from that proof's transcript.
+"""
+import ctypes
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import amd
+import oracle_ctypes as O
+
+pytestmark = pytest.mark.gpu
+
+SEEDS = [bytes([42] * 32), bytes([7] * 32), bytes(range(32))]
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def m():
+    mod = amd.load()
+    O.build()
+    return mod
+
+
+@pytest.fixture(scope="module")
+def p20(m):
+    p = m.Prover(20)
+    yield p
+    p.close()
+
+
+def test_device_table_matches_oracle(m, p20):
+    say("table k=20 seed 42")
+    seed = SEEDS[0]
+    p20.generate_table(seed)
+    # read the device table back through a commitment-free path: prove uses it; compare via the
+    # partials of a fixed point against the oracle on the oracle's own table
+    tab = O.table(seed, 20)
+    rng = np.random.default_rng(1)
+    r_hi = rng.integers(0, 2**63, size=(13, 2), dtype=np.uint64)
+    assert (p20.partials(r_hi) == O.partials(tab, 20, r_hi)).all()
+
+
+@pytest.mark.parametrize("seed", SEEDS)
+def test_prove_k20_equals_oracle(m, p20, seed):
+    say("prove k=20 seed", seed[:2].hex())
+    p20.generate_table(seed)
+    p20.set_modulus(O.DEFAULT_Q)
+    got, prof = p20.prove(seed)
+    want = O.prove(seed, 20)
+    assert len(got) == len(want) == O.proof_len(20)
+    assert got == want
+    assert O.verify(seed, 20, got) is None
+
+
+@pytest.mark.parametrize("q", [1 << 36, 55_144_219_393, 34_816_965_049, (1 << 95) + 12345, (1 << 96) - 1])
+def test_prove_k20_other_moduli(m, p20, q):
+    say("prove k=20 modulus", q)
+    seed = SEEDS[1]
+    p20.generate_table(seed)
+    p20.set_modulus(q)
+    got, _ = p20.prove(seed)
+    assert got == O.prove(seed, 20, q)
+
+
+def test_prove_k23_equals_oracle(m):
+    say("prove k=23")
+    seed = SEEDS[2]
+    p = m.Prover(23)
+    p.generate_table(seed)
+    p.set_modulus(O.DEFAULT_Q)
+    got, _ = p.prove(seed)
+    p.close()
+    assert got == O.prove(seed, 23)
+
+
+@pytest.mark.parametrize("kind", ["zeros", "ones", "alternating"])
+def test_edge_tables_k20(m, p20, kind):
+    say("edge table", kind)
+    n = (1 << 20) // 8
+    tab = {"zeros": np.zeros(n, np.uint8), "ones": np.full(n, 255, np.uint8),
+           "alternating": np.tile(np.array([0x55, 0xAA], np.uint8), n // 2)}[kind]
+    seed = SEEDS[0]
+    p20.load_table(tab)
+    p20.set_modulus(O.DEFAULT_Q)
+    got, _ = p20.prove(seed)
+    assert got == O.prove(seed, 20, O.DEFAULT_Q, tab)
+
+
+def test_components_k20(m, p20):
+    say("components k=20")
+    seed = SEEDS[2]
+    tab = O.table(seed, 20)
+    p20.load_table(tab)
+    p20.set_modulus(O.DEFAULT_Q)
+    rng = np.random.default_rng(7)
+    r_hi = rng.integers(0, 2**63, size=(13, 2), dtype=np.uint64)
+    assert (p20.partials(r_hi) == O.partials(tab, 20, r_hi)).all()
+    batch = rng.integers(0, 2**63, size=(128, 3), dtype=np.uint64)
+    batch[:, 2] &= (1 << 34) - 1
+    rhos = rng.integers(0, 2**63, size=(13, 3), dtype=np.uint64)
+    rhos[:, 2] &= (1 << 34) - 1
+    want_rounds, want_opened = O.switch_run(tab, 20, r_hi, batch, rhos)
+    p20.switch_begin(batch)
+    for r in range(13):
+        assert (p20.switch_msg() == want_rounds[r]).all(), f"round {r}"
+        p20.switch_fold(rhos[r])
+    assert (p20.switch_final() == want_opened).all()
+    p0 = rng.integers(0, 2**63, size=(3, 3), dtype=np.uint64)
+    p0[:, 2] &= (1 << 34) - 1
+    assert (p20.row_eval(p0) == O.row_eval(tab, 20, p0)).all()
+    ch = O.challenges_probe(b"labinius/test-fold", b"k20", 1024)
+    chg = (m.Challenge * 1024)()
+    ctypes.memmove(chg, ch, ctypes.sizeof(chg))
+    want, bad = O.fold(tab, 20, ch, 0, 2)
+    assert (p20.fold(chg) == want).all()
+    assert p20.commit() == O.commit_columns(tab, 20, O.DEFAULT_Q, 0, 1024).tobytes()
+
+
+def test_deterministic(m, p20):
+    say("determinism")
+    p20.generate_table(SEEDS[0])
+    p20.set_modulus(O.DEFAULT_Q)
+    a, _ = p20.prove(SEEDS[0])
+    b, _ = p20.prove(SEEDS[0])
+    assert a == b
+
+
+def test_errors(m, p20):
+    say("error paths")
+    with pytest.raises(m.LbnError) as e:
+        p20.load_table(np.zeros(100, np.uint8))
+    assert e.value.code == -1
+    with pytest.raises(m.LbnError) as e:
+        p20.set_modulus(34_816_965_048)
+    assert e.value.code == -4
+    with pytest.raises(m.LbnError) as e:
+        p20.set_modulus(1 << 96)
+    assert e.value.code == -4
+    fresh = m.Prover(20)
+    with pytest.raises(m.LbnError) as e:
+        fresh.switch_begin(np.zeros((128, 3), np.uint64))
+    assert e.value.code == -2
+    fresh.q = O.DEFAULT_Q  # python side only: the library has no modulus and no table
+    with pytest.raises(m.LbnError) as e:
+        fresh.commit()
+    assert e.value.code == -2
+    with pytest.raises(m.LbnError) as e:
+        fresh.prove(SEEDS[0])
+    assert e.value.code == -2
+    fresh.close()
+
+
+@pytest.fixture(scope="module")
+def proof31(m):
+    say("prove k=31")
+    seed = bytes([9] * 32)
+    p = m.Prover(31)
+    p.generate_table(seed)
+    p.set_modulus(O.DEFAULT_Q)
+    got, prof = p.prove(seed)
+    say("k=31 profile", {k: round(v) for k, v in prof.items()})
+    yield seed, p, got
+    p.close()
+
+
+def test_k31_table_matches_golden(m, proof31):
+    import json
+    seed, p, _ = proof31
+    gold = [t for t in json.load(open(os.path.join(os.path.dirname(__file__), "golden", "blake3_vectors.json")))["table"]
+            if bytes.fromhex(t["seed"]) == seed][0]
+    tab = O.table(seed, 31)
+    assert tab[:4096].tobytes().hex() == gold["prefix"]
+    assert tab[-64:].tobytes().hex() == gold["at_2pow28_minus_64"]
+    # the device table is the table the proof used: its partials equal the oracle's on `tab`
+    rng = np.random.default_rng(3)
+    r_hi = rng.integers(0, 2**63, size=(24, 2), dtype=np.uint64)
+    say("k=31 partials vs oracle")
+    assert (p.partials(r_hi) == O.partials(tab, 31, r_hi)).all()
+
+
+def test_k31_proof_is_accepted(m, proof31):
+    seed, _, proof = proof31
+    assert len(proof) == 13_958_612
+    say("k=31 oracle checker")
+    assert O.verify(seed, 31, proof) is None
+
+
+def test_k31_spot_checks(m, proof31):
+    seed, p, proof = proof31
+    tab = O.table(seed, 31)
+    r_hi, batch, coords, ch = O.replay(seed, 31, proof)
+    w = O.coefficient_bytes(O.DEFAULT_Q)
+    say("k=31 commitment columns")
+    for col in (0, 517, 1023):
+        want = O.commit_columns(tab, 31, O.DEFAULT_Q, col, 1).tobytes()
+        at = 28 + col * 648 * w
+        assert proof[at: at + 648 * w] == want, f"column {col}"
+    say("k=31 row columns")
+    p0 = coords[::-1][:14]
+    row = O.row_eval(tab, 31, np.ascontiguousarray(p0), 0, 8)
+    assert (p.row_eval(np.ascontiguousarray(p0))[:8] == row).all()
+    say("k=31 fold elements")
+    want, bad = O.fold(tab, 31, ch, 4090, 6)
+    start = len(proof) - 4 * 648 * 4096 + 4 * 648 * 4090
+    got = np.frombuffer(proof[start: start + 4 * 648 * 6], dtype="<i4").reshape(6, 648)
+    assert (got == want).all()

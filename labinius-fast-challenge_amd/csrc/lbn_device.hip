@@ -32,17 +32,21 @@ namespace tab {
 constexpr int TWB = 0;                 // [2][324][3] binary split+twist values (normal form)
 constexpr int TW = TWB + 2 * 324 * 3;  // [2][324] beta_h^i (Montgomery)
 constexpr int ETA = TW + 2 * 324;      // [2] eta_h (Montgomery)
-constexpr int FWD = ETA + 2;           // [323] DIF twiddles (Montgomery)
-constexpr int INV = FWD + 323;         // [323] DIT inverse twiddles (Montgomery)
-constexpr int W3 = INV + 323;          // [2] cube root of unity and its inverse (Montgomery)
-constexpr int UNTW = W3 + 2;           // [2][324] beta_h^-i / 324 (Montgomery)
+constexpr int UNTW = ETA + 2;          // [2][324] beta_h^-i / 324 (Montgomery)
 constexpr int ETA0 = UNTW + 2 * 324;   // eta_0 (Montgomery)
 constexpr int DINV = ETA0 + 1;         // 1 / (eta_0 - eta_1) (Montgomery)
-constexpr int SIZE = DINV + 1;
-// stage twiddle bases inside FWD/INV for spans 324, 162, 81, 27, 9, 3
-constexpr int ST0 = 0, ST1 = 162, ST2 = 243, ST3 = 297, ST4 = 315, ST5 = 321;
+constexpr int TWPF = DINV + 1;         // [18][18] rho^(n2 k1) (Montgomery), rho = w^6
+constexpr int TWPI = TWPF + 324;       // [18][18] rho^-(n2 k1)
+constexpr int R18F = TWPI + 324;       // [9] w18^j, w18 = rho^18 (Montgomery)
+constexpr int R18I = R18F + 9;         // [9] w18^-j
+constexpr int SIZE = R18I + 9;
 }  // namespace tab
 }  // namespace lbn
+
+// The 18-th roots of unity of one prime: w18^j and w18^-j for j < 9, Montgomery form.
+struct Roots18 {
+    uint32_t f[9], i[9];
+};
 
 struct lbn::DeviceState {
     hipStream_t stream = nullptr;
@@ -62,7 +66,8 @@ struct lbn::DeviceState {
     uint8_t *cbytes = nullptr;      // 1024 * 648 * width
     uint64_t *amat = nullptr;       // RE * 648 * 2 (staging for the matrix)
     size_t ahat_elems = 0;
-    hipEvent_t ev[2 * 9] = {};      // start/stop per timed pass
+    hipEvent_t ev[2 * 9] = {};
+    Roots18 roots[MAX_PRIMES] = {};      // start/stop per timed pass
 };
 
 // Timed passes (indices into lbn_timing's fields, in order).
@@ -450,14 +455,33 @@ __global__ __launch_bounds__(128) void k_row_finish(const uint32_t *U, uint64_t 
 // positions with NB-bit bit-sliced counters. Then X^648 = X^324 - 1 folds raw into 648 values:
 //   v[t] = R[t] - R[t+648] - R[t+972] (t < 324),  v[t] = R[t] + R[t+324] (t >= 324).
 // ---------------------------------------------------------------------------------------------
+// One signed counter per task: a negative term adds the complement of its word, since
+// -b = (1 - b) - 1, and the list length of the negatives is subtracted once at extraction.
+// Words are added four at a time through carry-save full adders, then rippled.
+__device__ __forceinline__ void full_add(uint32_t a, uint32_t b, uint32_t c, uint32_t &sum, uint32_t &carry) {
+    const uint32_t t = a ^ b;
+    sum = t ^ c;
+    carry = (a & b) | (c & t);
+}
+
 template <int NB>
-__device__ __forceinline__ void ctr_add(uint32_t (&c)[NB], uint32_t x) {
+__device__ __forceinline__ void ctr_ripple(uint32_t (&c)[NB], uint32_t x, int from) {
 #pragma unroll
-    for (int i = 0; i < NB; i++) {
-        uint32_t t = c[i] & x;
-        c[i] ^= x;
-        x = t;
-    }
+    for (int i = 0; i < NB; i++)
+        if (i >= from) {
+            const uint32_t t = c[i] & x;
+            c[i] ^= x;
+            x = t;
+        }
+}
+
+template <int NB>
+__device__ __forceinline__ void ctr_add4(uint32_t (&c)[NB], uint32_t x0, uint32_t x1, uint32_t x2, uint32_t x3) {
+    uint32_t s1, k1, k2, k3;
+    full_add(c[0], x0, x1, s1, k1);
+    full_add(s1, x2, x3, c[0], k2);
+    full_add(c[1], k1, k2, c[1], k3);
+    ctr_ripple<NB>(c, k3, 2);
 }
 
 template <int NB>
@@ -476,24 +500,49 @@ __global__ __launch_bounds__(256) void k_fold(const uint32_t *table32, size_t cl
     for (int task = tid; task < 162 * 16; task += 256) {
         const int pos = task >> 4, kw = task & 15;
         const uint32_t *W = s_w + kw * 1024;
-        uint32_t P[NB], Q[NB];
+        uint32_t c[NB];
 #pragma unroll
-        for (int i = 0; i < NB; i++) P[i] = 0, Q[i] = 0;
+        for (int i = 0; i < NB; i++) c[i] = 0;
         const int e0 = off[2 * pos], e1 = off[2 * pos + 1], e2 = off[2 * pos + 2];
-        for (int e = e0; e < e1; e++) ctr_add<NB>(P, W[cols[e]]);
-        for (int e = e1; e < e2; e++) ctr_add<NB>(Q, W[cols[e]]);
+        // positives then complemented negatives, as one stream of words
+        int e = e0;
+        auto word = [&](int ee) -> uint32_t { return ee < e1 ? W[cols[ee]] : ~W[cols[ee]]; };
+        for (; e + 4 <= e2; e += 4) ctr_add4<NB>(c, word(e), word(e + 1), word(e + 2), word(e + 3));
+        for (; e < e2; e++) ctr_ripple<NB>(c, word(e), 0);
+        const int negs = e2 - e1;
         const int k = kw >> 2, q0 = pos + 32 * (kw & 3);
-        for (int p = 0; p < 32; p++) {
-            int cnt = 0;
+        if (NB <= 8) {
+            // four counts per u32 (8-bit fields): positions p, p+8, p+16, p+24
 #pragma unroll
-            for (int i = 0; i < NB; i++) cnt += (int)(((P[i] >> p) & 1) << i) - (int)(((Q[i] >> p) & 1) << i);
-            if (cnt) atomicAdd(&s_raw[k * 292 + q0 + p], cnt);
+            for (int p = 0; p < 8; p++) {
+                uint32_t f = 0;
+#pragma unroll
+                for (int i = 0; i < NB; i++) f |= ((c[i] >> p) & 0x01010101u) << i;
+#pragma unroll
+                for (int b = 0; b < 4; b++) {
+                    const int cnt = (int)((f >> (8 * b)) & 0xFF) - negs;
+                    if (cnt) atomicAdd(&s_raw[k * 292 + q0 + p + 8 * b], cnt);
+                }
+            }
+        } else {
+            // two counts per u32 (16-bit fields): positions p, p+16
+#pragma unroll
+            for (int p = 0; p < 16; p++) {
+                uint32_t f = 0;
+#pragma unroll
+                for (int i = 0; i < NB; i++) f |= ((c[i] >> p) & 0x00010001u) << i;
+#pragma unroll
+                for (int b = 0; b < 2; b++) {
+                    const int cnt = (int)((f >> (16 * b)) & 0xFFFF) - negs;
+                    if (cnt) atomicAdd(&s_raw[k * 292 + q0 + p + 16 * b], cnt);
+                }
+            }
         }
     }
     __syncthreads();
     for (int t = tid; t < 648; t += 256) {
         auto R = [&](int tp) -> int { return tp <= 1159 ? s_raw[(tp & 3) * 292 + (tp >> 2)] : 0; };
-        int v = t < 324 ? R(t) - R(t + 648) - R(t + 972) : R(t) + R(t + 324);
+        const int v = t < 324 ? R(t) - R(t + 648) - R(t + 972) : R(t) + R(t + 324);
         out[m * 648 + t] = v;
         if (v < -32768 || v > 32767) atomicOr(flag, 1u);
     }
@@ -504,155 +553,186 @@ __global__ __launch_bounds__(256) void k_fold(const uint32_t *table32, size_t cl
 //
 // Per prime, R_p = Z_p[X]/(X^648 - X^324 + 1) splits as X^324 = eta_h (eta_0 = w^324,
 // eta_1 = w^1620, w a primitive 1944-th root); half h is twisted by beta_h^i (beta_0 = w,
-// beta_1 = w^5) into a cyclic length-324 DFT with root rho = w^6, done as a DIF with radices
-// 2,2,3,3,3,3 in LDS (output in digit-reversed slot order; the inverse is the matching DIT).
+// beta_1 = w^5) into a cyclic length-324 DFT with root rho = w^6, computed as an 18 x 18 four-step
+// transform: pass A takes, for each n2 < 18, the 18-point DFT over n1 of x[18 n1 + n2] and twists
+// output k1 by rho^(n2 k1); pass B takes, for each k1, the 18-point DFT over n2. Each 18-point DFT
+// (radices 2, 3, 3) lives in one thread's registers; one LDS transpose sits between the passes.
+// Thread (k1) of pass B holds output slots k1*18 + p, p < 18, which is where the MAC consumes
+// them, so the slot order is this transform's own (the matrix goes through the same code).
 // ---------------------------------------------------------------------------------------------
-template <int SPAN, int RAD>
-__device__ __forceinline__ void dif_stage(uint32_t *X, int nhalves, const uint32_t *tw, uint32_t w3, PC pc) {
-    constexpr int M = SPAN / RAD, NB = 324 / RAD;
-    const int total = nhalves * NB;
-    for (int bf = threadIdx.x; bf < total; bf += blockDim.x) {
-        const int hh = bf / NB, idx = bf - hh * NB, blk = idx / M, j = idx - blk * M;
-        uint32_t *b = X + hh * 324 + blk * SPAN + j;
-        if (RAD == 2) {
-            uint32_t y0 = b[0], y1 = b[M];
-            uint32_t x0 = addm(y0, y1, pc.p), x1 = subm(y0, y1, pc.p);
-            if (j) x1 = mont(x1, tw[j], pc);
-            b[0] = x0, b[M] = x1;
-        } else {
-            uint32_t y0 = b[0], y1 = b[M], y2 = b[2 * M];
-            uint32_t t = mont(subm(y1, y2, pc.p), w3, pc);
-            uint32_t x0 = addm(addm(y0, y1, pc.p), y2, pc.p);
-            uint32_t x1 = addm(subm(y0, y2, pc.p), t, pc.p);
-            uint32_t x2 = subm(subm(y0, y1, pc.p), t, pc.p);
-            if (j) x1 = mont(x1, tw[2 * j], pc), x2 = mont(x2, tw[2 * j + 1], pc);
-            b[0] = x0, b[M] = x1, b[2 * M] = x2;
-        }
-    }
-    __syncthreads();
+// 18-point DFT position p holds frequency KREV(p): p = 9 s1 + 3 s2 + s3 -> s1 + 2 s2 + 6 s3.
+__host__ __device__ constexpr int krev18(int p) { return p / 9 + 2 * ((p / 3) % 3) + 6 * (p % 3); }
+
+__device__ __forceinline__ void radix3(uint32_t &a, uint32_t &b, uint32_t &c, uint32_t w3, PC pc) {
+    const uint32_t t = mont(subm(b, c, pc.p), w3, pc);
+    const uint32_t x0 = addm(addm(a, b, pc.p), c, pc.p);
+    const uint32_t x1 = addm(subm(a, c, pc.p), t, pc.p);
+    const uint32_t x2 = subm(subm(a, b, pc.p), t, pc.p);
+    a = x0, b = x1, c = x2;
 }
 
-template <int SPAN, int RAD>
-__device__ __forceinline__ void dit_stage(uint32_t *X, int nhalves, const uint32_t *itw, uint32_t w3i, PC pc) {
-    constexpr int M = SPAN / RAD, NB = 324 / RAD;
-    const int total = nhalves * NB;
-    for (int bf = threadIdx.x; bf < total; bf += blockDim.x) {
-        const int hh = bf / NB, idx = bf - hh * NB, blk = idx / M, j = idx - blk * M;
-        uint32_t *b = X + hh * 324 + blk * SPAN + j;
-        if (RAD == 2) {
-            uint32_t y0 = b[0], y1 = b[M];
-            if (j) y1 = mont(y1, itw[j], pc);
-            b[0] = addm(y0, y1, pc.p), b[M] = subm(y0, y1, pc.p);
-        } else {
-            uint32_t y0 = b[0], y1 = b[M], y2 = b[2 * M];
-            if (j) y1 = mont(y1, itw[2 * j], pc), y2 = mont(y2, itw[2 * j + 1], pc);
-            uint32_t t = mont(subm(y1, y2, pc.p), w3i, pc);
-            b[0] = addm(addm(y0, y1, pc.p), y2, pc.p);
-            b[M] = addm(subm(y0, y2, pc.p), t, pc.p);
-            b[2 * M] = subm(subm(y0, y1, pc.p), t, pc.p);
-        }
-    }
-    __syncthreads();
-}
-
-__device__ __forceinline__ void dif_all(uint32_t *X, int nhalves, const uint32_t *fwd, uint32_t w3, PC pc) {
-    dif_stage<324, 2>(X, nhalves, fwd + tab::ST0, w3, pc);
-    dif_stage<162, 2>(X, nhalves, fwd + tab::ST1, w3, pc);
-    dif_stage<81, 3>(X, nhalves, fwd + tab::ST2, w3, pc);
-    dif_stage<27, 3>(X, nhalves, fwd + tab::ST3, w3, pc);
-    dif_stage<9, 3>(X, nhalves, fwd + tab::ST4, w3, pc);
-    dif_stage<3, 3>(X, nhalves, fwd + tab::ST5, w3, pc);
-}
-
-__device__ __forceinline__ void dit_all(uint32_t *X, int nhalves, const uint32_t *inv, uint32_t w3i, PC pc) {
-    dit_stage<3, 3>(X, nhalves, inv + tab::ST5, w3i, pc);
-    dit_stage<9, 3>(X, nhalves, inv + tab::ST4, w3i, pc);
-    dit_stage<27, 3>(X, nhalves, inv + tab::ST3, w3i, pc);
-    dit_stage<81, 3>(X, nhalves, inv + tab::ST2, w3i, pc);
-    dit_stage<162, 2>(X, nhalves, inv + tab::ST1, w3i, pc);
-    dit_stage<324, 2>(X, nhalves, inv + tab::ST0, w3i, pc);
-}
-
-// Transformed matrix: ahat[m][s] = NTT(A_m mod p)[s] * R mod p (Montgomery form for the MAC).
-__global__ __launch_bounds__(256) void k_ahat(const uint64_t *amat, const uint32_t *tabs, PC pc, uint32_t r2,
-                                              uint32_t *ahat) {
-    __shared__ uint32_t X[648];
-    __shared__ uint32_t a[648];
-    const size_t m = blockIdx.x;
-    for (int t = threadIdx.x; t < 648; t += 256) {
-        uint64_t lo = amat[(m * 648 + t) * 2], hi = amat[(m * 648 + t) * 2 + 1];
-        // (hi * 2^64 + lo) mod p
-        uint64_t two64 = ((uint64_t)0 - pc.p) % pc.p;  // 2^64 mod p
-        uint64_t x = ((hi % pc.p) * two64 + lo % pc.p) % pc.p;
-        a[t] = (uint32_t)x;
-    }
-    __syncthreads();
-    for (int v = threadIdx.x; v < 648; v += 256) {
-        const int h = v / 324, i = v - h * 324;
-        uint32_t u = addm(a[i], mont(a[i + 324], tabs[tab::ETA + h], pc), pc.p);
-        X[v] = mont(u, tabs[tab::TW + h * 324 + i], pc);
-    }
-    __syncthreads();
-    dif_all(X, 2, tabs + tab::FWD, tabs[tab::W3], pc);
-    for (int s = threadIdx.x; s < 648; s += 256) ahat[m * 648 + s] = mont(X[s], r2, pc);
-}
-
-// Forward transform of the binary elements of TC columns and MAC against ahat, over the ring
-// elements [m0, m1) of this block; partial sums are added into cacc (u64 atomics).
-constexpr int TC = 8;
-constexpr int CACC = (TC * 648 + 255) / 256;
-
-__global__ __launch_bounds__(256) void k_commit_mac(const uint32_t *table32, size_t clen, int m_per, int re,
-                                                    const uint32_t *ahat, const uint32_t *tabs, PC pc,
-                                                    unsigned long long *cacc) {
-    __shared__ uint32_t X[TC * 648];
-    __shared__ uint32_t s_twb[2 * 324 * 3];
-    __shared__ uint32_t s_fwd[323];
-    __shared__ uint32_t s_bits[TC * 16];
-    const int tid = threadIdx.x;
-    const int c0 = blockIdx.x * TC;
-    const int m0 = blockIdx.y * m_per, m1 = min(re, m0 + m_per);
-    for (int i = tid; i < 2 * 324 * 3; i += 256) s_twb[i] = tabs[tab::TWB + i];
-    for (int i = tid; i < 323; i += 256) s_fwd[i] = tabs[tab::FWD + i];
-    const uint32_t w3 = tabs[tab::W3];
-    unsigned long long acc[CACC];
+// Forward 18-point DFT (decimation in frequency: 2, then 3, 3), output in krev18 order.
+__device__ __forceinline__ void dft18_fwd(uint32_t (&v)[18], const uint32_t (&w)[9], PC pc) {
 #pragma unroll
-    for (int u = 0; u < CACC; u++) acc[u] = 0;
-    __syncthreads();
-    for (int m = m0; m < m1; m++) {
-        if (tid < TC * 16) {
-            const int col = tid >> 4, wd = tid & 15;  // word k = wd >> 2, u32 wd & 3
-            s_bits[tid] = table32[(((size_t)(c0 + col) * clen + 4 * (size_t)m + (wd >> 2)) << 2) + (wd & 3)];
-        }
-        __syncthreads();
-        for (int v = tid; v < TC * 648; v += 256) {
-            const int col = v / 648, s = v - col * 648, h = s / 324, i = s - h * 324;
-            const uint32_t *bits = s_bits + col * 16;
-            // coefficient c of the element is bit (c >> 2) of word (c & 3)
-            const int cl = i, ch = i + 324;
-            uint32_t lo = (bits[(cl & 3) * 4 + ((cl >> 2) >> 5)] >> ((cl >> 2) & 31)) & 1;
-            uint32_t hi = ch < 512 ? (bits[(ch & 3) * 4 + ((ch >> 2) >> 5)] >> ((ch >> 2) & 31)) & 1 : 0;
-            const uint32_t sel = lo | (hi << 1);
-            X[v] = sel ? s_twb[(h * 324 + i) * 3 + sel - 1] : 0u;
-        }
-        __syncthreads();
-        dif_all(X, 2 * TC, s_fwd, w3, pc);
-        const uint32_t *am = ahat + (size_t)m * 648;
+    for (int j = 0; j < 9; j++) {
+        const uint32_t a = v[j], b = v[j + 9];
+        v[j] = addm(a, b, pc.p);
+        const uint32_t d = subm(a, b, pc.p);
+        v[j + 9] = j ? mont(d, w[j], pc) : d;
+    }
 #pragma unroll
-        for (int u = 0; u < CACC; u++) {
-            const int v = tid + 256 * u;
-            if (v < TC * 648) {
-                const int s = v % 648;
-                acc[u] += mont_lazy(X[v], am[s], pc);
+    for (int B = 0; B < 18; B += 9)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            radix3(v[B + j], v[B + j + 3], v[B + j + 6], w[6], pc);
+            if (j) {
+                v[B + j + 3] = mont(v[B + j + 3], w[2 * j], pc);
+                v[B + j + 6] = mont(v[B + j + 6], w[4 * j], pc);
             }
         }
-        __syncthreads();
-    }
 #pragma unroll
-    for (int u = 0; u < CACC; u++) {
-        const int v = tid + 256 * u;
-        if (v < TC * 648) atomicAdd(cacc + (size_t)c0 * 648 + v, acc[u]);
+    for (int B = 0; B < 18; B += 3) radix3(v[B], v[B + 1], v[B + 2], w[6], pc);
+}
+
+// Exact inverse of dft18_fwd up to the factor 18 (decimation in time, inverse roots).
+__device__ __forceinline__ void dft18_inv(uint32_t (&v)[18], const uint32_t (&w)[9], PC pc) {
+#pragma unroll
+    for (int B = 0; B < 18; B += 3) radix3(v[B], v[B + 1], v[B + 2], w[6], pc);
+#pragma unroll
+    for (int B = 0; B < 18; B += 9)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            if (j) {
+                v[B + j + 3] = mont(v[B + j + 3], w[2 * j], pc);
+                v[B + j + 6] = mont(v[B + j + 6], w[4 * j], pc);
+            }
+            radix3(v[B + j], v[B + j + 3], v[B + j + 6], w[6], pc);
+        }
+#pragma unroll
+    for (int j = 0; j < 9; j++) {
+        const uint32_t a = v[j], b = j ? mont(v[j + 9], w[j], pc) : v[j + 9];
+        v[j] = addm(a, b, pc.p);
+        v[j + 9] = subm(a, b, pc.p);
     }
+}
+
+// Pass A for one half: thread n2 holds x[18 n1 + n2] in v; afterwards Y[k1][n2] is in LDS.
+__device__ __forceinline__ void passA_store(uint32_t (&v)[18], int n2, const uint32_t *twpf, uint32_t *Y,
+                                            const uint32_t (&w)[9], PC pc) {
+    dft18_fwd(v, w, pc);
+#pragma unroll
+    for (int p = 0; p < 18; p++) {
+        const int k1 = krev18(p);
+        const uint32_t y = k1 ? mont(v[p], twpf[n2 * 18 + k1], pc) : v[p];
+        Y[k1 * 18 + n2] = y;
+    }
+}
+
+// Transformed matrix: ahat[m][slot] = T(A_m mod p)[slot] * R mod p (Montgomery form for the MAC).
+// One element per block; threads 0..35 run the two halves' transforms.
+__global__ __launch_bounds__(64) void k_ahat(const uint64_t *amat, const uint32_t *tabs, PC pc, Roots18 R,
+                                             uint32_t r2, uint32_t *ahat) {
+    __shared__ uint32_t a[648];
+    __shared__ uint32_t Y[2 * 324];
+    const size_t m = blockIdx.x;
+    const int t = threadIdx.x;
+    const uint64_t two64 = ((uint64_t)0 - pc.p) % pc.p;  // 2^64 mod p
+    for (int i = t; i < 648; i += 64) {
+        const uint64_t lo = amat[(m * 648 + i) * 2], hi = amat[(m * 648 + i) * 2 + 1];
+        a[i] = (uint32_t)(((hi % pc.p) * two64 + lo % pc.p) % pc.p);
+    }
+    __syncthreads();
+    const int h = t / 18, idx = t % 18;
+    uint32_t w[9];
+#pragma unroll
+    for (int j = 0; j < 9; j++) w[j] = R.f[j];
+    if (t < 36) {
+        uint32_t v[18];
+#pragma unroll
+        for (int n1 = 0; n1 < 18; n1++) {
+            const int i = 18 * n1 + idx;
+            const uint32_t u = addm(a[i], mont(a[i + 324], tabs[tab::ETA + h], pc), pc.p);
+            v[n1] = mont(u, tabs[tab::TW + h * 324 + i], pc);
+        }
+        passA_store(v, idx, tabs + tab::TWPF, Y + h * 324, w, pc);
+    }
+    __syncthreads();
+    if (t < 36) {
+        uint32_t v[18];
+#pragma unroll
+        for (int n2 = 0; n2 < 18; n2++) v[n2] = Y[h * 324 + idx * 18 + n2];
+        dft18_fwd(v, w, pc);
+#pragma unroll
+        for (int p = 0; p < 18; p++) ahat[m * 648 + h * 324 + idx * 18 + p] = mont(v[p], r2, pc);
+    }
+}
+
+// Forward transform of the binary ring elements of TC columns, MAC against ahat, over the ring
+// elements [m0, m1) of this block; partial sums go into cacc with u64 atomics. 576 threads:
+// thread t handles column t / 36, half (t / 18) % 2, and n2 (pass A) / k1 (pass B) = t % 18.
+constexpr int TC = 16;
+constexpr int MAC_THREADS = TC * 36;
+
+__global__ __launch_bounds__(MAC_THREADS) void k_commit_mac(const uint32_t *table32, size_t clen, int m_per, int re,
+                                                            const uint32_t *ahat, const uint32_t *tabs, PC pc,
+                                                            Roots18 R, unsigned long long *cacc) {
+    __shared__ uint32_t Y[TC * 2 * 324];
+    __shared__ uint32_t s_twb[2 * 324 * 3];
+    __shared__ uint32_t s_twp[324];
+    __shared__ uint32_t s_bits[TC * 16];
+    const int t = threadIdx.x;
+    const int col = t / 36, h = (t / 18) & 1, idx = t % 18;
+    const int c0 = blockIdx.x * TC;
+    const int m0 = blockIdx.y * m_per, m1 = min(re, m0 + m_per);
+    for (int i = t; i < 2 * 324 * 3; i += MAC_THREADS) s_twb[i] = tabs[tab::TWB + i];
+    for (int i = t; i < 324; i += MAC_THREADS) s_twp[i] = tabs[tab::TWPF + i];
+    uint32_t w[9];
+#pragma unroll
+    for (int j = 0; j < 9; j++) w[j] = R.f[j];
+    unsigned long long acc[18];
+#pragma unroll
+    for (int p = 0; p < 18; p++) acc[p] = 0;
+    const uint32_t *bits = s_bits + col * 16;
+    const uint32_t *twb = s_twb + h * 324 * 3;
+    uint32_t *Yh = Y + (col * 2 + h) * 324;
+    for (int m = m0; m < m1; m++) {
+        if (t < TC * 16) {
+            const int bc = t >> 4, wd = t & 15;  // word wd >> 2, u32 wd & 3
+            s_bits[t] = table32[(((size_t)(c0 + bc) * clen + 4 * (size_t)m + (wd >> 2)) << 2) + (wd & 3)];
+        }
+        __syncthreads();
+        {
+            // pass A: x[i] for i = 18 n1 + n2; coefficient c is bit (c >> 2) of word (c & 3)
+            uint32_t v[18];
+#pragma unroll
+            for (int n1 = 0; n1 < 18; n1++) {
+                const int i = 18 * n1 + idx, ch = i + 324;
+                const uint32_t lo = (bits[(i & 3) * 4 + (i >> 7)] >> ((i >> 2) & 31)) & 1;
+                const uint32_t hi = ch < 512 ? (bits[(ch & 3) * 4 + (ch >> 7)] >> ((ch >> 2) & 31)) & 1 : 0;
+                const uint32_t sel = lo | (hi << 1);
+                v[n1] = sel ? twb[i * 3 + sel - 1] : 0u;
+            }
+            passA_store(v, idx, s_twp, Yh, w, pc);
+        }
+        __syncthreads();
+        {
+            // pass B and the MAC: slots h*324 + idx*18 + p
+            uint32_t v[18];
+#pragma unroll
+            for (int n2 = 0; n2 < 18; n2++) v[n2] = Yh[idx * 18 + n2];
+            dft18_fwd(v, w, pc);
+            const uint2 *am = (const uint2 *)(ahat + (size_t)m * 648 + h * 324 + idx * 18);
+#pragma unroll
+            for (int q = 0; q < 9; q++) {
+                const uint2 a2 = am[q];
+                acc[2 * q] += mont_lazy(v[2 * q], a2.x, pc);
+                acc[2 * q + 1] += mont_lazy(v[2 * q + 1], a2.y, pc);
+            }
+        }
+    }
+    unsigned long long *dst = cacc + (size_t)(c0 + col) * 648 + h * 324 + idx * 18;
+#pragma unroll
+    for (int p = 0; p < 18; p++) atomicAdd(dst + p, acc[p]);
 }
 
 // Inverse transform per column and prime, Garner over the primes, reduction mod Q, packing.
@@ -668,55 +748,84 @@ __device__ __forceinline__ uint32_t mulmod_u64(uint32_t a, uint32_t b, uint32_t 
     return (uint32_t)(((uint64_t)a * b) % p);
 }
 
-__global__ __launch_bounds__(256) void k_commit_finish(const unsigned long long *cacc, const uint32_t *tabs_all,
-                                                       CrtArg crt, PC pc0, PC pc1, PC pc2, PC pc3, uint8_t *outb) {
-    __shared__ uint32_t X[648];
+struct RootsAll {
+    Roots18 r[MAX_PRIMES];
+};
+
+__global__ __launch_bounds__(64) void k_commit_finish(const unsigned long long *cacc, const uint32_t *tabs_all,
+                                                      CrtArg crt, PC pc0, PC pc1, PC pc2, PC pc3, RootsAll roots,
+                                                      uint8_t *outb) {
+    __shared__ uint32_t Y[2 * 324];
+    __shared__ uint32_t x[648];
     __shared__ uint32_t res[MAX_PRIMES][648];
-    const int c = blockIdx.x;
+    const int c = blockIdx.x, t = threadIdx.x;
+    const int h = t / 18, idx = t % 18;
     const PC pcs[4] = {pc0, pc1, pc2, pc3};
     for (int k = 0; k < crt.nprimes; k++) {
         const PC pc = pcs[k];
         const uint32_t *tabs = tabs_all + k * tab::SIZE;
-        const unsigned long long *src = cacc + ((size_t)k * 1024 + c) * 648;
-        for (int s = threadIdx.x; s < 648; s += 256) X[s] = (uint32_t)(src[s] % pc.p);
+        uint32_t w[9];
+#pragma unroll
+        for (int j = 0; j < 9; j++) w[j] = roots.r[k].i[j];
+        if (t < 36) {
+            // inverse pass B: thread (h, k1 = idx)
+            const unsigned long long *src = cacc + ((size_t)k * 1024 + c) * 648 + h * 324 + idx * 18;
+            uint32_t v[18];
+#pragma unroll
+            for (int p = 0; p < 18; p++) v[p] = (uint32_t)(src[p] % pc.p);
+            dft18_inv(v, w, pc);
+#pragma unroll
+            for (int n2 = 0; n2 < 18; n2++) Y[h * 324 + idx * 18 + n2] = v[n2];
+        }
         __syncthreads();
-        dit_all(X, 2, tabs + tab::INV, tabs[tab::W3 + 1], pc);
-        for (int i = threadIdx.x; i < 324; i += 256) {
-            uint32_t u0 = mont(X[i], tabs[tab::UNTW + i], pc);
-            uint32_t u1 = mont(X[324 + i], tabs[tab::UNTW + 324 + i], pc);
-            uint32_t ahi = mont(subm(u0, u1, pc.p), tabs[tab::DINV], pc);
-            uint32_t alo = subm(u0, mont(ahi, tabs[tab::ETA0], pc), pc.p);
-            res[k][i] = alo;
+        if (t < 36) {
+            // inverse pass A: thread (h, n2 = idx)
+            uint32_t v[18];
+#pragma unroll
+            for (int p = 0; p < 18; p++) {
+                const int k1 = krev18(p);
+                const uint32_t y = Y[h * 324 + k1 * 18 + idx];
+                v[p] = k1 ? mont(y, tabs[tab::TWPI + idx * 18 + k1], pc) : y;
+            }
+            dft18_inv(v, w, pc);
+#pragma unroll
+            for (int n1 = 0; n1 < 18; n1++) x[h * 324 + 18 * n1 + idx] = v[n1];
+        }
+        __syncthreads();
+        for (int i = t; i < 324; i += 64) {
+            const uint32_t u0 = mont(x[i], tabs[tab::UNTW + i], pc);
+            const uint32_t u1 = mont(x[324 + i], tabs[tab::UNTW + 324 + i], pc);
+            const uint32_t ahi = mont(subm(u0, u1, pc.p), tabs[tab::DINV], pc);
+            res[k][i] = subm(u0, mont(ahi, tabs[tab::ETA0], pc), pc.p);
             res[k][i + 324] = ahi;
         }
         __syncthreads();
     }
     const u128 Q = ((u128)crt.q_hi << 64) | crt.q_lo;
     const u128 M = ((u128)crt.m_hi << 64) | crt.m_lo;
-    for (int t = threadIdx.x; t < 648; t += 256) {
+    for (int tt = t; tt < 648; tt += 64) {
         // Garner: x = v0 + p0 (v1 + p1 (v2 + p2 v3))
         uint32_t v[MAX_PRIMES];
         for (int i = 0; i < crt.nprimes; i++) {
-            uint32_t x = res[i][t];
+            uint32_t xx = res[i][tt];
             for (int j = 0; j < i; j++) {
-                uint32_t vj = v[j] % crt.p[i];
-                x = mulmod_u64(x >= vj ? x - vj : x + crt.p[i] - vj, crt.inv[i][j], crt.p[i]);
+                const uint32_t vj = v[j] % crt.p[i];
+                xx = mulmod_u64(xx >= vj ? xx - vj : xx + crt.p[i] - vj, crt.inv[i][j], crt.p[i]);
             }
-            v[i] = x;
+            v[i] = xx;
         }
-        u128 x = v[crt.nprimes - 1];
-        for (int i = crt.nprimes - 2; i >= 0; i--) x = x * crt.p[i] + v[i];
-        // centred lift: the exact coefficient is x or x - M
-        bool neg = x > (M >> 1);
-        u128 mag = neg ? M - x : x;
-        // mag mod Q by shift-subtract
-        u128 r = 0;
+        u128 X = v[crt.nprimes - 1];
+        for (int i = crt.nprimes - 2; i >= 0; i--) X = X * crt.p[i] + v[i];
+        // centred lift: the exact coefficient is X or X - M
+        const bool neg = X > (M >> 1);
+        const u128 mag = neg ? M - X : X;
+        u128 r = 0;  // mag mod Q by shift-subtract
         for (int b = 127; b >= 0; b--) {
             r = (r << 1) | ((mag >> b) & 1);
             if (r >= Q) r -= Q;
         }
         if (neg && r) r = Q - r;
-        uint8_t *o = outb + ((size_t)c * 648 + t) * crt.width;
+        uint8_t *o = outb + ((size_t)c * 648 + tt) * crt.width;
         for (int b = 0; b < crt.width; b++) o[b] = (uint8_t)(r >> (8 * b));
     }
 }
@@ -894,9 +1003,15 @@ extern "C" int lbn_set_modulus(lbn_ctx *ctx, uint64_t q_lo, uint64_t q_hi) {
     }
     for (int k = 0; k < np; k++) {
         PC pc{ctx->primes[k].p, ctx->primes[k].pinv_neg};
-        k_ahat<<<(unsigned)s.ring_elements, 256, 0, d->stream>>>(
-            d->amat, d->ntt_tabs + (size_t)k * tab::SIZE, pc, ctx->primes[k].r2,
-            d->ahat + (size_t)k * s.ring_elements * N);
+        Roots18 R;
+        for (int j = 0; j < 9; j++) {
+            R.f[j] = all[(size_t)k * tab::SIZE + tab::R18F + j];
+            R.i[j] = all[(size_t)k * tab::SIZE + tab::R18I + j];
+        }
+        d->roots[k] = R;
+        k_ahat<<<(unsigned)s.ring_elements, 64, 0, d->stream>>>(d->amat, d->ntt_tabs + (size_t)k * tab::SIZE, pc, R,
+                                                                ctx->primes[k].r2,
+                                                                d->ahat + (size_t)k * s.ring_elements * N);
         HIPCHK(ctx, hipGetLastError());
     }
     HIPCHK(ctx, hipStreamSynchronize(d->stream));
@@ -933,16 +1048,17 @@ extern "C" int lbn_commit(lbn_ctx *ctx, uint8_t *out, size_t out_len) {
     const int np = ctx->nprimes;
     fill_u32(d->stream, d->cacc, (size_t)np * COLUMNS * N * 2);
     const int re = (int)s.ring_elements;
-    int msplit = std::max(1, std::min(re, 2048 / (COLUMNS / TC)));  // ~2048 blocks per prime
+    int msplit = std::max(1, std::min(re, 1024 / (COLUMNS / TC)));  // ~1024 blocks per prime
     int m_per = (re + msplit - 1) / msplit;
     msplit = (re + m_per - 1) / m_per;
     tstart(ctx, T_MAC);
     for (int k = 0; k < np; k++) {
         PC pc{ctx->primes[k].p, ctx->primes[k].pinv_neg};
         dim3 grid(COLUMNS / TC, msplit);
-        k_commit_mac<<<grid, 256, 0, d->stream>>>((const uint32_t *)d->table, s.column_len, m_per, re,
-                                                   d->ahat + (size_t)k * re * N, d->ntt_tabs + (size_t)k * tab::SIZE,
-                                                   pc, d->cacc + (size_t)k * COLUMNS * N);
+        k_commit_mac<<<grid, MAC_THREADS, 0, d->stream>>>((const uint32_t *)d->table, s.column_len, m_per, re,
+                                                           d->ahat + (size_t)k * re * N,
+                                                           d->ntt_tabs + (size_t)k * tab::SIZE, pc, d->roots[k],
+                                                           d->cacc + (size_t)k * COLUMNS * N);
         HIPCHK(ctx, hipGetLastError());
     }
     tstop(ctx, T_MAC);
@@ -963,8 +1079,10 @@ extern "C" int lbn_commit(lbn_ctx *ctx, uint8_t *out, size_t out_len) {
     PC pcs[MAX_PRIMES];
     for (int i = 0; i < MAX_PRIMES; i++) pcs[i] = PC{i < np ? ctx->primes[i].p : 1u, i < np ? ctx->primes[i].pinv_neg : 0u};
     tstart(ctx, T_FINISH);
-    k_commit_finish<<<COLUMNS, 256, 0, d->stream>>>(d->cacc, d->ntt_tabs, crt, pcs[0], pcs[1], pcs[2], pcs[3],
-                                                      d->cbytes);
+    RootsAll ra{};
+    for (int i = 0; i < np; i++) ra.r[i] = d->roots[i];
+    k_commit_finish<<<COLUMNS, 64, 0, d->stream>>>(d->cacc, d->ntt_tabs, crt, pcs[0], pcs[1], pcs[2], pcs[3], ra,
+                                                     d->cbytes);
     tstop(ctx, T_FINISH);
     HIPCHK(ctx, hipGetLastError());
     HIPCHK(ctx, hipMemcpyAsync(out, d->cbytes, clen_bytes, hipMemcpyDeviceToHost, d->stream));
@@ -1193,7 +1311,8 @@ extern "C" int lbn_fold(lbn_ctx *ctx, const lbn_challenge *ch, int n_challenges,
             const int neg = (int)((ch[j].signs >> t) & 1) ^ (pos & 1);
             cols[fillp[2 * pos + neg]++] = (uint16_t)j;
         }
-    const int maxlen = *std::max_element(cnt.begin(), cnt.end());
+    int maxlen = 0;  // one counter per position holds positives and complemented negatives
+    for (int pos = 0; pos < 162; pos++) maxlen = std::max(maxlen, cnt[2 * pos] + cnt[2 * pos + 1]);
     int nb = 1;
     while ((1 << nb) <= maxlen) nb++;
     HIPCHK(ctx, hipSetDevice(ctx->device));
@@ -1216,6 +1335,7 @@ extern "C" int lbn_fold(lbn_ctx *ctx, const lbn_challenge *ch, int n_challenges,
         FOLD_CASE(9)
         FOLD_CASE(10)
         FOLD_CASE(11)
+        FOLD_CASE(12)
         default:
             return ctx->fail(LBN_E_ARG, "challenge lists too long");
     }

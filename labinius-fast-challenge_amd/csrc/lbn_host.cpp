@@ -309,40 +309,37 @@ bool prime_tables(const PrimeInfo &pi, std::vector<uint32_t> &t) {
     if (pw(w, 1944, p) != 1 || pw(w, 972, p) == 1 || pw(w, 648, p) == 1) return false;
     const uint32_t eta[2] = {pw(w, 324, p), pw(w, 1620, p)};
     const uint32_t beta[2] = {w, pw(w, 5, p)};
-    const uint32_t rho = pw(w, 6, p);
+    const uint32_t rho = pw(w, 6, p), w18 = pw(rho, 18, p);
     // layout must match lbn_device.hip namespace tab
-    const int TWB = 0, TW = TWB + 2 * 324 * 3, ETA = TW + 2 * 324, FWD = ETA + 2, INV = FWD + 323, W3 = INV + 323,
-              UNTW = W3 + 2, ETA0 = UNTW + 2 * 324, DINV = ETA0 + 1, SIZE = DINV + 1;
+    const int TWB = 0, TW = TWB + 2 * 324 * 3, ETA = TW + 2 * 324, UNTW = ETA + 2, ETA0 = UNTW + 2 * 324,
+              DINV = ETA0 + 1, TWPF = DINV + 1, TWPI = TWPF + 324, R18F = TWPI + 324, R18I = R18F + 9,
+              SIZE = R18I + 9;
     t.assign(SIZE, 0);
+    const uint32_t i324 = inv(324);
     for (int h = 0; h < 2; h++)
         for (int i = 0; i < 324; i++) {
-            uint32_t b = pw(beta[h], i, p);
+            const uint32_t b = pw(beta[h], i, p);
             t[TWB + (h * 324 + i) * 3 + 0] = b;
             t[TWB + (h * 324 + i) * 3 + 1] = (uint32_t)((uint64_t)eta[h] * b % p);
             t[TWB + (h * 324 + i) * 3 + 2] = (uint32_t)((uint64_t)(1 + eta[h]) % p * b % p);
             t[TW + h * 324 + i] = mf(b);
-            uint32_t ib = pw(inv(beta[h]), i, p);
-            t[UNTW + h * 324 + i] = mf((uint32_t)((uint64_t)ib * inv(324) % p));
+            t[UNTW + h * 324 + i] = mf((uint32_t)((uint64_t)pw(inv(beta[h]), i, p) * i324 % p));
         }
     t[ETA] = mf(eta[0]);
     t[ETA + 1] = mf(eta[1]);
-    const int spans[6] = {324, 162, 81, 27, 9, 3}, rads[6] = {2, 2, 3, 3, 3, 3};
-    int at = 0;
-    for (int s = 0; s < 6; s++) {
-        const int span = spans[s], r = rads[s], m = span / r;
-        const uint32_t ws = pw(rho, 324 / span, p), wsi = inv(ws);
-        for (int j = 0; j < m; j++)
-            for (int k = 1; k < r; k++) {
-                t[FWD + at] = mf(pw(ws, (uint64_t)j * k, p));
-                t[INV + at] = mf(pw(wsi, (uint64_t)j * k, p));
-                at++;
-            }
-    }
-    if (at != 323) return false;
-    t[W3] = mf(pw(rho, 108, p));
-    t[W3 + 1] = mf(inv(pw(rho, 108, p)));
     t[ETA0] = mf(eta[0]);
     t[DINV] = mf(inv((eta[0] + p - eta[1]) % p));
+    const uint32_t rhoi = inv(rho);
+    for (int n2 = 0; n2 < 18; n2++)
+        for (int k1 = 0; k1 < 18; k1++) {
+            t[TWPF + n2 * 18 + k1] = mf(pw(rho, (uint64_t)n2 * k1, p));
+            t[TWPI + n2 * 18 + k1] = mf(pw(rhoi, (uint64_t)n2 * k1, p));
+        }
+    const uint32_t w18i = inv(w18);
+    for (int j = 0; j < 9; j++) {
+        t[R18F + j] = mf(pw(w18, j, p));
+        t[R18I + j] = mf(pw(w18i, j, p));
+    }
     return true;
 }
 

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <vector>
 
@@ -29,8 +30,8 @@ bool prime_tables(const PrimeInfo &pi, std::vector<uint32_t> &tabs);
 // Layout of one prime's NTT constant tables (u32 offsets), built by prime_tables().
 namespace lbn {
 namespace tab {
-constexpr int TWB = 0;                 // [2][324][3] binary split+twist values (normal form)
-constexpr int TW = TWB + 2 * 324 * 3;  // [2][324] beta_h^i (Montgomery)
+constexpr int TWB = 0;                 // [2][324][4] binary split+twist values: 0, b, eta b, (1+eta) b
+constexpr int TW = TWB + 2 * 324 * 4;  // [2][324] beta_h^i (Montgomery)
 constexpr int ETA = TW + 2 * 324;      // [2] eta_h (Montgomery)
 constexpr int UNTW = ETA + 2;          // [2][324] beta_h^-i / 324 (Montgomery)
 constexpr int ETA0 = UNTW + 2 * 324;   // eta_0 (Montgomery)
@@ -572,26 +573,49 @@ __device__ __forceinline__ void radix3(uint32_t &a, uint32_t &b, uint32_t &c, ui
 }
 
 // Forward 18-point DFT (decimation in frequency: 2, then 3, 3), output in krev18 order.
+// Lazy reduction: inputs and outputs lie in [0, 2p); the primes satisfy 6p < 2^32, so sums of up
+// to three such values stay in 32 bits, and a Montgomery product with a constant below p accepts
+// any 32-bit operand and returns [0, 2p) — a value that is multiplied next is never corrected.
+__device__ __forceinline__ uint32_t red4(uint32_t x, uint32_t p2) { return min(x, x - p2); }  // [0,4p)->[0,2p)
+__device__ __forceinline__ uint32_t red6(uint32_t x, uint32_t p2, uint32_t p4) {             // [0,6p)->[0,2p)
+    x = min(x, x - p4);
+    return min(x, x - p2);
+}
+
+template <bool TW>
+__device__ __forceinline__ void radix3_lazy(uint32_t &a, uint32_t &b, uint32_t &c, uint32_t w3, uint32_t w1,
+                                            uint32_t w2, PC pc, uint32_t p2, uint32_t p4) {
+    const uint32_t t = mont_lazy(b - c + p2, w3, pc);
+    const uint32_t x0 = red6(a + b + c, p2, p4);
+    const uint32_t x1 = a - c + t + p2;  // (0, 6p)
+    const uint32_t x2 = a - b - t + p4;  // (0, 6p)
+    a = x0;
+    if (TW) {
+        b = mont_lazy(x1, w1, pc);
+        c = mont_lazy(x2, w2, pc);
+    } else {
+        b = red6(x1, p2, p4);
+        c = red6(x2, p2, p4);
+    }
+}
+
 __device__ __forceinline__ void dft18_fwd(uint32_t (&v)[18], const uint32_t (&w)[9], PC pc) {
+    const uint32_t p2 = 2 * pc.p, p4 = 4 * pc.p;
 #pragma unroll
     for (int j = 0; j < 9; j++) {
         const uint32_t a = v[j], b = v[j + 9];
-        v[j] = addm(a, b, pc.p);
-        const uint32_t d = subm(a, b, pc.p);
-        v[j + 9] = j ? mont(d, w[j], pc) : d;
+        v[j] = red4(a + b, p2);
+        const uint32_t d = a - b + p2;  // (0, 4p)
+        v[j + 9] = j ? mont_lazy(d, w[j], pc) : red4(d, p2);
     }
 #pragma unroll
-    for (int B = 0; B < 18; B += 9)
+    for (int B = 0; B < 18; B += 9) {
+        radix3_lazy<false>(v[B], v[B + 3], v[B + 6], w[6], 0, 0, pc, p2, p4);
+        radix3_lazy<true>(v[B + 1], v[B + 4], v[B + 7], w[6], w[2], w[4], pc, p2, p4);
+        radix3_lazy<true>(v[B + 2], v[B + 5], v[B + 8], w[6], w[4], w[8], pc, p2, p4);
+    }
 #pragma unroll
-        for (int j = 0; j < 3; j++) {
-            radix3(v[B + j], v[B + j + 3], v[B + j + 6], w[6], pc);
-            if (j) {
-                v[B + j + 3] = mont(v[B + j + 3], w[2 * j], pc);
-                v[B + j + 6] = mont(v[B + j + 6], w[4 * j], pc);
-            }
-        }
-#pragma unroll
-    for (int B = 0; B < 18; B += 3) radix3(v[B], v[B + 1], v[B + 2], w[6], pc);
+    for (int B = 0; B < 18; B += 3) radix3_lazy<false>(v[B], v[B + 1], v[B + 2], w[6], 0, 0, pc, p2, p4);
 }
 
 // Exact inverse of dft18_fwd up to the factor 18 (decimation in time, inverse roots).
@@ -623,23 +647,35 @@ __device__ __forceinline__ void passA_store(uint32_t (&v)[18], int n2, const uin
 #pragma unroll
     for (int p = 0; p < 18; p++) {
         const int k1 = krev18(p);
-        const uint32_t y = k1 ? mont(v[p], twpf[n2 * 18 + k1], pc) : v[p];
+        const uint32_t y = k1 ? mont_lazy(v[p], twpf[n2 * 18 + k1], pc) : v[p];
         Y[k1 * 18 + n2] = y;
     }
 }
 
 // Transformed matrix: ahat[m][slot] = T(A_m mod p)[slot] * R mod p (Montgomery form for the MAC).
 // One element per block; threads 0..35 run the two halves' transforms.
+// The matrix is taken centred (a > Q/2 stands for a - Q, the same residue mod Q), which halves
+// the range of the exact product and so the number of primes.
+__device__ __forceinline__ uint32_t u128_mod(u128 x, uint32_t p) {
+    const uint64_t two64 = ((uint64_t)0 - p) % p;  // 2^64 mod p
+    return (uint32_t)((((uint64_t)(x >> 64) % p) * two64 + (uint64_t)x % p) % p);
+}
+
 __global__ __launch_bounds__(64) void k_ahat(const uint64_t *amat, const uint32_t *tabs, PC pc, Roots18 R,
-                                             uint32_t r2, uint32_t *ahat) {
+                                             uint32_t r2, uint64_t q_lo, uint64_t q_hi, uint32_t *ahat) {
     __shared__ uint32_t a[648];
     __shared__ uint32_t Y[2 * 324];
     const size_t m = blockIdx.x;
     const int t = threadIdx.x;
-    const uint64_t two64 = ((uint64_t)0 - pc.p) % pc.p;  // 2^64 mod p
+    const u128 Q = ((u128)q_hi << 64) | q_lo;
     for (int i = t; i < 648; i += 64) {
-        const uint64_t lo = amat[(m * 648 + i) * 2], hi = amat[(m * 648 + i) * 2 + 1];
-        a[i] = (uint32_t)(((hi % pc.p) * two64 + lo % pc.p) % pc.p);
+        const u128 x = ((u128)amat[(m * 648 + i) * 2 + 1] << 64) | amat[(m * 648 + i) * 2];
+        if (x > Q / 2) {
+            const uint32_t r = u128_mod(Q - x, pc.p);  // residue of x - Q
+            a[i] = r ? pc.p - r : 0;
+        } else {
+            a[i] = u128_mod(x, pc.p);
+        }
     }
     __syncthreads();
     const int h = t / 18, idx = t % 18;
@@ -673,18 +709,52 @@ __global__ __launch_bounds__(64) void k_ahat(const uint64_t *amat, const uint32_
 constexpr int TC = 16;
 constexpr int MAC_THREADS = TC * 36;
 
+// 128-bit logical right shift by s in [0, 31] of four little-endian words.
+__device__ __forceinline__ void shr128(const uint4 w, uint32_t s, uint32_t (&r)[4]) {
+    r[0] = __builtin_amdgcn_alignbit(w.y, w.x, s);
+    r[1] = __builtin_amdgcn_alignbit(w.z, w.y, s);
+    r[2] = __builtin_amdgcn_alignbit(w.w, w.z, s);
+    r[3] = w.w >> s;
+}
+
+template <int POS>
+__device__ __forceinline__ uint32_t bit_at(const uint32_t (&r)[4]) {
+    if (POS >= 128) return 0;
+    return (r[(POS >> 5) & 3] >> (POS & 31)) & 1;
+}
+
+// The binary inputs of pass A for thread n2: x[i], i = 18 n1 + n2, from coefficient i (low) and
+// i + 324 (high); coefficient c is bit c >> 2 of word c & 3. As n1 runs, c & 3 alternates between
+// words n2 & 3 (n1 even) and (n2 + 2) & 3 (n1 odd), and after shifting those words right by
+// n2 >> 2 and (n2 + 18) >> 2 the bit of n1 = 2u (+1) sits at 9u (+81 for the high half), a
+// compile-time position. Bits shifted past 127 read as zero: those coefficients are >= 512.
+template <int U>
+__device__ __forceinline__ void gather_pair(const uint32_t (&ra)[4], const uint32_t (&rb)[4], const uint32_t *twb,
+                                            int n2, uint32_t (&v)[18]) {
+    {
+        const int n1 = 2 * U, i = 18 * n1 + n2;
+        const uint32_t sel = bit_at<9 * U>(ra) | (bit_at<9 * U + 81>(ra) << 1);
+        v[n1] = twb[i * 4 + sel];
+    }
+    if (2 * U + 1 < 18) {
+        const int n1 = 2 * U + 1, i = 18 * n1 + n2;
+        const uint32_t sel = bit_at<9 * U>(rb) | (bit_at<9 * U + 81>(rb) << 1);
+        v[n1] = twb[i * 4 + sel];
+    }
+}
+
 __global__ __launch_bounds__(MAC_THREADS) void k_commit_mac(const uint32_t *table32, size_t clen, int m_per, int re,
                                                             const uint32_t *ahat, const uint32_t *tabs, PC pc,
                                                             Roots18 R, unsigned long long *cacc) {
     __shared__ uint32_t Y[TC * 2 * 324];
-    __shared__ uint32_t s_twb[2 * 324 * 3];
+    __shared__ uint32_t s_twb[2 * 324 * 4];
     __shared__ uint32_t s_twp[324];
-    __shared__ uint32_t s_bits[TC * 16];
+    __shared__ uint4 s_bits[TC * 4];
     const int t = threadIdx.x;
     const int col = t / 36, h = (t / 18) & 1, idx = t % 18;
     const int c0 = blockIdx.x * TC;
     const int m0 = blockIdx.y * m_per, m1 = min(re, m0 + m_per);
-    for (int i = t; i < 2 * 324 * 3; i += MAC_THREADS) s_twb[i] = tabs[tab::TWB + i];
+    for (int i = t; i < 2 * 324 * 4; i += MAC_THREADS) s_twb[i] = tabs[tab::TWB + i];
     for (int i = t; i < 324; i += MAC_THREADS) s_twp[i] = tabs[tab::TWPF + i];
     uint32_t w[9];
 #pragma unroll
@@ -692,26 +762,31 @@ __global__ __launch_bounds__(MAC_THREADS) void k_commit_mac(const uint32_t *tabl
     unsigned long long acc[18];
 #pragma unroll
     for (int p = 0; p < 18; p++) acc[p] = 0;
-    const uint32_t *bits = s_bits + col * 16;
-    const uint32_t *twb = s_twb + h * 324 * 3;
+    const uint4 *bits = s_bits + col * 4;
+    const uint32_t *twb = s_twb + h * 324 * 4;
     uint32_t *Yh = Y + (col * 2 + h) * 324;
+    const int wa = idx & 3, wb = (idx + 2) & 3;
+    const uint32_t sa = idx >> 2, sb = (idx + 18) >> 2;
+    const uint4 *tab4 = (const uint4 *)table32;
     for (int m = m0; m < m1; m++) {
-        if (t < TC * 16) {
-            const int bc = t >> 4, wd = t & 15;  // word wd >> 2, u32 wd & 3
-            s_bits[t] = table32[(((size_t)(c0 + bc) * clen + 4 * (size_t)m + (wd >> 2)) << 2) + (wd & 3)];
+        if (t < TC * 4) {
+            const int bc = t >> 2, wd = t & 3;  // column bc, word wd of ring element m
+            s_bits[t] = tab4[(size_t)(c0 + bc) * clen + 4 * (size_t)m + wd];
         }
         __syncthreads();
         {
-            // pass A: x[i] for i = 18 n1 + n2; coefficient c is bit (c >> 2) of word (c & 3)
-            uint32_t v[18];
-#pragma unroll
-            for (int n1 = 0; n1 < 18; n1++) {
-                const int i = 18 * n1 + idx, ch = i + 324;
-                const uint32_t lo = (bits[(i & 3) * 4 + (i >> 7)] >> ((i >> 2) & 31)) & 1;
-                const uint32_t hi = ch < 512 ? (bits[(ch & 3) * 4 + (ch >> 7)] >> ((ch >> 2) & 31)) & 1 : 0;
-                const uint32_t sel = lo | (hi << 1);
-                v[n1] = sel ? twb[i * 3 + sel - 1] : 0u;
-            }
+            uint32_t v[18], ra[4], rb[4];
+            shr128(bits[wa], sa, ra);
+            shr128(bits[wb], sb, rb);
+            gather_pair<0>(ra, rb, twb, idx, v);
+            gather_pair<1>(ra, rb, twb, idx, v);
+            gather_pair<2>(ra, rb, twb, idx, v);
+            gather_pair<3>(ra, rb, twb, idx, v);
+            gather_pair<4>(ra, rb, twb, idx, v);
+            gather_pair<5>(ra, rb, twb, idx, v);
+            gather_pair<6>(ra, rb, twb, idx, v);
+            gather_pair<7>(ra, rb, twb, idx, v);
+            gather_pair<8>(ra, rb, twb, idx, v);
             passA_store(v, idx, s_twp, Yh, w, pc);
         }
         __syncthreads();
@@ -740,33 +815,52 @@ struct CrtArg {
     uint32_t p[MAX_PRIMES];
     uint32_t inv[MAX_PRIMES][MAX_PRIMES];  // inv[i][j] = p_j^-1 mod p_i (j < i)
     uint64_t q_lo, q_hi;
-    uint64_t m_lo, m_hi;  // product of the primes
+    uint64_t half_m[3];                     // floor(M / 2), M = product of the primes (192 bits)
+    uint64_t m[3];
     int nprimes, width;
+    PC pc[MAX_PRIMES];
+    Roots18 roots[MAX_PRIMES];
 };
 
 __device__ __forceinline__ uint32_t mulmod_u64(uint32_t a, uint32_t b, uint32_t p) {
     return (uint32_t)(((uint64_t)a * b) % p);
 }
 
-struct RootsAll {
-    Roots18 r[MAX_PRIMES];
-};
+// 192-bit helpers for the mixed-radix value (up to five 29.4-bit primes)
+__device__ __forceinline__ void u192_muladd(uint64_t (&x)[3], uint32_t p, uint32_t v) {
+    u128 t = (u128)x[0] * p + v;
+    x[0] = (uint64_t)t;
+    t = (u128)x[1] * p + (uint64_t)(t >> 64);
+    x[1] = (uint64_t)t;
+    t = (u128)x[2] * p + (uint64_t)(t >> 64);
+    x[2] = (uint64_t)t;
+}
+__device__ __forceinline__ bool u192_gt(const uint64_t (&a)[3], const uint64_t (&b)[3]) {
+    if (a[2] != b[2]) return a[2] > b[2];
+    if (a[1] != b[1]) return a[1] > b[1];
+    return a[0] > b[0];
+}
+__device__ __forceinline__ void u192_sub(const uint64_t (&a)[3], const uint64_t (&b)[3], uint64_t (&r)[3]) {
+    const uint64_t r0 = a[0] - b[0];
+    const uint64_t b0 = a[0] < b[0];
+    const uint64_t r1 = a[1] - b[1] - b0;
+    const uint64_t b1 = (a[1] < b[1]) || (a[1] == b[1] && b0);
+    r[0] = r0, r[1] = r1, r[2] = a[2] - b[2] - b1;
+}
 
 __global__ __launch_bounds__(64) void k_commit_finish(const unsigned long long *cacc, const uint32_t *tabs_all,
-                                                      CrtArg crt, PC pc0, PC pc1, PC pc2, PC pc3, RootsAll roots,
-                                                      uint8_t *outb) {
+                                                      CrtArg crt, uint8_t *outb) {
     __shared__ uint32_t Y[2 * 324];
     __shared__ uint32_t x[648];
     __shared__ uint32_t res[MAX_PRIMES][648];
     const int c = blockIdx.x, t = threadIdx.x;
     const int h = t / 18, idx = t % 18;
-    const PC pcs[4] = {pc0, pc1, pc2, pc3};
     for (int k = 0; k < crt.nprimes; k++) {
-        const PC pc = pcs[k];
+        const PC pc = crt.pc[k];
         const uint32_t *tabs = tabs_all + k * tab::SIZE;
         uint32_t w[9];
 #pragma unroll
-        for (int j = 0; j < 9; j++) w[j] = roots.r[k].i[j];
+        for (int j = 0; j < 9; j++) w[j] = crt.roots[k].i[j];
         if (t < 36) {
             // inverse pass B: thread (h, k1 = idx)
             const unsigned long long *src = cacc + ((size_t)k * 1024 + c) * 648 + h * 324 + idx * 18;
@@ -802,9 +896,8 @@ __global__ __launch_bounds__(64) void k_commit_finish(const unsigned long long *
         __syncthreads();
     }
     const u128 Q = ((u128)crt.q_hi << 64) | crt.q_lo;
-    const u128 M = ((u128)crt.m_hi << 64) | crt.m_lo;
     for (int tt = t; tt < 648; tt += 64) {
-        // Garner: x = v0 + p0 (v1 + p1 (v2 + p2 v3))
+        // Garner: x = v0 + p0 (v1 + p1 (v2 + ...))
         uint32_t v[MAX_PRIMES];
         for (int i = 0; i < crt.nprimes; i++) {
             uint32_t xx = res[i][tt];
@@ -814,14 +907,18 @@ __global__ __launch_bounds__(64) void k_commit_finish(const unsigned long long *
             }
             v[i] = xx;
         }
-        u128 X = v[crt.nprimes - 1];
-        for (int i = crt.nprimes - 2; i >= 0; i--) X = X * crt.p[i] + v[i];
+        uint64_t X[3] = {v[crt.nprimes - 1], 0, 0};
+        for (int i = crt.nprimes - 2; i >= 0; i--) u192_muladd(X, crt.p[i], v[i]);
         // centred lift: the exact coefficient is X or X - M
-        const bool neg = X > (M >> 1);
-        const u128 mag = neg ? M - X : X;
-        u128 r = 0;  // mag mod Q by shift-subtract
-        for (int b = 127; b >= 0; b--) {
-            r = (r << 1) | ((mag >> b) & 1);
+        const bool neg = u192_gt(X, crt.half_m);
+        uint64_t mag[3];
+        if (neg)
+            u192_sub(crt.m, X, mag);
+        else
+            mag[0] = X[0], mag[1] = X[1], mag[2] = X[2];
+        u128 r = 0;  // mag mod Q by shift-subtract (Q < 2^96)
+        for (int b = 191; b >= 0; b--) {
+            r = (r << 1) | ((mag[b >> 6] >> (b & 63)) & 1);
             if (r >= Q) r -= Q;
         }
         if (neg && r) r = Q - r;
@@ -958,24 +1055,20 @@ extern "C" int lbn_set_modulus(lbn_ctx *ctx, uint64_t q_lo, uint64_t q_hi) {
     HIPCHK(ctx, hipSetDevice(ctx->device));
     DeviceState *d = ctx->dev;
     const Shape &s = ctx->shape;
-    // Primes: the largest 31-bit primes = 1 (mod 1944), with a primitive 1944-th root each.
-    static const PrimeInfo PR[MAX_PRIMES] = {
-        {2147478481u, 0, 0, 1158804115u}, {2147449321u, 0, 0, 738766195u},
-        {2147429881u, 0, 0, 878989323u},  {2147425993u, 0, 0, 1104246913u}};
-    // |coefficient| <= RE * 1024 * (Q - 1): CRT range must exceed twice that.
-    u128 bound = (u128)s.ring_elements * 1024 * (q - 1);
+    // Primes: the largest p = 1 (mod 1944) with 6p < 2^32, each with a primitive 1944-th root.
+    static const PrimeInfo PR[MAX_PRIMES] = {{715809961u, 0, 0, 181294596u},
+                                             {715808017u, 0, 0, 501708134u},
+                                             {715806073u, 0, 0, 148596466u},
+                                             {715757473u, 0, 0, 282162646u},
+                                             {715722481u, 0, 0, 150155317u}};
+    // With the matrix centred (|a| <= Q/2) and at most 512 set bits per element, a coefficient of
+    // one product is at most 3 * 512 * Q/2 = 768 Q, so |coefficient| <= RE * 768 * Q; the CRT range
+    // must exceed twice that. Counted in doubles (M can pass 2^128): log2 M > log2(2 bound) + margin.
+    const double need_bits = std::log2(2.0 * (double)s.ring_elements * 768.0) + std::log2((double)q) + 0.01;
     int np = 0;
-    u128 M = 1;
-    bool big = false;
-    while (np < MAX_PRIMES) {
-        M *= PR[np].p;
-        np++;
-        if (M / 2 > bound) {
-            big = true;
-            break;
-        }
-    }
-    if (!big) return ctx->fail(LBN_E_MODULUS, "modulus too large for the CRT range");
+    double have = 0;
+    while (np < MAX_PRIMES && have <= need_bits) have += std::log2((double)PR[np++].p);
+    if (have <= need_bits) return ctx->fail(LBN_E_MODULUS, "modulus too large for the CRT range");
     ctx->nprimes = np;
     std::vector<uint32_t> all((size_t)MAX_PRIMES * tab::SIZE, 0);
     for (int k = 0; k < np; k++) {
@@ -1010,7 +1103,7 @@ extern "C" int lbn_set_modulus(lbn_ctx *ctx, uint64_t q_lo, uint64_t q_hi) {
         }
         d->roots[k] = R;
         k_ahat<<<(unsigned)s.ring_elements, 64, 0, d->stream>>>(d->amat, d->ntt_tabs + (size_t)k * tab::SIZE, pc, R,
-                                                                ctx->primes[k].r2,
+                                                                ctx->primes[k].r2, q_lo, q_hi,
                                                                 d->ahat + (size_t)k * s.ring_elements * N);
         HIPCHK(ctx, hipGetLastError());
     }
@@ -1067,22 +1160,27 @@ extern "C" int lbn_commit(lbn_ctx *ctx, uint8_t *out, size_t out_len) {
     crt.width = ctx->width;
     crt.q_lo = (uint64_t)ctx->q;
     crt.q_hi = (uint64_t)(ctx->q >> 64);
-    u128 M = 1;
-    for (int i = 0; i < np; i++) {
-        crt.p[i] = ctx->primes[i].p;
-        M *= ctx->primes[i].p;
-        for (int j = 0; j < i; j++) crt.inv[i][j] = powmod32(ctx->primes[j].p % ctx->primes[i].p, ctx->primes[i].p - 2,
-                                                            ctx->primes[i].p);
+    {
+        uint64_t M[3] = {1, 0, 0};  // product of the primes, 192 bits
+        for (int i = 0; i < np; i++) {
+            crt.p[i] = ctx->primes[i].p;
+            crt.pc[i] = PC{ctx->primes[i].p, ctx->primes[i].pinv_neg};
+            crt.roots[i] = d->roots[i];
+            u128 t = (u128)M[0] * crt.p[i];
+            M[0] = (uint64_t)t;
+            t = (u128)M[1] * crt.p[i] + (uint64_t)(t >> 64);
+            M[1] = (uint64_t)t;
+            M[2] = M[2] * crt.p[i] + (uint64_t)(t >> 64);
+            for (int j = 0; j < i; j++)
+                crt.inv[i][j] = powmod32(ctx->primes[j].p % ctx->primes[i].p, ctx->primes[i].p - 2, ctx->primes[i].p);
+        }
+        for (int w = 0; w < 3; w++) crt.m[w] = M[w];
+        crt.half_m[0] = (M[0] >> 1) | (M[1] << 63);
+        crt.half_m[1] = (M[1] >> 1) | (M[2] << 63);
+        crt.half_m[2] = M[2] >> 1;
     }
-    crt.m_lo = (uint64_t)M;
-    crt.m_hi = (uint64_t)(M >> 64);
-    PC pcs[MAX_PRIMES];
-    for (int i = 0; i < MAX_PRIMES; i++) pcs[i] = PC{i < np ? ctx->primes[i].p : 1u, i < np ? ctx->primes[i].pinv_neg : 0u};
     tstart(ctx, T_FINISH);
-    RootsAll ra{};
-    for (int i = 0; i < np; i++) ra.r[i] = d->roots[i];
-    k_commit_finish<<<COLUMNS, 64, 0, d->stream>>>(d->cacc, d->ntt_tabs, crt, pcs[0], pcs[1], pcs[2], pcs[3], ra,
-                                                     d->cbytes);
+    k_commit_finish<<<COLUMNS, 64, 0, d->stream>>>(d->cacc, d->ntt_tabs, crt, d->cbytes);
     tstop(ctx, T_FINISH);
     HIPCHK(ctx, hipGetLastError());
     HIPCHK(ctx, hipMemcpyAsync(out, d->cbytes, clen_bytes, hipMemcpyDeviceToHost, d->stream));

@@ -311,7 +311,7 @@ bool prime_tables(const PrimeInfo &pi, std::vector<uint32_t> &t) {
     const uint32_t beta[2] = {w, pw(w, 5, p)};
     const uint32_t rho = pw(w, 6, p), w18 = pw(rho, 18, p);
     // layout must match lbn_device.hip namespace tab
-    const int TWB = 0, TW = TWB + 2 * 324 * 3, ETA = TW + 2 * 324, UNTW = ETA + 2, ETA0 = UNTW + 2 * 324,
+    const int TWB = 0, TW = TWB + 2 * 324 * 4, ETA = TW + 2 * 324, UNTW = ETA + 2, ETA0 = UNTW + 2 * 324,
               DINV = ETA0 + 1, TWPF = DINV + 1, TWPI = TWPF + 324, R18F = TWPI + 324, R18I = R18F + 9,
               SIZE = R18I + 9;
     t.assign(SIZE, 0);
@@ -319,9 +319,10 @@ bool prime_tables(const PrimeInfo &pi, std::vector<uint32_t> &t) {
     for (int h = 0; h < 2; h++)
         for (int i = 0; i < 324; i++) {
             const uint32_t b = pw(beta[h], i, p);
-            t[TWB + (h * 324 + i) * 3 + 0] = b;
-            t[TWB + (h * 324 + i) * 3 + 1] = (uint32_t)((uint64_t)eta[h] * b % p);
-            t[TWB + (h * 324 + i) * 3 + 2] = (uint32_t)((uint64_t)(1 + eta[h]) % p * b % p);
+            t[TWB + (h * 324 + i) * 4 + 0] = 0;
+            t[TWB + (h * 324 + i) * 4 + 1] = b;
+            t[TWB + (h * 324 + i) * 4 + 2] = (uint32_t)((uint64_t)eta[h] * b % p);
+            t[TWB + (h * 324 + i) * 4 + 3] = (uint32_t)((uint64_t)(1 + eta[h]) % p * b % p);
             t[TW + h * 324 + i] = mf(b);
             t[UNTW + h * 324 + i] = mf((uint32_t)((uint64_t)pw(inv(beta[h]), i, p) * i324 % p));
         }

@@ -17,7 +17,7 @@ constexpr int PACK = 128;        // bits per GHASH word
 constexpr int COLUMN_LOG = 10;   // columns = 1024 (verifier/proof_verify/src/main.rs:37)
 constexpr int COLUMNS = 1 << COLUMN_LOG;
 constexpr int WEIGHT = 28;       // challenge weight (pcs/src/challenge.rs:56)
-constexpr int MAX_PRIMES = 4;    // 31-bit CRT primes for the exact commitment
+constexpr int MAX_PRIMES = 5;    // CRT primes (below 2^32/6) for the exact commitment
 
 struct Shape {
     int log_vars = 0, log_packed = 0, row_log = 0;

@@ -215,16 +215,19 @@ __device__ __forceinline__ uint32_t mont_lazy(uint32_t a, uint32_t b, PC c) {
     uint64_t u = t + (uint64_t)m * c.p;
     return (uint32_t)(u >> 32);
 }
+// Corrections use unsigned min: for r in [0, 2p), min(r, r - p) is r mod p (r - p wraps when
+// r < p). No compare-and-select, hence no scalar condition register between VALU ops.
 __device__ __forceinline__ uint32_t mont(uint32_t a, uint32_t b, PC c) {
-    uint32_t r = mont_lazy(a, b, c);
-    return r >= c.p ? r - c.p : r;
+    const uint32_t r = mont_lazy(a, b, c);
+    return min(r, r - c.p);
 }
 __device__ __forceinline__ uint32_t addm(uint32_t a, uint32_t b, uint32_t p) {
-    uint32_t r = a + b;
-    return r >= p ? r - p : r;
+    const uint32_t r = a + b;
+    return min(r, r - p);
 }
 __device__ __forceinline__ uint32_t subm(uint32_t a, uint32_t b, uint32_t p) {
-    return a >= b ? a - b : a + p - b;
+    const uint32_t d = a - b;
+    return min(d, d + p);
 }
 
 }  // namespace dev

@@ -485,67 +485,102 @@ __device__ __forceinline__ void ctr_add4(uint32_t (&c)[NB], uint32_t x0, uint32_
     ctr_ripple<NB>(c, k3, 2);
 }
 
+// One block folds ring elements m = blockIdx.x, blockIdx.x + gridDim.x, ...: the column lists
+// (28 x 1024 u16) are staged in LDS once, the 1024 columns' 64-byte element m is staged per m
+// (column-major, 16 u32 per column), and task (pos, k) counts the four words of word k of every
+// listed column with four NB-plane counters, so each list entry costs one u16 and one 16-byte LDS
+// read.
+constexpr int FOLD_THREADS = 704;  // 11 waves for the 648 (pos, k) tasks
+constexpr int FOLD_MAX_ENTRIES = 1024 * 32;
+
 template <int NB>
-__global__ __launch_bounds__(256) void k_fold(const uint32_t *table32, size_t clen, const int *off,
-                                              const uint16_t *cols, int32_t *out, uint32_t *flag) {
-    __shared__ uint32_t s_w[16 * 1024];
+__global__ __launch_bounds__(FOLD_THREADS) void k_fold(const uint4 *table, size_t clen, size_t re, const int *off,
+                                                       const uint16_t *cols, int n_entries, int32_t *out,
+                                                       uint32_t *flag) {
+    __shared__ uint4 s_w[1024 * 4];
+    __shared__ uint16_t s_cols[FOLD_MAX_ENTRIES];
+    __shared__ int s_off[2 * 162 + 1];
     __shared__ int32_t s_raw[4 * 292];
-    const size_t m = blockIdx.x;
     const int tid = threadIdx.x;
-    for (int idx = tid; idx < 16 * 1024; idx += 256) {
-        const int j = idx >> 4, kw = idx & 15, k = kw >> 2, wd = kw & 3;
-        s_w[kw * 1024 + j] = table32[(((size_t)j * clen + 4 * m + k) << 2) + wd];
-    }
-    for (int i = tid; i < 4 * 292; i += 256) s_raw[i] = 0;
-    __syncthreads();
-    for (int task = tid; task < 162 * 16; task += 256) {
-        const int pos = task >> 4, kw = task & 15;
-        const uint32_t *W = s_w + kw * 1024;
-        uint32_t c[NB];
-#pragma unroll
-        for (int i = 0; i < NB; i++) c[i] = 0;
-        const int e0 = off[2 * pos], e1 = off[2 * pos + 1], e2 = off[2 * pos + 2];
-        // positives then complemented negatives, as one stream of words
-        int e = e0;
-        auto word = [&](int ee) -> uint32_t { return ee < e1 ? W[cols[ee]] : ~W[cols[ee]]; };
-        for (; e + 4 <= e2; e += 4) ctr_add4<NB>(c, word(e), word(e + 1), word(e + 2), word(e + 3));
-        for (; e < e2; e++) ctr_ripple<NB>(c, word(e), 0);
-        const int negs = e2 - e1;
-        const int k = kw >> 2, q0 = pos + 32 * (kw & 3);
-        if (NB <= 8) {
-            // four counts per u32 (8-bit fields): positions p, p+8, p+16, p+24
-#pragma unroll
-            for (int p = 0; p < 8; p++) {
-                uint32_t f = 0;
-#pragma unroll
-                for (int i = 0; i < NB; i++) f |= ((c[i] >> p) & 0x01010101u) << i;
-#pragma unroll
-                for (int b = 0; b < 4; b++) {
-                    const int cnt = (int)((f >> (8 * b)) & 0xFF) - negs;
-                    if (cnt) atomicAdd(&s_raw[k * 292 + q0 + p + 8 * b], cnt);
-                }
-            }
-        } else {
-            // two counts per u32 (16-bit fields): positions p, p+16
-#pragma unroll
-            for (int p = 0; p < 16; p++) {
-                uint32_t f = 0;
-#pragma unroll
-                for (int i = 0; i < NB; i++) f |= ((c[i] >> p) & 0x00010001u) << i;
-#pragma unroll
-                for (int b = 0; b < 2; b++) {
-                    const int cnt = (int)((f >> (16 * b)) & 0xFFFF) - negs;
-                    if (cnt) atomicAdd(&s_raw[k * 292 + q0 + p + 16 * b], cnt);
-                }
-            }
+    for (int i = tid; i < n_entries; i += FOLD_THREADS) s_cols[i] = cols[i];
+    for (int i = tid; i < 2 * 162 + 1; i += FOLD_THREADS) s_off[i] = off[i];
+    for (size_t m = blockIdx.x; m < re; m += gridDim.x) {
+        for (int idx = tid; idx < 1024 * 4; idx += FOLD_THREADS) {
+            const int j = idx >> 2, k = idx & 3;
+            s_w[idx] = table[(size_t)j * clen + 4 * m + k];
         }
-    }
-    __syncthreads();
-    for (int t = tid; t < 648; t += 256) {
-        auto R = [&](int tp) -> int { return tp <= 1159 ? s_raw[(tp & 3) * 292 + (tp >> 2)] : 0; };
-        const int v = t < 324 ? R(t) - R(t + 648) - R(t + 972) : R(t) + R(t + 324);
-        out[m * 648 + t] = v;
-        if (v < -32768 || v > 32767) atomicOr(flag, 1u);
+        for (int i = tid; i < 4 * 292; i += FOLD_THREADS) s_raw[i] = 0;
+        __syncthreads();
+        if (tid < 162 * 4) {
+            const int pos = tid >> 2, k = tid & 3;
+            uint32_t c0[NB], c1[NB], c2[NB], c3[NB];
+#pragma unroll
+            for (int i = 0; i < NB; i++) c0[i] = c1[i] = c2[i] = c3[i] = 0;
+            const int e0 = s_off[2 * pos], e1 = s_off[2 * pos + 1], e2 = s_off[2 * pos + 2];
+            // positives then complemented negatives, four list entries at a time
+            int e = e0;
+            for (; e + 4 <= e2; e += 4) {
+                uint4 w[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    w[u] = s_w[s_cols[e + u] * 4 + k];
+                    if (e + u >= e1) w[u] = make_uint4(~w[u].x, ~w[u].y, ~w[u].z, ~w[u].w);
+                }
+                ctr_add4<NB>(c0, w[0].x, w[1].x, w[2].x, w[3].x);
+                ctr_add4<NB>(c1, w[0].y, w[1].y, w[2].y, w[3].y);
+                ctr_add4<NB>(c2, w[0].z, w[1].z, w[2].z, w[3].z);
+                ctr_add4<NB>(c3, w[0].w, w[1].w, w[2].w, w[3].w);
+            }
+            for (; e < e2; e++) {
+                uint4 w = s_w[s_cols[e] * 4 + k];
+                if (e >= e1) w = make_uint4(~w.x, ~w.y, ~w.z, ~w.w);
+                ctr_ripple<NB>(c0, w.x, 0);
+                ctr_ripple<NB>(c1, w.y, 0);
+                ctr_ripple<NB>(c2, w.z, 0);
+                ctr_ripple<NB>(c3, w.w, 0);
+            }
+            const int negs = e2 - e1;
+            int32_t *raw = s_raw + k * 292 + pos;
+            auto extract = [&](const uint32_t(&c)[NB], int base) {
+                if (NB <= 8) {
+#pragma unroll
+                    for (int p = 0; p < 8; p++) {
+                        uint32_t f = 0;
+#pragma unroll
+                        for (int i = 0; i < NB; i++) f |= ((c[i] >> p) & 0x01010101u) << i;
+#pragma unroll
+                        for (int b = 0; b < 4; b++) {
+                            const int cnt = (int)((f >> (8 * b)) & 0xFF) - negs;
+                            if (cnt) atomicAdd(raw + base + p + 8 * b, cnt);
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (int p = 0; p < 16; p++) {
+                        uint32_t f = 0;
+#pragma unroll
+                        for (int i = 0; i < NB; i++) f |= ((c[i] >> p) & 0x00010001u) << i;
+#pragma unroll
+                        for (int b = 0; b < 2; b++) {
+                            const int cnt = (int)((f >> (16 * b)) & 0xFFFF) - negs;
+                            if (cnt) atomicAdd(raw + base + p + 16 * b, cnt);
+                        }
+                    }
+                }
+            };
+            extract(c0, 0);
+            extract(c1, 32);
+            extract(c2, 64);
+            extract(c3, 96);
+        }
+        __syncthreads();
+        for (int t = tid; t < 648; t += FOLD_THREADS) {
+            auto R = [&](int tp) -> int { return tp <= 1159 ? s_raw[(tp & 3) * 292 + (tp >> 2)] : 0; };
+            const int v = t < 324 ? R(t) - R(t + 648) - R(t + 972) : R(t) + R(t + 324);
+            out[m * 648 + t] = v;
+            if (v < -32768 || v > 32767) atomicOr(flag, 1u);
+        }
+        __syncthreads();
     }
 }
 
@@ -1418,13 +1453,16 @@ extern "C" int lbn_fold(lbn_ctx *ctx, const lbn_challenge *ch, int n_challenges,
     HIPCHK(ctx, hipMemcpyAsync(d->fold_off, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice, d->stream));
     HIPCHK(ctx, hipMemcpyAsync(d->fold_cols, cols.data(), cols.size() * 2, hipMemcpyHostToDevice, d->stream));
     fill_u32(d->stream, d->fold_flag, 1);
-    const unsigned grid = (unsigned)s.ring_elements;
-    const uint32_t *t32 = (const uint32_t *)d->table;
+    int cus = 256;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess) cus = 256;
+    const unsigned grid = (unsigned)std::min<size_t>(s.ring_elements, (size_t)cus);
+    const int n_entries = off[2 * 162];
+    if (n_entries > FOLD_MAX_ENTRIES) return ctx->fail(LBN_E_ARG, "challenge lists too long");
     tstart(ctx, T_FOLD);
-#define FOLD_CASE(B)                                                                                   \
-    case B:                                                                                            \
-        k_fold<B><<<grid, 256, 0, d->stream>>>(t32, s.column_len, d->fold_off, d->fold_cols, d->fold_out, \
-                                               d->fold_flag);                                          \
+#define FOLD_CASE(B)                                                                                          \
+    case B:                                                                                                   \
+        k_fold<B><<<grid, FOLD_THREADS, 0, d->stream>>>(d->table, s.column_len, s.ring_elements, d->fold_off,   \
+                                                        d->fold_cols, n_entries, d->fold_out, d->fold_flag); \
         break;
     switch (std::max(nb, 6)) {
         FOLD_CASE(6)

@@ -27,22 +27,6 @@ void ajtai_matrix_host(u128 q, size_t elements, u128 *out);  // host BLAKE3 expa
 bool prime_tables(const PrimeInfo &pi, std::vector<uint32_t> &tabs);
 }  // namespace lbn
 
-// Layout of one prime's NTT constant tables (u32 offsets), built by prime_tables().
-namespace lbn {
-namespace tab {
-constexpr int TWB = 0;                 // [2][324][4] binary split+twist values: 0, b, eta b, (1+eta) b
-constexpr int TW = TWB + 2 * 324 * 4;  // [2][324] beta_h^i (Montgomery)
-constexpr int ETA = TW + 2 * 324;      // [2] eta_h (Montgomery)
-constexpr int UNTW = ETA + 2;          // [2][324] beta_h^-i / 324 (Montgomery)
-constexpr int ETA0 = UNTW + 2 * 324;   // eta_0 (Montgomery)
-constexpr int DINV = ETA0 + 1;         // 1 / (eta_0 - eta_1) (Montgomery)
-constexpr int TWPF = DINV + 1;         // [18][18] rho^(n2 k1) (Montgomery), rho = w^6
-constexpr int TWPI = TWPF + 324;       // [18][18] rho^-(n2 k1)
-constexpr int R18F = TWPI + 324;       // [9] w18^j, w18 = rho^18 (Montgomery)
-constexpr int R18I = R18F + 9;         // [9] w18^-j
-constexpr int SIZE = R18I + 9;
-}  // namespace tab
-}  // namespace lbn
 
 // The 18-th roots of unity of one prime: w18^j and w18^-j for j < 9, Montgomery form.
 struct Roots18 {
@@ -225,17 +209,17 @@ __global__ __launch_bounds__(256) void k_eq_expand(const uint4 *EH, const uint4 
 // partials v[k] = sum_j bit_k(w_j) eq_hi[j] (crossfield.rs:152-165).
 // ---------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_partials(const uint4 *table, const uint4 *E0, size_t n, uint32_t *out) {
-    __shared__ M4Lds<4> L;
-    __shared__ uint32_t red[4 * 128 * 4];
-    uint32_t acc[8][4] = {};
+    __shared__ uint32_t lds[4 * M4W<4>::WORDS];
+    uint32_t *W = lds + (threadIdx.x >> 6) * M4W<4>::WORDS;
+    uint32_t acc[2][4] = {};
     for (size_t base = (size_t)blockIdx.x * 256; base < n; base += (size_t)gridDim.x * 256) {
         const size_t j = base + threadIdx.x;
         uint4 w = table[j], e = E0[j];
         uint32_t sel[4] = {w.x, w.y, w.z, w.w};
         uint32_t val[4] = {e.x, e.y, e.z, e.w};
-        m4rm_step<4>(acc, sel, val, L);
+        m4w_step<4>(acc, sel, val, W);
     }
-    m4rm_flush<4>(acc, red, out);
+    m4w_flush<4, true>(acc, lds, out);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -246,9 +230,9 @@ __global__ __launch_bounds__(256) void k_partials(const uint4 *table, const uint
 // ---------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_switch_msg0(const uint4 *table, const uint4 *E0, size_t h, uint4 *E1,
                                                      uint32_t *T0, uint32_t *Tinf) {
-    __shared__ M4Lds<4> L;
-    __shared__ uint32_t red[4 * 128 * 4];
-    uint32_t acc0[8][4] = {}, acci[8][4] = {};
+    __shared__ uint32_t lds[4 * M4W<4>::WORDS];
+    uint32_t *W = lds + (threadIdx.x >> 6) * M4W<4>::WORDS;
+    uint32_t acc0[2][4] = {}, acci[2][4] = {};
     for (size_t base = (size_t)blockIdx.x * 256; base < h; base += (size_t)gridDim.x * 256) {
         const size_t j = base + threadIdx.x;
         uint4 ea = E0[j], eb = E0[j + h], wa = table[j], wb = table[j + h];
@@ -257,17 +241,17 @@ __global__ __launch_bounds__(256) void k_switch_msg0(const uint4 *table, const u
         {
             uint32_t sel[4] = {ea.x, ea.y, ea.z, ea.w};
             uint32_t val[4] = {wa.x, wa.y, wa.z, wa.w};
-            m4rm_step<4>(acc0, sel, val, L);
+            m4w_step<4>(acc0, sel, val, W);
         }
         {
             uint32_t sel[4] = {e1.x, e1.y, e1.z, e1.w};
             uint32_t val[4] = {wa.x ^ wb.x, wa.y ^ wb.y, wa.z ^ wb.z, wa.w ^ wb.w};
-            m4rm_step<4>(acci, sel, val, L);
+            m4w_step<4>(acci, sel, val, W);
         }
     }
-    m4rm_flush<4>(acc0, red, T0);
+    m4w_flush<4, true>(acc0, lds, T0);
     __syncthreads();
-    m4rm_flush<4>(acci, red, Tinf);
+    m4w_flush<4, true>(acci, lds, Tinf);
 }
 
 // Per-block nibble-window table of y -> rho * y in F162: 41 windows x 16 entries x 6 words.
@@ -326,12 +310,11 @@ __device__ __forceinline__ void rho_mul(const uint32_t *tab, const uint32_t (&y)
 template <bool WORDS>
 __global__ __launch_bounds__(256) void k_switch_round(const uint32_t *pin, size_t h2, RhoArg rho, const uint4 *Ein,
                                                       uint4 *Eout, uint32_t *pout, uint32_t *T0, uint32_t *Tinf) {
-    __shared__ uint32_t rbasis[162 * 6];
     __shared__ uint32_t rtab[RHO_WINDOWS * 16 * 6];
-    __shared__ M4Lds<6> L;
-    __shared__ uint32_t red[4 * 128 * 6];
-    build_rho_table(rho, rbasis, rtab);
-    uint32_t acc0[8][6] = {}, acci[8][6] = {};
+    __shared__ uint32_t lds[4 * M4W<6>::WORDS];  // also holds the basis while rtab is built
+    build_rho_table(rho, lds, rtab);
+    uint32_t *W = lds + (threadIdx.x >> 6) * M4W<6>::WORDS;
+    uint32_t acc0[2][6] = {}, acci[2][6] = {};
     const size_t h = 2 * h2;
     const size_t nthreads = (h2 + 255) / 256 * 256;
     for (size_t base = (size_t)blockIdx.x * 256; base < nthreads; base += (size_t)gridDim.x * 256) {
@@ -375,15 +358,15 @@ __global__ __launch_bounds__(256) void k_switch_round(const uint32_t *pin, size_
             e2[0] = ea.x ^ eb.x, e2[1] = ea.y ^ eb.y, e2[2] = ea.z ^ eb.z, e2[3] = ea.w ^ eb.w;
             Eout[j] = make_uint4(e2[0], e2[1], e2[2], e2[3]);
         }
-        m4rm_step<6>(acc0, e, q0, L);
+        m4w_step<6>(acc0, e, q0, W);
         uint32_t dq[6];
 #pragma unroll
         for (int k = 0; k < 6; k++) dq[k] = q0[k] ^ q1[k];
-        m4rm_step<6>(acci, e2, dq, L);
+        m4w_step<6>(acci, e2, dq, W);
     }
-    m4rm_flush<6>(acc0, red, T0);
+    m4w_flush<6, true>(acc0, lds, T0);
     __syncthreads();
-    m4rm_flush<6>(acci, red, Tinf);
+    m4w_flush<6, true>(acci, lds, Tinf);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -403,9 +386,9 @@ __global__ void k_eq_row(const uint64_t *lo, const uint64_t *hi, int lo_bits, si
 }
 
 __global__ __launch_bounds__(256) void k_row(const uint4 *table, const uint32_t *eqrow, size_t clen, uint32_t *U) {
-    __shared__ M4Lds<6> L;
-    __shared__ uint32_t red[4 * 128 * 6];
-    uint32_t acc[8][6] = {};
+    __shared__ uint32_t lds[4 * M4W<6>::WORDS];
+    uint32_t *W = lds + (threadIdx.x >> 6) * M4W<6>::WORDS;
+    uint32_t acc[2][6] = {};
     const size_t c = blockIdx.x;
     for (size_t base = 0; base < clen; base += 256) {
         const size_t i = base + threadIdx.x;
@@ -416,9 +399,9 @@ __global__ __launch_bounds__(256) void k_row(const uint4 *table, const uint32_t 
 #pragma unroll
             for (int k = 0; k < 6; k++) val[k] = eqrow[6 * i + k];
         }
-        m4rm_step<6>(acc, sel, val, L);
+        m4w_step<6>(acc, sel, val, W);
     }
-    m4rm_flush_store<6>(acc, red, U + c * 128 * 6);
+    m4w_flush<6, false>(acc, lds, U + c * 128 * 6);
 }
 
 // row[c] = sum_b x^b U_b: 128 threads per column, one b each, then an XOR reduction.
@@ -607,35 +590,38 @@ __device__ __forceinline__ void radix3(uint32_t &a, uint32_t &b, uint32_t &c, ui
     a = x0, b = x1, c = x2;
 }
 
-// Forward 18-point DFT (decimation in frequency: 2, then 3, 3), output in krev18 order.
-// Lazy reduction: inputs and outputs lie in [0, 2p); the primes satisfy 6p < 2^32, so sums of up
-// to three such values stay in 32 bits, and a Montgomery product with a constant below p accepts
-// any 32-bit operand and returns [0, 2p) — a value that is multiplied next is never corrected.
+// Lazy corrections by unsigned min (x - 2p wraps when x < 2p).
 __device__ __forceinline__ uint32_t red4(uint32_t x, uint32_t p2) { return min(x, x - p2); }  // [0,4p)->[0,2p)
 __device__ __forceinline__ uint32_t red6(uint32_t x, uint32_t p2, uint32_t p4) {             // [0,6p)->[0,2p)
     x = min(x, x - p4);
     return min(x, x - p2);
 }
 
-template <bool TW>
+// RED: outputs corrected into [0, 2p); otherwise x1 and x2 are left in (0, 6p), which suits a
+// Montgomery product as the next step.
+template <bool TW, bool RED = true>
 __device__ __forceinline__ void radix3_lazy(uint32_t &a, uint32_t &b, uint32_t &c, uint32_t w3, uint32_t w1,
                                             uint32_t w2, PC pc, uint32_t p2, uint32_t p4) {
     const uint32_t t = mont_lazy(b - c + p2, w3, pc);
-    const uint32_t x0 = red6(a + b + c, p2, p4);
+    const uint32_t s = a + b + c;        // [0, 6p)
     const uint32_t x1 = a - c + t + p2;  // (0, 6p)
     const uint32_t x2 = a - b - t + p4;  // (0, 6p)
-    a = x0;
+    a = RED ? red6(s, p2, p4) : s;
     if (TW) {
         b = mont_lazy(x1, w1, pc);
         c = mont_lazy(x2, w2, pc);
-    } else {
+    } else if (RED) {
         b = red6(x1, p2, p4);
         c = red6(x2, p2, p4);
+    } else {
+        b = x1;
+        c = x2;
     }
 }
 
-__device__ __forceinline__ void dft18_fwd(uint32_t (&v)[18], const uint32_t (&w)[9], PC pc) {
-    const uint32_t p2 = 2 * pc.p, p4 = 4 * pc.p;
+// First stage of the forward 18-point DFT: radix 2 over (j, j + 9) with twiddles w18^j.
+__device__ __forceinline__ void dft18_stage1(uint32_t (&v)[18], const uint32_t (&w)[9], PC pc) {
+    const uint32_t p2 = 2 * pc.p;
 #pragma unroll
     for (int j = 0; j < 9; j++) {
         const uint32_t a = v[j], b = v[j + 9];
@@ -643,6 +629,13 @@ __device__ __forceinline__ void dft18_fwd(uint32_t (&v)[18], const uint32_t (&w)
         const uint32_t d = a - b + p2;  // (0, 4p)
         v[j + 9] = j ? mont_lazy(d, w[j], pc) : red4(d, p2);
     }
+}
+
+// Remaining stages (3, then 3). RED_OUT = false leaves the outputs in [0, 6p): every consumer
+// that multiplies next (twiddle, MAC, Montgomery conversion) accepts any 32-bit operand.
+template <bool RED_OUT>
+__device__ __forceinline__ void dft18_stage23(uint32_t (&v)[18], const uint32_t (&w)[9], PC pc) {
+    const uint32_t p2 = 2 * pc.p, p4 = 4 * pc.p;
 #pragma unroll
     for (int B = 0; B < 18; B += 9) {
         radix3_lazy<false>(v[B], v[B + 3], v[B + 6], w[6], 0, 0, pc, p2, p4);
@@ -650,7 +643,17 @@ __device__ __forceinline__ void dft18_fwd(uint32_t (&v)[18], const uint32_t (&w)
         radix3_lazy<true>(v[B + 2], v[B + 5], v[B + 8], w[6], w[4], w[8], pc, p2, p4);
     }
 #pragma unroll
-    for (int B = 0; B < 18; B += 3) radix3_lazy<false>(v[B], v[B + 1], v[B + 2], w[6], 0, 0, pc, p2, p4);
+    for (int B = 0; B < 18; B += 3) radix3_lazy<false, RED_OUT>(v[B], v[B + 1], v[B + 2], w[6], 0, 0, pc, p2, p4);
+}
+
+// Forward 18-point DFT (decimation in frequency: 2, then 3, 3), output in krev18 order.
+// Lazy reduction: inputs lie in [0, 2p); the primes satisfy 6p < 2^32, so sums of up to three
+// such values stay in 32 bits, and a Montgomery product with a constant below p accepts any
+// 32-bit operand and returns [0, 2p) — a value that is multiplied next is never corrected.
+template <bool RED_OUT>
+__device__ __forceinline__ void dft18_fwd(uint32_t (&v)[18], const uint32_t (&w)[9], PC pc) {
+    dft18_stage1(v, w, pc);
+    dft18_stage23<RED_OUT>(v, w, pc);
 }
 
 // Exact inverse of dft18_fwd up to the factor 18 (decimation in time, inverse roots).
@@ -675,19 +678,20 @@ __device__ __forceinline__ void dft18_inv(uint32_t (&v)[18], const uint32_t (&w)
     }
 }
 
-// Pass A for one half: thread n2 holds x[18 n1 + n2] in v; afterwards Y[k1][n2] is in LDS.
-__device__ __forceinline__ void passA_store(uint32_t (&v)[18], int n2, const uint32_t *twpf, uint32_t *Y,
-                                            const uint32_t (&w)[9], PC pc) {
-    dft18_fwd(v, w, pc);
+// Pass A for one half, after the DFT stages (outputs in [0, 6p), krev18 order): twist output k1 of
+// thread n2 by rho^(n2 k1) and store Y[k1][n2]; the untwisted k1 = 0 is corrected into [0, 2p).
+__device__ __forceinline__ void passA_twist_store(const uint32_t (&v)[18], int n2, const uint32_t *twpf,
+                                                  uint32_t *Y, PC pc) {
+    const uint32_t p2 = 2 * pc.p, p4 = 4 * pc.p;
 #pragma unroll
     for (int p = 0; p < 18; p++) {
         const int k1 = krev18(p);
-        const uint32_t y = k1 ? mont_lazy(v[p], twpf[n2 * 18 + k1], pc) : v[p];
+        const uint32_t y = k1 ? mont_lazy(v[p], twpf[n2 * 18 + k1], pc) : red6(v[p], p2, p4);
         Y[k1 * 18 + n2] = y;
     }
 }
 
-// Transformed matrix: ahat[m][slot] = T(A_m mod p)[slot] * R mod p (Montgomery form for the MAC).
+// Transformed matrix: ahat[m][slot] = T(A_m mod p)[slot], canonical residues.
 // One element per block; threads 0..35 run the two halves' transforms.
 // The matrix is taken centred (a > Q/2 stands for a - Q, the same residue mod Q), which halves
 // the range of the exact product and so the number of primes.
@@ -725,22 +729,30 @@ __global__ __launch_bounds__(64) void k_ahat(const uint64_t *amat, const uint32_
             const uint32_t u = addm(a[i], mont(a[i + 324], tabs[tab::ETA + h], pc), pc.p);
             v[n1] = mont(u, tabs[tab::TW + h * 324 + i], pc);
         }
-        passA_store(v, idx, tabs + tab::TWPF, Y + h * 324, w, pc);
+        dft18_fwd<false>(v, w, pc);
+        passA_twist_store(v, idx, tabs + tab::TWPF, Y + h * 324, pc);
     }
     __syncthreads();
     if (t < 36) {
         uint32_t v[18];
 #pragma unroll
         for (int n2 = 0; n2 < 18; n2++) v[n2] = Y[h * 324 + idx * 18 + n2];
-        dft18_fwd(v, w, pc);
+        dft18_fwd<false>(v, w, pc);
 #pragma unroll
-        for (int p = 0; p < 18; p++) ahat[m * 648 + h * 324 + idx * 18 + p] = mont(v[p], r2, pc);
+        for (int p = 0; p < 18; p++) {
+            const uint32_t x = red6(v[p], 2 * pc.p, 4 * pc.p);
+            ahat[m * 648 + h * 324 + idx * 18 + p] = min(x, x - pc.p);
+        }
     }
 }
 
 // Forward transform of the binary ring elements of TC columns, MAC against ahat, over the ring
 // elements [m0, m1) of this block; partial sums go into cacc with u64 atomics. 576 threads:
 // thread t handles column t / 36, half (t / 18) % 2, and n2 (pass A) / k1 (pass B) = t % 18.
+// The MAC is a plain 64-bit multiply-add of v < 6p by a < p (each product below 6p^2 < 2^32 p),
+// folded every fourth element as acc = hi * (2^32 mod p) + lo (< 2^32 p): the largest value
+// before a fold, 2^32 p + 4 * 6p^2 < 5 * 2^61.42, fits 64 bits. Two folds at the end leave each
+// block's sum below p^2 + 2^32, so 16 blocks' atomics stay below 2^64 too.
 constexpr int TC = 16;
 constexpr int MAC_THREADS = TC * 36;
 
@@ -758,47 +770,55 @@ __device__ __forceinline__ uint32_t bit_at(const uint32_t (&r)[4]) {
     return (r[(POS >> 5) & 3] >> (POS & 31)) & 1;
 }
 
-// The binary inputs of pass A for thread n2: x[i], i = 18 n1 + n2, from coefficient i (low) and
-// i + 324 (high); coefficient c is bit c >> 2 of word c & 3. As n1 runs, c & 3 alternates between
-// words n2 & 3 (n1 even) and (n2 + 2) & 3 (n1 odd), and after shifting those words right by
-// n2 >> 2 and (n2 + 18) >> 2 the bit of n1 = 2u (+1) sits at 9u (+81 for the high half), a
-// compile-time position. Bits shifted past 127 read as zero: those coefficients are >= 512.
-template <int U>
-__device__ __forceinline__ void gather_pair(const uint32_t (&ra)[4], const uint32_t (&rb)[4], const uint32_t *twb,
-                                            int n2, uint32_t (&v)[18]) {
-    {
-        const int n1 = 2 * U, i = 18 * n1 + n2;
-        const uint32_t sel = bit_at<9 * U>(ra) | (bit_at<9 * U + 81>(ra) << 1);
-        v[n1] = twb[i * 4 + sel];
-    }
-    if (2 * U + 1 < 18) {
-        const int n1 = 2 * U + 1, i = 18 * n1 + n2;
-        const uint32_t sel = bit_at<9 * U>(rb) | (bit_at<9 * U + 81>(rb) << 1);
-        v[n1] = twb[i * 4 + sel];
-    }
+// The 4-bit index of the stage-1 table for pair (j, j + 9) of thread n2: the bits of coefficients
+// 18 n1 + n2 (low half) and + 324 (high half) for n1 = j and j + 9. Coefficient c is bit c >> 2
+// of word c & 3; for even n1 = 2u that is word n2 & 3, which shifted right by n2 >> 2 (ra) puts
+// the bit at 9u (+81 for the high half), and for odd n1 = 2u + 1 word (n2 + 2) & 3 shifted by
+// (n2 + 18) >> 2 (rb), again at 9u. Bits past 127 read as zero: coefficients >= 512 are zero.
+template <int J>
+__device__ __forceinline__ uint32_t stage1_sel(const uint32_t (&ra)[4], const uint32_t (&rb)[4]) {
+    constexpr int U = J / 2;
+    if constexpr ((J & 1) == 0)  // n1 = 2U (ra) and n1 = J + 9 = 2 (U + 4) + 1 (rb)
+        return bit_at<9 * U>(ra) | (bit_at<9 * U + 81>(ra) << 1) | (bit_at<9 * U + 36>(rb) << 2) |
+               (bit_at<9 * U + 117>(rb) << 3);
+    else  // n1 = 2U + 1 (rb) and n1 = J + 9 = 2 (U + 5) (ra)
+        return bit_at<9 * U>(rb) | (bit_at<9 * U + 81>(rb) << 1) | (bit_at<9 * U + 45>(ra) << 2) |
+               (bit_at<9 * U + 126>(ra) << 3);
+}
+
+template <int J>
+__device__ __forceinline__ void stage1_lookup(const uint32_t (&ra)[4], const uint32_t (&rb)[4], const uint32_t *st1,
+                                              uint32_t (&v)[18]) {
+    const uint32_t s = stage1_sel<J>(ra, rb);
+    v[J] = st1[((J * 2 + 0) * 16 + s) * 36];
+    v[J + 9] = st1[((J * 2 + 1) * 16 + s) * 36];
 }
 
 __global__ __launch_bounds__(MAC_THREADS) void k_commit_mac(const uint32_t *table32, size_t clen, int m_per, int re,
                                                             const uint32_t *ahat, const uint32_t *tabs, PC pc,
-                                                            Roots18 R, unsigned long long *cacc) {
+                                                            uint32_t c32, Roots18 R, unsigned long long *cacc) {
     __shared__ uint32_t Y[TC * 2 * 324];
-    __shared__ uint32_t s_twb[2 * 324 * 4];
+    __shared__ uint32_t s_st1[tab::ST1_SIZE];
     __shared__ uint32_t s_twp[324];
     __shared__ uint4 s_bits[TC * 4];
     const int t = threadIdx.x;
     const int col = t / 36, h = (t / 18) & 1, idx = t % 18;
     const int c0 = blockIdx.x * TC;
     const int m0 = blockIdx.y * m_per, m1 = min(re, m0 + m_per);
-    for (int i = t; i < 2 * 324 * 4; i += MAC_THREADS) s_twb[i] = tabs[tab::TWB + i];
+    for (int i = t; i < tab::ST1_SIZE; i += MAC_THREADS) s_st1[i] = tabs[tab::ST1 + i];
     for (int i = t; i < 324; i += MAC_THREADS) s_twp[i] = tabs[tab::TWPF + i];
     uint32_t w[9];
 #pragma unroll
     for (int j = 0; j < 9; j++) w[j] = R.f[j];
-    unsigned long long acc[18];
+    uint64_t acc[18];
 #pragma unroll
     for (int p = 0; p < 18; p++) acc[p] = 0;
+    auto fold = [&]() {
+#pragma unroll
+        for (int p = 0; p < 18; p++) acc[p] = (uint64_t)(uint32_t)(acc[p] >> 32) * c32 + (uint32_t)acc[p];
+    };
     const uint4 *bits = s_bits + col * 4;
-    const uint32_t *twb = s_twb + h * 324 * 4;
+    const uint32_t *st1 = s_st1 + h * 18 + idx;
     uint32_t *Yh = Y + (col * 2 + h) * 324;
     const int wa = idx & 3, wb = (idx + 2) & 3;
     const uint32_t sa = idx >> 2, sb = (idx + 18) >> 2;
@@ -810,19 +830,21 @@ __global__ __launch_bounds__(MAC_THREADS) void k_commit_mac(const uint32_t *tabl
         }
         __syncthreads();
         {
+            // pass A: stage 1 by lookup, stages 2 and 3, twist, transpose through LDS
             uint32_t v[18], ra[4], rb[4];
             shr128(bits[wa], sa, ra);
             shr128(bits[wb], sb, rb);
-            gather_pair<0>(ra, rb, twb, idx, v);
-            gather_pair<1>(ra, rb, twb, idx, v);
-            gather_pair<2>(ra, rb, twb, idx, v);
-            gather_pair<3>(ra, rb, twb, idx, v);
-            gather_pair<4>(ra, rb, twb, idx, v);
-            gather_pair<5>(ra, rb, twb, idx, v);
-            gather_pair<6>(ra, rb, twb, idx, v);
-            gather_pair<7>(ra, rb, twb, idx, v);
-            gather_pair<8>(ra, rb, twb, idx, v);
-            passA_store(v, idx, s_twp, Yh, w, pc);
+            stage1_lookup<0>(ra, rb, st1, v);
+            stage1_lookup<1>(ra, rb, st1, v);
+            stage1_lookup<2>(ra, rb, st1, v);
+            stage1_lookup<3>(ra, rb, st1, v);
+            stage1_lookup<4>(ra, rb, st1, v);
+            stage1_lookup<5>(ra, rb, st1, v);
+            stage1_lookup<6>(ra, rb, st1, v);
+            stage1_lookup<7>(ra, rb, st1, v);
+            stage1_lookup<8>(ra, rb, st1, v);
+            dft18_stage23<false>(v, w, pc);
+            passA_twist_store(v, idx, s_twp, Yh, pc);
         }
         __syncthreads();
         {
@@ -830,19 +852,22 @@ __global__ __launch_bounds__(MAC_THREADS) void k_commit_mac(const uint32_t *tabl
             uint32_t v[18];
 #pragma unroll
             for (int n2 = 0; n2 < 18; n2++) v[n2] = Yh[idx * 18 + n2];
-            dft18_fwd(v, w, pc);
+            dft18_fwd<false>(v, w, pc);
             const uint2 *am = (const uint2 *)(ahat + (size_t)m * 648 + h * 324 + idx * 18);
 #pragma unroll
             for (int q = 0; q < 9; q++) {
                 const uint2 a2 = am[q];
-                acc[2 * q] += mont_lazy(v[2 * q], a2.x, pc);
-                acc[2 * q + 1] += mont_lazy(v[2 * q + 1], a2.y, pc);
+                acc[2 * q] += (uint64_t)v[2 * q] * a2.x;
+                acc[2 * q + 1] += (uint64_t)v[2 * q + 1] * a2.y;
             }
+            if (((m - m0) & 3) == 3) fold();
         }
     }
+    fold();
+    fold();
     unsigned long long *dst = cacc + (size_t)(c0 + col) * 648 + h * 324 + idx * 18;
 #pragma unroll
-    for (int p = 0; p < 18; p++) atomicAdd(dst + p, acc[p]);
+    for (int p = 0; p < 18; p++) atomicAdd(dst + p, (unsigned long long)acc[p]);
 }
 
 // Inverse transform per column and prime, Garner over the primes, reduction mod Q, packing.
@@ -1176,7 +1201,9 @@ extern "C" int lbn_commit(lbn_ctx *ctx, uint8_t *out, size_t out_len) {
     const int np = ctx->nprimes;
     fill_u32(d->stream, d->cacc, (size_t)np * COLUMNS * N * 2);
     const int re = (int)s.ring_elements;
-    int msplit = std::max(1, std::min(re, 1024 / (COLUMNS / TC)));  // ~1024 blocks per prime
+    // ~1024 blocks per prime; at most 16 blocks add into one sum (the MAC's 64-bit bound)
+    int msplit = std::max(1, std::min(re, 1024 / (COLUMNS / TC)));
+    static_assert(1024 / (COLUMNS / TC) <= 16, "more than 16 partial sums per cacc entry");
     int m_per = (re + msplit - 1) / msplit;
     msplit = (re + m_per - 1) / m_per;
     tstart(ctx, T_MAC);
@@ -1185,7 +1212,8 @@ extern "C" int lbn_commit(lbn_ctx *ctx, uint8_t *out, size_t out_len) {
         dim3 grid(COLUMNS / TC, msplit);
         k_commit_mac<<<grid, MAC_THREADS, 0, d->stream>>>((const uint32_t *)d->table, s.column_len, m_per, re,
                                                            d->ahat + (size_t)k * re * N,
-                                                           d->ntt_tabs + (size_t)k * tab::SIZE, pc, d->roots[k],
+                                                           d->ntt_tabs + (size_t)k * tab::SIZE, pc,
+                                                           (uint32_t)((1ull << 32) % pc.p), d->roots[k],
                                                            d->cacc + (size_t)k * COLUMNS * N);
         HIPCHK(ctx, hipGetLastError());
     }

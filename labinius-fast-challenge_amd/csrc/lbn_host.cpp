@@ -310,10 +310,7 @@ bool prime_tables(const PrimeInfo &pi, std::vector<uint32_t> &t) {
     const uint32_t eta[2] = {pw(w, 324, p), pw(w, 1620, p)};
     const uint32_t beta[2] = {w, pw(w, 5, p)};
     const uint32_t rho = pw(w, 6, p), w18 = pw(rho, 18, p);
-    // layout must match lbn_device.hip namespace tab
-    const int TWB = 0, TW = TWB + 2 * 324 * 4, ETA = TW + 2 * 324, UNTW = ETA + 2, ETA0 = UNTW + 2 * 324,
-              DINV = ETA0 + 1, TWPF = DINV + 1, TWPI = TWPF + 324, R18F = TWPI + 324, R18I = R18F + 9,
-              SIZE = R18I + 9;
+    using namespace tab;
     t.assign(SIZE, 0);
     const uint32_t i324 = inv(324);
     for (int h = 0; h < 2; h++)
@@ -340,6 +337,18 @@ bool prime_tables(const PrimeInfo &pi, std::vector<uint32_t> &t) {
     for (int j = 0; j < 9; j++) {
         t[R18F + j] = mf(pw(w18, j, p));
         t[R18I + j] = mf(pw(w18i, j, p));
+    }
+    for (int j = 0; j < 9; j++) {
+        const uint64_t wj = pw(w18, j, p);
+        for (int sel = 0; sel < 16; sel++)
+            for (int h = 0; h < 2; h++)
+                for (int n2 = 0; n2 < 18; n2++) {
+                    const uint64_t x0 = t[TWB + (h * 324 + 18 * j + n2) * 4 + (sel & 3)];
+                    const uint64_t x1 = t[TWB + (h * 324 + 18 * (j + 9) + n2) * 4 + (sel >> 2)];
+                    const int col = h * 18 + n2;
+                    t[ST1 + ((j * 2 + 0) * 16 + sel) * 36 + col] = (uint32_t)((x0 + x1) % p);
+                    t[ST1 + ((j * 2 + 1) * 16 + sel) * 36 + col] = (uint32_t)((x0 + p - x1) % p * wj % p);
+                }
     }
     return true;
 }

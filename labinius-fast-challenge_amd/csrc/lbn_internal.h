@@ -50,6 +50,27 @@ struct PrimeInfo {
     uint32_t omega;            // primitive 1944-th root of unity
 };
 
+// Layout of one prime's commitment-transform constants (u32 offsets), built by prime_tables().
+namespace tab {
+constexpr int TWB = 0;                 // [2][324][4] binary split+twist values: 0, b, eta b, (1+eta) b
+constexpr int TW = TWB + 2 * 324 * 4;  // [2][324] beta_h^i (Montgomery)
+constexpr int ETA = TW + 2 * 324;      // [2] eta_h (Montgomery)
+constexpr int UNTW = ETA + 2;          // [2][324] beta_h^-i / 324 (Montgomery)
+constexpr int ETA0 = UNTW + 2 * 324;   // eta_0 (Montgomery)
+constexpr int DINV = ETA0 + 1;         // 1 / (eta_0 - eta_1) (Montgomery)
+constexpr int TWPF = DINV + 1;         // [18][18] rho^(n2 k1) (Montgomery), rho = w^6
+constexpr int TWPI = TWPF + 324;       // [18][18] rho^-(n2 k1)
+constexpr int R18F = TWPI + 324;       // [9] w18^j, w18 = rho^18 (Montgomery)
+constexpr int R18I = R18F + 9;         // [9] w18^-j
+// [9 j][2 o][16 s][36 = 18 h + n2]: the first radix-2 stage of the 18-point DFT over n1 of a
+// binary ring element, x[n1] = TWB[h][18 n1 + n2][2 bits]: o = 0 gives x[j] + x[j+9], o = 1 gives
+// (x[j] - x[j+9]) w18^j, for the 4 bits s = (bits of n1 = j) | (bits of n1 = j + 9) << 2. Plain
+// residues in [0, p).
+constexpr int ST1 = R18I + 9;
+constexpr int ST1_SIZE = 9 * 2 * 16 * 36;
+constexpr int SIZE = ST1 + ST1_SIZE;
+}  // namespace tab
+
 // Device-side state (owned by lbn_ctx; defined in lbn_device.hip).
 struct DeviceState;
 

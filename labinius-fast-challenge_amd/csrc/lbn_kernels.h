@@ -4,7 +4,7 @@
 // little-endian u32 words; F162 = GF(2)[x]/(x^162 + x^81 + 1), held as six u32 words (bits
 // 162..191 zero). CDNA4 has no carry-less multiply, so every product with a fixed operand is a
 // GF(2)-linear map evaluated through windowed tables in LDS, and sums of selected values use the
-// method of four Russians (M4RM) over 16-lane groups.
+// method of four Russians (M4RM) over half-wave groups.
 //
 // Ring arithmetic: 31-bit primes p = 1 (mod 1944) in Montgomery form (R = 2^32).
 #pragma once
@@ -79,126 +79,148 @@ __device__ inline void f162_mul_slow(const uint64_t a[3], const uint64_t b[3], u
 }
 
 // ------------------------------------------------------------------------------------------
-// M4RM: acc[k] ^= sum over a block step of bit_k(sel_j) * val_j, for k < 128.
+// M4RM: acc ^= sum over one step of bit_k(sel_j) * val_j, for k < 128, wave-local.
 //
-// 256 threads, one element per thread per step. Within each group of 16 lanes the 16 x 128
-// selection matrix is bit-transposed with four xor-shuffle stages, so that lane r holds, for each
-// 16-bit chunk c, the bits of k = 16c + r over the group's 16 rows. The rows' values are folded
-// into four 16-entry tables (rows 4t..4t+3), and each k costs four LDS lookups. Lane r keeps the
-// accumulators of its eight k values: acc[c] <-> k = 16c + r.
+// A wave (64 lanes, one row each per step) is one group. Its 64 x 128 selection matrix is
+// bit-transposed in registers, as two 64 x 64 blocks (words 2c, 2c+1 of every lane form block c):
+// one exchange across the half-waves (ds_bpermute), then five stages within them (DPP for
+// distances 1, 2 and 8, ds_swizzle for 4 and 16). Afterwards lane r holds in words 2c, 2c+1 the
+// bits of k = 64c + r over rows 0..31 and 32..63. The rows' values are folded into sixteen
+// 16-entry tables (rows 4t..4t+3; the four lanes of a quad build one table from their quad's
+// values, broadcast by DPP), and each k costs sixteen LDS lookups. No block barrier: the wave
+// orders its own LDS traffic (__syncwarp), so the waves of a block run independently.
+//
+// Table layout per wave (words): plane A = [16 tables][16 slots] x uint4 (value words 0..3), then
+// for V = 6 plane B = [16][16] x uint2 (words 4..5). Entry e of table t sits in slot e ^ t, which
+// spreads a quad's simultaneous writes over the LDS banks while the lookups into one table (all
+// lanes read table n together) stay on distinct banks.
+// acc[c][v] <-> k = 64c + lane.
 // ------------------------------------------------------------------------------------------
 
-constexpr int M4_THREADS = 256;
-
 template <int V>
-struct M4Lds {
-    uint32_t val[V * M4_THREADS];            // staged values, value-major
-    uint32_t tab[(M4_THREADS / 16) * 64 * V];  // per group: 4 tables x 16 entries x V
+struct M4W {
+    static_assert(V == 4 || V == 6, "M4RM values are 4 or 6 words");
+    static constexpr int WORDS = 256 * V;  // LDS words per wave
 };
 
-__device__ __forceinline__ void transpose16(uint32_t s[4], int r) {
-    const uint32_t masks[4] = {0xFF00FF00u, 0xF0F0F0F0u, 0xCCCCCCCCu, 0xAAAAAAAAu};
-    const int ds[4] = {8, 4, 2, 1};
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp(uint32_t x) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, CTRL, 0xF, 0xF, false);
+}
+template <int PATTERN>
+__device__ __forceinline__ uint32_t swz(uint32_t x) {
+    return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, PATTERN);
+}
+// Value of lane (lane ^ D), D < 32.
+template <int D>
+__device__ __forceinline__ uint32_t lane_xor(uint32_t x) {
+    if constexpr (D == 1) return dpp<0xB1>(x);      // quad_perm [1,0,3,2]
+    else if constexpr (D == 2) return dpp<0x4E>(x);  // quad_perm [2,3,0,1]
+    else if constexpr (D == 8) return dpp<0x128>(x); // row_ror:8 within a row of 16
+    else return swz<0x1F | (D << 10)>(x);            // bit mode: and 0x1F, xor D
+}
+
+// One exchange stage, branch-free: the lane with bit D of r clear keeps its bits outside M and
+// takes the partner's word rotated left by D into M; its partner keeps M and takes the word
+// rotated right by D into the complement. Both are one lane-dependent rotate and one bit select.
+template <int D, uint32_t M>
+__device__ __forceinline__ void transpose_stage(uint32_t (&s)[4], int r) {
+    const bool low = (r & D) == 0;
+    const uint32_t keep = low ? ~M : M;
+    const uint32_t rot = low ? 32u - D : (uint32_t)D;  // rotate right by rot
 #pragma unroll
-    for (int st = 0; st < 4; st++) {
-        const int d = ds[st];
-        const uint32_t M = masks[st];
-        const bool low = (r & d) == 0;
-#pragma unroll
-        for (int w = 0; w < 4; w++) {
-            uint32_t partner = (uint32_t)__shfl_xor((int)s[w], d);
-            s[w] = low ? ((s[w] & ~M) | ((partner << d) & M)) : ((s[w] & M) | ((partner >> d) & ~M));
-        }
+    for (int w = 0; w < 4; w++) {
+        const uint32_t p = lane_xor<D>(s[w]);
+        const uint32_t t = __builtin_amdgcn_alignbit(p, p, rot);
+        s[w] = (s[w] & keep) | (t & ~keep);
     }
 }
 
-template <int V>
-__device__ __forceinline__ void m4rm_step(uint32_t (&acc)[8][V], const uint32_t (&sel)[4],
-                                          const uint32_t (&val)[V], M4Lds<V> &L) {
-    const int tid = threadIdx.x, r = tid & 15, grp = tid >> 4;
+// 64 x 64 bit transpose of each block (s[2c], s[2c+1]) across the wave: afterwards bit i of the
+// block in lane r (bit i of s[2c] for i < 32, bit i - 32 of s[2c+1] otherwise) is bit r of the
+// block in lane i.
+__device__ __forceinline__ void transpose64(uint32_t (&s)[4], int lane) {
+    const bool lo = lane < 32;
 #pragma unroll
-    for (int v = 0; v < V; v++) L.val[v * M4_THREADS + tid] = val[v];
+    for (int c = 0; c < 2; c++) {  // distance 32: swap the off-diagonal 32 x 32 blocks
+        const uint32_t send = lo ? s[2 * c + 1] : s[2 * c];
+        const uint32_t got = (uint32_t)__shfl_xor((int)send, 32);
+        s[2 * c + 1] = lo ? got : s[2 * c + 1];
+        s[2 * c] = lo ? s[2 * c] : got;
+    }
+    transpose_stage<16, 0xFFFF0000u>(s, lane);
+    transpose_stage<8, 0xFF00FF00u>(s, lane);
+    transpose_stage<4, 0xF0F0F0F0u>(s, lane);
+    transpose_stage<2, 0xCCCCCCCCu>(s, lane);
+    transpose_stage<1, 0xAAAAAAAAu>(s, lane);
+}
+
+template <int V>
+__device__ __forceinline__ void m4w_step(uint32_t (&acc)[2][V], const uint32_t (&sel)[4], const uint32_t (&val)[V],
+                                         uint32_t *W) {
+    const int lane = threadIdx.x & 63;
     uint32_t s[4] = {sel[0], sel[1], sel[2], sel[3]};
-    transpose16(s, r);
-    __syncthreads();
+    transpose64(s, lane);
+    uint4 *A = (uint4 *)W;
+    uint2 *B = (uint2 *)(W + 1024);
     {
-        const int t = r >> 2, mh = r & 3, row0 = grp * 16 + 4 * t;
-        uint32_t *T = L.tab + (grp * 64 + t * 16 + 4 * mh) * V;
+        const int t = lane >> 2, mh = lane & 3;
+        uint32_t e[4][V];
 #pragma unroll
         for (int v = 0; v < V; v++) {
-            const uint32_t *col = L.val + v * M4_THREADS;
-            uint32_t base = ((mh & 1) ? col[row0 + 2] : 0u) ^ ((mh & 2) ? col[row0 + 3] : 0u);
-            uint32_t v0 = col[row0], v1 = col[row0 + 1];
-            T[v] = base;
-            T[V + v] = base ^ v0;
-            T[2 * V + v] = base ^ v1;
-            T[3 * V + v] = base ^ v0 ^ v1;
+            const uint32_t v0 = dpp<0x00>(val[v]), v1 = dpp<0x55>(val[v]);
+            const uint32_t v2 = dpp<0xAA>(val[v]), v3 = dpp<0xFF>(val[v]);
+            const uint32_t base = ((mh & 1) ? v2 : 0u) ^ ((mh & 2) ? v3 : 0u);
+            e[0][v] = base;
+            e[1][v] = base ^ v0;
+            e[2][v] = base ^ v1;
+            e[3][v] = base ^ v0 ^ v1;
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int slot = t * 16 + ((4 * mh + i) ^ t);
+            A[slot] = make_uint4(e[i][0], e[i][1], e[i][2], e[i][3]);
+            if constexpr (V == 6) B[slot] = make_uint2(e[i][4], e[i][5]);
         }
     }
-    __syncthreads();
-    const uint32_t *G = L.tab + grp * 64 * V;
+    __syncwarp();
 #pragma unroll
-    for (int c = 0; c < 8; c++) {
-        const uint32_t chunk = (s[c >> 1] >> (16 * (c & 1))) & 0xFFFFu;
-        const uint32_t *a = G + (0 * 16 + (chunk & 15)) * V;
-        const uint32_t *b = G + (1 * 16 + ((chunk >> 4) & 15)) * V;
-        const uint32_t *cc = G + (2 * 16 + ((chunk >> 8) & 15)) * V;
-        const uint32_t *d = G + (3 * 16 + (chunk >> 12)) * V;
+    for (int c = 0; c < 2; c++)
 #pragma unroll
-        for (int v = 0; v < V; v++) acc[c][v] ^= a[v] ^ b[v] ^ cc[v] ^ d[v];
-    }
-    __syncthreads();
+        for (int n = 0; n < 16; n++) {
+            const uint32_t word = s[2 * c + (n >> 3)];
+            const int slot = n * 16 + (((word >> (4 * (n & 7))) & 15) ^ n);
+            const uint4 a = A[slot];
+            acc[c][0] ^= a.x, acc[c][1] ^= a.y, acc[c][2] ^= a.z, acc[c][3] ^= a.w;
+            if constexpr (V == 6) {
+                const uint2 b = B[slot];
+                acc[c][4] ^= b.x, acc[c][5] ^= b.y;
+            }
+        }
+    __syncwarp();
 }
 
-// Block-reduce the accumulators and XOR them into out[k * V + v] (global, atomic).
-// `red` needs 4 * 128 * V words of LDS (it may alias M4Lds storage after the last step).
-template <int V>
-__device__ __forceinline__ void m4rm_flush(uint32_t (&acc)[8][V], uint32_t *red, uint32_t *out) {
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+// Block flush: every wave leaves its 128 x V sums (k-major) in its own LDS region
+// `lds + wave * M4W<V>::WORDS`; after a block barrier they are XORed together and ATOMIC adds the
+// result into out with atomicXor, otherwise the block owns out and stores it.
+template <int V, bool ATOMIC>
+__device__ __forceinline__ void m4w_flush(const uint32_t (&acc)[2][V], uint32_t *lds, uint32_t *out) {
+    const int lane = threadIdx.x & 63, waves = blockDim.x >> 6;
+    uint32_t *W = lds + (threadIdx.x >> 6) * M4W<V>::WORDS;
 #pragma unroll
-    for (int c = 0; c < 8; c++)
+    for (int c = 0; c < 2; c++)
 #pragma unroll
-        for (int v = 0; v < V; v++) {
-            uint32_t x = acc[c][v];
-            x ^= (uint32_t)__shfl_xor((int)x, 16);
-            x ^= (uint32_t)__shfl_xor((int)x, 32);
-            acc[c][v] = x;
+        for (int v = 0; v < V; v++) W[(64 * c + lane) * V + v] = acc[c][v];
+    __syncthreads();
+    for (int i = threadIdx.x; i < 128 * V; i += blockDim.x) {
+        uint32_t x = 0;
+        for (int w = 0; w < waves; w++) x ^= lds[w * M4W<V>::WORDS + i];
+        if (ATOMIC) {
+            if (x) atomicXor(out + i, x);
+        } else {
+            out[i] = x;
         }
-    __syncthreads();
-    if (lane < 16)
-#pragma unroll
-        for (int c = 0; c < 8; c++)
-#pragma unroll
-            for (int v = 0; v < V; v++) red[(wave * 128 + 16 * c + lane) * V + v] = acc[c][v];
-    __syncthreads();
-    for (int i = tid; i < 128 * V; i += M4_THREADS) {
-        uint32_t x = red[i] ^ red[128 * V + i] ^ red[256 * V + i] ^ red[384 * V + i];
-        if (x) atomicXor(out + i, x);
     }
-}
-
-// Same reduction, but the block owns `out` exclusively (plain stores).
-template <int V>
-__device__ __forceinline__ void m4rm_flush_store(uint32_t (&acc)[8][V], uint32_t *red, uint32_t *out) {
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-#pragma unroll
-    for (int c = 0; c < 8; c++)
-#pragma unroll
-        for (int v = 0; v < V; v++) {
-            uint32_t x = acc[c][v];
-            x ^= (uint32_t)__shfl_xor((int)x, 16);
-            x ^= (uint32_t)__shfl_xor((int)x, 32);
-            acc[c][v] = x;
-        }
-    __syncthreads();
-    if (lane < 16)
-#pragma unroll
-        for (int c = 0; c < 8; c++)
-#pragma unroll
-            for (int v = 0; v < V; v++) red[(wave * 128 + 16 * c + lane) * V + v] = acc[c][v];
-    __syncthreads();
-    for (int i = tid; i < 128 * V; i += M4_THREADS)
-        out[i] = red[i] ^ red[128 * V + i] ^ red[256 * V + i] ^ red[384 * V + i];
 }
 
 // ------------------------------------------------------------------------------------------

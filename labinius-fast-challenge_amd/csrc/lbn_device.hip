@@ -786,27 +786,28 @@ __device__ __forceinline__ uint32_t stage1_sel(const uint32_t (&ra)[4], const ui
                (bit_at<9 * U + 126>(ra) << 3);
 }
 
+// st1 points at the half's table: entry (j, o, s) at ((j * 2 + o) * 2) * 16 + s.
 template <int J>
 __device__ __forceinline__ void stage1_lookup(const uint32_t (&ra)[4], const uint32_t (&rb)[4], const uint32_t *st1,
                                               uint32_t (&v)[18]) {
     const uint32_t s = stage1_sel<J>(ra, rb);
-    v[J] = st1[((J * 2 + 0) * 16 + s) * 36];
-    v[J + 9] = st1[((J * 2 + 1) * 16 + s) * 36];
+    v[J] = st1[(J * 2 + 0) * 32 + s];
+    v[J + 9] = st1[(J * 2 + 1) * 32 + s];
 }
 
-__global__ __launch_bounds__(MAC_THREADS) void k_commit_mac(const uint32_t *table32, size_t clen, int m_per, int re,
+__global__ __launch_bounds__(MAC_THREADS, 5) void k_commit_mac(const uint32_t *table32, size_t clen, int m_per, int re,
                                                             const uint32_t *ahat, const uint32_t *tabs, PC pc,
                                                             uint32_t c32, Roots18 R, unsigned long long *cacc) {
     __shared__ uint32_t Y[TC * 2 * 324];
     __shared__ uint32_t s_st1[tab::ST1_SIZE];
-    __shared__ uint32_t s_twp[324];
+    __shared__ uint32_t s_twp[2 * 324];
     __shared__ uint4 s_bits[TC * 4];
     const int t = threadIdx.x;
     const int col = t / 36, h = (t / 18) & 1, idx = t % 18;
     const int c0 = blockIdx.x * TC;
     const int m0 = blockIdx.y * m_per, m1 = min(re, m0 + m_per);
     for (int i = t; i < tab::ST1_SIZE; i += MAC_THREADS) s_st1[i] = tabs[tab::ST1 + i];
-    for (int i = t; i < 324; i += MAC_THREADS) s_twp[i] = tabs[tab::TWPF + i];
+    for (int i = t; i < 2 * 324; i += MAC_THREADS) s_twp[i] = tabs[tab::TWPB + i];
     uint32_t w[9];
 #pragma unroll
     for (int j = 0; j < 9; j++) w[j] = R.f[j];
@@ -817,8 +818,8 @@ __global__ __launch_bounds__(MAC_THREADS) void k_commit_mac(const uint32_t *tabl
 #pragma unroll
         for (int p = 0; p < 18; p++) acc[p] = (uint64_t)(uint32_t)(acc[p] >> 32) * c32 + (uint32_t)acc[p];
     };
-    const uint4 *bits = s_bits + col * 4;
-    const uint32_t *st1 = s_st1 + h * 18 + idx;
+    const uint32_t *st1 = s_st1 + h * 16;
+    const uint32_t *twp = s_twp + (h * 18 + idx) * 18;
     uint32_t *Yh = Y + (col * 2 + h) * 324;
     const int wa = idx & 3, wb = (idx + 2) & 3;
     const uint32_t sa = idx >> 2, sb = (idx + 18) >> 2;
@@ -829,8 +830,10 @@ __global__ __launch_bounds__(MAC_THREADS) void k_commit_mac(const uint32_t *tabl
             s_bits[t] = tab4[(size_t)(c0 + bc) * clen + 4 * (size_t)m + wd];
         }
         __syncthreads();
+        const uint4 *bits = s_bits + col * 4;
         {
-            // pass A: stage 1 by lookup, stages 2 and 3, twist, transpose through LDS
+            // pass A: stage 1 by lookup, stages 2 and 3, twist by beta^n2 rho^(n2 k1), transpose
+            // through LDS
             uint32_t v[18], ra[4], rb[4];
             shr128(bits[wa], sa, ra);
             shr128(bits[wb], sb, rb);
@@ -844,7 +847,11 @@ __global__ __launch_bounds__(MAC_THREADS) void k_commit_mac(const uint32_t *tabl
             stage1_lookup<7>(ra, rb, st1, v);
             stage1_lookup<8>(ra, rb, st1, v);
             dft18_stage23<false>(v, w, pc);
-            passA_twist_store(v, idx, s_twp, Yh, pc);
+#pragma unroll
+            for (int p = 0; p < 18; p++) {
+                const int k1 = krev18(p);
+                Yh[k1 * 18 + idx] = mont_lazy(v[p], twp[k1], pc);
+            }
         }
         __syncthreads();
         {

@@ -338,17 +338,21 @@ bool prime_tables(const PrimeInfo &pi, std::vector<uint32_t> &t) {
         t[R18F + j] = mf(pw(w18, j, p));
         t[R18I + j] = mf(pw(w18i, j, p));
     }
-    for (int j = 0; j < 9; j++) {
-        const uint64_t wj = pw(w18, j, p);
-        for (int sel = 0; sel < 16; sel++)
-            for (int h = 0; h < 2; h++)
-                for (int n2 = 0; n2 < 18; n2++) {
-                    const uint64_t x0 = t[TWB + (h * 324 + 18 * j + n2) * 4 + (sel & 3)];
-                    const uint64_t x1 = t[TWB + (h * 324 + 18 * (j + 9) + n2) * 4 + (sel >> 2)];
-                    const int col = h * 18 + n2;
-                    t[ST1 + ((j * 2 + 0) * 16 + sel) * 36 + col] = (uint32_t)((x0 + x1) % p);
-                    t[ST1 + ((j * 2 + 1) * 16 + sel) * 36 + col] = (uint32_t)((x0 + p - x1) % p * wj % p);
-                }
+    for (int h = 0; h < 2; h++) {
+        const uint64_t c[4] = {0, 1, eta[h], (1 + (uint64_t)eta[h]) % p};
+        for (int j = 0; j < 9; j++) {
+            const uint64_t wj = pw(w18, j, p);
+            const uint64_t b0 = pw(beta[h], 18 * j, p), b1 = pw(beta[h], 18 * (j + 9), p);
+            for (int sel = 0; sel < 16; sel++) {
+                const uint64_t x0 = c[sel & 3] * b0 % p, x1 = c[sel >> 2] * b1 % p;
+                t[ST1 + ((j * 2 + 0) * 2 + h) * 16 + sel] = (uint32_t)((x0 + x1) % p);
+                t[ST1 + ((j * 2 + 1) * 2 + h) * 16 + sel] = (uint32_t)((x0 + p - x1) % p * wj % p);
+            }
+        }
+        for (int n2 = 0; n2 < 18; n2++)
+            for (int k1 = 0; k1 < 18; k1++)
+                t[TWPB + (h * 18 + n2) * 18 + k1] =
+                    mf((uint32_t)((uint64_t)pw(beta[h], n2, p) * pw(rho, (uint64_t)n2 * k1, p) % p));
     }
     return true;
 }

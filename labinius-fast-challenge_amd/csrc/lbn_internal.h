@@ -62,13 +62,17 @@ constexpr int TWPF = DINV + 1;         // [18][18] rho^(n2 k1) (Montgomery), rho
 constexpr int TWPI = TWPF + 324;       // [18][18] rho^-(n2 k1)
 constexpr int R18F = TWPI + 324;       // [9] w18^j, w18 = rho^18 (Montgomery)
 constexpr int R18I = R18F + 9;         // [9] w18^-j
-// [9 j][2 o][16 s][36 = 18 h + n2]: the first radix-2 stage of the 18-point DFT over n1 of a
-// binary ring element, x[n1] = TWB[h][18 n1 + n2][2 bits]: o = 0 gives x[j] + x[j+9], o = 1 gives
-// (x[j] - x[j+9]) w18^j, for the 4 bits s = (bits of n1 = j) | (bits of n1 = j + 9) << 2. Plain
-// residues in [0, p).
+// The commitment's pass A over a binary ring element: x[n1] = TWB[h][18 n1 + n2][2 bits]
+// = c(bits) beta_h^(18 n1) * beta_h^n2, so the DFT over n1 is that of x'[n1] = c(bits)
+// beta_h^(18 n1), the same for every n2, scaled by beta_h^n2.
+// ST1 [9 j][2 o][2 h][16 s]: its first radix-2 stage, o = 0 giving x'[j] + x'[j+9] and o = 1
+// giving (x'[j] - x'[j+9]) w18^j, for the 4 bits s = (bits of n1 = j) | (bits of n1 = j+9) << 2.
+// Plain residues in [0, p).
+// TWPB [2 h][18 n2][18 k1]: beta_h^n2 rho^(n2 k1) (Montgomery), the pass-A twist with that scale.
 constexpr int ST1 = R18I + 9;
-constexpr int ST1_SIZE = 9 * 2 * 16 * 36;
-constexpr int SIZE = ST1 + ST1_SIZE;
+constexpr int ST1_SIZE = 9 * 2 * 2 * 16;
+constexpr int TWPB = ST1 + ST1_SIZE;
+constexpr int SIZE = TWPB + 2 * 324;
 }  // namespace tab
 
 // Device-side state (owned by lbn_ctx; defined in lbn_device.hip).

@@ -2,7 +2,6 @@
 #include "lbn_blake3.h"
 
 #include <algorithm>
-#include <thread>
 #include <vector>
 
 namespace lbn {
@@ -143,36 +142,24 @@ void Blake3::update(const void *data, size_t len) {
     }
 }
 
-// A whole chunk's chaining value at chunk index `index` (not the root).
-static void chunk_cv_of(const uint32_t key[8], uint32_t flags, const uint8_t *chunk, uint64_t index, uint32_t out[8]) {
-    uint32_t cv[8], m[16], o[16];
-    memcpy(cv, key, 32);
-    for (int b = 0; b < 16; b++) {
-        for (int i = 0; i < 16; i++) m[i] = rd32(chunk + 64 * b + 4 * i);
-        uint32_t f = flags | (b == 0 ? Blake3::CHUNK_START : 0) | (b == 15 ? Blake3::CHUNK_END : 0);
-        Blake3::compress(cv, m, index, 64, f, o);
-        memcpy(cv, o, 32);
-    }
-    memcpy(out, cv, 32);
-}
-
-void Blake3::update_parallel(const void *data, size_t len, int threads) {
-    const uint8_t *p = (const uint8_t *)data;
-    // Finish the current chunk sequentially.
-    size_t in_chunk = (size_t)blocks_ * 64 + buf_len_;
-    if (in_chunk < CHUNK) {
-        size_t take = std::min(len, CHUNK - in_chunk);
-        update(p, take);
-        p += take;
-        len -= take;
-    }
-    // Whole chunks strictly before the last byte go to the threads; the rest stays sequential.
-    size_t whole = len ? (len - 1) / CHUNK : 0;
-    if (whole < 8 || threads <= 1) {
-        update(p, len);
+void Blake3::update_with(const uint8_t *p, size_t len,
+                         const std::function<void(size_t, uint64_t, size_t, uint32_t *)> &cvs_of) {
+    // The head completes the current chunk; whole chunks strictly before the last byte are
+    // hashed by cvs_of (called before any byte is read, so it may also provide the bytes); the
+    // rest is absorbed here.
+    const size_t in_chunk = (size_t)blocks_ * 64 + buf_len_;
+    const size_t take = in_chunk < CHUNK ? std::min(len, CHUNK - in_chunk) : 0;
+    const size_t rest = len - take;
+    const size_t whole = rest ? (rest - 1) / CHUNK : 0;
+    std::vector<uint32_t> cvs(whole * 8);
+    const uint64_t first = chunk_ + 1;  // the current chunk is closed before the whole ones
+    cvs_of(take, first, whole, cvs.data());
+    update(p, take);
+    p += take;
+    if (whole == 0) {
+        update(p, rest);
         return;
     }
-    // The current (full) chunk is closed by the first byte of more input.
     {
         uint32_t cv[8];
         chunk_cv(cv);
@@ -182,28 +169,10 @@ void Blake3::update_parallel(const void *data, size_t len, int threads) {
         buf_len_ = 0;
         memcpy(cv_, key_, 32);
     }
-    std::vector<uint32_t> cvs(whole * 8);
-    const uint64_t first = chunk_;
-    auto work = [&](size_t lo, size_t hi) {
-        for (size_t c = lo; c < hi; c++) chunk_cv_of(key_, flags_, p + c * CHUNK, first + c, &cvs[8 * c]);
-    };
-    int nt = (int)std::min<size_t>((size_t)threads, whole / 4);
-    std::vector<std::thread> pool;
-    size_t per = (whole + nt - 1) / nt;
-    for (int t = 1; t < nt; t++) {
-        size_t lo = t * per, hi = std::min(whole, lo + per);
-        if (lo < hi) pool.emplace_back(work, lo, hi);
-    }
-    work(0, std::min(whole, per));
-    for (auto &th : pool) th.join();
-    for (size_t c = 0; c + 1 < whole; c++) add_chunk_cv(&cvs[8 * c], first + c + 1);
-    // The last whole chunk is pushed too, since more input (at least one byte) follows it.
-    add_chunk_cv(&cvs[8 * (whole - 1)], first + whole);
+    // Every whole chunk is pushed, the last one too, since at least one more byte follows it.
+    for (size_t c = 0; c < whole; c++) add_chunk_cv(&cvs[8 * c], first + c + 1);
     chunk_ = first + whole;
-    blocks_ = 0;
-    buf_len_ = 0;
-    memcpy(cv_, key_, 32);
-    update(p + whole * CHUNK, len - whole * CHUNK);
+    update(p + whole * CHUNK, rest - whole * CHUNK);
 }
 
 void Blake3::xof(uint64_t offset, uint8_t *out, size_t len) const {

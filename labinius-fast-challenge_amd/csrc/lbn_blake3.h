@@ -1,11 +1,12 @@
 // BLAKE3 for the prover's host side: the Fiat-Shamir transcript, the public matrix expansion and
-// the challenge sign key. A portable implementation of the published specification, with a
-// multi-threaded absorb that hashes whole chunks on several threads and pushes their chaining
-// values in order, so the state equals a sequential absorb of the same bytes.
+// the challenge sign key. A portable implementation of the published specification, with an
+// absorb whose whole chunks are hashed elsewhere (the device holds the commitment) and whose
+// chaining values are pushed in order, so the state equals a sequential absorb of the same bytes.
 #pragma once
 #include <cstddef>
 #include <cstdint>
 #include <cstring>
+#include <functional>
 
 namespace lbn {
 
@@ -20,8 +21,15 @@ class Blake3 {
     static Blake3 derive_key(const char *context);
 
     void update(const void *data, size_t len);
-    // The same state as update(), with whole chunks hashed on `threads` threads.
-    void update_parallel(const void *data, size_t len, int threads);
+    // The same state as update(data, len), with the whole chunks strictly inside the input hashed
+    // elsewhere: cvs_of(offset, first, count, cvs) is called once, before any byte of data is
+    // read, and must leave in cvs the chaining values of the `count` chunks data[offset + i * CHUNK
+    // ..] with chunk indices first + i (key and flags of this hasher). Only the bytes outside those
+    // chunks are read here, after the call.
+    void update_with(const uint8_t *data, size_t len,
+                     const std::function<void(size_t, uint64_t, size_t, uint32_t *)> &cvs_of);
+    const uint32_t *key_words() const { return key_; }
+    uint32_t flags() const { return flags_; }
     void update_u64(uint64_t x) {
         uint8_t b[8];
         for (int i = 0; i < 8; i++) b[i] = (uint8_t)(x >> (8 * i));

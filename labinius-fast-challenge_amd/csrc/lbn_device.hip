@@ -51,6 +51,13 @@ struct lbn::DeviceState {
     uint8_t *cbytes = nullptr;      // 1024 * 648 * width
     uint64_t *amat = nullptr;       // RE * 648 * 2 (staging for the matrix)
     size_t ahat_elems = 0;
+    uint32_t *cvs = nullptr;        // chunk chaining values of the commitment bytes
+    // Pinned host staging: the commitment bytes (copied on copy_stream once the commitment is
+    // ready, while the opening runs) and the fold output.
+    uint8_t *hstage = nullptr;
+    size_t hstage_fold = 0;         // byte offset of the fold region
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t ev_commit = nullptr, ev_staged = nullptr;
     hipEvent_t ev[2 * 9] = {};
     Roots18 roots[MAX_PRIMES] = {};      // start/stop per timed pass
 };
@@ -121,15 +128,8 @@ struct B3Root {
     uint32_t len, flags;
 };
 
-__global__ void k_table_xof(B3Root root, uint32_t *out, size_t blocks) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= blocks) return;
-    const uint32_t IV0 = 0x6A09E667u, IV1 = 0xBB67AE85u, IV2 = 0x3C6EF372u, IV3 = 0xA54FF53Au;
-    uint32_t s[16] = {root.cv[0], root.cv[1], root.cv[2], root.cv[3], root.cv[4], root.cv[5], root.cv[6],
-                      root.cv[7], IV0, IV1, IV2, IV3, (uint32_t)i, (uint32_t)(i >> 32), root.len, root.flags};
-    uint32_t m[16];
-#pragma unroll
-    for (int k = 0; k < 16; k++) m[k] = root.m[k];
+// The seven rounds of the BLAKE3 compression on state s with message m (m is permuted in place).
+__device__ __forceinline__ void b3_rounds(uint32_t (&s)[16], uint32_t (&m)[16]) {
     const int perm[16] = {2, 6, 3, 10, 7, 0, 4, 13, 1, 11, 12, 5, 9, 14, 15, 8};
 #pragma unroll
     for (int r = 0; r < 7; r++) {
@@ -147,12 +147,65 @@ __global__ void k_table_xof(B3Root root, uint32_t *out, size_t blocks) {
 #pragma unroll
         for (int k = 0; k < 16; k++) m[k] = t[k];
     }
+}
+
+__global__ void k_table_xof(B3Root root, uint32_t *out, size_t blocks) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= blocks) return;
+    const uint32_t IV0 = 0x6A09E667u, IV1 = 0xBB67AE85u, IV2 = 0x3C6EF372u, IV3 = 0xA54FF53Au;
+    uint32_t s[16] = {root.cv[0], root.cv[1], root.cv[2], root.cv[3], root.cv[4], root.cv[5], root.cv[6],
+                      root.cv[7], IV0, IV1, IV2, IV3, (uint32_t)i, (uint32_t)(i >> 32), root.len, root.flags};
+    uint32_t m[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) m[k] = root.m[k];
+    b3_rounds(s, m);
     uint32_t *o = out + 16 * i;
 #pragma unroll
     for (int k = 0; k < 8; k++) {
         o[k] = s[k] ^ s[k + 8];
         o[k + 8] = s[k + 8] ^ root.cv[k];
     }
+}
+
+// Chaining values of whole 1 KiB chunks of a byte string (the commitment as the transcript
+// absorbs it): thread c hashes bytes [off + 1024 c, +1024) as chunk index first + c, sixteen
+// 64-byte blocks chained from the key. The start need not be 4-byte aligned: message words are
+// funnel-shifted from aligned loads (the buffer has slack past the data).
+struct B3Key {
+    uint32_t k[8];
+    uint32_t flags;
+};
+
+__global__ __launch_bounds__(64) void k_chunk_cvs(const uint8_t *bytes, size_t off, uint64_t first, size_t count,
+                                                  B3Key key, uint32_t *cvs) {
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= count) return;
+    const size_t base = off + c * 1024;
+    const uint32_t *w = (const uint32_t *)(bytes + (base & ~(size_t)3));
+    const uint32_t sh = (uint32_t)(base & 3) * 8;
+    const uint64_t index = first + c;
+    uint32_t cv[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) cv[k] = key.k[k];
+    for (int b = 0; b < 16; b++) {
+        uint32_t m[16];
+        uint32_t lo = w[16 * b];
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const uint32_t hi = w[16 * b + i + 1];
+            m[i] = __builtin_amdgcn_alignbit(hi, lo, sh);
+            lo = hi;
+        }
+        const uint32_t flags = key.flags | (b == 0 ? 1u : 0u) | (b == 15 ? 2u : 0u);  // CHUNK_START, CHUNK_END
+        uint32_t st[16] = {cv[0], cv[1], cv[2], cv[3], cv[4], cv[5], cv[6], cv[7],
+                           0x6A09E667u, 0xBB67AE85u, 0x3C6EF372u, 0xA54FF53Au,
+                           (uint32_t)index, (uint32_t)(index >> 32), 64u, flags};
+        b3_rounds(st, m);
+#pragma unroll
+        for (int k = 0; k < 8; k++) cv[k] = st[k] ^ st[k + 8];
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k++) cvs[8 * c + k] = cv[k];
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -260,11 +313,33 @@ struct RhoArg {
 };
 constexpr int RHO_WINDOWS = 41;
 
+// rho * x^s for s < 128: the shifted product (below 2^290) reduced modulo x^162 + x^81 + 1.
+__device__ __forceinline__ void f162_mulx_pow(const uint64_t (&a)[3], int s, uint64_t (&out)[3]) {
+    const int ws = s >> 6, bs = s & 63;
+    uint64_t sh[4];  // a << bs over four limbs
+    sh[0] = a[0] << bs;
+    sh[1] = (a[1] << bs) | (bs ? a[0] >> (64 - bs) : 0);
+    sh[2] = (a[2] << bs) | (bs ? a[1] >> (64 - bs) : 0);
+    sh[3] = bs ? a[2] >> (64 - bs) : 0;
+    uint64_t v[5];
+    v[0] = ws ? 0 : sh[0];
+    v[1] = ws ? sh[0] : sh[1];
+    v[2] = ws ? sh[1] : sh[2];
+    v[3] = ws ? sh[2] : sh[3];
+    v[4] = ws ? sh[3] : 0;
+    f162_reduce5(v, out);
+}
+
 __device__ void build_rho_table(RhoArg rho, uint32_t *basis /*162*6*/, uint32_t *tab /*41*16*6*/) {
     const int t = threadIdx.x;
     if (t < 162) {
-        uint64_t a0 = rho.r[0], a1 = rho.r[1], a2 = rho.r[2];
-        for (int s = 0; s < t; s++) f162_mulx(a0, a1, a2);
+        // basis[t] = rho * x^t, through rho * x^128 for t >= 128
+        const uint64_t r[3] = {rho.r[0], rho.r[1], rho.r[2]};
+        uint64_t r128[3], x[3];
+        f162_mulx_pow(r, 64, x);
+        f162_mulx_pow(x, 64, r128);
+        f162_mulx_pow(t < 128 ? r : r128, t & 127, x);
+        const uint64_t a0 = x[0], a1 = x[1], a2 = x[2];
         uint32_t *b = basis + 6 * t;
         b[0] = (uint32_t)a0, b[1] = (uint32_t)(a0 >> 32), b[2] = (uint32_t)a1, b[3] = (uint32_t)(a1 >> 32);
         b[4] = (uint32_t)a2, b[5] = (uint32_t)(a2 >> 32);
@@ -307,9 +382,13 @@ __device__ __forceinline__ void rho_mul(const uint32_t *tab, const uint32_t (&y)
 //   Tinf_k += bit_k(E_j + E_{j+h2}) (p'_j + p'_{j+h2}),   E^(r+2)_j = E_j + E_{j+h2} (written)
 // WORDS: p is the table (lifted words) in round 0, F162 (6 words) afterwards.
 // ---------------------------------------------------------------------------------------------
+// Block 0 also clears `clear` (1536 words), the message buffer of the round after next.
 template <bool WORDS>
 __global__ __launch_bounds__(256) void k_switch_round(const uint32_t *pin, size_t h2, RhoArg rho, const uint4 *Ein,
-                                                      uint4 *Eout, uint32_t *pout, uint32_t *T0, uint32_t *Tinf) {
+                                                      uint4 *Eout, uint32_t *pout, uint32_t *T0, uint32_t *Tinf,
+                                                      uint32_t *clear) {
+    if (blockIdx.x == 0)
+        for (int i = threadIdx.x; i < 1536; i += blockDim.x) clear[i] = 0;
     __shared__ uint32_t rtab[RHO_WINDOWS * 16 * 6];
     __shared__ uint32_t lds[4 * M4W<6>::WORDS];  // also holds the basis while rtab is built
     build_rho_table(rho, lds, rtab);
@@ -1021,7 +1100,7 @@ int lbn::dev_init(lbn_ctx *ctx) {
     HIPCHK(ctx, hipMalloc(&d->E[1], s.n_words / 2 * 16));
     HIPCHK(ctx, hipMalloc(&d->P[0], s.n_words / 2 * 24));
     HIPCHK(ctx, hipMalloc(&d->P[1], s.n_words / 4 * 24));
-    HIPCHK(ctx, hipMalloc(&d->T, 2048 * 4));
+    HIPCHK(ctx, hipMalloc(&d->T, 3584 * 4));
     HIPCHK(ctx, hipMalloc(&d->EH, ((size_t)1 << (s.log_packed - 12)) * 16));
     HIPCHK(ctx, hipMalloc(&d->EL, 4096 * 16));
     HIPCHK(ctx, hipMalloc(&d->eqrow, s.column_len * 24));
@@ -1035,6 +1114,12 @@ int lbn::dev_init(lbn_ctx *ctx) {
     HIPCHK(ctx, hipMalloc(&d->ntt_tabs, (size_t)MAX_PRIMES * tab::SIZE * 4));
     HIPCHK(ctx, hipMalloc(&d->cacc, (size_t)MAX_PRIMES * COLUMNS * N * 8));
     HIPCHK(ctx, hipMalloc(&d->cbytes, (size_t)COLUMNS * N * 16));
+    HIPCHK(ctx, hipMalloc(&d->cvs, (size_t)COLUMNS * N * 16 / 1024 * 32 + 64));
+    d->hstage_fold = (size_t)COLUMNS * N * 16;
+    HIPCHK(ctx, hipHostMalloc((void **)&d->hstage, d->hstage_fold + s.ring_elements * N * sizeof(int32_t)));
+    HIPCHK(ctx, hipStreamCreateWithFlags(&d->copy_stream, hipStreamNonBlocking));
+    HIPCHK(ctx, hipEventCreateWithFlags(&d->ev_commit, hipEventDisableTiming));
+    HIPCHK(ctx, hipEventCreateWithFlags(&d->ev_staged, hipEventDisableTiming));
     for (auto &e : d->ev) HIPCHK(ctx, hipEventCreate(&e));
     return LBN_OK;
 }
@@ -1045,11 +1130,16 @@ void lbn::dev_free(lbn_ctx *ctx) {
     hipSetDevice(ctx->device);
     void *ptrs[] = {d->table, d->E[0], d->E[1], d->P[0], d->P[1], d->T, d->EH, d->EL, d->eqrow, d->eqlo,
                     d->rowU, d->rowOut, d->fold_flag, d->fold_off, d->fold_cols, d->fold_out, d->ntt_tabs,
-                    d->ahat, d->cacc, d->cbytes, d->amat};
+                    d->ahat, d->cacc, d->cbytes, d->amat, d->cvs};
+    if (d->copy_stream) hipStreamSynchronize(d->copy_stream);
     for (void *p : ptrs)
         if (p) hipFree(p);
+    if (d->hstage) hipHostFree(d->hstage);
     for (auto &e : d->ev)
         if (e) hipEventDestroy(e);
+    if (d->ev_commit) hipEventDestroy(d->ev_commit);
+    if (d->ev_staged) hipEventDestroy(d->ev_staged);
+    if (d->copy_stream) hipStreamDestroy(d->copy_stream);
     if (d->stream) hipStreamDestroy(d->stream);
     delete d;
     ctx->dev = nullptr;
@@ -1197,13 +1287,71 @@ static uint32_t powmod32(uint64_t a, uint64_t e, uint32_t p) {
     return (uint32_t)r;
 }
 
+// Host memcpy split over the worker pool (1 MiB pieces).
+static void copy_parallel(uint8_t *dst, const uint8_t *src, size_t n) {
+    const size_t piece = (size_t)1 << 20, pieces = (n + piece - 1) / piece;
+    parallel_for(pieces, [&](size_t i) {
+        const size_t lo = i * piece;
+        memcpy(dst + lo, src + lo, std::min(piece, n - lo));
+    });
+}
+
 extern "C" int lbn_commit(lbn_ctx *ctx, uint8_t *out, size_t out_len) {
     if (!ctx || !out) return LBN_E_ARG;
     if (!ctx->table_loaded || !ctx->modulus_set) return ctx->fail(LBN_E_STATE, "table and modulus must be loaded");
-    const Shape &s = ctx->shape;
     const size_t clen_bytes = (size_t)COLUMNS * N * ctx->width;
     if (out_len != clen_bytes) return ctx->fail(LBN_E_ARG, "commitment buffer has the wrong length");
+    const int rc = dev_commit_launch(ctx);
+    if (rc != LBN_OK) return rc;
+    DeviceState *d = ctx->dev;
+    HIPCHK(ctx, hipMemcpyAsync(out, d->cbytes, clen_bytes, hipMemcpyDeviceToHost, d->stream));
+    HIPCHK(ctx, hipStreamSynchronize(d->stream));
+    tcollect(ctx, T_MAC);
+    tcollect(ctx, T_FINISH);
+    return LBN_OK;
+}
+
+int lbn::dev_commit_cvs(lbn_ctx *ctx, size_t off, uint64_t first, size_t count, const uint32_t key[8],
+                        uint32_t flags, uint32_t *cvs, uint8_t *dst, size_t head, size_t tail) {
+    DeviceState *d = ctx->dev;
+    const size_t clen = (size_t)COLUMNS * N * ctx->width;
+    if (off + count * 1024 > clen || head > clen || tail > clen)
+        return ctx->fail(LBN_E_ARG, "commitment chunk range out of bounds");
     HIPCHK(ctx, hipSetDevice(ctx->device));
+    // the staging copy runs on its own stream once the commitment is final
+    HIPCHK(ctx, hipStreamWaitEvent(d->copy_stream, d->ev_commit, 0));
+    HIPCHK(ctx, hipMemcpyAsync(d->hstage, d->cbytes, clen, hipMemcpyDeviceToHost, d->copy_stream));
+    HIPCHK(ctx, hipEventRecord(d->ev_staged, d->copy_stream));
+    if (count) {
+        B3Key k;
+        for (int i = 0; i < 8; i++) k.k[i] = key[i];
+        k.flags = flags;
+        k_chunk_cvs<<<(unsigned)((count + 63) / 64), 64, 0, d->stream>>>(d->cbytes, off, first, count, k, d->cvs);
+        HIPCHK(ctx, hipGetLastError());
+        HIPCHK(ctx, hipMemcpyAsync(cvs, d->cvs, count * 32, hipMemcpyDeviceToHost, d->stream));
+    }
+    if (head) HIPCHK(ctx, hipMemcpyAsync(dst, d->cbytes, head, hipMemcpyDeviceToHost, d->stream));
+    if (tail < clen) HIPCHK(ctx, hipMemcpyAsync(dst + tail, d->cbytes + tail, clen - tail, hipMemcpyDeviceToHost, d->stream));
+    HIPCHK(ctx, hipStreamSynchronize(d->stream));
+    tcollect(ctx, T_MAC);
+    tcollect(ctx, T_FINISH);
+    return LBN_OK;
+}
+
+int lbn::dev_commit_bytes(lbn_ctx *ctx, uint8_t *dst) {
+    DeviceState *d = ctx->dev;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, hipEventSynchronize(d->ev_staged));
+    copy_parallel(dst, d->hstage, (size_t)COLUMNS * N * ctx->width);
+    return LBN_OK;
+}
+
+int lbn::dev_commit_launch(lbn_ctx *ctx) {
+    if (!ctx->table_loaded || !ctx->modulus_set) return ctx->fail(LBN_E_STATE, "table and modulus must be loaded");
+    const Shape &s = ctx->shape;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    // a staging copy of an earlier commitment still reading cbytes finishes first
+    HIPCHK(ctx, hipStreamWaitEvent(ctx->dev->stream, ctx->dev->ev_staged, 0));
     DeviceState *d = ctx->dev;
     const int np = ctx->nprimes;
     fill_u32(d->stream, d->cacc, (size_t)np * COLUMNS * N * 2);
@@ -1253,10 +1401,7 @@ extern "C" int lbn_commit(lbn_ctx *ctx, uint8_t *out, size_t out_len) {
     k_commit_finish<<<COLUMNS, 64, 0, d->stream>>>(d->cacc, d->ntt_tabs, crt, d->cbytes);
     tstop(ctx, T_FINISH);
     HIPCHK(ctx, hipGetLastError());
-    HIPCHK(ctx, hipMemcpyAsync(out, d->cbytes, clen_bytes, hipMemcpyDeviceToHost, d->stream));
-    HIPCHK(ctx, hipStreamSynchronize(d->stream));
-    tcollect(ctx, T_MAC);
-    tcollect(ctx, T_FINISH);
+    HIPCHK(ctx, hipEventRecord(d->ev_commit, d->stream));
     return LBN_OK;
 }
 
@@ -1311,6 +1456,10 @@ static F162 combine(const std::vector<F162> &L, const uint32_t *T, int V) {
     return acc;
 }
 
+// Message sums of switch round r (T0 then Tinf, 768 words each) alternate between two buffers;
+// the kernel that produces round r + 1's sums clears round r's buffer for round r + 2.
+static uint32_t *msg_buf(DeviceState *d, int r) { return d->T + ((r & 1) ? 2048 : 0); }
+
 static int launch_blocks(size_t items) { return (int)std::max<size_t>(1, std::min<size_t>((items + 255) / 256, 1024)); }
 
 extern "C" int lbn_switch_begin(lbn_ctx *ctx, const uint64_t *batch) {
@@ -1321,7 +1470,8 @@ extern "C" int lbn_switch_begin(lbn_ctx *ctx, const uint64_t *batch) {
     ctx->lmap.assign((const F162 *)batch, (const F162 *)batch + 128);
     ctx->round = 0;
     const size_t h = ctx->shape.n_words / 2;
-    fill_u32(d->stream, d->T, 1536);
+    fill_u32(d->stream, msg_buf(d, 0), 1536);
+    fill_u32(d->stream, msg_buf(d, 1), 1536);
     tstart(ctx, T_SWITCH0);
     k_switch_msg0<<<launch_blocks(h), 256, 0, d->stream>>>(d->table, d->E[0], h, d->E[1], d->T, d->T + 768);
     tstop(ctx, T_SWITCH0);
@@ -1335,7 +1485,7 @@ extern "C" int lbn_switch_msg(lbn_ctx *ctx, uint64_t *msg) {
     HIPCHK(ctx, hipSetDevice(ctx->device));
     DeviceState *d = ctx->dev;
     uint32_t T[1536];
-    HIPCHK(ctx, hipMemcpyAsync(T, d->T, sizeof T, hipMemcpyDeviceToHost, d->stream));
+    HIPCHK(ctx, hipMemcpyAsync(T, msg_buf(d, ctx->round), sizeof T, hipMemcpyDeviceToHost, d->stream));
     HIPCHK(ctx, hipStreamSynchronize(d->stream));
     tcollect(ctx, ctx->round == 0 ? T_SWITCH0 : T_ROUNDS);
     const int V = ctx->round == 0 ? 4 : 6;
@@ -1343,6 +1493,29 @@ extern "C" int lbn_switch_msg(lbn_ctx *ctx, uint64_t *msg) {
     memcpy(msg, e0.w, 24);
     memcpy(msg + 3, einf.w, 24);
     return LBN_OK;
+}
+
+// L_{r+1}(y) = L_r((1 + c) y) + rho L_r(y), c = r_hi[r]: the images of x^k, k < 128, through
+// 16 tables of 256 sums of L_r over 8-bit windows of y (method of four Russians).
+static void update_lmap(lbn_ctx *ctx, int r, const F162 &rho) {
+    const std::vector<F162> &L = ctx->lmap;
+    std::vector<F162> tab(16 * 256);
+    for (int w = 0; w < 16; w++) {
+        F162 *T = tab.data() + 256 * w;
+        T[0] = F162{{0, 0, 0}};
+        for (int b = 1; b < 256; b++) T[b] = f162_add(T[b & (b - 1)], L[8 * w + __builtin_ctz(b)]);
+    }
+    std::vector<F162> next(128);
+    B128 z = b128_add(ctx->r_hi[r], B128{{1, 0}});  // (1 + c) x^k
+    for (int k = 0; k < 128; k++) {
+        F162 acc{{0, 0, 0}};
+        for (int w = 0; w < 16; w++) acc = f162_add(acc, tab[256 * w + ((z.w[w >> 3] >> (8 * (w & 7))) & 255)]);
+        next[k] = f162_add(acc, f162_mul(rho, L[k]));
+        const uint64_t carry = z.w[1] >> 63;  // z *= x
+        z.w[1] = (z.w[1] << 1) | (z.w[0] >> 63);
+        z.w[0] = (z.w[0] << 1) ^ (carry ? 0x87ull : 0ull);
+    }
+    ctx->lmap.swap(next);
 }
 
 extern "C" int lbn_switch_fold(lbn_ctx *ctx, const uint64_t *rr) {
@@ -1353,22 +1526,6 @@ extern "C" int lbn_switch_fold(lbn_ctx *ctx, const uint64_t *rr) {
     DeviceState *d = ctx->dev;
     const int r = ctx->round;
     F162 rho{{rr[0], rr[1], rr[2]}};
-    // L_{r+1}(y) = L_r((1 + c) y) + rho L_r(y), c = r_hi[r].
-    {
-        const B128 one_c = b128_add(ctx->r_hi[r], B128{{1, 0}});
-        std::vector<F162> next(128);
-        B128 z = one_c;  // (1 + c) x^k
-        for (int k = 0; k < 128; k++) {
-            F162 acc{{0, 0, 0}};
-            for (int m = 0; m < 128; m++)
-                if ((z.w[m >> 6] >> (m & 63)) & 1) acc = f162_add(acc, ctx->lmap[m]);
-            next[k] = f162_add(acc, f162_mul(rho, ctx->lmap[k]));
-            uint64_t carry = z.w[1] >> 63;  // z *= x
-            z.w[1] = (z.w[1] << 1) | (z.w[0] >> 63);
-            z.w[0] = (z.w[0] << 1) ^ (carry ? 0x87ull : 0ull);
-        }
-        ctx->lmap.swap(next);
-    }
     const size_t n = ctx->shape.n_words;
     const size_t h = n >> (r + 1);  // current half
     if (r == lp - 1) {
@@ -1398,16 +1555,17 @@ extern "C" int lbn_switch_fold(lbn_ctx *ctx, const uint64_t *rr) {
     const uint4 *Ein = d->E[(r + 1) & 1];
     uint4 *Eout = d->E[r & 1];
     uint32_t *pout = d->P[r & 1];
-    fill_u32(d->stream, d->T, 1536);
+    uint32_t *T = msg_buf(d, r + 1);
     tstart(ctx, T_ROUNDS);
     if (r == 0)
         k_switch_round<true><<<launch_blocks(h2), 256, 0, d->stream>>>((const uint32_t *)d->table, h2, ra, Ein, Eout,
-                                                                        pout, d->T, d->T + 768);
+                                                                        pout, T, T + 768, msg_buf(d, r));
     else
         k_switch_round<false><<<launch_blocks(h2), 256, 0, d->stream>>>(d->P[(r - 1) & 1], h2, ra, Ein, Eout, pout,
-                                                                         d->T, d->T + 768);
+                                                                         T, T + 768, msg_buf(d, r));
     tstop(ctx, T_ROUNDS);
     HIPCHK(ctx, hipGetLastError());
+    update_lmap(ctx, r, rho);  // on the host while the kernel runs; only the next message needs it
     ctx->round = r + 1;
     return LBN_OK;
 }
@@ -1514,11 +1672,13 @@ extern "C" int lbn_fold(lbn_ctx *ctx, const lbn_challenge *ch, int n_challenges,
     tstop(ctx, T_FOLD);
     HIPCHK(ctx, hipGetLastError());
     uint32_t flag = 0;
-    HIPCHK(ctx, hipMemcpyAsync(out, d->fold_out, n_out * 4, hipMemcpyDeviceToHost, d->stream));
+    uint8_t *stage = d->hstage + d->hstage_fold;  // pinned: the copy runs at full link speed
+    HIPCHK(ctx, hipMemcpyAsync(stage, d->fold_out, n_out * 4, hipMemcpyDeviceToHost, d->stream));
     HIPCHK(ctx, hipMemcpyAsync(&flag, d->fold_flag, 4, hipMemcpyDeviceToHost, d->stream));
     HIPCHK(ctx, hipStreamSynchronize(d->stream));
     tcollect(ctx, T_FOLD);
     if (flag) return ctx->fail(LBN_E_RANGE, "a fold coefficient does not fit the wire codec");
+    copy_parallel((uint8_t *)out, stage, n_out * 4);
     return LBN_OK;
 }
 

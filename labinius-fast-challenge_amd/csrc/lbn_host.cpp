@@ -5,7 +5,10 @@
 #include <immintrin.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <condition_variable>
+#include <mutex>
 #include <cmath>
 #include <cstring>
 #include <thread>
@@ -91,24 +94,84 @@ void eq_table_f162(const F162 *p, int n, F162 *t) {
         }
 }
 
-static int host_threads() {
+int host_threads() {
     unsigned n = std::thread::hardware_concurrency();
     return (int)std::max(1u, std::min(n ? n : 1u, 16u));
 }
 
-template <class F>
-static void parallel_for(size_t n, F f) {
-    int nt = (int)std::min<size_t>((size_t)host_threads(), n);
-    if (nt <= 1) {
+// A persistent pool: workers sleep on a condition variable between jobs; a job hands out
+// indices through an atomic counter, and the caller works on it too. One job at a time.
+namespace {
+class Pool {
+   public:
+    static Pool &get() {
+        static Pool pool;
+        return pool;
+    }
+    void run(size_t n, const std::function<void(size_t)> &f) {
+        std::lock_guard<std::mutex> job_lock(job_);
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            f_ = &f;
+            n_ = n;
+            next_.store(0);
+            active_ = (int)workers_.size();
+            gen_++;
+        }
+        cv_.notify_all();
+        work();
+        std::unique_lock<std::mutex> lk(m_);
+        done_.wait(lk, [&] { return active_ == 0; });
+        f_ = nullptr;
+    }
+    ~Pool() {
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto &t : workers_) t.join();
+    }
+
+   private:
+    Pool() {
+        for (int i = 1; i < host_threads(); i++) workers_.emplace_back([this] { loop(); });
+    }
+    void work() {
+        for (size_t i; (i = next_.fetch_add(1)) < n_;) (*f_)(i);
+    }
+    void loop() {
+        uint64_t seen = 0;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> lk(m_);
+                cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+                if (stop_) return;
+                seen = gen_;
+            }
+            work();
+            std::lock_guard<std::mutex> lk(m_);
+            if (--active_ == 0) done_.notify_one();
+        }
+    }
+    std::mutex job_, m_;
+    std::condition_variable cv_, done_;
+    std::vector<std::thread> workers_;
+    const std::function<void(size_t)> *f_ = nullptr;
+    size_t n_ = 0;
+    std::atomic<size_t> next_{0};
+    int active_ = 0;
+    uint64_t gen_ = 0;
+    bool stop_ = false;
+};
+}  // namespace
+
+void parallel_for(size_t n, const std::function<void(size_t)> &f) {
+    if (n <= 1 || host_threads() <= 1) {
         for (size_t i = 0; i < n; i++) f(i);
         return;
     }
-    std::vector<std::thread> pool;
-    for (int t = 0; t < nt; t++)
-        pool.emplace_back([&, t] {
-            for (size_t i = t; i < n; i += nt) f(i);
-        });
-    for (auto &th : pool) th.join();
+    Pool::get().run(n, f);
 }
 
 // ============================================================================= transcript
@@ -122,13 +185,17 @@ class Transcript {
         h_.update_u64(n);
         h_.update(domain, n);
     }
-    void absorb_bytes(const void *b, size_t n, int threads = 1) {
+    void absorb_bytes(const void *b, size_t n) {
         h_.update_u64(n);
-        if (threads > 1)
-            h_.update_parallel(b, n, threads);
-        else
-            h_.update(b, n);
+        h_.update(b, n);
     }
+    // absorb_bytes with the whole chunks hashed by cvs_of (see Blake3::update_with).
+    void absorb_bytes_with(const uint8_t *b, size_t n,
+                           const std::function<void(size_t, uint64_t, size_t, uint32_t *)> &cvs_of) {
+        h_.update_u64(n);
+        h_.update_with(b, n, cvs_of);
+    }
+    const Blake3 &hasher() const { return h_; }
     void absorb_u64(uint64_t x) { h_.update_u64(x); }
     void absorb_b128(const B128 *xs, size_t n) {
         h_.update_u64(n);
@@ -182,6 +249,7 @@ class Transcript {
 struct ChallengeSampler {
     double cosv[243], sinv[243];
     int units[81];
+    uint8_t angle[81][162];  // (position * units[k]) mod 243
     Blake3 sign_key;
     ChallengeSampler() : sign_key(Blake3::derive_key("labinius 2026 challenge signs v1")) {
         for (int m = 0; m < 243; m++) {
@@ -192,6 +260,8 @@ struct ChallengeSampler {
         int n = 0;
         for (int x = 1; x < 243 && n < 81; x++)
             if (x % 3) units[n++] = x;
+        for (int k = 0; k < 81; k++)
+            for (int p = 0; p < 162; p++) angle[k][p] = (uint8_t)((p * units[k]) % 243);
     }
     static uint32_t below(XofReader &x, uint32_t n) {
         const uint32_t limit = (65536u / n) * n;
@@ -228,7 +298,7 @@ struct ChallengeSampler {
         for (int k = 0; k < 81; k++) {
             double re = 0.0, im = 0.0;
             for (uint32_t t = 0; t < c.weight; t++) {
-                const int m = (c.positions[t] * units[k]) % 243;
+                const int m = angle[k][c.positions[t]];
                 const bool neg = (c.signs >> t) & 1;
                 re = re + (neg ? -cosv[m] : cosv[m]);
                 im = im + (neg ? -sinv[m] : sinv[m]);
@@ -421,16 +491,18 @@ double now_us() {
 }
 // wire::pack_f162 (pcs/src/wire/mod.rs:445-453): 162 bits per element, least significant first.
 void pack_f162(const F162 *v, size_t n, uint8_t *out) {
-    memset(out, 0, f162_bytes(n));
-    size_t bit = 0;
+    std::vector<uint64_t> w((n * 162 + 63) / 64 + 2, 0);
+    auto put = [&](size_t bit, uint64_t x) {  // x < 2^64 at bit offset `bit`
+        const size_t i = bit >> 6, b = bit & 63;
+        w[i] |= x << b;
+        if (b) w[i + 1] |= x >> (64 - b);
+    };
     for (size_t i = 0; i < n; i++) {
-        for (int l = 0; l < 3; l++) {
-            const int nb = l < 2 ? 64 : 34;
-            uint64_t x = v[i].w[l];
-            for (int b = 0; b < nb; b++, bit++)
-                if ((x >> b) & 1) out[bit >> 3] |= (uint8_t)(1u << (bit & 7));
-        }
+        put(162 * i, v[i].w[0]);
+        put(162 * i + 64, v[i].w[1]);
+        put(162 * i + 128, v[i].w[2] & ((1ull << 34) - 1));
     }
+    for (size_t k = 0; k < f162_bytes(n); k++) out[k] = (uint8_t)(w[k >> 3] >> (8 * (k & 7)));
 }
 }  // namespace
 
@@ -462,8 +534,7 @@ extern "C" int lbn_prove(lbn_ctx *ctx, const uint8_t seed[32], uint8_t *proof, s
     lbn_profile pf{};
     const double t0 = now_us();
 
-    TRY(lbn_commit(ctx, proof + off_commit, clen));
-    const double t1 = now_us();
+    TRY(dev_commit_launch(ctx));
 
     // verifier/proof_verify/src/main.rs:166-182 (absorb k and 10, the shape of this instance)
     Transcript t("labinius-fast/v1");
@@ -473,7 +544,15 @@ extern "C" int lbn_prove(lbn_ctx *ctx, const uint8_t seed[32], uint8_t *proof, s
     uint8_t qb[16];
     for (int i = 0; i < 16; i++) qb[i] = (uint8_t)(q >> (8 * i));
     t.absorb_bytes(qb, 16);
-    t.absorb_bytes(proof + off_commit, clen, host_threads());
+    // The commitment's whole BLAKE3 chunks are hashed where the bytes are (device); the host reads
+    // back their chaining values and the bytes around them. All bytes reach the proof at the end.
+    int cvs_rc = LBN_OK;
+    t.absorb_bytes_with(proof + off_commit, clen, [&](size_t off, uint64_t first, size_t count, uint32_t *cvs) {
+        cvs_rc = dev_commit_cvs(ctx, off, first, count, t.hasher().key_words(), t.hasher().flags(), cvs,
+                                proof + off_commit, off, off + count * Blake3::CHUNK);
+    });
+    TRY(cvs_rc);
+    const double t1 = now_us();
     std::vector<B128> r_lo = t.sample_b128("r-lo", 7);
     std::vector<B128> r_hi = t.sample_b128("r-hi", lp);
     const double t2 = now_us();
@@ -528,6 +607,7 @@ extern "C" int lbn_prove(lbn_ctx *ctx, const uint8_t seed[32], uint8_t *proof, s
     const double t6 = now_us();
 
     TRY(lbn_fold(ctx, ch.data(), COLUMNS, (int32_t *)(proof + off_fold + 8), (size_t)N * s.ring_elements));
+    TRY(dev_commit_bytes(ctx, proof + off_commit));
     const double t7 = now_us();
 
     Writer w{proof};

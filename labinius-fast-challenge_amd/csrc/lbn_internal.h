@@ -2,6 +2,7 @@
 #pragma once
 #include <cstddef>
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -109,4 +110,17 @@ namespace lbn {
 // Device entry points used by the host prover (lbn_device.hip).
 int dev_init(lbn_ctx *ctx);
 void dev_free(lbn_ctx *ctx);
+// The commitment kernels, leaving the packed commitment in device memory.
+int dev_commit_launch(lbn_ctx *ctx);
+// Chaining values of `count` whole BLAKE3 chunks of the device commitment from byte `off`, chunk
+// indices first.., under key/flags, into cvs (8 words each); also copies the commitment bytes
+// [0, head) and [tail, clen) to the same offsets of dst, and starts copying all of it to pinned
+// host memory (dev_commit_bytes finishes that copy into dst).
+int dev_commit_cvs(lbn_ctx *ctx, size_t off, uint64_t first, size_t count, const uint32_t key[8], uint32_t flags,
+                   uint32_t *cvs, uint8_t *dst, size_t head, size_t tail);
+int dev_commit_bytes(lbn_ctx *ctx, uint8_t *dst);
+
+// Host worker pool: f(i) for every i < n, on up to host_threads() threads (the caller included).
+int host_threads();
+void parallel_for(size_t n, const std::function<void(size_t)> &f);
 }  // namespace lbn

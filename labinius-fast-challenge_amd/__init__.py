@@ -177,13 +177,23 @@ class Prover:
         self._check(lib().lbn_fold(self.h, ctypes.cast(challenges, ctypes.c_void_p), 1024, _p(out), out.size))
         return out
 
-    def prove(self, seed: bytes):
-        cap = proof_len(self.log_vars, self.q)
-        out = np.zeros(cap, dtype=np.uint8)
+    def prove_into(self, seed: bytes, out: np.ndarray):
+        """lbn_prove into a caller-owned uint8 array (at least proof_len bytes); returns the proof
+        length and the per-phase profile."""
+        if out.dtype != np.uint8 or not out.flags.c_contiguous or not out.flags.writeable:
+            raise ValueError("out must be a writable contiguous uint8 array")
         n = ctypes.c_size_t(0)
         prof = Profile()
-        self._check(lib().lbn_prove(self.h, _b(seed), _p(out), cap, ctypes.byref(n), ctypes.addressof(prof)))
-        return out[: n.value].tobytes(), {f: getattr(prof, f) for f, _ in Profile._fields_}
+        self._check(lib().lbn_prove(self.h, _b(seed), _p(out), out.size, ctypes.byref(n), ctypes.addressof(prof)))
+        return n.value, {f: getattr(prof, f) for f, _ in Profile._fields_}
+
+    def prove(self, seed: bytes):
+        cap = proof_len(self.log_vars, self.q)
+        buf = getattr(self, "_buf", None)
+        if buf is None or buf.size < cap:
+            buf = self._buf = np.zeros(cap, dtype=np.uint8)
+        n, prof = self.prove_into(seed, buf)
+        return buf[:n].tobytes(), prof
 
     def commit_primes(self) -> int:
         return lib().lbn_commit_primes(self.h)

@@ -44,7 +44,6 @@ struct lbn::DeviceState {
     uint32_t *fold_flag = nullptr;
     int *fold_off = nullptr;
     uint16_t *fold_cols = nullptr;
-    int32_t *fold_out = nullptr;
     uint32_t *ntt_tabs = nullptr;   // MAX_PRIMES * tab::SIZE
     uint32_t *ahat = nullptr;       // nprimes * RE * 648
     unsigned long long *cacc = nullptr;  // nprimes * 1024 * 648
@@ -56,6 +55,7 @@ struct lbn::DeviceState {
     // ready, while the opening runs) and the fold output.
     uint8_t *hstage = nullptr;
     size_t hstage_fold = 0;         // byte offset of the fold region
+    int32_t *fold_host = nullptr;   // device address of the fold region: the fold kernel writes it
     hipStream_t copy_stream = nullptr;
     hipEvent_t ev_commit = nullptr, ev_staged = nullptr;
     hipEvent_t ev[2 * 9] = {};
@@ -1110,13 +1110,13 @@ int lbn::dev_init(lbn_ctx *ctx) {
     HIPCHK(ctx, hipMalloc(&d->fold_flag, 4));
     HIPCHK(ctx, hipMalloc(&d->fold_off, (2 * 162 + 1) * sizeof(int)));
     HIPCHK(ctx, hipMalloc(&d->fold_cols, (size_t)COLUMNS * 32 * sizeof(uint16_t)));
-    HIPCHK(ctx, hipMalloc(&d->fold_out, s.ring_elements * N * sizeof(int32_t)));
     HIPCHK(ctx, hipMalloc(&d->ntt_tabs, (size_t)MAX_PRIMES * tab::SIZE * 4));
     HIPCHK(ctx, hipMalloc(&d->cacc, (size_t)MAX_PRIMES * COLUMNS * N * 8));
     HIPCHK(ctx, hipMalloc(&d->cbytes, (size_t)COLUMNS * N * 16));
     HIPCHK(ctx, hipMalloc(&d->cvs, (size_t)COLUMNS * N * 16 / 1024 * 32 + 64));
     d->hstage_fold = (size_t)COLUMNS * N * 16;
     HIPCHK(ctx, hipHostMalloc((void **)&d->hstage, d->hstage_fold + s.ring_elements * N * sizeof(int32_t)));
+    HIPCHK(ctx, hipHostGetDevicePointer((void **)&d->fold_host, d->hstage + d->hstage_fold, 0));
     HIPCHK(ctx, hipStreamCreateWithFlags(&d->copy_stream, hipStreamNonBlocking));
     HIPCHK(ctx, hipEventCreateWithFlags(&d->ev_commit, hipEventDisableTiming));
     HIPCHK(ctx, hipEventCreateWithFlags(&d->ev_staged, hipEventDisableTiming));
@@ -1129,7 +1129,7 @@ void lbn::dev_free(lbn_ctx *ctx) {
     if (!d) return;
     hipSetDevice(ctx->device);
     void *ptrs[] = {d->table, d->E[0], d->E[1], d->P[0], d->P[1], d->T, d->EH, d->EL, d->eqrow, d->eqlo,
-                    d->rowU, d->rowOut, d->fold_flag, d->fold_off, d->fold_cols, d->fold_out, d->ntt_tabs,
+                    d->rowU, d->rowOut, d->fold_flag, d->fold_off, d->fold_cols, d->ntt_tabs,
                     d->ahat, d->cacc, d->cbytes, d->amat, d->cvs};
     if (d->copy_stream) hipStreamSynchronize(d->copy_stream);
     for (void *p : ptrs)
@@ -1655,7 +1655,7 @@ extern "C" int lbn_fold(lbn_ctx *ctx, const lbn_challenge *ch, int n_challenges,
 #define FOLD_CASE(B)                                                                                          \
     case B:                                                                                                   \
         k_fold<B><<<grid, FOLD_THREADS, 0, d->stream>>>(d->table, s.column_len, s.ring_elements, d->fold_off,   \
-                                                        d->fold_cols, n_entries, d->fold_out, d->fold_flag); \
+                                                        d->fold_cols, n_entries, d->fold_host, d->fold_flag); \
         break;
     switch (std::max(nb, 6)) {
         FOLD_CASE(6)
@@ -1671,9 +1671,10 @@ extern "C" int lbn_fold(lbn_ctx *ctx, const lbn_challenge *ch, int n_challenges,
 #undef FOLD_CASE
     tstop(ctx, T_FOLD);
     HIPCHK(ctx, hipGetLastError());
+    // The kernel wrote the fold straight into pinned host memory over the host link, overlapped
+    // with its own work; once the stream is idle the values are visible here.
     uint32_t flag = 0;
-    uint8_t *stage = d->hstage + d->hstage_fold;  // pinned: the copy runs at full link speed
-    HIPCHK(ctx, hipMemcpyAsync(stage, d->fold_out, n_out * 4, hipMemcpyDeviceToHost, d->stream));
+    const uint8_t *stage = d->hstage + d->hstage_fold;
     HIPCHK(ctx, hipMemcpyAsync(&flag, d->fold_flag, 4, hipMemcpyDeviceToHost, d->stream));
     HIPCHK(ctx, hipStreamSynchronize(d->stream));
     tcollect(ctx, T_FOLD);

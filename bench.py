@@ -10,7 +10,8 @@ Besides the JSON line it prints per-pass kernel times (HIP events on the library
 dominant kernel's roofline, and a CPU baseline: the scalar C oracle (same algorithm, port) timed on
 a bounded sample (a full proof at 2^23 evaluations) and scaled by the work ratio to 2^31.
 
-    python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--log-vars 31] [--no-cpu-baseline]
+    python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--log-vars 31] [--no-cpu-baseline] [--no-batched]
+    python bench.py --serving [--requests 10000] [--rate 400] [--workers 8]
 """
 import argparse
 import hashlib
@@ -27,11 +28,13 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
-VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # int32 VALU: 256 CUs x 64 lanes x 2.4 GHz = 39.3 Tops/s
-# Integer ops of the commit transform per (ring element, CRT prime): 3,242 Montgomery
-# multiplications (2,594 twiddle/cube-root products of the 2,2,3,3,3,3 DIF + 648 MAC) at 4 ops and
-# 7,344 modular additions/subtractions at 2 ops.
-COMMIT_OPS_PER_ELEMENT_PRIME = 3242 * 4 + 7344 * 2
+VALU_SLOTS_T = 256 * 64 * 2.4e9 / 1e12  # int32 VALU issue: 256 CUs x 64 lanes x 2.4 GHz = 39.3 T/s
+# Commit roofline in the algorithmic unit of the reference's transform: per ring element and CRT
+# prime, 324 (Phi6 split) + 2*324 (radix 2) + 4*216*3 (radix 3 with the cube root) = 3,564 modular
+# products plus 648 MAC = 4,212 "mulmod-eq". Peak: the int32 VALU issue rate at 5 slots per
+# Montgomery product (two half-rate 32x32->64 multiply-adds and one 32-bit multiply).
+MULMOD_PER_ELEMENT_PRIME = 4212
+MULMOD_PEAK_T = VALU_SLOTS_T / 5
 
 
 def bench_seed(i: int) -> bytes:
@@ -54,6 +57,88 @@ def cpu_baseline(log_vars_target: int):
             "sample": f"CPU oracle whole proof at k={k} ({dt * 1000:.0f} ms, OpenMP), x{scale} work ratio to k={log_vars_target}"}
 
 
+def batched_instances(n=64, first=100):
+    """The batched workload: instance i has seed bench_seed(first + i) and k = 20 + seed[0] mod 9."""
+    return [(20 + bench_seed(first + i)[0] % 9, bench_seed(first + i)) for i in range(n)]
+
+
+def warm(sch, ks, workers):
+    """Context set-up outside the clock: `workers` contexts of each size, created concurrently."""
+    hs = [sch.submit(kk, bench_seed(999_000 + 100 * kk + j)) for kk in ks for j in range(workers)]
+    sch.drain()
+    for h in hs:
+        sch.result(h)
+
+
+def run_jobs(m, jobs, workers, budget=200 << 30):
+    """All jobs submitted at once to a scheduler with `workers` proofs in flight (contexts warmed
+    first, one per table size); returns (proofs, seconds)."""
+    sch = m.Scheduler(m.DEFAULT_Q, hbm_budget=budget, workers=workers)
+    try:
+        warm(sch, sorted({kk for kk, _ in jobs}), workers)
+        t0 = time.perf_counter()
+        hs = [sch.submit(kk, seed) for kk, seed in jobs]
+        sch.drain()
+        dt = time.perf_counter() - t0
+        return [sch.result(h)[0] for h in hs], dt
+    finally:
+        sch.close()
+
+
+def batched(m, workers):
+    """64 independent proofs of mixed size (k in [20, 28]) in flight together vs one at a time."""
+    jobs = batched_instances()
+    seq, t_seq = run_jobs(m, jobs, 1)
+    par, t_par = run_jobs(m, jobs, workers)
+    return {"instances": len(jobs), "k": sorted({kk for kk, _ in jobs}), "workers": workers,
+            "proofs_per_s": round(len(jobs) / t_par, 1), "sequential_proofs_per_s": round(len(jobs) / t_seq, 1),
+            "speedup": round(t_seq / t_par, 2), "identical_to_sequential": seq == par}
+
+
+def serving(m, requests, rate, workers, budget):
+    """Poisson arrivals of proving requests, k in {20..31} with P(k) ~ 2^-(k-20), fresh seed per
+    request; latency = completion - arrival. The same arrival sequence is served by `workers`
+    proofs in flight and by one at a time."""
+    import random
+    rng = random.Random(20261203)
+    ks = list(range(20, 32))
+    wts = [2.0 ** -(kk - 20) for kk in ks]
+    reqs, t = [], 0.0
+    for i in range(requests):
+        t += rng.expovariate(rate)
+        reqs.append((t, rng.choices(ks, wts)[0], bench_seed(2_000_000 + i)))
+
+    def serve(w):
+        sch = m.Scheduler(m.DEFAULT_Q, hbm_budget=budget, workers=w)
+        try:
+            warm(sch, ks, w)
+            hs = []
+            t0 = time.perf_counter()
+            for at, kk, seed in reqs:
+                delay = at - (time.perf_counter() - t0)
+                if delay > 0:
+                    time.sleep(delay)
+                lag = (time.perf_counter() - t0) - at  # submission behind the scheduled arrival
+                hs.append((lag, sch.submit(kk, seed)))
+            sch.drain()
+            wall = time.perf_counter() - t0
+            lat = sorted(lag * 1000 + (h.queue_us + h.run_us) / 1000 for lag, h in hs)
+            st = sch.stats()
+            proofs = [sch.result(h)[0] for _, h in hs[:32]]  # a sample, for the isolation check
+            return {"p50_ms": round(statistics.median(lat), 3), "p99_ms": round(lat[int(0.99 * (len(lat) - 1))], 3),
+                    "proofs_per_s": round(len(hs) / wall, 1), "wall_s": round(wall, 2),
+                    "contexts": st["contexts_created"], "evictions": st["evictions"],
+                    "peak_gib": round(st["peak_bytes"] / 2**30, 2)}, proofs
+        finally:
+            sch.close()
+
+    mix, proofs_mix = serve(workers)
+    seq, proofs_seq = serve(1)
+    return {"metric": "serving p50/p99 ms and proofs/s on 1 MI355X", "requests": requests, "rate_per_s": rate,
+            "workers": workers, "hbm_budget_gib": budget / 2**30, "batched": mix, "sequential": seq,
+            "identical_sample": proofs_mix == proofs_seq}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -61,6 +146,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--log-vars", type=int, default=31)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 64-instance batched measurement")
+    ap.add_argument("--workers", type=int, default=8, help="proofs in flight for the batched / serving runs")
+    ap.add_argument("--serving", action="store_true", help="run the serving loop instead (prints its own line)")
+    ap.add_argument("--requests", type=int, default=10_000)
+    ap.add_argument("--rate", type=float, default=400.0, help="serving arrivals per second")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the prover's kernels run on one device per proof; "
@@ -68,6 +158,9 @@ def main():
         sys.exit(2)
     import amd
     m = amd.load()
+    if args.serving:
+        print(json.dumps(serving(m, args.requests, args.rate, args.workers, 200 << 30)), flush=True)
+        return
     k = args.log_vars
     p = m.Prover(k)
     p.set_modulus(m.DEFAULT_Q)
@@ -96,9 +189,10 @@ def main():
     re = (1 << (k - 17)) // 4
     table_bytes = n_words * 16
     primes = p.commit_primes()
+    per_launch_ms = mean_t["commit_mac_ms"] / primes  # one k_commit_mac launch per CRT prime
+    mulmod_t = re * 1024 * MULMOD_PER_ELEMENT_PRIME / (per_launch_ms / 1e3) / 1e12
     passes = {
-        "commit_mac": {"ms": mean_t["commit_mac_ms"], "bound": "valu",
-                       "achieved_tops": re * 1024 * primes * COMMIT_OPS_PER_ELEMENT_PRIME / (mean_t["commit_mac_ms"] / 1e3) / 1e12},
+        "commit_mac": {"ms": mean_t["commit_mac_ms"], "bound": "valu"},
         "partials": {"ms": mean_t["eq_ms"] + mean_t["partials_ms"], "bound": "hbm",
                      "achieved_gbs": table_bytes / ((mean_t["eq_ms"] + mean_t["partials_ms"]) / 1e3) / 1e9},
         "switch": {"ms": mean_t["switch0_ms"] + mean_t["switch_rounds_ms"], "bound": "hbm",
@@ -108,16 +202,9 @@ def main():
                  "achieved_gbs": (table_bytes + re * 648 * 4) / (mean_t["fold_ms"] / 1e3) / 1e9},
         "commit_finish": {"ms": mean_t["commit_finish_ms"], "bound": "valu"},
     }
-    dom_name = max(passes, key=lambda n: passes[n]["ms"])
-    dom = passes[dom_name]
-    if dom["bound"] == "valu" and dom_name == "commit_mac":
-        roofline = {"bound": "valu", "kernel": "k_commit_mac", "achieved": round(dom["achieved_tops"], 3),
-                    "peak": round(VALU_PEAK_TOPS, 1), "unit": "Tops/s",
-                    "frac": round(dom["achieved_tops"] / VALU_PEAK_TOPS, 4), "traffic": None}
-    else:
-        roofline = {"bound": dom["bound"], "kernel": dom_name, "achieved": round(dom.get("achieved_gbs", 0), 1),
-                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(dom.get("achieved_gbs", 0) / HBM_PEAK_GBS, 4),
-                    "traffic": None}
+    roofline = {"bound": "valu", "kernel": "k_commit_mac", "achieved": round(mulmod_t, 3),
+                "peak": round(MULMOD_PEAK_T, 3), "unit": "Tmulmod/s", "frac": round(mulmod_t / MULMOD_PEAK_T, 4),
+                "traffic": None, "launch_ms": round(per_launch_ms, 3)}
     line = {
         "metric": "prover ms (commit+open, 2^31 evals) on 1 MI355X",
         "value": round(med, 3), "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -130,11 +217,16 @@ def main():
         "host_phases_ms": {f.replace("_us", ""): round(statistics.mean(pr[f] for pr in profs) / 1000, 3)
                            for f in profs[0]},
         "step_ms": [round(t, 3) for t in times],
+        "hbm_passes": {n: {"ms": round(v["ms"], 3), "gbs": round(v["achieved_gbs"], 1),
+                           "frac": round(v["achieved_gbs"] / HBM_PEAK_GBS, 4)}
+                       for n, v in passes.items() if v["bound"] == "hbm"},
     }
+    p.close()
+    if not args.no_batched:
+        line["batched"] = batched(m, args.workers)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(k)
     print(json.dumps(line), flush=True)
-    p.close()
 
 
 if __name__ == "__main__":

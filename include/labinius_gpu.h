@@ -117,6 +117,53 @@ int lbn_set_timing(lbn_ctx *ctx, int enable);
 int lbn_reset_timing(lbn_ctx *ctx);
 int lbn_get_timing(const lbn_ctx *ctx, lbn_timing *out);
 
+/* ----------------------------------------------------------------------------------------------
+ * Batched and continuous proving on one GPU. No counterpart in the reference, which proves one
+ * table per process (solution/proof_generate/src/main.rs:2050-2090); these serve many proofs of
+ * mixed table sizes at once: worker threads each run lbn_prove on a context of their own (own HIP
+ * streams), contexts are cached per table size, and jobs are admitted against an HBM budget
+ * (idle contexts of other sizes are evicted, least recently used first). Each proof equals the
+ * one lbn_prove gives for the same seed, table and Q alone.
+ * ---------------------------------------------------------------------------------------------- */
+
+/* Device bytes one context of 2^log_vars evaluations holds under modulus Q (0 if the shape or Q
+ * is invalid): the admission unit of the scheduler. */
+size_t lbn_ctx_footprint(int log_vars, uint64_t q_lo, uint64_t q_hi);
+
+typedef struct lbn_sched lbn_sched;
+
+typedef struct {
+    int log_vars;        /* 19..31 */
+    uint8_t seed[32];    /* proof seed; also the table seed when table is NULL */
+    const void *table;   /* optional host witness table (2^log_vars / 8 bytes); NULL: generated on
+                            the device from seed (verifier/table_writer/src/main.rs:14-22) */
+    uint8_t *proof;      /* caller-owned output, at least lbn_proof_len bytes */
+    size_t cap;
+    size_t len;          /* out: proof length */
+    int status;          /* out: 1 while queued or running, then 0 or a negative LBN_E_* code */
+    double queue_us;     /* out: submission to admission */
+    double run_us;       /* out: admission to completion (context set-up, table, proof) */
+} lbn_job;
+
+typedef struct {
+    uint64_t submitted, completed, failed, contexts_created, evictions;
+    uint64_t peak_bytes; /* most device bytes reserved at once */
+    uint64_t used_bytes; /* reserved now */
+} lbn_sched_stats;
+
+/* workers: proofs in flight at once (1..64); hbm_budget: device bytes the contexts may hold. */
+int lbn_sched_create(int device, uint64_t q_lo, uint64_t q_hi, size_t hbm_budget, int workers,
+                     lbn_sched **out);
+/* Queues a job (it must stay valid until finished); LBN_E_RANGE if it can never fit the budget. */
+int lbn_sched_submit(lbn_sched *s, lbn_job *job);
+/* Blocks until the job has finished; returns its status. */
+int lbn_sched_wait(lbn_sched *s, const lbn_job *job);
+/* Blocks until every submitted job has finished. */
+int lbn_sched_drain(lbn_sched *s);
+int lbn_sched_get_stats(lbn_sched *s, lbn_sched_stats *out);
+/* Drains, stops the workers and frees every context. */
+void lbn_sched_destroy(lbn_sched *s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -40,6 +40,17 @@ class Timing(ctypes.Structure):
                [("switch_rounds", ctypes.c_int)]
 
 
+class Job(ctypes.Structure):
+    _fields_ = [("log_vars", ctypes.c_int), ("seed", ctypes.c_uint8 * 32), ("table", ctypes.c_void_p),
+                ("proof", ctypes.c_void_p), ("cap", ctypes.c_size_t), ("len", ctypes.c_size_t),
+                ("status", ctypes.c_int), ("queue_us", ctypes.c_double), ("run_us", ctypes.c_double)]
+
+
+class SchedStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in ("submitted", "completed", "failed", "contexts_created", "evictions",
+                                              "peak_bytes", "used_bytes")]
+
+
 _lib = None
 
 
@@ -74,6 +85,16 @@ def lib():
         L.lbn_set_timing.argtypes = [P, ctypes.c_int]
         L.lbn_reset_timing.argtypes = [P]
         L.lbn_get_timing.argtypes = [P, P]
+        L.lbn_ctx_footprint.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64]
+        L.lbn_ctx_footprint.restype = ctypes.c_size_t
+        L.lbn_sched_create.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_size_t, ctypes.c_int,
+                                       ctypes.POINTER(P)]
+        L.lbn_sched_submit.argtypes = [P, P]
+        L.lbn_sched_wait.argtypes = [P, P]
+        L.lbn_sched_drain.argtypes = [P]
+        L.lbn_sched_get_stats.argtypes = [P, P]
+        L.lbn_sched_destroy.argtypes = [P]
+        L.lbn_sched_destroy.restype = None
         _lib = L
     return _lib
 
@@ -213,3 +234,75 @@ def exported_symbols():
     import re
     text = open(HEADER).read()
     return sorted(set(re.findall(r"\b(lbn_[a-z_0-9]+)\s*\(", text)))
+
+
+def ctx_footprint(log_vars: int, q: int = DEFAULT_Q) -> int:
+    """Device bytes of one context (the scheduler's admission unit)."""
+    return lib().lbn_ctx_footprint(log_vars, q & (2**64 - 1), q >> 64)
+
+
+class Scheduler:
+    """Batched / continuous proving (lbn_sched_*): `workers` proofs in flight, contexts cached per
+    table size under an HBM budget. submit() returns a handle; each finished proof equals the one a
+    single Prover gives for the same seed and table."""
+
+    def __init__(self, q: int = DEFAULT_Q, hbm_budget: int = 200 << 30, workers: int = 8, device: int = 0):
+        self.q = q
+        h = ctypes.c_void_p()
+        rc = lib().lbn_sched_create(device, q & (2**64 - 1), q >> 64, hbm_budget, workers, ctypes.byref(h))
+        if rc != 0:
+            raise LbnError(rc, "lbn_sched_create failed")
+        self.h = h
+        self._live = {}
+
+    def submit(self, log_vars: int, seed: bytes, table: bytes = None):
+        """Queues one proof; returns a handle for wait()/result(). The table (2^log_vars / 8 bytes)
+        is generated on the device from the seed when not given."""
+        job = Job()
+        job.log_vars = log_vars
+        ctypes.memmove(job.seed, seed, 32)
+        keep = [job]
+        if table is not None:
+            buf = ctypes.create_string_buffer(table, len(table))
+            keep.append(buf)
+            job.table = ctypes.cast(buf, ctypes.c_void_p)
+        out = np.empty(proof_len(log_vars, self.q), dtype=np.uint8)
+        keep.append(out)
+        job.proof = out.ctypes.data
+        job.cap = out.size
+        rc = lib().lbn_sched_submit(self.h, ctypes.addressof(job))
+        if rc != 0:
+            raise LbnError(rc, "lbn_sched_submit failed")
+        self._live[id(job)] = keep
+        return job
+
+    def wait(self, job):
+        return lib().lbn_sched_wait(self.h, ctypes.addressof(job))
+
+    def drain(self):
+        lib().lbn_sched_drain(self.h)
+
+    def result(self, job):
+        """(proof bytes, queue_us, run_us) of a finished job; raises on failure."""
+        rc = self.wait(job)
+        keep = self._live.pop(id(job))
+        if rc != 0:
+            raise LbnError(rc, "proof job failed")
+        return keep[-1][: job.len].tobytes(), job.queue_us, job.run_us
+
+    def stats(self) -> dict:
+        st = SchedStats()
+        lib().lbn_sched_get_stats(self.h, ctypes.addressof(st))
+        return {f: getattr(st, f) for f, _ in SchedStats._fields_}
+
+    def close(self):
+        if self.h:
+            lib().lbn_sched_destroy(self.h)
+            self.h = None
+            self._live.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -40,7 +40,7 @@ def build(verbose=False):
         obj = os.path.join(OUT, src + ".o")
         run([HIPCC, *DEVICE_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
-    for src in ("lbn_host.cpp", "lbn_blake3.cpp"):
+    for src in ("lbn_host.cpp", "lbn_blake3.cpp", "lbn_sched.cpp"):
         obj = os.path.join(OUT, src + ".o")
         run(["g++", *HOST_FLAGS, "-I/opt/rocm/include", "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)

@@ -14,6 +14,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
 #include <vector>
 
 #include "lbn_internal.h"
@@ -50,12 +53,13 @@ struct lbn::DeviceState {
     uint8_t *cbytes = nullptr;      // 1024 * 648 * width
     uint64_t *amat = nullptr;       // RE * 648 * 2 (staging for the matrix)
     size_t ahat_elems = 0;
+    int alloc_primes = 0, alloc_width = 0;  // sizes of cacc / cbytes / cvs / hstage
     uint32_t *cvs = nullptr;        // chunk chaining values of the commitment bytes
     // Pinned host staging: the commitment bytes (copied on copy_stream once the commitment is
-    // ready, while the opening runs) and the fold output.
+    // ready, while the opening runs) and the fold output, which the fold kernel writes directly.
     uint8_t *hstage = nullptr;
-    size_t hstage_fold = 0;         // byte offset of the fold region
-    int32_t *fold_host = nullptr;   // device address of the fold region: the fold kernel writes it
+    int32_t *hfold = nullptr;
+    int32_t *fold_host = nullptr;   // device address of hfold
     hipStream_t copy_stream = nullptr;
     hipEvent_t ev_commit = nullptr, ev_staged = nullptr;
     hipEvent_t ev[2 * 9] = {};
@@ -1111,12 +1115,8 @@ int lbn::dev_init(lbn_ctx *ctx) {
     HIPCHK(ctx, hipMalloc(&d->fold_off, (2 * 162 + 1) * sizeof(int)));
     HIPCHK(ctx, hipMalloc(&d->fold_cols, (size_t)COLUMNS * 32 * sizeof(uint16_t)));
     HIPCHK(ctx, hipMalloc(&d->ntt_tabs, (size_t)MAX_PRIMES * tab::SIZE * 4));
-    HIPCHK(ctx, hipMalloc(&d->cacc, (size_t)MAX_PRIMES * COLUMNS * N * 8));
-    HIPCHK(ctx, hipMalloc(&d->cbytes, (size_t)COLUMNS * N * 16));
-    HIPCHK(ctx, hipMalloc(&d->cvs, (size_t)COLUMNS * N * 16 / 1024 * 32 + 64));
-    d->hstage_fold = (size_t)COLUMNS * N * 16;
-    HIPCHK(ctx, hipHostMalloc((void **)&d->hstage, d->hstage_fold + s.ring_elements * N * sizeof(int32_t)));
-    HIPCHK(ctx, hipHostGetDevicePointer((void **)&d->fold_host, d->hstage + d->hstage_fold, 0));
+    HIPCHK(ctx, hipHostMalloc((void **)&d->hfold, s.ring_elements * N * sizeof(int32_t)));
+    HIPCHK(ctx, hipHostGetDevicePointer((void **)&d->fold_host, d->hfold, 0));
     HIPCHK(ctx, hipStreamCreateWithFlags(&d->copy_stream, hipStreamNonBlocking));
     HIPCHK(ctx, hipEventCreateWithFlags(&d->ev_commit, hipEventDisableTiming));
     HIPCHK(ctx, hipEventCreateWithFlags(&d->ev_staged, hipEventDisableTiming));
@@ -1135,6 +1135,7 @@ void lbn::dev_free(lbn_ctx *ctx) {
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (d->hstage) hipHostFree(d->hstage);
+    if (d->hfold) hipHostFree(d->hfold);
     for (auto &e : d->ev)
         if (e) hipEventDestroy(e);
     if (d->ev_commit) hipEventDestroy(d->ev_commit);
@@ -1204,6 +1205,91 @@ extern "C" int lbn_generate_table(lbn_ctx *ctx, const uint8_t seed[32]) {
 
 static u128 mul_mod_small(u128 a, uint32_t b, u128 m) { return (a * b) % m; }
 
+// Primes: the largest p = 1 (mod 1944) with 6p < 2^32, each with a primitive 1944-th root.
+static const PrimeInfo CRT_PRIMES[MAX_PRIMES] = {{715809961u, 0, 0, 181294596u},
+                                                 {715808017u, 0, 0, 501708134u},
+                                                 {715806073u, 0, 0, 148596466u},
+                                                 {715757473u, 0, 0, 282162646u},
+                                                 {715722481u, 0, 0, 150155317u}};
+
+// CRT primes for the exact commitment of `re` ring elements per column under modulus q (0 if
+// the range cannot be covered). With the matrix centred (|a| <= Q/2) and at most 512 set bits
+// per element, a coefficient of one product is at most 3 * 512 * Q/2 = 768 Q, so |coefficient|
+// <= re * 768 * Q; the CRT range must exceed twice that. Counted in doubles (M can pass 2^128).
+static int crt_prime_count(u128 q, size_t re) {
+    const double need_bits = std::log2(2.0 * (double)re * 768.0) + std::log2((double)q) + 0.01;
+    int np = 0;
+    double have = 0;
+    while (np < MAX_PRIMES && have <= need_bits) have += std::log2((double)CRT_PRIMES[np++].p);
+    return have > need_bits ? np : 0;
+}
+
+static int modulus_width(u128 q) {
+    int bits = 0;
+    for (u128 x = q - 1; x; x >>= 1) bits++;
+    return (bits + 7) / 8;
+}
+
+// Device bytes of the buffers that scale with the CRT prime count and the coefficient width.
+static size_t commit_buffer_bytes(int np, int width) {
+    const size_t clen = (size_t)COLUMNS * N * width;
+    return (size_t)np * COLUMNS * N * 8 + (clen + 64) + (clen / 1024 * 32 + 64);
+}
+
+extern "C" size_t lbn_ctx_footprint(int log_vars, uint64_t q_lo, uint64_t q_hi) {
+    Shape s;
+    if (!Shape::of(log_vars, s)) return 0;
+    const u128 q = ((u128)q_hi << 64) | q_lo;
+    if (q < (u128)34816965049ull || q >= ((u128)1 << 96)) return 0;
+    const int np = crt_prime_count(q, s.ring_elements);
+    if (!np) return 0;
+    const size_t n = s.n_words;
+    size_t b = n * 16 + n * 16 + n / 2 * 16 + n / 2 * 24 + n / 4 * 24;  // table, E0, E1, P0, P1
+    b += 3584 * 4 + ((size_t)1 << (s.log_packed - 12)) * 16 + 4096 * 16;  // T, EH, EL
+    b += s.column_len * 24 + 256 * 24 + (size_t)COLUMNS * 128 * 24 + (size_t)COLUMNS * 24;
+    b += 4 + (2 * 162 + 1) * sizeof(int) + (size_t)COLUMNS * 32 * sizeof(uint16_t);
+    b += (size_t)MAX_PRIMES * tab::SIZE * 4;
+    b += (size_t)np * s.ring_elements * N * 4;  // ahat
+    b += s.ring_elements * N * 16;              // matrix staging while the modulus is set
+    b += commit_buffer_bytes(np, modulus_width(q));
+    return b;
+}
+
+// The expanded matrix depends only on (Q, ring elements); contexts of one shape share it.
+static std::shared_ptr<const std::vector<u128>> public_matrix(u128 q, size_t re) {
+    static std::mutex m;
+    static std::map<std::pair<u128, size_t>, std::shared_ptr<const std::vector<u128>>> cache;
+    {
+        std::lock_guard<std::mutex> lk(m);
+        auto it = cache.find({q, re});
+        if (it != cache.end()) return it->second;
+    }
+    auto A = std::make_shared<std::vector<u128>>(re * N);
+    ajtai_matrix_host(q, re, A->data());
+    std::lock_guard<std::mutex> lk(m);
+    if (cache.size() >= 16) cache.clear();
+    return cache.emplace(std::make_pair(q, re), A).first->second;
+}
+
+static int alloc_commit_buffers(lbn_ctx *ctx, int np, int width) {
+    DeviceState *d = ctx->dev;
+    if (d->alloc_primes >= np && d->alloc_width >= width) return LBN_OK;
+    HIPCHK(ctx, hipStreamSynchronize(d->copy_stream));
+    for (void *p : {(void *)d->cacc, (void *)d->cbytes, (void *)d->cvs})
+        if (p) HIPCHK(ctx, hipFree(p));
+    if (d->hstage) HIPCHK(ctx, hipHostFree(d->hstage));
+    d->cacc = nullptr, d->cbytes = nullptr, d->cvs = nullptr, d->hstage = nullptr;
+    d->alloc_primes = d->alloc_width = 0;
+    const size_t clen = (size_t)COLUMNS * N * width;
+    HIPCHK(ctx, hipMalloc(&d->cacc, (size_t)np * COLUMNS * N * 8));
+    HIPCHK(ctx, hipMalloc(&d->cbytes, clen + 64));  // slack: the chunk hashing reads one word past
+    HIPCHK(ctx, hipMalloc(&d->cvs, clen / 1024 * 32 + 64));
+    HIPCHK(ctx, hipHostMalloc((void **)&d->hstage, clen));
+    d->alloc_primes = np;
+    d->alloc_width = width;
+    return LBN_OK;
+}
+
 extern "C" int lbn_set_modulus(lbn_ctx *ctx, uint64_t q_lo, uint64_t q_hi) {
     if (!ctx) return LBN_E_ARG;
     const u128 q = ((u128)q_hi << 64) | q_lo;
@@ -1212,24 +1298,17 @@ extern "C" int lbn_set_modulus(lbn_ctx *ctx, uint64_t q_lo, uint64_t q_hi) {
     HIPCHK(ctx, hipSetDevice(ctx->device));
     DeviceState *d = ctx->dev;
     const Shape &s = ctx->shape;
-    // Primes: the largest p = 1 (mod 1944) with 6p < 2^32, each with a primitive 1944-th root.
-    static const PrimeInfo PR[MAX_PRIMES] = {{715809961u, 0, 0, 181294596u},
-                                             {715808017u, 0, 0, 501708134u},
-                                             {715806073u, 0, 0, 148596466u},
-                                             {715757473u, 0, 0, 282162646u},
-                                             {715722481u, 0, 0, 150155317u}};
-    // With the matrix centred (|a| <= Q/2) and at most 512 set bits per element, a coefficient of
-    // one product is at most 3 * 512 * Q/2 = 768 Q, so |coefficient| <= RE * 768 * Q; the CRT range
-    // must exceed twice that. Counted in doubles (M can pass 2^128): log2 M > log2(2 bound) + margin.
-    const double need_bits = std::log2(2.0 * (double)s.ring_elements * 768.0) + std::log2((double)q) + 0.01;
-    int np = 0;
-    double have = 0;
-    while (np < MAX_PRIMES && have <= need_bits) have += std::log2((double)PR[np++].p);
-    if (have <= need_bits) return ctx->fail(LBN_E_MODULUS, "modulus too large for the CRT range");
+    const int np = crt_prime_count(q, s.ring_elements);
+    if (!np) return ctx->fail(LBN_E_MODULUS, "modulus too large for the CRT range");
+    ctx->modulus_set = false;
+    {
+        const int rc = alloc_commit_buffers(ctx, np, modulus_width(q));
+        if (rc != LBN_OK) return rc;
+    }
     ctx->nprimes = np;
     std::vector<uint32_t> all((size_t)MAX_PRIMES * tab::SIZE, 0);
     for (int k = 0; k < np; k++) {
-        PrimeInfo pi = PR[k];
+        PrimeInfo pi = CRT_PRIMES[k];
         uint32_t inv = 1;  // p^-1 mod 2^32 by Newton
         for (int it = 0; it < 5; it++) inv *= 2 - pi.p * inv;
         pi.pinv_neg = (uint32_t)(0u - inv);
@@ -1241,10 +1320,9 @@ extern "C" int lbn_set_modulus(lbn_ctx *ctx, uint64_t q_lo, uint64_t q_hi) {
     }
     HIPCHK(ctx, hipMemcpyAsync(d->ntt_tabs, all.data(), all.size() * 4, hipMemcpyHostToDevice, d->stream));
     // The public matrix (rq.rs:114-144), expanded on the host, reduced and transformed here.
-    std::vector<u128> A(s.ring_elements * N);
-    ajtai_matrix_host(q, s.ring_elements, A.data());
-    if (!d->amat) HIPCHK(ctx, hipMalloc(&d->amat, s.ring_elements * N * 16));
-    HIPCHK(ctx, hipMemcpyAsync(d->amat, A.data(), A.size() * 16, hipMemcpyHostToDevice, d->stream));
+    const std::shared_ptr<const std::vector<u128>> A = public_matrix(q, s.ring_elements);
+    HIPCHK(ctx, hipMalloc(&d->amat, s.ring_elements * N * 16));
+    HIPCHK(ctx, hipMemcpyAsync(d->amat, A->data(), A->size() * 16, hipMemcpyHostToDevice, d->stream));
     const size_t need = (size_t)np * s.ring_elements * N;
     if (d->ahat_elems < need) {
         if (d->ahat) HIPCHK(ctx, hipFree(d->ahat));
@@ -1265,10 +1343,10 @@ extern "C" int lbn_set_modulus(lbn_ctx *ctx, uint64_t q_lo, uint64_t q_hi) {
         HIPCHK(ctx, hipGetLastError());
     }
     HIPCHK(ctx, hipStreamSynchronize(d->stream));
+    HIPCHK(ctx, hipFree(d->amat));  // only the transform needs the matrix itself
+    d->amat = nullptr;
     ctx->q = q;
-    int bits = 0;
-    for (u128 x = q - 1; x; x >>= 1) bits++;
-    ctx->width = (bits + 7) / 8;
+    ctx->width = modulus_width(q);
     ctx->modulus_set = true;
     (void)mul_mod_small;
     return LBN_OK;
@@ -1674,7 +1752,7 @@ extern "C" int lbn_fold(lbn_ctx *ctx, const lbn_challenge *ch, int n_challenges,
     // The kernel wrote the fold straight into pinned host memory over the host link, overlapped
     // with its own work; once the stream is idle the values are visible here.
     uint32_t flag = 0;
-    const uint8_t *stage = d->hstage + d->hstage_fold;
+    const uint8_t *stage = (const uint8_t *)d->hfold;
     HIPCHK(ctx, hipMemcpyAsync(&flag, d->fold_flag, 4, hipMemcpyDeviceToHost, d->stream));
     HIPCHK(ctx, hipStreamSynchronize(d->stream));
     tcollect(ctx, T_FOLD);

@@ -100,7 +100,8 @@ int host_threads() {
 }
 
 // A persistent pool: workers sleep on a condition variable between jobs; a job hands out
-// indices through an atomic counter, and the caller works on it too. One job at a time.
+// indices through an atomic counter, and the caller works on it too. One job at a time: a caller
+// that finds the pool busy (another proof on another thread) runs its loop itself.
 namespace {
 class Pool {
    public:
@@ -108,8 +109,10 @@ class Pool {
         static Pool pool;
         return pool;
     }
-    void run(size_t n, const std::function<void(size_t)> &f) {
-        std::lock_guard<std::mutex> job_lock(job_);
+    // False (nothing run) when another thread's job holds the pool.
+    bool try_run(size_t n, const std::function<void(size_t)> &f) {
+        std::unique_lock<std::mutex> job_lock(job_, std::try_to_lock);
+        if (!job_lock.owns_lock()) return false;
         {
             std::lock_guard<std::mutex> lk(m_);
             f_ = &f;
@@ -123,6 +126,7 @@ class Pool {
         std::unique_lock<std::mutex> lk(m_);
         done_.wait(lk, [&] { return active_ == 0; });
         f_ = nullptr;
+        return true;
     }
     ~Pool() {
         {
@@ -171,7 +175,8 @@ void parallel_for(size_t n, const std::function<void(size_t)> &f) {
         for (size_t i = 0; i < n; i++) f(i);
         return;
     }
-    Pool::get().run(n, f);
+    if (!Pool::get().try_run(n, f))
+        for (size_t i = 0; i < n; i++) f(i);
 }
 
 // ============================================================================= transcript

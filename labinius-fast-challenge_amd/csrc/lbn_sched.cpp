@@ -1,0 +1,239 @@
+// Batched and continuous proving on one GPU (the scheduler half of liblabinius_gpu).
+//
+// Worker threads each drive one proof at a time through the whole-proof path (lbn_prove) on a
+// context of their own, so every proof in flight has its own HIP streams: one proof's kernels run
+// while another's host phases (transcript, challenge sampling, the switch round trips) run on
+// other cores, and kernels of different proofs share the device. Contexts are cached per table
+// size and admitted against an HBM budget: a job takes an idle context of its size, else a new
+// one if its footprint fits, else idle contexts of other sizes are evicted (least recently used
+// first); a job that fits nowhere waits until a running proof releases memory. Admission is in
+// arrival order with bounded overtaking: a later job may start on a free context or free memory
+// while the first waits, but the first cannot be overtaken more than MAX_BYPASS times.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "lbn_internal.h"
+
+namespace {
+
+constexpr int MAX_BYPASS = 64;
+
+double now_us() {
+    return std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct Slot {
+    lbn_ctx *ctx = nullptr;
+    int log_vars = 0;
+    size_t bytes = 0;
+    bool busy = false;
+    bool ready = false;  // created and modulus set
+    uint64_t last_used = 0;
+};
+
+struct Entry {
+    lbn_job *job;
+    double submitted;
+    int bypassed = 0;
+};
+
+}  // namespace
+
+struct lbn_sched {
+    int device = 0;
+    uint64_t q_lo = 0, q_hi = 0;
+    size_t budget = 0;
+    std::mutex m;
+    std::condition_variable wake, done;
+    std::deque<Entry> queue;
+    std::vector<Slot *> slots;
+    std::vector<std::thread> workers;
+    size_t used = 0;
+    size_t pending = 0;
+    uint64_t tick = 0;
+    bool stop = false;
+    lbn_sched_stats stats{};
+
+    // Under m: pick a job and a slot for it, or return false. Reserves memory for a new slot and
+    // collects evicted contexts in `evict` (destroyed by the caller outside the lock).
+    bool admit(Entry &out, Slot *&slot, std::vector<lbn_ctx *> &evict) {
+        for (size_t qi = 0; qi < queue.size(); qi++) {
+            Entry &e = queue[qi];
+            const bool front = qi == 0;
+            if (!front && queue.front().bypassed >= MAX_BYPASS) break;
+            const int k = e.job->log_vars;
+            Slot *idle = nullptr;
+            for (Slot *s : slots)
+                if (!s->busy && s->ready && s->log_vars == k && (!idle || s->last_used > idle->last_used)) idle = s;
+            if (!idle) {
+                const size_t need = lbn_ctx_footprint(k, q_lo, q_hi);
+                if (used + need > budget) {
+                    if (!front) continue;
+                    // the first job may evict idle contexts of other sizes, least recently used first
+                    std::vector<Slot *> cand;
+                    for (Slot *s : slots)
+                        if (!s->busy && s->ready) cand.push_back(s);
+                    std::sort(cand.begin(), cand.end(), [](Slot *a, Slot *b) { return a->last_used < b->last_used; });
+                    size_t freed = 0;
+                    size_t take = 0;
+                    while (take < cand.size() && used - freed + need > budget) freed += cand[take++]->bytes;
+                    if (used - freed + need > budget) return false;  // wait for running proofs
+                    for (size_t i = 0; i < take; i++) {
+                        Slot *s = cand[i];
+                        evict.push_back(s->ctx);
+                        used -= s->bytes;
+                        slots.erase(std::find(slots.begin(), slots.end(), s));
+                        delete s;
+                        stats.evictions++;
+                    }
+                }
+                idle = new Slot();
+                idle->log_vars = k;
+                idle->bytes = need;
+                used += need;
+                stats.peak_bytes = std::max<uint64_t>(stats.peak_bytes, used);
+                slots.push_back(idle);
+            }
+            idle->busy = true;
+            idle->last_used = ++tick;
+            slot = idle;
+            out = e;
+            for (size_t i = 0; i < qi; i++) queue[i].bypassed++;
+            queue.erase(queue.begin() + (long)qi);
+            return true;
+        }
+        return false;
+    }
+
+    void run_job(Entry &e, Slot *slot) {
+        lbn_job *job = e.job;
+        const double start = now_us();
+        int rc = LBN_OK;
+        if (!slot->ready) {
+            lbn_ctx *ctx = nullptr;
+            rc = lbn_create(device, job->log_vars, &ctx);
+            if (rc == LBN_OK) rc = lbn_set_modulus(ctx, q_lo, q_hi);
+            if (rc == LBN_OK) {
+                slot->ctx = ctx;
+                slot->ready = true;
+                std::lock_guard<std::mutex> lk(m);
+                stats.contexts_created++;
+            } else if (ctx) {
+                lbn_destroy(ctx);
+            }
+        }
+        if (rc == LBN_OK) {
+            lbn_ctx *ctx = slot->ctx;
+            rc = job->table ? lbn_load_table(ctx, job->table, ((size_t)1 << job->log_vars) / 8)
+                            : lbn_generate_table(ctx, job->seed);
+            if (rc == LBN_OK) rc = lbn_prove(ctx, job->seed, job->proof, job->cap, &job->len, nullptr);
+        }
+        const double end = now_us();
+        std::lock_guard<std::mutex> lk(m);
+        job->queue_us = start - e.submitted;
+        job->run_us = end - start;
+        job->status = rc;
+        if (slot->ready) {
+            slot->busy = false;
+        } else {  // creation failed: give the reservation back
+            used -= slot->bytes;
+            slots.erase(std::find(slots.begin(), slots.end(), slot));
+            delete slot;
+        }
+        stats.completed++;
+        if (rc != LBN_OK) stats.failed++;
+        pending--;
+        done.notify_all();
+        wake.notify_all();
+    }
+
+    void loop() {
+        for (;;) {
+            Entry e{};
+            Slot *slot = nullptr;
+            std::vector<lbn_ctx *> evict;
+            {
+                std::unique_lock<std::mutex> lk(m);
+                wake.wait(lk, [&] { return stop || (!queue.empty() && admit(e, slot, evict)); });
+                if (!slot) return;  // stop with nothing admitted
+            }
+            for (lbn_ctx *c : evict) lbn_destroy(c);
+            run_job(e, slot);
+        }
+    }
+};
+
+extern "C" int lbn_sched_create(int device, uint64_t q_lo, uint64_t q_hi, size_t hbm_budget, int workers,
+                                lbn_sched **out) {
+    if (!out || workers < 1 || workers > 64 || hbm_budget == 0) return LBN_E_ARG;
+    *out = nullptr;
+    if (lbn_ctx_footprint(31, q_lo, q_hi) == 0 && lbn_ctx_footprint(20, q_lo, q_hi) == 0) return LBN_E_MODULUS;
+    auto *s = new lbn_sched();
+    s->device = device;
+    s->q_lo = q_lo;
+    s->q_hi = q_hi;
+    s->budget = hbm_budget;
+    for (int i = 0; i < workers; i++) s->workers.emplace_back([s] { s->loop(); });
+    *out = s;
+    return LBN_OK;
+}
+
+extern "C" int lbn_sched_submit(lbn_sched *s, lbn_job *job) {
+    if (!s || !job || !job->proof) return LBN_E_ARG;
+    const size_t need = lbn_ctx_footprint(job->log_vars, s->q_lo, s->q_hi);
+    if (need == 0) return LBN_E_ARG;
+    std::lock_guard<std::mutex> lk(s->m);
+    if (s->stop) return LBN_E_STATE;
+    if (need > s->budget) return LBN_E_RANGE;  // could never be admitted
+    job->status = 1;
+    job->len = 0;
+    job->queue_us = job->run_us = 0;
+    s->queue.push_back(Entry{job, now_us(), 0});
+    s->pending++;
+    s->stats.submitted++;
+    s->wake.notify_one();
+    return LBN_OK;
+}
+
+extern "C" int lbn_sched_wait(lbn_sched *s, const lbn_job *job) {
+    if (!s || !job) return LBN_E_ARG;
+    std::unique_lock<std::mutex> lk(s->m);
+    s->done.wait(lk, [&] { return job->status != 1; });
+    return job->status;
+}
+
+extern "C" int lbn_sched_drain(lbn_sched *s) {
+    if (!s) return LBN_E_ARG;
+    std::unique_lock<std::mutex> lk(s->m);
+    s->done.wait(lk, [&] { return s->pending == 0; });
+    return LBN_OK;
+}
+
+extern "C" int lbn_sched_get_stats(lbn_sched *s, lbn_sched_stats *out) {
+    if (!s || !out) return LBN_E_ARG;
+    std::lock_guard<std::mutex> lk(s->m);
+    *out = s->stats;
+    out->used_bytes = s->used;
+    return LBN_OK;
+}
+
+extern "C" void lbn_sched_destroy(lbn_sched *s) {
+    if (!s) return;
+    lbn_sched_drain(s);
+    {
+        std::lock_guard<std::mutex> lk(s->m);
+        s->stop = true;
+    }
+    s->wake.notify_all();
+    for (auto &t : s->workers) t.join();
+    for (Slot *slot : s->slots) {
+        if (slot->ctx) lbn_destroy(slot->ctx);
+        delete slot;
+    }
+    delete s;
+}

@@ -29,3 +29,32 @@ def test_bad_shape_is_rejected_without_a_device():
     assert m.lib().lbn_create(0, 18, ctypes.byref(h)) == -1
     assert m.lib().lbn_create(0, 32, ctypes.byref(h)) == -1
     assert m.proof_len(18) == 0
+
+
+def test_context_footprint_without_a_device():
+    m = amd.load()
+    sizes = [m.ctx_footprint(k) for k in range(19, 32)]
+    assert all(a < b for a, b in zip(sizes, sizes[1:]))
+    # the table, its eq expansion and the switch buffers dominate at k = 31: 2^24 words x 16 B x 2.5
+    # + 2^23 x 24 B x 1.5
+    assert sizes[-1] > (1 << 24) * 16 * 2.5 + (1 << 23) * 24 * 1.5
+    assert m.ctx_footprint(31, (1 << 95) + 12345) > m.ctx_footprint(31)  # more CRT primes
+    assert m.ctx_footprint(18) == 0 and m.ctx_footprint(31, 1 << 20) == 0
+
+
+def test_scheduler_lifecycle_and_errors_without_a_device():
+    m = amd.load()
+    import ctypes
+    h = ctypes.c_void_p()
+    assert m.lib().lbn_sched_create(0, 1 << 20, 0, 1 << 30, 2, ctypes.byref(h)) == -4  # Q below the gate
+    assert m.lib().lbn_sched_create(0, m.DEFAULT_Q, 0, 1 << 30, 0, ctypes.byref(h)) == -1
+    sch = m.Scheduler(m.DEFAULT_Q, hbm_budget=512 << 20, workers=2)
+    try:
+        import pytest
+        with pytest.raises(m.LbnError):
+            sch.submit(31, bytes(32))  # never fits the budget
+        with pytest.raises(m.LbnError):
+            sch.submit(40, bytes(32))
+        assert sch.stats()["submitted"] == 0
+    finally:
+        sch.close()

@@ -143,6 +143,7 @@ typedef struct {
     int status;          /* out: 1 while queued or running, then 0 or a negative LBN_E_* code */
     double queue_us;     /* out: submission to admission */
     double run_us;       /* out: admission to completion (context set-up, table, proof) */
+    lbn_profile profile; /* out: the proof's host phases (lbn_prove) */
 } lbn_job;
 
 typedef struct {

@@ -34,16 +34,19 @@ class Profile(ctypes.Structure):
                                               "challenges_us", "fold_us", "encode_us", "total_us")]
 
 
+class Job(ctypes.Structure):
+    _fields_ = [("log_vars", ctypes.c_int), ("seed", ctypes.c_uint8 * 32), ("table", ctypes.c_void_p),
+                ("proof", ctypes.c_void_p), ("cap", ctypes.c_size_t), ("len", ctypes.c_size_t),
+                ("status", ctypes.c_int), ("queue_us", ctypes.c_double), ("run_us", ctypes.c_double),
+                ("profile", Profile)]
+
+
 class Timing(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in ("table_ms", "eq_ms", "partials_ms", "switch0_ms", "switch_rounds_ms",
                                               "row_ms", "fold_ms", "commit_mac_ms", "commit_finish_ms")] + \
                [("switch_rounds", ctypes.c_int)]
 
 
-class Job(ctypes.Structure):
-    _fields_ = [("log_vars", ctypes.c_int), ("seed", ctypes.c_uint8 * 32), ("table", ctypes.c_void_p),
-                ("proof", ctypes.c_void_p), ("cap", ctypes.c_size_t), ("len", ctypes.c_size_t),
-                ("status", ctypes.c_int), ("queue_us", ctypes.c_double), ("run_us", ctypes.c_double)]
 
 
 class SchedStats(ctypes.Structure):

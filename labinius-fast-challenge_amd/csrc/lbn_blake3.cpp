@@ -175,13 +175,13 @@ void Blake3::update_with(const uint8_t *p, size_t len,
     update(p + whole * CHUNK, rest - whole * CHUNK);
 }
 
-void Blake3::xof(uint64_t offset, uint8_t *out, size_t len) const {
-    uint32_t cv[8], m[16], blen, flags;
+Blake3::Root Blake3::root() const {
+    Root r;
     if (depth_ == 0) {
-        memcpy(cv, cv_, 32);
-        words16(buf_, buf_len_, m);
-        blen = (uint32_t)buf_len_;
-        flags = flags_ | CHUNK_END | (blocks_ == 0 ? CHUNK_START : 0);
+        memcpy(r.cv, cv_, 32);
+        words16(buf_, buf_len_, r.m);
+        r.len = (uint32_t)buf_len_;
+        r.flags = flags_ | CHUNK_END | (blocks_ == 0 ? CHUNK_START : 0);
     } else {
         uint32_t right[8];
         chunk_cv(right);
@@ -192,20 +192,29 @@ void Blake3::xof(uint64_t offset, uint8_t *out, size_t len) const {
             compress(key_, mm, 0, 64, flags_ | PARENT, o);
             memcpy(right, o, 32);
         }
-        memcpy(cv, key_, 32);
-        memcpy(m, stack_[0], 32);
-        memcpy(m + 8, right, 32);
-        blen = 64;
-        flags = flags_ | PARENT;
+        memcpy(r.cv, key_, 32);
+        memcpy(r.m, stack_[0], 32);
+        memcpy(r.m + 8, right, 32);
+        r.len = 64;
+        r.flags = flags_ | PARENT;
     }
-    uint64_t counter = offset / 64;
+    return r;
+}
+
+void Blake3::root_block(const Root &r, uint64_t block, uint8_t out[64]) {
+    uint32_t o[16];
+    compress(r.cv, r.m, block, r.len, r.flags | ROOT, o);
+    for (int i = 0; i < 16; i++)
+        for (int b = 0; b < 4; b++) out[4 * i + b] = (uint8_t)(o[i] >> (8 * b));
+}
+
+void Blake3::xof(uint64_t offset, uint8_t *out, size_t len) const {
+    const Root r = root();
+    uint64_t block = offset / 64;
     size_t skip = offset % 64;
     while (len) {
-        uint32_t o[16];
-        compress(cv, m, counter++, blen, flags | ROOT, o);
         uint8_t bytes[64];
-        for (int i = 0; i < 16; i++)
-            for (int b = 0; b < 4; b++) bytes[4 * i + b] = (uint8_t)(o[i] >> (8 * b));
+        root_block(r, block++, bytes);
         size_t take = std::min(len, (size_t)64 - skip);
         memcpy(out, bytes + skip, take);
         out += take;

@@ -29,6 +29,31 @@ class Blake3 {
     void update_with(const uint8_t *data, size_t len,
                      const std::function<void(size_t, uint64_t, size_t, uint32_t *)> &cvs_of);
     const uint32_t *key_words() const { return key_; }
+    // The whole state as plain data (counter: the caller's transcript counter, carried along).
+    template <class S>
+    void save_state(S &s) const {
+        memcpy(s.key, key_, 32);
+        s.flags = flags_;
+        s.depth = depth_;
+        memcpy(s.stack, stack_, sizeof stack_);
+        memcpy(s.cv, cv_, 32);
+        s.chunk = chunk_;
+        s.blocks = blocks_;
+        s.buf_len = (uint32_t)buf_len_;
+        memcpy(s.buf, buf_, 64);
+    }
+    template <class S>
+    void load_state(const S &s) {
+        memcpy(key_, s.key, 32);
+        flags_ = s.flags;
+        depth_ = s.depth;
+        memcpy(stack_, s.stack, sizeof stack_);
+        memcpy(cv_, s.cv, 32);
+        chunk_ = s.chunk;
+        blocks_ = s.blocks;
+        buf_len_ = s.buf_len;
+        memcpy(buf_, s.buf, 64);
+    }
     uint32_t flags() const { return flags_; }
     void update_u64(uint64_t x) {
         uint8_t b[8];
@@ -37,6 +62,12 @@ class Blake3 {
     }
     // Extendable output from byte `offset`; the state is unchanged.
     void xof(uint64_t offset, uint8_t *out, size_t len) const;
+    // The root node of the current state: output block i is compress(cv, m, i, len, flags | ROOT).
+    struct Root {
+        uint32_t cv[8], m[16], len, flags;
+    };
+    Root root() const;
+    static void root_block(const Root &r, uint64_t block, uint8_t out[64]);
 
     static void compress(const uint32_t cv[8], const uint32_t m[16], uint64_t counter, uint32_t len,
                          uint32_t flags, uint32_t out[16]);
@@ -61,7 +92,7 @@ class Blake3 {
 // An XOF stream read strictly in order (the transcript's readers and the matrix expansion).
 class XofReader {
    public:
-    explicit XofReader(const Blake3 &h) : h_(h) {}
+    explicit XofReader(const Blake3 &h) : root_(h.root()) {}
     uint8_t byte() {
         if (pos_ == sizeof buf_) refill();
         return buf_[pos_++];
@@ -72,13 +103,12 @@ class XofReader {
 
    private:
     void refill() {
-        h_.xof(off_, buf_, sizeof buf_);
-        off_ += sizeof buf_;
+        Blake3::root_block(root_, block_++, buf_);
         pos_ = 0;
     }
-    Blake3 h_;
-    uint64_t off_ = 0;
-    uint8_t buf_[1024];
+    Blake3::Root root_;  // computed once; each refill is one compression
+    uint64_t block_ = 0;
+    uint8_t buf_[64];
     size_t pos_ = sizeof buf_;
 };
 

@@ -36,6 +36,8 @@ struct Roots18 {
     uint32_t f[9], i[9];
 };
 
+struct SwitchDev;
+
 struct lbn::DeviceState {
     hipStream_t stream = nullptr;
     uint4 *table = nullptr;
@@ -59,12 +61,35 @@ struct lbn::DeviceState {
     // ready, while the opening runs) and the fold output, which the fold kernel writes directly.
     uint8_t *hstage = nullptr;
     int32_t *hfold = nullptr;
+    uint8_t *hbox = nullptr;        // pinned mailbox for the small transfers (regions in namespace mb)
+    uint32_t *hcvs = nullptr;       // pinned: chunk chaining values of the commitment
+    struct SwitchDev *sw = nullptr; // the switch rounds' device state (transcript, L-map, outputs)
     int32_t *fold_host = nullptr;   // device address of hfold
     hipStream_t copy_stream = nullptr;
     hipEvent_t ev_commit = nullptr, ev_staged = nullptr;
     hipEvent_t ev[2 * 9] = {};
     Roots18 roots[MAX_PRIMES] = {};      // start/stop per timed pass
 };
+
+// Regions of a context's pinned mailbox. Every small host<->device transfer goes through it, so
+// each is a plain DMA on the context's stream (pageable copies take the runtime's shared staging
+// path, which serialises proofs running on other threads). Each entry point synchronises its stream
+// before returning, so a region is never rewritten while a transfer from it is in flight.
+namespace mb {
+constexpr size_t EH = 0;                  // 2^(log_packed - 12) B128 <= 64 KiB
+constexpr size_t EL = EH + 65536;         // 4096 B128
+constexpr size_t EQ = EL + 65536;         // row eq tables: 128 + 128 F162
+constexpr size_t FOFF = EQ + 8192;        // fold list offsets (325 int)
+constexpr size_t FCOLS = FOFF + 2048;     // fold column lists (<= 1024 x 32 u16)
+constexpr size_t T = FCOLS + 65536;       // switch message sums (1536 u32)
+constexpr size_t PART = T + 6144;         // partials (512 u32)
+constexpr size_t ROW = PART + 2048;       // row (1024 F162)
+constexpr size_t MISC = ROW + 24576;      // flags, the last two switch values (256 B)
+constexpr size_t HEAD = MISC + 256;       // commitment bytes before the hashed chunks (<= 1 KiB)
+constexpr size_t TAIL = HEAD + 1024;      // and after them (<= 1 KiB)
+constexpr size_t SW = TAIL + 1024;        // the device switch state (SwitchDev, < 8 KiB)
+constexpr size_t SIZE = SW + 8192;
+}  // namespace mb
 
 // Timed passes (indices into lbn_timing's fields, in order).
 enum Pass { T_TABLE = 0, T_EQ, T_PARTIALS, T_SWITCH0, T_ROUNDS, T_ROW, T_FOLD, T_MAC, T_FINISH, T_COUNT };
@@ -387,14 +412,16 @@ __device__ __forceinline__ void rho_mul(const uint32_t *tab, const uint32_t (&y)
 // WORDS: p is the table (lifted words) in round 0, F162 (6 words) afterwards.
 // ---------------------------------------------------------------------------------------------
 // Block 0 also clears `clear` (1536 words), the message buffer of the round after next.
+// rho_dev, when not null, holds the challenge (written by the device transcript) instead of rho.
 template <bool WORDS>
 __global__ __launch_bounds__(256) void k_switch_round(const uint32_t *pin, size_t h2, RhoArg rho, const uint4 *Ein,
                                                       uint4 *Eout, uint32_t *pout, uint32_t *T0, uint32_t *Tinf,
-                                                      uint32_t *clear) {
+                                                      uint32_t *clear, const uint64_t *rho_dev) {
     if (blockIdx.x == 0)
         for (int i = threadIdx.x; i < 1536; i += blockDim.x) clear[i] = 0;
     __shared__ uint32_t rtab[RHO_WINDOWS * 16 * 6];
     __shared__ uint32_t lds[4 * M4W<6>::WORDS];  // also holds the basis while rtab is built
+    if (rho_dev) rho = RhoArg{{rho_dev[0], rho_dev[1], rho_dev[2]}};
     build_rho_table(rho, lds, rtab);
     uint32_t *W = lds + (threadIdx.x >> 6) * M4W<6>::WORDS;
     uint32_t acc0[2][6] = {}, acci[2][6] = {};
@@ -450,6 +477,413 @@ __global__ __launch_bounds__(256) void k_switch_round(const uint32_t *pin, size_
     m4w_flush<6, true>(acc0, lds, T0);
     __syncthreads();
     m4w_flush<6, true>(acci, lds, Tinf);
+}
+
+// ---------------------------------------------------------------------------------------------
+// The switch rounds on the device (sumcheck.rs:89-139 driven as main.rs:975-992): after each
+// round kernel, one block forms the round's message e0 = sum_k L[k] T0_k, einf = sum_k L[k]
+// Tinf_k, absorbs it into the Fiat-Shamir transcript, samples the challenge rho
+// (challenge.rs:226-326: absorb_f162 then sample_f162("switch-round")), and updates the linear
+// map L_{r+1}[k] = L_r((1 + c) x^k) + rho L_r[k] (c = r_hi[r]); the next round kernel reads rho
+// from device memory. The transcript is BLAKE3 over the absorbed stream, carried here in the
+// same plain state the host hasher saves, so the host continues from where the device stops.
+// ---------------------------------------------------------------------------------------------
+static_assert(sizeof(B3State) % 4 == 0, "the transcript state is copied as words");
+struct SwitchDev {
+    B3State ts;
+    uint64_t lmap[128][3];
+    uint64_t r_hi[32][2];
+    uint64_t rho[3];
+    uint64_t rounds[32][6];
+    uint64_t coords[32][3];
+    uint64_t opened[3];
+};
+
+// BLAKE3 over a B3State in LDS, run by the four lanes of one quad (threads 0..3): the compression
+// puts state column q in lane q (rows a = s[q], b = s[4+q], c = s[8+q], d = s[12+q]) and turns
+// the diagonal step into quad rotations (DPP), so a compression is 7 x 2 G steps per lane. The
+// bookkeeping (buffer, chunk, chaining-value stack) is the host Blake3 class statement by
+// statement, done identically by the four lanes; every absorb here is a whole number of 32-bit
+// words at a word-aligned buffer position, so the buffer is handled as words.
+namespace b3d {
+constexpr uint32_t CHUNK_START = 1, CHUNK_END = 2, PARENT = 4, ROOT = 8;
+
+// x_q by bit selects on lane masks (no compare: its condition register would stall the DPP chain).
+__device__ __forceinline__ uint32_t sel4(int q, uint32_t x0, uint32_t x1, uint32_t x2, uint32_t x3) {
+    const uint32_t m1 = 0u - (uint32_t)(q & 1), m2 = 0u - (uint32_t)((q >> 1) & 1);
+    const uint32_t lo = (x0 & ~m1) | (x1 & m1), hi = (x2 & ~m1) | (x3 & m1);
+    return (lo & ~m2) | (hi & m2);
+}
+template <int J>
+__device__ __forceinline__ uint32_t bcast(uint32_t x) {  // lane J of the quad
+    return dpp<J * 0x55>(x);
+}
+__device__ __forceinline__ void g(uint32_t &a, uint32_t &b, uint32_t &c, uint32_t &d, uint32_t x, uint32_t y) {
+    b3_g(a, b, c, d, x, y);
+}
+
+// out (16 words, the same in the four lanes) = compress(cv, m, counter, len, flags).
+__device__ __forceinline__ void compress(int q, const uint32_t (&cv)[8], const uint32_t (&m_in)[16], uint64_t counter, uint32_t len,
+                         uint32_t flags, uint32_t (&out)[16]) {
+    uint32_t m[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) m[i] = m_in[i];
+    uint32_t a = sel4(q, cv[0], cv[1], cv[2], cv[3]), b = sel4(q, cv[4], cv[5], cv[6], cv[7]);
+    uint32_t c = sel4(q, 0x6A09E667u, 0xBB67AE85u, 0x3C6EF372u, 0xA54FF53Au);
+    uint32_t d = sel4(q, (uint32_t)counter, (uint32_t)(counter >> 32), len, flags);
+    const int perm[16] = {2, 6, 3, 10, 7, 0, 4, 13, 1, 11, 12, 5, 9, 14, 15, 8};
+#pragma unroll
+    for (int r = 0; r < 7; r++) {
+        g(a, b, c, d, sel4(q, m[0], m[2], m[4], m[6]), sel4(q, m[1], m[3], m[5], m[7]));
+        b = dpp<0x39>(b), c = dpp<0x4E>(c), d = dpp<0x93>(d);  // from lanes q+1, q+2, q+3
+        g(a, b, c, d, sel4(q, m[8], m[10], m[12], m[14]), sel4(q, m[9], m[11], m[13], m[15]));
+        b = dpp<0x93>(b), c = dpp<0x4E>(c), d = dpp<0x39>(d);  // back
+        uint32_t t[16];
+#pragma unroll
+        for (int k = 0; k < 16; k++) t[k] = m[perm[k]];
+#pragma unroll
+        for (int k = 0; k < 16; k++) m[k] = t[k];
+    }
+    const uint32_t o0 = a ^ c, o1 = b ^ d, o2 = c ^ sel4(q, cv[0], cv[1], cv[2], cv[3]),
+                   o3 = d ^ sel4(q, cv[4], cv[5], cv[6], cv[7]);
+    out[0] = bcast<0>(o0), out[1] = bcast<1>(o0), out[2] = bcast<2>(o0), out[3] = bcast<3>(o0);
+    out[4] = bcast<0>(o1), out[5] = bcast<1>(o1), out[6] = bcast<2>(o1), out[7] = bcast<3>(o1);
+    out[8] = bcast<0>(o2), out[9] = bcast<1>(o2), out[10] = bcast<2>(o2), out[11] = bcast<3>(o2);
+    out[12] = bcast<0>(o3), out[13] = bcast<1>(o3), out[14] = bcast<2>(o3), out[15] = bcast<3>(o3);
+}
+__device__ __forceinline__ uint32_t *bufw(B3State &s) { return (uint32_t *)s.buf; }
+__device__ __forceinline__ void words16(B3State &s, uint32_t (&m)[16]) {  // the buffer, zero past buf_len
+    const uint32_t n = s.buf_len;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        const uint32_t w = bufw(s)[i];
+        m[i] = 4 * (uint32_t)i < n ? w : 0u;
+    }
+}
+__device__ __forceinline__ void chunk_cv(int q, B3State &s, uint32_t (&out)[8]) {
+    uint32_t m[16], cv[8], o[16];
+    words16(s, m);
+#pragma unroll
+    for (int i = 0; i < 8; i++) cv[i] = s.cv[i];
+    compress(q, cv, m, s.chunk, s.buf_len, s.flags | CHUNK_END | (s.blocks == 0 ? CHUNK_START : 0), o);
+#pragma unroll
+    for (int i = 0; i < 8; i++) out[i] = o[i];
+}
+__device__ __forceinline__ void add_chunk_cv(int q, B3State &s, uint32_t (&cv)[8], uint64_t total) {
+    uint32_t key[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) key[i] = s.key[i];
+    while ((total & 1) == 0) {
+        uint32_t m[16], o[16];
+        const int dpt = s.depth - 1;
+#pragma unroll
+        for (int i = 0; i < 8; i++) m[i] = s.stack[dpt][i], m[i + 8] = cv[i];
+        compress(q, key, m, 0, 64, s.flags | PARENT, o);
+#pragma unroll
+        for (int i = 0; i < 8; i++) cv[i] = o[i];
+        s.depth = dpt;
+        total >>= 1;
+    }
+#pragma unroll
+    for (int i = 0; i < 8; i++) s.stack[s.depth][i] = cv[i];
+    s.depth = s.depth + 1;
+}
+// Absorb one word (buf_len a multiple of 4).
+__device__ __forceinline__ void update_word(int q, B3State &s, uint32_t w) {
+    if ((uint32_t)s.blocks * 64 + s.buf_len == 1024) {  // full chunk and more input: close it
+        uint32_t cv[8];
+        chunk_cv(q, s, cv);
+        add_chunk_cv(q, s, cv, s.chunk + 1);
+        s.chunk = s.chunk + 1;
+        s.blocks = 0;
+        s.buf_len = 0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) s.cv[k] = s.key[k];
+    }
+    if (s.buf_len == 64) {
+        uint32_t m[16], cv[8], o[16];
+        words16(s, m);
+#pragma unroll
+        for (int k = 0; k < 8; k++) cv[k] = s.cv[k];
+        compress(q, cv, m, s.chunk, 64, s.flags | (s.blocks == 0 ? CHUNK_START : 0), o);
+#pragma unroll
+        for (int k = 0; k < 8; k++) s.cv[k] = o[k];
+        s.blocks = s.blocks + 1;
+        s.buf_len = 0;
+    }
+    bufw(s)[s.buf_len / 4] = w;
+    s.buf_len = s.buf_len + 4;
+}
+template <int N>
+__device__ __forceinline__ void update_u64s(int q, B3State &s, const uint64_t (&x)[N]) {
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        update_word(q, s, (uint32_t)x[i]);
+        update_word(q, s, (uint32_t)(x[i] >> 32));
+    }
+}
+// First output block of the root node.
+__device__ __forceinline__ void root_block0(int q, B3State &s, uint32_t (&out)[16]) {
+    uint32_t cv[8], m[16];
+    uint32_t len, flags;
+    if (s.depth == 0) {
+#pragma unroll
+        for (int i = 0; i < 8; i++) cv[i] = s.cv[i];
+        words16(s, m);
+        len = s.buf_len;
+        flags = s.flags | CHUNK_END | (s.blocks == 0 ? CHUNK_START : 0);
+    } else {
+        uint32_t right[8], key[8];
+        chunk_cv(q, s, right);
+#pragma unroll
+        for (int k = 0; k < 8; k++) key[k] = s.key[k];
+        for (int i = s.depth - 1; i > 0; i--) {
+            uint32_t mm[16], o[16];
+#pragma unroll
+            for (int k = 0; k < 8; k++) mm[k] = s.stack[i][k], mm[k + 8] = right[k];
+            compress(q, key, mm, 0, 64, s.flags | PARENT, o);
+#pragma unroll
+            for (int k = 0; k < 8; k++) right[k] = o[k];
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++) cv[k] = key[k], m[k] = s.stack[0][k], m[k + 8] = right[k];
+        len = 64;
+        flags = s.flags | PARENT;
+    }
+    compress(q, cv, m, 0, len, flags | ROOT, out);
+}
+}  // namespace b3d
+
+__device__ __forceinline__ void f162_mulx4(uint64_t (&a)[3]) {
+    const uint64_t top = a[2] >> 30;  // bits 158..161
+    a[2] = ((a[2] << 4) | (a[1] >> 60)) & ((1ull << 34) - 1);
+    a[1] = (a[1] << 4) | (a[0] >> 60);
+    a[0] <<= 4;
+    a[0] ^= top;        // x^162 = x^81 + 1
+    a[1] ^= top << 17;  // bit 81 is limb 1 bit 17
+}
+
+// v * x^k in GF(2^128), GHASH basis (x^128 = x^7 + x^2 + x + 1), k < 128.
+__device__ __forceinline__ u128 b128_mulx_pow(u128 v, int k) {
+    if (k == 0) return v;
+    u128 lo = v << k;
+    const u128 hi = v >> (128 - k);
+    const u128 t = hi ^ (hi << 1) ^ (hi << 2) ^ (hi << 7);
+    const u128 o = (hi >> 127) ^ (hi >> 126) ^ (hi >> 121);
+    return lo ^ t ^ o ^ (o << 1) ^ (o << 2) ^ (o << 7);
+}
+
+__global__ __launch_bounds__(1024) void k_switch_ctl(SwitchDev *sd, int r, int lp, int V, const uint32_t *T,
+                                                     const uint32_t *pair) {
+    __shared__ B3State st;
+    __shared__ B3State cl;
+    __shared__ uint64_t L[128][3];
+    __shared__ uint64_t part[32][3];
+    __shared__ uint64_t tabL[32][16][3];
+    __shared__ uint64_t Tt[2][192][2];
+    __shared__ uint64_t rbas[162][3];
+    __shared__ uint64_t tabR41[41][16][3];
+    __shared__ uint64_t lpart[4][128][3];
+    __shared__ uint64_t prod[2][3];
+    __shared__ uint64_t rho_s[3];
+    const int t = threadIdx.x, q = t & 3;
+    {
+        const uint32_t *src = (const uint32_t *)&sd->ts;
+        uint32_t *dst = (uint32_t *)&st;
+        for (int i = t; i < (int)(sizeof(B3State) / 4); i += 1024) dst[i] = src[i];
+        if (t < 384) L[t / 3][t % 3] = sd->lmap[t / 3][t % 3];
+    }
+    __syncthreads();
+    // 4-bit tables of L: tabL[w][j] = sum of L[4w + i] over the set bits i of j (also used for the
+    // map update below), and the message sums staged in LDS
+    for (int e = t; e < 32 * 16; e += 1024) {
+        const int w = e >> 4, nib = e & 15;
+        uint64_t v[3] = {0, 0, 0};
+        for (int b = 0; b < 4; b++)
+            if ((nib >> b) & 1)
+                for (int k = 0; k < 3; k++) v[k] ^= L[4 * w + b][k];
+        for (int k = 0; k < 3; k++) tabL[w][nib][k] = v[k];
+    }
+    // Bit-transpose of the message sums by wave ballots: Tt[kind][b][half] holds bit b of T_k for
+    // k = 64 half + lane. Wave 2 word + half of kind 0 (waves 0..11), then kind 1.
+    {
+        const int wave = t >> 6, lane = t & 63;
+        if (wave < 2 * V) {
+            const int word = wave >> 1, half = wave & 1, k = 64 * half + lane;
+            for (int kind = 0; kind < 2; kind++) {
+                const uint32_t x = T[768 * kind + k * V + word];
+                for (int bit = 0; bit < 32; bit++) {
+                    const uint64_t mk = __ballot((x >> bit) & 1);
+                    if (lane == 0) Tt[kind][32 * word + bit][half] = mk;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    {
+        // e = sum_k L[k] T_k = sum_b x^b U_b with U_b = sum_{k: bit b of T_k} L[k]: thread b (< 162)
+        // for e0, 162 + b for einf; U_b through the tables of L, four values of k per lookup.
+        const int kind = t < 162 ? 0 : 1, b = t - 162 * kind;
+        uint64_t acc[3] = {0, 0, 0};
+        if (t < 324 && b < 32 * V) {
+            const uint64_t m0 = Tt[kind][b][0], m1 = Tt[kind][b][1];
+#pragma unroll
+            for (int w = 0; w < 32; w++) {
+                const uint32_t nib = (uint32_t)(((w < 16 ? m0 : m1) >> (4 * (w & 15))) & 15);
+                for (int k = 0; k < 3; k++) acc[k] ^= tabL[w][nib][k];
+            }
+            uint64_t y[3];  // U_b x^b
+            if (b < 128) {
+                f162_mulx_pow(acc, b, y);
+            } else {
+                uint64_t z[3];
+                f162_mulx_pow(acc, 64, z);
+                f162_mulx_pow(z, 64, y);
+                f162_mulx_pow(y, b - 128, z);
+                y[0] = z[0], y[1] = z[1], y[2] = z[2];
+            }
+            acc[0] = y[0], acc[1] = y[1], acc[2] = y[2];
+        }
+        // XOR over each half-wave (lane exchanges within 32 lanes), then over the half-waves of
+        // each kind (the kind changes inside half-wave 5)
+        uint32_t x[12];
+#pragma unroll
+        for (int w = 0; w < 3; w++) {
+            x[4 * w + 0] = kind == 0 ? (uint32_t)acc[w] : 0u;
+            x[4 * w + 1] = kind == 0 ? (uint32_t)(acc[w] >> 32) : 0u;
+            x[4 * w + 2] = kind == 1 ? (uint32_t)acc[w] : 0u;
+            x[4 * w + 3] = kind == 1 ? (uint32_t)(acc[w] >> 32) : 0u;
+        }
+#pragma unroll
+        for (int i = 0; i < 12; i++) x[i] ^= lane_xor<1>(x[i]);
+#pragma unroll
+        for (int i = 0; i < 12; i++) x[i] ^= lane_xor<2>(x[i]);
+#pragma unroll
+        for (int i = 0; i < 12; i++) x[i] ^= lane_xor<4>(x[i]);
+#pragma unroll
+        for (int i = 0; i < 12; i++) x[i] ^= lane_xor<8>(x[i]);
+#pragma unroll
+        for (int i = 0; i < 12; i++) x[i] ^= lane_xor<16>(x[i]);
+        if ((t & 31) == 0 && t < 352)
+#pragma unroll
+            for (int w = 0; w < 3; w++) {
+                part[t >> 5][w] = (uint64_t)x[4 * w] | ((uint64_t)x[4 * w + 1] << 32);
+                part[16 + (t >> 5)][w] = (uint64_t)x[4 * w + 2] | ((uint64_t)x[4 * w + 3] << 32);
+            }
+    }
+    __syncthreads();
+    if (t < 6) {  // half-waves 0..10 hold both kinds
+        const int kind = t / 3, w = t % 3;
+        uint64_t x = 0;
+        for (int i = 0; i < 11; i++) x ^= part[16 * kind + i][w];
+        prod[kind][w] = x;
+    }
+    __syncthreads();
+    if (t < 4) {  // absorb_f162([e0, einf])
+        const uint64_t msg[7] = {2, prod[0][0], prod[0][1], prod[0][2], prod[1][0], prod[1][1], prod[1][2]};
+        if (t == 0)
+            for (int w = 0; w < 6; w++) sd->rounds[r][w] = msg[w + 1];
+        b3d::update_u64s(q, st, msg);
+    }
+    __syncthreads();
+    {  // sample_f162("switch-round", 1): a reader at the counter, then counter += 1
+        const uint32_t *src = (const uint32_t *)&st;
+        uint32_t *dst = (uint32_t *)&cl;
+        for (int i = t; i < (int)(sizeof(B3State) / 4); i += 1024) dst[i] = src[i];
+    }
+    __syncthreads();
+    if (t < 4) {
+        const uint64_t head[2] = {st.counter, 12};
+        b3d::update_u64s(q, cl, head);
+        b3d::update_word(q, cl, 0x74697773u);  // "switch-round", little-endian words
+        b3d::update_word(q, cl, 0x722d6863u);
+        b3d::update_word(q, cl, 0x646e756fu);
+        uint32_t o[16];
+        b3d::root_block0(q, cl, o);
+        if (t == 0) {
+            st.counter = st.counter + 1;
+            rho_s[0] = (uint64_t)o[0] | ((uint64_t)o[1] << 32);
+            rho_s[1] = (uint64_t)o[2] | ((uint64_t)o[3] << 32);
+            rho_s[2] = ((uint64_t)o[4] | ((uint64_t)o[5] << 32)) & ((1ull << 34) - 1);
+            for (int w = 0; w < 3; w++) sd->rho[w] = rho_s[w], sd->coords[r][w] = rho_s[w];
+        }
+    }
+    __syncthreads();
+    if (r == lp - 1) {  // opened = p0 + rho (p0 + p1), then absorb_f162([opened])
+        if (t < 4) {
+            const uint64_t p0[3] = {(uint64_t)pair[0] | ((uint64_t)pair[1] << 32),
+                                    (uint64_t)pair[2] | ((uint64_t)pair[3] << 32),
+                                    (uint64_t)pair[4] | ((uint64_t)pair[5] << 32)};
+            const uint64_t d[3] = {p0[0] ^ ((uint64_t)pair[6] | ((uint64_t)pair[7] << 32)),
+                                   p0[1] ^ ((uint64_t)pair[8] | ((uint64_t)pair[9] << 32)),
+                                   p0[2] ^ ((uint64_t)pair[10] | ((uint64_t)pair[11] << 32))};
+            const uint64_t rr[3] = {rho_s[0], rho_s[1], rho_s[2]};
+            uint64_t m[3];
+            f162_mul_slow(rr, d, m);
+            const uint64_t msg[4] = {1, p0[0] ^ m[0], p0[1] ^ m[1], p0[2] ^ m[2]};
+            if (t == 0)
+                for (int w = 0; w < 3; w++) sd->opened[w] = msg[w + 1];
+            b3d::update_u64s(q, st, msg);
+        }
+    } else {
+        // L_{r+1}[k] = L_r((1 + c) x^k) + rho L_r[k], through the tables of L built above and 4-bit
+        // window tables of y -> rho y (from rho x^i, i < 162)
+        if (t < 162) {
+            const uint64_t rr[3] = {rho_s[0], rho_s[1], rho_s[2]};
+            uint64_t z[3], y[3];
+            if (t < 128) {
+                f162_mulx_pow(rr, t, y);
+            } else {
+                f162_mulx_pow(rr, 64, z);
+                f162_mulx_pow(z, 64, y);
+                f162_mulx_pow(y, t - 128, z);
+                y[0] = z[0], y[1] = z[1], y[2] = z[2];
+            }
+            rbas[t][0] = y[0], rbas[t][1] = y[1], rbas[t][2] = y[2];
+        }
+        __syncthreads();
+        if (t < 41 * 16) {
+            const int w = t >> 4, nib = t & 15;
+            uint64_t v[3] = {0, 0, 0};
+            for (int b = 0; b < 4; b++)
+                if (((nib >> b) & 1) && 4 * w + b < 162)
+                    for (int k = 0; k < 3; k++) v[k] ^= rbas[4 * w + b][k];
+            for (int k = 0; k < 3; k++) tabR41[w][nib][k] = v[k];
+        }
+        __syncthreads();
+        if (t < 512) {  // k = t % 128; four threads per k: two halves of each table walk
+            const int k = t & 127, part4 = t >> 7;
+            uint64_t acc[3] = {0, 0, 0};
+            if (part4 < 2) {
+                const u128 one_c = (((u128)sd->r_hi[r][1] << 64) | sd->r_hi[r][0]) ^ (u128)1;
+                const u128 z = b128_mulx_pow(one_c, k);  // (1 + c) x^k
+#pragma unroll
+                for (int i = 0; i < 16; i++) {
+                    const int w = 16 * part4 + i;
+                    const int nib = (int)((z >> (4 * w)) & 15);
+                    for (int j = 0; j < 3; j++) acc[j] ^= tabL[w][nib][j];
+                }
+            } else {  // rho * L[k], windows 0..20 or 21..40
+                const int w0 = part4 == 2 ? 0 : 21, w1 = part4 == 2 ? 21 : 41;
+                for (int w = w0; w < w1; w++) {
+                    const int nib = (int)((L[k][w >> 4] >> (4 * (w & 15))) & 15);
+                    for (int j = 0; j < 3; j++) acc[j] ^= tabR41[w][nib][j];
+                }
+            }
+            for (int j = 0; j < 3; j++) lpart[part4][k][j] = acc[j];
+        }
+        __syncthreads();
+        if (t < 384) {
+            const int k = t / 3, j = t % 3;
+            sd->lmap[k][j] = lpart[0][k][j] ^ lpart[1][k][j] ^ lpart[2][k][j] ^ lpart[3][k][j];
+        }
+    }
+    __syncthreads();
+    {
+        const uint32_t *src = (const uint32_t *)&st;
+        uint32_t *dst = (uint32_t *)&sd->ts;
+        for (int i = t; i < (int)(sizeof(B3State) / 4); i += 1024) dst[i] = src[i];
+    }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -880,7 +1314,8 @@ __device__ __forceinline__ void stage1_lookup(const uint32_t (&ra)[4], const uin
 
 __global__ __launch_bounds__(MAC_THREADS, 5) void k_commit_mac(const uint32_t *table32, size_t clen, int m_per, int re,
                                                             const uint32_t *ahat, const uint32_t *tabs, PC pc,
-                                                            uint32_t c32, Roots18 R, unsigned long long *cacc) {
+                                                            uint32_t c32, uint32_t r2, Roots18 R,
+                                                            unsigned long long *cacc) {
     __shared__ uint32_t Y[TC * 2 * 324];
     __shared__ uint32_t s_st1[tab::ST1_SIZE];
     __shared__ uint32_t s_twp[2 * 324];
@@ -953,17 +1388,22 @@ __global__ __launch_bounds__(MAC_THREADS, 5) void k_commit_mac(const uint32_t *t
             if (((m - m0) & 3) == 3) fold();
         }
     }
-    fold();
+    // Reduce to [0, p): one fold leaves acc < 2^32 p, so REDC(acc) = acc R^-1 mod p in [0, 2p), and a
+    // Montgomery product with R^2 gives acc mod p. At most 16 blocks add into one entry: < 16 p.
     fold();
     unsigned long long *dst = cacc + (size_t)(c0 + col) * 648 + h * 324 + idx * 18;
 #pragma unroll
-    for (int p = 0; p < 18; p++) atomicAdd(dst + p, (unsigned long long)acc[p]);
+    for (int p = 0; p < 18; p++) {
+        const uint32_t m = (uint32_t)acc[p] * pc.pinv_neg;
+        const uint32_t redc = (uint32_t)((acc[p] + (uint64_t)m * pc.p) >> 32);
+        atomicAdd(dst + p, (unsigned long long)mont(redc, r2, pc));
+    }
 }
 
 // Inverse transform per column and prime, Garner over the primes, reduction mod Q, packing.
 struct CrtArg {
     uint32_t p[MAX_PRIMES];
-    uint32_t inv[MAX_PRIMES][MAX_PRIMES];  // inv[i][j] = p_j^-1 mod p_i (j < i)
+    uint32_t inv[MAX_PRIMES][MAX_PRIMES];  // inv[i][j] = p_j^-1 R mod p_i (j < i), Montgomery form
     uint64_t q_lo, q_hi;
     uint64_t half_m[3];                     // floor(M / 2), M = product of the primes (192 bits)
     uint64_t m[3];
@@ -972,9 +1412,6 @@ struct CrtArg {
     Roots18 roots[MAX_PRIMES];
 };
 
-__device__ __forceinline__ uint32_t mulmod_u64(uint32_t a, uint32_t b, uint32_t p) {
-    return (uint32_t)(((uint64_t)a * b) % p);
-}
 
 // 192-bit helpers for the mixed-radix value (up to five 29.4-bit primes)
 __device__ __forceinline__ void u192_muladd(uint64_t (&x)[3], uint32_t p, uint32_t v) {
@@ -1016,7 +1453,13 @@ __global__ __launch_bounds__(64) void k_commit_finish(const unsigned long long *
             const unsigned long long *src = cacc + ((size_t)k * 1024 + c) * 648 + h * 324 + idx * 18;
             uint32_t v[18];
 #pragma unroll
-            for (int p = 0; p < 18; p++) v[p] = (uint32_t)(src[p] % pc.p);
+            for (int p = 0; p < 18; p++) {  // sums of at most 16 residues below p
+                uint64_t x = src[p];
+                x = x >= 8ull * pc.p ? x - 8ull * pc.p : x;
+                x = x >= 4ull * pc.p ? x - 4ull * pc.p : x;
+                x = x >= 2ull * pc.p ? x - 2ull * pc.p : x;
+                v[p] = (uint32_t)(x >= pc.p ? x - pc.p : x);
+            }
             dft18_inv(v, w, pc);
 #pragma unroll
             for (int n2 = 0; n2 < 18; n2++) Y[h * 324 + idx * 18 + n2] = v[n2];
@@ -1050,10 +1493,11 @@ __global__ __launch_bounds__(64) void k_commit_finish(const unsigned long long *
         // Garner: x = v0 + p0 (v1 + p1 (v2 + ...))
         uint32_t v[MAX_PRIMES];
         for (int i = 0; i < crt.nprimes; i++) {
+            const uint32_t pi = crt.p[i];
             uint32_t xx = res[i][tt];
             for (int j = 0; j < i; j++) {
-                const uint32_t vj = v[j] % crt.p[i];
-                xx = mulmod_u64(xx >= vj ? xx - vj : xx + crt.p[i] - vj, crt.inv[i][j], crt.p[i]);
+                const uint32_t vj = v[j] >= pi ? v[j] - pi : v[j];  // v[j] < p_j < 2 p_i
+                xx = mont(subm(xx, vj, pi), crt.inv[i][j], crt.pc[i]);
             }
             v[i] = xx;
         }
@@ -1066,10 +1510,15 @@ __global__ __launch_bounds__(64) void k_commit_finish(const unsigned long long *
             u192_sub(crt.m, X, mag);
         else
             mag[0] = X[0], mag[1] = X[1], mag[2] = X[2];
-        u128 r = 0;  // mag mod Q by shift-subtract (Q < 2^96)
-        for (int b = 191; b >= 0; b--) {
-            r = (r << 1) | ((mag[b >> 6] >> (b & 63)) & 1);
-            if (r >= Q) r -= Q;
+        u128 r = 0;  // mag mod Q
+        if (mag[1] == 0 && mag[2] == 0 && crt.q_hi == 0) {
+            r = mag[0] % crt.q_lo;
+        } else {  // shift-subtract from the top set bit (Q < 2^96)
+            const int top = mag[2] ? 128 + 63 - __clzll(mag[2]) : mag[1] ? 64 + 63 - __clzll(mag[1]) : 63;
+            for (int b = top; b >= 0; b--) {
+                r = (r << 1) | ((mag[b >> 6] >> (b & 63)) & 1);
+                if (r >= Q) r -= Q;
+            }
         }
         if (neg && r) r = Q - r;
         uint8_t *o = outb + ((size_t)c * 648 + tt) * crt.width;
@@ -1116,6 +1565,8 @@ int lbn::dev_init(lbn_ctx *ctx) {
     HIPCHK(ctx, hipMalloc(&d->fold_cols, (size_t)COLUMNS * 32 * sizeof(uint16_t)));
     HIPCHK(ctx, hipMalloc(&d->ntt_tabs, (size_t)MAX_PRIMES * tab::SIZE * 4));
     HIPCHK(ctx, hipHostMalloc((void **)&d->hfold, s.ring_elements * N * sizeof(int32_t)));
+    HIPCHK(ctx, hipHostMalloc((void **)&d->hbox, mb::SIZE));
+    HIPCHK(ctx, hipMalloc(&d->sw, sizeof(SwitchDev)));
     HIPCHK(ctx, hipHostGetDevicePointer((void **)&d->fold_host, d->hfold, 0));
     HIPCHK(ctx, hipStreamCreateWithFlags(&d->copy_stream, hipStreamNonBlocking));
     HIPCHK(ctx, hipEventCreateWithFlags(&d->ev_commit, hipEventDisableTiming));
@@ -1130,12 +1581,14 @@ void lbn::dev_free(lbn_ctx *ctx) {
     hipSetDevice(ctx->device);
     void *ptrs[] = {d->table, d->E[0], d->E[1], d->P[0], d->P[1], d->T, d->EH, d->EL, d->eqrow, d->eqlo,
                     d->rowU, d->rowOut, d->fold_flag, d->fold_off, d->fold_cols, d->ntt_tabs,
-                    d->ahat, d->cacc, d->cbytes, d->amat, d->cvs};
+                    d->ahat, d->cacc, d->cbytes, d->amat, d->cvs, d->sw};
     if (d->copy_stream) hipStreamSynchronize(d->copy_stream);
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (d->hstage) hipHostFree(d->hstage);
     if (d->hfold) hipHostFree(d->hfold);
+    if (d->hbox) hipHostFree(d->hbox);
+    if (d->hcvs) hipHostFree(d->hcvs);
     for (auto &e : d->ev)
         if (e) hipEventDestroy(e);
     if (d->ev_commit) hipEventDestroy(d->ev_commit);
@@ -1278,13 +1731,15 @@ static int alloc_commit_buffers(lbn_ctx *ctx, int np, int width) {
     for (void *p : {(void *)d->cacc, (void *)d->cbytes, (void *)d->cvs})
         if (p) HIPCHK(ctx, hipFree(p));
     if (d->hstage) HIPCHK(ctx, hipHostFree(d->hstage));
-    d->cacc = nullptr, d->cbytes = nullptr, d->cvs = nullptr, d->hstage = nullptr;
+    if (d->hcvs) HIPCHK(ctx, hipHostFree(d->hcvs));
+    d->cacc = nullptr, d->cbytes = nullptr, d->cvs = nullptr, d->hstage = nullptr, d->hcvs = nullptr;
     d->alloc_primes = d->alloc_width = 0;
     const size_t clen = (size_t)COLUMNS * N * width;
     HIPCHK(ctx, hipMalloc(&d->cacc, (size_t)np * COLUMNS * N * 8));
     HIPCHK(ctx, hipMalloc(&d->cbytes, clen + 64));  // slack: the chunk hashing reads one word past
     HIPCHK(ctx, hipMalloc(&d->cvs, clen / 1024 * 32 + 64));
     HIPCHK(ctx, hipHostMalloc((void **)&d->hstage, clen));
+    HIPCHK(ctx, hipHostMalloc((void **)&d->hcvs, clen / 1024 * 32 + 64));
     d->alloc_primes = np;
     d->alloc_width = width;
     return LBN_OK;
@@ -1406,11 +1861,16 @@ int lbn::dev_commit_cvs(lbn_ctx *ctx, size_t off, uint64_t first, size_t count, 
         k.flags = flags;
         k_chunk_cvs<<<(unsigned)((count + 63) / 64), 64, 0, d->stream>>>(d->cbytes, off, first, count, k, d->cvs);
         HIPCHK(ctx, hipGetLastError());
-        HIPCHK(ctx, hipMemcpyAsync(cvs, d->cvs, count * 32, hipMemcpyDeviceToHost, d->stream));
+        HIPCHK(ctx, hipMemcpyAsync(d->hcvs, d->cvs, count * 32, hipMemcpyDeviceToHost, d->stream));
     }
-    if (head) HIPCHK(ctx, hipMemcpyAsync(dst, d->cbytes, head, hipMemcpyDeviceToHost, d->stream));
-    if (tail < clen) HIPCHK(ctx, hipMemcpyAsync(dst + tail, d->cbytes + tail, clen - tail, hipMemcpyDeviceToHost, d->stream));
+    if (head > 1024 || clen - tail > 1024) return ctx->fail(LBN_E_ARG, "commitment head or tail above one chunk");
+    if (head) HIPCHK(ctx, hipMemcpyAsync(d->hbox + mb::HEAD, d->cbytes, head, hipMemcpyDeviceToHost, d->stream));
+    if (tail < clen)
+        HIPCHK(ctx, hipMemcpyAsync(d->hbox + mb::TAIL, d->cbytes + tail, clen - tail, hipMemcpyDeviceToHost, d->stream));
     HIPCHK(ctx, hipStreamSynchronize(d->stream));
+    memcpy(cvs, d->hcvs, count * 32);
+    memcpy(dst, d->hbox + mb::HEAD, head);
+    memcpy(dst + tail, d->hbox + mb::TAIL, clen - tail);
     tcollect(ctx, T_MAC);
     tcollect(ctx, T_FINISH);
     return LBN_OK;
@@ -1446,7 +1906,8 @@ int lbn::dev_commit_launch(lbn_ctx *ctx) {
         k_commit_mac<<<grid, MAC_THREADS, 0, d->stream>>>((const uint32_t *)d->table, s.column_len, m_per, re,
                                                            d->ahat + (size_t)k * re * N,
                                                            d->ntt_tabs + (size_t)k * tab::SIZE, pc,
-                                                           (uint32_t)((1ull << 32) % pc.p), d->roots[k],
+                                                           (uint32_t)((1ull << 32) % pc.p), ctx->primes[k].r2,
+                                                           d->roots[k],
                                                            d->cacc + (size_t)k * COLUMNS * N);
         HIPCHK(ctx, hipGetLastError());
     }
@@ -1468,7 +1929,9 @@ int lbn::dev_commit_launch(lbn_ctx *ctx) {
             M[1] = (uint64_t)t;
             M[2] = M[2] * crt.p[i] + (uint64_t)(t >> 64);
             for (int j = 0; j < i; j++)
-                crt.inv[i][j] = powmod32(ctx->primes[j].p % ctx->primes[i].p, ctx->primes[i].p - 2, ctx->primes[i].p);
+                crt.inv[i][j] = (uint32_t)(((uint64_t)powmod32(ctx->primes[j].p % ctx->primes[i].p,
+                                                                ctx->primes[i].p - 2, ctx->primes[i].p)
+                                            << 32) % ctx->primes[i].p);
         }
         for (int w = 0; w < 3; w++) crt.m[w] = M[w];
         crt.half_m[0] = (M[0] >> 1) | (M[1] << 63);
@@ -1496,8 +1959,10 @@ extern "C" int lbn_partials(lbn_ctx *ctx, const uint64_t *r_hi, uint64_t *partia
     std::vector<B128> EH((size_t)1 << nh), EL(4096);
     eq_expand_b128(ctx->r_hi.data(), nh, EH.data());
     eq_expand_b128(ctx->r_hi.data() + nh, 12, EL.data());
-    HIPCHK(ctx, hipMemcpyAsync(d->EH, EH.data(), EH.size() * 16, hipMemcpyHostToDevice, d->stream));
-    HIPCHK(ctx, hipMemcpyAsync(d->EL, EL.data(), EL.size() * 16, hipMemcpyHostToDevice, d->stream));
+    memcpy(d->hbox + mb::EH, EH.data(), EH.size() * 16);
+    memcpy(d->hbox + mb::EL, EL.data(), EL.size() * 16);
+    HIPCHK(ctx, hipMemcpyAsync(d->EH, d->hbox + mb::EH, EH.size() * 16, hipMemcpyHostToDevice, d->stream));
+    HIPCHK(ctx, hipMemcpyAsync(d->EL, d->hbox + mb::EL, EL.size() * 16, hipMemcpyHostToDevice, d->stream));
     tstart(ctx, T_EQ);
     k_eq_expand<<<(unsigned)EH.size(), 256, 0, d->stream>>>(d->EH, d->EL, d->E[0]);
     tstop(ctx, T_EQ);
@@ -1508,8 +1973,8 @@ extern "C" int lbn_partials(lbn_ctx *ctx, const uint64_t *r_hi, uint64_t *partia
     k_partials<<<blocks, 256, 0, d->stream>>>(d->table, d->E[0], s.n_words, d->T + 1536);
     tstop(ctx, T_PARTIALS);
     HIPCHK(ctx, hipGetLastError());
-    uint32_t out[512];
-    HIPCHK(ctx, hipMemcpyAsync(out, d->T + 1536, sizeof out, hipMemcpyDeviceToHost, d->stream));
+    const uint32_t *out = (const uint32_t *)(d->hbox + mb::PART);
+    HIPCHK(ctx, hipMemcpyAsync(d->hbox + mb::PART, d->T + 1536, 512 * 4, hipMemcpyDeviceToHost, d->stream));
     HIPCHK(ctx, hipStreamSynchronize(d->stream));
     tcollect(ctx, T_EQ);
     tcollect(ctx, T_PARTIALS);
@@ -1562,14 +2027,63 @@ extern "C" int lbn_switch_msg(lbn_ctx *ctx, uint64_t *msg) {
     if (ctx->round < 0 || ctx->round >= ctx->shape.log_packed) return ctx->fail(LBN_E_STATE, "no switch round pending");
     HIPCHK(ctx, hipSetDevice(ctx->device));
     DeviceState *d = ctx->dev;
-    uint32_t T[1536];
-    HIPCHK(ctx, hipMemcpyAsync(T, msg_buf(d, ctx->round), sizeof T, hipMemcpyDeviceToHost, d->stream));
+    const uint32_t *T = (const uint32_t *)(d->hbox + mb::T);
+    HIPCHK(ctx, hipMemcpyAsync(d->hbox + mb::T, msg_buf(d, ctx->round), 1536 * 4, hipMemcpyDeviceToHost, d->stream));
     HIPCHK(ctx, hipStreamSynchronize(d->stream));
     tcollect(ctx, ctx->round == 0 ? T_SWITCH0 : T_ROUNDS);
     const int V = ctx->round == 0 ? 4 : 6;
     F162 e0 = combine(ctx->lmap, T, V), einf = combine(ctx->lmap, T + 768, V);
     memcpy(msg, e0.w, 24);
     memcpy(msg + 3, einf.w, 24);
+    return LBN_OK;
+}
+
+// Fold of round r with rho (by value, or from device memory when rho_dev is set) fused with the
+// message of round r + 1.
+static void launch_round(lbn_ctx *ctx, int r, RhoArg ra, const uint64_t *rho_dev) {
+    DeviceState *d = ctx->dev;
+    const size_t h2 = (ctx->shape.n_words >> (r + 1)) / 2;
+    const uint4 *Ein = d->E[(r + 1) & 1];
+    uint4 *Eout = d->E[r & 1];
+    uint32_t *pout = d->P[r & 1];
+    uint32_t *T = msg_buf(d, r + 1);
+    if (r == 0)
+        k_switch_round<true><<<launch_blocks(h2), 256, 0, d->stream>>>((const uint32_t *)d->table, h2, ra, Ein, Eout,
+                                                                        pout, T, T + 768, msg_buf(d, r), rho_dev);
+    else
+        k_switch_round<false><<<launch_blocks(h2), 256, 0, d->stream>>>(d->P[(r - 1) & 1], h2, ra, Ein, Eout, pout,
+                                                                         T, T + 768, msg_buf(d, r), rho_dev);
+}
+
+int lbn::dev_switch_rounds(lbn_ctx *ctx, B3State &ts, uint64_t *rounds, uint64_t *coords, uint64_t *opened) {
+    if (ctx->round != 0) return ctx->fail(LBN_E_STATE, "lbn_switch_begin must run first");
+    const int lp = ctx->shape.log_packed;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    DeviceState *d = ctx->dev;
+    SwitchDev *h = (SwitchDev *)(d->hbox + mb::SW);
+    static_assert(sizeof(SwitchDev) <= 8192, "mailbox region");
+    h->ts = ts;
+    memcpy(h->lmap, ctx->lmap.data(), 128 * sizeof(F162));
+    for (int r = 0; r < lp; r++) h->r_hi[r][0] = ctx->r_hi[r].w[0], h->r_hi[r][1] = ctx->r_hi[r].w[1];
+    HIPCHK(ctx, hipMemcpyAsync(d->sw, h, sizeof(SwitchDev), hipMemcpyHostToDevice, d->stream));
+    tstart(ctx, T_ROUNDS);
+    for (int r = 0; r < lp; r++) {
+        const uint32_t *pair = r == lp - 1 ? d->P[(r - 1) & 1] : nullptr;  // log_packed >= 12
+        k_switch_ctl<<<1, 1024, 0, d->stream>>>(d->sw, r, lp, r == 0 ? 4 : 6, msg_buf(d, r), pair);
+        if (r < lp - 1) launch_round(ctx, r, RhoArg{{0, 0, 0}}, d->sw->rho);
+    }
+    tstop(ctx, T_ROUNDS);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipMemcpyAsync(h, d->sw, sizeof(SwitchDev), hipMemcpyDeviceToHost, d->stream));
+    HIPCHK(ctx, hipStreamSynchronize(d->stream));
+    tcollect(ctx, T_SWITCH0);
+    tcollect(ctx, T_ROUNDS);
+    ts = h->ts;
+    memcpy(rounds, h->rounds, (size_t)lp * 48);
+    memcpy(coords, h->coords, (size_t)lp * 24);
+    memcpy(opened, h->opened, 24);
+    memcpy(ctx->opened.w, h->opened, 24);
+    ctx->round = lp;
     return LBN_OK;
 }
 
@@ -1610,15 +2124,14 @@ extern "C" int lbn_switch_fold(lbn_ctx *ctx, const uint64_t *rr) {
         // p^(lp-1) has two elements: opened = p0 + rho (p0 + p1).
         uint32_t two[12];
         const uint32_t *src = (r == 0) ? (const uint32_t *)d->table : d->P[(r - 1) & 1];
+        const uint32_t *box = (const uint32_t *)(d->hbox + mb::MISC);
+        HIPCHK(ctx, hipMemcpyAsync(d->hbox + mb::MISC, src, r == 0 ? 32 : 48, hipMemcpyDeviceToHost, d->stream));
+        HIPCHK(ctx, hipStreamSynchronize(d->stream));
         if (r == 0) {
-            uint32_t w[8];
-            HIPCHK(ctx, hipMemcpyAsync(w, src, 32, hipMemcpyDeviceToHost, d->stream));
-            HIPCHK(ctx, hipStreamSynchronize(d->stream));
             for (int u = 0; u < 2; u++)
-                for (int k = 0; k < 6; k++) two[6 * u + k] = k < 4 ? w[4 * u + k] : 0;
+                for (int k = 0; k < 6; k++) two[6 * u + k] = k < 4 ? box[4 * u + k] : 0;
         } else {
-            HIPCHK(ctx, hipMemcpyAsync(two, src, 48, hipMemcpyDeviceToHost, d->stream));
-            HIPCHK(ctx, hipStreamSynchronize(d->stream));
+            memcpy(two, box, 48);
         }
         F162 p0{{(uint64_t)two[0] | ((uint64_t)two[1] << 32), (uint64_t)two[2] | ((uint64_t)two[3] << 32),
                  (uint64_t)two[4] | ((uint64_t)two[5] << 32)}};
@@ -1628,19 +2141,8 @@ extern "C" int lbn_switch_fold(lbn_ctx *ctx, const uint64_t *rr) {
         ctx->round = lp;
         return LBN_OK;
     }
-    const size_t h2 = h / 2;
-    RhoArg ra{{rr[0], rr[1], rr[2]}};
-    const uint4 *Ein = d->E[(r + 1) & 1];
-    uint4 *Eout = d->E[r & 1];
-    uint32_t *pout = d->P[r & 1];
-    uint32_t *T = msg_buf(d, r + 1);
     tstart(ctx, T_ROUNDS);
-    if (r == 0)
-        k_switch_round<true><<<launch_blocks(h2), 256, 0, d->stream>>>((const uint32_t *)d->table, h2, ra, Ein, Eout,
-                                                                        pout, T, T + 768, msg_buf(d, r));
-    else
-        k_switch_round<false><<<launch_blocks(h2), 256, 0, d->stream>>>(d->P[(r - 1) & 1], h2, ra, Ein, Eout, pout,
-                                                                         T, T + 768, msg_buf(d, r));
+    launch_round(ctx, r, RhoArg{{rr[0], rr[1], rr[2]}}, nullptr);
     tstop(ctx, T_ROUNDS);
     HIPCHK(ctx, hipGetLastError());
     update_lmap(ctx, r, rho);  // on the host while the kernel runs; only the next message needs it
@@ -1668,9 +2170,9 @@ extern "C" int lbn_row_eval(lbn_ctx *ctx, const uint64_t *p0, int n_p0, uint64_t
     std::vector<F162> lo((size_t)1 << lo_bits), hi((size_t)1 << hi_bits);
     eq_table_f162((const F162 *)p0, lo_bits, lo.data());
     eq_table_f162((const F162 *)p0 + lo_bits, hi_bits, hi.data());
-    HIPCHK(ctx, hipMemcpyAsync(d->eqlo, lo.data(), lo.size() * 24, hipMemcpyHostToDevice, d->stream));
-    HIPCHK(ctx, hipMemcpyAsync((uint8_t *)d->eqlo + 128 * 24, hi.data(), hi.size() * 24, hipMemcpyHostToDevice,
-                               d->stream));
+    memcpy(d->hbox + mb::EQ, lo.data(), lo.size() * 24);
+    memcpy(d->hbox + mb::EQ + 128 * 24, hi.data(), hi.size() * 24);
+    HIPCHK(ctx, hipMemcpyAsync(d->eqlo, d->hbox + mb::EQ, (128 + hi.size()) * 24, hipMemcpyHostToDevice, d->stream));
     tstart(ctx, T_ROW);
     k_eq_row<<<(unsigned)((s.column_len + 255) / 256), 256, 0, d->stream>>>(
         (const uint64_t *)d->eqlo, (const uint64_t *)((uint8_t *)d->eqlo + 128 * 24), lo_bits, s.column_len, d->eqrow);
@@ -1680,8 +2182,9 @@ extern "C" int lbn_row_eval(lbn_ctx *ctx, const uint64_t *p0, int n_p0, uint64_t
     k_row_finish<<<COLUMNS, 128, 0, d->stream>>>(d->rowU, (uint64_t *)d->rowOut);
     tstop(ctx, T_ROW);
     HIPCHK(ctx, hipGetLastError());
-    HIPCHK(ctx, hipMemcpyAsync(row, d->rowOut, (size_t)COLUMNS * 24, hipMemcpyDeviceToHost, d->stream));
+    HIPCHK(ctx, hipMemcpyAsync(d->hbox + mb::ROW, d->rowOut, (size_t)COLUMNS * 24, hipMemcpyDeviceToHost, d->stream));
     HIPCHK(ctx, hipStreamSynchronize(d->stream));
+    memcpy(row, d->hbox + mb::ROW, (size_t)COLUMNS * 24);
     tcollect(ctx, T_ROW);
     return LBN_OK;
 }
@@ -1721,8 +2224,11 @@ extern "C" int lbn_fold(lbn_ctx *ctx, const lbn_challenge *ch, int n_challenges,
     while ((1 << nb) <= maxlen) nb++;
     HIPCHK(ctx, hipSetDevice(ctx->device));
     DeviceState *d = ctx->dev;
-    HIPCHK(ctx, hipMemcpyAsync(d->fold_off, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice, d->stream));
-    HIPCHK(ctx, hipMemcpyAsync(d->fold_cols, cols.data(), cols.size() * 2, hipMemcpyHostToDevice, d->stream));
+    memcpy(d->hbox + mb::FOFF, off.data(), off.size() * sizeof(int));
+    memcpy(d->hbox + mb::FCOLS, cols.data(), cols.size() * 2);
+    HIPCHK(ctx, hipMemcpyAsync(d->fold_off, d->hbox + mb::FOFF, off.size() * sizeof(int), hipMemcpyHostToDevice,
+                               d->stream));
+    HIPCHK(ctx, hipMemcpyAsync(d->fold_cols, d->hbox + mb::FCOLS, cols.size() * 2, hipMemcpyHostToDevice, d->stream));
     fill_u32(d->stream, d->fold_flag, 1);
     int cus = 256;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess) cus = 256;
@@ -1751,10 +2257,11 @@ extern "C" int lbn_fold(lbn_ctx *ctx, const lbn_challenge *ch, int n_challenges,
     HIPCHK(ctx, hipGetLastError());
     // The kernel wrote the fold straight into pinned host memory over the host link, overlapped
     // with its own work; once the stream is idle the values are visible here.
-    uint32_t flag = 0;
     const uint8_t *stage = (const uint8_t *)d->hfold;
-    HIPCHK(ctx, hipMemcpyAsync(&flag, d->fold_flag, 4, hipMemcpyDeviceToHost, d->stream));
+    HIPCHK(ctx, hipMemcpyAsync(d->hbox + mb::MISC, d->fold_flag, 4, hipMemcpyDeviceToHost, d->stream));
     HIPCHK(ctx, hipStreamSynchronize(d->stream));
+    uint32_t flag;
+    memcpy(&flag, d->hbox + mb::MISC, 4);
     tcollect(ctx, T_FOLD);
     if (flag) return ctx->fail(LBN_E_RANGE, "a fold coefficient does not fit the wire codec");
     copy_parallel((uint8_t *)out, stage, n_out * 4);

@@ -99,34 +99,40 @@ int host_threads() {
     return (int)std::max(1u, std::min(n ? n : 1u, 16u));
 }
 
-// A persistent pool: workers sleep on a condition variable between jobs; a job hands out
-// indices through an atomic counter, and the caller works on it too. One job at a time: a caller
-// that finds the pool busy (another proof on another thread) runs its loop itself.
+// A persistent pool shared by every caller: workers sleep on a condition variable until a job is
+// posted; each job hands out indices through an atomic counter. Jobs of concurrent callers (proofs
+// on other threads) are served side by side: a worker takes the job with the fewest helpers, and
+// every caller works on its own job too. A job lives on its caller's stack, so the caller returns
+// only when all indices are done and no worker still holds the job.
 namespace {
+struct PoolJob {
+    const std::function<void(size_t)> *f;
+    size_t n;
+    std::atomic<size_t> next{0};
+    size_t done = 0;  // under the pool mutex
+    int users = 0;    // workers inside the job (under the pool mutex)
+};
+
 class Pool {
    public:
     static Pool &get() {
         static Pool pool;
         return pool;
     }
-    // False (nothing run) when another thread's job holds the pool.
-    bool try_run(size_t n, const std::function<void(size_t)> &f) {
-        std::unique_lock<std::mutex> job_lock(job_, std::try_to_lock);
-        if (!job_lock.owns_lock()) return false;
+    void run(size_t n, const std::function<void(size_t)> &f) {
+        PoolJob job;
+        job.f = &f;
+        job.n = n;
         {
             std::lock_guard<std::mutex> lk(m_);
-            f_ = &f;
-            n_ = n;
-            next_.store(0);
-            active_ = (int)workers_.size();
-            gen_++;
+            jobs_.push_back(&job);
         }
         cv_.notify_all();
-        work();
+        size_t mine = work(job);
         std::unique_lock<std::mutex> lk(m_);
-        done_.wait(lk, [&] { return active_ == 0; });
-        f_ = nullptr;
-        return true;
+        job.done += mine;
+        done_.wait(lk, [&] { return job.done == n && job.users == 0; });
+        jobs_.erase(std::find(jobs_.begin(), jobs_.end(), &job));
     }
     ~Pool() {
         {
@@ -141,31 +147,36 @@ class Pool {
     Pool() {
         for (int i = 1; i < host_threads(); i++) workers_.emplace_back([this] { loop(); });
     }
-    void work() {
-        for (size_t i; (i = next_.fetch_add(1)) < n_;) (*f_)(i);
+    static size_t work(PoolJob &job) {
+        size_t count = 0;
+        for (size_t i; (i = job.next.fetch_add(1)) < job.n; count++) (*job.f)(i);
+        return count;
+    }
+    PoolJob *pick() {  // under m_: an unfinished job with the fewest helpers
+        PoolJob *best = nullptr;
+        for (PoolJob *j : jobs_)
+            if (j->next.load() < j->n && (!best || j->users < best->users)) best = j;
+        return best;
     }
     void loop() {
-        uint64_t seen = 0;
+        std::unique_lock<std::mutex> lk(m_);
         for (;;) {
-            {
-                std::unique_lock<std::mutex> lk(m_);
-                cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
-                if (stop_) return;
-                seen = gen_;
-            }
-            work();
-            std::lock_guard<std::mutex> lk(m_);
-            if (--active_ == 0) done_.notify_one();
+            PoolJob *job = nullptr;
+            cv_.wait(lk, [&] { return stop_ || (job = pick()) != nullptr; });
+            if (stop_) return;
+            job->users++;
+            lk.unlock();
+            const size_t count = work(*job);
+            lk.lock();
+            job->done += count;
+            job->users--;
+            if (job->done == job->n && job->users == 0) done_.notify_all();
         }
     }
-    std::mutex job_, m_;
+    std::mutex m_;
     std::condition_variable cv_, done_;
     std::vector<std::thread> workers_;
-    const std::function<void(size_t)> *f_ = nullptr;
-    size_t n_ = 0;
-    std::atomic<size_t> next_{0};
-    int active_ = 0;
-    uint64_t gen_ = 0;
+    std::vector<PoolJob *> jobs_;
     bool stop_ = false;
 };
 }  // namespace
@@ -175,8 +186,7 @@ void parallel_for(size_t n, const std::function<void(size_t)> &f) {
         for (size_t i = 0; i < n; i++) f(i);
         return;
     }
-    if (!Pool::get().try_run(n, f))
-        for (size_t i = 0; i < n; i++) f(i);
+    Pool::get().run(n, f);
 }
 
 // ============================================================================= transcript
@@ -201,6 +211,14 @@ class Transcript {
         h_.update_with(b, n, cvs_of);
     }
     const Blake3 &hasher() const { return h_; }
+    void save(B3State &s) const {
+        h_.save_state(s);
+        s.counter = counter_;
+    }
+    void load(const B3State &s) {
+        h_.load_state(s);
+        counter_ = s.counter;
+    }
     void absorb_u64(uint64_t x) { h_.update_u64(x); }
     void absorb_b128(const B128 *xs, size_t n) {
         h_.update_u64(n);
@@ -577,17 +595,17 @@ extern "C" int lbn_prove(lbn_ctx *ctx, const uint8_t seed[32], uint8_t *proof, s
     eq_expand_f162(rb.data(), 7, batch.data());
     const double t3 = now_us();
 
+    // The rounds run on the device with the transcript (main.rs:975-992: per round absorb [e0, einf],
+    // sample "switch-round", fold; then absorb [opened]); the host continues from its final state.
     std::vector<F162> rounds(2 * (size_t)lp), coords(lp);
-    TRY(lbn_switch_begin(ctx, (const uint64_t *)batch.data()));
-    for (int r = 0; r < lp; r++) {
-        TRY(lbn_switch_msg(ctx, (uint64_t *)&rounds[2 * r]));
-        t.absorb_f162(&rounds[2 * r], 2);
-        coords[r] = t.sample_f162("switch-round", 1)[0];
-        TRY(lbn_switch_fold(ctx, coords[r].w));
-    }
     F162 opened;
-    TRY(lbn_switch_final(ctx, opened.w));
-    t.absorb_f162(&opened, 1);
+    TRY(lbn_switch_begin(ctx, (const uint64_t *)batch.data()));
+    {
+        B3State ts;
+        t.save(ts);
+        TRY(dev_switch_rounds(ctx, ts, (uint64_t *)rounds.data(), (uint64_t *)coords.data(), opened.w));
+        t.load(ts);
+    }
     const double t4 = now_us();
 
     // EvaluationPoint::msb_first (scheme/commitment.rs:163-174): p0 = reverse(coords[10..]).

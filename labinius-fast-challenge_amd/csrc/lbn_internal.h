@@ -51,6 +51,21 @@ struct PrimeInfo {
     uint32_t omega;            // primitive 1944-th root of unity
 };
 
+// A BLAKE3 hasher's whole state plus a transcript's sample counter, as plain data: the transcript
+// moves to the device for the switch rounds and back (Blake3::save_state / load_state).
+struct B3State {
+    uint32_t key[8];
+    uint32_t flags;
+    int32_t depth;
+    uint32_t stack[54][8];
+    uint32_t cv[8];
+    uint64_t chunk;
+    int32_t blocks;
+    uint32_t buf_len;
+    uint8_t buf[64];
+    uint64_t counter;
+};
+
 // Layout of one prime's commitment-transform constants (u32 offsets), built by prime_tables().
 namespace tab {
 constexpr int TWB = 0;                 // [2][324][4] binary split+twist values: 0, b, eta b, (1+eta) b
@@ -119,6 +134,10 @@ int dev_commit_launch(lbn_ctx *ctx);
 int dev_commit_cvs(lbn_ctx *ctx, size_t off, uint64_t first, size_t count, const uint32_t key[8], uint32_t flags,
                    uint32_t *cvs, uint8_t *dst, size_t head, size_t tail);
 int dev_commit_bytes(lbn_ctx *ctx, uint8_t *dst);
+// Every switch round on the device after lbn_switch_begin: the transcript (state ts in, advanced
+// state out) absorbs each message and samples each challenge there. rounds: lp x 2 F162,
+// coords: lp F162, opened: F162 (as 3 u64 each).
+int dev_switch_rounds(lbn_ctx *ctx, B3State &ts, uint64_t *rounds, uint64_t *coords, uint64_t *opened);
 
 // Host worker pool: f(i) for every i < n, on up to host_threads() threads (the caller included).
 int host_threads();

@@ -131,7 +131,7 @@ struct lbn_sched {
             lbn_ctx *ctx = slot->ctx;
             rc = job->table ? lbn_load_table(ctx, job->table, ((size_t)1 << job->log_vars) / 8)
                             : lbn_generate_table(ctx, job->seed);
-            if (rc == LBN_OK) rc = lbn_prove(ctx, job->seed, job->proof, job->cap, &job->len, nullptr);
+            if (rc == LBN_OK) rc = lbn_prove(ctx, job->seed, job->proof, job->cap, &job->len, &job->profile);
         }
         const double end = now_us();
         std::lock_guard<std::mutex> lk(m);

@@ -57,6 +57,32 @@ def cpu_baseline(log_vars_target: int):
             "sample": f"CPU oracle whole proof at k={k} ({dt * 1000:.0f} ms, OpenMP), x{scale} work ratio to k={log_vars_target}"}
 
 
+def pmc_traffic(pmc_dir, log_vars, kernel="k_commit_mac", copy_kernel="k_copy_u4"):
+    """HBM bytes per launch of `kernel` from rocprofv3 counter CSVs: FETCH_SIZE and WRITE_SIZE (KiB per
+    dispatch), each scaled so that the copy kernel (table bytes read once and written once) reads exactly
+    its size."""
+    import csv
+    import glob
+
+    def per_kernel(sub, counter):
+        vals = {}
+        for f in glob.glob(os.path.join(pmc_dir, sub, "*counter_collection.csv")):
+            for r in csv.DictReader(open(f)):
+                if r["Counter_Name"] == counter:
+                    vals.setdefault(r["Kernel_Name"].split("(")[0], []).append(float(r["Counter_Value"]) * 1024)
+        return vals
+
+    fetch, write = per_kernel("fetch", "FETCH_SIZE"), per_kernel("write", "WRITE_SIZE")
+    if kernel not in fetch or kernel not in write or copy_kernel not in fetch or copy_kernel not in write:
+        return None, None
+    size = (1 << log_vars) // 8
+    f_scale, w_scale = size / max(fetch[copy_kernel]), size / max(write[copy_kernel])
+    f = statistics.mean(fetch[kernel]) * f_scale
+    w = statistics.mean(write[kernel]) * w_scale
+    return round(f + w), {"fetch_bytes": round(f), "write_bytes": round(w), "fetch_scale": round(f_scale, 4),
+                          "write_scale": round(w_scale, 4)}
+
+
 def batched_instances(n=64, first=100):
     """The batched workload: instance i has seed bench_seed(first + i) and k = 20 + seed[0] mod 9."""
     return [(20 + bench_seed(first + i)[0] % 9, bench_seed(first + i)) for i in range(n)]
@@ -151,6 +177,12 @@ def main():
     ap.add_argument("--serving", action="store_true", help="run the serving loop instead (prints its own line)")
     ap.add_argument("--requests", type=int, default=10_000)
     ap.add_argument("--rate", type=float, default=400.0, help="serving arrivals per second")
+    ap.add_argument("--pmc-dir", default=None,
+                    help="directory holding FETCH_SIZE and WRITE_SIZE rocprofv3 --pmc runs of this bench with "
+                         "--copy-calibration (subdirectories fetch/ and write/): fills roofline.traffic")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run one plain device-to-device copy of the table size first (a known-size kernel "
+                         "for scaling profiler byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the prover's kernels run on one device per proof; "
@@ -164,6 +196,18 @@ def main():
     k = args.log_vars
     p = m.Prover(k)
     p.set_modulus(m.DEFAULT_Q)
+    if args.copy_calibration:  # the library's copy kernel over 2^k/8 bytes: read once, written once
+        import torch
+        src = torch.empty((1 << k) // 8, dtype=torch.uint8, device="cuda")  # allocation only
+        p.load_table_device(src.data_ptr(), src.numel())
+        del src
+    # The table handed over from host memory (pageable) as the reference's loader does: the
+    # PCIe-inclusive cost, reported beside the step time, never inside it.
+    host_table = np.frombuffer(np.random.default_rng(7).bytes((1 << k) // 8), dtype=np.uint8)
+    t0 = time.perf_counter()
+    p.load_table(host_table)
+    table_h2d_ms = (time.perf_counter() - t0) * 1000
+    del host_table
     for w in range(args.warmup):      # with the pass timers on, so their first use is not timed
         p.generate_table(bench_seed(1000 + w))
         p.set_timing(True)
@@ -205,6 +249,8 @@ def main():
     roofline = {"bound": "valu", "kernel": "k_commit_mac", "achieved": round(mulmod_t, 3),
                 "peak": round(MULMOD_PEAK_T, 3), "unit": "Tmulmod/s", "frac": round(mulmod_t / MULMOD_PEAK_T, 4),
                 "traffic": None, "launch_ms": round(per_launch_ms, 3)}
+    if args.pmc_dir:
+        roofline["traffic"], roofline["traffic_detail"] = pmc_traffic(args.pmc_dir, k)
     line = {
         "metric": "prover ms (commit+open, 2^31 evals) on 1 MI355X",
         "value": round(med, 3), "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -217,6 +263,7 @@ def main():
         "host_phases_ms": {f.replace("_us", ""): round(statistics.mean(pr[f] for pr in profs) / 1000, 3)
                            for f in profs[0]},
         "step_ms": [round(t, 3) for t in times],
+        "table_h2d_ms": round(table_h2d_ms, 3),
         "hbm_passes": {n: {"ms": round(v["ms"], 3), "gbs": round(v["achieved_gbs"], 1),
                            "frac": round(v["achieved_gbs"] / HBM_PEAK_GBS, 4)}
                        for n, v in passes.items() if v["bound"] == "hbm"},

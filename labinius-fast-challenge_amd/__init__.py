@@ -153,6 +153,10 @@ class Prover:
         t = np.ascontiguousarray(table, dtype=np.uint8)
         self._check(lib().lbn_load_table(self.h, _p(t), t.size))
 
+    def load_table_device(self, device_ptr: int, nbytes: int):
+        """Copy the table from device memory (device_ptr: an address on this context's GPU)."""
+        self._check(lib().lbn_load_table_device(self.h, ctypes.c_void_p(device_ptr), nbytes))
+
     def generate_table(self, seed: bytes):
         self._check(lib().lbn_generate_table(self.h, _b(seed)))
 

@@ -51,11 +51,12 @@ struct lbn::DeviceState {
     uint16_t *fold_cols = nullptr;
     uint32_t *ntt_tabs = nullptr;   // MAX_PRIMES * tab::SIZE
     uint32_t *ahat = nullptr;       // nprimes * RE * 648
-    unsigned long long *cacc = nullptr;  // nprimes * 1024 * 648
+    uint32_t *cacc = nullptr;       // nprimes * splits * 1024 * 648: the MAC's partial sums (< p)
     uint8_t *cbytes = nullptr;      // 1024 * 648 * width
     uint64_t *amat = nullptr;       // RE * 648 * 2 (staging for the matrix)
     size_t ahat_elems = 0;
     int alloc_primes = 0, alloc_width = 0;  // sizes of cacc / cbytes / cvs / hstage
+    int splits = 0;                 // ring-element ranges of the MAC (blocks per column group)
     uint32_t *cvs = nullptr;        // chunk chaining values of the commitment bytes
     // Pinned host staging: the commitment bytes (copied on copy_stream once the commitment is
     // ready, while the opening runs) and the fold output, which the fold kernel writes directly.
@@ -119,6 +120,11 @@ static void tcollect(lbn_ctx *ctx, int pass) {
 // =============================================================================================
 // small kernels
 // =============================================================================================
+
+__global__ void k_copy_u4(const uint4 *src, uint4 *dst, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
 
 __global__ void k_fill_u32(uint32_t *p, size_t n, uint32_t v) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1264,7 +1270,8 @@ __global__ __launch_bounds__(64) void k_ahat(const uint64_t *amat, const uint32_
 }
 
 // Forward transform of the binary ring elements of TC columns, MAC against ahat, over the ring
-// elements [m0, m1) of this block; partial sums go into cacc with u64 atomics. 576 threads:
+// elements [m0, m1) of this block; its partial sums, reduced mod p, go to its own slice of cacc
+// (slice blockIdx.y; the finish kernel adds the slices). 576 threads:
 // thread t handles column t / 36, half (t / 18) % 2, and n2 (pass A) / k1 (pass B) = t % 18.
 // The MAC is a plain 64-bit multiply-add of v < 6p by a < p (each product below 6p^2 < 2^32 p),
 // folded every fourth element as acc = hi * (2^32 mod p) + lo (< 2^32 p): the largest value
@@ -1315,7 +1322,7 @@ __device__ __forceinline__ void stage1_lookup(const uint32_t (&ra)[4], const uin
 __global__ __launch_bounds__(MAC_THREADS, 5) void k_commit_mac(const uint32_t *table32, size_t clen, int m_per, int re,
                                                             const uint32_t *ahat, const uint32_t *tabs, PC pc,
                                                             uint32_t c32, uint32_t r2, Roots18 R,
-                                                            unsigned long long *cacc) {
+                                                            uint32_t *cacc) {
     __shared__ uint32_t Y[TC * 2 * 324];
     __shared__ uint32_t s_st1[tab::ST1_SIZE];
     __shared__ uint32_t s_twp[2 * 324];
@@ -1389,15 +1396,18 @@ __global__ __launch_bounds__(MAC_THREADS, 5) void k_commit_mac(const uint32_t *t
         }
     }
     // Reduce to [0, p): one fold leaves acc < 2^32 p, so REDC(acc) = acc R^-1 mod p in [0, 2p), and a
-    // Montgomery product with R^2 gives acc mod p. At most 16 blocks add into one entry: < 16 p.
+    // Montgomery product with R^2 gives acc mod p.
     fold();
-    unsigned long long *dst = cacc + (size_t)(c0 + col) * 648 + h * 324 + idx * 18;
+    uint32_t red[18];
 #pragma unroll
     for (int p = 0; p < 18; p++) {
         const uint32_t m = (uint32_t)acc[p] * pc.pinv_neg;
         const uint32_t redc = (uint32_t)((acc[p] + (uint64_t)m * pc.p) >> 32);
-        atomicAdd(dst + p, (unsigned long long)mont(redc, r2, pc));
+        red[p] = mont(redc, r2, pc);
     }
+    uint2 *dst = (uint2 *)(cacc + ((size_t)blockIdx.y * COLUMNS + c0 + col) * 648 + h * 324 + idx * 18);
+#pragma unroll
+    for (int q = 0; q < 9; q++) dst[q] = make_uint2(red[2 * q], red[2 * q + 1]);
 }
 
 // Inverse transform per column and prime, Garner over the primes, reduction mod Q, packing.
@@ -1407,7 +1417,7 @@ struct CrtArg {
     uint64_t q_lo, q_hi;
     uint64_t half_m[3];                     // floor(M / 2), M = product of the primes (192 bits)
     uint64_t m[3];
-    int nprimes, width;
+    int nprimes, width, splits;
     PC pc[MAX_PRIMES];
     Roots18 roots[MAX_PRIMES];
 };
@@ -1435,7 +1445,7 @@ __device__ __forceinline__ void u192_sub(const uint64_t (&a)[3], const uint64_t 
     r[0] = r0, r[1] = r1, r[2] = a[2] - b[2] - b1;
 }
 
-__global__ __launch_bounds__(64) void k_commit_finish(const unsigned long long *cacc, const uint32_t *tabs_all,
+__global__ __launch_bounds__(64) void k_commit_finish(const uint32_t *cacc, const uint32_t *tabs_all,
                                                       CrtArg crt, uint8_t *outb) {
     __shared__ uint32_t Y[2 * 324];
     __shared__ uint32_t x[648];
@@ -1448,18 +1458,23 @@ __global__ __launch_bounds__(64) void k_commit_finish(const unsigned long long *
         uint32_t w[9];
 #pragma unroll
         for (int j = 0; j < 9; j++) w[j] = crt.roots[k].i[j];
+        {  // the sum over the MAC's slices (at most 16 residues below p), coalesced over the slots
+            const uint32_t *src = cacc + (size_t)k * crt.splits * COLUMNS * 648 + (size_t)c * 648;
+            for (int i = t; i < 648; i += 64) {
+                uint64_t sum = 0;
+                for (int sl = 0; sl < crt.splits; sl++) sum += src[(size_t)sl * COLUMNS * 648 + i];
+                sum = sum >= 8ull * pc.p ? sum - 8ull * pc.p : sum;
+                sum = sum >= 4ull * pc.p ? sum - 4ull * pc.p : sum;
+                sum = sum >= 2ull * pc.p ? sum - 2ull * pc.p : sum;
+                x[i] = (uint32_t)(sum >= pc.p ? sum - pc.p : sum);
+            }
+        }
+        __syncthreads();
         if (t < 36) {
             // inverse pass B: thread (h, k1 = idx)
-            const unsigned long long *src = cacc + ((size_t)k * 1024 + c) * 648 + h * 324 + idx * 18;
             uint32_t v[18];
 #pragma unroll
-            for (int p = 0; p < 18; p++) {  // sums of at most 16 residues below p
-                uint64_t x = src[p];
-                x = x >= 8ull * pc.p ? x - 8ull * pc.p : x;
-                x = x >= 4ull * pc.p ? x - 4ull * pc.p : x;
-                x = x >= 2ull * pc.p ? x - 2ull * pc.p : x;
-                v[p] = (uint32_t)(x >= pc.p ? x - pc.p : x);
-            }
+            for (int p = 0; p < 18; p++) v[p] = x[h * 324 + idx * 18 + p];
             dft18_inv(v, w, pc);
 #pragma unroll
             for (int n2 = 0; n2 < 18; n2++) Y[h * 324 + idx * 18 + n2] = v[n2];
@@ -1617,7 +1632,12 @@ extern "C" int lbn_load_table_device(lbn_ctx *ctx, const void *device_bytes, siz
     if (bytes != ctx->shape.table_bytes)
         return ctx->fail(LBN_E_ARG, "witness table has the wrong length");
     HIPCHK(ctx, hipSetDevice(ctx->device));
-    HIPCHK(ctx, hipMemcpyAsync(ctx->dev->table, device_bytes, bytes, hipMemcpyDeviceToDevice, ctx->dev->stream));
+    if ((uintptr_t)device_bytes % 16) return ctx->fail(LBN_E_ARG, "device table must be 16-byte aligned");
+    // a plain 16-byte-per-lane copy kernel (reads the table once, writes it once)
+    const size_t words = bytes / 16;
+    k_copy_u4<<<(unsigned)std::min<size_t>((words + 255) / 256, 65536), 256, 0, ctx->dev->stream>>>(
+        (const uint4 *)device_bytes, ctx->dev->table, words);
+    HIPCHK(ctx, hipGetLastError());
     HIPCHK(ctx, hipStreamSynchronize(ctx->dev->stream));
     ctx->table_loaded = true;
     ctx->partials_done = false;
@@ -1684,9 +1704,21 @@ static int modulus_width(u128 q) {
 }
 
 // Device bytes of the buffers that scale with the CRT prime count and the coefficient width.
-static size_t commit_buffer_bytes(int np, int width) {
+// Ring-element ranges of the MAC: about 1024 blocks per prime (64 column groups x splits). The
+// finish kernel reduces a sum of at most 16 slices with four conditional subtractions.
+static int mac_m_per(size_t re) {
+    const int msplit = (int)std::max<size_t>(1, std::min<size_t>(re, 1024 / (COLUMNS / TC)));
+    return (int)((re + msplit - 1) / msplit);
+}
+static int mac_splits(size_t re) {
+    const int m_per = mac_m_per(re);
+    return (int)((re + m_per - 1) / m_per);
+}
+static_assert(1024 / (COLUMNS / TC) <= 16, "more than 16 partial sums per MAC slot");
+
+static size_t commit_buffer_bytes(int np, int width, size_t re) {
     const size_t clen = (size_t)COLUMNS * N * width;
-    return (size_t)np * COLUMNS * N * 8 + (clen + 64) + (clen / 1024 * 32 + 64);
+    return (size_t)np * mac_splits(re) * COLUMNS * N * 4 + (clen + 64) + (clen / 1024 * 32 + 64);
 }
 
 extern "C" size_t lbn_ctx_footprint(int log_vars, uint64_t q_lo, uint64_t q_hi) {
@@ -1704,7 +1736,7 @@ extern "C" size_t lbn_ctx_footprint(int log_vars, uint64_t q_lo, uint64_t q_hi) 
     b += (size_t)MAX_PRIMES * tab::SIZE * 4;
     b += (size_t)np * s.ring_elements * N * 4;  // ahat
     b += s.ring_elements * N * 16;              // matrix staging while the modulus is set
-    b += commit_buffer_bytes(np, modulus_width(q));
+    b += commit_buffer_bytes(np, modulus_width(q), s.ring_elements);
     return b;
 }
 
@@ -1735,7 +1767,8 @@ static int alloc_commit_buffers(lbn_ctx *ctx, int np, int width) {
     d->cacc = nullptr, d->cbytes = nullptr, d->cvs = nullptr, d->hstage = nullptr, d->hcvs = nullptr;
     d->alloc_primes = d->alloc_width = 0;
     const size_t clen = (size_t)COLUMNS * N * width;
-    HIPCHK(ctx, hipMalloc(&d->cacc, (size_t)np * COLUMNS * N * 8));
+    d->splits = mac_splits(ctx->shape.ring_elements);
+    HIPCHK(ctx, hipMalloc(&d->cacc, (size_t)np * d->splits * COLUMNS * N * 4));
     HIPCHK(ctx, hipMalloc(&d->cbytes, clen + 64));  // slack: the chunk hashing reads one word past
     HIPCHK(ctx, hipMalloc(&d->cvs, clen / 1024 * 32 + 64));
     HIPCHK(ctx, hipHostMalloc((void **)&d->hstage, clen));
@@ -1892,13 +1925,9 @@ int lbn::dev_commit_launch(lbn_ctx *ctx) {
     HIPCHK(ctx, hipStreamWaitEvent(ctx->dev->stream, ctx->dev->ev_staged, 0));
     DeviceState *d = ctx->dev;
     const int np = ctx->nprimes;
-    fill_u32(d->stream, d->cacc, (size_t)np * COLUMNS * N * 2);
     const int re = (int)s.ring_elements;
-    // ~1024 blocks per prime; at most 16 blocks add into one sum (the MAC's 64-bit bound)
-    int msplit = std::max(1, std::min(re, 1024 / (COLUMNS / TC)));
-    static_assert(1024 / (COLUMNS / TC) <= 16, "more than 16 partial sums per cacc entry");
-    int m_per = (re + msplit - 1) / msplit;
-    msplit = (re + m_per - 1) / m_per;
+    const int m_per = mac_m_per(re), msplit = mac_splits(re);
+    if (msplit != d->splits) return ctx->fail(LBN_E_STATE, "MAC slices not allocated for this shape");
     tstart(ctx, T_MAC);
     for (int k = 0; k < np; k++) {
         PC pc{ctx->primes[k].p, ctx->primes[k].pinv_neg};
@@ -1908,13 +1937,14 @@ int lbn::dev_commit_launch(lbn_ctx *ctx) {
                                                            d->ntt_tabs + (size_t)k * tab::SIZE, pc,
                                                            (uint32_t)((1ull << 32) % pc.p), ctx->primes[k].r2,
                                                            d->roots[k],
-                                                           d->cacc + (size_t)k * COLUMNS * N);
+                                                           d->cacc + (size_t)k * msplit * COLUMNS * N);
         HIPCHK(ctx, hipGetLastError());
     }
     tstop(ctx, T_MAC);
     CrtArg crt{};
     crt.nprimes = np;
     crt.width = ctx->width;
+    crt.splits = msplit;
     crt.q_lo = (uint64_t)ctx->q;
     crt.q_hi = (uint64_t)(ctx->q >> 64);
     {

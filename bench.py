@@ -11,6 +11,7 @@ dominant kernel's roofline, and a CPU baseline: the scalar C oracle (same algori
 a bounded sample (a full proof at 2^23 evaluations) and scaled by the work ratio to 2^31.
 
     python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--log-vars 31] [--no-cpu-baseline] [--no-batched]
+                    [--no-commit-micro]
     python bench.py --serving [--requests 10000] [--rate 400] [--workers 8]
 """
 import argparse
@@ -35,6 +36,18 @@ VALU_SLOTS_T = 256 * 64 * 2.4e9 / 1e12  # int32 VALU issue: 256 CUs x 64 lanes x
 # Montgomery product (two half-rate 32x32->64 multiply-adds and one 32-bit multiply).
 MULMOD_PER_ELEMENT_PRIME = 4212
 MULMOD_PEAK_T = VALU_SLOTS_T / 5
+
+
+# Commit micro-benchmark: 64 columns x 4096 binary ring elements (a 16 MiB witness from seed 0),
+# Q = 3889 * 9721 * 17497, keys of 1 and 16 ring rows. Its roofline counts int32 VALU issue slots:
+# per (ring element, column, CRT prime) the transform's 3,564 modular products at 5 slots each, and
+# per key row 648 multiply-adds at 2 slots (one half-rate 32x32->64 multiply-add).
+Q_C3 = 661_473_542_593
+C3_COLUMNS, C3_ELEMENTS = 64, 4096
+
+
+def c3_slots(rows: int) -> int:
+    return 3564 * 5 + rows * 648 * 2
 
 
 def bench_seed(i: int) -> bytes:
@@ -165,6 +178,36 @@ def serving(m, requests, rate, workers, budget):
             "identical_sample": proofs_mix == proofs_seq}
 
 
+def commit_micro(m, steps, warmup):
+    out = {"workload": f"commit of {C3_COLUMNS} columns x {C3_ELEMENTS} binary ring elements (16 MiB witness, "
+                       f"seed 0), Q={Q_C3}", "q": Q_C3}
+    for rows in (1, 16):
+        c = m.Commitment(C3_COLUMNS, C3_ELEMENTS, rows, Q_C3)
+        try:
+            c.generate(bench_seed(0))
+            for _ in range(max(1, warmup)):
+                c.launch()
+            c.read()
+            c.set_timing(True)
+            for _ in range(steps):
+                c.launch()
+            t = c.timing()
+            primes = c.primes
+        finally:
+            c.close()
+        ms = t["total_ms"] / t["runs"]
+        slots = C3_COLUMNS * C3_ELEMENTS * primes * c3_slots(rows)
+        achieved = slots / (ms / 1e3) / 1e12
+        out[f"rows{rows}"] = {
+            "rows": rows, "primes": primes, "ms": round(ms, 4),
+            "kernels_ms": {k.replace("_ms", ""): round(t[k] / t["runs"], 4)
+                           for k in ("transform_ms", "mac_ms", "sum_ms", "finish_ms")},
+            "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": round(VALU_SLOTS_T, 3),
+                         "unit": "T int32 slots/s", "frac": round(achieved / VALU_SLOTS_T, 4)},
+        }
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -173,6 +216,9 @@ def main():
     ap.add_argument("--log-vars", type=int, default=31)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-instance batched measurement")
+    ap.add_argument("--no-commit-micro", action="store_true", help="skip the commit micro-benchmark")
+    ap.add_argument("--commit-micro-only", action="store_true",
+                    help="run only the commit micro-benchmark (prints its own line)")
     ap.add_argument("--workers", type=int, default=8, help="proofs in flight for the batched / serving runs")
     ap.add_argument("--serving", action="store_true", help="run the serving loop instead (prints its own line)")
     ap.add_argument("--requests", type=int, default=10_000)
@@ -192,6 +238,9 @@ def main():
     m = amd.load()
     if args.serving:
         print(json.dumps(serving(m, args.requests, args.rate, args.workers, 200 << 30)), flush=True)
+        return
+    if args.commit_micro_only:
+        print(json.dumps(commit_micro(m, args.steps, args.warmup)), flush=True)
         return
     k = args.log_vars
     p = m.Prover(k)
@@ -269,6 +318,8 @@ def main():
                        for n, v in passes.items() if v["bound"] == "hbm"},
     }
     p.close()
+    if not args.no_commit_micro:
+        line["commit_micro"] = commit_micro(m, max(args.steps, 5), args.warmup)
     if not args.no_batched:
         line["batched"] = batched(m, args.workers)
     if not args.no_cpu_baseline:

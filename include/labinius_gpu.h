@@ -165,6 +165,52 @@ int lbn_sched_get_stats(lbn_sched *s, lbn_sched_stats *out);
 /* Drains, stops the workers and frees every context. */
 void lbn_sched_destroy(lbn_sched *s);
 
+/* ----------------------------------------------------------------------------------------------
+ * Standalone commitment (the commit kernels on their own, for keys of one or several ring rows).
+ * Replaces the library's column commitment sms::commit_columns_rows
+ * (pcs/src/simd/sms_commit.rs:1473) and its NTT-domain form simd::commit::commit_limbs
+ * (pcs/tests/commit.rs:29-42), with the result in the coefficient domain mod Q as rq.rs defines it
+ * (verifier/proof_verify/src/rq.rs:264-287). A witness is `columns` columns of `elements` binary
+ * ring elements, 4 u128 words (64 bytes) per element, column-major like the prover's table
+ * (element m of column c at bytes (c * elements + m) * 64). Ring element m of key row r is element
+ * r * elements + m of rq::ajtai_matrix's expansion for (Q, elements) (rq.rs:53-144), so row 0 is
+ * ajtai_matrix(Q, elements) itself. Output: rows x columns x 648 coefficients in [0, Q), each
+ * ceil(bits(Q-1)/8) bytes LE, row-major ([row][column][t]).
+ * ---------------------------------------------------------------------------------------------- */
+typedef struct lbn_cmt lbn_cmt;
+
+typedef struct {
+    double total_ms;     /* whole commitments (first kernel start to last kernel end) */
+    double transform_ms; /* forward transforms of the witness (key of several rows only) */
+    double mac_ms;       /* products with the key (rows == 1: transform and MAC fused) */
+    double sum_ms;       /* sums of the partial-sum slices */
+    double finish_ms;    /* inverse transforms, CRT, reduction mod Q, packing */
+    int runs;
+} lbn_cmt_timing;
+
+/* Output bytes for a key of `rows` rows over `columns` columns under Q (0 if invalid). */
+size_t lbn_cmt_len(int columns, int rows, uint64_t q_lo, uint64_t q_hi);
+/* rows in [1, 16]; elements in [1, 65536]; columns in [16, 4096], a multiple of 16 (rows == 1)
+ * or of 64 (rows > 1); Q in [34,816,965,049, 2^96). Expands and transforms the key (setup). */
+int lbn_cmt_create(int device, int columns, int elements, int rows, uint64_t q_lo, uint64_t q_hi,
+                   lbn_cmt **out);
+void lbn_cmt_destroy(lbn_cmt *c);
+const char *lbn_cmt_error(const lbn_cmt *c);
+int lbn_cmt_primes(const lbn_cmt *c);
+/* The witness: columns * elements * 64 bytes from host or (16-byte aligned) device memory, or the
+ * table writer's XOF of a seed (verifier/table_writer/src/main.rs:14-22), its first bytes. */
+int lbn_cmt_load(lbn_cmt *c, const void *host_words, size_t bytes);
+int lbn_cmt_load_device(lbn_cmt *c, const void *device_words, size_t bytes);
+int lbn_cmt_generate(lbn_cmt *c, const uint8_t seed[32]);
+/* Queues one commitment on the context's stream (returns without waiting). */
+int lbn_cmt_launch(lbn_cmt *c);
+/* Waits for the queued commitments and copies the last one's packed coefficients. */
+int lbn_cmt_read(lbn_cmt *c, uint8_t *out, size_t len);
+/* Kernel timing with HIP events on the context's stream (diagnostics, no counterpart in the
+ * reference): set_timing resets the sums; get_timing waits and returns them. */
+int lbn_cmt_set_timing(lbn_cmt *c, int enable);
+int lbn_cmt_get_timing(lbn_cmt *c, lbn_cmt_timing *t);
+
 #ifdef __cplusplus
 }
 #endif

@@ -49,6 +49,11 @@ class Timing(ctypes.Structure):
 
 
 
+class CmtTiming(ctypes.Structure):
+    _fields_ = [(f, ctypes.c_double) for f in ("total_ms", "transform_ms", "mac_ms", "sum_ms", "finish_ms")] + [
+        ("runs", ctypes.c_int)]
+
+
 class SchedStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint64) for n in ("submitted", "completed", "failed", "contexts_created", "evictions",
                                               "peak_bytes", "used_bytes")]
@@ -98,6 +103,22 @@ def lib():
         L.lbn_sched_get_stats.argtypes = [P, P]
         L.lbn_sched_destroy.argtypes = [P]
         L.lbn_sched_destroy.restype = None
+        L.lbn_cmt_len.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64]
+        L.lbn_cmt_len.restype = ctypes.c_size_t
+        L.lbn_cmt_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
+                                     ctypes.c_uint64, ctypes.POINTER(P)]
+        L.lbn_cmt_destroy.argtypes = [P]
+        L.lbn_cmt_destroy.restype = None
+        L.lbn_cmt_error.argtypes = [P]
+        L.lbn_cmt_error.restype = ctypes.c_char_p
+        L.lbn_cmt_primes.argtypes = [P]
+        L.lbn_cmt_load.argtypes = [P, P, ctypes.c_size_t]
+        L.lbn_cmt_load_device.argtypes = [P, P, ctypes.c_size_t]
+        L.lbn_cmt_generate.argtypes = [P, P]
+        L.lbn_cmt_launch.argtypes = [P]
+        L.lbn_cmt_read.argtypes = [P, P, ctypes.c_size_t]
+        L.lbn_cmt_set_timing.argtypes = [P, ctypes.c_int]
+        L.lbn_cmt_get_timing.argtypes = [P, P]
         _lib = L
     return _lib
 
@@ -307,6 +328,69 @@ class Scheduler:
             lib().lbn_sched_destroy(self.h)
             self.h = None
             self._live.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Commitment:
+    """One lbn_cmt: the commitment of `columns` witness columns of `elements` binary ring elements
+    under a key of `rows` ring rows (row r's element m is matrix element r * elements + m)."""
+
+    def __init__(self, columns: int, elements: int, rows: int = 1, q: int = DEFAULT_Q, device: int = 0):
+        self.columns, self.elements, self.rows, self.q = columns, elements, rows, q
+        h = ctypes.c_void_p()
+        rc = lib().lbn_cmt_create(device, columns, elements, rows, q & (2**64 - 1), q >> 64, ctypes.byref(h))
+        if rc != 0:
+            raise LbnError(rc, "lbn_cmt_create failed")
+        self.h = h
+        self.out_len = lib().lbn_cmt_len(columns, rows, q & (2**64 - 1), q >> 64)
+
+    def _check(self, rc):
+        if rc != 0:
+            raise LbnError(rc, lib().lbn_cmt_error(self.h).decode())
+
+    @property
+    def primes(self) -> int:
+        return lib().lbn_cmt_primes(self.h)
+
+    def load(self, words: bytes):
+        self._check(lib().lbn_cmt_load(self.h, _b(bytes(words)), len(words)))
+
+    def load_device(self, ptr: int, nbytes: int):
+        self._check(lib().lbn_cmt_load_device(self.h, ctypes.c_void_p(ptr), nbytes))
+
+    def generate(self, seed: bytes):
+        self._check(lib().lbn_cmt_generate(self.h, _b(seed)))
+
+    def launch(self):
+        self._check(lib().lbn_cmt_launch(self.h))
+
+    def read(self) -> np.ndarray:
+        """The last commitment: uint8 array of shape (rows, columns, 648 * width)."""
+        out = np.empty(self.out_len, dtype=np.uint8)
+        self._check(lib().lbn_cmt_read(self.h, _p(out), out.size))
+        return out.reshape(self.rows, self.columns, -1)
+
+    def commit(self) -> np.ndarray:
+        self.launch()
+        return self.read()
+
+    def set_timing(self, enable: bool = True):
+        self._check(lib().lbn_cmt_set_timing(self.h, 1 if enable else 0))
+
+    def timing(self) -> dict:
+        t = CmtTiming()
+        self._check(lib().lbn_cmt_get_timing(self.h, ctypes.addressof(t)))
+        return {f: getattr(t, f) for f, _ in CmtTiming._fields_}
+
+    def close(self):
+        if self.h:
+            lib().lbn_cmt_destroy(self.h)
+            self.h = None
 
     def __del__(self):
         try:

@@ -27,6 +27,7 @@ using namespace lbn::dev;
 
 namespace lbn {
 void ajtai_matrix_host(u128 q, size_t elements, u128 *out);  // host BLAKE3 expansion
+void ajtai_elements_host(u128 q, size_t elements, size_t first, size_t count, u128 *out);
 bool prime_tables(const PrimeInfo &pi, std::vector<uint32_t> &tabs);
 }  // namespace lbn
 
@@ -68,8 +69,8 @@ struct lbn::DeviceState {
     int32_t *fold_host = nullptr;   // device address of hfold
     hipStream_t copy_stream = nullptr;
     hipEvent_t ev_commit = nullptr, ev_staged = nullptr;
-    hipEvent_t ev[2 * 9] = {};
-    Roots18 roots[MAX_PRIMES] = {};      // start/stop per timed pass
+    hipEvent_t ev[2 * 9] = {};           // start/stop per timed pass
+    Roots18 roots[MAX_PRIMES] = {};
 };
 
 // Regions of a context's pinned mailbox. Every small host<->device transfer goes through it, so
@@ -1214,7 +1215,9 @@ __device__ __forceinline__ void passA_twist_store(const uint32_t (&v)[18], int n
     }
 }
 
-// Transformed matrix: ahat[m][slot] = T(A_m mod p)[slot], canonical residues.
+// Transformed matrix: ahat[m][slot] = T(A_m mod p)[slot], canonical residues. With row_stride
+// > 0 the block's element is m = blockIdx.x of key row `row` and lands at ahat[m][slot][row] with
+// row_stride rows per slot (the layout of the key-row products).
 // One element per block; threads 0..35 run the two halves' transforms.
 // The matrix is taken centred (a > Q/2 stands for a - Q, the same residue mod Q), which halves
 // the range of the exact product and so the number of primes.
@@ -1224,7 +1227,8 @@ __device__ __forceinline__ uint32_t u128_mod(u128 x, uint32_t p) {
 }
 
 __global__ __launch_bounds__(64) void k_ahat(const uint64_t *amat, const uint32_t *tabs, PC pc, Roots18 R,
-                                             uint32_t r2, uint64_t q_lo, uint64_t q_hi, uint32_t *ahat) {
+                                             uint32_t r2, uint64_t q_lo, uint64_t q_hi, uint32_t *ahat,
+                                             int row, int row_stride) {
     __shared__ uint32_t a[648];
     __shared__ uint32_t Y[2 * 324];
     const size_t m = blockIdx.x;
@@ -1264,7 +1268,8 @@ __global__ __launch_bounds__(64) void k_ahat(const uint64_t *amat, const uint32_
 #pragma unroll
         for (int p = 0; p < 18; p++) {
             const uint32_t x = red6(v[p], 2 * pc.p, 4 * pc.p);
-            ahat[m * 648 + h * 324 + idx * 18 + p] = min(x, x - pc.p);
+            const size_t slot = m * 648 + h * 324 + idx * 18 + p;
+            ahat[row_stride ? slot * row_stride + row : slot] = min(x, x - pc.p);
         }
     }
 }
@@ -1275,8 +1280,9 @@ __global__ __launch_bounds__(64) void k_ahat(const uint64_t *amat, const uint32_
 // thread t handles column t / 36, half (t / 18) % 2, and n2 (pass A) / k1 (pass B) = t % 18.
 // The MAC is a plain 64-bit multiply-add of v < 6p by a < p (each product below 6p^2 < 2^32 p),
 // folded every fourth element as acc = hi * (2^32 mod p) + lo (< 2^32 p): the largest value
-// before a fold, 2^32 p + 4 * 6p^2 < 5 * 2^61.42, fits 64 bits. Two folds at the end leave each
-// block's sum below p^2 + 2^32, so 16 blocks' atomics stay below 2^64 too.
+// before a fold, 2^32 p + 4 * 6p^2 < 5 * 2^61.42, fits 64 bits. The block's sums are reduced to
+// [0, p) and stored into slice blockIdx.y, laid out [slice][column][slot] over gridDim.x * TC
+// columns.
 constexpr int TC = 16;
 constexpr int MAC_THREADS = TC * 36;
 
@@ -1319,6 +1325,40 @@ __device__ __forceinline__ void stage1_lookup(const uint32_t (&ra)[4], const uin
     v[J + 9] = st1[(J * 2 + 1) * 32 + s];
 }
 
+// Pass A of one binary ring element for thread (half, n2 = idx): stage 1 by lookup, stages 2
+// and 3, the twist by beta^n2 rho^(n2 k1), and the store of Y[k1][n2] of the half (bits: the
+// element's four words; wa, wb, sa, sb: the thread's word indices and shifts).
+__device__ __forceinline__ void commit_pass_a(const uint4 *bits, int wa, int wb, uint32_t sa, uint32_t sb,
+                                              const uint32_t *st1, const uint32_t *twp, const uint32_t (&w)[9],
+                                              PC pc, uint32_t *Yh, int idx) {
+    uint32_t v[18], ra[4], rb[4];
+    shr128(bits[wa], sa, ra);
+    shr128(bits[wb], sb, rb);
+    stage1_lookup<0>(ra, rb, st1, v);
+    stage1_lookup<1>(ra, rb, st1, v);
+    stage1_lookup<2>(ra, rb, st1, v);
+    stage1_lookup<3>(ra, rb, st1, v);
+    stage1_lookup<4>(ra, rb, st1, v);
+    stage1_lookup<5>(ra, rb, st1, v);
+    stage1_lookup<6>(ra, rb, st1, v);
+    stage1_lookup<7>(ra, rb, st1, v);
+    stage1_lookup<8>(ra, rb, st1, v);
+    dft18_stage23<false>(v, w, pc);
+#pragma unroll
+    for (int p = 0; p < 18; p++) {
+        const int k1 = krev18(p);
+        Yh[k1 * 18 + idx] = mont_lazy(v[p], twp[k1], pc);
+    }
+}
+
+// Pass B for thread (half, k1 = idx): the 18 outputs of slots h*324 + idx*18 + p, in [0, 6p).
+__device__ __forceinline__ void commit_pass_b(const uint32_t *Yh, int idx, const uint32_t (&w)[9], PC pc,
+                                              uint32_t (&v)[18]) {
+#pragma unroll
+    for (int n2 = 0; n2 < 18; n2++) v[n2] = Yh[idx * 18 + n2];
+    dft18_fwd<false>(v, w, pc);
+}
+
 __global__ __launch_bounds__(MAC_THREADS, 5) void k_commit_mac(const uint32_t *table32, size_t clen, int m_per, int re,
                                                             const uint32_t *ahat, const uint32_t *tabs, PC pc,
                                                             uint32_t c32, uint32_t r2, Roots18 R,
@@ -1355,36 +1395,12 @@ __global__ __launch_bounds__(MAC_THREADS, 5) void k_commit_mac(const uint32_t *t
             s_bits[t] = tab4[(size_t)(c0 + bc) * clen + 4 * (size_t)m + wd];
         }
         __syncthreads();
-        const uint4 *bits = s_bits + col * 4;
-        {
-            // pass A: stage 1 by lookup, stages 2 and 3, twist by beta^n2 rho^(n2 k1), transpose
-            // through LDS
-            uint32_t v[18], ra[4], rb[4];
-            shr128(bits[wa], sa, ra);
-            shr128(bits[wb], sb, rb);
-            stage1_lookup<0>(ra, rb, st1, v);
-            stage1_lookup<1>(ra, rb, st1, v);
-            stage1_lookup<2>(ra, rb, st1, v);
-            stage1_lookup<3>(ra, rb, st1, v);
-            stage1_lookup<4>(ra, rb, st1, v);
-            stage1_lookup<5>(ra, rb, st1, v);
-            stage1_lookup<6>(ra, rb, st1, v);
-            stage1_lookup<7>(ra, rb, st1, v);
-            stage1_lookup<8>(ra, rb, st1, v);
-            dft18_stage23<false>(v, w, pc);
-#pragma unroll
-            for (int p = 0; p < 18; p++) {
-                const int k1 = krev18(p);
-                Yh[k1 * 18 + idx] = mont_lazy(v[p], twp[k1], pc);
-            }
-        }
+        commit_pass_a(s_bits + col * 4, wa, wb, sa, sb, st1, twp, w, pc, Yh, idx);
         __syncthreads();
         {
             // pass B and the MAC: slots h*324 + idx*18 + p
             uint32_t v[18];
-#pragma unroll
-            for (int n2 = 0; n2 < 18; n2++) v[n2] = Yh[idx * 18 + n2];
-            dft18_fwd<false>(v, w, pc);
+            commit_pass_b(Yh, idx, w, pc, v);
             const uint2 *am = (const uint2 *)(ahat + (size_t)m * 648 + h * 324 + idx * 18);
 #pragma unroll
             for (int q = 0; q < 9; q++) {
@@ -1405,7 +1421,8 @@ __global__ __launch_bounds__(MAC_THREADS, 5) void k_commit_mac(const uint32_t *t
         const uint32_t redc = (uint32_t)((acc[p] + (uint64_t)m * pc.p) >> 32);
         red[p] = mont(redc, r2, pc);
     }
-    uint2 *dst = (uint2 *)(cacc + ((size_t)blockIdx.y * COLUMNS + c0 + col) * 648 + h * 324 + idx * 18);
+    const size_t ncols = (size_t)gridDim.x * TC;
+    uint2 *dst = (uint2 *)(cacc + ((size_t)blockIdx.y * ncols + c0 + col) * 648 + h * 324 + idx * 18);
 #pragma unroll
     for (int q = 0; q < 9; q++) dst[q] = make_uint2(red[2 * q], red[2 * q + 1]);
 }
@@ -1417,7 +1434,9 @@ struct CrtArg {
     uint64_t q_lo, q_hi;
     uint64_t half_m[3];                     // floor(M / 2), M = product of the primes (192 bits)
     uint64_t m[3];
+    uint32_t r2[MAX_PRIMES];                // R^2 mod p
     int nprimes, width, splits;
+    int cols;                               // outputs per slice (columns, or key rows x columns)
     PC pc[MAX_PRIMES];
     Roots18 roots[MAX_PRIMES];
 };
@@ -1458,15 +1477,13 @@ __global__ __launch_bounds__(64) void k_commit_finish(const uint32_t *cacc, cons
         uint32_t w[9];
 #pragma unroll
         for (int j = 0; j < 9; j++) w[j] = crt.roots[k].i[j];
-        {  // the sum over the MAC's slices (at most 16 residues below p), coalesced over the slots
-            const uint32_t *src = cacc + (size_t)k * crt.splits * COLUMNS * 648 + (size_t)c * 648;
+        {  // the sum over the MAC's slices (at most 128 residues below p, so below 2^32 p),
+           // coalesced over the slots, reduced as REDC then a Montgomery product with R^2
+            const uint32_t *src = cacc + (size_t)k * crt.splits * crt.cols * 648 + (size_t)c * 648;
             for (int i = t; i < 648; i += 64) {
                 uint64_t sum = 0;
-                for (int sl = 0; sl < crt.splits; sl++) sum += src[(size_t)sl * COLUMNS * 648 + i];
-                sum = sum >= 8ull * pc.p ? sum - 8ull * pc.p : sum;
-                sum = sum >= 4ull * pc.p ? sum - 4ull * pc.p : sum;
-                sum = sum >= 2ull * pc.p ? sum - 2ull * pc.p : sum;
-                x[i] = (uint32_t)(sum >= pc.p ? sum - pc.p : sum);
+                for (int sl = 0; sl < crt.splits; sl++) sum += src[(size_t)sl * crt.cols * 648 + i];
+                x[i] = mont(redc64(sum, pc), crt.r2[k], pc);
             }
         }
         __syncthreads();
@@ -1539,6 +1556,194 @@ __global__ __launch_bounds__(64) void k_commit_finish(const uint32_t *cacc, cons
         uint8_t *o = outb + ((size_t)c * 648 + tt) * crt.width;
         for (int b = 0; b < crt.width; b++) o[b] = (uint8_t)(r >> (8 * b));
     }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Keys of several ring rows. The MAC above keeps 18 sums per thread for one key row; with R rows
+// each transformed witness value meets R key values, a product of a (rows x elements) key by an
+// (elements x columns) witness per slot, which needs R x columns sums per slot. So the transform
+// and the products are split: k_commit_v stores the transformed witness of a chunk of ring
+// elements (a buffer small enough to stay in the last-level cache), k_commit_gemm forms the
+// products per slot with register tiles of 4 rows x 8 columns, k_commit_rowsum adds the slices and
+// transposes them into the finish kernel's per-output layout.
+// ---------------------------------------------------------------------------------------------
+
+// Forward transform of the binary ring elements [m0, m1) of TC columns, no MAC: each value is
+// reduced to [0, p) and stored at V[(m - mbase) * 648 + slot][column] (ncols columns per slot
+// row), staged through LDS so that each slot row's TC columns leave as one 64-byte segment. The
+// witness words are loaded one element ahead.
+constexpr int VPAD = TC + 1;
+static_assert(648 * VPAD >= TC * 2 * 324, "staging buffer smaller than the transpose buffer");
+__global__ __launch_bounds__(MAC_THREADS, 2) void k_commit_v(const uint32_t *table32, size_t clen, int mbase,
+                                                          int mend, int m_per, const uint32_t *tabs, PC pc,
+                                                          Roots18 R, uint32_t *V, int ncols) {
+    __shared__ uint32_t Y[648 * VPAD];  // the pass A -> B transpose, then the staged outputs
+    __shared__ uint32_t s_st1[tab::ST1_SIZE];
+    __shared__ uint32_t s_twp[2 * 324];
+    __shared__ uint4 s_bits[TC * 4];
+    const int t = threadIdx.x;
+    const int col = t / 36, h = (t / 18) & 1, idx = t % 18;
+    const int c0 = blockIdx.x * TC;
+    const int m0 = mbase + blockIdx.y * m_per, m1 = min(mend, m0 + m_per);
+    for (int i = t; i < tab::ST1_SIZE; i += MAC_THREADS) s_st1[i] = tabs[tab::ST1 + i];
+    for (int i = t; i < 2 * 324; i += MAC_THREADS) s_twp[i] = tabs[tab::TWPB + i];
+    uint32_t w[9];
+#pragma unroll
+    for (int j = 0; j < 9; j++) w[j] = R.f[j];
+    const uint32_t *st1 = s_st1 + h * 16;
+    const uint32_t *twp = s_twp + (h * 18 + idx) * 18;
+    uint32_t *Yh = Y + (col * 2 + h) * 324;
+    const int wa = idx & 3, wb = (idx + 2) & 3;
+    const uint32_t sa = idx >> 2, sb = (idx + 18) >> 2;
+    const uint4 *tab4 = (const uint4 *)table32;
+    const uint32_t p2 = 2 * pc.p, p4 = 4 * pc.p;
+    // threads below TC * 4 load word t & 3 of column c0 + t / 4, one element ahead
+    const uint4 *mine = tab4 + (size_t)(c0 + (t >> 2)) * clen + (t & 3);
+    uint4 next = make_uint4(0, 0, 0, 0);
+    if (t < TC * 4 && m0 < m1) next = mine[4 * (size_t)m0];
+    for (int m = m0; m < m1; m++) {
+        if (t < TC * 4) {
+            s_bits[t] = next;
+            if (m + 1 < m1) next = mine[4 * (size_t)(m + 1)];
+        }
+        __syncthreads();  // the bits are in; the previous element's staged outputs are written out
+        commit_pass_a(s_bits + col * 4, wa, wb, sa, sb, st1, twp, w, pc, Yh, idx);
+        __syncthreads();
+        uint32_t v[18];
+        commit_pass_b(Yh, idx, w, pc, v);
+        __syncthreads();  // every thread has its pass-B inputs: Y becomes the staging buffer
+#pragma unroll
+        for (int p = 0; p < 18; p++) {
+            const uint32_t x = red6(v[p], p2, p4);
+            Y[(h * 324 + idx * 18 + p) * VPAD + col] = min(x, x - pc.p);
+        }
+        __syncthreads();
+        uint32_t *dst = V + (size_t)(m - mbase) * 648 * ncols + c0;
+        for (int i = t; i < 648 * TC / 4; i += MAC_THREADS) {  // 16-byte stores of 4 columns
+            const int slot = i / (TC / 4), c = 4 * (i % (TC / 4));
+            const uint32_t *y = Y + slot * VPAD + c;
+            *(uint4 *)(dst + (size_t)slot * ncols + c) = make_uint4(y[0], y[1], y[2], y[3]);
+        }
+    }
+}
+
+// Key-row products, for keys of up to 16 rows (ahat16[m][slot][16], rows past the key's zero).
+// Block (x, y, z): slots 8x .. 8x+7, ring elements [y m_per, (y+1) m_per) of the chunk, columns
+// 64z .. 64z+63:   out[slot][r][c] = sum_m ahat16[m][slot][r] * V[m][slot][c]  (mod p)
+// into slice y, laid out [slot][16][ncols]. Thread t: slot 8x + t / 32, rows 4 (t & 3) .. +3,
+// columns 64z + 8 ((t >> 2) & 7) .. +7. The 32 sums are 64-bit multiply-adds of canonical
+// residues folded every 16 elements as in the MAC: after a fold a sum is below 2^32 (p + 1), and
+// 16 products below p^2 < 2^58.9 keep it below 2^64.
+constexpr int G_SLOTS = 8, G_MB = 16, G_THREADS = G_SLOTS * 32, G_VROW = 68;
+static_assert(648 % G_SLOTS == 0, "slot groups must tile the 648 slots");
+__global__ __launch_bounds__(G_THREADS) void k_commit_gemm(const uint32_t *ahat16, const uint32_t *V, int ncols,
+                                                           int mcount, int m_per, PC pc, uint32_t c32, uint32_t r2,
+                                                           uint32_t *out) {
+    __shared__ uint4 Al[G_MB * G_SLOTS * 4];          // [mm][slot][16 rows]
+    __shared__ uint32_t Vl[G_MB * G_SLOTS * G_VROW];  // [mm][slot][64 columns + 4 pad]
+    const int t = threadIdx.x;
+    const int js = t >> 5, rg = t & 3, cg = (t >> 2) & 7;
+    const int s0 = blockIdx.x * G_SLOTS, cb = blockIdx.z * 64;
+    const int m0 = blockIdx.y * m_per, m1 = min(mcount, m0 + m_per);
+    uint64_t acc[4][8];
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+        for (int c = 0; c < 8; c++) acc[r][c] = 0;
+    for (int mb = m0; mb < m1; mb += G_MB) {
+        const int nm = min(G_MB, m1 - mb);
+        __syncthreads();  // the previous tiles are consumed
+        for (int i = t; i < nm * G_SLOTS * 4; i += G_THREADS) {
+            const int mm = i / (G_SLOTS * 4), q = i % (G_SLOTS * 4);
+            Al[i] = ((const uint4 *)(ahat16 + ((size_t)(mb + mm) * 648 + s0) * 16))[q];
+        }
+        for (int i = t; i < nm * G_SLOTS * 16; i += G_THREADS) {
+            const int row = i >> 4, q = i & 15;  // row = mm * G_SLOTS + slot
+            const int mm = row / G_SLOTS, j = row % G_SLOTS;
+            *(uint4 *)(Vl + row * G_VROW + 4 * q) =
+                ((const uint4 *)(V + ((size_t)(mb + mm) * 648 + s0 + j) * ncols + cb))[q];
+        }
+        __syncthreads();
+        for (int mm = 0; mm < nm; mm++) {
+            const uint4 a = Al[(mm * G_SLOTS + js) * 4 + rg];
+            const uint32_t *vr = Vl + (mm * G_SLOTS + js) * G_VROW + cg * 8;
+            const uint4 v0 = *(const uint4 *)vr, v1 = *(const uint4 *)(vr + 4);
+            const uint32_t av[4] = {a.x, a.y, a.z, a.w};
+            const uint32_t vv[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+#pragma unroll
+                for (int c = 0; c < 8; c++) acc[r][c] += (uint64_t)av[r] * vv[c];
+        }
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+#pragma unroll
+            for (int c = 0; c < 8; c++) acc[r][c] = (uint64_t)(uint32_t)(acc[r][c] >> 32) * c32 + (uint32_t)acc[r][c];
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        uint32_t red[8];
+#pragma unroll
+        for (int c = 0; c < 8; c++) red[c] = mont(redc64(acc[r][c], pc), r2, pc);
+        uint4 *dst = (uint4 *)(out + (size_t)blockIdx.y * 648 * 16 * ncols +
+                               ((size_t)(s0 + js) * 16 + 4 * rg + r) * ncols + cb + 8 * cg);
+        dst[0] = make_uint4(red[0], red[1], red[2], red[3]);
+        dst[1] = make_uint4(red[4], red[5], red[6], red[7]);
+    }
+}
+
+// The key-row products' slices added and transposed for the finish kernel:
+// crow[(r * ncols + c) * 648 + slot] = sum_sl slices[sl][slot][r][c] mod p. Block (x, y): slots
+// 16x .. 16x+15 (the last block 8), outputs rc = 64y .. 64y+63 of the row-major (r, c) order;
+// each thread keeps four independent sums in flight over the slices.
+__global__ __launch_bounds__(256) void k_commit_rowsum(const uint32_t *slices, int nslices, int ncols, PC pc,
+                                                       uint32_t r2, uint32_t *crow) {
+    __shared__ uint32_t tile[16][65];
+    const int t = threadIdx.x;
+    const int sb = blockIdx.x * 16, rc0 = blockIdx.y * 64;
+    const int r = rc0 / ncols, c0 = rc0 % ncols;
+    const size_t sw = (size_t)648 * 16 * ncols;
+    const int ci = t & 63;
+    for (int k = t >> 6; k < 16; k += 4) {
+        if (sb + k >= 648) break;
+        const uint32_t *src = slices + ((size_t)(sb + k) * 16 + r) * ncols + c0 + ci;
+        uint64_t a0 = 0, a1 = 0, a2 = 0, a3 = 0;  // at most 128 residues below p in all
+        int sl = 0;
+        for (; sl + 4 <= nslices; sl += 4) {
+            a0 += src[(size_t)sl * sw];
+            a1 += src[(size_t)(sl + 1) * sw];
+            a2 += src[(size_t)(sl + 2) * sw];
+            a3 += src[(size_t)(sl + 3) * sw];
+        }
+        for (; sl < nslices; sl++) a0 += src[(size_t)sl * sw];
+        tile[k][ci] = mont(redc64(a0 + a1 + a2 + a3, pc), r2, pc);
+    }
+    __syncthreads();
+    const int rc = t >> 2, q = t & 3;  // four slots of one output per thread, 16-byte stores
+    if (sb + 4 * q < 648)
+        *(uint4 *)(crow + (size_t)(rc0 + rc) * 648 + sb + 4 * q) =
+            make_uint4(tile[4 * q][rc], tile[4 * q + 1][rc], tile[4 * q + 2][rc], tile[4 * q + 3][rc]);
+}
+
+// Slices of the one-row MAC added: out[i] = sum_sl in[sl * n + i] mod p for i < n (n = columns x
+// 648 per prime, blockIdx.y the prime), so that the finish kernel reads one slice.
+__global__ __launch_bounds__(256) void k_commit_slicesum(const uint32_t *in, int nslices, size_t n, CrtArg crt,
+                                                         uint32_t *out) {
+    const int k = blockIdx.y;
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const PC pc = crt.pc[k];
+    const uint32_t *src = in + (size_t)k * nslices * n + i;
+    uint64_t a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+    int sl = 0;
+    for (; sl + 4 <= nslices; sl += 4) {
+        a0 += src[(size_t)sl * n];
+        a1 += src[(size_t)(sl + 1) * n];
+        a2 += src[(size_t)(sl + 2) * n];
+        a3 += src[(size_t)(sl + 3) * n];
+    }
+    for (; sl < nslices; sl++) a0 += src[(size_t)sl * n];
+    out[(size_t)k * n + i] = mont(redc64(a0 + a1 + a2 + a3, pc), crt.r2[k], pc);
 }
 
 // =============================================================================================
@@ -1645,9 +1850,9 @@ extern "C" int lbn_load_table_device(lbn_ctx *ctx, const void *device_bytes, siz
     return LBN_OK;
 }
 
-extern "C" int lbn_generate_table(lbn_ctx *ctx, const uint8_t seed[32]) {
-    if (!ctx || !seed) return LBN_E_ARG;
-    HIPCHK(ctx, hipSetDevice(ctx->device));
+// The root node of the table writer's XOF for a seed (verifier/table_writer/src/main.rs:14-22):
+// keyed by the seed, one block holding the domain string.
+static B3Root table_root(const uint8_t seed[32]) {
     static const char CTXT[] = "labinius-fast/binary-table/v1";
     B3Root root{};
     for (int i = 0; i < 8; i++)
@@ -1660,6 +1865,13 @@ extern "C" int lbn_generate_table(lbn_ctx *ctx, const uint8_t seed[32]) {
                     ((uint32_t)block[4 * i + 3] << 24);
     root.len = sizeof CTXT - 1;
     root.flags = 16u /*KEYED_HASH*/ | 1u /*CHUNK_START*/ | 2u /*CHUNK_END*/ | 8u /*ROOT*/;
+    return root;
+}
+
+extern "C" int lbn_generate_table(lbn_ctx *ctx, const uint8_t seed[32]) {
+    if (!ctx || !seed) return LBN_E_ARG;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    const B3Root root = table_root(seed);
     const size_t blocks = ctx->shape.table_bytes / 64;
     tstart(ctx, T_TABLE);
     k_table_xof<<<(unsigned)((blocks + 255) / 256), 256, 0, ctx->dev->stream>>>(root, (uint32_t *)ctx->dev->table,
@@ -1703,18 +1915,19 @@ static int modulus_width(u128 q) {
     return (bits + 7) / 8;
 }
 
+// Ring-element ranges of the MAC over `groups` groups of TC columns: about 1024 blocks per prime
+// (groups x splits), at most 128 slices (the finish kernel's slice sum stays below 2^32 p).
 // Device bytes of the buffers that scale with the CRT prime count and the coefficient width.
-// Ring-element ranges of the MAC: about 1024 blocks per prime (64 column groups x splits). The
-// finish kernel reduces a sum of at most 16 slices with four conditional subtractions.
-static int mac_m_per(size_t re) {
-    const int msplit = (int)std::max<size_t>(1, std::min<size_t>(re, 1024 / (COLUMNS / TC)));
+constexpr int MAX_SLICES = 128;
+static int mac_m_per(size_t re, int groups = COLUMNS / TC) {
+    const size_t want = std::max<size_t>(1, 1024 / (size_t)groups);
+    const int msplit = (int)std::max<size_t>(1, std::min<size_t>({re, want, (size_t)MAX_SLICES}));
     return (int)((re + msplit - 1) / msplit);
 }
-static int mac_splits(size_t re) {
-    const int m_per = mac_m_per(re);
+static int mac_splits(size_t re, int groups = COLUMNS / TC) {
+    const int m_per = mac_m_per(re, groups);
     return (int)((re + m_per - 1) / m_per);
 }
-static_assert(1024 / (COLUMNS / TC) <= 16, "more than 16 partial sums per MAC slot");
 
 static size_t commit_buffer_bytes(int np, int width, size_t re) {
     const size_t clen = (size_t)COLUMNS * N * width;
@@ -1756,6 +1969,28 @@ static std::shared_ptr<const std::vector<u128>> public_matrix(u128 q, size_t re)
     return cache.emplace(std::make_pair(q, re), A).first->second;
 }
 
+// The first np CRT primes with their Montgomery constants, their transform tables (all,
+// MAX_PRIMES * tab::SIZE words) and their 18-th roots.
+static bool setup_primes(int np, PrimeInfo *primes, Roots18 *roots, std::vector<uint32_t> &all) {
+    all.assign((size_t)MAX_PRIMES * tab::SIZE, 0);
+    for (int k = 0; k < np; k++) {
+        PrimeInfo pi = CRT_PRIMES[k];
+        uint32_t inv = 1;  // p^-1 mod 2^32 by Newton
+        for (int it = 0; it < 5; it++) inv *= 2 - pi.p * inv;
+        pi.pinv_neg = (uint32_t)(0u - inv);
+        pi.r2 = (uint32_t)(((u128)1 << 64) % pi.p);
+        primes[k] = pi;
+        std::vector<uint32_t> t;
+        if (!prime_tables(pi, t)) return false;
+        std::copy(t.begin(), t.end(), all.begin() + (size_t)k * tab::SIZE);
+        for (int j = 0; j < 9; j++) {
+            roots[k].f[j] = all[(size_t)k * tab::SIZE + tab::R18F + j];
+            roots[k].i[j] = all[(size_t)k * tab::SIZE + tab::R18I + j];
+        }
+    }
+    return true;
+}
+
 static int alloc_commit_buffers(lbn_ctx *ctx, int np, int width) {
     DeviceState *d = ctx->dev;
     if (d->alloc_primes >= np && d->alloc_width >= width) return LBN_OK;
@@ -1794,18 +2029,8 @@ extern "C" int lbn_set_modulus(lbn_ctx *ctx, uint64_t q_lo, uint64_t q_hi) {
         if (rc != LBN_OK) return rc;
     }
     ctx->nprimes = np;
-    std::vector<uint32_t> all((size_t)MAX_PRIMES * tab::SIZE, 0);
-    for (int k = 0; k < np; k++) {
-        PrimeInfo pi = CRT_PRIMES[k];
-        uint32_t inv = 1;  // p^-1 mod 2^32 by Newton
-        for (int it = 0; it < 5; it++) inv *= 2 - pi.p * inv;
-        pi.pinv_neg = (uint32_t)(0u - inv);
-        pi.r2 = (uint32_t)(((u128)1 << 64) % pi.p);
-        ctx->primes[k] = pi;
-        std::vector<uint32_t> t;
-        if (!prime_tables(pi, t)) return ctx->fail(LBN_E_ARG, "NTT table construction failed");
-        std::copy(t.begin(), t.end(), all.begin() + (size_t)k * tab::SIZE);
-    }
+    std::vector<uint32_t> all;
+    if (!setup_primes(np, ctx->primes, d->roots, all)) return ctx->fail(LBN_E_ARG, "NTT table construction failed");
     HIPCHK(ctx, hipMemcpyAsync(d->ntt_tabs, all.data(), all.size() * 4, hipMemcpyHostToDevice, d->stream));
     // The public matrix (rq.rs:114-144), expanded on the host, reduced and transformed here.
     const std::shared_ptr<const std::vector<u128>> A = public_matrix(q, s.ring_elements);
@@ -1819,15 +2044,10 @@ extern "C" int lbn_set_modulus(lbn_ctx *ctx, uint64_t q_lo, uint64_t q_hi) {
     }
     for (int k = 0; k < np; k++) {
         PC pc{ctx->primes[k].p, ctx->primes[k].pinv_neg};
-        Roots18 R;
-        for (int j = 0; j < 9; j++) {
-            R.f[j] = all[(size_t)k * tab::SIZE + tab::R18F + j];
-            R.i[j] = all[(size_t)k * tab::SIZE + tab::R18I + j];
-        }
-        d->roots[k] = R;
+        const Roots18 R = d->roots[k];
         k_ahat<<<(unsigned)s.ring_elements, 64, 0, d->stream>>>(d->amat, d->ntt_tabs + (size_t)k * tab::SIZE, pc, R,
                                                                 ctx->primes[k].r2, q_lo, q_hi,
-                                                                d->ahat + (size_t)k * s.ring_elements * N);
+                                                                d->ahat + (size_t)k * s.ring_elements * N, 0, 0);
         HIPCHK(ctx, hipGetLastError());
     }
     HIPCHK(ctx, hipStreamSynchronize(d->stream));
@@ -1917,6 +2137,39 @@ int lbn::dev_commit_bytes(lbn_ctx *ctx, uint8_t *dst) {
     return LBN_OK;
 }
 
+// Arguments of the finish kernel: the primes, their Garner constants and the range M.
+static CrtArg make_crt(const PrimeInfo *primes, const Roots18 *roots, int np, u128 q, int width, int splits,
+                       int cols) {
+    CrtArg crt{};
+    crt.nprimes = np;
+    crt.width = width;
+    crt.splits = splits;
+    crt.cols = cols;
+    crt.q_lo = (uint64_t)q;
+    crt.q_hi = (uint64_t)(q >> 64);
+    uint64_t M[3] = {1, 0, 0};  // product of the primes, 192 bits
+    for (int i = 0; i < np; i++) {
+        crt.p[i] = primes[i].p;
+        crt.pc[i] = PC{primes[i].p, primes[i].pinv_neg};
+        crt.r2[i] = primes[i].r2;
+        crt.roots[i] = roots[i];
+        u128 t = (u128)M[0] * crt.p[i];
+        M[0] = (uint64_t)t;
+        t = (u128)M[1] * crt.p[i] + (uint64_t)(t >> 64);
+        M[1] = (uint64_t)t;
+        M[2] = M[2] * crt.p[i] + (uint64_t)(t >> 64);
+        for (int j = 0; j < i; j++)
+            crt.inv[i][j] =
+                (uint32_t)(((uint64_t)powmod32(primes[j].p % primes[i].p, primes[i].p - 2, primes[i].p) << 32) %
+                           primes[i].p);
+    }
+    for (int w = 0; w < 3; w++) crt.m[w] = M[w];
+    crt.half_m[0] = (M[0] >> 1) | (M[1] << 63);
+    crt.half_m[1] = (M[1] >> 1) | (M[2] << 63);
+    crt.half_m[2] = M[2] >> 1;
+    return crt;
+}
+
 int lbn::dev_commit_launch(lbn_ctx *ctx) {
     if (!ctx->table_loaded || !ctx->modulus_set) return ctx->fail(LBN_E_STATE, "table and modulus must be loaded");
     const Shape &s = ctx->shape;
@@ -1941,33 +2194,7 @@ int lbn::dev_commit_launch(lbn_ctx *ctx) {
         HIPCHK(ctx, hipGetLastError());
     }
     tstop(ctx, T_MAC);
-    CrtArg crt{};
-    crt.nprimes = np;
-    crt.width = ctx->width;
-    crt.splits = msplit;
-    crt.q_lo = (uint64_t)ctx->q;
-    crt.q_hi = (uint64_t)(ctx->q >> 64);
-    {
-        uint64_t M[3] = {1, 0, 0};  // product of the primes, 192 bits
-        for (int i = 0; i < np; i++) {
-            crt.p[i] = ctx->primes[i].p;
-            crt.pc[i] = PC{ctx->primes[i].p, ctx->primes[i].pinv_neg};
-            crt.roots[i] = d->roots[i];
-            u128 t = (u128)M[0] * crt.p[i];
-            M[0] = (uint64_t)t;
-            t = (u128)M[1] * crt.p[i] + (uint64_t)(t >> 64);
-            M[1] = (uint64_t)t;
-            M[2] = M[2] * crt.p[i] + (uint64_t)(t >> 64);
-            for (int j = 0; j < i; j++)
-                crt.inv[i][j] = (uint32_t)(((uint64_t)powmod32(ctx->primes[j].p % ctx->primes[i].p,
-                                                                ctx->primes[i].p - 2, ctx->primes[i].p)
-                                            << 32) % ctx->primes[i].p);
-        }
-        for (int w = 0; w < 3; w++) crt.m[w] = M[w];
-        crt.half_m[0] = (M[0] >> 1) | (M[1] << 63);
-        crt.half_m[1] = (M[1] >> 1) | (M[2] << 63);
-        crt.half_m[2] = M[2] >> 1;
-    }
+    const CrtArg crt = make_crt(ctx->primes, d->roots, np, ctx->q, ctx->width, msplit, COLUMNS);
     tstart(ctx, T_FINISH);
     k_commit_finish<<<COLUMNS, 64, 0, d->stream>>>(d->cacc, d->ntt_tabs, crt, d->cbytes);
     tstop(ctx, T_FINISH);
@@ -2313,5 +2540,311 @@ extern "C" int lbn_reset_timing(lbn_ctx *ctx) {
 extern "C" int lbn_get_timing(const lbn_ctx *ctx, lbn_timing *out) {
     if (!ctx || !out) return LBN_E_ARG;
     *out = ctx->times;
+    return LBN_OK;
+}
+
+// ============================================================================= standalone commitment
+// lbn_cmt: the commitment of `columns` witness columns of `elements` binary ring elements (four
+// words each, column-major like the prover's table) under a key of `rows` ring rows, row r's
+// element m being element r * elements + m of the matrix expansion for (Q, elements).
+
+namespace {
+struct CmtEv {
+    int kind;
+    hipEvent_t a, b;
+};
+}  // namespace
+
+struct lbn_cmt {
+    int device = 0, columns = 0, elements = 0, rows = 0, nprimes = 0, width = 0;
+    u128 q = 0;
+    std::string err;
+    hipStream_t stream = nullptr;
+    uint4 *table = nullptr;
+    uint32_t *tabs = nullptr;  // MAX_PRIMES * tab::SIZE
+    uint32_t *ahat = nullptr;  // rows == 1: [k][m][648]; rows > 1: [k][m][648][16]
+    uint32_t *V = nullptr;     // rows > 1: transformed witness of one chunk, [m][648][columns]
+    uint32_t *cacc = nullptr;  // MAC slices [k][slice][column][648], or product slices [k][slice][648][16][columns]
+    uint32_t *crow = nullptr;  // summed slices [k][row * columns + column][648]
+    uint8_t *out = nullptr;    // rows * columns * 648 * width
+    PrimeInfo primes[MAX_PRIMES] = {};
+    Roots18 roots[MAX_PRIMES] = {};
+    int splits = 0, m_per = 0;                 // rows == 1: MAC slices
+    int chunk = 0, gsplit = 0, nslices = 0;    // rows > 1: elements per chunk, product slices per chunk, total
+    bool loaded = false, timing = false;
+    std::vector<CmtEv> evs;
+    size_t ev_used = 0;
+    lbn_cmt_timing times{};
+    int fail(int code, const std::string &m) {
+        err = m;
+        return code;
+    }
+};
+
+namespace {
+constexpr int CMT_CHUNK = 1024;  // ring elements per transformed chunk (columns x 2.6 KB each)
+constexpr int CMT_VM_PER = 4;    // ring elements per block of k_commit_v
+enum CmtKind { CK_TOTAL = 0, CK_TRANSFORM, CK_MAC, CK_SUM, CK_FINISH };
+
+int cmt_begin(lbn_cmt *c, int kind) {
+    if (!c->timing) return -1;
+    if (c->ev_used == c->evs.size()) {
+        CmtEv e{};
+        if (hipEventCreate(&e.a) != hipSuccess || hipEventCreate(&e.b) != hipSuccess) return -1;
+        c->evs.push_back(e);
+    }
+    CmtEv &e = c->evs[c->ev_used];
+    e.kind = kind;
+    hipEventRecord(e.a, c->stream);
+    return (int)c->ev_used++;
+}
+void cmt_end(lbn_cmt *c, int i) {
+    if (i >= 0) hipEventRecord(c->evs[(size_t)i].b, c->stream);
+}
+void cmt_free(lbn_cmt *c) {
+    for (void *p : {(void *)c->table, (void *)c->tabs, (void *)c->ahat, (void *)c->V, (void *)c->cacc,
+                    (void *)c->crow, (void *)c->out})
+        if (p) hipFree(p);
+    for (CmtEv &e : c->evs) {
+        hipEventDestroy(e.a);
+        hipEventDestroy(e.b);
+    }
+    if (c->stream) hipStreamDestroy(c->stream);
+}
+}  // namespace
+
+extern "C" size_t lbn_cmt_len(int columns, int rows, uint64_t q_lo, uint64_t q_hi) {
+    const u128 q = ((u128)q_hi << 64) | q_lo;
+    if (columns <= 0 || rows <= 0 || q < 2) return 0;
+    return (size_t)rows * columns * N * modulus_width(q);
+}
+
+extern "C" int lbn_cmt_create(int device, int columns, int elements, int rows, uint64_t q_lo, uint64_t q_hi,
+                              lbn_cmt **out) {
+    if (!out) return LBN_E_ARG;
+    *out = nullptr;
+    if (rows < 1 || rows > 16 || elements < 1 || elements > 65536 || columns < TC || columns > 4096 ||
+        columns % (rows == 1 ? TC : 64))
+        return LBN_E_ARG;
+    const u128 q = ((u128)q_hi << 64) | q_lo;
+    if (q < (u128)34816965049ull || q >= ((u128)1 << 96)) return LBN_E_MODULUS;
+    const int np = crt_prime_count(q, (size_t)elements);
+    if (!np) return LBN_E_MODULUS;
+    auto *c = new lbn_cmt();
+    c->device = device, c->columns = columns, c->elements = elements, c->rows = rows;
+    c->q = q, c->nprimes = np, c->width = modulus_width(q);
+    const int rc = [&]() -> int {
+        HIPCHK(c, hipSetDevice(device));
+        HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        const size_t words = (size_t)columns * elements * 4;
+        HIPCHK(c, hipMalloc(&c->table, words * 16));
+        std::vector<uint32_t> all;
+        if (!setup_primes(np, c->primes, c->roots, all)) return c->fail(LBN_E_ARG, "NTT table construction failed");
+        HIPCHK(c, hipMalloc(&c->tabs, all.size() * 4));
+        HIPCHK(c, hipMemcpyAsync(c->tabs, all.data(), all.size() * 4, hipMemcpyHostToDevice, c->stream));
+        const size_t re = (size_t)elements;
+        const int rstride = rows == 1 ? 0 : 16;
+        HIPCHK(c, hipMalloc(&c->ahat, (size_t)np * re * N * (rows == 1 ? 1 : 16) * 4));
+        if (rows > 1) HIPCHK(c, hipMemsetAsync(c->ahat, 0, (size_t)np * re * N * 16 * 4, c->stream));
+        // the key, one row at a time: expanded on the host, reduced and transformed on the device
+        std::vector<u128> A(re * N);
+        uint64_t *amat = nullptr;
+        HIPCHK(c, hipMalloc(&amat, re * N * 16));
+        for (int r = 0; r < rows; r++) {
+            ajtai_elements_host(q, re, (size_t)r * re, re, A.data());
+            HIPCHK(c, hipMemcpyAsync(amat, A.data(), re * N * 16, hipMemcpyHostToDevice, c->stream));
+            for (int k = 0; k < np; k++) {
+                PC pc{c->primes[k].p, c->primes[k].pinv_neg};
+                k_ahat<<<(unsigned)re, 64, 0, c->stream>>>(amat, c->tabs + (size_t)k * tab::SIZE, pc, c->roots[k],
+                                                           c->primes[k].r2, q_lo, q_hi,
+                                                           c->ahat + (size_t)k * re * N * (rows == 1 ? 1 : 16), r,
+                                                           rstride);
+                HIPCHK(c, hipGetLastError());
+            }
+            HIPCHK(c, hipStreamSynchronize(c->stream));  // amat and A are reused by the next row
+        }
+        HIPCHK(c, hipFree(amat));
+        if (rows == 1) {
+            c->m_per = mac_m_per(re, columns / TC);
+            c->splits = mac_splits(re, columns / TC);
+            HIPCHK(c, hipMalloc(&c->cacc, (size_t)np * c->splits * columns * N * 4));
+            HIPCHK(c, hipMalloc(&c->crow, (size_t)np * columns * N * 4));
+        } else {
+            c->chunk = std::min(elements, CMT_CHUNK);
+            const int nchunks = (elements + c->chunk - 1) / c->chunk;
+            c->gsplit = std::max(1, std::min(8, MAX_SLICES / nchunks));
+            const int gm_per = (c->chunk + c->gsplit - 1) / c->gsplit;
+            c->nslices = 0;
+            for (int mb = 0; mb < elements; mb += c->chunk)
+                c->nslices += (std::min(c->chunk, elements - mb) + gm_per - 1) / gm_per;
+            if (c->nslices > MAX_SLICES) return c->fail(LBN_E_ARG, "too many product slices");
+            HIPCHK(c, hipMalloc(&c->V, (size_t)c->chunk * N * columns * 4));
+            HIPCHK(c, hipMalloc(&c->cacc, (size_t)np * c->nslices * N * 16 * columns * 4));
+            HIPCHK(c, hipMalloc(&c->crow, (size_t)np * rows * columns * N * 4));
+        }
+        HIPCHK(c, hipMalloc(&c->out, (size_t)rows * columns * N * c->width));
+        return LBN_OK;
+    }();
+    if (rc != LBN_OK) {
+        cmt_free(c);
+        delete c;
+        return rc;
+    }
+    *out = c;
+    return LBN_OK;
+}
+
+extern "C" void lbn_cmt_destroy(lbn_cmt *c) {
+    if (!c) return;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    cmt_free(c);
+    delete c;
+}
+
+extern "C" const char *lbn_cmt_error(const lbn_cmt *c) { return c ? c->err.c_str() : "null commitment context"; }
+extern "C" int lbn_cmt_primes(const lbn_cmt *c) { return c ? c->nprimes : 0; }
+
+extern "C" int lbn_cmt_load(lbn_cmt *c, const void *host_words, size_t bytes) {
+    if (!c || !host_words) return LBN_E_ARG;
+    if (bytes != (size_t)c->columns * c->elements * 64) return c->fail(LBN_E_ARG, "witness has the wrong length");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemcpyAsync(c->table, host_words, bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->loaded = true;
+    return LBN_OK;
+}
+
+extern "C" int lbn_cmt_load_device(lbn_cmt *c, const void *device_words, size_t bytes) {
+    if (!c || !device_words) return LBN_E_ARG;
+    if (bytes != (size_t)c->columns * c->elements * 64) return c->fail(LBN_E_ARG, "witness has the wrong length");
+    if ((uintptr_t)device_words % 16) return c->fail(LBN_E_ARG, "device witness must be 16-byte aligned");
+    HIPCHK(c, hipSetDevice(c->device));
+    const size_t words = bytes / 16;
+    k_copy_u4<<<(unsigned)std::min<size_t>((words + 255) / 256, 65536), 256, 0, c->stream>>>(
+        (const uint4 *)device_words, c->table, words);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->loaded = true;
+    return LBN_OK;
+}
+
+extern "C" int lbn_cmt_generate(lbn_cmt *c, const uint8_t seed[32]) {
+    if (!c || !seed) return LBN_E_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    const B3Root root = table_root(seed);
+    const size_t blocks = (size_t)c->columns * c->elements;  // 64 bytes per ring element
+    k_table_xof<<<(unsigned)((blocks + 255) / 256), 256, 0, c->stream>>>(root, (uint32_t *)c->table, blocks);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->loaded = true;
+    return LBN_OK;
+}
+
+extern "C" int lbn_cmt_launch(lbn_cmt *c) {
+    if (!c) return LBN_E_ARG;
+    if (!c->loaded) return c->fail(LBN_E_STATE, "no witness loaded");
+    HIPCHK(c, hipSetDevice(c->device));
+    const int np = c->nprimes, cols = c->columns, re = c->elements;
+    const size_t clen = (size_t)re * 4;
+    const int tot = cmt_begin(c, CK_TOTAL);
+    CrtArg crt;
+    const uint32_t *fin = c->crow;  // the finish kernel's input: one slice per prime
+    if (c->rows == 1) {
+        const int e = cmt_begin(c, CK_MAC);
+        for (int k = 0; k < np; k++) {
+            const PC pc{c->primes[k].p, c->primes[k].pinv_neg};
+            k_commit_mac<<<dim3(cols / TC, c->splits), MAC_THREADS, 0, c->stream>>>(
+                (const uint32_t *)c->table, clen, c->m_per, re, c->ahat + (size_t)k * re * N,
+                c->tabs + (size_t)k * tab::SIZE, pc, (uint32_t)((1ull << 32) % pc.p), c->primes[k].r2, c->roots[k],
+                c->cacc + (size_t)k * c->splits * cols * N);
+            HIPCHK(c, hipGetLastError());
+        }
+        cmt_end(c, e);
+        crt = make_crt(c->primes, c->roots, np, c->q, c->width, 1, cols);
+        const size_t n = (size_t)cols * N;
+        const int es = cmt_begin(c, CK_SUM);
+        k_commit_slicesum<<<dim3((unsigned)((n + 255) / 256), np), 256, 0, c->stream>>>(c->cacc, c->splits, n, crt,
+                                                                                        c->crow);
+        cmt_end(c, es);
+        HIPCHK(c, hipGetLastError());
+    } else {
+        const int gm_per = (c->chunk + c->gsplit - 1) / c->gsplit;
+        const size_t slice_words = (size_t)N * 16 * cols;
+        for (int k = 0; k < np; k++) {
+            const PC pc{c->primes[k].p, c->primes[k].pinv_neg};
+            const uint32_t c32 = (uint32_t)((1ull << 32) % pc.p);
+            uint32_t *slices = c->cacc + (size_t)k * c->nslices * slice_words;
+            int sl = 0;
+            for (int mb = 0; mb < re; mb += c->chunk) {
+                const int mc = std::min(c->chunk, re - mb);
+                int e = cmt_begin(c, CK_TRANSFORM);
+                k_commit_v<<<dim3(cols / TC, (mc + CMT_VM_PER - 1) / CMT_VM_PER), MAC_THREADS, 0, c->stream>>>(
+                    (const uint32_t *)c->table, clen, mb, mb + mc, CMT_VM_PER, c->tabs + (size_t)k * tab::SIZE, pc,
+                    c->roots[k], c->V, cols);
+                cmt_end(c, e);
+                HIPCHK(c, hipGetLastError());
+                const int gy = (mc + gm_per - 1) / gm_per;
+                e = cmt_begin(c, CK_MAC);
+                k_commit_gemm<<<dim3(N / G_SLOTS, gy, cols / 64), G_THREADS, 0, c->stream>>>(
+                    c->ahat + ((size_t)k * re + mb) * N * 16, c->V, cols, mc, gm_per, pc, c32, c->primes[k].r2,
+                    slices + (size_t)sl * slice_words);
+                cmt_end(c, e);
+                HIPCHK(c, hipGetLastError());
+                sl += gy;
+            }
+            const int e = cmt_begin(c, CK_SUM);
+            k_commit_rowsum<<<dim3((N + 15) / 16, c->rows * cols / 64), 256, 0, c->stream>>>(
+                slices, sl, cols, pc, c->primes[k].r2, c->crow + (size_t)k * c->rows * cols * N);
+            cmt_end(c, e);
+            HIPCHK(c, hipGetLastError());
+        }
+        crt = make_crt(c->primes, c->roots, np, c->q, c->width, 1, c->rows * cols);
+    }
+    const int e = cmt_begin(c, CK_FINISH);
+    k_commit_finish<<<(unsigned)(c->rows * cols), 64, 0, c->stream>>>(fin, c->tabs, crt, c->out);
+    cmt_end(c, e);
+    HIPCHK(c, hipGetLastError());
+    cmt_end(c, tot);
+    return LBN_OK;
+}
+
+extern "C" int lbn_cmt_read(lbn_cmt *c, uint8_t *out, size_t len) {
+    if (!c || !out) return LBN_E_ARG;
+    const size_t need = (size_t)c->rows * c->columns * N * c->width;
+    if (len < need) return c->fail(LBN_E_ARG, "output buffer too small");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemcpyAsync(out, c->out, need, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return LBN_OK;
+}
+
+extern "C" int lbn_cmt_set_timing(lbn_cmt *c, int enable) {
+    if (!c) return LBN_E_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->timing = enable != 0;
+    c->ev_used = 0;
+    c->times = lbn_cmt_timing{};
+    return LBN_OK;
+}
+
+extern "C" int lbn_cmt_get_timing(lbn_cmt *c, lbn_cmt_timing *t) {
+    if (!c || !t) return LBN_E_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i < c->ev_used; i++) {
+        float ms = 0;
+        HIPCHK(c, hipEventElapsedTime(&ms, c->evs[i].a, c->evs[i].b));
+        switch (c->evs[i].kind) {
+            case CK_TOTAL: c->times.total_ms += ms, c->times.runs++; break;
+            case CK_TRANSFORM: c->times.transform_ms += ms; break;
+            case CK_MAC: c->times.mac_ms += ms; break;
+            case CK_SUM: c->times.sum_ms += ms; break;
+            default: c->times.finish_ms += ms; break;
+        }
+    }
+    c->ev_used = 0;
+    *t = c->times;
     return LBN_OK;
 }

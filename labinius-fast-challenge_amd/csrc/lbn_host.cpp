@@ -347,7 +347,10 @@ struct ChallengeSampler {
 // [0x5a; 32] over domain || q (16 LE) || u64(elements) || u64(i); width-byte LE draws masked to
 // bits(q-1), rejected while >= q.
 
-void ajtai_matrix_host(u128 q, size_t elements, u128 *out) {
+// Elements first .. first + count - 1 of the expansion for (q, elements); indices past `elements`
+// continue the same keyed stream family (a key of several ring rows takes row r's element m at
+// index r * elements + m, so row 0 is the matrix itself).
+void ajtai_elements_host(u128 q, size_t elements, size_t first, size_t count, u128 *out) {
     int bits = 0;
     for (u128 x = q - 1; x; x >>= 1) bits++;
     const int width = (bits + 7) / 8;
@@ -361,9 +364,9 @@ void ajtai_matrix_host(u128 q, size_t elements, u128 *out) {
     for (int i = 0; i < 16; i++) qb[i] = (uint8_t)(q >> (8 * i));
     base.update(qb, 16);
     base.update_u64(elements);
-    parallel_for(elements, [&](size_t i) {
+    parallel_for(count, [&](size_t i) {
         Blake3 h = base;
-        h.update_u64(i);
+        h.update_u64(first + i);
         XofReader x(h);
         for (int t = 0; t < N; t++) {
             for (;;) {
@@ -380,6 +383,8 @@ void ajtai_matrix_host(u128 q, size_t elements, u128 *out) {
         }
     });
 }
+
+void ajtai_matrix_host(u128 q, size_t elements, u128 *out) { ajtai_elements_host(q, elements, 0, elements, out); }
 
 // ============================================================================= NTT tables
 

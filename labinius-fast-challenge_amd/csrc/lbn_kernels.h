@@ -243,6 +243,11 @@ __device__ __forceinline__ uint32_t mont(uint32_t a, uint32_t b, PC c) {
     const uint32_t r = mont_lazy(a, b, c);
     return min(r, r - c.p);
 }
+// x * R^-1 mod p, in [0, 2p), for any x < 2^32 p (so x + m p stays below 2^64).
+__device__ __forceinline__ uint32_t redc64(uint64_t x, PC c) {
+    const uint32_t m = (uint32_t)x * c.pinv_neg;
+    return (uint32_t)((x + (uint64_t)m * c.p) >> 32);
+}
 __device__ __forceinline__ uint32_t addm(uint32_t a, uint32_t b, uint32_t p) {
     const uint32_t r = a + b;
     return min(r, r - p);

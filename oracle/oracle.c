@@ -543,15 +543,16 @@ static void fold_into_ring(i128 *raw, u128 q, u128 *out) {
 }
 
 /* rq::commit_binary_column (rq.rs:264-287) for column `column` of the table. */
-void orc_commit_column(const uint8_t *table, int log_vars, u128 q, const u128 *matrix, int column,
-                       u128 *out) {
-    orc_shape s;
-    orc_shape_of(log_vars, &s);
+/* Schoolbook commitment of one column of `elements` ring elements (column c at word
+ * c * 4 * elements): bit p of word k of element m is coefficient 4p + k, and each set bit adds
+ * X^(4p+k) A_m to the raw product (rq.rs:264-287), folded into R_Q at the end. */
+static void commit_column_words(const uint8_t *table, size_t elements, u128 q, const u128 *matrix, int column,
+                                u128 *out) {
     i128 *raw = (i128 *)calloc(2 * N - 1, sizeof(i128));
-    for (size_t m = 0; m < s.ring_elements; m++) {
+    for (size_t m = 0; m < elements; m++) {
         const u128 *a = matrix + N * m;
         for (int k = 0; k < 4; k++) {
-            orc_b128 w = word_at(table, (size_t)column * s.column_len + 4 * m + k);
+            orc_b128 w = word_at(table, (size_t)column * 4 * elements + 4 * m + k);
             for (int p = 0; p < 128; p++) {
                 if (!bit_of(w.w, p)) continue;
                 i128 *win = raw + 4 * p + k;
@@ -561,6 +562,13 @@ void orc_commit_column(const uint8_t *table, int log_vars, u128 q, const u128 *m
     }
     fold_into_ring(raw, q, out);
     free(raw);
+}
+
+void orc_commit_column(const uint8_t *table, int log_vars, u128 q, const u128 *matrix, int column,
+                       u128 *out) {
+    orc_shape s;
+    orc_shape_of(log_vars, &s);
+    commit_column_words(table, s.ring_elements, q, matrix, column, out);
 }
 
 static void pack_coeff(u128 x, int width, uint8_t *dst) {
@@ -1385,4 +1393,25 @@ int orc_replay(const uint8_t seed[32], int log_vars, const uint8_t *proof, size_
                uint64_t *r_hi, uint64_t *batch, uint64_t *coords, orc_challenge *challenges) {
     char err[128];
     return verify_impl(seed, log_vars, proof, len, err, sizeof err, r_hi, batch, coords, challenges, 0);
+}
+
+/* Commitment of columns first .. first + count - 1 of a witness of `elements` ring elements per
+ * column under key row `row`: the row's element m is element row * elements + m of the matrix
+ * expansion for (q, elements) (rq.rs:53-144 with the index continued past `elements`). Packed
+ * coefficient bytes, [column][t]. */
+void orc_commit_key_row(const uint8_t *words, size_t elements, int row, uint64_t q_lo, uint64_t q_hi, int first,
+                        int count, uint8_t *out) {
+    u128 q = ((u128)q_hi << 64) | q_lo;
+    u128 *key = (u128 *)malloc(sizeof(u128) * N * elements);
+#pragma omp parallel for schedule(static)
+    for (long long m = 0; m < (long long)elements; m++)
+        orc_ajtai_element(q, elements, (size_t)row * elements + (size_t)m, key + N * m);
+    int width = orc_coefficient_bytes(q);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int i = 0; i < count; i++) {
+        u128 col[N];
+        commit_column_words(words, elements, q, key, first + i, col);
+        for (int t = 0; t < N; t++) pack_coeff(col[t], width, out + ((size_t)i * N + t) * width);
+    }
+    free(key);
 }

@@ -75,6 +75,11 @@ void orc_commit_column(const uint8_t *table, int log_vars, u128 q, const u128 *m
                        int column, u128 *out648);
 void orc_commit_columns_packed(const uint8_t *table, int log_vars, uint64_t q_lo, uint64_t q_hi,
                                int first, int count, uint8_t *out);
+/* Columns first .. first+count-1 of a witness of `elements` ring elements per column (4 words
+ * each, column-major) committed under key row `row` (matrix elements row * elements + m), packed
+ * [column][t]. */
+void orc_commit_key_row(const uint8_t *words, size_t elements, int row, uint64_t q_lo, uint64_t q_hi, int first,
+                        int count, uint8_t *out);
 
 /* ---- switch and opening components ---- */
 /* eq_expand_b128 (crossfield.rs:87-97): r[0] becomes the MSB of the index. */

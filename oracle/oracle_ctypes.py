@@ -68,6 +68,8 @@ def lib():
         L.orc_sign_bits.restype = ctypes.c_uint32
         L.orc_sign_bits.argtypes = [P]
         L.orc_ajtai_element.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_size_t, ctypes.c_size_t, P]
+        L.orc_commit_key_row.argtypes = [P, ctypes.c_size_t, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64,
+                                         ctypes.c_int, ctypes.c_int, P]
     return _lib
 
 
@@ -167,6 +169,15 @@ def commit_columns(tab, log_vars, q, first, count):
 
 def coefficient_bytes(q: int) -> int:
     return ((q - 1).bit_length() + 7) // 8
+
+
+def commit_key_row(words, elements, row, q, first, count):
+    """Packed commitment of `count` columns from `first` of a witness of `elements` ring elements per
+    column (uint8 array, 64 bytes per element) under key row `row`: shape (count, 648 * width)."""
+    w = coefficient_bytes(q)
+    out = np.zeros(count * 648 * w, dtype=np.uint8)
+    lib().orc_commit_key_row(_p(words), elements, row, q & (2**64 - 1), q >> 64, first, count, _p(out))
+    return out.reshape(count, 648 * w)
 
 
 def b128_mul(a, b, bitwise=False):

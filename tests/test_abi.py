@@ -58,3 +58,17 @@ def test_scheduler_lifecycle_and_errors_without_a_device():
         assert sch.stats()["submitted"] == 0
     finally:
         sch.close()
+
+
+def test_commitment_arguments_without_a_device():
+    m = amd.load()
+    import ctypes
+    L = m.lib()
+    h = ctypes.c_void_p()
+    q = 661_473_542_593
+    assert L.lbn_cmt_len(64, 16, q, 0) == 16 * 64 * 648 * 5
+    assert L.lbn_cmt_len(64, 1, (1 << 95) & (2**64 - 1), 1 << 31) == 64 * 648 * 12
+    for cols, elems, rows in [(48, 64, 16), (8, 64, 1), (64, 0, 1), (64, 70000, 1), (64, 64, 17), (64, 64, 0)]:
+        assert L.lbn_cmt_create(0, cols, elems, rows, q, 0, ctypes.byref(h)) == -1, (cols, elems, rows)
+    assert L.lbn_cmt_create(0, 64, 64, 1, 1 << 30, 0, ctypes.byref(h)) == -4  # below the modulus gate
+    assert not h.value

@@ -245,3 +245,52 @@ def test_commitment_of_one_bit_is_a_shifted_row():
         raw[t] = 0
     want = b"".join((x % q).to_bytes(5, "little") for x in raw[:648])
     assert got.tobytes() == want
+
+
+Q_C3 = 661_473_542_593  # 3889 * 9721 * 17497
+
+
+def ring_product_of_bit(a, shift, q):
+    """X^shift * a mod (X^648 - X^324 + 1), coefficients in [0, q)."""
+    raw = [0] * 1295
+    for t in range(648):
+        raw[t + shift] += a[t]
+    for t in range(1294, 647, -1):
+        raw[t - 324] += raw[t]
+        raw[t - 648] -= raw[t]
+        raw[t] = 0
+    return [x % q for x in raw[:648]]
+
+
+def test_key_row_zero_is_the_matrix():
+    """Key row 0 of a witness laid out like a 2^20 table is the prover's commitment."""
+    tab = O.table(bytes(range(32)), 20)
+    got = O.commit_key_row(tab, 2, 0, O.DEFAULT_Q, 0, 6)
+    want = O.commit_columns(tab, 20, O.DEFAULT_Q, 0, 6)
+    assert got.tobytes() == want.tobytes()
+
+
+def test_key_row_of_one_bit_is_a_shifted_key_element():
+    """One set bit (bit p of word k of element m, column c) commits, in key row r, to
+    X^(4p+k) A[r * elements + m]."""
+    elements, cols = 3, 2
+    for (c, m, k, p, r) in [(1, 2, 3, 127, 5), (0, 0, 0, 0, 0), (1, 1, 1, 64, 15)]:
+        w = np.zeros(cols * elements * 64, dtype=np.uint8)
+        byte = ((c * elements + m) * 4 + k) * 16 + p // 8
+        w[byte] |= 1 << (p % 8)
+        got = O.commit_key_row(w, elements, r, Q_C3, c, 1)[0]
+        a = O.ajtai_element(Q_C3, elements, r * elements + m)
+        want = b"".join(x.to_bytes(5, "little") for x in ring_product_of_bit(a, 4 * p + k, Q_C3))
+        assert got.tobytes() == want, (c, m, k, p, r)
+        other = O.commit_key_row(w, elements, r, Q_C3, 1 - c, 1)[0]
+        assert not other.any()  # the other column is empty
+
+
+def test_key_row_commitment_is_additive_over_disjoint_bits():
+    rng = np.random.default_rng(7)
+    elements = 4
+    w = rng.integers(0, 256, size=2 * elements * 64, dtype=np.uint8)
+    mask = rng.integers(0, 256, size=w.size, dtype=np.uint8)
+    parts = [O.commit_key_row(x, elements, 3, Q_C3, 0, 2) for x in (w, w & mask, w & ~mask)]
+    dec = [np.frombuffer(np.pad(x.reshape(-1, 5), ((0, 0), (0, 3))).tobytes(), dtype="<u8") for x in parts]
+    assert ((dec[1] + dec[2]) % Q_C3 == dec[0]).all()

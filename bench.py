@@ -25,6 +25,11 @@ import time
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
+# Proofs in flight each run on streams of their own; HIP maps a process's streams onto
+# GPU_MAX_HW_QUEUES hardware queues (4 unless set), and kernels of streams sharing a queue run one
+# after another. The runtime reads the variable when it initialises, so it is set before the
+# library loads; a value already in the environment is kept.
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
@@ -130,6 +135,7 @@ def batched(m, workers):
     seq, t_seq = run_jobs(m, jobs, 1)
     par, t_par = run_jobs(m, jobs, workers)
     return {"instances": len(jobs), "k": sorted({kk for kk, _ in jobs}), "workers": workers,
+            "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")),
             "proofs_per_s": round(len(jobs) / t_par, 1), "sequential_proofs_per_s": round(len(jobs) / t_seq, 1),
             "speedup": round(t_seq / t_par, 2), "identical_to_sequential": seq == par}
 
@@ -174,6 +180,7 @@ def serving(m, requests, rate, workers, budget):
     mix, proofs_mix = serve(workers)
     seq, proofs_seq = serve(1)
     return {"metric": "serving p50/p99 ms and proofs/s on 1 MI355X", "requests": requests, "rate_per_s": rate,
+            "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")),
             "workers": workers, "hbm_budget_gib": budget / 2**30, "batched": mix, "sequential": seq,
             "identical_sample": proofs_mix == proofs_seq}
 

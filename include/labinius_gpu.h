@@ -102,6 +102,13 @@ size_t lbn_proof_len(int log_vars, uint64_t q_lo, uint64_t q_hi);
 int lbn_prove(lbn_ctx *ctx, const uint8_t seed[32], uint8_t *proof, size_t cap, size_t *len,
               lbn_profile *prof);
 
+/* Pinned host memory for proof buffers (no counterpart in the reference, which writes the proof
+ * from an ordinary Vec): when lbn_prove's proof buffer lies in an lbn_host_alloc allocation, the
+ * fold kernel writes its output into it directly and the commitment bytes are copied into it by DMA,
+ * instead of both being staged and copied by the host. Any other buffer works as before. */
+void *lbn_host_alloc(size_t bytes);
+void lbn_host_free(void *p);
+
 /* Number of CRT primes the commitment uses for the current modulus (diagnostics). */
 int lbn_commit_primes(const lbn_ctx *ctx);
 

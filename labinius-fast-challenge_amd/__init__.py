@@ -103,6 +103,10 @@ def lib():
         L.lbn_sched_get_stats.argtypes = [P, P]
         L.lbn_sched_destroy.argtypes = [P]
         L.lbn_sched_destroy.restype = None
+        L.lbn_host_alloc.argtypes = [ctypes.c_size_t]
+        L.lbn_host_alloc.restype = ctypes.c_void_p
+        L.lbn_host_free.argtypes = [P]
+        L.lbn_host_free.restype = None
         L.lbn_cmt_len.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64]
         L.lbn_cmt_len.restype = ctypes.c_size_t
         L.lbn_cmt_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
@@ -328,6 +332,30 @@ class Scheduler:
             lib().lbn_sched_destroy(self.h)
             self.h = None
             self._live.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class PinnedBuffer:
+    """Host memory from lbn_host_alloc (pinned, mapped for the device); `.array` is a uint8 view. A
+    proof written into it skips the host-side staging copies."""
+
+    def __init__(self, nbytes: int):
+        ptr = lib().lbn_host_alloc(nbytes)
+        if not ptr:
+            raise LbnError(-3, "lbn_host_alloc failed")
+        self.ptr, self.nbytes = ptr, nbytes
+        self.array = np.ctypeslib.as_array((ctypes.c_uint8 * nbytes).from_address(ptr))
+
+    def close(self):
+        if self.ptr:
+            self.array = None
+            lib().lbn_host_free(ctypes.c_void_p(self.ptr))
+            self.ptr = None
 
     def __del__(self):
         try:

@@ -62,6 +62,7 @@ struct lbn::DeviceState {
     // Pinned host staging: the commitment bytes (copied on copy_stream once the commitment is
     // ready, while the opening runs) and the fold output, which the fold kernel writes directly.
     uint8_t *hstage = nullptr;
+    bool staged_direct = false;     // the last commitment was copied straight into the proof buffer
     int32_t *hfold = nullptr;
     uint8_t *hbox = nullptr;        // pinned mailbox for the small transfers (regions in namespace mb)
     uint32_t *hcvs = nullptr;       // pinned: chunk chaining values of the commitment
@@ -1027,11 +1028,18 @@ __global__ __launch_bounds__(FOLD_THREADS) void k_fold(const uint4 *table, size_
             // positives then complemented negatives, four list entries at a time
             int e = e0;
             for (; e + 4 <= e2; e += 4) {
+                // the four list entries' indices, then their words, then the complement of the
+                // negatives as a mask (no branch: lanes' sign boundaries differ)
+                int ci[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) ci[u] = s_cols[e + u];
                 uint4 w[4];
 #pragma unroll
+                for (int u = 0; u < 4; u++) w[u] = s_w[ci[u] * 4 + k];
+#pragma unroll
                 for (int u = 0; u < 4; u++) {
-                    w[u] = s_w[s_cols[e + u] * 4 + k];
-                    if (e + u >= e1) w[u] = make_uint4(~w[u].x, ~w[u].y, ~w[u].z, ~w[u].w);
+                    const uint32_t neg = e + u >= e1 ? ~0u : 0u;
+                    w[u] = make_uint4(w[u].x ^ neg, w[u].y ^ neg, w[u].z ^ neg, w[u].w ^ neg);
                 }
                 ctr_add4<NB>(c0, w[0].x, w[1].x, w[2].x, w[3].x);
                 ctr_add4<NB>(c1, w[0].y, w[1].y, w[2].y, w[3].y);
@@ -1040,7 +1048,8 @@ __global__ __launch_bounds__(FOLD_THREADS) void k_fold(const uint4 *table, size_
             }
             for (; e < e2; e++) {
                 uint4 w = s_w[s_cols[e] * 4 + k];
-                if (e >= e1) w = make_uint4(~w.x, ~w.y, ~w.z, ~w.w);
+                const uint32_t neg = e >= e1 ? ~0u : 0u;
+                w = make_uint4(w.x ^ neg, w.y ^ neg, w.z ^ neg, w.w ^ neg);
                 ctr_ripple<NB>(c0, w.x, 0);
                 ctr_ripple<NB>(c1, w.y, 0);
                 ctr_ripple<NB>(c2, w.z, 0);
@@ -1886,6 +1895,56 @@ extern "C" int lbn_generate_table(lbn_ctx *ctx, const uint8_t seed[32]) {
     return LBN_OK;
 }
 
+// ----------------------------------------------------------------------------- pinned buffers
+// Host buffers from lbn_host_alloc are pinned and mapped into the device's address space; the
+// prover writes the fold and the commitment bytes straight into such a proof buffer instead of
+// staging them. Only ranges inside a live lbn_host_alloc allocation qualify.
+namespace {
+std::mutex g_pin_m;
+std::map<uintptr_t, size_t> g_pins;  // base -> bytes
+}  // namespace
+
+extern "C" void *lbn_host_alloc(size_t bytes) {
+    if (!bytes) return nullptr;
+    void *p = nullptr;
+    if (hipHostMalloc(&p, bytes, hipHostMallocPortable | hipHostMallocMapped) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(g_pin_m);
+    g_pins[(uintptr_t)p] = bytes;
+    return p;
+}
+
+extern "C" void lbn_host_free(void *p) {
+    if (!p) return;
+    {
+        std::lock_guard<std::mutex> lk(g_pin_m);
+        if (!g_pins.erase((uintptr_t)p)) return;  // not ours
+    }
+    hipHostFree(p);
+}
+
+// The device address of [p, p + n) if it lies in one lbn_host_alloc allocation, else nullptr.
+static void *pinned_alias(const void *p, size_t n) {
+    const uintptr_t a = (uintptr_t)p;
+    uintptr_t base;
+    {
+        std::lock_guard<std::mutex> lk(g_pin_m);
+        auto it = g_pins.upper_bound(a);
+        if (it == g_pins.begin()) return nullptr;
+        --it;
+        if (a + n > it->first + it->second) return nullptr;
+        base = it->first;
+    }
+    void *dev = nullptr;
+    if (hipHostGetDevicePointer(&dev, (void *)base, 0) != hipSuccess || !dev) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    return (uint8_t *)dev + (a - base);
+}
+
 // ----------------------------------------------------------------------------- modulus, commit
 
 static u128 mul_mod_small(u128 a, uint32_t b, u128 m) { return (a * b) % m; }
@@ -2104,9 +2163,12 @@ int lbn::dev_commit_cvs(lbn_ctx *ctx, size_t off, uint64_t first, size_t count, 
     if (off + count * 1024 > clen || head > clen || tail > clen)
         return ctx->fail(LBN_E_ARG, "commitment chunk range out of bounds");
     HIPCHK(ctx, hipSetDevice(ctx->device));
-    // the staging copy runs on its own stream once the commitment is final
+    // the staging copy runs on its own stream once the commitment is final, into the caller's
+    // buffer when it is one of ours (pinned)
+    d->staged_direct = pinned_alias(dst, clen) != nullptr;
     HIPCHK(ctx, hipStreamWaitEvent(d->copy_stream, d->ev_commit, 0));
-    HIPCHK(ctx, hipMemcpyAsync(d->hstage, d->cbytes, clen, hipMemcpyDeviceToHost, d->copy_stream));
+    HIPCHK(ctx, hipMemcpyAsync(d->staged_direct ? dst : d->hstage, d->cbytes, clen, hipMemcpyDeviceToHost,
+                               d->copy_stream));
     HIPCHK(ctx, hipEventRecord(d->ev_staged, d->copy_stream));
     if (count) {
         B3Key k;
@@ -2133,7 +2195,7 @@ int lbn::dev_commit_bytes(lbn_ctx *ctx, uint8_t *dst) {
     DeviceState *d = ctx->dev;
     HIPCHK(ctx, hipSetDevice(ctx->device));
     HIPCHK(ctx, hipEventSynchronize(d->ev_staged));
-    copy_parallel(dst, d->hstage, (size_t)COLUMNS * N * ctx->width);
+    if (!d->staged_direct) copy_parallel(dst, d->hstage, (size_t)COLUMNS * N * ctx->width);
     return LBN_OK;
 }
 
@@ -2492,11 +2554,14 @@ extern "C" int lbn_fold(lbn_ctx *ctx, const lbn_challenge *ch, int n_challenges,
     const unsigned grid = (unsigned)std::min<size_t>(s.ring_elements, (size_t)cus);
     const int n_entries = off[2 * 162];
     if (n_entries > FOLD_MAX_ENTRIES) return ctx->fail(LBN_E_ARG, "challenge lists too long");
+    // into the caller's buffer when it is one of ours (pinned), else into the context's staging
+    int32_t *direct = ((uintptr_t)out & 3) ? nullptr : (int32_t *)pinned_alias(out, n_out * 4);
+    int32_t *dst = direct ? direct : d->fold_host;
     tstart(ctx, T_FOLD);
 #define FOLD_CASE(B)                                                                                          \
     case B:                                                                                                   \
         k_fold<B><<<grid, FOLD_THREADS, 0, d->stream>>>(d->table, s.column_len, s.ring_elements, d->fold_off,   \
-                                                        d->fold_cols, n_entries, d->fold_host, d->fold_flag); \
+                                                        d->fold_cols, n_entries, dst, d->fold_flag);          \
         break;
     switch (std::max(nb, 6)) {
         FOLD_CASE(6)
@@ -2521,7 +2586,7 @@ extern "C" int lbn_fold(lbn_ctx *ctx, const lbn_challenge *ch, int n_challenges,
     memcpy(&flag, d->hbox + mb::MISC, 4);
     tcollect(ctx, T_FOLD);
     if (flag) return ctx->fail(LBN_E_RANGE, "a fold coefficient does not fit the wire codec");
-    copy_parallel((uint8_t *)out, stage, n_out * 4);
+    if (!direct) copy_parallel((uint8_t *)out, stage, n_out * 4);
     return LBN_OK;
 }
 

@@ -72,3 +72,13 @@ def test_commitment_arguments_without_a_device():
         assert L.lbn_cmt_create(0, cols, elems, rows, q, 0, ctypes.byref(h)) == -1, (cols, elems, rows)
     assert L.lbn_cmt_create(0, 64, 64, 1, 1 << 30, 0, ctypes.byref(h)) == -4  # below the modulus gate
     assert not h.value
+
+
+def test_pinned_allocation_arguments_without_a_device():
+    m = amd.load()
+    import ctypes
+    L = m.lib()
+    assert not L.lbn_host_alloc(0)
+    L.lbn_host_free(None)
+    buf = ctypes.create_string_buffer(64)
+    L.lbn_host_free(ctypes.cast(buf, ctypes.c_void_p))  # not from lbn_host_alloc: ignored

@@ -137,6 +137,28 @@ def test_deterministic(m, p20):
     assert a == b
 
 
+def test_pinned_proof_buffer(m, p20):
+    """A proof written into library-pinned memory (the fold kernel and the commitment copy write it
+    directly) equals the proof written through staging into an ordinary array; an unaligned slice
+    of the pinned buffer takes the staged path and gives the same bytes."""
+    say("pinned proof buffer")
+    p20.generate_table(SEEDS[1])
+    p20.set_modulus(O.DEFAULT_Q)
+    want, _ = p20.prove(SEEDS[1])
+    n = len(want)
+    buf = m.PinnedBuffer(n + 64)
+    try:
+        buf.array[:] = 0xA5
+        got_n, _ = p20.prove_into(SEEDS[1], buf.array[:n])
+        assert got_n == n and buf.array[:n].tobytes() == want
+        assert (buf.array[n:] == 0xA5).all()  # nothing written past the proof
+        buf.array[:] = 0
+        got_n, _ = p20.prove_into(SEEDS[1], buf.array[1:n + 1])
+        assert buf.array[1:n + 1].tobytes() == want
+    finally:
+        buf.close()
+
+
 def test_errors(m, p20):
     say("error paths")
     with pytest.raises(m.LbnError) as e:

@@ -97,9 +97,12 @@ void Blake3::chunk_cv(uint32_t out[8]) const {
     memcpy(out, o, 32);
 }
 
-void Blake3::add_chunk_cv(const uint32_t cv_in[8], uint64_t total) {
+// The stack holds one subtree per set bit of the chunks completed so far; a subtree of 2^level
+// chunks starting at a multiple of its size merges like a chunk does, counted in its own units.
+void Blake3::add_subtree_cv(const uint32_t cv_in[8], uint64_t total, int level) {
     uint32_t cv[8];
     memcpy(cv, cv_in, 32);
+    total >>= level;
     while ((total & 1) == 0) {
         uint32_t m[16], o[16];
         depth_--;
@@ -142,8 +145,21 @@ void Blake3::update(const void *data, size_t len) {
     }
 }
 
+Blake3::Subtrees Blake3::subtrees(uint64_t first, size_t count) {
+    Subtrees out;
+    uint64_t s = first;
+    const uint64_t end = first + count;
+    while (s < end) {
+        int j = 0;
+        while (j < 62 && ((s >> j) & 1) == 0 && s + (2ull << j) <= end) j++;
+        out.emplace_back(s, j);
+        s += 1ull << j;
+    }
+    return out;
+}
+
 void Blake3::update_with(const uint8_t *p, size_t len,
-                         const std::function<void(size_t, uint64_t, size_t, uint32_t *)> &cvs_of) {
+                         const std::function<void(size_t, uint64_t, size_t, const Subtrees &, uint32_t *)> &cvs_of) {
     // The head completes the current chunk; whole chunks strictly before the last byte are
     // hashed by cvs_of (called before any byte is read, so it may also provide the bytes); the
     // rest is absorbed here.
@@ -151,9 +167,10 @@ void Blake3::update_with(const uint8_t *p, size_t len,
     const size_t take = in_chunk < CHUNK ? std::min(len, CHUNK - in_chunk) : 0;
     const size_t rest = len - take;
     const size_t whole = rest ? (rest - 1) / CHUNK : 0;
-    std::vector<uint32_t> cvs(whole * 8);
     const uint64_t first = chunk_ + 1;  // the current chunk is closed before the whole ones
-    cvs_of(take, first, whole, cvs.data());
+    const Subtrees trees = subtrees(first, whole);
+    std::vector<uint32_t> cvs(trees.size() * 8);
+    cvs_of(take, first, whole, trees, cvs.data());
     update(p, take);
     p += take;
     if (whole == 0) {
@@ -169,8 +186,10 @@ void Blake3::update_with(const uint8_t *p, size_t len,
         buf_len_ = 0;
         memcpy(cv_, key_, 32);
     }
-    // Every whole chunk is pushed, the last one too, since at least one more byte follows it.
-    for (size_t c = 0; c < whole; c++) add_chunk_cv(&cvs[8 * c], first + c + 1);
+    // Every whole chunk is pushed (as its subtree), the last one too, since at least one more byte
+    // follows it; chunk index c completes c + 1 chunks.
+    for (size_t i = 0; i < trees.size(); i++)
+        add_subtree_cv(&cvs[8 * i], trees[i].first + (1ull << trees[i].second), trees[i].second);
     chunk_ = first + whole;
     update(p + whole * CHUNK, rest - whole * CHUNK);
 }

@@ -7,6 +7,8 @@
 #include <cstdint>
 #include <cstring>
 #include <functional>
+#include <utility>
+#include <vector>
 
 namespace lbn {
 
@@ -21,13 +23,19 @@ class Blake3 {
     static Blake3 derive_key(const char *context);
 
     void update(const void *data, size_t len);
+    // A run of whole chunks split into aligned complete subtrees: (first chunk index, log2 of the
+    // chunk count), each start a multiple of its size, in order, every subtree as large as the run
+    // allows. Pushing each subtree's chaining value leaves the same tree state as pushing its
+    // chunks one by one.
+    using Subtrees = std::vector<std::pair<uint64_t, int>>;
+    static Subtrees subtrees(uint64_t first, size_t count);
     // The same state as update(data, len), with the whole chunks strictly inside the input hashed
-    // elsewhere: cvs_of(offset, first, count, cvs) is called once, before any byte of data is
-    // read, and must leave in cvs the chaining values of the `count` chunks data[offset + i * CHUNK
-    // ..] with chunk indices first + i (key and flags of this hasher). Only the bytes outside those
-    // chunks are read here, after the call.
+    // elsewhere: cvs_of(offset, first, count, trees, cvs) is called once, before any byte of data
+    // is read, and must leave in cvs the chaining value of each subtree of `trees` (the split of the
+    // `count` chunks data[offset + i * CHUNK ..] with chunk indices first + i, key and flags of
+    // this hasher). Only the bytes outside those chunks are read here, after the call.
     void update_with(const uint8_t *data, size_t len,
-                     const std::function<void(size_t, uint64_t, size_t, uint32_t *)> &cvs_of);
+                     const std::function<void(size_t, uint64_t, size_t, const Subtrees &, uint32_t *)> &cvs_of);
     const uint32_t *key_words() const { return key_; }
     // The whole state as plain data (counter: the caller's transcript counter, carried along).
     template <class S>
@@ -76,7 +84,9 @@ class Blake3 {
    private:
     void init(const uint32_t key[8], uint32_t flags);
     void chunk_cv(uint32_t out[8]) const;  // current chunk as a non-root chunk
-    void add_chunk_cv(const uint32_t cv[8], uint64_t total);
+    void add_chunk_cv(const uint32_t cv[8], uint64_t total) { add_subtree_cv(cv, total, 0); }
+    // a complete subtree of 2^level chunks, after which `total` chunks are complete
+    void add_subtree_cv(const uint32_t cv[8], uint64_t total, int level);
     void absorb_block();
     uint32_t key_[8];
     uint32_t flags_;

@@ -245,6 +245,59 @@ __global__ __launch_bounds__(64) void k_chunk_cvs(const uint8_t *bytes, size_t o
     for (int k = 0; k < 8; k++) cvs[8 * c + k] = cv[k];
 }
 
+// The chaining values of aligned complete subtrees of those chunks: tree b covers chunk CVs
+// cvs[start[b] ..] (2^level[b] of them, relative to the run's first chunk). One workgroup merges
+// level by level in place (parent = compress(key, left || right, counter 0, 64 bytes,
+// flags | PARENT)), then leaves tree b's value at cvs[8 b]. Levels are separated by a device-scope
+// fence and a barrier, so each level reads the previous level's stores.
+constexpr int MAX_CV_TREES = 64;
+struct CvTrees {
+    int n;
+    uint32_t start[MAX_CV_TREES];
+    int level[MAX_CV_TREES];
+};
+
+__global__ __launch_bounds__(1024) void k_cv_tree(uint32_t *cvs, CvTrees tr, B3Key key) {
+    const int t = threadIdx.x;
+    int maxl = 0;
+    for (int b = 0; b < tr.n; b++) maxl = max(maxl, tr.level[b]);
+    for (int l = 0; l < maxl; l++) {
+        int total = 0;
+        for (int b = 0; b < tr.n; b++)
+            if (tr.level[b] > l) total += 1 << (tr.level[b] - l - 1);
+        for (int w = t; w < total; w += blockDim.x) {
+            int b = 0, rem = w;
+            for (;; b++) {
+                if (tr.level[b] <= l) continue;
+                const int cnt = 1 << (tr.level[b] - l - 1);
+                if (rem < cnt) break;
+                rem -= cnt;
+            }
+            uint32_t *left = cvs + 8 * ((size_t)tr.start[b] + ((size_t)rem << (l + 1)));
+            const uint32_t *right = left + 8 * ((size_t)1 << l);
+            uint32_t m[16];
+#pragma unroll
+            for (int k = 0; k < 8; k++) m[k] = left[k], m[k + 8] = right[k];
+            uint32_t st[16] = {key.k[0], key.k[1], key.k[2], key.k[3], key.k[4], key.k[5], key.k[6], key.k[7],
+                               0x6A09E667u, 0xBB67AE85u, 0x3C6EF372u, 0xA54FF53Au,
+                               0u, 0u, 64u, key.flags | 4u /*PARENT*/};
+            b3_rounds(st, m);
+#pragma unroll
+            for (int k = 0; k < 8; k++) left[k] = st[k] ^ st[k + 8];
+        }
+        __threadfence();
+        __syncthreads();
+    }
+    uint32_t v[8];
+    if (t < tr.n)
+#pragma unroll
+        for (int k = 0; k < 8; k++) v[k] = cvs[8 * (size_t)tr.start[t] + k];
+    __syncthreads();
+    if (t < tr.n)
+#pragma unroll
+        for (int k = 0; k < 8; k++) cvs[8 * t + k] = v[k];
+}
+
 // ---------------------------------------------------------------------------------------------
 // eq_hi[jh * 4096 + jl] = EH[jh] * EL[jl] in GF(2^128): EH[jh] times the 12-coordinate
 // expansion EL, through a 16 x 256 byte-window table of y -> EH[jh] * y.
@@ -2156,12 +2209,26 @@ extern "C" int lbn_commit(lbn_ctx *ctx, uint8_t *out, size_t out_len) {
     return LBN_OK;
 }
 
-int lbn::dev_commit_cvs(lbn_ctx *ctx, size_t off, uint64_t first, size_t count, const uint32_t key[8],
-                        uint32_t flags, uint32_t *cvs, uint8_t *dst, size_t head, size_t tail) {
+int lbn::dev_commit_cvs(lbn_ctx *ctx, size_t off, uint64_t first, size_t count, const Blake3::Subtrees &trees,
+                        const uint32_t key[8], uint32_t flags, uint32_t *cvs, uint8_t *dst, size_t head, size_t tail) {
     DeviceState *d = ctx->dev;
     const size_t clen = (size_t)COLUMNS * N * ctx->width;
     if (off + count * 1024 > clen || head > clen || tail > clen)
         return ctx->fail(LBN_E_ARG, "commitment chunk range out of bounds");
+    if (trees.size() > MAX_CV_TREES) return ctx->fail(LBN_E_ARG, "too many commitment subtrees");
+    CvTrees tr{};
+    tr.n = (int)trees.size();
+    {
+        uint64_t at = first;  // the subtrees tile [first, first + count) in order
+        for (int b = 0; b < tr.n; b++) {
+            if (trees[b].first != at || trees[b].second < 0 || trees[b].second > 30)
+                return ctx->fail(LBN_E_ARG, "commitment subtrees do not tile the chunks");
+            tr.start[b] = (uint32_t)(at - first);
+            tr.level[b] = trees[b].second;
+            at += 1ull << trees[b].second;
+        }
+        if (at != first + count) return ctx->fail(LBN_E_ARG, "commitment subtrees do not tile the chunks");
+    }
     HIPCHK(ctx, hipSetDevice(ctx->device));
     // the staging copy runs on its own stream once the commitment is final, into the caller's
     // buffer when it is one of ours (pinned)
@@ -2176,14 +2243,16 @@ int lbn::dev_commit_cvs(lbn_ctx *ctx, size_t off, uint64_t first, size_t count, 
         k.flags = flags;
         k_chunk_cvs<<<(unsigned)((count + 63) / 64), 64, 0, d->stream>>>(d->cbytes, off, first, count, k, d->cvs);
         HIPCHK(ctx, hipGetLastError());
-        HIPCHK(ctx, hipMemcpyAsync(d->hcvs, d->cvs, count * 32, hipMemcpyDeviceToHost, d->stream));
+        k_cv_tree<<<1, 1024, 0, d->stream>>>(d->cvs, tr, k);
+        HIPCHK(ctx, hipGetLastError());
+        HIPCHK(ctx, hipMemcpyAsync(d->hcvs, d->cvs, trees.size() * 32, hipMemcpyDeviceToHost, d->stream));
     }
     if (head > 1024 || clen - tail > 1024) return ctx->fail(LBN_E_ARG, "commitment head or tail above one chunk");
     if (head) HIPCHK(ctx, hipMemcpyAsync(d->hbox + mb::HEAD, d->cbytes, head, hipMemcpyDeviceToHost, d->stream));
     if (tail < clen)
         HIPCHK(ctx, hipMemcpyAsync(d->hbox + mb::TAIL, d->cbytes + tail, clen - tail, hipMemcpyDeviceToHost, d->stream));
     HIPCHK(ctx, hipStreamSynchronize(d->stream));
-    memcpy(cvs, d->hcvs, count * 32);
+    memcpy(cvs, d->hcvs, trees.size() * 32);
     memcpy(dst, d->hbox + mb::HEAD, head);
     memcpy(dst + tail, d->hbox + mb::TAIL, clen - tail);
     tcollect(ctx, T_MAC);

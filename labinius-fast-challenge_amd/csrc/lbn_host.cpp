@@ -206,7 +206,7 @@ class Transcript {
     }
     // absorb_bytes with the whole chunks hashed by cvs_of (see Blake3::update_with).
     void absorb_bytes_with(const uint8_t *b, size_t n,
-                           const std::function<void(size_t, uint64_t, size_t, uint32_t *)> &cvs_of) {
+                           const std::function<void(size_t, uint64_t, size_t, const Blake3::Subtrees &, uint32_t *)> &cvs_of) {
         h_.update_u64(n);
         h_.update_with(b, n, cvs_of);
     }
@@ -575,10 +575,12 @@ extern "C" int lbn_prove(lbn_ctx *ctx, const uint8_t seed[32], uint8_t *proof, s
     // The commitment's whole BLAKE3 chunks are hashed where the bytes are (device); the host reads
     // back their chaining values and the bytes around them. All bytes reach the proof at the end.
     int cvs_rc = LBN_OK;
-    t.absorb_bytes_with(proof + off_commit, clen, [&](size_t off, uint64_t first, size_t count, uint32_t *cvs) {
-        cvs_rc = dev_commit_cvs(ctx, off, first, count, t.hasher().key_words(), t.hasher().flags(), cvs,
-                                proof + off_commit, off, off + count * Blake3::CHUNK);
-    });
+    t.absorb_bytes_with(proof + off_commit, clen,
+                        [&](size_t off, uint64_t first, size_t count, const Blake3::Subtrees &trees, uint32_t *cvs) {
+                            cvs_rc = dev_commit_cvs(ctx, off, first, count, trees, t.hasher().key_words(),
+                                                    t.hasher().flags(), cvs, proof + off_commit, off,
+                                                    off + count * Blake3::CHUNK);
+                        });
     TRY(cvs_rc);
     const double t1 = now_us();
     std::vector<B128> r_lo = t.sample_b128("r-lo", 7);

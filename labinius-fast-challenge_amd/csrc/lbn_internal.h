@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "../../include/labinius_gpu.h"
+#include "lbn_blake3.h"
 
 typedef unsigned __int128 u128;
 
@@ -131,8 +132,8 @@ int dev_commit_launch(lbn_ctx *ctx);
 // indices first.., under key/flags, into cvs (8 words each); also copies the commitment bytes
 // [0, head) and [tail, clen) to the same offsets of dst, and starts copying all of it to pinned
 // host memory (dev_commit_bytes finishes that copy into dst).
-int dev_commit_cvs(lbn_ctx *ctx, size_t off, uint64_t first, size_t count, const uint32_t key[8], uint32_t flags,
-                   uint32_t *cvs, uint8_t *dst, size_t head, size_t tail);
+int dev_commit_cvs(lbn_ctx *ctx, size_t off, uint64_t first, size_t count, const Blake3::Subtrees &trees,
+                   const uint32_t key[8], uint32_t flags, uint32_t *cvs, uint8_t *dst, size_t head, size_t tail);
 int dev_commit_bytes(lbn_ctx *ctx, uint8_t *dst);
 // Every switch round on the device after lbn_switch_begin: the transcript (state ts in, advanced
 // state out) absorbs each message and samples each challenge there. rounds: lp x 2 F162,

@@ -67,6 +67,8 @@ struct lbn::DeviceState {
     uint8_t *hbox = nullptr;        // pinned mailbox for the small transfers (regions in namespace mb)
     uint32_t *hcvs = nullptr;       // pinned: chunk chaining values of the commitment
     struct SwitchDev *sw = nullptr; // the switch rounds' device state (transcript, L-map, outputs)
+    ChalArg *chal = nullptr;        // the challenge sampler's input
+    lbn_challenge *chals = nullptr; // and output
     int32_t *fold_host = nullptr;   // device address of hfold
     hipStream_t copy_stream = nullptr;
     hipEvent_t ev_commit = nullptr, ev_staged = nullptr;
@@ -91,7 +93,9 @@ constexpr size_t MISC = ROW + 24576;      // flags, the last two switch values (
 constexpr size_t HEAD = MISC + 256;       // commitment bytes before the hashed chunks (<= 1 KiB)
 constexpr size_t TAIL = HEAD + 1024;      // and after them (<= 1 KiB)
 constexpr size_t SW = TAIL + 1024;        // the device switch state (SwitchDev, < 8 KiB)
-constexpr size_t SIZE = SW + 8192;
+constexpr size_t CH = SW + 8192;          // the challenge sampler's input (ChalArg, < 8 KiB)
+constexpr size_t CHOUT = CH + 8192;       // the sampled challenges (1024 lbn_challenge)
+constexpr size_t SIZE = CHOUT + 1024 * 40;
 }  // namespace mb
 
 // Timed passes (indices into lbn_timing's fields, in order).
@@ -1808,6 +1812,241 @@ __global__ __launch_bounds__(256) void k_commit_slicesum(const uint32_t *in, int
     out[(size_t)k * n + i] = mont(redc64(a0 + a1 + a2 + a3, pc), crt.r2[k], pc);
 }
 
+// ---------------------------------------------------------------------------------------------
+// The folding challenges (pcs/src/challenge.rs:396-715; sampled as main.rs:1877-1893): challenge
+// i comes from the transcript reader at counter + i under "short-challenge" (the transcript
+// absorbs u64(counter + i), u64(15) and the label into a copy of its state, and reads the XOF of
+// that copy's root). One wave per challenge: lane 0 runs the copy's chunk state and the output
+// stream (the transcript's chaining-value stack is shared and read-only in LDS; entries a completed
+// chunk merges with are popped from a private view of it) and draws the positions and signs; the
+// 81 sums of the norm check are split over the lanes and the wave votes. A rejected attempt
+// continues the same stream, as the host sampler does.
+// ---------------------------------------------------------------------------------------------
+namespace chal {
+constexpr uint32_t START = 1, END = 2, PARENT = 4, ROOT = 8;
+constexpr int WAVES = 4;  // challenges per block
+
+__device__ __forceinline__ void compress(const uint32_t *cv, const uint32_t *m_in, uint64_t counter, uint32_t len,
+                                         uint32_t flags, uint32_t *out, bool full) {
+    uint32_t s[16] = {cv[0], cv[1], cv[2], cv[3], cv[4], cv[5], cv[6], cv[7], 0x6A09E667u, 0xBB67AE85u,
+                      0x3C6EF372u, 0xA54FF53Au, (uint32_t)counter, (uint32_t)(counter >> 32), len, flags};
+    uint32_t m[16], c[8];
+#pragma unroll
+    for (int k = 0; k < 16; k++) m[k] = m_in[k];
+#pragma unroll
+    for (int k = 0; k < 8; k++) c[k] = cv[k];
+    b3_rounds(s, m);
+#pragma unroll
+    for (int k = 0; k < 8; k++) out[k] = s[k] ^ s[k + 8];
+    if (full)
+#pragma unroll
+        for (int k = 0; k < 8; k++) out[k + 8] = s[k + 8] ^ c[k];
+}
+
+// Lane 0's state, all arrays in the wave's LDS slice.
+struct Wave {
+    uint32_t cv[8], bw[16];  // chunk chaining value and the open block
+    uint32_t priv[2][8];     // stack entries above the shared ones
+    uint32_t rcv[8], rm[16]; // root node
+    uint32_t out[16];        // current output block
+    uint8_t perm[164];
+    uint8_t pos[32];
+    uint32_t signs, fail;
+};
+
+struct Reader {
+    Wave *w;
+    const uint32_t (*stack)[8];  // shared, in LDS
+    const uint32_t *key;
+    uint32_t flags, buf_len, rlen, rflags;
+    int blocks, sd, np, pos;
+    uint64_t chunk, block;
+
+    __device__ void chunk_cv(uint32_t *o) const {
+        compress(w->cv, w->bw, chunk, buf_len, flags | END | (blocks == 0 ? START : 0), o, false);
+    }
+    __device__ const uint32_t *entry(int i) const { return i < sd ? stack[i] : w->priv[i - sd]; }
+    __device__ void parent(const uint32_t *l, uint32_t *r) const {
+        uint32_t m[16];
+#pragma unroll
+        for (int k = 0; k < 8; k++) m[k] = l[k], m[k + 8] = r[k];
+        compress(key, m, 0, 64, flags | PARENT, r, false);
+    }
+    __device__ void byte(uint32_t b) {
+        if ((uint32_t)blocks * 64 + buf_len == 1024) {  // a full chunk and more input: close it
+            uint32_t c[8];
+            chunk_cv(c);
+            uint64_t total = chunk + 1;
+            while ((total & 1) == 0) {
+                const uint32_t *top = entry(sd + np - 1);
+                uint32_t l[8];
+#pragma unroll
+                for (int k = 0; k < 8; k++) l[k] = top[k];
+                if (np) np--;
+                else sd--;
+                parent(l, c);
+                total >>= 1;
+            }
+#pragma unroll
+            for (int k = 0; k < 8; k++) w->priv[np][k] = c[k];
+            np++;
+            chunk++;
+            blocks = 0;
+            buf_len = 0;
+#pragma unroll
+            for (int k = 0; k < 8; k++) w->cv[k] = key[k];
+#pragma unroll
+            for (int k = 0; k < 16; k++) w->bw[k] = 0;
+        }
+        if (buf_len == 64) {
+            uint32_t o[8];
+            compress(w->cv, w->bw, chunk, 64, flags | (blocks == 0 ? START : 0), o, false);
+#pragma unroll
+            for (int k = 0; k < 8; k++) w->cv[k] = o[k];
+#pragma unroll
+            for (int k = 0; k < 16; k++) w->bw[k] = 0;
+            blocks++;
+            buf_len = 0;
+        }
+        w->bw[buf_len >> 2] |= (b & 0xFFu) << (8 * (buf_len & 3));
+        buf_len++;
+    }
+    __device__ void finish() {
+        const int depth = sd + np;
+        if (depth == 0) {
+#pragma unroll
+            for (int k = 0; k < 8; k++) w->rcv[k] = w->cv[k];
+#pragma unroll
+            for (int k = 0; k < 16; k++) w->rm[k] = w->bw[k];
+            rlen = buf_len;
+            rflags = flags | END | (blocks == 0 ? START : 0);
+        } else {
+            uint32_t right[8];
+            chunk_cv(right);
+            for (int i = depth - 1; i > 0; i--) parent(entry(i), right);
+            const uint32_t *l = entry(0);
+#pragma unroll
+            for (int k = 0; k < 8; k++) w->rcv[k] = key[k], w->rm[k] = l[k], w->rm[k + 8] = right[k];
+            rlen = 64;
+            rflags = flags | PARENT;
+        }
+        block = 0;
+        pos = 64;
+    }
+    __device__ uint32_t next() {
+        if (pos == 64) {
+            compress(w->rcv, w->rm, block++, rlen, rflags | ROOT, w->out, true);
+            pos = 0;
+        }
+        const uint32_t b = (w->out[pos >> 2] >> (8 * (pos & 3))) & 0xFF;
+        pos++;
+        return b;
+    }
+};
+}  // namespace chal
+
+__global__ __launch_bounds__(64 * chal::WAVES) void k_challenges(const ChalArg *arg, lbn_challenge *out) {
+    __shared__ chal::Wave waves[chal::WAVES];
+    __shared__ uint32_t s_stack[54][8];
+    __shared__ uint32_t s_key[8];
+    __shared__ double s_cos[243], s_sin[243];
+    __shared__ int s_units[81];
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const B3State *ts = &arg->ts;
+    const int depth = ts->depth;
+    for (int k = t; k < depth * 8; k += blockDim.x) s_stack[k >> 3][k & 7] = ts->stack[k >> 3][k & 7];
+    for (int k = t; k < 243; k += blockDim.x) s_cos[k] = arg->cosv[k], s_sin[k] = arg->sinv[k];
+    if (t < 81) s_units[t] = arg->units[t];
+    if (t < 8) s_key[t] = ts->key[t];
+    __syncthreads();
+    const int i = blockIdx.x * chal::WAVES + wv;  // this wave's challenge
+    chal::Wave *w = &waves[wv];
+    chal::Reader r;
+    if (lane == 0) {
+        r.w = w;
+        r.stack = s_stack;
+        r.key = s_key;
+        r.flags = ts->flags;
+        for (int k = 0; k < 8; k++) w->cv[k] = ts->cv[k];
+        for (int k = 0; k < 16; k++) w->bw[k] = 0;
+        r.buf_len = ts->buf_len;
+        for (uint32_t b = 0; b < r.buf_len; b++) w->bw[b >> 2] |= (uint32_t)ts->buf[b] << (8 * (b & 3));
+        r.blocks = ts->blocks;
+        r.chunk = ts->chunk;
+        r.sd = depth;
+        r.np = 0;
+        // reader_at("short-challenge", i): u64(counter + i), u64(15), the label
+        const uint64_t ctr = ts->counter + (uint64_t)i;
+        for (int k = 0; k < 8; k++) r.byte((uint32_t)(ctr >> (8 * k)));
+        for (int k = 0; k < 8; k++) r.byte(k == 0 ? 15u : 0u);
+        const char label[16] = "short-challenge";
+        for (int k = 0; k < 15; k++) r.byte((uint32_t)(uint8_t)label[k]);
+        r.finish();
+    }
+    const double bound_sq = arg->bound_sq;
+    for (;;) {
+        if (lane == 0) {
+            // partial Fisher-Yates over 162 positions, two-byte draws below a multiple of the range
+            for (int k = 0; k < 162; k++) w->perm[k] = (uint8_t)k;
+            uint64_t seen[3] = {0, 0, 0};
+            for (int k = 0; k < WEIGHT; k++) {
+                const uint32_t n = 162 - k, limit = (65536u / n) * n;
+                uint32_t v;
+                for (;;) {
+                    const uint32_t lo = r.next(), hi = r.next();
+                    v = lo | (hi << 8);
+                    if (v < limit) break;
+                }
+                const int j = k + (int)(v % n);
+                const uint8_t p = w->perm[j];
+                w->perm[j] = w->perm[k];
+                seen[p >> 6] |= 1ull << (p & 63);
+            }
+            int n = 0;
+            for (int ww = 0; ww < 3; ww++)
+                for (uint64_t b = seen[ww]; b; b &= b - 1) w->pos[n++] = (uint8_t)(64 * ww + __builtin_ctzll(b));
+            // signs: the sign hasher over u64(WEIGHT) || positions, first 4 output bytes
+            uint32_t m[16], o[8];
+            for (int k = 0; k < 16; k++) m[k] = 0;
+            m[0] = WEIGHT;
+            for (int k = 0; k < WEIGHT; k++) m[2 + (k >> 2)] |= (uint32_t)w->pos[k] << (8 * (k & 3));
+            chal::compress(arg->sign_key, m, 0, 8 + WEIGHT, arg->sign_flags | chal::START | chal::END | chal::ROOT, o, false);
+            w->signs = o[0] & ((1u << WEIGHT) - 1);
+        }
+        __syncwarp();
+        // canonical embedding norm: unit k of Z/243 per lane (k = lane, lane + 64), the same
+        // additions in the same order as the host
+        const uint32_t signs = w->signs;
+        bool bad = false;
+        for (int k = lane; k < 81; k += 64) {
+            const int u = s_units[k];
+            double re = 0.0, im = 0.0;
+            for (int tt = 0; tt < WEIGHT; tt++) {
+                const int a = ((int)w->pos[tt] * u) % 243;
+                const bool neg = (signs >> tt) & 1;
+                re = re + (neg ? -s_cos[a] : s_cos[a]);
+                im = im + (neg ? -s_sin[a] : s_sin[a]);
+            }
+            const double a2 = re * re, b2 = im * im;
+            if (a2 + b2 > bound_sq) bad = true;
+        }
+        const bool reject = __ballot(bad) != 0;
+        __syncwarp();
+        if (!reject) break;
+    }
+    if (lane < 8) {  // positions (32 bytes, zero past WEIGHT), signs, weight
+        uint32_t v;
+        if (lane < 7)
+            v = (uint32_t)w->pos[4 * lane] | ((uint32_t)w->pos[4 * lane + 1] << 8) | ((uint32_t)w->pos[4 * lane + 2] << 16) |
+                ((uint32_t)w->pos[4 * lane + 3] << 24);
+        else
+            v = 0;
+        ((uint32_t *)&out[i])[lane] = v;
+    }
+    if (lane == 8) out[i].signs = w->signs;
+    if (lane == 9) out[i].weight = WEIGHT;
+}
+
 // =============================================================================================
 // Host side of the device passes
 // =============================================================================================
@@ -1849,6 +2088,8 @@ int lbn::dev_init(lbn_ctx *ctx) {
     HIPCHK(ctx, hipHostMalloc((void **)&d->hfold, s.ring_elements * N * sizeof(int32_t)));
     HIPCHK(ctx, hipHostMalloc((void **)&d->hbox, mb::SIZE));
     HIPCHK(ctx, hipMalloc(&d->sw, sizeof(SwitchDev)));
+    HIPCHK(ctx, hipMalloc(&d->chal, sizeof(ChalArg)));
+    HIPCHK(ctx, hipMalloc(&d->chals, COLUMNS * sizeof(lbn_challenge)));
     HIPCHK(ctx, hipHostGetDevicePointer((void **)&d->fold_host, d->hfold, 0));
     HIPCHK(ctx, hipStreamCreateWithFlags(&d->copy_stream, hipStreamNonBlocking));
     HIPCHK(ctx, hipEventCreateWithFlags(&d->ev_commit, hipEventDisableTiming));
@@ -1863,7 +2104,7 @@ void lbn::dev_free(lbn_ctx *ctx) {
     hipSetDevice(ctx->device);
     void *ptrs[] = {d->table, d->E[0], d->E[1], d->P[0], d->P[1], d->T, d->EH, d->EL, d->eqrow, d->eqlo,
                     d->rowU, d->rowOut, d->fold_flag, d->fold_off, d->fold_cols, d->ntt_tabs,
-                    d->ahat, d->cacc, d->cbytes, d->amat, d->cvs, d->sw};
+                    d->ahat, d->cacc, d->cbytes, d->amat, d->cvs, d->sw, d->chal, d->chals};
     if (d->copy_stream) hipStreamSynchronize(d->copy_stream);
     for (void *p : ptrs)
         if (p) hipFree(p);
@@ -2656,6 +2897,21 @@ extern "C" int lbn_fold(lbn_ctx *ctx, const lbn_challenge *ch, int n_challenges,
     tcollect(ctx, T_FOLD);
     if (flag) return ctx->fail(LBN_E_RANGE, "a fold coefficient does not fit the wire codec");
     if (!direct) copy_parallel((uint8_t *)out, stage, n_out * 4);
+    return LBN_OK;
+}
+
+int lbn::dev_challenges(lbn_ctx *ctx, const ChalArg &arg, lbn_challenge *out) {
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    DeviceState *d = ctx->dev;
+    static_assert(sizeof(ChalArg) <= 8192, "mailbox region");
+    memcpy(d->hbox + mb::CH, &arg, sizeof(ChalArg));
+    HIPCHK(ctx, hipMemcpyAsync(d->chal, d->hbox + mb::CH, sizeof(ChalArg), hipMemcpyHostToDevice, d->stream));
+    k_challenges<<<COLUMNS / chal::WAVES, 64 * chal::WAVES, 0, d->stream>>>(d->chal, d->chals);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipMemcpyAsync(d->hbox + mb::CHOUT, d->chals, COLUMNS * sizeof(lbn_challenge), hipMemcpyDeviceToHost,
+                               d->stream));
+    HIPCHK(ctx, hipStreamSynchronize(d->stream));
+    memcpy(out, d->hbox + mb::CHOUT, COLUMNS * sizeof(lbn_challenge));
     return LBN_OK;
 }
 

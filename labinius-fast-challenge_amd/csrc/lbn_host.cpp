@@ -269,76 +269,25 @@ class Transcript {
 // 429-441, rejection by max_u |c(zeta^u)|^2 <= 144 + 1e-12 in f64 with the reference's order of
 // operations (565-623): per root, terms added in position order from 0.0, then re*re + im*im.
 
+// The sampler's fixed data for the device (challenge.rs:396-715): the cos/sin tables of the 243rd
+// roots of unity as the host computes them, the units of Z/243 in increasing order, the sign
+// hasher (derive-key context "labinius 2026 challenge signs v1") and the norm bound 12^2 + 1e-12.
 struct ChallengeSampler {
-    double cosv[243], sinv[243];
-    int units[81];
-    uint8_t angle[81][162];  // (position * units[k]) mod 243
-    Blake3 sign_key;
-    ChallengeSampler() : sign_key(Blake3::derive_key("labinius 2026 challenge signs v1")) {
+    ChalArg arg{};
+    ChallengeSampler() {
         for (int m = 0; m < 243; m++) {
             volatile double angle = 2.0 * M_PI * (double)m / 243.0;
-            cosv[m] = std::cos(angle);
-            sinv[m] = std::sin(angle);
+            arg.cosv[m] = std::cos(angle);
+            arg.sinv[m] = std::sin(angle);
         }
         int n = 0;
         for (int x = 1; x < 243 && n < 81; x++)
-            if (x % 3) units[n++] = x;
-        for (int k = 0; k < 81; k++)
-            for (int p = 0; p < 162; p++) angle[k][p] = (uint8_t)((p * units[k]) % 243);
-    }
-    static uint32_t below(XofReader &x, uint32_t n) {
-        const uint32_t limit = (65536u / n) * n;
-        for (;;) {
-            uint32_t lo = x.byte(), hi = x.byte();
-            uint32_t r = lo | (hi << 8);
-            if (r < limit) return r % n;
-        }
-    }
-    void attempt(XofReader &x, lbn_challenge &c) const {
-        uint8_t perm[162];
-        uint64_t seen[3] = {0, 0, 0};
-        for (int i = 0; i < 162; i++) perm[i] = (uint8_t)i;
-        for (int i = 0; i < WEIGHT; i++) {
-            int j = i + (int)below(x, (uint32_t)(162 - i));
-            uint8_t p = perm[j];
-            perm[j] = perm[i];
-            seen[p >> 6] |= 1ull << (p & 63);
-        }
-        memset(&c, 0, sizeof c);
-        c.weight = WEIGHT;
-        int n = 0;
-        for (int w = 0; w < 3; w++)
-            for (uint64_t b = seen[w]; b; b &= b - 1) c.positions[n++] = (uint8_t)(64 * w + __builtin_ctzll(b));
-        Blake3 h = sign_key;
-        h.update_u64(WEIGHT);
-        h.update(c.positions, WEIGHT);
-        uint8_t o[4];
-        h.xof(0, o, 4);
-        uint32_t word = (uint32_t)o[0] | ((uint32_t)o[1] << 8) | ((uint32_t)o[2] << 16) | ((uint32_t)o[3] << 24);
-        c.signs = word & ((1u << WEIGHT) - 1);
-    }
-    bool within(const lbn_challenge &c, double bound_sq) const {
-        for (int k = 0; k < 81; k++) {
-            double re = 0.0, im = 0.0;
-            for (uint32_t t = 0; t < c.weight; t++) {
-                const int m = angle[k][c.positions[t]];
-                const bool neg = (c.signs >> t) & 1;
-                re = re + (neg ? -cosv[m] : cosv[m]);
-                im = im + (neg ? -sinv[m] : sinv[m]);
-            }
-            const double a = re * re, b = im * im;
-            if (a + b > bound_sq) return false;
-        }
-        return true;
-    }
-    lbn_challenge sample(const Blake3 &reader) const {
-        const double bound = 12.0, bound_sq = bound * bound + 1e-12;
-        XofReader x(reader);
-        lbn_challenge c;
-        for (;;) {
-            attempt(x, c);
-            if (within(c, bound_sq)) return c;
-        }
+            if (x % 3) arg.units[n++] = x;
+        const Blake3 sign_key = Blake3::derive_key("labinius 2026 challenge signs v1");
+        memcpy(arg.sign_key, sign_key.key_words(), 32);
+        arg.sign_flags = sign_key.flags();
+        const double bound = 12.0;
+        arg.bound_sq = bound * bound + 1e-12;
     }
 };
 
@@ -632,7 +581,11 @@ extern "C" int lbn_prove(lbn_ctx *ctx, const uint8_t seed[32], uint8_t *proof, s
     }
     static const ChallengeSampler sampler;
     std::vector<lbn_challenge> ch(COLUMNS);
-    parallel_for(COLUMNS, [&](size_t i) { ch[i] = sampler.sample(t.reader_at("short-challenge", i)); });
+    {
+        ChalArg ca = sampler.arg;
+        t.save(ca.ts);  // challenge i reads the transcript at its counter + i
+        TRY(dev_challenges(ctx, ca, ch.data()));
+    }
     t.skip(COLUMNS);
     const double t6 = now_us();
 

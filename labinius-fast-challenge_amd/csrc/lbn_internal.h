@@ -67,6 +67,19 @@ struct B3State {
     uint64_t counter;
 };
 
+// Everything the device needs to sample the folding challenges exactly as the host sampler does:
+// the transcript state after the row evaluation, the sign hasher's key and flags, the norm bound
+// and the host's cos/sin tables of the 243rd roots (so both sides add the same doubles).
+struct ChalArg {
+    B3State ts;
+    uint32_t sign_key[8];
+    uint32_t sign_flags;
+    uint32_t pad;
+    double bound_sq;
+    double cosv[243], sinv[243];
+    int32_t units[81];
+};
+
 // Layout of one prime's commitment-transform constants (u32 offsets), built by prime_tables().
 namespace tab {
 constexpr int TWB = 0;                 // [2][324][4] binary split+twist values: 0, b, eta b, (1+eta) b
@@ -139,6 +152,9 @@ int dev_commit_bytes(lbn_ctx *ctx, uint8_t *dst);
 // state out) absorbs each message and samples each challenge there. rounds: lp x 2 F162,
 // coords: lp F162, opened: F162 (as 3 u64 each).
 int dev_switch_rounds(lbn_ctx *ctx, B3State &ts, uint64_t *rounds, uint64_t *coords, uint64_t *opened);
+// The 1024 folding challenges sampled on the device from the transcript state in arg (the
+// counter of arg->ts is the first challenge's); out receives them in column order.
+int dev_challenges(lbn_ctx *ctx, const ChalArg &arg, lbn_challenge *out);
 
 // Host worker pool: f(i) for every i < n, on up to host_threads() threads (the caller included).
 int host_threads();

@@ -249,59 +249,6 @@ __global__ __launch_bounds__(64) void k_chunk_cvs(const uint8_t *bytes, size_t o
     for (int k = 0; k < 8; k++) cvs[8 * c + k] = cv[k];
 }
 
-// The chaining values of aligned complete subtrees of those chunks: tree b covers chunk CVs
-// cvs[start[b] ..] (2^level[b] of them, relative to the run's first chunk). One workgroup merges
-// level by level in place (parent = compress(key, left || right, counter 0, 64 bytes,
-// flags | PARENT)), then leaves tree b's value at cvs[8 b]. Levels are separated by a device-scope
-// fence and a barrier, so each level reads the previous level's stores.
-constexpr int MAX_CV_TREES = 64;
-struct CvTrees {
-    int n;
-    uint32_t start[MAX_CV_TREES];
-    int level[MAX_CV_TREES];
-};
-
-__global__ __launch_bounds__(1024) void k_cv_tree(uint32_t *cvs, CvTrees tr, B3Key key) {
-    const int t = threadIdx.x;
-    int maxl = 0;
-    for (int b = 0; b < tr.n; b++) maxl = max(maxl, tr.level[b]);
-    for (int l = 0; l < maxl; l++) {
-        int total = 0;
-        for (int b = 0; b < tr.n; b++)
-            if (tr.level[b] > l) total += 1 << (tr.level[b] - l - 1);
-        for (int w = t; w < total; w += blockDim.x) {
-            int b = 0, rem = w;
-            for (;; b++) {
-                if (tr.level[b] <= l) continue;
-                const int cnt = 1 << (tr.level[b] - l - 1);
-                if (rem < cnt) break;
-                rem -= cnt;
-            }
-            uint32_t *left = cvs + 8 * ((size_t)tr.start[b] + ((size_t)rem << (l + 1)));
-            const uint32_t *right = left + 8 * ((size_t)1 << l);
-            uint32_t m[16];
-#pragma unroll
-            for (int k = 0; k < 8; k++) m[k] = left[k], m[k + 8] = right[k];
-            uint32_t st[16] = {key.k[0], key.k[1], key.k[2], key.k[3], key.k[4], key.k[5], key.k[6], key.k[7],
-                               0x6A09E667u, 0xBB67AE85u, 0x3C6EF372u, 0xA54FF53Au,
-                               0u, 0u, 64u, key.flags | 4u /*PARENT*/};
-            b3_rounds(st, m);
-#pragma unroll
-            for (int k = 0; k < 8; k++) left[k] = st[k] ^ st[k + 8];
-        }
-        __threadfence();
-        __syncthreads();
-    }
-    uint32_t v[8];
-    if (t < tr.n)
-#pragma unroll
-        for (int k = 0; k < 8; k++) v[k] = cvs[8 * (size_t)tr.start[t] + k];
-    __syncthreads();
-    if (t < tr.n)
-#pragma unroll
-        for (int k = 0; k < 8; k++) cvs[8 * t + k] = v[k];
-}
-
 // ---------------------------------------------------------------------------------------------
 // eq_hi[jh * 4096 + jl] = EH[jh] * EL[jl] in GF(2^128): EH[jh] times the 12-coordinate
 // expansion EL, through a 16 x 256 byte-window table of y -> EH[jh] * y.
@@ -1812,12 +1759,88 @@ __global__ __launch_bounds__(256) void k_commit_slicesum(const uint32_t *in, int
     out[(size_t)k * n + i] = mont(redc64(a0 + a1 + a2 + a3, pc), crt.r2[k], pc);
 }
 
+// The chaining values of aligned complete subtrees of the commitment's chunks: tree b covers chunk
+// CVs cvs[start[b] ..] (2^level[b] of them, relative to the run's first chunk). One workgroup
+// merges level by level (parent = compress(key, left || right, counter 0, 64 bytes,
+// flags | PARENT)), in LDS when the run fits (IN_LDS: count * 32 bytes of dynamic LDS), else in
+// place in global memory with a device-scope fence between levels. Wide levels take one lane per
+// parent, narrow ones (at most a quarter of the lanes busy) one quad. Tree b's value is left at
+// cvs[8 b].
+constexpr int MAX_CV_TREES = 64;
+constexpr size_t CV_TREE_LDS_MAX = 120 * 1024;
+struct CvTrees {
+    int n;
+    uint32_t start[MAX_CV_TREES];
+    int level[MAX_CV_TREES];
+};
+
+template <bool IN_LDS>
+__global__ __launch_bounds__(1024) void k_cv_tree(uint32_t *cvs, int count, CvTrees tr, B3Key key) {
+    extern __shared__ uint32_t s_cv[];
+    uint32_t *base = IN_LDS ? s_cv : cvs;
+    const int t = threadIdx.x, q = t & 3, quad = t >> 2;
+    if (IN_LDS) {
+        for (int k = t; k < count * 8; k += blockDim.x) s_cv[k] = cvs[k];
+        __syncthreads();
+    }
+    uint32_t kw[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) kw[k] = key.k[k];
+    int maxl = 0;
+    for (int b = 0; b < tr.n; b++) maxl = max(maxl, tr.level[b]);
+    for (int l = 0; l < maxl; l++) {
+        int total = 0;
+        for (int b = 0; b < tr.n; b++)
+            if (tr.level[b] > l) total += 1 << (tr.level[b] - l - 1);
+        const bool quads = total <= (int)blockDim.x / 4;
+        const int first = quads ? quad : t, step = quads ? blockDim.x / 4 : blockDim.x;
+        for (int w = first; w < total; w += step) {  // with quads, the same w in the quad's lanes
+            int b = 0, rem = w;
+            for (;; b++) {
+                if (tr.level[b] <= l) continue;
+                const int cnt = 1 << (tr.level[b] - l - 1);
+                if (rem < cnt) break;
+                rem -= cnt;
+            }
+            uint32_t *left = base + 8 * ((size_t)tr.start[b] + ((size_t)rem << (l + 1)));
+            const uint32_t *right = left + 8 * ((size_t)1 << l);
+            uint32_t m[16];
+#pragma unroll
+            for (int k = 0; k < 8; k++) m[k] = left[k], m[k + 8] = right[k];
+            if (quads) {
+                uint32_t o[16];
+                b3d::compress(q, kw, m, 0, 64, key.flags | b3d::PARENT, o);
+                if (q == 0)
+#pragma unroll
+                    for (int k = 0; k < 8; k++) left[k] = o[k];
+            } else {
+                uint32_t st[16] = {kw[0], kw[1], kw[2], kw[3], kw[4], kw[5], kw[6], kw[7],
+                                   0x6A09E667u, 0xBB67AE85u, 0x3C6EF372u, 0xA54FF53Au,
+                                   0u, 0u, 64u, key.flags | b3d::PARENT};
+                b3_rounds(st, m);
+#pragma unroll
+                for (int k = 0; k < 8; k++) left[k] = st[k] ^ st[k + 8];
+            }
+        }
+        if (!IN_LDS) __threadfence();
+        __syncthreads();
+    }
+    uint32_t v[8];
+    if (t < tr.n)
+#pragma unroll
+        for (int k = 0; k < 8; k++) v[k] = base[8 * (size_t)tr.start[t] + k];
+    __syncthreads();
+    if (t < tr.n)
+#pragma unroll
+        for (int k = 0; k < 8; k++) cvs[8 * t + k] = v[k];
+}
+
 // ---------------------------------------------------------------------------------------------
 // The folding challenges (pcs/src/challenge.rs:396-715; sampled as main.rs:1877-1893): challenge
 // i comes from the transcript reader at counter + i under "short-challenge" (the transcript
 // absorbs u64(counter + i), u64(15) and the label into a copy of its state, and reads the XOF of
-// that copy's root). One wave per challenge: lane 0 runs the copy's chunk state and the output
-// stream (the transcript's chaining-value stack is shared and read-only in LDS; entries a completed
+// that copy's root). One wave per challenge: its first quad runs the copy's chunk state and the
+// output stream (the transcript's chaining-value stack is shared and read-only in LDS; entries a completed
 // chunk merges with are popped from a private view of it) and draws the positions and signs; the
 // 81 sums of the norm check are split over the lanes and the wave votes. A rejected attempt
 // continues the same stream, as the host sampler does.
@@ -1826,24 +1849,25 @@ namespace chal {
 constexpr uint32_t START = 1, END = 2, PARENT = 4, ROOT = 8;
 constexpr int WAVES = 4;  // challenges per block
 
+// The compression by the wave's first quad (lanes 0..3 run every step of the stream with the same
+// values; their LDS stores agree), out written by lane 0.
 __device__ __forceinline__ void compress(const uint32_t *cv, const uint32_t *m_in, uint64_t counter, uint32_t len,
                                          uint32_t flags, uint32_t *out, bool full) {
-    uint32_t s[16] = {cv[0], cv[1], cv[2], cv[3], cv[4], cv[5], cv[6], cv[7], 0x6A09E667u, 0xBB67AE85u,
-                      0x3C6EF372u, 0xA54FF53Au, (uint32_t)counter, (uint32_t)(counter >> 32), len, flags};
-    uint32_t m[16], c[8];
-#pragma unroll
-    for (int k = 0; k < 16; k++) m[k] = m_in[k];
+    const int q = threadIdx.x & 3;
+    uint32_t c[8], m[16], o[16];
 #pragma unroll
     for (int k = 0; k < 8; k++) c[k] = cv[k];
-    b3_rounds(s, m);
 #pragma unroll
-    for (int k = 0; k < 8; k++) out[k] = s[k] ^ s[k + 8];
+    for (int k = 0; k < 16; k++) m[k] = m_in[k];
+    b3d::compress(q, c, m, counter, len, flags, o);
+#pragma unroll
+    for (int k = 0; k < 8; k++) out[k] = o[k];
     if (full)
 #pragma unroll
-        for (int k = 0; k < 8; k++) out[k + 8] = s[k + 8] ^ c[k];
+        for (int k = 0; k < 8; k++) out[k + 8] = o[k + 8];
 }
 
-// Lane 0's state, all arrays in the wave's LDS slice.
+// The first quad's state, all arrays in the wave's LDS slice.
 struct Wave {
     uint32_t cv[8], bw[16];  // chunk chaining value and the open block
     uint32_t priv[2][8];     // stack entries above the shared ones
@@ -1962,7 +1986,7 @@ __global__ __launch_bounds__(64 * chal::WAVES) void k_challenges(const ChalArg *
     const int i = blockIdx.x * chal::WAVES + wv;  // this wave's challenge
     chal::Wave *w = &waves[wv];
     chal::Reader r;
-    if (lane == 0) {
+    if (lane < 4) {
         r.w = w;
         r.stack = s_stack;
         r.key = s_key;
@@ -1985,7 +2009,7 @@ __global__ __launch_bounds__(64 * chal::WAVES) void k_challenges(const ChalArg *
     }
     const double bound_sq = arg->bound_sq;
     for (;;) {
-        if (lane == 0) {
+        if (lane < 4) {
             // partial Fisher-Yates over 162 positions, two-byte draws below a multiple of the range
             for (int k = 0; k < 162; k++) w->perm[k] = (uint8_t)k;
             uint64_t seen[3] = {0, 0, 0};
@@ -2484,7 +2508,10 @@ int lbn::dev_commit_cvs(lbn_ctx *ctx, size_t off, uint64_t first, size_t count, 
         k.flags = flags;
         k_chunk_cvs<<<(unsigned)((count + 63) / 64), 64, 0, d->stream>>>(d->cbytes, off, first, count, k, d->cvs);
         HIPCHK(ctx, hipGetLastError());
-        k_cv_tree<<<1, 1024, 0, d->stream>>>(d->cvs, tr, k);
+        if (count * 32 <= CV_TREE_LDS_MAX)
+            k_cv_tree<true><<<1, 1024, count * 32, d->stream>>>(d->cvs, (int)count, tr, k);
+        else
+            k_cv_tree<false><<<1, 1024, 0, d->stream>>>(d->cvs, (int)count, tr, k);
         HIPCHK(ctx, hipGetLastError());
         HIPCHK(ctx, hipMemcpyAsync(d->hcvs, d->cvs, trees.size() * 32, hipMemcpyDeviceToHost, d->stream));
     }

@@ -26,10 +26,9 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 # Proofs in flight each run on streams of their own; HIP maps a process's streams onto
-# GPU_MAX_HW_QUEUES hardware queues (4 unless set), and kernels of streams sharing a queue run one
-# after another. The runtime reads the variable when it initialises, so it is set before the
-# library loads; a value already in the environment is kept.
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+# GPU_MAX_HW_QUEUES hardware queues (HIP's default is 4), and kernels of streams sharing a queue
+# run one after another. The runtime reads the variable when it initialises, so main() sets it
+# (--hw-queues) before the library loads.
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
@@ -44,15 +43,23 @@ MULMOD_PEAK_T = VALU_SLOTS_T / 5
 
 
 # Commit micro-benchmark: 64 columns x 4096 binary ring elements (a 16 MiB witness from seed 0),
-# Q = 3889 * 9721 * 17497, keys of 1 and 16 ring rows. Its roofline counts int32 VALU issue slots:
-# per (ring element, column, CRT prime) the transform's 3,564 modular products at 5 slots each, and
-# per key row 648 multiply-adds at 2 slots (one half-rate 32x32->64 multiply-add).
+# Q = 3889 * 9721 * 17497, keys of 1 and 16 ring rows. The one-row commitment is one VALU kernel;
+# its roofline counts int32 VALU issue slots per (ring element, column, CRT prime): the
+# transform's 3,564 modular products at 5 slots each and the key row's 648 multiply-adds at 2 slots
+# (one half-rate 32x32->64 multiply-add). With 16 rows the transform (the same 3,564 x 5 slots) is
+# the larger kernel; the key-row products run on the matrix cores and are priced in bytes: the
+# transformed witness (4 bytes per slot, column and element) and the key's digits (4 bytes per
+# slot, row and element) read once per CRT prime.
 Q_C3 = 661_473_542_593
 C3_COLUMNS, C3_ELEMENTS = 64, 4096
 
 
 def c3_slots(rows: int) -> int:
     return 3564 * 5 + rows * 648 * 2
+
+
+def c3_product_bytes(rows: int, primes: int) -> int:
+    return primes * C3_ELEMENTS * 648 * 4 * (C3_COLUMNS + 16)
 
 
 def bench_seed(i: int) -> bytes:
@@ -203,15 +210,24 @@ def commit_micro(m, steps, warmup):
         finally:
             c.close()
         ms = t["total_ms"] / t["runs"]
-        slots = C3_COLUMNS * C3_ELEMENTS * primes * c3_slots(rows)
-        achieved = slots / (ms / 1e3) / 1e12
-        out[f"rows{rows}"] = {
-            "rows": rows, "primes": primes, "ms": round(ms, 4),
-            "kernels_ms": {k.replace("_ms", ""): round(t[k] / t["runs"], 4)
-                           for k in ("transform_ms", "mac_ms", "sum_ms", "finish_ms")},
-            "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": round(VALU_SLOTS_T, 3),
-                         "unit": "T int32 slots/s", "frac": round(achieved / VALU_SLOTS_T, 4)},
-        }
+        kms = {k.replace("_ms", ""): t[k] / t["runs"] for k in ("transform_ms", "mac_ms", "sum_ms", "finish_ms")}
+        rec = {"rows": rows, "primes": primes, "ms": round(ms, 4),
+               "kernels_ms": {k: round(v, 4) for k, v in kms.items()}}
+        if rows == 1:
+            # one kernel: transform and MAC fused
+            achieved = C3_COLUMNS * C3_ELEMENTS * primes * c3_slots(1) / (ms / 1e3) / 1e12
+            rec["roofline"] = {"bound": "valu", "kernel": "k_commit_mac (+ slice sum, finish)",
+                               "achieved": round(achieved, 3), "peak": round(VALU_SLOTS_T, 3),
+                               "unit": "T int32 slots/s", "frac": round(achieved / VALU_SLOTS_T, 4)}
+        else:
+            achieved = C3_COLUMNS * C3_ELEMENTS * primes * 3564 * 5 / (kms["transform"] / 1e3) / 1e12
+            rec["roofline"] = {"bound": "valu", "kernel": "k_commit_v (transform)", "achieved": round(achieved, 3),
+                               "peak": round(VALU_SLOTS_T, 3), "unit": "T int32 slots/s",
+                               "frac": round(achieved / VALU_SLOTS_T, 4)}
+            gbs = c3_product_bytes(rows, primes) / (kms["mac"] / 1e3) / 1e9
+            rec["products_roofline"] = {"bound": "hbm", "kernel": "k_commit_gemm_mfma", "achieved": round(gbs, 1),
+                                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4)}
+        out[f"rows{rows}"] = rec
     return out
 
 
@@ -227,6 +243,8 @@ def main():
     ap.add_argument("--commit-micro-only", action="store_true",
                     help="run only the commit micro-benchmark (prints its own line)")
     ap.add_argument("--workers", type=int, default=8, help="proofs in flight for the batched / serving runs")
+    ap.add_argument("--hw-queues", type=int, default=16,
+                    help="GPU_MAX_HW_QUEUES for this process (hardware queues the proofs' streams map onto)")
     ap.add_argument("--serving", action="store_true", help="run the serving loop instead (prints its own line)")
     ap.add_argument("--requests", type=int, default=10_000)
     ap.add_argument("--rate", type=float, default=400.0, help="serving arrivals per second")
@@ -241,6 +259,9 @@ def main():
         print("bench.py measures one GPU: the prover's kernels run on one device per proof; "
               "a multi-GPU sharding is described in DESIGN.md but not built (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
+    if not 1 <= args.hw_queues <= 32:
+        ap.error("--hw-queues must be between 1 and 32")
+    os.environ["GPU_MAX_HW_QUEUES"] = str(args.hw_queues)
     import amd
     m = amd.load()
     if args.serving:

@@ -303,7 +303,7 @@ __global__ __launch_bounds__(256) void k_eq_expand(const uint4 *EH, const uint4 
 // partials v[k] = sum_j bit_k(w_j) eq_hi[j] (crossfield.rs:152-165).
 // ---------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_partials(const uint4 *table, const uint4 *E0, size_t n, uint32_t *out) {
-    __shared__ uint32_t lds[4 * M4W<4>::WORDS];
+    __shared__ __align__(16) uint32_t lds[4 * M4W<4>::WORDS];
     uint32_t *W = lds + (threadIdx.x >> 6) * M4W<4>::WORDS;
     uint32_t acc[2][4] = {};
     for (size_t base = (size_t)blockIdx.x * 256; base < n; base += (size_t)gridDim.x * 256) {
@@ -324,7 +324,7 @@ __global__ __launch_bounds__(256) void k_partials(const uint4 *table, const uint
 // ---------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_switch_msg0(const uint4 *table, const uint4 *E0, size_t h, uint4 *E1,
                                                      uint32_t *T0, uint32_t *Tinf) {
-    __shared__ uint32_t lds[4 * M4W<4>::WORDS];
+    __shared__ __align__(16) uint32_t lds[4 * M4W<4>::WORDS];
     uint32_t *W = lds + (threadIdx.x >> 6) * M4W<4>::WORDS;
     uint32_t acc0[2][4] = {}, acci[2][4] = {};
     for (size_t base = (size_t)blockIdx.x * 256; base < h; base += (size_t)gridDim.x * 256) {
@@ -432,7 +432,7 @@ __global__ __launch_bounds__(256) void k_switch_round(const uint32_t *pin, size_
     if (blockIdx.x == 0)
         for (int i = threadIdx.x; i < 1536; i += blockDim.x) clear[i] = 0;
     __shared__ uint32_t rtab[RHO_WINDOWS * 16 * 6];
-    __shared__ uint32_t lds[4 * M4W<6>::WORDS];  // also holds the basis while rtab is built
+    __shared__ __align__(16) uint32_t lds[4 * M4W<6>::WORDS];  // also holds the basis while rtab is built
     if (rho_dev) rho = RhoArg{{rho_dev[0], rho_dev[1], rho_dev[2]}};
     build_rho_table(rho, lds, rtab);
     uint32_t *W = lds + (threadIdx.x >> 6) * M4W<6>::WORDS;
@@ -915,7 +915,7 @@ __global__ void k_eq_row(const uint64_t *lo, const uint64_t *hi, int lo_bits, si
 }
 
 __global__ __launch_bounds__(256) void k_row(const uint4 *table, const uint32_t *eqrow, size_t clen, uint32_t *U) {
-    __shared__ uint32_t lds[4 * M4W<6>::WORDS];
+    __shared__ __align__(16) uint32_t lds[4 * M4W<6>::WORDS];
     uint32_t *W = lds + (threadIdx.x >> 6) * M4W<6>::WORDS;
     uint32_t acc[2][6] = {};
     const size_t c = blockIdx.x;
@@ -1576,9 +1576,9 @@ __global__ __launch_bounds__(64) void k_commit_finish(const uint32_t *cacc, cons
 // each transformed witness value meets R key values, a product of a (rows x elements) key by an
 // (elements x columns) witness per slot, which needs R x columns sums per slot. So the transform
 // and the products are split: k_commit_v stores the transformed witness of a chunk of ring
-// elements (a buffer small enough to stay in the last-level cache), k_commit_gemm forms the
-// products per slot with register tiles of 4 rows x 8 columns, k_commit_rowsum adds the slices and
-// transposes them into the finish kernel's per-output layout.
+// elements (a buffer small enough to stay in the last-level cache), k_commit_gemm_mfma forms the
+// products per slot on the matrix cores (int8 digits of the residues, exact int32 sums),
+// k_commit_rowsum adds the slices and transposes them into the finish kernel's per-output layout.
 // ---------------------------------------------------------------------------------------------
 
 // Forward transform of the binary ring elements [m0, m1) of TC columns, no MAC: each value is
@@ -1640,69 +1640,169 @@ __global__ __launch_bounds__(MAC_THREADS, 2) void k_commit_v(const uint32_t *tab
     }
 }
 
-// Key-row products, for keys of up to 16 rows (ahat16[m][slot][16], rows past the key's zero).
-// Block (x, y, z): slots 8x .. 8x+7, ring elements [y m_per, (y+1) m_per) of the chunk, columns
-// 64z .. 64z+63:   out[slot][r][c] = sum_m ahat16[m][slot][r] * V[m][slot][c]  (mod p)
-// into slice y, laid out [slot][16][ncols]. Thread t: slot 8x + t / 32, rows 4 (t & 3) .. +3,
-// columns 64z + 8 ((t >> 2) & 7) .. +7. The 32 sums are 64-bit multiply-adds of canonical
-// residues folded every 16 elements as in the MAC: after a fold a sum is below 2^32 (p + 1), and
-// 16 products below p^2 < 2^58.9 keep it below 2^64.
-constexpr int G_SLOTS = 8, G_MB = 16, G_THREADS = G_SLOTS * 32, G_VROW = 68;
-static_assert(648 % G_SLOTS == 0, "slot groups must tile the 648 slots");
-__global__ __launch_bounds__(G_THREADS) void k_commit_gemm(const uint32_t *ahat16, const uint32_t *V, int ncols,
-                                                           int mcount, int m_per, PC pc, uint32_t c32, uint32_t r2,
-                                                           uint32_t *out) {
-    __shared__ uint4 Al[G_MB * G_SLOTS * 4];          // [mm][slot][16 rows]
-    __shared__ uint32_t Vl[G_MB * G_SLOTS * G_VROW];  // [mm][slot][64 columns + 4 pad]
-    const int t = threadIdx.x;
-    const int js = t >> 5, rg = t & 3, cg = (t >> 2) & 7;
-    const int s0 = blockIdx.x * G_SLOTS, cb = blockIdx.z * 64;
+// Key-row products on the matrix cores (v_mfma_i32_16x16x64_i8). A residue x < p < 2^30 is
+// four balanced signed bytes: y = x + 0x808080 and digit i = byte i of y ^ 0x00808080 (bytes 0..2
+// less 128, byte 3 below 65), so x = sum_i digit_i 256^i. The product of two residues is then
+// sum_{i,j} 256^(i+j) (digit_i a)(digit_j v): sixteen 8-bit products per pair, summed exactly in
+// seven i32 classes s = i + j (|class| <= 4 * 128 * 128 * 1024 = 2^26 over at most 1024
+// elements per block) and recombined mod p at the end. MFMA lane l holds 16 bytes of its row (A: key row
+// l % 16) or column (B: witness column l % 16); byte e of lane l is ring element 16 (l / 16) + e of
+// the 64-element step, the same for A and B. The result of tile n: lane l holds rows
+// 4 (l / 16) .. +3 of column 16 n + l % 16.
+__device__ __forceinline__ uint32_t bal_digits(uint32_t x) { return (x + 0x808080u) ^ 0x808080u; }
+
+// Four words w0..w3 (four digits each) to four words d0..d3, d_i = digit i of w0..w3 (a 4 x 4 byte
+// transpose).
+__device__ __forceinline__ void byte_transpose4(uint32_t w0, uint32_t w1, uint32_t w2, uint32_t w3, uint32_t &d0,
+                                                uint32_t &d1, uint32_t &d2, uint32_t &d3) {
+    const uint32_t lo01 = __builtin_amdgcn_perm(w1, w0, 0x05010400u), hi01 = __builtin_amdgcn_perm(w1, w0, 0x07030602u);
+    const uint32_t lo23 = __builtin_amdgcn_perm(w3, w2, 0x05010400u), hi23 = __builtin_amdgcn_perm(w3, w2, 0x07030602u);
+    d0 = __builtin_amdgcn_perm(lo23, lo01, 0x05040100u);
+    d1 = __builtin_amdgcn_perm(lo23, lo01, 0x07060302u);
+    d2 = __builtin_amdgcn_perm(hi23, hi01, 0x05040100u);
+    d3 = __builtin_amdgcn_perm(hi23, hi01, 0x07060302u);
+}
+
+// The key's MFMA operands (set-up): amf[slot][64-element block][digit][lane] (16 bytes each) from
+// ahat16[m][slot][16 rows]; elements past `elements` are zero.
+__global__ __launch_bounds__(64) void k_ahat_mfma(const uint32_t *ahat16, int elements, int nmblk, uint4 *amf) {
+    const int s = blockIdx.x, mb = blockIdx.y, l = threadIdx.x;
+    const int r = l & 15, kb = l >> 4;
+    uint32_t w[16];
+#pragma unroll
+    for (int e = 0; e < 16; e++) {
+        const int m = mb * 64 + 16 * kb + e;
+        w[e] = bal_digits(m < elements ? ahat16[((size_t)m * 648 + s) * 16 + r] : 0u);
+    }
+    uint32_t d[4][4];
+#pragma unroll
+    for (int g = 0; g < 4; g++) byte_transpose4(w[4 * g], w[4 * g + 1], w[4 * g + 2], w[4 * g + 3], d[0][g], d[1][g], d[2][g], d[3][g]);
+    uint4 *dst = amf + ((size_t)s * nmblk + mb) * 4 * 64;
+#pragma unroll
+    for (int i = 0; i < 4; i++) dst[i * 64 + l] = make_uint4(d[i][0], d[i][1], d[i][2], d[i][3]);
+}
+
+// One wave per block: slot blockIdx.x, ring elements [y m_per, (y + 1) m_per) of the chunk (m_per a
+// multiple of 64, at most 1024), columns 16 NT z .. 16 NT z + 16 NT - 1 (NT column tiles of 16);
+// the block's sums mod p go to slice y ([slot][16][ncols]), stored or, with `accumulate`, added to
+// what the slice holds (the chunks of one commitment share the slices). mblk0: the chunk's first
+// 64-element block in the key's operand layout. A step's witness tile (64 elements x 16 NT
+// columns of the slot) is loaded into registers one step ahead, written to LDS at the top of its
+// step, and read back a column per lane (16 elements each) for the MFMA operands. A step may run
+// past the block's range only at the end of the key (the chunk then ends with the key): those
+// rows of V are allocated and hold stale values, and the key's digits there are zero, so they
+// add nothing.
+struct MfWeights {
+    uint32_t w[7];  // 256^c R mod p: the digit classes' weights in Montgomery form
+};
+constexpr int MF_RSRC3 = 0x00020000;  // buffer resource word 3: 32-bit data format, raw access
+typedef int mfma_v4i __attribute__((ext_vector_type(4)));
+typedef unsigned int mf_v4u __attribute__((ext_vector_type(4)));
+template <int NT>
+__global__ __launch_bounds__(64) void k_commit_gemm_mfma(const uint4 *amf, int nmblk, int mblk0, const uint32_t *V,
+                                                         int ncols, int mcount, int m_per, PC pc, MfWeights wt,
+                                                         int accumulate, uint32_t *out) {
+    constexpr int W = 16 * NT;           // columns per block
+    constexpr int ROW = W + 1;           // LDS row of the tile (one word of padding spreads the column reads)
+    constexpr int QL = W / 4;            // 16-byte column groups per row
+    constexpr int RPL = 64 / QL;         // rows per load instruction
+    constexpr int NQ = 64 / RPL;         // load instructions per step
+    __shared__ uint32_t Vt[64 * ROW];
+    const int l = threadIdx.x, s = blockIdx.x, cb = blockIdx.z * W;
+    const int kb = l >> 4, cl = l & 15;
     const int m0 = blockIdx.y * m_per, m1 = min(mcount, m0 + m_per);
-    uint64_t acc[4][8];
+    const size_t rs = (size_t)648 * ncols;  // V's stride between ring elements
+    // lane l loads row l / QL + RPL q (q < NQ) at columns 4 (l % QL) .. +3 of each step. The loads
+    // are buffer loads over a per-step base (offsets below 64 * 648 * 4096 * 4 < 2^31): as
+    // intrinsics they stay where they are written, one step ahead, instead of being merged with
+    // the prologue's loads into one load at the top of the loop.
+    const int lr = l / QL, lc = l % QL;
+    const uint32_t lane_v = (uint32_t)((lr * rs + 4 * lc) * 4), lane_a = (uint32_t)l * 16;
+    const char *vb = (const char *)(V + (size_t)s * ncols + cb);
+    const char *ab = (const char *)(amf + ((size_t)s * nmblk + mblk0) * 256);
+    auto load_v = [&](int mk, mf_v4u(&x)[NQ]) {
+        const __amdgpu_buffer_rsrc_t r =
+            __builtin_amdgcn_make_buffer_rsrc((void *)(vb + (size_t)mk * rs * 4), (short)0, 0x7fffffff, MF_RSRC3);
 #pragma unroll
-    for (int r = 0; r < 4; r++)
+        for (int q = 0; q < NQ; q++)
+            x[q] = __builtin_amdgcn_raw_buffer_load_b128(r, lane_v, (uint32_t)(RPL * q * rs * 4), 0);
+    };
+    auto load_a = [&](int mk, mf_v4u(&a)[4]) {
+        const __amdgpu_buffer_rsrc_t r =
+            __builtin_amdgcn_make_buffer_rsrc((void *)(ab + (size_t)(mk / 64) * 4096), (short)0, 4096, MF_RSRC3);
 #pragma unroll
-        for (int c = 0; c < 8; c++) acc[r][c] = 0;
-    for (int mb = m0; mb < m1; mb += G_MB) {
-        const int nm = min(G_MB, m1 - mb);
-        __syncthreads();  // the previous tiles are consumed
-        for (int i = t; i < nm * G_SLOTS * 4; i += G_THREADS) {
-            const int mm = i / (G_SLOTS * 4), q = i % (G_SLOTS * 4);
-            Al[i] = ((const uint4 *)(ahat16 + ((size_t)(mb + mm) * 648 + s0) * 16))[q];
-        }
-        for (int i = t; i < nm * G_SLOTS * 16; i += G_THREADS) {
-            const int row = i >> 4, q = i & 15;  // row = mm * G_SLOTS + slot
-            const int mm = row / G_SLOTS, j = row % G_SLOTS;
-            *(uint4 *)(Vl + row * G_VROW + 4 * q) =
-                ((const uint4 *)(V + ((size_t)(mb + mm) * 648 + s0 + j) * ncols + cb))[q];
+        for (int i = 0; i < 4; i++) a[i] = __builtin_amdgcn_raw_buffer_load_b128(r, lane_a + i * 1024, 0, 0);
+    };
+    mfma_v4i acc[7][NT];
+#pragma unroll
+    for (int c = 0; c < 7; c++)
+#pragma unroll
+        for (int n = 0; n < NT; n++) acc[c][n] = mfma_v4i{0, 0, 0, 0};
+    // past the range the prefetch repeats the last step (cache hits, unused)
+    const int last = m0 + (m1 - m0 - 1) / 64 * 64;
+    mf_v4u vx[NQ], an[4];  // the next step's operands, in flight during a step
+    load_v(m0, vx);
+    load_a(m0, an);
+    for (int mk = m0; mk < m1; mk += 64) {
+        mfma_v4i av[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) av[i] = __builtin_bit_cast(mfma_v4i, an[i]);
+        __syncthreads();  // the previous step's reads of the tile are done
+#pragma unroll
+        for (int q = 0; q < NQ; q++) {
+            uint32_t *d = Vt + (lr + RPL * q) * ROW + 4 * lc;
+            d[0] = vx[q][0], d[1] = vx[q][1], d[2] = vx[q][2], d[3] = vx[q][3];
         }
         __syncthreads();
-        for (int mm = 0; mm < nm; mm++) {
-            const uint4 a = Al[(mm * G_SLOTS + js) * 4 + rg];
-            const uint32_t *vr = Vl + (mm * G_SLOTS + js) * G_VROW + cg * 8;
-            const uint4 v0 = *(const uint4 *)vr, v1 = *(const uint4 *)(vr + 4);
-            const uint32_t av[4] = {a.x, a.y, a.z, a.w};
-            const uint32_t vv[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+        const int mn = min(mk + 64, last);
+        load_v(mn, vx);
+        load_a(mn, an);
+        __builtin_amdgcn_sched_barrier(0);  // all of the next step's loads issue before the products
 #pragma unroll
-            for (int r = 0; r < 4; r++)
+        for (int n = 0; n < NT; n++) {
+            uint32_t w[16];
 #pragma unroll
-                for (int c = 0; c < 8; c++) acc[r][c] += (uint64_t)av[r] * vv[c];
+            for (int e = 0; e < 16; e++) w[e] = bal_digits(Vt[(16 * kb + e) * ROW + 16 * n + cl]);
+            uint32_t d[4][4];
+#pragma unroll
+            for (int g = 0; g < 4; g++)
+                byte_transpose4(w[4 * g], w[4 * g + 1], w[4 * g + 2], w[4 * g + 3], d[0][g], d[1][g], d[2][g], d[3][g]);
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const mfma_v4i bv = mfma_v4i{(int)d[j][0], (int)d[j][1], (int)d[j][2], (int)d[j][3]};
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+                    acc[i + j][n] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[i], bv, acc[i + j][n], 0, 0, 0);
+            }
         }
-#pragma unroll
-        for (int r = 0; r < 4; r++)
-#pragma unroll
-            for (int c = 0; c < 8; c++) acc[r][c] = (uint64_t)(uint32_t)(acc[r][c] >> 32) * c32 + (uint32_t)acc[r][c];
     }
+    // recombine: sum_c 256^c class_c mod p, as two Montgomery reductions of four and three
+    // products below p^2 each (4 p^2 < 2^32 p)
+    uint32_t *dst = out + (size_t)blockIdx.y * 648 * 16 * ncols + ((size_t)s * 16 + 4 * kb) * ncols + cb + cl;
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-        uint32_t red[8];
+    for (int n = 0; n < NT; n++)
 #pragma unroll
-        for (int c = 0; c < 8; c++) red[c] = mont(redc64(acc[r][c], pc), r2, pc);
-        uint4 *dst = (uint4 *)(out + (size_t)blockIdx.y * 648 * 16 * ncols +
-                               ((size_t)(s0 + js) * 16 + 4 * rg + r) * ncols + cb + 8 * cg);
-        dst[0] = make_uint4(red[0], red[1], red[2], red[3]);
-        dst[1] = make_uint4(red[4], red[5], red[6], red[7]);
-    }
+        for (int i = 0; i < 4; i++) {
+            uint64_t lo = 0, hi = 0;
+#pragma unroll
+            for (int c = 0; c < 7; c++) {
+                const int32_t v = acc[c][n][i];  // |v| <= 2^26 < p
+                const uint64_t t = (uint64_t)(v < 0 ? (uint32_t)(v + (int32_t)pc.p) : (uint32_t)v) * wt.w[c];
+                if (c < 4)
+                    lo += t;
+                else
+                    hi += t;
+            }
+            uint32_t r = redc64(lo, pc) + redc64(hi, pc);  // < 4p
+            r = min(r, r - 2 * pc.p);
+            r = min(r, r - pc.p);
+            uint32_t *o = dst + (size_t)i * ncols + 16 * n;
+            if (accumulate) {
+                r += *o;
+                r = min(r, r - pc.p);
+            }
+            *o = r;
+        }
 }
 
 // The key-row products' slices added and transposed for the finish kernel:
@@ -1872,10 +1972,19 @@ struct Wave {
     uint32_t cv[8], bw[16];  // chunk chaining value and the open block
     uint32_t priv[2][8];     // stack entries above the shared ones
     uint32_t rcv[8], rm[16]; // root node
-    uint32_t out[16];        // current output block
+    uint32_t rlen, rflags;   // its length and flags
+    uint8_t pos[32];         // the chosen attempt's positions and signs
+    uint32_t signs;
+};
+
+// One speculative attempt per quad: attempt g read from byte 56 g of the stream, which is where
+// it starts when every earlier attempt drew its 28 two-byte values without a rejected draw.
+constexpr int SPEC = 16;
+struct Spec {
+    uint32_t out[16];  // current output block of the stream
     uint8_t perm[164];
     uint8_t pos[32];
-    uint32_t signs, fail;
+    uint32_t signs, used;  // sign bits; bytes the attempt consumed
 };
 
 struct Reader {
@@ -1883,8 +1992,8 @@ struct Reader {
     const uint32_t (*stack)[8];  // shared, in LDS
     const uint32_t *key;
     uint32_t flags, buf_len, rlen, rflags;
-    int blocks, sd, np, pos;
-    uint64_t chunk, block;
+    int blocks, sd, np;
+    uint64_t chunk;
 
     __device__ void chunk_cv(uint32_t *o) const {
         compress(w->cv, w->bw, chunk, buf_len, flags | END | (blocks == 0 ? START : 0), o, false);
@@ -1954,23 +2063,13 @@ struct Reader {
             rlen = 64;
             rflags = flags | PARENT;
         }
-        block = 0;
-        pos = 64;
-    }
-    __device__ uint32_t next() {
-        if (pos == 64) {
-            compress(w->rcv, w->rm, block++, rlen, rflags | ROOT, w->out, true);
-            pos = 0;
-        }
-        const uint32_t b = (w->out[pos >> 2] >> (8 * (pos & 3))) & 0xFF;
-        pos++;
-        return b;
     }
 };
 }  // namespace chal
 
 __global__ __launch_bounds__(64 * chal::WAVES) void k_challenges(const ChalArg *arg, lbn_challenge *out) {
     __shared__ chal::Wave waves[chal::WAVES];
+    __shared__ chal::Spec spec[chal::WAVES][chal::SPEC];
     __shared__ uint32_t s_stack[54][8];
     __shared__ uint32_t s_key[8];
     __shared__ double s_cos[243], s_sin[243];
@@ -2006,47 +2105,18 @@ __global__ __launch_bounds__(64 * chal::WAVES) void k_challenges(const ChalArg *
         const char label[16] = "short-challenge";
         for (int k = 0; k < 15; k++) r.byte((uint32_t)(uint8_t)label[k]);
         r.finish();
+        w->rlen = r.rlen, w->rflags = r.rflags;
     }
     const double bound_sq = arg->bound_sq;
-    for (;;) {
-        if (lane < 4) {
-            // partial Fisher-Yates over 162 positions, two-byte draws below a multiple of the range
-            for (int k = 0; k < 162; k++) w->perm[k] = (uint8_t)k;
-            uint64_t seen[3] = {0, 0, 0};
-            for (int k = 0; k < WEIGHT; k++) {
-                const uint32_t n = 162 - k, limit = (65536u / n) * n;
-                uint32_t v;
-                for (;;) {
-                    const uint32_t lo = r.next(), hi = r.next();
-                    v = lo | (hi << 8);
-                    if (v < limit) break;
-                }
-                const int j = k + (int)(v % n);
-                const uint8_t p = w->perm[j];
-                w->perm[j] = w->perm[k];
-                seen[p >> 6] |= 1ull << (p & 63);
-            }
-            int n = 0;
-            for (int ww = 0; ww < 3; ww++)
-                for (uint64_t b = seen[ww]; b; b &= b - 1) w->pos[n++] = (uint8_t)(64 * ww + __builtin_ctzll(b));
-            // signs: the sign hasher over u64(WEIGHT) || positions, first 4 output bytes
-            uint32_t m[16], o[8];
-            for (int k = 0; k < 16; k++) m[k] = 0;
-            m[0] = WEIGHT;
-            for (int k = 0; k < WEIGHT; k++) m[2 + (k >> 2)] |= (uint32_t)w->pos[k] << (8 * (k & 3));
-            chal::compress(arg->sign_key, m, 0, 8 + WEIGHT, arg->sign_flags | chal::START | chal::END | chal::ROOT, o, false);
-            w->signs = o[0] & ((1u << WEIGHT) - 1);
-        }
-        __syncwarp();
-        // canonical embedding norm: unit k of Z/243 per lane (k = lane, lane + 64), the same
-        // additions in the same order as the host
-        const uint32_t signs = w->signs;
+    // canonical embedding norm of (pos, signs): unit k of Z/243 per lane (k = lane, lane + 64), the
+    // same additions in the same order as the host; true (on every lane) if a unit exceeds the bound
+    auto too_long = [&](const uint8_t *pos, uint32_t signs) {
         bool bad = false;
         for (int k = lane; k < 81; k += 64) {
             const int u = s_units[k];
             double re = 0.0, im = 0.0;
             for (int tt = 0; tt < WEIGHT; tt++) {
-                const int a = ((int)w->pos[tt] * u) % 243;
+                const int a = ((int)pos[tt] * u) % 243;
                 const bool neg = (signs >> tt) & 1;
                 re = re + (neg ? -s_cos[a] : s_cos[a]);
                 im = im + (neg ? -s_sin[a] : s_sin[a]);
@@ -2054,9 +2124,91 @@ __global__ __launch_bounds__(64 * chal::WAVES) void k_challenges(const ChalArg *
             const double a2 = re * re, b2 = im * im;
             if (a2 + b2 > bound_sq) bad = true;
         }
-        const bool reject = __ballot(bad) != 0;
+        return __ballot(bad) != 0;
+    };
+    // positions and signs of one attempt from a byte source: partial Fisher-Yates over 162
+    // positions, two-byte draws below a multiple of the range; then the sign hasher over
+    // u64(WEIGHT) || positions, first 4 output bytes. Run by the four lanes of one quad.
+    auto attempt = [&](auto &&next, uint8_t *perm, uint8_t *pos, uint32_t &signs) {
+        for (int k = 0; k < 162; k++) perm[k] = (uint8_t)k;
+        uint64_t seen[3] = {0, 0, 0};
+        for (int k = 0; k < WEIGHT; k++) {
+            const uint32_t n = 162 - k, limit = (65536u / n) * n;
+            uint32_t v;
+            for (;;) {
+                const uint32_t lo = next(), hi = next();
+                v = lo | (hi << 8);
+                if (v < limit) break;
+            }
+            const int j = k + (int)(v % n);
+            const uint8_t pj = perm[j];
+            perm[j] = perm[k];
+            seen[pj >> 6] |= 1ull << (pj & 63);
+        }
+        int n = 0;
+        for (int ww = 0; ww < 3; ww++)
+            for (uint64_t b = seen[ww]; b; b &= b - 1) pos[n++] = (uint8_t)(64 * ww + __builtin_ctzll(b));
+        uint32_t m[16], o[8];
+        for (int k = 0; k < 16; k++) m[k] = 0;
+        m[0] = WEIGHT;
+        for (int k = 0; k < WEIGHT; k++) m[2 + (k >> 2)] |= (uint32_t)pos[k] << (8 * (k & 3));
+        chal::compress(arg->sign_key, m, 0, 8 + WEIGHT, arg->sign_flags | chal::START | chal::END | chal::ROOT, o, false);
+        signs = o[0] & ((1u << WEIGHT) - 1);
+    };
+    // The attempts in parallel, one per quad, then the first that passes: a rejection (about a
+    // third of the attempts) no longer costs a whole attempt's latency. Quad h runs the attempt
+    // that starts at byte base + 56 h, its start when the attempts before it consumed 56 bytes
+    // each (no rejected draw); scanning in order, an attempt that consumed more ends the round and
+    // the next round starts right after it.
+    __syncwarp();  // the root node is in LDS
+    uint64_t base = 0;
+    for (;;) {
+        {
+            chal::Spec *sq = &spec[wv][lane >> 2];
+            const uint64_t start = base + 56u * (uint32_t)(lane >> 2);
+            uint64_t blk = start >> 6;
+            uint32_t pp = (uint32_t)(start & 63), used = 0;
+            chal::compress(w->rcv, w->rm, blk, w->rlen, w->rflags | chal::ROOT, sq->out, true);
+            auto next = [&]() -> uint32_t {
+                if (pp == 64) {
+                    chal::compress(w->rcv, w->rm, ++blk, w->rlen, w->rflags | chal::ROOT, sq->out, true);
+                    pp = 0;
+                }
+                const uint32_t b = (sq->out[pp >> 2] >> (8 * (pp & 3))) & 0xFF;
+                pp++;
+                used++;
+                return b;
+            };
+            uint32_t sg;
+            attempt(next, sq->perm, sq->pos, sg);
+            sq->signs = sg;
+            sq->used = used;
+        }
         __syncwarp();
-        if (!reject) break;
+        int chosen = -1;
+        uint64_t nbase = base + 56u * chal::SPEC;
+        for (int g = 0; g < chal::SPEC; g++) {
+            const chal::Spec *sq = &spec[wv][g];
+            if (!too_long(sq->pos, sq->signs)) {
+                chosen = g;
+                break;
+            }
+            if (sq->used != 56) {
+                nbase = base + 56u * g + sq->used;
+                break;
+            }
+        }
+        if (chosen >= 0) {
+            if (lane < 4) {
+                const chal::Spec *sq = &spec[wv][chosen];
+                for (int k = 0; k < 32; k++) w->pos[k] = sq->pos[k];
+                w->signs = sq->signs;
+            }
+            __syncwarp();
+            break;
+        }
+        __syncwarp();  // every lane has read the round's attempts before they are overwritten
+        base = nbase;
     }
     if (lane < 8) {  // positions (32 bytes, zero past WEIGHT), signs, weight
         uint32_t v;
@@ -2979,7 +3131,8 @@ struct lbn_cmt {
     hipStream_t stream = nullptr;
     uint4 *table = nullptr;
     uint32_t *tabs = nullptr;  // MAX_PRIMES * tab::SIZE
-    uint32_t *ahat = nullptr;  // rows == 1: [k][m][648]; rows > 1: [k][m][648][16]
+    uint32_t *ahat = nullptr;  // rows == 1: [k][m][648] (rows > 1: [k][m][648][16] during set-up)
+    uint4 *amf = nullptr;      // rows > 1: the key's MFMA operands [k][slot][64-element block][digit][lane]
     uint32_t *V = nullptr;     // rows > 1: transformed witness of one chunk, [m][648][columns]
     uint32_t *cacc = nullptr;  // MAC slices [k][slice][column][648], or product slices [k][slice][648][16][columns]
     uint32_t *crow = nullptr;  // summed slices [k][row * columns + column][648]
@@ -2987,7 +3140,7 @@ struct lbn_cmt {
     PrimeInfo primes[MAX_PRIMES] = {};
     Roots18 roots[MAX_PRIMES] = {};
     int splits = 0, m_per = 0;                 // rows == 1: MAC slices
-    int chunk = 0, gsplit = 0, nslices = 0;    // rows > 1: elements per chunk, product slices per chunk, total
+    int chunk = 0, nmblk = 0, nslices = 0;     // rows > 1: elements per chunk, 64-element blocks, slices
     bool loaded = false, timing = false;
     std::vector<CmtEv> evs;
     size_t ev_used = 0;
@@ -3001,6 +3154,10 @@ struct lbn_cmt {
 namespace {
 constexpr int CMT_CHUNK = 1024;  // ring elements per transformed chunk (columns x 2.6 KB each)
 constexpr int CMT_VM_PER = 4;    // ring elements per block of k_commit_v
+constexpr int CMT_MF_PER = 1024;  // ring elements of a chunk per block of k_commit_gemm_mfma (a whole chunk)
+constexpr int CMT_MF_NT = 2;      // its column tiles of 16 (columns are a multiple of 64)
+static_assert(CMT_CHUNK % 64 == 0 && CMT_MF_PER % 64 == 0 && CMT_MF_PER <= 1024,
+              "the product kernel steps over whole 64-element blocks, at most 1024 per block");
 enum CmtKind { CK_TOTAL = 0, CK_TRANSFORM, CK_MAC, CK_SUM, CK_FINISH };
 
 int cmt_begin(lbn_cmt *c, int kind) {
@@ -3019,8 +3176,8 @@ void cmt_end(lbn_cmt *c, int i) {
     if (i >= 0) hipEventRecord(c->evs[(size_t)i].b, c->stream);
 }
 void cmt_free(lbn_cmt *c) {
-    for (void *p : {(void *)c->table, (void *)c->tabs, (void *)c->ahat, (void *)c->V, (void *)c->cacc,
-                    (void *)c->crow, (void *)c->out})
+    for (void *p : {(void *)c->table, (void *)c->tabs, (void *)c->ahat, (void *)c->amf, (void *)c->V,
+                    (void *)c->cacc, (void *)c->crow, (void *)c->out})
         if (p) hipFree(p);
     for (CmtEv &e : c->evs) {
         hipEventDestroy(e.a);
@@ -3087,15 +3244,24 @@ extern "C" int lbn_cmt_create(int device, int columns, int elements, int rows, u
             HIPCHK(c, hipMalloc(&c->cacc, (size_t)np * c->splits * columns * N * 4));
             HIPCHK(c, hipMalloc(&c->crow, (size_t)np * columns * N * 4));
         } else {
+            // the key rows' digits in the matrix cores' operand order; the chunks start on 64-element
+            // blocks (CMT_CHUNK is a multiple of 64)
+            c->nmblk = (elements + 63) / 64;
+            const size_t per = (size_t)N * c->nmblk * 4 * 64;
+            HIPCHK(c, hipMalloc(&c->amf, (size_t)np * per * 16));
+            for (int k = 0; k < np; k++) {
+                k_ahat_mfma<<<dim3(N, c->nmblk), 64, 0, c->stream>>>(c->ahat + (size_t)k * re * N * 16, elements,
+                                                                      c->nmblk, c->amf + (size_t)k * per);
+                HIPCHK(c, hipGetLastError());
+            }
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            HIPCHK(c, hipFree(c->ahat));
+            c->ahat = nullptr;
             c->chunk = std::min(elements, CMT_CHUNK);
-            const int nchunks = (elements + c->chunk - 1) / c->chunk;
-            c->gsplit = std::max(1, std::min(8, MAX_SLICES / nchunks));
-            const int gm_per = (c->chunk + c->gsplit - 1) / c->gsplit;
-            c->nslices = 0;
-            for (int mb = 0; mb < elements; mb += c->chunk)
-                c->nslices += (std::min(c->chunk, elements - mb) + gm_per - 1) / gm_per;
-            if (c->nslices > MAX_SLICES) return c->fail(LBN_E_ARG, "too many product slices");
-            HIPCHK(c, hipMalloc(&c->V, (size_t)c->chunk * N * columns * 4));
+            c->nslices = (c->chunk + CMT_MF_PER - 1) / CMT_MF_PER;
+            // whole 64-element steps of the product kernel stay inside V
+            HIPCHK(c, hipMalloc(&c->V, (size_t)((c->chunk + 63) / 64 * 64) * N * columns * 4));
+            HIPCHK(c, hipMemsetAsync(c->V, 0, (size_t)((c->chunk + 63) / 64 * 64) * N * columns * 4, c->stream));
             HIPCHK(c, hipMalloc(&c->cacc, (size_t)np * c->nslices * N * 16 * columns * 4));
             HIPCHK(c, hipMalloc(&c->crow, (size_t)np * rows * columns * N * 4));
         }
@@ -3186,13 +3352,13 @@ extern "C" int lbn_cmt_launch(lbn_cmt *c) {
         cmt_end(c, es);
         HIPCHK(c, hipGetLastError());
     } else {
-        const int gm_per = (c->chunk + c->gsplit - 1) / c->gsplit;
         const size_t slice_words = (size_t)N * 16 * cols;
+        const size_t per = (size_t)N * c->nmblk * 4 * 64;
         for (int k = 0; k < np; k++) {
             const PC pc{c->primes[k].p, c->primes[k].pinv_neg};
-            const uint32_t c32 = (uint32_t)((1ull << 32) % pc.p);
+            MfWeights wt;
+            for (uint64_t c8 = 0, x = 1; c8 < 7; c8++, x = (x << 8) % pc.p) wt.w[c8] = (uint32_t)((x << 32) % pc.p);
             uint32_t *slices = c->cacc + (size_t)k * c->nslices * slice_words;
-            int sl = 0;
             for (int mb = 0; mb < re; mb += c->chunk) {
                 const int mc = std::min(c->chunk, re - mb);
                 int e = cmt_begin(c, CK_TRANSFORM);
@@ -3201,18 +3367,16 @@ extern "C" int lbn_cmt_launch(lbn_cmt *c) {
                     c->roots[k], c->V, cols);
                 cmt_end(c, e);
                 HIPCHK(c, hipGetLastError());
-                const int gy = (mc + gm_per - 1) / gm_per;
+                const int gy = (mc + CMT_MF_PER - 1) / CMT_MF_PER;
                 e = cmt_begin(c, CK_MAC);
-                k_commit_gemm<<<dim3(N / G_SLOTS, gy, cols / 64), G_THREADS, 0, c->stream>>>(
-                    c->ahat + ((size_t)k * re + mb) * N * 16, c->V, cols, mc, gm_per, pc, c32, c->primes[k].r2,
-                    slices + (size_t)sl * slice_words);
+                k_commit_gemm_mfma<CMT_MF_NT><<<dim3(N, gy, cols / (16 * CMT_MF_NT)), 64, 0, c->stream>>>(
+                    c->amf + (size_t)k * per, c->nmblk, mb / 64, c->V, cols, mc, CMT_MF_PER, pc, wt, mb > 0, slices);
                 cmt_end(c, e);
                 HIPCHK(c, hipGetLastError());
-                sl += gy;
             }
             const int e = cmt_begin(c, CK_SUM);
             k_commit_rowsum<<<dim3((N + 15) / 16, c->rows * cols / 64), 256, 0, c->stream>>>(
-                slices, sl, cols, pc, c->primes[k].r2, c->crow + (size_t)k * c->rows * cols * N);
+                slices, c->nslices, cols, pc, c->primes[k].r2, c->crow + (size_t)k * c->rows * cols * N);
             cmt_end(c, e);
             HIPCHK(c, hipGetLastError());
         }

@@ -100,7 +100,7 @@ __device__ inline void f162_mul_slow(const uint64_t a[3], const uint64_t b[3], u
 template <int V>
 struct M4W {
     static_assert(V == 4 || V == 6, "M4RM values are 4 or 6 words");
-    static constexpr int WORDS = 256 * V;  // LDS words per wave
+    static constexpr int WORDS = 256 * V;  // LDS words per wave (16-byte aligned: read as 16-byte rows)
 };
 
 template <int CTRL>

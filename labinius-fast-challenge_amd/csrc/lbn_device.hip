@@ -306,12 +306,25 @@ __global__ __launch_bounds__(256) void k_partials(const uint4 *table, const uint
     __shared__ __align__(16) uint32_t lds[4 * M4W<4>::WORDS];
     uint32_t *W = lds + (threadIdx.x >> 6) * M4W<4>::WORDS;
     uint32_t acc[2][4] = {};
-    for (size_t base = (size_t)blockIdx.x * 256; base < n; base += (size_t)gridDim.x * 256) {
+    // two items per pass (n is a multiple of 256): both items' loads are in flight before the first
+    // step, so the second step's operands arrive during the first
+    const size_t stride = (size_t)gridDim.x * 256;
+    for (size_t base = (size_t)blockIdx.x * 256; base < n; base += 2 * stride) {
         const size_t j = base + threadIdx.x;
-        uint4 w = table[j], e = E0[j];
-        uint32_t sel[4] = {w.x, w.y, w.z, w.w};
-        uint32_t val[4] = {e.x, e.y, e.z, e.w};
-        m4w_step<4>(acc, sel, val, W);
+        const bool two = base + stride < n;
+        const uint4 w = table[j], e = E0[j];
+        uint4 w2 = make_uint4(0, 0, 0, 0), e2 = make_uint4(0, 0, 0, 0);
+        if (two) w2 = table[j + stride], e2 = E0[j + stride];
+        {
+            uint32_t sel[4] = {w.x, w.y, w.z, w.w};
+            uint32_t val[4] = {e.x, e.y, e.z, e.w};
+            m4w_step<4>(acc, sel, val, W);
+        }
+        if (two) {
+            uint32_t sel[4] = {w2.x, w2.y, w2.z, w2.w};
+            uint32_t val[4] = {e2.x, e2.y, e2.z, e2.w};
+            m4w_step<4>(acc, sel, val, W);
+        }
     }
     m4w_flush<4, true>(acc, lds, out);
 }
@@ -327,20 +340,36 @@ __global__ __launch_bounds__(256) void k_switch_msg0(const uint4 *table, const u
     __shared__ __align__(16) uint32_t lds[4 * M4W<4>::WORDS];
     uint32_t *W = lds + (threadIdx.x >> 6) * M4W<4>::WORDS;
     uint32_t acc0[2][4] = {}, acci[2][4] = {};
-    for (size_t base = (size_t)blockIdx.x * 256; base < h; base += (size_t)gridDim.x * 256) {
-        const size_t j = base + threadIdx.x;
-        uint4 ea = E0[j], eb = E0[j + h], wa = table[j], wb = table[j + h];
-        uint4 e1 = make_uint4(ea.x ^ eb.x, ea.y ^ eb.y, ea.z ^ eb.z, ea.w ^ eb.w);
-        E1[j] = e1;
-        {
-            uint32_t sel[4] = {ea.x, ea.y, ea.z, ea.w};
-            uint32_t val[4] = {wa.x, wa.y, wa.z, wa.w};
-            m4w_step<4>(acc0, sel, val, W);
+    // two items per pass (h is a multiple of 256), both items' loads issued first
+    const size_t stride = (size_t)gridDim.x * 256;
+    for (size_t base = (size_t)blockIdx.x * 256; base < h; base += 2 * stride) {
+        const bool two = base + stride < h;
+        uint4 ea[2], eb[2], wa[2], wb[2];
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            const size_t j = base + u * stride + threadIdx.x;
+            if (u == 0 || two) {
+                ea[u] = E0[j], eb[u] = E0[j + h], wa[u] = table[j], wb[u] = table[j + h];
+            } else {
+                ea[u] = eb[u] = wa[u] = wb[u] = make_uint4(0, 0, 0, 0);
+            }
         }
-        {
-            uint32_t sel[4] = {e1.x, e1.y, e1.z, e1.w};
-            uint32_t val[4] = {wa.x ^ wb.x, wa.y ^ wb.y, wa.z ^ wb.z, wa.w ^ wb.w};
-            m4w_step<4>(acci, sel, val, W);
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            if (u == 1 && !two) break;
+            const size_t j = base + u * stride + threadIdx.x;
+            const uint4 e1 = make_uint4(ea[u].x ^ eb[u].x, ea[u].y ^ eb[u].y, ea[u].z ^ eb[u].z, ea[u].w ^ eb[u].w);
+            E1[j] = e1;
+            {
+                uint32_t sel[4] = {ea[u].x, ea[u].y, ea[u].z, ea[u].w};
+                uint32_t val[4] = {wa[u].x, wa[u].y, wa[u].z, wa[u].w};
+                m4w_step<4>(acc0, sel, val, W);
+            }
+            {
+                uint32_t sel[4] = {e1.x, e1.y, e1.z, e1.w};
+                uint32_t val[4] = {wa[u].x ^ wb[u].x, wa[u].y ^ wb[u].y, wa[u].z ^ wb[u].z, wa[u].w ^ wb[u].w};
+                m4w_step<4>(acci, sel, val, W);
+            }
         }
     }
     m4w_flush<4, true>(acc0, lds, T0);
@@ -426,43 +455,52 @@ __device__ __forceinline__ void rho_mul(const uint32_t *tab, const uint32_t (&y)
 // Block 0 also clears `clear` (1536 words), the message buffer of the round after next.
 // rho_dev, when not null, holds the challenge (written by the device transcript) instead of rho.
 template <bool WORDS>
-__global__ __launch_bounds__(256) void k_switch_round(const uint32_t *pin, size_t h2, RhoArg rho, const uint4 *Ein,
-                                                      uint4 *Eout, uint32_t *pout, uint32_t *T0, uint32_t *Tinf,
+__global__ __launch_bounds__(256) void k_switch_round(const uint32_t *__restrict__ pin, size_t h2, RhoArg rho,
+                                                      const uint4 *__restrict__ Ein, uint4 *__restrict__ Eout,
+                                                      uint32_t *__restrict__ pout, uint32_t *T0, uint32_t *Tinf,
                                                       uint32_t *clear, const uint64_t *rho_dev) {
     if (blockIdx.x == 0)
         for (int i = threadIdx.x; i < 1536; i += blockDim.x) clear[i] = 0;
     __shared__ uint32_t rtab[RHO_WINDOWS * 16 * 6];
     __shared__ __align__(16) uint32_t lds[4 * M4W<6>::WORDS];  // also holds the basis while rtab is built
+    const size_t h = 2 * h2;
+    const size_t nthreads = (h2 + 255) / 256 * 256, stride = (size_t)gridDim.x * 256;
+    // one item's operands: p_j, p_{j+h2}, p_{j+h}, p_{j+h+h2} and E_j, E_{j+h2} (zero past h2)
+    uint32_t a[4][6];
+    uint4 ea, eb;
+    auto load_item = [&](size_t j) {
+        const bool live = j < h2;
+        const size_t jj = live ? j : 0;
+        const size_t idx[4] = {jj, jj + h2, jj + h, jj + h + h2};
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            if (WORDS) {
+                const uint4 w = ((const uint4 *)pin)[idx[u]];
+                a[u][0] = w.x, a[u][1] = w.y, a[u][2] = w.z, a[u][3] = w.w, a[u][4] = 0, a[u][5] = 0;
+            } else {
+                const uint2 *src = (const uint2 *)(pin + 6 * idx[u]);
+                const uint2 x0 = src[0], x1 = src[1], x2 = src[2];
+                a[u][0] = x0.x, a[u][1] = x0.y, a[u][2] = x1.x, a[u][3] = x1.y, a[u][4] = x2.x, a[u][5] = x2.y;
+            }
+        }
+        ea = Ein[jj], eb = Ein[jj + h2];
+    };
+    // the first item's loads are in flight while the challenge's tables are built
+    size_t base = (size_t)blockIdx.x * 256;
+    if (base < nthreads) load_item(base + threadIdx.x);
     if (rho_dev) rho = RhoArg{{rho_dev[0], rho_dev[1], rho_dev[2]}};
     build_rho_table(rho, lds, rtab);
     uint32_t *W = lds + (threadIdx.x >> 6) * M4W<6>::WORDS;
     uint32_t acc0[2][6] = {}, acci[2][6] = {};
-    const size_t h = 2 * h2;
-    const size_t nthreads = (h2 + 255) / 256 * 256;
-    for (size_t base = (size_t)blockIdx.x * 256; base < nthreads; base += (size_t)gridDim.x * 256) {
+    for (; base < nthreads; base += stride) {
         const size_t j = base + threadIdx.x;
         const bool live = j < h2;
         uint32_t q0[6] = {0, 0, 0, 0, 0, 0}, q1[6] = {0, 0, 0, 0, 0, 0};
         uint32_t e[4] = {0, 0, 0, 0}, e2[4] = {0, 0, 0, 0};
         if (live) {
-            uint32_t a0[6], a1[6], b0[6], b1[6];
-            const size_t idx[4] = {j, j + h2, j + h, j + h + h2};
-            uint32_t *dst[4] = {a0, a1, b0, b1};
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-                if (WORDS) {
-                    uint4 w = ((const uint4 *)pin)[idx[u]];
-                    dst[u][0] = w.x, dst[u][1] = w.y, dst[u][2] = w.z, dst[u][3] = w.w, dst[u][4] = 0, dst[u][5] = 0;
-                } else {
-                    const uint2 *s = (const uint2 *)(pin + 6 * idx[u]);
-                    uint2 x0 = s[0], x1 = s[1], x2 = s[2];
-                    dst[u][0] = x0.x, dst[u][1] = x0.y, dst[u][2] = x1.x, dst[u][3] = x1.y, dst[u][4] = x2.x,
-                    dst[u][5] = x2.y;
-                }
-            }
             uint32_t d0[6], d1[6], m0[6], m1[6];
 #pragma unroll
-            for (int k = 0; k < 6; k++) d0[k] = a0[k] ^ b0[k], d1[k] = a1[k] ^ b1[k];
+            for (int k = 0; k < 6; k++) d0[k] = a[0][k] ^ a[2][k], d1[k] = a[1][k] ^ a[3][k];
             if (WORDS) {
                 rho_mul<32>(rtab, d0, m0);
                 rho_mul<32>(rtab, d1, m1);
@@ -471,15 +509,15 @@ __global__ __launch_bounds__(256) void k_switch_round(const uint32_t *pin, size_
                 rho_mul<RHO_WINDOWS>(rtab, d1, m1);
             }
 #pragma unroll
-            for (int k = 0; k < 6; k++) q0[k] = a0[k] ^ m0[k], q1[k] = a1[k] ^ m1[k];
+            for (int k = 0; k < 6; k++) q0[k] = a[0][k] ^ m0[k], q1[k] = a[1][k] ^ m1[k];
             uint2 *o0 = (uint2 *)(pout + 6 * j), *o1 = (uint2 *)(pout + 6 * (j + h2));
             o0[0] = make_uint2(q0[0], q0[1]), o0[1] = make_uint2(q0[2], q0[3]), o0[2] = make_uint2(q0[4], q0[5]);
             o1[0] = make_uint2(q1[0], q1[1]), o1[1] = make_uint2(q1[2], q1[3]), o1[2] = make_uint2(q1[4], q1[5]);
-            uint4 ea = Ein[j], eb = Ein[j + h2];
             e[0] = ea.x, e[1] = ea.y, e[2] = ea.z, e[3] = ea.w;
             e2[0] = ea.x ^ eb.x, e2[1] = ea.y ^ eb.y, e2[2] = ea.z ^ eb.z, e2[3] = ea.w ^ eb.w;
             Eout[j] = make_uint4(e2[0], e2[1], e2[2], e2[3]);
         }
+        if (base + stride < nthreads) load_item(base + stride + threadIdx.x);  // the next item's operands
         m4w_step<6>(acc0, e, q0, W);
         uint32_t dq[6];
 #pragma unroll
@@ -685,30 +723,42 @@ __device__ __forceinline__ u128 b128_mulx_pow(u128 v, int k) {
     return lo ^ t ^ o ^ (o << 1) ^ (o << 2) ^ (o << 7);
 }
 
-__global__ __launch_bounds__(1024) void k_switch_ctl(SwitchDev *sd, int r, int lp, int V, const uint32_t *T,
-                                                     const uint32_t *pair) {
-    __shared__ B3State st;
-    __shared__ B3State cl;
-    __shared__ uint64_t L[128][3];
-    __shared__ uint64_t part[32][3];
-    __shared__ uint64_t tabL[32][16][3];
-    __shared__ uint64_t Tt[2][192][2];
-    __shared__ uint64_t rbas[162][3];
-    __shared__ uint64_t tabR41[41][16][3];
-    __shared__ uint64_t lpart[4][128][3];
-    __shared__ uint64_t prod[2][3];
-    __shared__ uint64_t rho_s[3];
+// The control step's LDS: the transcript, the map, the challenge and its window tables, and the
+// step's scratch tables.
+struct CtlKeep {
+    B3State st, cl;
+    uint64_t L[128][3];
+    uint64_t tabR41[41][16][3];  // y -> rho y over 4-bit windows
+    uint64_t prod[2][3];
+    uint64_t rho_s[3];
+};
+struct CtlTemp {
+    uint64_t part[32][3];
+    uint64_t tabL[32][16][3];
+    uint64_t Tt[2][192][2];
+    uint64_t rbas[162][3];
+    uint64_t lpart[4][128][3];
+};
+
+// One control step for round r (at least 768 threads), on the transcript K.st and map K.L in
+// LDS; the message comes from T, the round's records and the challenge go to sd.
+__device__ __forceinline__ void switch_ctl_step(CtlKeep &K, CtlTemp &X, SwitchDev *sd, int r, int lp, int V, const uint32_t *T,
+                                const uint32_t *pair) {
+    auto &st = K.st;
+    auto &cl = K.cl;
+    auto &L = K.L;
+    auto &tabR41 = K.tabR41;
+    auto &prod = K.prod;
+    auto &rho_s = K.rho_s;
+    auto &part = X.part;
+    auto &tabL = X.tabL;
+    auto &Tt = X.Tt;
+    auto &rbas = X.rbas;
+    auto &lpart = X.lpart;
     const int t = threadIdx.x, q = t & 3;
-    {
-        const uint32_t *src = (const uint32_t *)&sd->ts;
-        uint32_t *dst = (uint32_t *)&st;
-        for (int i = t; i < (int)(sizeof(B3State) / 4); i += 1024) dst[i] = src[i];
-        if (t < 384) L[t / 3][t % 3] = sd->lmap[t / 3][t % 3];
-    }
-    __syncthreads();
     // 4-bit tables of L: tabL[w][j] = sum of L[4w + i] over the set bits i of j (also used for the
     // map update below), and the message sums staged in LDS
-    for (int e = t; e < 32 * 16; e += 1024) {
+    for (int e = t; e < 32 * 16; e += blockDim.x) {
         const int w = e >> 4, nib = e & 15;
         uint64_t v[3] = {0, 0, 0};
         for (int b = 0; b < 4; b++)
@@ -719,9 +769,9 @@ __global__ __launch_bounds__(1024) void k_switch_ctl(SwitchDev *sd, int r, int l
     // Bit-transpose of the message sums by wave ballots: Tt[kind][b][half] holds bit b of T_k for
     // k = 64 half + lane. Wave 2 word + half of kind 0 (waves 0..11), then kind 1.
     {
-        const int wave = t >> 6, lane = t & 63;
-        if (wave < 2 * V) {
-            const int word = wave >> 1, half = wave & 1, k = 64 * half + lane;
+        const int lane = t & 63;
+        for (int task = t >> 6; task < 2 * V; task += blockDim.x >> 6) {
+            const int word = task >> 1, half = task & 1, k = 64 * half + lane;
             for (int kind = 0; kind < 2; kind++) {
                 const uint32_t x = T[768 * kind + k * V + word];
                 for (int bit = 0; bit < 32; bit++) {
@@ -801,7 +851,7 @@ __global__ __launch_bounds__(1024) void k_switch_ctl(SwitchDev *sd, int r, int l
     {  // sample_f162("switch-round", 1): a reader at the counter, then counter += 1
         const uint32_t *src = (const uint32_t *)&st;
         uint32_t *dst = (uint32_t *)&cl;
-        for (int i = t; i < (int)(sizeof(B3State) / 4); i += 1024) dst[i] = src[i];
+        for (int i = t; i < (int)(sizeof(B3State) / 4); i += blockDim.x) dst[i] = src[i];
     }
     __syncthreads();
     if (t < 4) {
@@ -854,8 +904,8 @@ __global__ __launch_bounds__(1024) void k_switch_ctl(SwitchDev *sd, int r, int l
             rbas[t][0] = y[0], rbas[t][1] = y[1], rbas[t][2] = y[2];
         }
         __syncthreads();
-        if (t < 41 * 16) {
-            const int w = t >> 4, nib = t & 15;
+        for (int e = t; e < 41 * 16; e += blockDim.x) {
+            const int w = e >> 4, nib = e & 15;
             uint64_t v[3] = {0, 0, 0};
             for (int b = 0; b < 4; b++)
                 if (((nib >> b) & 1) && 4 * w + b < 162)
@@ -887,15 +937,35 @@ __global__ __launch_bounds__(1024) void k_switch_ctl(SwitchDev *sd, int r, int l
         __syncthreads();
         if (t < 384) {
             const int k = t / 3, j = t % 3;
-            sd->lmap[k][j] = lpart[0][k][j] ^ lpart[1][k][j] ^ lpart[2][k][j] ^ lpart[3][k][j];
+            L[k][j] = lpart[0][k][j] ^ lpart[1][k][j] ^ lpart[2][k][j] ^ lpart[3][k][j];
         }
     }
     __syncthreads();
-    {
-        const uint32_t *src = (const uint32_t *)&st;
-        uint32_t *dst = (uint32_t *)&sd->ts;
-        for (int i = t; i < (int)(sizeof(B3State) / 4); i += 1024) dst[i] = src[i];
-    }
+}
+
+__device__ __forceinline__ void switch_state_load(CtlKeep &K, const SwitchDev *sd) {
+    const int t = threadIdx.x;
+    const uint32_t *src = (const uint32_t *)&sd->ts;
+    uint32_t *dst = (uint32_t *)&K.st;
+    for (int i = t; i < (int)(sizeof(B3State) / 4); i += blockDim.x) dst[i] = src[i];
+    for (int i = t; i < 384; i += blockDim.x) K.L[i / 3][i % 3] = sd->lmap[i / 3][i % 3];
+    __syncthreads();
+}
+__device__ __forceinline__ void switch_state_store(const CtlKeep &K, SwitchDev *sd) {
+    const int t = threadIdx.x;
+    const uint32_t *src = (const uint32_t *)&K.st;
+    uint32_t *dst = (uint32_t *)&sd->ts;
+    for (int i = t; i < (int)(sizeof(B3State) / 4); i += blockDim.x) dst[i] = src[i];
+    for (int i = t; i < 384; i += blockDim.x) sd->lmap[i / 3][i % 3] = K.L[i / 3][i % 3];
+}
+
+__global__ __launch_bounds__(1024) void k_switch_ctl(SwitchDev *sd, int r, int lp, int V, const uint32_t *T,
+                                                     const uint32_t *pair) {
+    __shared__ CtlKeep K;
+    __shared__ CtlTemp X;
+    switch_state_load(K, sd);
+    switch_ctl_step(K, X, sd, r, lp, V, T, pair);
+    switch_state_store(K, sd);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -919,16 +989,25 @@ __global__ __launch_bounds__(256) void k_row(const uint4 *table, const uint32_t 
     uint32_t *W = lds + (threadIdx.x >> 6) * M4W<6>::WORDS;
     uint32_t acc[2][6] = {};
     const size_t c = blockIdx.x;
-    for (size_t base = 0; base < clen; base += 256) {
-        const size_t i = base + threadIdx.x;
-        uint32_t sel[4] = {0, 0, 0, 0}, val[6] = {0, 0, 0, 0, 0, 0};
-        if (i < clen) {
-            uint4 w = table[c * clen + i];
-            sel[0] = w.x, sel[1] = w.y, sel[2] = w.z, sel[3] = w.w;
+    // two items per pass, both items' loads issued first
+    for (size_t base = 0; base < clen; base += 512) {
+        uint32_t sel[2][4], val[2][6];
 #pragma unroll
-            for (int k = 0; k < 6; k++) val[k] = eqrow[6 * i + k];
+        for (int u = 0; u < 2; u++) {
+            const size_t i = base + 256 * u + threadIdx.x;
+#pragma unroll
+            for (int k = 0; k < 4; k++) sel[u][k] = 0;
+#pragma unroll
+            for (int k = 0; k < 6; k++) val[u][k] = 0;
+            if (i < clen) {
+                const uint4 w = table[c * clen + i];
+                sel[u][0] = w.x, sel[u][1] = w.y, sel[u][2] = w.z, sel[u][3] = w.w;
+#pragma unroll
+                for (int k = 0; k < 6; k++) val[u][k] = eqrow[6 * i + k];
+            }
         }
-        m4w_step<6>(acc, sel, val, W);
+        m4w_step<6>(acc, sel[0], val[0], W);
+        if (base + 256 < clen) m4w_step<6>(acc, sel[1], val[1], W);
     }
     m4w_flush<6, false>(acc, lds, U + c * 128 * 6);
 }

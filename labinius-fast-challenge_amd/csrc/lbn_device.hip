@@ -84,7 +84,7 @@ namespace mb {
 constexpr size_t EH = 0;                  // 2^(log_packed - 12) B128 <= 64 KiB
 constexpr size_t EL = EH + 65536;         // 4096 B128
 constexpr size_t EQ = EL + 65536;         // row eq tables: 128 + 128 F162
-constexpr size_t FOFF = EQ + 8192;        // fold list offsets (325 int)
+constexpr size_t FOFF = EQ + 8192;        // fold list offsets (325 int) and position order (162 int)
 constexpr size_t FCOLS = FOFF + 2048;     // fold column lists (<= 1024 x 32 u16)
 constexpr size_t T = FCOLS + 65536;       // switch message sums (1536 u32)
 constexpr size_t PART = T + 6144;         // partials (512 u32)
@@ -1076,11 +1076,27 @@ __device__ __forceinline__ void ctr_add4(uint32_t (&c)[NB], uint32_t x0, uint32_
     ctr_ripple<NB>(c, k3, 2);
 }
 
+// Eight words into the counter, Harley-Seal style: planes 0..2 act as the ones / twos / fours of
+// a carry-save tree (seven full adders), and only the eights ripple into the planes above.
+template <int NB>
+__device__ __forceinline__ void ctr_add8(uint32_t (&c)[NB], const uint32_t (&x)[8]) {
+    uint32_t tA, tB, fA, fB, e8;
+    full_add(c[0], x[0], x[1], c[0], tA);
+    full_add(c[0], x[2], x[3], c[0], tB);
+    full_add(c[1], tA, tB, c[1], fA);
+    full_add(c[0], x[4], x[5], c[0], tA);
+    full_add(c[0], x[6], x[7], c[0], tB);
+    full_add(c[1], tA, tB, c[1], fB);
+    full_add(c[2], fA, fB, c[2], e8);
+    ctr_ripple<NB>(c, e8, 3);
+}
+
 // One block folds ring elements m = blockIdx.x, blockIdx.x + gridDim.x, ...: the column lists
 // (28 x 1024 u16) are staged in LDS once, the 1024 columns' 64-byte element m is staged per m
 // (column-major, 16 u32 per column), and task (pos, k) counts the four words of word k of every
 // listed column with four NB-plane counters, so each list entry costs one u16 and one 16-byte LDS
-// read.
+// read. Tasks take the positions in the host's order (longest list first), so the 16 positions of
+// a wave have lists of similar length.
 constexpr int FOLD_THREADS = 704;  // 11 waves for the 648 (pos, k) tasks
 constexpr int FOLD_MAX_ENTRIES = 1024 * 32;
 
@@ -1090,11 +1106,11 @@ __global__ __launch_bounds__(FOLD_THREADS) void k_fold(const uint4 *table, size_
                                                        uint32_t *flag) {
     __shared__ uint4 s_w[1024 * 4];
     __shared__ uint16_t s_cols[FOLD_MAX_ENTRIES];
-    __shared__ int s_off[2 * 162 + 1];
+    __shared__ int s_off[2 * 162 + 1 + 162];  // list offsets, then the positions in task order
     __shared__ int32_t s_raw[4 * 292];
     const int tid = threadIdx.x;
     for (int i = tid; i < n_entries; i += FOLD_THREADS) s_cols[i] = cols[i];
-    for (int i = tid; i < 2 * 162 + 1; i += FOLD_THREADS) s_off[i] = off[i];
+    for (int i = tid; i < 2 * 162 + 1 + 162; i += FOLD_THREADS) s_off[i] = off[i];
     for (size_t m = blockIdx.x; m < re; m += gridDim.x) {
         for (int idx = tid; idx < 1024 * 4; idx += FOLD_THREADS) {
             const int j = idx >> 2, k = idx & 3;
@@ -1103,13 +1119,31 @@ __global__ __launch_bounds__(FOLD_THREADS) void k_fold(const uint4 *table, size_
         for (int i = tid; i < 4 * 292; i += FOLD_THREADS) s_raw[i] = 0;
         __syncthreads();
         if (tid < 162 * 4) {
-            const int pos = tid >> 2, k = tid & 3;
+            const int pos = s_off[2 * 162 + 1 + (tid >> 2)], k = tid & 3;
             uint32_t c0[NB], c1[NB], c2[NB], c3[NB];
 #pragma unroll
             for (int i = 0; i < NB; i++) c0[i] = c1[i] = c2[i] = c3[i] = 0;
             const int e0 = s_off[2 * pos], e1 = s_off[2 * pos + 1], e2 = s_off[2 * pos + 2];
-            // positives then complemented negatives, four list entries at a time
+            // positives then complemented negatives, eight list entries at a time, then four
             int e = e0;
+            for (; e + 8 <= e2; e += 8) {
+                int ci[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) ci[u] = s_cols[e + u];
+                uint4 w[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) w[u] = s_w[ci[u] * 4 + k];
+                uint32_t x[8], y[8], z[8], v[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    const uint32_t neg = e + u >= e1 ? ~0u : 0u;
+                    x[u] = w[u].x ^ neg, y[u] = w[u].y ^ neg, z[u] = w[u].z ^ neg, v[u] = w[u].w ^ neg;
+                }
+                ctr_add8<NB>(c0, x);
+                ctr_add8<NB>(c1, y);
+                ctr_add8<NB>(c2, z);
+                ctr_add8<NB>(c3, v);
+            }
             for (; e + 4 <= e2; e += 4) {
                 // the four list entries' indices, then their words, then the complement of the
                 // negatives as a mask (no branch: lanes' sign boundaries differ)
@@ -2337,7 +2371,7 @@ int lbn::dev_init(lbn_ctx *ctx) {
     HIPCHK(ctx, hipMalloc(&d->rowU, (size_t)COLUMNS * 128 * 24));
     HIPCHK(ctx, hipMalloc(&d->rowOut, (size_t)COLUMNS * 24));
     HIPCHK(ctx, hipMalloc(&d->fold_flag, 4));
-    HIPCHK(ctx, hipMalloc(&d->fold_off, (2 * 162 + 1) * sizeof(int)));
+    HIPCHK(ctx, hipMalloc(&d->fold_off, (2 * 162 + 1 + 162) * sizeof(int)));
     HIPCHK(ctx, hipMalloc(&d->fold_cols, (size_t)COLUMNS * 32 * sizeof(uint16_t)));
     HIPCHK(ctx, hipMalloc(&d->ntt_tabs, (size_t)MAX_PRIMES * tab::SIZE * 4));
     HIPCHK(ctx, hipHostMalloc((void **)&d->hfold, s.ring_elements * N * sizeof(int32_t)));
@@ -3107,6 +3141,12 @@ extern "C" int lbn_fold(lbn_ctx *ctx, const lbn_challenge *ch, int n_challenges,
         }
     int maxlen = 0;  // one counter per position holds positives and complemented negatives
     for (int pos = 0; pos < 162; pos++) maxlen = std::max(maxlen, cnt[2 * pos] + cnt[2 * pos + 1]);
+    // positions by list length, longest first: a wave's 16 positions then walk lists of similar
+    // length (a wave runs as long as its longest list)
+    for (int pos = 0; pos < 162; pos++) off.push_back(pos);
+    std::stable_sort(off.begin() + 2 * 162 + 1, off.end(), [&](int a, int b) {
+        return cnt[2 * a] + cnt[2 * a + 1] > cnt[2 * b] + cnt[2 * b + 1];
+    });
     int nb = 1;
     while ((1 << nb) <= maxlen) nb++;
     HIPCHK(ctx, hipSetDevice(ctx->device));
@@ -3121,6 +3161,7 @@ extern "C" int lbn_fold(lbn_ctx *ctx, const lbn_challenge *ch, int n_challenges,
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess) cus = 256;
     const unsigned grid = (unsigned)std::min<size_t>(s.ring_elements, (size_t)cus);
     const int n_entries = off[2 * 162];
+    static_assert((2 * 162 + 1 + 162) * sizeof(int) <= mb::FCOLS - mb::FOFF, "fold offsets region");
     if (n_entries > FOLD_MAX_ENTRIES) return ctx->fail(LBN_E_ARG, "challenge lists too long");
     // into the caller's buffer when it is one of ours (pinned), else into the context's staging
     int32_t *direct = ((uintptr_t)out & 3) ? nullptr : (int32_t *)pinned_alias(out, n_out * 4);

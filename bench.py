@@ -114,7 +114,10 @@ def batched_instances(n=64, first=100):
 
 
 def warm(sch, ks, workers):
-    """Context set-up outside the clock: `workers` contexts of each size, created concurrently."""
+    """Context set-up outside the clock: `workers` contexts of each size, then a round of proofs
+    (first launches of every kernel)."""
+    for kk in ks:
+        sch.prewarm(kk, workers)
     hs = [sch.submit(kk, bench_seed(999_000 + 100 * kk + j)) for kk in ks for j in range(workers)]
     sch.drain()
     for h in hs:

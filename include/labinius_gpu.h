@@ -121,6 +121,12 @@ typedef struct {
     int switch_rounds;
 } lbn_timing;
 int lbn_set_timing(lbn_ctx *ctx, int enable);
+
+/* Graph replay of the device-driven switch (default on): the 2 log_packed - 1 kernel launches of
+ * the rounds are captured as a HIP graph on a context's first proof and replayed as one graph
+ * launch by every later proof; disabled, they are launched one by one. The proof bytes are the
+ * same either way. No counterpart in the reference. */
+int lbn_set_graphs(lbn_ctx *ctx, int enable);
 int lbn_reset_timing(lbn_ctx *ctx);
 int lbn_get_timing(const lbn_ctx *ctx, lbn_timing *out);
 
@@ -171,6 +177,11 @@ int lbn_sched_drain(lbn_sched *s);
 int lbn_sched_get_stats(lbn_sched *s, lbn_sched_stats *out);
 /* Drains, stops the workers and frees every context. */
 void lbn_sched_destroy(lbn_sched *s);
+/* Creates idle contexts of 2^log_vars evaluations (modulus set) until the scheduler holds count
+ * of that size, on the calling thread, so that later jobs of that size find a context ready (a
+ * context created for a job costs milliseconds and synchronises the device). LBN_E_RANGE, with
+ * none created, when the budget cannot hold them. No counterpart in the reference. */
+int lbn_sched_prewarm(lbn_sched *s, int log_vars, int count);
 
 /* ----------------------------------------------------------------------------------------------
  * Standalone commitment (the commit kernels on their own, for keys of one or several ring rows).

@@ -91,6 +91,7 @@ def lib():
         L.lbn_prove.argtypes = [P, P, P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t), P]
         L.lbn_commit_primes.argtypes = [P]
         L.lbn_set_timing.argtypes = [P, ctypes.c_int]
+        L.lbn_set_graphs.argtypes = [P, ctypes.c_int]
         L.lbn_reset_timing.argtypes = [P]
         L.lbn_get_timing.argtypes = [P, P]
         L.lbn_ctx_footprint.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64]
@@ -103,6 +104,7 @@ def lib():
         L.lbn_sched_get_stats.argtypes = [P, P]
         L.lbn_sched_destroy.argtypes = [P]
         L.lbn_sched_destroy.restype = None
+        L.lbn_sched_prewarm.argtypes = [P, ctypes.c_int, ctypes.c_int]
         L.lbn_host_alloc.argtypes = [ctypes.c_size_t]
         L.lbn_host_alloc.restype = ctypes.c_void_p
         L.lbn_host_free.argtypes = [P]
@@ -255,6 +257,10 @@ class Prover:
         self._check(lib().lbn_set_timing(self.h, 1 if enable else 0))
         self._check(lib().lbn_reset_timing(self.h))
 
+    def set_graphs(self, enable=True):
+        """Replay the switch rounds as one captured HIP graph (the default) or launch them one by one."""
+        self._check(lib().lbn_set_graphs(self.h, 1 if enable else 0))
+
     def timing(self) -> dict:
         t = Timing()
         self._check(lib().lbn_get_timing(self.h, ctypes.addressof(t)))
@@ -321,6 +327,12 @@ class Scheduler:
         if rc != 0:
             raise LbnError(rc, "proof job failed")
         return keep[-1][: job.len].tobytes(), job.queue_us, job.run_us
+
+    def prewarm(self, log_vars: int, count: int):
+        """Idle contexts of this size until the scheduler holds `count` of them (created now)."""
+        rc = lib().lbn_sched_prewarm(self.h, log_vars, count)
+        if rc != 0:
+            raise LbnError(rc, "lbn_sched_prewarm failed")
 
     def stats(self) -> dict:
         st = SchedStats()

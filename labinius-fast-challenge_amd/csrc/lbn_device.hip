@@ -74,7 +74,14 @@ struct lbn::DeviceState {
     hipEvent_t ev_commit = nullptr, ev_staged = nullptr;
     hipEvent_t ev[2 * 9] = {};           // start/stop per timed pass
     Roots18 roots[MAX_PRIMES] = {};
+    // The switch rounds' launches (the same for every proof of this context: shapes, buffers
+    // and the device state they read are fixed) captured once and replayed as one graph launch.
+    hipGraphExec_t switch_graph = nullptr;
+    bool graphs = true;
+    hipStream_t own_stream = nullptr, own_copy = nullptr;  // created here; stream / copy_stream may be a caller's
 };
+
+static int capture_switch(lbn_ctx *ctx);
 
 // Regions of a context's pinned mailbox. Every small host<->device transfer goes through it, so
 // each is a plain DMA on the context's stream (pageable copies take the runtime's shared staging
@@ -2356,7 +2363,8 @@ int lbn::dev_init(lbn_ctx *ctx) {
     auto *d = new DeviceState();
     ctx->dev = d;
     HIPCHK(ctx, hipSetDevice(ctx->device));
-    HIPCHK(ctx, hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
+    HIPCHK(ctx, hipStreamCreateWithFlags(&d->own_stream, hipStreamNonBlocking));
+    d->stream = d->own_stream;
     const Shape &s = ctx->shape;
     HIPCHK(ctx, hipMalloc(&d->table, s.n_words * 16));
     HIPCHK(ctx, hipMalloc(&d->E[0], s.n_words * 16));
@@ -2380,11 +2388,18 @@ int lbn::dev_init(lbn_ctx *ctx) {
     HIPCHK(ctx, hipMalloc(&d->chal, sizeof(ChalArg)));
     HIPCHK(ctx, hipMalloc(&d->chals, COLUMNS * sizeof(lbn_challenge)));
     HIPCHK(ctx, hipHostGetDevicePointer((void **)&d->fold_host, d->hfold, 0));
-    HIPCHK(ctx, hipStreamCreateWithFlags(&d->copy_stream, hipStreamNonBlocking));
+    HIPCHK(ctx, hipStreamCreateWithFlags(&d->own_copy, hipStreamNonBlocking));
+    d->copy_stream = d->own_copy;
     HIPCHK(ctx, hipEventCreateWithFlags(&d->ev_commit, hipEventDisableTiming));
     HIPCHK(ctx, hipEventCreateWithFlags(&d->ev_staged, hipEventDisableTiming));
     for (auto &e : d->ev) HIPCHK(ctx, hipEventCreate(&e));
-    return LBN_OK;
+    return capture_switch(ctx);
+}
+
+void lbn::dev_use_streams(lbn_ctx *ctx, void *main, void *copy) {
+    DeviceState *d = ctx->dev;
+    d->stream = main ? (hipStream_t)main : d->own_stream;
+    d->copy_stream = copy ? (hipStream_t)copy : d->own_copy;
 }
 
 void lbn::dev_free(lbn_ctx *ctx) {
@@ -2394,6 +2409,7 @@ void lbn::dev_free(lbn_ctx *ctx) {
     void *ptrs[] = {d->table, d->E[0], d->E[1], d->P[0], d->P[1], d->T, d->EH, d->EL, d->eqrow, d->eqlo,
                     d->rowU, d->rowOut, d->fold_flag, d->fold_off, d->fold_cols, d->ntt_tabs,
                     d->ahat, d->cacc, d->cbytes, d->amat, d->cvs, d->sw, d->chal, d->chals};
+    if (d->stream) hipStreamSynchronize(d->stream);
     if (d->copy_stream) hipStreamSynchronize(d->copy_stream);
     for (void *p : ptrs)
         if (p) hipFree(p);
@@ -2405,8 +2421,9 @@ void lbn::dev_free(lbn_ctx *ctx) {
         if (e) hipEventDestroy(e);
     if (d->ev_commit) hipEventDestroy(d->ev_commit);
     if (d->ev_staged) hipEventDestroy(d->ev_staged);
-    if (d->copy_stream) hipStreamDestroy(d->copy_stream);
-    if (d->stream) hipStreamDestroy(d->stream);
+    if (d->switch_graph) hipGraphExecDestroy(d->switch_graph);
+    if (d->own_copy) hipStreamDestroy(d->own_copy);
+    if (d->own_stream) hipStreamDestroy(d->own_stream);
     delete d;
     ctx->dev = nullptr;
 }
@@ -2976,6 +2993,34 @@ static void launch_round(lbn_ctx *ctx, int r, RhoArg ra, const uint64_t *rho_dev
                                                                          T, T + 768, msg_buf(d, r), rho_dev);
 }
 
+// The switch rounds' launches: control step r, then round r's fold and round r + 1's message.
+// Everything they read (shape, buffers, the device state d->sw) is fixed per context.
+static void launch_switch(lbn_ctx *ctx) {
+    DeviceState *d = ctx->dev;
+    const int lp = ctx->shape.log_packed;
+    for (int r = 0; r < lp; r++) {
+        const uint32_t *pair = r == lp - 1 ? d->P[(r - 1) & 1] : nullptr;  // log_packed >= 12
+        k_switch_ctl<<<1, 1024, 0, d->stream>>>(d->sw, r, lp, r == 0 ? 4 : 6, msg_buf(d, r), pair);
+        if (r < lp - 1) launch_round(ctx, r, RhoArg{{0, 0, 0}}, d->sw->rho);
+    }
+}
+
+// Captured once per context (at creation) and replayed by every proof.
+static int capture_switch(lbn_ctx *ctx) {
+    DeviceState *d = ctx->dev;
+    HIPCHK(ctx, hipStreamBeginCapture(d->stream, hipStreamCaptureModeThreadLocal));
+    launch_switch(ctx);
+    hipGraph_t g = nullptr;
+    const hipError_t cap = hipGetLastError(), end = hipStreamEndCapture(d->stream, &g);
+    hipError_t inst = hipSuccess;
+    if (cap == hipSuccess && end == hipSuccess) inst = hipGraphInstantiate(&d->switch_graph, g, nullptr, nullptr, 0);
+    if (g) hipGraphDestroy(g);
+    HIPCHK(ctx, cap);
+    HIPCHK(ctx, end);
+    HIPCHK(ctx, inst);
+    return LBN_OK;
+}
+
 int lbn::dev_switch_rounds(lbn_ctx *ctx, B3State &ts, uint64_t *rounds, uint64_t *coords, uint64_t *opened) {
     if (ctx->round != 0) return ctx->fail(LBN_E_STATE, "lbn_switch_begin must run first");
     const int lp = ctx->shape.log_packed;
@@ -2988,11 +3033,10 @@ int lbn::dev_switch_rounds(lbn_ctx *ctx, B3State &ts, uint64_t *rounds, uint64_t
     for (int r = 0; r < lp; r++) h->r_hi[r][0] = ctx->r_hi[r].w[0], h->r_hi[r][1] = ctx->r_hi[r].w[1];
     HIPCHK(ctx, hipMemcpyAsync(d->sw, h, sizeof(SwitchDev), hipMemcpyHostToDevice, d->stream));
     tstart(ctx, T_ROUNDS);
-    for (int r = 0; r < lp; r++) {
-        const uint32_t *pair = r == lp - 1 ? d->P[(r - 1) & 1] : nullptr;  // log_packed >= 12
-        k_switch_ctl<<<1, 1024, 0, d->stream>>>(d->sw, r, lp, r == 0 ? 4 : 6, msg_buf(d, r), pair);
-        if (r < lp - 1) launch_round(ctx, r, RhoArg{{0, 0, 0}}, d->sw->rho);
-    }
+    if (d->graphs)
+        HIPCHK(ctx, hipGraphLaunch(d->switch_graph, d->stream));
+    else
+        launch_switch(ctx);
     tstop(ctx, T_ROUNDS);
     HIPCHK(ctx, hipGetLastError());
     HIPCHK(ctx, hipMemcpyAsync(h, d->sw, sizeof(SwitchDev), hipMemcpyDeviceToHost, d->stream));
@@ -3215,6 +3259,12 @@ int lbn::dev_challenges(lbn_ctx *ctx, const ChalArg &arg, lbn_challenge *out) {
 }
 
 // ----------------------------------------------------------------------------- timing
+
+extern "C" int lbn_set_graphs(lbn_ctx *ctx, int enable) {
+    if (!ctx) return LBN_E_ARG;
+    ctx->dev->graphs = enable != 0;
+    return LBN_OK;
+}
 
 extern "C" int lbn_set_timing(lbn_ctx *ctx, int enable) {
     if (!ctx) return LBN_E_ARG;

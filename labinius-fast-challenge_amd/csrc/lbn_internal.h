@@ -155,6 +155,10 @@ int dev_switch_rounds(lbn_ctx *ctx, B3State &ts, uint64_t *rounds, uint64_t *coo
 // The 1024 folding challenges sampled on the device from the transcript state in arg (the
 // counter of arg->ts is the first challenge's); out receives them in column order.
 int dev_challenges(lbn_ctx *ctx, const ChalArg &arg, lbn_challenge *out);
+// The streams the context's work goes to (hipStream_t, as void *): a caller driving many contexts
+// from a few threads gives each thread its own pair; null restores the context's own streams.
+// Only between entry points (every entry point leaves its streams idle).
+void dev_use_streams(lbn_ctx *ctx, void *main, void *copy);
 
 // Host worker pool: f(i) for every i < n, on up to host_threads() threads (the caller included).
 int host_threads();

@@ -9,6 +9,13 @@
 // first); a job that fits nowhere waits until a running proof releases memory. Admission is in
 // arrival order with bounded overtaking: a later job may start on a free context or free memory
 // while the first waits, but the first cannot be overtaken more than MAX_BYPASS times.
+//
+// Each worker owns a pair of HIP streams and runs its proofs on them, whatever context they use:
+// HIP maps streams onto its hardware queues in creation order, so the workers' streams (created
+// together) sit on distinct queues, while the contexts' own streams (created one per context, any
+// number of them) would put two proofs in flight on one queue, one after the other, by chance.
+#include <hip/hip_runtime_api.h>
+
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -53,6 +60,24 @@ struct lbn_sched {
     std::deque<Entry> queue;
     std::vector<Slot *> slots;
     std::vector<std::thread> workers;
+    std::vector<hipStream_t> streams;  // two per worker: main, copy (created together at the first job)
+    int nworkers = 0;
+
+    // Under m.
+    bool make_streams() {
+        if (!streams.empty()) return true;
+        if (hipSetDevice(device) != hipSuccess) return false;
+        for (int i = 0; i < 2 * nworkers; i++) {
+            hipStream_t st = nullptr;
+            if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+                for (hipStream_t x : streams) hipStreamDestroy(x);
+                streams.clear();
+                return false;
+            }
+            streams.push_back(st);
+        }
+        return true;
+    }
     size_t used = 0;
     size_t pending = 0;
     uint64_t tick = 0;
@@ -110,7 +135,7 @@ struct lbn_sched {
         return false;
     }
 
-    void run_job(Entry &e, Slot *slot) {
+    void run_job(Entry &e, Slot *slot, int w) {
         lbn_job *job = e.job;
         const double start = now_us();
         int rc = LBN_OK;
@@ -128,10 +153,16 @@ struct lbn_sched {
             }
         }
         if (rc == LBN_OK) {
+            std::lock_guard<std::mutex> lk(m);
+            if (!make_streams()) rc = LBN_E_DEVICE;
+        }
+        if (rc == LBN_OK) {
             lbn_ctx *ctx = slot->ctx;
+            lbn::dev_use_streams(ctx, streams[2 * w], streams[2 * w + 1]);
             rc = job->table ? lbn_load_table(ctx, job->table, ((size_t)1 << job->log_vars) / 8)
                             : lbn_generate_table(ctx, job->seed);
             if (rc == LBN_OK) rc = lbn_prove(ctx, job->seed, job->proof, job->cap, &job->len, &job->profile);
+            lbn::dev_use_streams(ctx, nullptr, nullptr);
         }
         const double end = now_us();
         std::lock_guard<std::mutex> lk(m);
@@ -152,7 +183,7 @@ struct lbn_sched {
         wake.notify_all();
     }
 
-    void loop() {
+    void loop(int w) {
         for (;;) {
             Entry e{};
             Slot *slot = nullptr;
@@ -163,7 +194,7 @@ struct lbn_sched {
                 if (!slot) return;  // stop with nothing admitted
             }
             for (lbn_ctx *c : evict) lbn_destroy(c);
-            run_job(e, slot);
+            run_job(e, slot, w);
         }
     }
 };
@@ -178,7 +209,8 @@ extern "C" int lbn_sched_create(int device, uint64_t q_lo, uint64_t q_hi, size_t
     s->q_lo = q_lo;
     s->q_hi = q_hi;
     s->budget = hbm_budget;
-    for (int i = 0; i < workers; i++) s->workers.emplace_back([s] { s->loop(); });
+    s->nworkers = workers;
+    for (int i = 0; i < workers; i++) s->workers.emplace_back([s, i] { s->loop(i); });
     *out = s;
     return LBN_OK;
 }
@@ -235,5 +267,60 @@ extern "C" void lbn_sched_destroy(lbn_sched *s) {
         if (slot->ctx) lbn_destroy(slot->ctx);
         delete slot;
     }
+    for (hipStream_t x : s->streams) hipStreamDestroy(x);
     delete s;
+}
+
+extern "C" int lbn_sched_prewarm(lbn_sched *s, int log_vars, int count) {
+    if (!s || count < 0 || count > 64) return LBN_E_ARG;
+    const size_t need = lbn_ctx_footprint(log_vars, s->q_lo, s->q_hi);
+    if (need == 0) return LBN_E_ARG;
+    std::vector<Slot *> mine;
+    {
+        std::lock_guard<std::mutex> lk(s->m);
+        if (s->stop) return LBN_E_STATE;
+        int have = 0;
+        for (Slot *x : s->slots)
+            if (x->log_vars == log_vars) have++;
+        for (; have < count && s->used + need <= s->budget; have++) {
+            Slot *x = new Slot();
+            x->log_vars = log_vars;
+            x->bytes = need;
+            x->busy = true;  // reserved: no job takes it before it is ready
+            s->used += need;
+            s->stats.peak_bytes = std::max<uint64_t>(s->stats.peak_bytes, s->used);
+            s->slots.push_back(x);
+            mine.push_back(x);
+        }
+        if (have < count) {
+            for (Slot *x : mine) {
+                s->used -= x->bytes;
+                s->slots.erase(std::find(s->slots.begin(), s->slots.end(), x));
+                delete x;
+            }
+            return LBN_E_RANGE;  // the budget holds fewer
+        }
+    }
+    int rc = LBN_OK;
+    for (Slot *x : mine) {
+        lbn_ctx *ctx = nullptr;
+        int r = lbn_create(s->device, log_vars, &ctx);
+        if (r == LBN_OK) r = lbn_set_modulus(ctx, s->q_lo, s->q_hi);
+        std::lock_guard<std::mutex> lk(s->m);
+        if (r == LBN_OK) {
+            x->ctx = ctx;
+            x->ready = true;
+            x->busy = false;
+            x->last_used = ++s->tick;
+            s->stats.contexts_created++;
+        } else {
+            if (ctx) lbn_destroy(ctx);
+            s->used -= x->bytes;
+            s->slots.erase(std::find(s->slots.begin(), s->slots.end(), x));
+            delete x;
+            rc = r;
+        }
+    }
+    s->wake.notify_all();
+    return rc;
 }

@@ -137,6 +137,27 @@ def test_deterministic(m, p20):
     assert a == b
 
 
+def test_switch_graph_replays(m):
+    """The captured switch graph replayed by several proofs (other work and host syncs between the
+    replays, different seeds and tables) gives the bytes of the launch-by-launch switch."""
+    say("switch graph replays")
+    p = m.Prover(20)
+    try:
+        p.set_modulus(O.DEFAULT_Q)
+        for t_seed, seeds in ((SEEDS[0], SEEDS[:3]), (SEEDS[1], SEEDS[1:3])):
+            p.generate_table(t_seed)
+            p.set_graphs(False)
+            eager = [p.prove(sd)[0] for sd in seeds]
+            p.set_graphs(True)
+            for _ in range(2):
+                for sd, want in zip(seeds, eager):
+                    assert p.prove(sd)[0] == want
+        tab = O.table(SEEDS[1], 20)
+        assert eager[-1] == O.prove(SEEDS[2], 20, O.DEFAULT_Q, tab)
+    finally:
+        p.close()
+
+
 def test_pinned_proof_buffer(m, p20):
     """A proof written into library-pinned memory (the fold kernel and the commitment copy write it
     directly) equals the proof written through staging into an ordinary array; an unaligned slice

@@ -72,6 +72,28 @@ def test_mix_equals_alone_and_oracle(m, alone):
         assert alone(k, seed_of(i)) == O.prove(seed_of(i), k)
 
 
+def test_prewarmed_contexts(m, alone):
+    say("scheduler: prewarm 3 contexts of k=20 and 2 of k=21, 6 jobs on 3 workers")
+    sch = m.Scheduler(m.DEFAULT_Q, hbm_budget=8 << 30, workers=3)
+    try:
+        sch.prewarm(20, 3)
+        sch.prewarm(21, 2)
+        sch.prewarm(20, 2)  # already held: creates nothing
+        assert sch.stats()["contexts_created"] == 5
+        jobs = [(20, 10), (21, 11), (20, 12), (20, 13), (21, 14), (20, 15)]
+        hs = [(k, seed_of(i), sch.submit(k, seed_of(i))) for k, i in jobs]
+        sch.drain()
+        for k, seed, h in hs:
+            assert sch.result(h)[0] == alone(k, seed), (k, seed[:2].hex())
+        st = sch.stats()
+        assert st["contexts_created"] == 5 and st["completed"] == len(jobs) and st["failed"] == 0
+        with pytest.raises(m.LbnError):
+            sch.prewarm(31, 64)  # beyond the budget: nothing created
+        assert sch.stats()["contexts_created"] == 5
+    finally:
+        sch.close()
+
+
 def test_eviction_under_a_tight_budget(m, alone):
     # room for one k=23 context plus a little: the mix of sizes must evict idle contexts
     budget = m.ctx_footprint(23) + m.ctx_footprint(20)

@@ -108,6 +108,19 @@ def pmc_traffic(pmc_dir, log_vars, kernel="k_commit_mac", copy_kernel="k_copy_u4
                           "write_scale": round(w_scale, 4)}
 
 
+def committed_traffic(path=os.path.join(ROOT, "profiles", "commit_mac_traffic.txt")):
+    """The commit kernel's HBM bytes per launch at k = 31 as the last counter runs measured them (the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes and copy-kernel scales written into the profile; a run
+    with --pmc-dir measures them afresh)."""
+    try:
+        kv = dict(line.split("=", 1) for line in open(path).read().split() if "=" in line)
+        f, w = int(kv["fetch_bytes"]), int(kv["write_bytes"])
+        return f + w, {"fetch_bytes": f, "write_bytes": w, "fetch_scale": float(kv["fetch_scale"]),
+                       "write_scale": float(kv["write_scale"]), "source": "profiles/commit_mac_traffic.txt"}
+    except (OSError, KeyError, ValueError):
+        return None, None
+
+
 def batched_instances(n=64, first=100):
     """The batched workload: instance i has seed bench_seed(first + i) and k = 20 + seed[0] mod 9."""
     return [(20 + bench_seed(first + i)[0] % 9, bench_seed(first + i)) for i in range(n)]
@@ -335,6 +348,8 @@ def main():
                 "traffic": None, "launch_ms": round(per_launch_ms, 3)}
     if args.pmc_dir:
         roofline["traffic"], roofline["traffic_detail"] = pmc_traffic(args.pmc_dir, k)
+    elif k == 31:
+        roofline["traffic"], roofline["traffic_detail"] = committed_traffic()
     line = {
         "metric": "prover ms (commit+open, 2^31 evals) on 1 MI355X",
         "value": round(med, 3), "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,

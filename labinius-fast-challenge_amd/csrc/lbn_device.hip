@@ -1597,7 +1597,9 @@ __device__ __forceinline__ void u192_sub(const uint64_t (&a)[3], const uint64_t 
     r[0] = r0, r[1] = r1, r[2] = a[2] - b[2] - b1;
 }
 
-__global__ __launch_bounds__(64) void k_commit_finish(const uint32_t *cacc, const uint32_t *tabs_all,
+// Four waves per column: the transforms use 36 threads, the slice sums and the CRT lift all.
+constexpr int FIN_THREADS = 256;
+__global__ __launch_bounds__(FIN_THREADS) void k_commit_finish(const uint32_t *cacc, const uint32_t *tabs_all,
                                                       CrtArg crt, uint8_t *outb) {
     __shared__ uint32_t Y[2 * 324];
     __shared__ uint32_t x[648];
@@ -1611,13 +1613,23 @@ __global__ __launch_bounds__(64) void k_commit_finish(const uint32_t *cacc, cons
 #pragma unroll
         for (int j = 0; j < 9; j++) w[j] = crt.roots[k].i[j];
         {  // the sum over the MAC's slices (at most 128 residues below p, so below 2^32 p),
-           // coalesced over the slots, reduced as REDC then a Montgomery product with R^2
+           // coalesced over the slots, reduced as REDC then a Montgomery product with R^2; each
+           // slice's eleven loads per thread are issued together, so the slices cost one latency
+           // each rather than one per slot
             const uint32_t *src = cacc + (size_t)k * crt.splits * crt.cols * 648 + (size_t)c * 648;
-            for (int i = t; i < 648; i += 64) {
-                uint64_t sum = 0;
-                for (int sl = 0; sl < crt.splits; sl++) sum += src[(size_t)sl * crt.cols * 648 + i];
-                x[i] = mont(redc64(sum, pc), crt.r2[k], pc);
+            constexpr int PER = (648 + FIN_THREADS - 1) / FIN_THREADS;
+            uint64_t sum[PER];
+#pragma unroll
+            for (int j = 0; j < PER; j++) sum[j] = 0;
+            for (int sl = 0; sl < crt.splits; sl++) {
+                const uint32_t *row = src + (size_t)sl * crt.cols * 648;
+#pragma unroll
+                for (int j = 0; j < PER; j++)
+                    if (t + FIN_THREADS * j < 648) sum[j] += row[t + FIN_THREADS * j];
             }
+#pragma unroll
+            for (int j = 0; j < PER; j++)
+                if (t + FIN_THREADS * j < 648) x[t + FIN_THREADS * j] = mont(redc64(sum[j], pc), crt.r2[k], pc);
         }
         __syncthreads();
         if (t < 36) {
@@ -1644,7 +1656,7 @@ __global__ __launch_bounds__(64) void k_commit_finish(const uint32_t *cacc, cons
             for (int n1 = 0; n1 < 18; n1++) x[h * 324 + 18 * n1 + idx] = v[n1];
         }
         __syncthreads();
-        for (int i = t; i < 324; i += 64) {
+        for (int i = t; i < 324; i += FIN_THREADS) {
             const uint32_t u0 = mont(x[i], tabs[tab::UNTW + i], pc);
             const uint32_t u1 = mont(x[324 + i], tabs[tab::UNTW + 324 + i], pc);
             const uint32_t ahi = mont(subm(u0, u1, pc.p), tabs[tab::DINV], pc);
@@ -1654,7 +1666,7 @@ __global__ __launch_bounds__(64) void k_commit_finish(const uint32_t *cacc, cons
         __syncthreads();
     }
     const u128 Q = ((u128)crt.q_hi << 64) | crt.q_lo;
-    for (int tt = t; tt < 648; tt += 64) {
+    for (int tt = t; tt < 648; tt += FIN_THREADS) {
         // Garner: x = v0 + p0 (v1 + p1 (v2 + ...))
         uint32_t v[MAX_PRIMES];
         for (int i = 0; i < crt.nprimes; i++) {
@@ -2877,7 +2889,7 @@ int lbn::dev_commit_launch(lbn_ctx *ctx) {
     tstop(ctx, T_MAC);
     const CrtArg crt = make_crt(ctx->primes, d->roots, np, ctx->q, ctx->width, msplit, COLUMNS);
     tstart(ctx, T_FINISH);
-    k_commit_finish<<<COLUMNS, 64, 0, d->stream>>>(d->cacc, d->ntt_tabs, crt, d->cbytes);
+    k_commit_finish<<<COLUMNS, FIN_THREADS, 0, d->stream>>>(d->cacc, d->ntt_tabs, crt, d->cbytes);
     tstop(ctx, T_FINISH);
     HIPCHK(ctx, hipGetLastError());
     HIPCHK(ctx, hipEventRecord(d->ev_commit, d->stream));
@@ -3553,7 +3565,7 @@ extern "C" int lbn_cmt_launch(lbn_cmt *c) {
         crt = make_crt(c->primes, c->roots, np, c->q, c->width, 1, c->rows * cols);
     }
     const int e = cmt_begin(c, CK_FINISH);
-    k_commit_finish<<<(unsigned)(c->rows * cols), 64, 0, c->stream>>>(fin, c->tabs, crt, c->out);
+    k_commit_finish<<<(unsigned)(c->rows * cols), FIN_THREADS, 0, c->stream>>>(fin, c->tabs, crt, c->out);
     cmt_end(c, e);
     HIPCHK(c, hipGetLastError());
     cmt_end(c, tot);

@@ -224,18 +224,26 @@ struct B3Key {
     uint32_t flags;
 };
 
+// One wave per window of 64 chunk indices (absolute indices 64 W .. 64 W + 63, lane = index mod 64):
+// the lanes whose chunk lies in the run hash it, then the wave merges the aligned complete
+// subtrees of up to 64 chunks that lie in the run (CV_WIN_LEVELS levels), leaving each subtree's
+// value at its first chunk's slot, as k_cv_tree does for the levels above.
+constexpr int CV_WIN_LEVELS = 6;
 __global__ __launch_bounds__(64) void k_chunk_cvs(const uint8_t *bytes, size_t off, uint64_t first, size_t count,
                                                   B3Key key, uint32_t *cvs) {
-    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= count) return;
-    const size_t base = off + c * 1024;
-    const uint32_t *w = (const uint32_t *)(bytes + (base & ~(size_t)3));
-    const uint32_t sh = (uint32_t)(base & 3) * 8;
-    const uint64_t index = first + c;
+    __shared__ uint32_t s_cv[64 * 8];
+    const int lane = threadIdx.x;
+    const uint64_t a = (first / 64 + blockIdx.x) * 64 + lane, end = first + count;
+    const bool in = a >= first && a < end;
+    const size_t c = (size_t)(a - first);
+    const uint64_t index = a;
     uint32_t cv[8];
 #pragma unroll
     for (int k = 0; k < 8; k++) cv[k] = key.k[k];
-    for (int b = 0; b < 16; b++) {
+    const size_t base = off + (in ? c : 0) * 1024;
+    const uint32_t *w = (const uint32_t *)(bytes + (base & ~(size_t)3));
+    const uint32_t sh = (uint32_t)(base & 3) * 8;
+    for (int b = 0; b < 16 && in; b++) {
         uint32_t m[16];
         uint32_t lo = w[16 * b];
 #pragma unroll
@@ -252,8 +260,27 @@ __global__ __launch_bounds__(64) void k_chunk_cvs(const uint8_t *bytes, size_t o
 #pragma unroll
         for (int k = 0; k < 8; k++) cv[k] = st[k] ^ st[k + 8];
     }
+    for (int l = 0; l < CV_WIN_LEVELS; l++) {
+        const uint64_t span = 2ull << l;
 #pragma unroll
-    for (int k = 0; k < 8; k++) cvs[8 * c + k] = cv[k];
+        for (int k = 0; k < 8; k++) s_cv[8 * lane + k] = cv[k];
+        __syncthreads();
+        if (in && a % span == 0 && a + span <= end) {  // both halves in the run: a complete subtree
+            uint32_t m[16];
+#pragma unroll
+            for (int k = 0; k < 8; k++) m[k] = cv[k], m[k + 8] = s_cv[8 * (lane + (1 << l)) + k];
+            uint32_t st[16] = {key.k[0], key.k[1], key.k[2], key.k[3], key.k[4], key.k[5], key.k[6], key.k[7],
+                               0x6A09E667u, 0xBB67AE85u, 0x3C6EF372u, 0xA54FF53Au,
+                               0u, 0u, 64u, key.flags | 4u};  // PARENT
+            b3_rounds(st, m);
+#pragma unroll
+            for (int k = 0; k < 8; k++) cv[k] = st[k] ^ st[k + 8];
+        }
+        __syncthreads();
+    }
+    if (in)
+#pragma unroll
+        for (int k = 0; k < 8; k++) cvs[8 * c + k] = cv[k];
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -2007,12 +2034,15 @@ struct CvTrees {
 };
 
 template <bool IN_LDS>
-__global__ __launch_bounds__(1024) void k_cv_tree(uint32_t *cvs, int count, CvTrees tr, B3Key key) {
-    extern __shared__ uint32_t s_cv[];
+__global__ __launch_bounds__(1024) void k_cv_tree(uint32_t *cvs, int count, CvTrees tr, B3Key key, int l0) {
+    extern __shared__ __align__(16) uint32_t s_cv[];
     uint32_t *base = IN_LDS ? s_cv : cvs;
     const int t = threadIdx.x, q = t & 3, quad = t >> 2;
-    if (IN_LDS) {
-        for (int k = t; k < count * 8; k += blockDim.x) s_cv[k] = cvs[k];
+    if (IN_LDS) {  // 16-byte copies, several in flight per thread
+        const uint4 *src = (const uint4 *)cvs;
+        uint4 *dst = (uint4 *)s_cv;
+#pragma unroll 8
+        for (int k = t; k < count * 2; k += blockDim.x) dst[k] = src[k];
         __syncthreads();
     }
     uint32_t kw[8];
@@ -2020,7 +2050,7 @@ __global__ __launch_bounds__(1024) void k_cv_tree(uint32_t *cvs, int count, CvTr
     for (int k = 0; k < 8; k++) kw[k] = key.k[k];
     int maxl = 0;
     for (int b = 0; b < tr.n; b++) maxl = max(maxl, tr.level[b]);
-    for (int l = 0; l < maxl; l++) {
+    for (int l = l0; l < maxl; l++) {  // the levels below l0 are merged already
         int total = 0;
         for (int b = 0; b < tr.n; b++)
             if (tr.level[b] > l) total += 1 << (tr.level[b] - l - 1);
@@ -2800,12 +2830,13 @@ int lbn::dev_commit_cvs(lbn_ctx *ctx, size_t off, uint64_t first, size_t count, 
         B3Key k;
         for (int i = 0; i < 8; i++) k.k[i] = key[i];
         k.flags = flags;
-        k_chunk_cvs<<<(unsigned)((count + 63) / 64), 64, 0, d->stream>>>(d->cbytes, off, first, count, k, d->cvs);
+        const unsigned windows = (unsigned)((first + count - 1) / 64 - first / 64 + 1);
+        k_chunk_cvs<<<windows, 64, 0, d->stream>>>(d->cbytes, off, first, count, k, d->cvs);
         HIPCHK(ctx, hipGetLastError());
         if (count * 32 <= CV_TREE_LDS_MAX)
-            k_cv_tree<true><<<1, 1024, count * 32, d->stream>>>(d->cvs, (int)count, tr, k);
+            k_cv_tree<true><<<1, 1024, count * 32, d->stream>>>(d->cvs, (int)count, tr, k, CV_WIN_LEVELS);
         else
-            k_cv_tree<false><<<1, 1024, 0, d->stream>>>(d->cvs, (int)count, tr, k);
+            k_cv_tree<false><<<1, 1024, 0, d->stream>>>(d->cvs, (int)count, tr, k, CV_WIN_LEVELS);
         HIPCHK(ctx, hipGetLastError());
         HIPCHK(ctx, hipMemcpyAsync(d->hcvs, d->cvs, trees.size() * 32, hipMemcpyDeviceToHost, d->stream));
     }

@@ -36,17 +36,19 @@ HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
 VALU_SLOTS_T = 256 * 64 * 2.4e9 / 1e12  # int32 VALU issue: 256 CUs x 64 lanes x 2.4 GHz = 39.3 T/s
 # Commit roofline in the algorithmic unit of the reference's transform: per ring element and CRT
 # prime, 324 (Phi6 split) + 2*324 (radix 2) + 4*216*3 (radix 3 with the cube root) = 3,564 modular
-# products plus 648 MAC = 4,212 "mulmod-eq". Peak: the int32 VALU issue rate at 5 slots per
-# Montgomery product (two half-rate 32x32->64 multiply-adds and one 32-bit multiply).
+# products plus 648 MAC = 4,212 "mulmod-eq". Peak: the int32 VALU issue rate at 3 slots per
+# Montgomery product (two 32x32->64 multiply-adds and one 32-bit multiply, each one issue slot on
+# gfx950: build/valu_rates measures chains of them at 0.86-0.95 of that rate).
 MULMOD_PER_ELEMENT_PRIME = 4212
-MULMOD_PEAK_T = VALU_SLOTS_T / 5
+MULMOD_SLOTS = 3
+MULMOD_PEAK_T = VALU_SLOTS_T / MULMOD_SLOTS
 
 
 # Commit micro-benchmark: 64 columns x 4096 binary ring elements (a 16 MiB witness from seed 0),
 # Q = 3889 * 9721 * 17497, keys of 1 and 16 ring rows. The one-row commitment is one VALU kernel;
 # its roofline counts int32 VALU issue slots per (ring element, column, CRT prime): the
-# transform's 3,564 modular products at 5 slots each and the key row's 648 multiply-adds at 2 slots
-# (one half-rate 32x32->64 multiply-add). With 16 rows the transform (the same 3,564 x 5 slots) is
+# transform's 3,564 modular products at 3 slots each and the key row's 648 multiply-adds at 1 slot
+# (one 32x32->64 multiply-add). With 16 rows the transform (the same 3,564 x 3 slots) is
 # the larger kernel; the key-row products run on the matrix cores and are priced in bytes: the
 # transformed witness (4 bytes per slot, column and element) and the key's digits (4 bytes per
 # slot, row and element) read once per CRT prime.
@@ -55,7 +57,7 @@ C3_COLUMNS, C3_ELEMENTS = 64, 4096
 
 
 def c3_slots(rows: int) -> int:
-    return 3564 * 5 + rows * 648 * 2
+    return 3564 * MULMOD_SLOTS + rows * 648
 
 
 def c3_product_bytes(rows: int, primes: int) -> int:
@@ -236,7 +238,7 @@ def commit_micro(m, steps, warmup):
                                "achieved": round(achieved, 3), "peak": round(VALU_SLOTS_T, 3),
                                "unit": "T int32 slots/s", "frac": round(achieved / VALU_SLOTS_T, 4)}
         else:
-            achieved = C3_COLUMNS * C3_ELEMENTS * primes * 3564 * 5 / (kms["transform"] / 1e3) / 1e12
+            achieved = C3_COLUMNS * C3_ELEMENTS * primes * 3564 * MULMOD_SLOTS / (kms["transform"] / 1e3) / 1e12
             rec["roofline"] = {"bound": "valu", "kernel": "k_commit_v (transform)", "achieved": round(achieved, 3),
                                "peak": round(VALU_SLOTS_T, 3), "unit": "T int32 slots/s",
                                "frac": round(achieved / VALU_SLOTS_T, 4)}

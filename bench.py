@@ -145,10 +145,12 @@ def run_jobs(m, jobs, workers, budget=200 << 30):
     sch = m.Scheduler(m.DEFAULT_Q, hbm_budget=budget, workers=workers)
     try:
         warm(sch, sorted({kk for kk, _ in jobs}), workers)
+        w0 = time.monotonic_ns()
         t0 = time.perf_counter()
         hs = [sch.submit(kk, seed) for kk, seed in jobs]
         sch.drain()
         dt = time.perf_counter() - t0
+        run_jobs.window_ns = (w0, time.monotonic_ns())  # the timed batch, on the kernel tracer's clock
         return [sch.result(h)[0] for h in hs], dt
     finally:
         sch.close()
@@ -158,7 +160,9 @@ def batched(m, workers):
     """64 independent proofs of mixed size (k in [20, 28]) in flight together vs one at a time."""
     jobs = batched_instances()
     seq, t_seq = run_jobs(m, jobs, 1)
+    w_seq = run_jobs.window_ns
     par, t_par = run_jobs(m, jobs, workers)
+    batched.windows_ns = {"sequential": list(w_seq), "parallel": list(run_jobs.window_ns)}
     return {"instances": len(jobs), "k": sorted({kk for kk, _ in jobs}), "workers": workers,
             "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")),
             "proofs_per_s": round(len(jobs) / t_par, 1), "sequential_proofs_per_s": round(len(jobs) / t_seq, 1),
@@ -264,6 +268,9 @@ def main():
     ap.add_argument("--hw-queues", type=int, default=16,
                     help="GPU_MAX_HW_QUEUES for this process (hardware queues the proofs' streams map onto)")
     ap.add_argument("--serving", action="store_true", help="run the serving loop instead (prints its own line)")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="run only the batched measurement; its line also gives the monotonic-clock windows "
+                         "(ns) of the sequential and the parallel batch, to cut a kernel trace to them")
     ap.add_argument("--requests", type=int, default=10_000)
     ap.add_argument("--rate", type=float, default=400.0, help="serving arrivals per second")
     ap.add_argument("--pmc-dir", default=None,
@@ -287,6 +294,11 @@ def main():
         return
     if args.commit_micro_only:
         print(json.dumps(commit_micro(m, args.steps, args.warmup)), flush=True)
+        return
+    if args.batched_only:
+        b = batched(m, args.workers)
+        b["windows_ns"] = batched.windows_ns
+        print(json.dumps(b), flush=True)
         return
     k = args.log_vars
     p = m.Prover(k)

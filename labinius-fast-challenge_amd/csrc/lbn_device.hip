@@ -44,12 +44,11 @@ struct lbn::DeviceState {
     uint4 *table = nullptr;
     uint4 *E[2] = {nullptr, nullptr};
     uint32_t *P[2] = {nullptr, nullptr};
-    uint32_t *T = nullptr;        // [0,768): T0, [768,1536): Tinf, [1536, 2048): partials
-    uint4 *EH = nullptr, *EL = nullptr;
-    uint32_t *eqrow = nullptr, *rowU = nullptr, *rowOut = nullptr, *eqlo = nullptr;
-    uint32_t *fold_flag = nullptr;
+    uint32_t *T = nullptr;  // [0,768): T0, [768,1536): Tinf, [1536, 2048): partials, [2048, 3584): odd rounds
+    uint4 *EL = nullptr, *EH = nullptr;  // EH = EL + 4096 (one allocation)
+    uint32_t *eqrow = nullptr, *rowU = nullptr, *eqlo = nullptr;
     int *fold_off = nullptr;
-    uint16_t *fold_cols = nullptr;
+    uint16_t *fold_cols = nullptr;  // inside fold_off's allocation
     uint32_t *ntt_tabs = nullptr;   // MAX_PRIMES * tab::SIZE
     uint32_t *ahat = nullptr;       // nprimes * RE * 648
     uint32_t *cacc = nullptr;       // nprimes * splits * 1024 * 648: the MAC's partial sums (< p)
@@ -65,10 +64,11 @@ struct lbn::DeviceState {
     bool staged_direct = false;     // the last commitment was copied straight into the proof buffer
     int32_t *hfold = nullptr;
     uint8_t *hbox = nullptr;        // pinned mailbox for the small transfers (regions in namespace mb)
+    uint8_t *hbox_dev = nullptr;    // its device address: kernels write small results straight into it
+    bool msg_clean = false;         // the switch message sums are zero (cleared with the partials)
     uint32_t *hcvs = nullptr;       // pinned: chunk chaining values of the commitment
     struct SwitchDev *sw = nullptr; // the switch rounds' device state (transcript, L-map, outputs)
-    ChalArg *chal = nullptr;        // the challenge sampler's input
-    lbn_challenge *chals = nullptr; // and output
+    ChalArg *chal = nullptr;        // the challenge sampler's input (its output goes to the mailbox)
     int32_t *fold_host = nullptr;   // device address of hfold
     hipStream_t copy_stream = nullptr;
     hipEvent_t ev_commit = nullptr, ev_staged = nullptr;
@@ -88,15 +88,15 @@ static int capture_switch(lbn_ctx *ctx);
 // path, which serialises proofs running on other threads). Each entry point synchronises its stream
 // before returning, so a region is never rewritten while a transfer from it is in flight.
 namespace mb {
-constexpr size_t EH = 0;                  // 2^(log_packed - 12) B128 <= 64 KiB
-constexpr size_t EL = EH + 65536;         // 4096 B128
-constexpr size_t EQ = EL + 65536;         // row eq tables: 128 + 128 F162
+constexpr size_t EL = 0;                  // 4096 B128
+constexpr size_t EH = EL + 65536;         // 2^(log_packed - 12) B128 <= 64 KiB (EL and EH: one copy)
+constexpr size_t EQ = EH + 65536;         // row eq tables: 128 + 128 F162
 constexpr size_t FOFF = EQ + 8192;        // fold list offsets (325 int) and position order (162 int)
-constexpr size_t FCOLS = FOFF + 2048;     // fold column lists (<= 1024 x 32 u16)
+constexpr size_t FCOLS = FOFF + 2048;     // fold column lists (<= 1024 x 32 u16); FOFF..FCOLS: one copy
 constexpr size_t T = FCOLS + 65536;       // switch message sums (1536 u32)
 constexpr size_t PART = T + 6144;         // partials (512 u32)
 constexpr size_t ROW = PART + 2048;       // row (1024 F162)
-constexpr size_t MISC = ROW + 24576;      // flags, the last two switch values (256 B)
+constexpr size_t MISC = ROW + 24576;      // the fold's range flag, the last two switch values (256 B)
 constexpr size_t HEAD = MISC + 256;       // commitment bytes before the hashed chunks (<= 1 KiB)
 constexpr size_t TAIL = HEAD + 1024;      // and after them (<= 1 KiB)
 constexpr size_t SW = TAIL + 1024;        // the device switch state (SwitchDev, < 8 KiB)
@@ -287,11 +287,16 @@ __global__ __launch_bounds__(64) void k_chunk_cvs(const uint8_t *bytes, size_t o
 // eq_hi[jh * 4096 + jl] = EH[jh] * EL[jl] in GF(2^128): EH[jh] times the 12-coordinate
 // expansion EL, through a 16 x 256 byte-window table of y -> EH[jh] * y.
 // ---------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_eq_expand(const uint4 *EH, const uint4 *EL, uint4 *E0) {
+// Block 0 also zeroes `clear` (the partials' and the switch messages' XOR accumulators, which
+// the kernels after it fill), so that no separate fill is launched.
+__global__ __launch_bounds__(256) void k_eq_expand(const uint4 *EH, const uint4 *EL, uint4 *E0, uint32_t *clear,
+                                                   int nclear) {
     __shared__ uint4 basis[128];
     __shared__ uint4 tab[16 * 256];
     const int t = threadIdx.x;
     const uint32_t jh = blockIdx.x;
+    if (jh == 0)
+        for (int i = t; i < nclear; i += 256) clear[i] = 0;
     if (t < 128) {
         uint4 e = EH[jh];
         uint64_t lo = (uint64_t)e.x | ((uint64_t)e.y << 32), hi = (uint64_t)e.z | ((uint64_t)e.w << 32);
@@ -1245,7 +1250,7 @@ __global__ __launch_bounds__(FOLD_THREADS) void k_fold(const uint4 *table, size_
             auto R = [&](int tp) -> int { return tp <= 1159 ? s_raw[(tp & 3) * 292 + (tp >> 2)] : 0; };
             const int v = t < 324 ? R(t) - R(t + 648) - R(t + 972) : R(t) + R(t + 324);
             out[m * 648 + t] = v;
-            if (v < -32768 || v > 32767) atomicOr(flag, 1u);
+            if (v < -32768 || v > 32767) *(volatile uint32_t *)flag = 1u;  // host memory: a plain store
         }
         __syncthreads();
     }
@@ -2414,22 +2419,23 @@ int lbn::dev_init(lbn_ctx *ctx) {
     HIPCHK(ctx, hipMalloc(&d->P[0], s.n_words / 2 * 24));
     HIPCHK(ctx, hipMalloc(&d->P[1], s.n_words / 4 * 24));
     HIPCHK(ctx, hipMalloc(&d->T, 3584 * 4));
-    HIPCHK(ctx, hipMalloc(&d->EH, ((size_t)1 << (s.log_packed - 12)) * 16));
-    HIPCHK(ctx, hipMalloc(&d->EL, 4096 * 16));
+    // EL then EH in one allocation, laid out like their mailbox regions (one copy per proof)
+    HIPCHK(ctx, hipMalloc(&d->EL, (4096 + ((size_t)1 << (s.log_packed - 12))) * 16));
+    d->EH = d->EL + 4096;
     HIPCHK(ctx, hipMalloc(&d->eqrow, s.column_len * 24));
     HIPCHK(ctx, hipMalloc(&d->eqlo, 256 * 24));
     HIPCHK(ctx, hipMalloc(&d->rowU, (size_t)COLUMNS * 128 * 24));
-    HIPCHK(ctx, hipMalloc(&d->rowOut, (size_t)COLUMNS * 24));
-    HIPCHK(ctx, hipMalloc(&d->fold_flag, 4));
-    HIPCHK(ctx, hipMalloc(&d->fold_off, (2 * 162 + 1 + 162) * sizeof(int)));
-    HIPCHK(ctx, hipMalloc(&d->fold_cols, (size_t)COLUMNS * 32 * sizeof(uint16_t)));
+    // the fold's offsets then its column lists, at the mailbox's relative offset (one copy per proof)
+    static_assert((2 * 162 + 1 + 162) * sizeof(int) <= mb::FCOLS - mb::FOFF, "fold offsets region");
+    HIPCHK(ctx, hipMalloc(&d->fold_off, (mb::FCOLS - mb::FOFF) + (size_t)COLUMNS * 32 * sizeof(uint16_t)));
+    d->fold_cols = (uint16_t *)((uint8_t *)d->fold_off + (mb::FCOLS - mb::FOFF));
     HIPCHK(ctx, hipMalloc(&d->ntt_tabs, (size_t)MAX_PRIMES * tab::SIZE * 4));
     HIPCHK(ctx, hipHostMalloc((void **)&d->hfold, s.ring_elements * N * sizeof(int32_t)));
     HIPCHK(ctx, hipHostMalloc((void **)&d->hbox, mb::SIZE));
     HIPCHK(ctx, hipMalloc(&d->sw, sizeof(SwitchDev)));
     HIPCHK(ctx, hipMalloc(&d->chal, sizeof(ChalArg)));
-    HIPCHK(ctx, hipMalloc(&d->chals, COLUMNS * sizeof(lbn_challenge)));
     HIPCHK(ctx, hipHostGetDevicePointer((void **)&d->fold_host, d->hfold, 0));
+    HIPCHK(ctx, hipHostGetDevicePointer((void **)&d->hbox_dev, d->hbox, 0));
     HIPCHK(ctx, hipStreamCreateWithFlags(&d->own_copy, hipStreamNonBlocking));
     d->copy_stream = d->own_copy;
     HIPCHK(ctx, hipEventCreateWithFlags(&d->ev_commit, hipEventDisableTiming));
@@ -2448,9 +2454,9 @@ void lbn::dev_free(lbn_ctx *ctx) {
     DeviceState *d = ctx->dev;
     if (!d) return;
     hipSetDevice(ctx->device);
-    void *ptrs[] = {d->table, d->E[0], d->E[1], d->P[0], d->P[1], d->T, d->EH, d->EL, d->eqrow, d->eqlo,
-                    d->rowU, d->rowOut, d->fold_flag, d->fold_off, d->fold_cols, d->ntt_tabs,
-                    d->ahat, d->cacc, d->cbytes, d->amat, d->cvs, d->sw, d->chal, d->chals};
+    void *ptrs[] = {d->table, d->E[0], d->E[1], d->P[0], d->P[1], d->T, d->EL, d->eqrow, d->eqlo,
+                    d->rowU, d->fold_off, d->ntt_tabs, d->ahat, d->cacc, d->cbytes, d->amat, d->cvs, d->sw,
+                    d->chal};
     if (d->stream) hipStreamSynchronize(d->stream);
     if (d->copy_stream) hipStreamSynchronize(d->copy_stream);
     for (void *p : ptrs)
@@ -2942,13 +2948,14 @@ extern "C" int lbn_partials(lbn_ctx *ctx, const uint64_t *r_hi, uint64_t *partia
     eq_expand_b128(ctx->r_hi.data() + nh, 12, EL.data());
     memcpy(d->hbox + mb::EH, EH.data(), EH.size() * 16);
     memcpy(d->hbox + mb::EL, EL.data(), EL.size() * 16);
-    HIPCHK(ctx, hipMemcpyAsync(d->EH, d->hbox + mb::EH, EH.size() * 16, hipMemcpyHostToDevice, d->stream));
-    HIPCHK(ctx, hipMemcpyAsync(d->EL, d->hbox + mb::EL, EL.size() * 16, hipMemcpyHostToDevice, d->stream));
+    static_assert(mb::EH == mb::EL + 4096 * 16, "EL and EH adjacent, as on the device");
+    HIPCHK(ctx, hipMemcpyAsync(d->EL, d->hbox + mb::EL, (4096 + EH.size()) * 16, hipMemcpyHostToDevice, d->stream));
     tstart(ctx, T_EQ);
-    k_eq_expand<<<(unsigned)EH.size(), 256, 0, d->stream>>>(d->EH, d->EL, d->E[0]);
+    // also zeroes T: the partials' accumulator and both switch message buffers
+    k_eq_expand<<<(unsigned)EH.size(), 256, 0, d->stream>>>(d->EH, d->EL, d->E[0], d->T, 3584);
     tstop(ctx, T_EQ);
     HIPCHK(ctx, hipGetLastError());
-    fill_u32(d->stream, d->T + 1536, 512);
+    d->msg_clean = true;
     const unsigned blocks = (unsigned)std::min<size_t>(s.n_words / 256, 1024);
     tstart(ctx, T_PARTIALS);
     k_partials<<<blocks, 256, 0, d->stream>>>(d->table, d->E[0], s.n_words, d->T + 1536);
@@ -2994,8 +3001,8 @@ extern "C" int lbn_switch_begin(lbn_ctx *ctx, const uint64_t *batch) {
     ctx->lmap.assign((const F162 *)batch, (const F162 *)batch + 128);
     ctx->round = 0;
     const size_t h = ctx->shape.n_words / 2;
-    fill_u32(d->stream, msg_buf(d, 0), 1536);
-    fill_u32(d->stream, msg_buf(d, 1), 1536);
+    if (!d->msg_clean) fill_u32(d->stream, d->T, 3584);  // both message buffers (cleared with the partials)
+    d->msg_clean = false;
     tstart(ctx, T_SWITCH0);
     k_switch_msg0<<<launch_blocks(h), 256, 0, d->stream>>>(d->table, d->E[0], h, d->E[1], d->T, d->T + 768);
     tstop(ctx, T_SWITCH0);
@@ -3187,10 +3194,9 @@ extern "C" int lbn_row_eval(lbn_ctx *ctx, const uint64_t *p0, int n_p0, uint64_t
     HIPCHK(ctx, hipGetLastError());
     k_row<<<COLUMNS, 256, 0, d->stream>>>(d->table, d->eqrow, s.column_len, d->rowU);
     HIPCHK(ctx, hipGetLastError());
-    k_row_finish<<<COLUMNS, 128, 0, d->stream>>>(d->rowU, (uint64_t *)d->rowOut);
+    k_row_finish<<<COLUMNS, 128, 0, d->stream>>>(d->rowU, (uint64_t *)(d->hbox_dev + mb::ROW));  // into the mailbox
     tstop(ctx, T_ROW);
     HIPCHK(ctx, hipGetLastError());
-    HIPCHK(ctx, hipMemcpyAsync(d->hbox + mb::ROW, d->rowOut, (size_t)COLUMNS * 24, hipMemcpyDeviceToHost, d->stream));
     HIPCHK(ctx, hipStreamSynchronize(d->stream));
     memcpy(row, d->hbox + mb::ROW, (size_t)COLUMNS * 24);
     tcollect(ctx, T_ROW);
@@ -3240,10 +3246,12 @@ extern "C" int lbn_fold(lbn_ctx *ctx, const lbn_challenge *ch, int n_challenges,
     DeviceState *d = ctx->dev;
     memcpy(d->hbox + mb::FOFF, off.data(), off.size() * sizeof(int));
     memcpy(d->hbox + mb::FCOLS, cols.data(), cols.size() * 2);
-    HIPCHK(ctx, hipMemcpyAsync(d->fold_off, d->hbox + mb::FOFF, off.size() * sizeof(int), hipMemcpyHostToDevice,
-                               d->stream));
-    HIPCHK(ctx, hipMemcpyAsync(d->fold_cols, d->hbox + mb::FCOLS, cols.size() * 2, hipMemcpyHostToDevice, d->stream));
-    fill_u32(d->stream, d->fold_flag, 1);
+    // offsets and lists in one copy (the device buffers keep the mailbox's relative layout)
+    HIPCHK(ctx, hipMemcpyAsync(d->fold_off, d->hbox + mb::FOFF, (mb::FCOLS - mb::FOFF) + cols.size() * 2,
+                               hipMemcpyHostToDevice, d->stream));
+    // the range flag lives in the mailbox: cleared here, set by the kernel, read after the sync
+    volatile uint32_t *flag_host = (volatile uint32_t *)(d->hbox + mb::MISC);
+    *flag_host = 0;
     int cus = 256;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess) cus = 256;
     const unsigned grid = (unsigned)std::min<size_t>(s.ring_elements, (size_t)cus);
@@ -3257,7 +3265,8 @@ extern "C" int lbn_fold(lbn_ctx *ctx, const lbn_challenge *ch, int n_challenges,
 #define FOLD_CASE(B)                                                                                          \
     case B:                                                                                                   \
         k_fold<B><<<grid, FOLD_THREADS, 0, d->stream>>>(d->table, s.column_len, s.ring_elements, d->fold_off,   \
-                                                        d->fold_cols, n_entries, dst, d->fold_flag);          \
+                                                        d->fold_cols, n_entries, dst,                         \
+                                                        (uint32_t *)(d->hbox_dev + mb::MISC));                \
         break;
     switch (std::max(nb, 6)) {
         FOLD_CASE(6)
@@ -3276,10 +3285,8 @@ extern "C" int lbn_fold(lbn_ctx *ctx, const lbn_challenge *ch, int n_challenges,
     // The kernel wrote the fold straight into pinned host memory over the host link, overlapped
     // with its own work; once the stream is idle the values are visible here.
     const uint8_t *stage = (const uint8_t *)d->hfold;
-    HIPCHK(ctx, hipMemcpyAsync(d->hbox + mb::MISC, d->fold_flag, 4, hipMemcpyDeviceToHost, d->stream));
     HIPCHK(ctx, hipStreamSynchronize(d->stream));
-    uint32_t flag;
-    memcpy(&flag, d->hbox + mb::MISC, 4);
+    const uint32_t flag = *flag_host;
     tcollect(ctx, T_FOLD);
     if (flag) return ctx->fail(LBN_E_RANGE, "a fold coefficient does not fit the wire codec");
     if (!direct) copy_parallel((uint8_t *)out, stage, n_out * 4);
@@ -3292,10 +3299,10 @@ int lbn::dev_challenges(lbn_ctx *ctx, const ChalArg &arg, lbn_challenge *out) {
     static_assert(sizeof(ChalArg) <= 8192, "mailbox region");
     memcpy(d->hbox + mb::CH, &arg, sizeof(ChalArg));
     HIPCHK(ctx, hipMemcpyAsync(d->chal, d->hbox + mb::CH, sizeof(ChalArg), hipMemcpyHostToDevice, d->stream));
-    k_challenges<<<COLUMNS / chal::WAVES, 64 * chal::WAVES, 0, d->stream>>>(d->chal, d->chals);
+    // the challenges go straight into the mailbox
+    k_challenges<<<COLUMNS / chal::WAVES, 64 * chal::WAVES, 0, d->stream>>>(
+        d->chal, (lbn_challenge *)(d->hbox_dev + mb::CHOUT));
     HIPCHK(ctx, hipGetLastError());
-    HIPCHK(ctx, hipMemcpyAsync(d->hbox + mb::CHOUT, d->chals, COLUMNS * sizeof(lbn_challenge), hipMemcpyDeviceToHost,
-                               d->stream));
     HIPCHK(ctx, hipStreamSynchronize(d->stream));
     memcpy(out, d->hbox + mb::CHOUT, COLUMNS * sizeof(lbn_challenge));
     return LBN_OK;

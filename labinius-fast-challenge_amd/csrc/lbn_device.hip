@@ -44,7 +44,7 @@ struct lbn::DeviceState {
     uint4 *table = nullptr;
     uint4 *E[2] = {nullptr, nullptr};
     uint32_t *P[2] = {nullptr, nullptr};
-    uint32_t *T = nullptr;  // [0,768): T0, [768,1536): Tinf, [1536, 2048): partials, [2048, 3584): odd rounds
+    uint32_t *T = nullptr;  // [0,768): T0, [768,1536): Tinf, [1536,2048): partials, [2048,3584): odd rounds
     uint4 *EL = nullptr, *EH = nullptr;  // EH = EL + 4096 (one allocation)
     uint32_t *eqrow = nullptr, *rowU = nullptr, *eqlo = nullptr;
     int *fold_off = nullptr;
@@ -67,6 +67,7 @@ struct lbn::DeviceState {
     uint8_t *hbox_dev = nullptr;    // its device address: kernels write small results straight into it
     bool msg_clean = false;         // the switch message sums are zero (cleared with the partials)
     uint32_t *hcvs = nullptr;       // pinned: chunk chaining values of the commitment
+    uint32_t *hcvs_dev = nullptr;   // its device address
     struct SwitchDev *sw = nullptr; // the switch rounds' device state (transcript, L-map, outputs)
     ChalArg *chal = nullptr;        // the challenge sampler's input (its output goes to the mailbox)
     int32_t *fold_host = nullptr;   // device address of hfold
@@ -998,13 +999,31 @@ __device__ __forceinline__ void switch_state_store(const CtlKeep &K, SwitchDev *
     for (int i = t; i < 384; i += blockDim.x) sd->lmap[i / 3][i % 3] = K.L[i / 3][i % 3];
 }
 
-__global__ __launch_bounds__(1024) void k_switch_ctl(SwitchDev *sd, int r, int lp, int V, const uint32_t *T,
-                                                     const uint32_t *pair) {
+// The first step takes the whole state from the host's pinned copy (`host`) and the last one returns
+// it there, so the rounds need no copies around them.
+__global__ __launch_bounds__(1024) void k_switch_ctl(SwitchDev *sd, SwitchDev *host, int r, int lp, int V,
+                                                     const uint32_t *T, const uint32_t *pair) {
     __shared__ CtlKeep K;
     __shared__ CtlTemp X;
+    static_assert(sizeof(SwitchDev) % 4 == 0, "state copied as words");
+    constexpr int SW_WORDS = (int)(sizeof(SwitchDev) / 4);
+    if (r == 0) {
+        const volatile uint32_t *src = (const volatile uint32_t *)host;
+        uint32_t *dst = (uint32_t *)sd;
+        for (int i = threadIdx.x; i < SW_WORDS; i += blockDim.x) dst[i] = src[i];
+        __threadfence_block();
+        __syncthreads();
+    }
     switch_state_load(K, sd);
     switch_ctl_step(K, X, sd, r, lp, V, T, pair);
     switch_state_store(K, sd);
+    if (r == lp - 1) {
+        __threadfence_block();
+        __syncthreads();
+        const uint32_t *src = (const uint32_t *)sd;
+        volatile uint32_t *dst = (volatile uint32_t *)host;
+        for (int i = threadIdx.x; i < SW_WORDS; i += blockDim.x) dst[i] = src[i];
+    }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -2039,10 +2058,17 @@ struct CvTrees {
 };
 
 template <bool IN_LDS>
-__global__ __launch_bounds__(1024) void k_cv_tree(uint32_t *cvs, int count, CvTrees tr, B3Key key, int l0) {
+// The subtree roots go to `roots` (the host's pinned buffer), and the commitment bytes before and
+// after the hashed chunks (cb[0, head), cb[tail, tail + tail_len)) to head_out / tail_out (the
+// host's mailbox), so the host needs no copies after the kernel.
+__global__ __launch_bounds__(1024) void k_cv_tree(uint32_t *cvs, int count, CvTrees tr, B3Key key, int l0,
+                                                  uint32_t *roots, const uint8_t *cb, int head, size_t tail,
+                                                  int tail_len, uint8_t *head_out, uint8_t *tail_out) {
     extern __shared__ __align__(16) uint32_t s_cv[];
     uint32_t *base = IN_LDS ? s_cv : cvs;
     const int t = threadIdx.x, q = t & 3, quad = t >> 2;
+    for (int i = t; i < head; i += blockDim.x) head_out[i] = cb[i];
+    for (int i = t; i < tail_len; i += blockDim.x) tail_out[i] = cb[tail + i];
     if (IN_LDS) {  // 16-byte copies, several in flight per thread
         const uint4 *src = (const uint4 *)cvs;
         uint4 *dst = (uint4 *)s_cv;
@@ -2096,10 +2122,9 @@ __global__ __launch_bounds__(1024) void k_cv_tree(uint32_t *cvs, int count, CvTr
     if (t < tr.n)
 #pragma unroll
         for (int k = 0; k < 8; k++) v[k] = base[8 * (size_t)tr.start[t] + k];
-    __syncthreads();
     if (t < tr.n)
 #pragma unroll
-        for (int k = 0; k < 8; k++) cvs[8 * t + k] = v[k];
+        for (int k = 0; k < 8; k++) roots[8 * t + k] = v[k];
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -2707,6 +2732,7 @@ static int alloc_commit_buffers(lbn_ctx *ctx, int np, int width) {
     if (d->hstage) HIPCHK(ctx, hipHostFree(d->hstage));
     if (d->hcvs) HIPCHK(ctx, hipHostFree(d->hcvs));
     d->cacc = nullptr, d->cbytes = nullptr, d->cvs = nullptr, d->hstage = nullptr, d->hcvs = nullptr;
+    d->hcvs_dev = nullptr;
     d->alloc_primes = d->alloc_width = 0;
     const size_t clen = (size_t)COLUMNS * N * width;
     d->splits = mac_splits(ctx->shape.ring_elements);
@@ -2715,6 +2741,7 @@ static int alloc_commit_buffers(lbn_ctx *ctx, int np, int width) {
     HIPCHK(ctx, hipMalloc(&d->cvs, clen / 1024 * 32 + 64));
     HIPCHK(ctx, hipHostMalloc((void **)&d->hstage, clen));
     HIPCHK(ctx, hipHostMalloc((void **)&d->hcvs, clen / 1024 * 32 + 64));
+    HIPCHK(ctx, hipHostGetDevicePointer((void **)&d->hcvs_dev, d->hcvs, 0));
     d->alloc_primes = np;
     d->alloc_width = width;
     return LBN_OK;
@@ -2832,6 +2859,7 @@ int lbn::dev_commit_cvs(lbn_ctx *ctx, size_t off, uint64_t first, size_t count, 
     HIPCHK(ctx, hipMemcpyAsync(d->staged_direct ? dst : d->hstage, d->cbytes, clen, hipMemcpyDeviceToHost,
                                d->copy_stream));
     HIPCHK(ctx, hipEventRecord(d->ev_staged, d->copy_stream));
+    if (head > 1024 || clen - tail > 1024) return ctx->fail(LBN_E_ARG, "commitment head or tail above one chunk");
     if (count) {
         B3Key k;
         for (int i = 0; i < 8; i++) k.k[i] = key[i];
@@ -2839,17 +2867,22 @@ int lbn::dev_commit_cvs(lbn_ctx *ctx, size_t off, uint64_t first, size_t count, 
         const unsigned windows = (unsigned)((first + count - 1) / 64 - first / 64 + 1);
         k_chunk_cvs<<<windows, 64, 0, d->stream>>>(d->cbytes, off, first, count, k, d->cvs);
         HIPCHK(ctx, hipGetLastError());
+        // the roots, head and tail go straight to pinned host memory
+        const int tail_len = (int)(clen - tail);
+        uint8_t *ho = d->hbox_dev + mb::HEAD, *to = d->hbox_dev + mb::TAIL;
         if (count * 32 <= CV_TREE_LDS_MAX)
-            k_cv_tree<true><<<1, 1024, count * 32, d->stream>>>(d->cvs, (int)count, tr, k, CV_WIN_LEVELS);
+            k_cv_tree<true><<<1, 1024, count * 32, d->stream>>>(d->cvs, (int)count, tr, k, CV_WIN_LEVELS, d->hcvs_dev,
+                                                                d->cbytes, (int)head, tail, tail_len, ho, to);
         else
-            k_cv_tree<false><<<1, 1024, 0, d->stream>>>(d->cvs, (int)count, tr, k, CV_WIN_LEVELS);
+            k_cv_tree<false><<<1, 1024, 0, d->stream>>>(d->cvs, (int)count, tr, k, CV_WIN_LEVELS, d->hcvs_dev,
+                                                        d->cbytes, (int)head, tail, tail_len, ho, to);
         HIPCHK(ctx, hipGetLastError());
-        HIPCHK(ctx, hipMemcpyAsync(d->hcvs, d->cvs, trees.size() * 32, hipMemcpyDeviceToHost, d->stream));
+    } else {
+        if (head) HIPCHK(ctx, hipMemcpyAsync(d->hbox + mb::HEAD, d->cbytes, head, hipMemcpyDeviceToHost, d->stream));
+        if (tail < clen)
+            HIPCHK(ctx, hipMemcpyAsync(d->hbox + mb::TAIL, d->cbytes + tail, clen - tail, hipMemcpyDeviceToHost,
+                                       d->stream));
     }
-    if (head > 1024 || clen - tail > 1024) return ctx->fail(LBN_E_ARG, "commitment head or tail above one chunk");
-    if (head) HIPCHK(ctx, hipMemcpyAsync(d->hbox + mb::HEAD, d->cbytes, head, hipMemcpyDeviceToHost, d->stream));
-    if (tail < clen)
-        HIPCHK(ctx, hipMemcpyAsync(d->hbox + mb::TAIL, d->cbytes + tail, clen - tail, hipMemcpyDeviceToHost, d->stream));
     HIPCHK(ctx, hipStreamSynchronize(d->stream));
     memcpy(cvs, d->hcvs, trees.size() * 32);
     memcpy(dst, d->hbox + mb::HEAD, head);
@@ -3050,7 +3083,8 @@ static void launch_switch(lbn_ctx *ctx) {
     const int lp = ctx->shape.log_packed;
     for (int r = 0; r < lp; r++) {
         const uint32_t *pair = r == lp - 1 ? d->P[(r - 1) & 1] : nullptr;  // log_packed >= 12
-        k_switch_ctl<<<1, 1024, 0, d->stream>>>(d->sw, r, lp, r == 0 ? 4 : 6, msg_buf(d, r), pair);
+        k_switch_ctl<<<1, 1024, 0, d->stream>>>(d->sw, (SwitchDev *)(d->hbox_dev + mb::SW), r, lp, r == 0 ? 4 : 6,
+                                                msg_buf(d, r), pair);
         if (r < lp - 1) launch_round(ctx, r, RhoArg{{0, 0, 0}}, d->sw->rho);
     }
 }
@@ -3081,7 +3115,7 @@ int lbn::dev_switch_rounds(lbn_ctx *ctx, B3State &ts, uint64_t *rounds, uint64_t
     h->ts = ts;
     memcpy(h->lmap, ctx->lmap.data(), 128 * sizeof(F162));
     for (int r = 0; r < lp; r++) h->r_hi[r][0] = ctx->r_hi[r].w[0], h->r_hi[r][1] = ctx->r_hi[r].w[1];
-    HIPCHK(ctx, hipMemcpyAsync(d->sw, h, sizeof(SwitchDev), hipMemcpyHostToDevice, d->stream));
+    // the first control step reads the state from the mailbox, the last writes it back there
     tstart(ctx, T_ROUNDS);
     if (d->graphs)
         HIPCHK(ctx, hipGraphLaunch(d->switch_graph, d->stream));
@@ -3089,7 +3123,6 @@ int lbn::dev_switch_rounds(lbn_ctx *ctx, B3State &ts, uint64_t *rounds, uint64_t
         launch_switch(ctx);
     tstop(ctx, T_ROUNDS);
     HIPCHK(ctx, hipGetLastError());
-    HIPCHK(ctx, hipMemcpyAsync(h, d->sw, sizeof(SwitchDev), hipMemcpyDeviceToHost, d->stream));
     HIPCHK(ctx, hipStreamSynchronize(d->stream));
     tcollect(ctx, T_SWITCH0);
     tcollect(ctx, T_ROUNDS);

@@ -46,7 +46,7 @@ struct lbn::DeviceState {
     uint32_t *P[2] = {nullptr, nullptr};
     uint32_t *T = nullptr;  // [0,768): T0, [768,1536): Tinf, [1536,2048): partials, [2048,3584): odd rounds
     uint4 *EL = nullptr, *EH = nullptr;  // EH = EL + 4096 (one allocation)
-    uint32_t *eqrow = nullptr, *rowU = nullptr, *eqlo = nullptr;
+    uint32_t *eqrow = nullptr, *eqlo = nullptr;
     int *fold_off = nullptr;
     uint16_t *fold_cols = nullptr;  // inside fold_off's allocation
     uint32_t *ntt_tabs = nullptr;   // MAX_PRIMES * tab::SIZE
@@ -1042,8 +1042,12 @@ __global__ void k_eq_row(const uint64_t *lo, const uint64_t *hi, int lo_bits, si
     o[4] = (uint32_t)r[2], o[5] = (uint32_t)(r[2] >> 32);
 }
 
-__global__ __launch_bounds__(256) void k_row(const uint4 *table, const uint32_t *eqrow, size_t clen, uint32_t *U) {
+// Block c: U_b = sum_i bit_b(w_{c,i}) eq_i for b < 128 (M4RM over the column), then in the same block
+// row[c] = sum_b x^b U_b (mod x^162 + x^81 + 1), one b per thread and an XOR reduction.
+__global__ __launch_bounds__(256) void k_row(const uint4 *table, const uint32_t *eqrow, size_t clen, uint64_t *row) {
     __shared__ __align__(16) uint32_t lds[4 * M4W<6>::WORDS];
+    __shared__ uint32_t s_u[128 * 6];
+    __shared__ uint64_t part[128 * 3];
     uint32_t *W = lds + (threadIdx.x >> 6) * M4W<6>::WORDS;
     uint32_t acc[2][6] = {};
     const size_t c = blockIdx.x;
@@ -1067,34 +1071,32 @@ __global__ __launch_bounds__(256) void k_row(const uint4 *table, const uint32_t 
         m4w_step<6>(acc, sel[0], val[0], W);
         if (base + 256 < clen) m4w_step<6>(acc, sel[1], val[1], W);
     }
-    m4w_flush<6, false>(acc, lds, U + c * 128 * 6);
-}
-
-// row[c] = sum_b x^b U_b: 128 threads per column, one b each, then an XOR reduction.
-__global__ __launch_bounds__(128) void k_row_finish(const uint32_t *U, uint64_t *row) {
-    __shared__ uint64_t part[128 * 3];
-    const int b = threadIdx.x;
-    const uint32_t *u = U + ((size_t)blockIdx.x * 128 + b) * 6;
-    uint64_t x[3] = {(uint64_t)u[0] | ((uint64_t)u[1] << 32), (uint64_t)u[2] | ((uint64_t)u[3] << 32),
-                     (uint64_t)u[4] | ((uint64_t)u[5] << 32)};
-    // x << b into five words (b < 128)
-    uint64_t v[5] = {0, 0, 0, 0, 0};
-    const int wsh = b >> 6, bsh = b & 63;
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-        v[k + wsh] ^= x[k] << bsh;
-        if (bsh) v[k + wsh + 1] ^= x[k] >> (64 - bsh);
-    }
-    uint64_t r[3];
-    f162_reduce5(v, r);
-    part[3 * b] = r[0], part[3 * b + 1] = r[1], part[3 * b + 2] = r[2];
+    m4w_flush<6, false>(acc, lds, s_u);  // U_b of this column into LDS
     __syncthreads();
-    for (int s = 64; s > 0; s >>= 1) {
-        if (b < s)
-            for (int k = 0; k < 3; k++) part[3 * b + k] ^= part[3 * (b + s) + k];
+    const int b = threadIdx.x;
+    if (b < 128) {
+        const uint32_t *u = s_u + b * 6;
+        const uint64_t x[3] = {(uint64_t)u[0] | ((uint64_t)u[1] << 32), (uint64_t)u[2] | ((uint64_t)u[3] << 32),
+                               (uint64_t)u[4] | ((uint64_t)u[5] << 32)};
+        // x << b into five words (b < 128)
+        uint64_t v[5] = {0, 0, 0, 0, 0};
+        const int wsh = b >> 6, bsh = b & 63;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            v[k + wsh] ^= x[k] << bsh;
+            if (bsh) v[k + wsh + 1] ^= x[k] >> (64 - bsh);
+        }
+        uint64_t r[3];
+        f162_reduce5(v, r);
+        part[3 * b] = r[0], part[3 * b + 1] = r[1], part[3 * b + 2] = r[2];
+    }
+    __syncthreads();
+    for (int st = 64; st > 0; st >>= 1) {
+        if (b < st)
+            for (int k = 0; k < 3; k++) part[3 * b + k] ^= part[3 * (b + st) + k];
         __syncthreads();
     }
-    if (b < 3) row[(size_t)blockIdx.x * 3 + b] = part[b];
+    if (b < 3) row[c * 3 + b] = part[b];
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -2449,7 +2451,6 @@ int lbn::dev_init(lbn_ctx *ctx) {
     d->EH = d->EL + 4096;
     HIPCHK(ctx, hipMalloc(&d->eqrow, s.column_len * 24));
     HIPCHK(ctx, hipMalloc(&d->eqlo, 256 * 24));
-    HIPCHK(ctx, hipMalloc(&d->rowU, (size_t)COLUMNS * 128 * 24));
     // the fold's offsets then its column lists, at the mailbox's relative offset (one copy per proof)
     static_assert((2 * 162 + 1 + 162) * sizeof(int) <= mb::FCOLS - mb::FOFF, "fold offsets region");
     HIPCHK(ctx, hipMalloc(&d->fold_off, (mb::FCOLS - mb::FOFF) + (size_t)COLUMNS * 32 * sizeof(uint16_t)));
@@ -2480,7 +2481,7 @@ void lbn::dev_free(lbn_ctx *ctx) {
     if (!d) return;
     hipSetDevice(ctx->device);
     void *ptrs[] = {d->table, d->E[0], d->E[1], d->P[0], d->P[1], d->T, d->EL, d->eqrow, d->eqlo,
-                    d->rowU, d->fold_off, d->ntt_tabs, d->ahat, d->cacc, d->cbytes, d->amat, d->cvs, d->sw,
+                    d->fold_off, d->ntt_tabs, d->ahat, d->cacc, d->cbytes, d->amat, d->cvs, d->sw,
                     d->chal};
     if (d->stream) hipStreamSynchronize(d->stream);
     if (d->copy_stream) hipStreamSynchronize(d->copy_stream);
@@ -3225,9 +3226,8 @@ extern "C" int lbn_row_eval(lbn_ctx *ctx, const uint64_t *p0, int n_p0, uint64_t
     k_eq_row<<<(unsigned)((s.column_len + 255) / 256), 256, 0, d->stream>>>(
         (const uint64_t *)d->eqlo, (const uint64_t *)((uint8_t *)d->eqlo + 128 * 24), lo_bits, s.column_len, d->eqrow);
     HIPCHK(ctx, hipGetLastError());
-    k_row<<<COLUMNS, 256, 0, d->stream>>>(d->table, d->eqrow, s.column_len, d->rowU);
-    HIPCHK(ctx, hipGetLastError());
-    k_row_finish<<<COLUMNS, 128, 0, d->stream>>>(d->rowU, (uint64_t *)(d->hbox_dev + mb::ROW));  // into the mailbox
+    k_row<<<COLUMNS, 256, 0, d->stream>>>(d->table, d->eqrow, s.column_len,
+                                          (uint64_t *)(d->hbox_dev + mb::ROW));  // into the mailbox
     tstop(ctx, T_ROW);
     HIPCHK(ctx, hipGetLastError());
     HIPCHK(ctx, hipStreamSynchronize(d->stream));

@@ -103,7 +103,9 @@ constexpr size_t TAIL = HEAD + 1024;      // and after them (<= 1 KiB)
 constexpr size_t SW = TAIL + 1024;        // the device switch state (SwitchDev, < 8 KiB)
 constexpr size_t CH = SW + 8192;          // the challenge sampler's input (ChalArg, < 8 KiB)
 constexpr size_t CHOUT = CH + 8192;       // the sampled challenges (1024 lbn_challenge)
-constexpr size_t SIZE = CHOUT + 1024 * 40;
+constexpr size_t HP = CHOUT + 1024 * 40;  // the switch handoff: the host's first fold input (128 F162)
+constexpr size_t HE = HP + 128 * 24;      // and its eq values (64 B128)
+constexpr size_t SIZE = HE + 64 * 16;
 }  // namespace mb
 
 // Timed passes (indices into lbn_timing's fields, in order).
@@ -999,10 +1001,13 @@ __device__ __forceinline__ void switch_state_store(const CtlKeep &K, SwitchDev *
     for (int i = t; i < 384; i += blockDim.x) sd->lmap[i / 3][i % 3] = K.L[i / 3][i % 3];
 }
 
-// The first step takes the whole state from the host's pinned copy (`host`) and the last one returns
-// it there, so the rounds need no copies around them.
-__global__ __launch_bounds__(1024) void k_switch_ctl(SwitchDev *sd, SwitchDev *host, int r, int lp, int V,
-                                                     const uint32_t *T, const uint32_t *pair) {
+// The first step takes the whole state from the host's pinned copy (`host`) and the last device step
+// (r == last) returns it there, with the next fold's input (hp_n F162 words from hp_src, he_n B128
+// from he_src) into hp_dst / he_dst, so the rounds need no copies around them.
+__global__ __launch_bounds__(1024) void k_switch_ctl(SwitchDev *sd, SwitchDev *host, int r, int lp, int last, int V,
+                                                     const uint32_t *T, const uint32_t *pair, const uint32_t *hp_src,
+                                                     uint32_t *hp_dst, int hp_n, const uint4 *he_src, uint4 *he_dst,
+                                                     int he_n) {
     __shared__ CtlKeep K;
     __shared__ CtlTemp X;
     static_assert(sizeof(SwitchDev) % 4 == 0, "state copied as words");
@@ -1017,12 +1022,14 @@ __global__ __launch_bounds__(1024) void k_switch_ctl(SwitchDev *sd, SwitchDev *h
     switch_state_load(K, sd);
     switch_ctl_step(K, X, sd, r, lp, V, T, pair);
     switch_state_store(K, sd);
-    if (r == lp - 1) {
+    if (r == last) {
         __threadfence_block();
         __syncthreads();
         const uint32_t *src = (const uint32_t *)sd;
         volatile uint32_t *dst = (volatile uint32_t *)host;
         for (int i = threadIdx.x; i < SW_WORDS; i += blockDim.x) dst[i] = src[i];
+        for (int i = threadIdx.x; i < 6 * hp_n; i += blockDim.x) hp_dst[i] = hp_src[i];
+        for (int i = threadIdx.x; i < he_n; i += blockDim.x) he_dst[i] = he_src[i];
     }
 }
 
@@ -3077,16 +3084,23 @@ static void launch_round(lbn_ctx *ctx, int r, RhoArg ra, const uint64_t *rho_dev
                                                                          T, T + 768, msg_buf(d, r), rho_dev);
 }
 
-// The switch rounds' launches: control step r, then round r's fold and round r + 1's message.
-// Everything they read (shape, buffers, the device state d->sw) is fixed per context.
+// The switch rounds' device launches: control step r, then round r's fold and round r + 1's message,
+// for r < r0 = lp - SWITCH_HOST_STEPS (control steps) and r < r0 - 1 (folds); control step r0 - 1 hands
+// the state and round r0 - 1's fold input (written by round r0 - 2) to the host. Everything they read
+// (shape, buffers, the device state d->sw) is fixed per context.
+static int switch_host_from(int lp) { return lp - SWITCH_HOST_STEPS; }
 static void launch_switch(lbn_ctx *ctx) {
     DeviceState *d = ctx->dev;
-    const int lp = ctx->shape.log_packed;
-    for (int r = 0; r < lp; r++) {
-        const uint32_t *pair = r == lp - 1 ? d->P[(r - 1) & 1] : nullptr;  // log_packed >= 12
-        k_switch_ctl<<<1, 1024, 0, d->stream>>>(d->sw, (SwitchDev *)(d->hbox_dev + mb::SW), r, lp, r == 0 ? 4 : 6,
-                                                msg_buf(d, r), pair);
-        if (r < lp - 1) launch_round(ctx, r, RhoArg{{0, 0, 0}}, d->sw->rho);
+    const int lp = ctx->shape.log_packed, r0 = switch_host_from(lp);
+    const int h2 = 1 << (lp - r0 - 1);  // the handoff fold's items
+    static_assert(4 * 32 * 24 <= mb::HE - mb::HP && 2 * 32 * 16 <= mb::SIZE - mb::HE, "handoff regions");
+    for (int r = 0; r < r0; r++) {
+        const bool hand = r == r0 - 1;  // r0 - 2 >= 1 (log_packed >= 12): the input is a P buffer
+        k_switch_ctl<<<1, 1024, 0, d->stream>>>(
+            d->sw, (SwitchDev *)(d->hbox_dev + mb::SW), r, lp, r0 - 1, r == 0 ? 4 : 6, msg_buf(d, r), nullptr,
+            hand ? d->P[(r0 - 2) & 1] : nullptr, (uint32_t *)(d->hbox_dev + mb::HP), hand ? 4 * h2 : 0,
+            hand ? d->E[r0 & 1] : nullptr, (uint4 *)(d->hbox_dev + mb::HE), hand ? 2 * h2 : 0);
+        if (r < r0 - 1) launch_round(ctx, r, RhoArg{{0, 0, 0}}, d->sw->rho);
     }
 }
 
@@ -3106,9 +3120,9 @@ static int capture_switch(lbn_ctx *ctx) {
     return LBN_OK;
 }
 
-int lbn::dev_switch_rounds(lbn_ctx *ctx, B3State &ts, uint64_t *rounds, uint64_t *coords, uint64_t *opened) {
+int lbn::dev_switch_rounds(lbn_ctx *ctx, B3State &ts, uint64_t *rounds, uint64_t *coords, SwitchHandoff &ho) {
     if (ctx->round != 0) return ctx->fail(LBN_E_STATE, "lbn_switch_begin must run first");
-    const int lp = ctx->shape.log_packed;
+    const int lp = ctx->shape.log_packed, r0 = switch_host_from(lp);
     HIPCHK(ctx, hipSetDevice(ctx->device));
     DeviceState *d = ctx->dev;
     SwitchDev *h = (SwitchDev *)(d->hbox + mb::SW);
@@ -3128,36 +3142,23 @@ int lbn::dev_switch_rounds(lbn_ctx *ctx, B3State &ts, uint64_t *rounds, uint64_t
     tcollect(ctx, T_SWITCH0);
     tcollect(ctx, T_ROUNDS);
     ts = h->ts;
-    memcpy(rounds, h->rounds, (size_t)lp * 48);
-    memcpy(coords, h->coords, (size_t)lp * 24);
-    memcpy(opened, h->opened, 24);
-    memcpy(ctx->opened.w, h->opened, 24);
-    ctx->round = lp;
+    memcpy(rounds, h->rounds, (size_t)r0 * 48);
+    memcpy(coords, h->coords, (size_t)r0 * 24);
+    const size_t h2 = (size_t)1 << (lp - r0 - 1);
+    ho.r0 = r0;
+    memcpy(ho.rho.w, h->rho, 24);
+    ho.L.assign((const F162 *)h->lmap, (const F162 *)h->lmap + 128);
+    ho.p.resize(4 * h2);
+    memcpy(ho.p.data(), d->hbox + mb::HP, 4 * h2 * sizeof(F162));
+    ho.e.resize(2 * h2);
+    memcpy(ho.e.data(), d->hbox + mb::HE, 2 * h2 * sizeof(B128));
+    ctx->round = lp;  // the host finishes the rounds (and sets ctx->opened)
     return LBN_OK;
 }
 
 // L_{r+1}(y) = L_r((1 + c) y) + rho L_r(y), c = r_hi[r]: the images of x^k, k < 128, through
 // 16 tables of 256 sums of L_r over 8-bit windows of y (method of four Russians).
-static void update_lmap(lbn_ctx *ctx, int r, const F162 &rho) {
-    const std::vector<F162> &L = ctx->lmap;
-    std::vector<F162> tab(16 * 256);
-    for (int w = 0; w < 16; w++) {
-        F162 *T = tab.data() + 256 * w;
-        T[0] = F162{{0, 0, 0}};
-        for (int b = 1; b < 256; b++) T[b] = f162_add(T[b & (b - 1)], L[8 * w + __builtin_ctz(b)]);
-    }
-    std::vector<F162> next(128);
-    B128 z = b128_add(ctx->r_hi[r], B128{{1, 0}});  // (1 + c) x^k
-    for (int k = 0; k < 128; k++) {
-        F162 acc{{0, 0, 0}};
-        for (int w = 0; w < 16; w++) acc = f162_add(acc, tab[256 * w + ((z.w[w >> 3] >> (8 * (w & 7))) & 255)]);
-        next[k] = f162_add(acc, f162_mul(rho, L[k]));
-        const uint64_t carry = z.w[1] >> 63;  // z *= x
-        z.w[1] = (z.w[1] << 1) | (z.w[0] >> 63);
-        z.w[0] = (z.w[0] << 1) ^ (carry ? 0x87ull : 0ull);
-    }
-    ctx->lmap.swap(next);
-}
+static void update_lmap(lbn_ctx *ctx, int r, const F162 &rho) { lmap_next(ctx->lmap, ctx->r_hi[r], rho); }
 
 extern "C" int lbn_switch_fold(lbn_ctx *ctx, const uint64_t *rr) {
     if (!ctx || !rr) return LBN_E_ARG;

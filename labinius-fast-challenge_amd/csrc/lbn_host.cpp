@@ -264,6 +264,80 @@ class Transcript {
     uint64_t counter_ = 0;
 };
 
+// L_{r+1}(y) = L_r((1 + c) y) + rho L_r(y): the images of x^k, k < 128, through 16 tables of 256
+// sums of L_r over 8-bit windows of y (method of four Russians).
+void lmap_next(std::vector<F162> &L, B128 c, const F162 &rho) {
+    std::vector<F162> tab(16 * 256);
+    for (int w = 0; w < 16; w++) {
+        F162 *T = tab.data() + 256 * w;
+        T[0] = F162{{0, 0, 0}};
+        for (int b = 1; b < 256; b++) T[b] = f162_add(T[b & (b - 1)], L[8 * w + __builtin_ctz(b)]);
+    }
+    std::vector<F162> next(128);
+    B128 z = b128_add(c, B128{{1, 0}});  // (1 + c) x^k
+    for (int k = 0; k < 128; k++) {
+        F162 acc{{0, 0, 0}};
+        for (int w = 0; w < 16; w++) acc = f162_add(acc, tab[256 * w + ((z.w[w >> 3] >> (8 * (w & 7))) & 255)]);
+        next[k] = f162_add(acc, f162_mul(rho, L[k]));
+        const uint64_t carry = z.w[1] >> 63;  // z *= x
+        z.w[1] = (z.w[1] << 1) | (z.w[0] >> 63);
+        z.w[0] = (z.w[0] << 1) ^ (carry ? 0x87ull : 0ull);
+    }
+    L.swap(next);
+}
+
+// The last switch rounds on the host (sumcheck.rs:89-139 driven as main.rs:975-992), from where the
+// device stopped: round r's fold of p by rho (p'_j = p_j + rho (p_j + p_{j+h}), j < h = |p| / 2) with
+// the message sums of round r + 1 (T0_k = sum of p'_j over the j < h2 whose eq value has bit k,
+// Tinf_k = sum of p'_j + p'_{j+h2} over the bits of E_j + E_{j+h2}), then control step r + 1: the
+// message (sum_k L[k] T_k), absorb, sample rho, update L, and after the last one the opened value
+// p0 + rho (p0 + p1), absorbed. The same steps as the device's round and control kernels.
+static void switch_host_tail(lbn_ctx *ctx, Transcript &t, SwitchHandoff &ho, F162 *rounds, F162 *coords) {
+    const int lp = ctx->shape.log_packed;
+    std::vector<F162> p = std::move(ho.p), q;
+    std::vector<B128> e = std::move(ho.e), e2;
+    std::vector<F162> &L = ho.L;
+    F162 rho = ho.rho;
+    for (int r = ho.r0 - 1; r < lp - 1; r++) {
+        const size_t h2 = p.size() / 4, h = 2 * h2;
+        q.assign(h, F162{{0, 0, 0}});
+        e2.assign(h2, B128{{0, 0}});
+        F162 T0[128], Ti[128];
+        for (int k = 0; k < 128; k++) T0[k] = Ti[k] = F162{{0, 0, 0}};
+        for (size_t j = 0; j < h2; j++) {
+            const F162 q0 = f162_add(p[j], f162_mul(rho, f162_add(p[j], p[j + h])));
+            const F162 q1 = f162_add(p[j + h2], f162_mul(rho, f162_add(p[j + h2], p[j + h + h2])));
+            q[j] = q0, q[j + h2] = q1;
+            const B128 ej = e[j], ei = b128_add(e[j], e[j + h2]);
+            e2[j] = ei;
+            const F162 dq = f162_add(q0, q1);
+            for (int k = 0; k < 128; k++) {
+                if ((ej.w[k >> 6] >> (k & 63)) & 1) T0[k] = f162_add(T0[k], q0);
+                if ((ei.w[k >> 6] >> (k & 63)) & 1) Ti[k] = f162_add(Ti[k], dq);
+            }
+        }
+        p.swap(q);
+        e.swap(e2);
+        const int s = r + 1;  // control step s
+        F162 msg[2] = {F162{{0, 0, 0}}, F162{{0, 0, 0}}};
+        for (int k = 0; k < 128; k++) {
+            msg[0] = f162_add(msg[0], f162_mul(L[k], T0[k]));
+            msg[1] = f162_add(msg[1], f162_mul(L[k], Ti[k]));
+        }
+        rounds[2 * s] = msg[0], rounds[2 * s + 1] = msg[1];
+        t.absorb_f162(msg, 2);
+        rho = t.sample_f162("switch-round", 1)[0];
+        coords[s] = rho;
+        if (s == lp - 1) {
+            const F162 opened = f162_add(p[0], f162_mul(rho, f162_add(p[0], p[1])));
+            ctx->opened = opened;
+            t.absorb_f162(&opened, 1);
+        } else {
+            lmap_next(L, ctx->r_hi[s], rho);
+        }
+    }
+}
+
 // ============================================================================= short challenges
 // pcs/src/challenge.rs: Fisher-Yates attempt 662-688 over one XOF stream, keyed sign word
 // 429-441, rejection by max_u |c(zeta^u)|^2 <= 144 + 1e-12 in f64 with the reference's order of
@@ -559,8 +633,11 @@ extern "C" int lbn_prove(lbn_ctx *ctx, const uint8_t seed[32], uint8_t *proof, s
     {
         B3State ts;
         t.save(ts);
-        TRY(dev_switch_rounds(ctx, ts, (uint64_t *)rounds.data(), (uint64_t *)coords.data(), opened.w));
+        SwitchHandoff ho;
+        TRY(dev_switch_rounds(ctx, ts, (uint64_t *)rounds.data(), (uint64_t *)coords.data(), ho));
         t.load(ts);
+        switch_host_tail(ctx, t, ho, rounds.data(), coords.data());
+        opened = ctx->opened;
     }
     const double t4 = now_us();
 

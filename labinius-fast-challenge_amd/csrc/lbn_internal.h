@@ -148,10 +148,22 @@ int dev_commit_launch(lbn_ctx *ctx);
 int dev_commit_cvs(lbn_ctx *ctx, size_t off, uint64_t first, size_t count, const Blake3::Subtrees &trees,
                    const uint32_t key[8], uint32_t flags, uint32_t *cvs, uint8_t *dst, size_t head, size_t tail);
 int dev_commit_bytes(lbn_ctx *ctx, uint8_t *dst);
-// Every switch round on the device after lbn_switch_begin: the transcript (state ts in, advanced
-// state out) absorbs each message and samples each challenge there. rounds: lp x 2 F162,
-// coords: lp F162, opened: F162 (as 3 u64 each).
-int dev_switch_rounds(lbn_ctx *ctx, B3State &ts, uint64_t *rounds, uint64_t *coords, uint64_t *opened);
+// The switch rounds after lbn_switch_begin, all but the last few on the device: control steps
+// 0 .. r0 - 1 and the folds of rounds 0 .. r0 - 2, with the transcript (state ts in, advanced state
+// out) absorbing each message and sampling each challenge there. The last SWITCH_HOST_STEPS control
+// steps and the folds before them (at most 32 items each) cost less on the host than their launches;
+// the handoff carries what they start from. rounds: r0 x 2 F162, coords: r0 F162 (3 u64 each).
+constexpr int SWITCH_HOST_STEPS = 6;
+struct SwitchHandoff {
+    int r0 = 0;              // first control step left to the host
+    F162 rho{};              // the challenge of round r0 - 1
+    std::vector<F162> L;     // the linear map L_{r0} (128 entries)
+    std::vector<F162> p;     // the input of round r0 - 1's fold: 4 h2 values, h2 = 2^(lp - r0 - 1)
+    std::vector<B128> e;     // and its eq values (2 h2)
+};
+int dev_switch_rounds(lbn_ctx *ctx, B3State &ts, uint64_t *rounds, uint64_t *coords, SwitchHandoff &ho);
+// The map update of switch round r: L_{r+1}[k] = L_r((1 + c) x^k) + rho L_r[k], c = r_hi[r].
+void lmap_next(std::vector<F162> &L, B128 c, const F162 &rho);
 // The 1024 folding challenges sampled on the device from the transcript state in arg (the
 // counter of arg->ts is the first challenge's); out receives them in column order.
 int dev_challenges(lbn_ctx *ctx, const ChalArg &arg, lbn_challenge *out);

@@ -103,9 +103,10 @@ constexpr size_t TAIL = HEAD + 1024;      // and after them (<= 1 KiB)
 constexpr size_t SW = TAIL + 1024;        // the device switch state (SwitchDev, < 8 KiB)
 constexpr size_t CH = SW + 8192;          // the challenge sampler's input (ChalArg, < 8 KiB)
 constexpr size_t CHOUT = CH + 8192;       // the sampled challenges (1024 lbn_challenge)
-constexpr size_t HP = CHOUT + 1024 * 40;  // the switch handoff: the host's first fold input (128 F162)
-constexpr size_t HE = HP + 128 * 24;      // and its eq values (64 B128)
-constexpr size_t SIZE = HE + 64 * 16;
+constexpr size_t HANDOFF_H2 = (size_t)1 << (SWITCH_HOST_STEPS - 1);  // items of the host's first fold
+constexpr size_t HP = CHOUT + 1024 * 40;  // the switch handoff: that fold's input (4 h2 F162)
+constexpr size_t HE = HP + 4 * HANDOFF_H2 * 24;  // and its eq values (2 h2 B128)
+constexpr size_t SIZE = HE + 2 * HANDOFF_H2 * 16;
 }  // namespace mb
 
 // Timed passes (indices into lbn_timing's fields, in order).
@@ -3092,10 +3093,9 @@ static int switch_host_from(int lp) { return lp - SWITCH_HOST_STEPS; }
 static void launch_switch(lbn_ctx *ctx) {
     DeviceState *d = ctx->dev;
     const int lp = ctx->shape.log_packed, r0 = switch_host_from(lp);
-    const int h2 = 1 << (lp - r0 - 1);  // the handoff fold's items
-    static_assert(4 * 32 * 24 <= mb::HE - mb::HP && 2 * 32 * 16 <= mb::SIZE - mb::HE, "handoff regions");
+    const int h2 = 1 << (lp - r0 - 1);  // the handoff fold's items (mb::HANDOFF_H2)
     for (int r = 0; r < r0; r++) {
-        const bool hand = r == r0 - 1;  // r0 - 2 >= 1 (log_packed >= 12): the input is a P buffer
+        const bool hand = r == r0 - 1;  // r0 - 2 >= 3 (log_packed >= 12): the input is a P buffer
         k_switch_ctl<<<1, 1024, 0, d->stream>>>(
             d->sw, (SwitchDev *)(d->hbox_dev + mb::SW), r, lp, r0 - 1, r == 0 ? 4 : 6, msg_buf(d, r), nullptr,
             hand ? d->P[(r0 - 2) & 1] : nullptr, (uint32_t *)(d->hbox_dev + mb::HP), hand ? 4 * h2 : 0,

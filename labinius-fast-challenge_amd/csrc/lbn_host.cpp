@@ -264,21 +264,29 @@ class Transcript {
     uint64_t counter_ = 0;
 };
 
-// L_{r+1}(y) = L_r((1 + c) y) + rho L_r(y): the images of x^k, k < 128, through 16 tables of 256
-// sums of L_r over 8-bit windows of y (method of four Russians).
-void lmap_next(std::vector<F162> &L, B128 c, const F162 &rho) {
-    std::vector<F162> tab(16 * 256);
-    for (int w = 0; w < 16; w++) {
-        F162 *T = tab.data() + 256 * w;
-        T[0] = F162{{0, 0, 0}};
-        for (int b = 1; b < 256; b++) T[b] = f162_add(T[b & (b - 1)], L[8 * w + __builtin_ctz(b)]);
+// The linear map L: GF(2^128) -> F162, y -> sum of L[k] over the set bits k of y, through 32 tables of
+// the 16 sums over a 4-bit window (12 KB, built with one add per entry).
+struct LmapTables {
+    F162 t[32][16];
+    explicit LmapTables(const std::vector<F162> &L) {
+        for (int w = 0; w < 32; w++) {
+            t[w][0] = F162{{0, 0, 0}};
+            for (int b = 1; b < 16; b++) t[w][b] = f162_add(t[w][b & (b - 1)], L[4 * w + __builtin_ctz(b)]);
+        }
     }
+    F162 operator()(B128 y) const {
+        F162 acc{{0, 0, 0}};
+        for (int w = 0; w < 32; w++) acc = f162_add(acc, t[w][(y.w[w >> 4] >> (4 * (w & 15))) & 15]);
+        return acc;
+    }
+};
+
+// L_{r+1}[k] = L_r((1 + c) x^k) + rho L_r[k] for k < 128, from the tables of L_r.
+static void lmap_next(std::vector<F162> &L, const LmapTables &tab, B128 c, const F162 &rho) {
     std::vector<F162> next(128);
     B128 z = b128_add(c, B128{{1, 0}});  // (1 + c) x^k
     for (int k = 0; k < 128; k++) {
-        F162 acc{{0, 0, 0}};
-        for (int w = 0; w < 16; w++) acc = f162_add(acc, tab[256 * w + ((z.w[w >> 3] >> (8 * (w & 7))) & 255)]);
-        next[k] = f162_add(acc, f162_mul(rho, L[k]));
+        next[k] = f162_add(tab(z), f162_mul(rho, L[k]));
         const uint64_t carry = z.w[1] >> 63;  // z *= x
         z.w[1] = (z.w[1] << 1) | (z.w[0] >> 63);
         z.w[0] = (z.w[0] << 1) ^ (carry ? 0x87ull : 0ull);
@@ -286,12 +294,17 @@ void lmap_next(std::vector<F162> &L, B128 c, const F162 &rho) {
     L.swap(next);
 }
 
+void lmap_next(std::vector<F162> &L, B128 c, const F162 &rho) {
+    const LmapTables tab(L);
+    lmap_next(L, tab, c, rho);
+}
+
 // The last switch rounds on the host (sumcheck.rs:89-139 driven as main.rs:975-992), from where the
-// device stopped: round r's fold of p by rho (p'_j = p_j + rho (p_j + p_{j+h}), j < h = |p| / 2) with
-// the message sums of round r + 1 (T0_k = sum of p'_j over the j < h2 whose eq value has bit k,
-// Tinf_k = sum of p'_j + p'_{j+h2} over the bits of E_j + E_{j+h2}), then control step r + 1: the
-// message (sum_k L[k] T_k), absorb, sample rho, update L, and after the last one the opened value
-// p0 + rho (p0 + p1), absorbed. The same steps as the device's round and control kernels.
+// device stopped: round r's fold of p by rho (p'_j = p_j + rho (p_j + p_{j+h}), j < h = |p| / 2), then
+// control step r + 1: the message, absorb, sample rho, update L, and after the last one the opened
+// value p0 + rho (p0 + p1), absorbed. The message of round r + 1 is e0 = sum_k L[k] T0_k with
+// T0_k = sum of p'_j over the j < h2 whose eq value E_j has bit k, that is e0 = sum_j p'_j L(E_j),
+// and einf = sum_j (p'_j + p'_{j+h2}) L(E_j + E_{j+h2}); the device sums per bit first.
 static void switch_host_tail(lbn_ctx *ctx, Transcript &t, SwitchHandoff &ho, F162 *rounds, F162 *coords) {
     const int lp = ctx->shape.log_packed;
     std::vector<F162> p = std::move(ho.p), q;
@@ -299,31 +312,22 @@ static void switch_host_tail(lbn_ctx *ctx, Transcript &t, SwitchHandoff &ho, F16
     std::vector<F162> &L = ho.L;
     F162 rho = ho.rho;
     for (int r = ho.r0 - 1; r < lp - 1; r++) {
+        const int s = r + 1;  // the control step after this fold
+        const LmapTables tab(L);  // of L_s
         const size_t h2 = p.size() / 4, h = 2 * h2;
         q.assign(h, F162{{0, 0, 0}});
         e2.assign(h2, B128{{0, 0}});
-        F162 T0[128], Ti[128];
-        for (int k = 0; k < 128; k++) T0[k] = Ti[k] = F162{{0, 0, 0}};
+        F162 msg[2] = {F162{{0, 0, 0}}, F162{{0, 0, 0}}};
         for (size_t j = 0; j < h2; j++) {
             const F162 q0 = f162_add(p[j], f162_mul(rho, f162_add(p[j], p[j + h])));
             const F162 q1 = f162_add(p[j + h2], f162_mul(rho, f162_add(p[j + h2], p[j + h + h2])));
             q[j] = q0, q[j + h2] = q1;
-            const B128 ej = e[j], ei = b128_add(e[j], e[j + h2]);
-            e2[j] = ei;
-            const F162 dq = f162_add(q0, q1);
-            for (int k = 0; k < 128; k++) {
-                if ((ej.w[k >> 6] >> (k & 63)) & 1) T0[k] = f162_add(T0[k], q0);
-                if ((ei.w[k >> 6] >> (k & 63)) & 1) Ti[k] = f162_add(Ti[k], dq);
-            }
+            e2[j] = b128_add(e[j], e[j + h2]);
+            msg[0] = f162_add(msg[0], f162_mul(q0, tab(e[j])));
+            msg[1] = f162_add(msg[1], f162_mul(f162_add(q0, q1), tab(e2[j])));
         }
         p.swap(q);
         e.swap(e2);
-        const int s = r + 1;  // control step s
-        F162 msg[2] = {F162{{0, 0, 0}}, F162{{0, 0, 0}}};
-        for (int k = 0; k < 128; k++) {
-            msg[0] = f162_add(msg[0], f162_mul(L[k], T0[k]));
-            msg[1] = f162_add(msg[1], f162_mul(L[k], Ti[k]));
-        }
         rounds[2 * s] = msg[0], rounds[2 * s + 1] = msg[1];
         t.absorb_f162(msg, 2);
         rho = t.sample_f162("switch-round", 1)[0];
@@ -333,7 +337,7 @@ static void switch_host_tail(lbn_ctx *ctx, Transcript &t, SwitchHandoff &ho, F16
             ctx->opened = opened;
             t.absorb_f162(&opened, 1);
         } else {
-            lmap_next(L, ctx->r_hi[s], rho);
+            lmap_next(L, tab, ctx->r_hi[s], rho);
         }
     }
 }

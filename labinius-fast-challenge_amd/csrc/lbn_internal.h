@@ -151,9 +151,10 @@ int dev_commit_bytes(lbn_ctx *ctx, uint8_t *dst);
 // The switch rounds after lbn_switch_begin, all but the last few on the device: control steps
 // 0 .. r0 - 1 and the folds of rounds 0 .. r0 - 2, with the transcript (state ts in, advanced state
 // out) absorbing each message and sampling each challenge there. The last SWITCH_HOST_STEPS control
-// steps and the folds before them (at most 32 items each) cost less on the host than their launches;
-// the handoff carries what they start from. rounds: r0 x 2 F162, coords: r0 F162 (3 u64 each).
-constexpr int SWITCH_HOST_STEPS = 6;
+// steps and the folds before them (at most 2^(SWITCH_HOST_STEPS - 1) items each) cost less on the
+// host than their launches; the handoff carries what they start from. rounds: r0 x 2 F162, coords:
+// r0 F162 (3 u64 each).
+constexpr int SWITCH_HOST_STEPS = 7;
 struct SwitchHandoff {
     int r0 = 0;              // first control step left to the host
     F162 rho{};              // the challenge of round r0 - 1

@@ -785,8 +785,8 @@ struct CtlTemp {
 
 // One control step for round r (at least 768 threads), on the transcript K.st and map K.L in
 // LDS; the message comes from T, the round's records and the challenge go to sd.
-__device__ __forceinline__ void switch_ctl_step(CtlKeep &K, CtlTemp &X, SwitchDev *sd, int r, int lp, int V, const uint32_t *T,
-                                const uint32_t *pair) {
+__device__ __forceinline__ void switch_ctl_step(CtlKeep &K, CtlTemp &X, SwitchDev *sd, int r, int V,
+                                                const uint32_t *T) {
     auto &st = K.st;
     auto &cl = K.cl;
     auto &L = K.L;
@@ -914,23 +914,8 @@ __device__ __forceinline__ void switch_ctl_step(CtlKeep &K, CtlTemp &X, SwitchDe
         }
     }
     __syncthreads();
-    if (r == lp - 1) {  // opened = p0 + rho (p0 + p1), then absorb_f162([opened])
-        if (t < 4) {
-            const uint64_t p0[3] = {(uint64_t)pair[0] | ((uint64_t)pair[1] << 32),
-                                    (uint64_t)pair[2] | ((uint64_t)pair[3] << 32),
-                                    (uint64_t)pair[4] | ((uint64_t)pair[5] << 32)};
-            const uint64_t d[3] = {p0[0] ^ ((uint64_t)pair[6] | ((uint64_t)pair[7] << 32)),
-                                   p0[1] ^ ((uint64_t)pair[8] | ((uint64_t)pair[9] << 32)),
-                                   p0[2] ^ ((uint64_t)pair[10] | ((uint64_t)pair[11] << 32))};
-            const uint64_t rr[3] = {rho_s[0], rho_s[1], rho_s[2]};
-            uint64_t m[3];
-            f162_mul_slow(rr, d, m);
-            const uint64_t msg[4] = {1, p0[0] ^ m[0], p0[1] ^ m[1], p0[2] ^ m[2]};
-            if (t == 0)
-                for (int w = 0; w < 3; w++) sd->opened[w] = msg[w + 1];
-            b3d::update_u64s(q, st, msg);
-        }
-    } else {
+    {
+        // (the device runs the steps before the last; the host forms the opened value)
         // L_{r+1}[k] = L_r((1 + c) x^k) + rho L_r[k], through the tables of L built above and 4-bit
         // window tables of y -> rho y (from rho x^i, i < 162)
         if (t < 162) {
@@ -1005,8 +990,8 @@ __device__ __forceinline__ void switch_state_store(const CtlKeep &K, SwitchDev *
 // The first step takes the whole state from the host's pinned copy (`host`) and the last device step
 // (r == last) returns it there, with the next fold's input (hp_n F162 words from hp_src, he_n B128
 // from he_src) into hp_dst / he_dst, so the rounds need no copies around them.
-__global__ __launch_bounds__(1024) void k_switch_ctl(SwitchDev *sd, SwitchDev *host, int r, int lp, int last, int V,
-                                                     const uint32_t *T, const uint32_t *pair, const uint32_t *hp_src,
+__global__ __launch_bounds__(1024) void k_switch_ctl(SwitchDev *sd, SwitchDev *host, int r, int last, int V,
+                                                     const uint32_t *T, const uint32_t *hp_src,
                                                      uint32_t *hp_dst, int hp_n, const uint4 *he_src, uint4 *he_dst,
                                                      int he_n) {
     __shared__ CtlKeep K;
@@ -1021,7 +1006,7 @@ __global__ __launch_bounds__(1024) void k_switch_ctl(SwitchDev *sd, SwitchDev *h
         __syncthreads();
     }
     switch_state_load(K, sd);
-    switch_ctl_step(K, X, sd, r, lp, V, T, pair);
+    switch_ctl_step(K, X, sd, r, V, T);
     switch_state_store(K, sd);
     if (r == last) {
         __threadfence_block();
@@ -3097,7 +3082,7 @@ static void launch_switch(lbn_ctx *ctx) {
     for (int r = 0; r < r0; r++) {
         const bool hand = r == r0 - 1;  // r0 - 2 >= 3 (log_packed >= 12): the input is a P buffer
         k_switch_ctl<<<1, 1024, 0, d->stream>>>(
-            d->sw, (SwitchDev *)(d->hbox_dev + mb::SW), r, lp, r0 - 1, r == 0 ? 4 : 6, msg_buf(d, r), nullptr,
+            d->sw, (SwitchDev *)(d->hbox_dev + mb::SW), r, r0 - 1, r == 0 ? 4 : 6, msg_buf(d, r),
             hand ? d->P[(r0 - 2) & 1] : nullptr, (uint32_t *)(d->hbox_dev + mb::HP), hand ? 4 * h2 : 0,
             hand ? d->E[r0 & 1] : nullptr, (uint4 *)(d->hbox_dev + mb::HE), hand ? 2 * h2 : 0);
         if (r < r0 - 1) launch_round(ctx, r, RhoArg{{0, 0, 0}}, d->sw->rho);

@@ -1236,8 +1236,8 @@ __global__ __launch_bounds__(FOLD_THREADS) void k_fold(const uint4 *table, size_
                         for (int i = 0; i < NB; i++) f |= ((c[i] >> p) & 0x01010101u) << i;
 #pragma unroll
                         for (int b = 0; b < 4; b++) {
-                            const int cnt = (int)((f >> (8 * b)) & 0xFF) - negs;
-                            if (cnt) atomicAdd(raw + base + p + 8 * b, cnt);
+                            // unconditional: adding a zero count costs less than branching around it
+                            atomicAdd(raw + base + p + 8 * b, (int)((f >> (8 * b)) & 0xFF) - negs);
                         }
                     }
                 } else {
@@ -1248,8 +1248,7 @@ __global__ __launch_bounds__(FOLD_THREADS) void k_fold(const uint4 *table, size_
                         for (int i = 0; i < NB; i++) f |= ((c[i] >> p) & 0x00010001u) << i;
 #pragma unroll
                         for (int b = 0; b < 2; b++) {
-                            const int cnt = (int)((f >> (16 * b)) & 0xFFFF) - negs;
-                            if (cnt) atomicAdd(raw + base + p + 16 * b, cnt);
+                            atomicAdd(raw + base + p + 16 * b, (int)((f >> (16 * b)) & 0xFFFF) - negs);
                         }
                     }
                 }

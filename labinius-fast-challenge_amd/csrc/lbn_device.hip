@@ -1109,6 +1109,18 @@ __device__ __forceinline__ void full_add(uint32_t a, uint32_t b, uint32_t c, uin
     carry = (a & b) | (c & t);
 }
 
+// One stage of an 8 x 8 bit transpose in each byte lane: rows k and k + J exchange their J x J blocks.
+template <int J>
+__device__ __forceinline__ void delta_swaps(uint32_t (&x)[8], uint32_t m) {
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+        if (!(k & J)) {
+            const uint32_t t = ((x[k] >> J) ^ x[k | J]) & m;
+            x[k] ^= t << J;
+            x[k | J] ^= t;
+        }
+}
+
 template <int NB>
 __device__ __forceinline__ void ctr_ripple(uint32_t (&c)[NB], uint32_t x, int from) {
 #pragma unroll
@@ -1229,11 +1241,17 @@ __global__ __launch_bounds__(FOLD_THREADS) void k_fold(const uint4 *table, size_
             int32_t *raw = s_raw + k * 292 + pos;
             auto extract = [&](const uint32_t(&c)[NB], int base) {
                 if (NB <= 8) {
+                    // byte b of x[p] = the count of bit 8 b + p (bit i of it from plane i): an 8 x 8
+                    // bit transpose in each byte lane, three delta swaps
+                    uint32_t x[8];
+#pragma unroll
+                    for (int i = 0; i < 8; i++) x[i] = i < NB ? c[i] : 0u;
+                    delta_swaps<4>(x, 0x0F0F0F0Fu);
+                    delta_swaps<2>(x, 0x33333333u);
+                    delta_swaps<1>(x, 0x55555555u);
 #pragma unroll
                     for (int p = 0; p < 8; p++) {
-                        uint32_t f = 0;
-#pragma unroll
-                        for (int i = 0; i < NB; i++) f |= ((c[i] >> p) & 0x01010101u) << i;
+                        const uint32_t f = x[p];
 #pragma unroll
                         for (int b = 0; b < 4; b++) {
                             // unconditional: adding a zero count costs less than branching around it

@@ -139,34 +139,43 @@ def warm(sch, ks, workers):
         sch.result(h)
 
 
-def run_jobs(m, jobs, workers, budget=200 << 30):
+def run_jobs(m, jobs, workers, reps=1, budget=200 << 30):
     """All jobs submitted at once to a scheduler with `workers` proofs in flight (contexts warmed
-    first, one per table size); returns (proofs, seconds)."""
+    first), timed `reps` times on the same scheduler; returns (proofs of every run, seconds per run)."""
     sch = m.Scheduler(m.DEFAULT_Q, hbm_budget=budget, workers=workers)
     try:
         warm(sch, sorted({kk for kk, _ in jobs}), workers)
-        w0 = time.monotonic_ns()
-        t0 = time.perf_counter()
-        hs = [sch.submit(kk, seed) for kk, seed in jobs]
-        sch.drain()
-        dt = time.perf_counter() - t0
-        run_jobs.window_ns = (w0, time.monotonic_ns())  # the timed batch, on the kernel tracer's clock
-        return [sch.result(h)[0] for h in hs], dt
+        proofs, times = [], []
+        for rep in range(reps):
+            w0 = time.monotonic_ns()
+            t0 = time.perf_counter()
+            hs = [sch.submit(kk, seed) for kk, seed in jobs]
+            sch.drain()
+            times.append(time.perf_counter() - t0)
+            if rep == 0:  # the first timed batch, on the kernel tracer's clock
+                run_jobs.window_ns = (w0, time.monotonic_ns())
+            proofs.append([sch.result(h)[0] for h in hs])
+        return proofs, times
     finally:
         sch.close()
 
 
-def batched(m, workers):
-    """64 independent proofs of mixed size (k in [20, 28]) in flight together vs one at a time."""
+def batched(m, workers, reps=3):
+    """64 independent proofs of mixed size (k in [20, 28]) in flight together vs one at a time; each
+    way timed `reps` times (the batch is short), medians reported."""
     jobs = batched_instances()
-    seq, t_seq = run_jobs(m, jobs, 1)
+    seq, t_seqs = run_jobs(m, jobs, 1, reps)
     w_seq = run_jobs.window_ns
-    par, t_par = run_jobs(m, jobs, workers)
+    par, t_pars = run_jobs(m, jobs, workers, reps)
     batched.windows_ns = {"sequential": list(w_seq), "parallel": list(run_jobs.window_ns)}
+    t_seq, t_par = statistics.median(t_seqs), statistics.median(t_pars)
     return {"instances": len(jobs), "k": sorted({kk for kk, _ in jobs}), "workers": workers,
             "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")),
             "proofs_per_s": round(len(jobs) / t_par, 1), "sequential_proofs_per_s": round(len(jobs) / t_seq, 1),
-            "speedup": round(t_seq / t_par, 2), "identical_to_sequential": seq == par}
+            "speedup": round(t_seq / t_par, 2), "runs": reps,
+            "proofs_per_s_runs": [round(len(jobs) / t, 1) for t in t_pars],
+            "sequential_proofs_per_s_runs": [round(len(jobs) / t, 1) for t in t_seqs],
+            "identical_to_sequential": all(p == seq[0] for p in seq + par)}
 
 
 def serving(m, requests, rate, workers, budget):

@@ -306,14 +306,31 @@ typedef struct {
     uint64_t off;
     uint8_t buf[256];
     int pos;
+    /* Test hook: when raw is set, bytes come from raw[0..raw_len) instead of the XOF (reads past
+     * the end return 0 and set overrun); consumed counts every byte read either way. */
+    const uint8_t *raw;
+    size_t raw_len, consumed;
+    int overrun;
 } xof_t;
 
 static void xof_init(xof_t *x, const b3_hasher *h) {
     x->h = *h;
     x->off = 0;
     x->pos = 256;
+    x->raw = NULL;
+    x->raw_len = x->consumed = 0;
+    x->overrun = 0;
 }
 static uint8_t xof_byte(xof_t *x) {
+    if (x->raw) {
+        if (x->consumed >= x->raw_len) {
+            x->overrun = 1;
+            x->consumed++;
+            return 0;
+        }
+        return x->raw[x->consumed++];
+    }
+    x->consumed++;
     if (x->pos == 256) {
         b3_finalize_seek(&x->h, x->off, x->buf, 256);
         x->off += 256;
@@ -462,6 +479,45 @@ static void sample_short_challenge(tr_t *t, orc_challenge *out) {
         attempt(&x, WEIGHT, out);
         out->signs = orc_sign_bits(out);
         if (within(out, bound_sq)) return;
+    }
+}
+
+/* One attempt (Fisher-Yates draw of `weight` positions, unsigned) read from a given byte stream in
+ * place of the XOF, for known-answer and byte-accounting tests of the draw. Returns the bytes the
+ * attempt consumed, or -1 when it needed more than len bytes. */
+long orc_attempt_from_bytes(const uint8_t *stream, size_t len, int weight, orc_challenge *out) {
+    xof_t x;
+    memset(&x, 0, sizeof x);
+    x.raw = stream;
+    x.raw_len = len;
+    attempt(&x, weight, out);
+    return x.overrun ? -1 : (long)x.consumed;
+}
+
+/* Bytes of the "short-challenge" stream that sample_short_challenge consumes per attempt, for
+ * `count` challenges drawn after absorbing msg into a transcript started with domain:
+ * attempts[i] = attempts challenge i took, consumed[i] = its bytes. */
+void orc_challenges_accounting(const uint8_t *domain, size_t dlen, const uint8_t *msg, size_t mlen, int count,
+                               uint32_t *attempts, uint64_t *consumed) {
+    tr_t t;
+    tr_new(&t, domain, dlen);
+    tr_absorb_bytes(&t, msg, mlen);
+    const double bound_sq = 12.0 * 12.0 + 1e-12;
+    for (int i = 0; i < count; i++) {
+        b3_hasher h;
+        tr_reader(&t, "short-challenge", &h);
+        xof_t x;
+        xof_init(&x, &h);
+        orc_challenge c;
+        uint32_t n = 0;
+        for (;;) {
+            n++;
+            attempt(&x, WEIGHT, &c);
+            c.signs = orc_sign_bits(&c);
+            if (within(&c, bound_sq)) break;
+        }
+        attempts[i] = n;
+        consumed[i] = x.consumed;
     }
 }
 

@@ -62,6 +62,12 @@ void orc_transcript_probe(const uint8_t *domain, size_t dlen, const uint8_t *msg
  * transcript started with `domain`. */
 void orc_challenges_probe(const uint8_t *domain, size_t dlen, const uint8_t *msg, size_t mlen,
                           int count, orc_challenge *out);
+/* One unsigned attempt (attempt, challenge.rs:662-688) drawn from `stream` in place of the XOF;
+ * returns the bytes consumed, or -1 when the stream was too short. */
+long orc_attempt_from_bytes(const uint8_t *stream, size_t len, int weight, orc_challenge *out);
+/* Attempts and XOF bytes consumed per challenge for `count` challenges (as orc_challenges_probe). */
+void orc_challenges_accounting(const uint8_t *domain, size_t dlen, const uint8_t *msg, size_t mlen, int count,
+                               uint32_t *attempts, uint64_t *consumed);
 double orc_canonical_inf_norm_sq(const orc_challenge *c);
 double orc_canonical_inf_norm_sq_naive(const orc_challenge *c);
 uint32_t orc_sign_bits(const orc_challenge *c);

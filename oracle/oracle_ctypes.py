@@ -61,6 +61,9 @@ def lib():
         L.orc_transcript_probe.argtypes = [P, ctypes.c_size_t, P, ctypes.c_size_t, P, ctypes.c_size_t, ctypes.c_int,
                                            P, P]
         L.orc_challenges_probe.argtypes = [P, ctypes.c_size_t, P, ctypes.c_size_t, ctypes.c_int, P]
+        L.orc_attempt_from_bytes.restype = ctypes.c_long
+        L.orc_attempt_from_bytes.argtypes = [P, ctypes.c_size_t, ctypes.c_int, P]
+        L.orc_challenges_accounting.argtypes = [P, ctypes.c_size_t, P, ctypes.c_size_t, ctypes.c_int, P, P]
         L.orc_canonical_inf_norm_sq.restype = ctypes.c_double
         L.orc_canonical_inf_norm_sq.argtypes = [P]
         L.orc_canonical_inf_norm_sq_naive.restype = ctypes.c_double
@@ -208,6 +211,21 @@ def challenges_probe(domain: bytes, msg: bytes, count: int):
     ch = (Challenge * count)()
     lib().orc_challenges_probe(_buf(domain), len(domain), _buf(msg), len(msg), count, ctypes.cast(ch, ctypes.c_void_p))
     return ch
+
+
+def attempt_from_bytes(stream: bytes, weight: int = 28):
+    """One unsigned Fisher-Yates attempt read from `stream` instead of the XOF: (sorted positions,
+    bytes consumed), consumed = -1 when the stream ran out."""
+    c = Challenge()
+    n = lib().orc_attempt_from_bytes(_buf(stream), len(stream), weight, ctypes.byref(c))
+    return list(c.positions[:weight]), n
+
+
+def challenges_accounting(domain: bytes, msg: bytes, count: int):
+    att = np.zeros(count, dtype=np.uint32)
+    used = np.zeros(count, dtype=np.uint64)
+    lib().orc_challenges_accounting(_buf(domain), len(domain), _buf(msg), len(msg), count, _p(att), _p(used))
+    return att, used
 
 
 def ajtai_element(q: int, elements: int, index: int):

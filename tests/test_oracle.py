@@ -294,3 +294,90 @@ def test_key_row_commitment_is_additive_over_disjoint_bits():
     parts = [O.commit_key_row(x, elements, 3, Q_C3, 0, 2) for x in (w, w & mask, w & ~mask)]
     dec = [np.frombuffer(np.pad(x.reshape(-1, 5), ((0, 0), (0, 3))).tobytes(), dtype="<u8") for x in parts]
     assert ((dec[1] + dec[2]) % Q_C3 == dec[0]).all()
+
+
+# ---- the Fisher-Yates draw of the short challenges (pcs/src/challenge.rs:354-372, 662-688) ----
+# The reference holds no vector for `attempt` / `Xof::below`; these checks fix what the definition
+# fixes: known answers worked by hand from the stream bytes, the byte accounting of the u16
+# draws, the reference's division table against plain r mod n, and uniformity of the draw.
+
+def _u16s(*vals):
+    return b"".join(int(v).to_bytes(2, "little") for v in vals)
+
+
+def test_attempt_known_answers():
+    # every draw 0: j = i each time, so positions 0..27, 28 u16 = 56 bytes
+    pos, used = O.attempt_from_bytes(_u16s(*[0] * 28))
+    assert pos == list(range(28)) and used == 56
+    # bytes a1 00 are u16 161 (little-endian; big-endian would be 41216 = 68 mod 162): i = 0 picks
+    # 161, then 1..27
+    pos, used = O.attempt_from_bytes(bytes([0xA1, 0x00]) + _u16s(*[0] * 27))
+    assert pos == list(range(1, 28)) + [161] and used == 56
+    # n = 162 accepts r < 404 * 162 = 65448: 65448 is redrawn, 65447 = 161 mod 162 is kept
+    pos, used = O.attempt_from_bytes(_u16s(65448, 65447, *[0] * 27))
+    assert pos == list(range(1, 28)) + [161] and used == 58
+    # r = 200 at i = 0: 200 mod 162 = 38; the swap leaves 0 at index 38, which i < 28 never reaches
+    pos, used = O.attempt_from_bytes(_u16s(200, *[0] * 27))
+    assert pos == list(range(1, 28)) + [38] and used == 56
+    # j = i + below(162 - i) with r_i = 161 - i always lands on index 161, which holds the value
+    # the previous step swapped out: 161, then 0, 1, ..., 26
+    pos, used = O.attempt_from_bytes(_u16s(*[161 - i for i in range(28)]))
+    assert pos == list(range(27)) + [161] and used == 56
+    # a short stream is reported, not read past
+    assert O.attempt_from_bytes(_u16s(*[0] * 27))[1] == -1
+
+
+def test_attempt_byte_accounting():
+    rng = np.random.default_rng(11)
+    for trial in range(200):
+        # accepted draws below 65374 (every limit for n in 135..162 exceeds 65536 - 162), and k
+        # rejected 0xffff draws (65536 mod n > 0 for every such n) placed at random
+        vals = list(rng.integers(0, 65374, size=28))
+        k = int(rng.integers(0, 6))
+        for _ in range(k):
+            vals.insert(int(rng.integers(0, len(vals))), 0xFFFF)
+        pos, used = O.attempt_from_bytes(_u16s(*vals) + bytes(64))
+        assert used == 56 + 2 * k, trial
+        assert pos == sorted(set(pos)) and len(pos) == 28 and max(pos) < 162
+
+
+def test_division_table_is_r_mod_n():
+    # Xof::below's DIVIDE[n] = (floor(65536 / n) n, floor(2^32 / n) + 1): for every accepted r the
+    # multiply-shift quotient gives r mod n, which is what the oracle and the device compute
+    for n in range(1, 163):
+        limit = (65536 // n) * n
+        magic = (1 << 32) // n + 1
+        r = np.arange(limit, dtype=np.uint64)
+        q = (r * np.uint64(magic)) >> np.uint64(32)
+        assert ((r - q * np.uint64(n)) == r % np.uint64(n)).all(), n
+
+
+def test_attempt_positions_are_uniform():
+    from scipy.stats import chisquare
+    rng = np.random.default_rng(5)
+    counts = np.zeros(162, dtype=np.int64)
+    pairs = np.zeros((2, 162), dtype=np.int64)  # low/high position parity against the first draw
+    n_att = 12000
+    stream = rng.bytes(n_att * 64)
+    for a in range(n_att):
+        pos, used = O.attempt_from_bytes(stream[a * 64:(a + 1) * 64])
+        assert used >= 56
+        counts[pos] += 1
+        pairs[a & 1, pos] += 1
+    # each attempt is a uniform 28-subset of 0..161 when the draw is right: 28/162 per position
+    assert chisquare(counts).pvalue > 1e-6
+    assert chisquare(pairs[0]).pvalue > 1e-6 and chisquare(pairs[1]).pvalue > 1e-6
+
+
+def test_challenge_stream_accounting():
+    # the transcript-driven sampler: each challenge reads one continuous stream, 56 bytes per
+    # attempt plus 2 per redrawn u16, and about a third of the attempts fail the norm bound
+    att, used = O.challenges_accounting(b"labinius/test-accounting", b"prefix", 400)
+    extra = used.astype(np.int64) - 56 * att.astype(np.int64)
+    assert (extra >= 0).all() and (extra % 2 == 0).all()
+    assert 1.2 < att.mean() < 2.0
+    # redraws are rare (at most 162 / 65536 per draw)
+    assert extra.sum() / 2 < 0.01 * 28 * att.sum()
+    # and the probe's challenges are the accounted ones
+    ch = O.challenges_probe(b"labinius/test-accounting", b"prefix", 400)
+    assert all(c.weight == 28 for c in ch)

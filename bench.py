@@ -38,7 +38,8 @@ VALU_SLOTS_T = 256 * 64 * 2.4e9 / 1e12  # int32 VALU issue: 256 CUs x 64 lanes x
 # prime, 324 (Phi6 split) + 2*324 (radix 2) + 4*216*3 (radix 3 with the cube root) = 3,564 modular
 # products plus 648 MAC = 4,212 "mulmod-eq". Peak: the int32 VALU issue rate at 3 slots per
 # Montgomery product (two 32x32->64 multiply-adds and one 32-bit multiply, each one issue slot on
-# gfx950: build/valu_rates measures chains of them at 0.86-0.95 of that rate).
+# gfx950: a throughput probe measured chains of them at 0.86-0.95 of that rate,
+# profiles/valu_multiply_rates.md).
 MULMOD_PER_ELEMENT_PRIME = 4212
 MULMOD_SLOTS = 3
 MULMOD_PEAK_T = VALU_SLOTS_T / MULMOD_SLOTS
@@ -315,10 +316,7 @@ def main():
     p = m.Prover(k)
     p.set_modulus(m.DEFAULT_Q)
     if args.copy_calibration:  # the library's copy kernel over 2^k/8 bytes: read once, written once
-        import torch
-        src = torch.empty((1 << k) // 8, dtype=torch.uint8, device="cuda")  # allocation only
-        p.load_table_device(src.data_ptr(), src.numel())
-        del src
+        p.copy_probe()
     # The table handed over from host memory (pageable) as the reference's loader does: the
     # PCIe-inclusive cost, reported beside the step time, never inside it.
     host_table = np.frombuffer(np.random.default_rng(7).bytes((1 << k) // 8), dtype=np.uint8)

@@ -109,6 +109,11 @@ int lbn_prove(lbn_ctx *ctx, const uint8_t seed[32], uint8_t *proof, size_t cap, 
 void *lbn_host_alloc(size_t bytes);
 void lbn_host_free(void *p);
 
+/* Diagnostics (no counterpart in the reference): one plain copy of the table's bytes into a device
+ * buffer of the same size, 16 bytes per lane, read once and written once: a kernel of known traffic
+ * that scales a profiler's byte counters. Clears the state lbn_partials left. */
+int lbn_copy_probe(lbn_ctx *ctx);
+
 /* Number of CRT primes the commitment uses for the current modulus (diagnostics). */
 int lbn_commit_primes(const lbn_ctx *ctx);
 

@@ -90,6 +90,7 @@ def lib():
         L.lbn_proof_len.restype = ctypes.c_size_t
         L.lbn_prove.argtypes = [P, P, P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t), P]
         L.lbn_commit_primes.argtypes = [P]
+        L.lbn_copy_probe.argtypes = [P]
         L.lbn_set_timing.argtypes = [P, ctypes.c_int]
         L.lbn_set_graphs.argtypes = [P, ctypes.c_int]
         L.lbn_reset_timing.argtypes = [P]
@@ -252,6 +253,10 @@ class Prover:
 
     def commit_primes(self) -> int:
         return lib().lbn_commit_primes(self.h)
+
+    def copy_probe(self):
+        """One plain copy of the table's bytes on the device (a known-traffic kernel for counters)."""
+        self._check(lib().lbn_copy_probe(self.h))
 
     def set_timing(self, enable=True):
         self._check(lib().lbn_set_timing(self.h, 1 if enable else 0))

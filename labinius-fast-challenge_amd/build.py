@@ -1,5 +1,4 @@
-"""Build recipe of the MI355X prover: liblabinius_gpu.so, the labinius-fast executable and the
-valu_rates probe.
+"""Build recipe of the MI355X prover: liblabinius_gpu.so and the labinius-fast executable.
 
 hipcc compiles every source for gfx950 only. Packed FP32 instructions are disabled for all HIP
 code (-target-feature -packed-fp32-ops), and floating point is not contracted, so the challenge
@@ -17,7 +16,6 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "build")
 LIB = os.path.join(OUT, "liblabinius_gpu.so")
 CLI = os.path.join(OUT, "labinius-fast")
-RATES = os.path.join(OUT, "valu_rates")  # integer VALU throughput probe (the commitment roofline's peak)
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 DEVICE_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
@@ -51,7 +49,6 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     run(["g++", "-O2", "-std=c++17", "-o", CLI, os.path.join(CSRC, "labinius_fast.cpp"), "-L" + OUT,
          "-llabinius_gpu", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib"])
-    run([HIPCC, *DEVICE_FLAGS, "-o", RATES, os.path.join(CSRC, "valu_rates.hip")])
     if verbose:
         print("built", LIB, CLI)
     return LIB

@@ -2543,6 +2543,21 @@ extern "C" int lbn_load_table_device(lbn_ctx *ctx, const void *device_bytes, siz
     return LBN_OK;
 }
 
+extern "C" int lbn_copy_probe(lbn_ctx *ctx) {
+    if (!ctx) return LBN_E_ARG;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    // the table's bytes into the eq buffer (the same size) with the plain copy kernel; the eq
+    // expansion is rebuilt by the next lbn_partials
+    const size_t words = ctx->shape.table_bytes / 16;
+    k_copy_u4<<<(unsigned)std::min<size_t>((words + 255) / 256, 65536), 256, 0, ctx->dev->stream>>>(
+        ctx->dev->table, ctx->dev->E[0], words);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipStreamSynchronize(ctx->dev->stream));
+    ctx->partials_done = false;
+    ctx->round = -1;
+    return LBN_OK;
+}
+
 // The root node of the table writer's XOF for a seed (verifier/table_writer/src/main.rs:14-22):
 // keyed by the seed, one block holding the domain string.
 static B3Root table_root(const uint8_t seed[32]) {
@@ -2862,15 +2877,15 @@ int lbn::dev_commit_cvs(lbn_ctx *ctx, size_t off, uint64_t first, size_t count, 
         }
         if (at != first + count) return ctx->fail(LBN_E_ARG, "commitment subtrees do not tile the chunks");
     }
+    if (head > 1024 || clen - tail > 1024) return ctx->fail(LBN_E_ARG, "commitment head or tail above one chunk");
     HIPCHK(ctx, hipSetDevice(ctx->device));
     // the staging copy runs on its own stream once the commitment is final, into the caller's
-    // buffer when it is one of ours (pinned)
+    // buffer when it is one of ours (pinned); every argument is checked before it is queued
     d->staged_direct = pinned_alias(dst, clen) != nullptr;
     HIPCHK(ctx, hipStreamWaitEvent(d->copy_stream, d->ev_commit, 0));
     HIPCHK(ctx, hipMemcpyAsync(d->staged_direct ? dst : d->hstage, d->cbytes, clen, hipMemcpyDeviceToHost,
                                d->copy_stream));
     HIPCHK(ctx, hipEventRecord(d->ev_staged, d->copy_stream));
-    if (head > 1024 || clen - tail > 1024) return ctx->fail(LBN_E_ARG, "commitment head or tail above one chunk");
     if (count) {
         B3Key k;
         for (int i = 0; i < 8; i++) k.k[i] = key[i];
@@ -2901,6 +2916,14 @@ int lbn::dev_commit_cvs(lbn_ctx *ctx, size_t off, uint64_t first, size_t count, 
     tcollect(ctx, T_MAC);
     tcollect(ctx, T_FINISH);
     return LBN_OK;
+}
+
+void lbn::dev_quiesce(lbn_ctx *ctx) {
+    DeviceState *d = ctx->dev;
+    if (!d) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(d->copy_stream);
+    (void)hipStreamSynchronize(d->stream);
 }
 
 int lbn::dev_commit_bytes(lbn_ctx *ctx, uint8_t *dst) {

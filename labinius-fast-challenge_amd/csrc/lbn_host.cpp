@@ -588,6 +588,15 @@ extern "C" int lbn_prove(lbn_ctx *ctx, const uint8_t seed[32], uint8_t *proof, s
     const size_t off_fold = off_row + 8 + row_bytes;
     lbn_profile pf{};
     const double t0 = now_us();
+    // From the first launch on, an error return first waits for the device: the commitment's copy
+    // and the fold kernel may write into `proof` (directly when it is pinned).
+    struct Quiesce {
+        lbn_ctx *ctx;
+        bool armed = true;
+        ~Quiesce() {
+            if (armed) dev_quiesce(ctx);
+        }
+    } quiesce{ctx};
 
     TRY(dev_commit_launch(ctx));
 
@@ -692,6 +701,7 @@ extern "C" int lbn_prove(lbn_ctx *ctx, const uint8_t seed[32], uint8_t *proof, s
     w.at += row_bytes;
     w.u64(fold_bytes);
     *len = plen;
+    quiesce.armed = false;
     const double t8 = now_us();
     pf.commit_us = t1 - t0;
     pf.transcript_us = t2 - t1;

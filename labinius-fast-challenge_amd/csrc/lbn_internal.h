@@ -148,6 +148,9 @@ int dev_commit_launch(lbn_ctx *ctx);
 int dev_commit_cvs(lbn_ctx *ctx, size_t off, uint64_t first, size_t count, const Blake3::Subtrees &trees,
                    const uint32_t key[8], uint32_t flags, uint32_t *cvs, uint8_t *dst, size_t head, size_t tail);
 int dev_commit_bytes(lbn_ctx *ctx, uint8_t *dst);
+// Waits for everything queued on the context's streams (errors ignored): a failing lbn_prove calls
+// it before returning, so no copy or kernel still writes the caller's proof buffer.
+void dev_quiesce(lbn_ctx *ctx);
 // The switch rounds after lbn_switch_begin, all but the last few on the device: control steps
 // 0 .. r0 - 1 and the folds of rounds 0 .. r0 - 2, with the transcript (state ts in, advanced state
 // out) absorbing each message and sampling each challenge there. The last SWITCH_HOST_STEPS control

@@ -1,6 +1,8 @@
 /* ORACLE — TEST INFRASTRUCTURE ONLY. Command line of the CPU oracle:
  *   labinius-oracle --write-table FILE        (verifier/table_writer/src/main.rs:14-22)
- *   labinius-oracle --prove FILE --modulus Q  (verifier/proof_verify/src/main.rs:423-430)
+ *   labinius-oracle --prove FILE [--table FILE] [--modulus Q]
+ *                                             (verifier/proof_verify/src/main.rs:423-430; the
+ *                                              prover's form, solution/proof_generate/src/main.rs:2056-2069)
  *   labinius-oracle --verify FILE             (verifier/proof_verify/src/main.rs:431-455)
  * Seed from LABINIUS_SEED (64 hex), shape from LABINIUS_LOG_VARS (default 31). */
 #include "oracle.h"
@@ -46,12 +48,31 @@ int main(int argc, char **argv) {
         fclose(f);
         return 0;
     }
-    if (argc == 5 && !strcmp(argv[1], "--prove") && !strcmp(argv[3], "--modulus")) {
-        u128 q;
-        if (parse_u128(argv[4], &q)) { fprintf(stderr, "invalid modulus\n"); return 2; }
+    if (argc >= 3 && !strcmp(argv[1], "--prove")) {
+        /* --prove FILE [--table FILE] [--modulus Q], as the prover's command line
+         * (solution/proof_generate/src/main.rs:2056-2069); without --table the table is the seed's */
+        const char *table_path = NULL;
+        u128 q = 35039150080ull; /* solution/proof_generate/src/parameters.rs:14 */
+        for (int i = 3; i < argc; i += 2) {
+            if (i + 1 >= argc) { fprintf(stderr, "missing value for %s\n", argv[i]); return 2; }
+            if (!strcmp(argv[i], "--table")) table_path = argv[i + 1];
+            else if (!strcmp(argv[i], "--modulus")) {
+                if (parse_u128(argv[i + 1], &q)) { fprintf(stderr, "invalid modulus\n"); return 2; }
+            } else { fprintf(stderr, "unknown argument %s\n", argv[i]); return 2; }
+        }
+        uint8_t *table = NULL;
+        if (table_path) {
+            FILE *tf = fopen(table_path, "rb");
+            if (!tf) { fprintf(stderr, "cannot open table\n"); return 1; }
+            table = malloc(s.table_bytes + 1);
+            size_t got = fread(table, 1, s.table_bytes + 1, tf);
+            fclose(tf);
+            if (got != s.table_bytes) { fprintf(stderr, "table has the wrong length\n"); return 1; }
+        }
         size_t cap = orc_proof_len(k, (uint64_t)q, (uint64_t)(q >> 64)), len = 0;
+        if (!cap) { fprintf(stderr, "modulus outside the accepted range\n"); return 2; }
         uint8_t *p = malloc(cap);
-        int rc = orc_prove(seed, k, (uint64_t)q, (uint64_t)(q >> 64), NULL, p, cap, &len);
+        int rc = orc_prove(seed, k, (uint64_t)q, (uint64_t)(q >> 64), table, p, cap, &len);
         if (rc) { fprintf(stderr, "prove failed: %d\n", rc); return 1; }
         FILE *f = fopen(argv[2], "wb");
         if (!f || fwrite(p, 1, len, f) != len) { fprintf(stderr, "write failed\n"); return 1; }
@@ -79,6 +100,6 @@ int main(int argc, char **argv) {
                qs + at, orc_modulus_bits(q), 1ull << k, 1ull << (k - 7));
         return 0;
     }
-    fprintf(stderr, "usage: labinius-oracle --write-table FILE | --prove FILE --modulus Q | --verify FILE\n");
+    fprintf(stderr, "usage: labinius-oracle --write-table FILE | --prove FILE [--table FILE] [--modulus Q] | --verify FILE\n");
     return 2;
 }

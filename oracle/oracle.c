@@ -49,12 +49,22 @@ int orc_shape_of(int log_vars, orc_shape *s) {
 
 /* =============================== fields ================================================== */
 
+/* Carry-less 64 x 64 -> 128 product: PCLMULQDQ where the build enables it, else the shift-and-xor
+ * definition. Both are checked against the bit-by-bit field products in tests/test_oracle.py. */
+#if defined(__PCLMUL__) && defined(__SSE4_1__)
+#include <immintrin.h>
+static u128 clmul64(uint64_t a, uint64_t b) {
+    __m128i r = _mm_clmulepi64_si128(_mm_cvtsi64_si128((long long)a), _mm_cvtsi64_si128((long long)b), 0);
+    return ((u128)(uint64_t)_mm_extract_epi64(r, 1) << 64) | (uint64_t)_mm_cvtsi128_si64(r);
+}
+#else
 static u128 clmul64(uint64_t a, uint64_t b) {
     u128 r = 0;
     for (int i = 0; i < 64; i++)
         if ((a >> i) & 1) r ^= (u128)b << i;
     return r;
 }
+#endif
 
 /* GF(2^128) in the GHASH polynomial basis, modulo x^128 + x^7 + x^2 + x + 1
  * (pcs/src/fields/scalar.rs:4, 46-68). */
@@ -103,16 +113,22 @@ void orc_f162_mul(const uint64_t a[3], const uint64_t b[3], uint64_t out[3]) {
             p[i + j] ^= (uint64_t)t;
             p[i + j + 1] ^= (uint64_t)(t >> 64);
         }
-    /* Fold bits 161+k .. down: x^(162+k) = x^(81+k) + x^k, highest first. */
-    for (int j = 323; j >= 162; j--)
-        if (bit_of(p, j)) {
-            flip(p, j);
-            flip(p, j - 162);
-            flip(p, j - 81);
-        }
-    out[0] = p[0];
-    out[1] = p[1];
-    out[2] = p[2];
+    /* x^(162+k) = x^(81+k) + x^k. The product has degree <= 322; with H = p >> 162 (161 bits) and
+     * L = p mod x^162, p = L + H + H x^81, and H x^81 = (H mod x^81) x^81 + G x^162 with
+     * G = H >> 81 (80 bits), which folds once more into G x^81 + G, both below x^162. */
+    uint64_t H[3], G[2], t[3];
+    for (int i = 0; i < 3; i++) H[i] = (p[i + 2] >> 34) | (i + 3 < 6 ? p[i + 3] << 30 : 0);
+    H[2] &= (1ull << 33) - 1; /* bits 162..322 */
+    G[0] = (H[1] >> 17) | (H[2] << 47);
+    G[1] = H[2] >> 17;
+    /* t = (H mod x^81) + G, placed at x^81 */
+    uint64_t hm0 = H[0] ^ G[0], hm1 = (H[1] & ((1ull << 17) - 1)) ^ G[1];
+    t[0] = 0; /* x^81 is bit 17 of limb 1 */
+    t[1] = hm0 << 17;
+    t[2] = (hm0 >> 47) | (hm1 << 17);
+    out[0] = p[0] ^ H[0] ^ G[0] ^ t[0];
+    out[1] = p[1] ^ H[1] ^ G[1] ^ t[1];
+    out[2] = (p[2] & ((1ull << 34) - 1)) ^ H[2] ^ t[2];
 }
 
 void orc_f162_mul_bitwise(const uint64_t a[3], const uint64_t b[3], uint64_t out[3]) {
@@ -604,6 +620,28 @@ static void fold_into_ring(i128 *raw, u128 q, u128 *out) {
  * X^(4p+k) A_m to the raw product (rq.rs:264-287), folded into R_Q at the end. */
 static void commit_column_words(const uint8_t *table, size_t elements, u128 q, const u128 *matrix, int column,
                                 u128 *out) {
+    if (q * (u128)(elements * 512) < ((u128)1 << 63)) {
+        /* The same sums in u64: every raw coefficient is a sum of at most elements * 512 matrix
+         * values below q, so it stays below 2^63 and converts to i128 exactly. */
+        uint64_t *raw64 = (uint64_t *)calloc(2 * N - 1, sizeof(uint64_t));
+        uint64_t a64[N];
+        for (size_t m = 0; m < elements; m++) {
+            for (int t = 0; t < N; t++) a64[t] = (uint64_t)matrix[N * m + t];
+            for (int k = 0; k < 4; k++) {
+                orc_b128 w = word_at(table, (size_t)column * 4 * elements + 4 * m + k);
+                for (int h = 0; h < 2; h++)
+                    for (uint64_t b = w.w[h]; b; b &= b - 1) {
+                        uint64_t *win = raw64 + 4 * (64 * h + __builtin_ctzll(b)) + k;
+                        for (int t = 0; t < N; t++) win[t] += a64[t];
+                    }
+            }
+        }
+        i128 raw[2 * N - 1];
+        for (int t = 0; t < 2 * N - 1; t++) raw[t] = (i128)raw64[t];
+        free(raw64);
+        fold_into_ring(raw, q, out);
+        return;
+    }
     i128 *raw = (i128 *)calloc(2 * N - 1, sizeof(i128));
     for (size_t m = 0; m < elements; m++) {
         const u128 *a = matrix + N * m;
@@ -774,8 +812,11 @@ static void partials_from_eq(const uint8_t *table, size_t n, const orc_b128 *eq,
 #pragma omp for schedule(static)
         for (long long j = 0; j < (long long)n; j++) {
             orc_b128 w = word_at(table, (size_t)j);
-            for (int k = 0; k < PACK; k++)
-                if (bit_of(w.w, k)) loc[k] = badd(loc[k], eq[j]);
+            for (int h = 0; h < 2; h++)
+                for (uint64_t b = w.w[h]; b; b &= b - 1) {
+                    const int k = 64 * h + __builtin_ctzll(b);
+                    loc[k] = badd(loc[k], eq[j]);
+                }
         }
 #pragma omp critical
         for (int k = 0; k < PACK; k++) v[k] = badd(v[k], loc[k]);
@@ -791,13 +832,6 @@ void orc_partials(const uint8_t *table, int log_vars, const uint64_t *r_hi, uint
     free(eq);
 }
 
-/* psi (crossfield.rs:116-129 via poly_from_eq 403-445): a = sum_k bit_k(x) batch[k]. */
-static orc_f162 psi(orc_b128 x, const orc_f162 *batch) {
-    orc_f162 acc = F_ZERO;
-    for (int k = 0; k < PACK; k++)
-        if (bit_of(x.w, k)) acc = fadd(acc, batch[k]);
-    return acc;
-}
 
 typedef struct {
     orc_f162 *a, *p;
@@ -809,11 +843,24 @@ static void switch_init(switch_t *sw, const uint8_t *table, size_t n, const orc_
                         const orc_f162 *batch) {
     sw->a = (orc_f162 *)malloc(sizeof(orc_f162) * n);
     sw->p = (orc_f162 *)malloc(sizeof(orc_f162) * n);
+    /* a_j = psi(eq_j) = sum_k bit_k(eq_j) batch[k] (crossfield.rs:116-129 via poly_from_eq 403-445),
+     * summed per byte of eq_j: byte_sums[i][v] = sum_{b: bit b of v} batch[8 i + b] */
+    orc_f162 (*byte_sums)[256] = (orc_f162 (*)[256])malloc(sizeof(orc_f162) * 16 * 256);
+    for (int i = 0; i < 16; i++) {
+        byte_sums[i][0] = F_ZERO;
+        for (int v = 1; v < 256; v++) {
+            int b = __builtin_ctz((unsigned)v);
+            byte_sums[i][v] = fadd(byte_sums[i][v & (v - 1)], batch[8 * i + b]);
+        }
+    }
 #pragma omp parallel for schedule(static)
     for (long long j = 0; j < (long long)n; j++) {
-        sw->a[j] = psi(eq[j], batch);
+        orc_f162 acc = F_ZERO;
+        for (int i = 0; i < 16; i++) acc = fadd(acc, byte_sums[i][(eq[j].w[i >> 3] >> (8 * (i & 7))) & 255]);
+        sw->a[j] = acc;
         sw->p[j] = lift(word_at(table, (size_t)j));
     }
+    free(byte_sums);
     sw->half = n;
 }
 /* sumcheck::msg (sumcheck.rs:89-117). */
@@ -821,9 +868,20 @@ static void switch_msg(switch_t *sw, orc_f162 msg[2]) {
     sw->half /= 2;
     size_t h = sw->half;
     orc_f162 e0 = F_ZERO, einf = F_ZERO;
-    for (size_t j = 0; j < h; j++) {
-        e0 = fadd(e0, fmul(sw->a[j], sw->p[j]));
-        einf = fadd(einf, fmul(fadd(sw->a[j], sw->a[j + h]), fadd(sw->p[j], sw->p[j + h])));
+    /* field sums: each thread adds its range, the partial sums are added after (order-free) */
+#pragma omp parallel if (h >= 4096)
+    {
+        orc_f162 l0 = F_ZERO, linf = F_ZERO;
+#pragma omp for schedule(static)
+        for (long long j = 0; j < (long long)h; j++) {
+            l0 = fadd(l0, fmul(sw->a[j], sw->p[j]));
+            linf = fadd(linf, fmul(fadd(sw->a[j], sw->a[j + h]), fadd(sw->p[j], sw->p[j + h])));
+        }
+#pragma omp critical
+        {
+            e0 = fadd(e0, l0);
+            einf = fadd(einf, linf);
+        }
     }
     msg[0] = e0;
     msg[1] = einf;

@@ -179,18 +179,26 @@ def batched(m, workers, reps=3):
             "identical_to_sequential": all(p == seq[0] for p in seq + par)}
 
 
-def serving(m, requests, rate, workers, budget):
-    """Poisson arrivals of proving requests, k in {20..31} with P(k) ~ 2^-(k-20), fresh seed per
-    request; latency = completion - arrival. The same arrival sequence is served by `workers`
-    proofs in flight and by one at a time."""
+def serving_requests(requests, rate, rng_seed=20261203):
+    """The serving workload: (arrival s, k, seed) per request, Poisson arrivals at `rate` per second,
+    k in {20..31} with P(k) ~ 2^-(k-20), request i's seed bench_seed(2_000_000 + i)."""
     import random
-    rng = random.Random(20261203)
+    rng = random.Random(rng_seed)
     ks = list(range(20, 32))
     wts = [2.0 ** -(kk - 20) for kk in ks]
     reqs, t = [], 0.0
     for i in range(requests):
         t += rng.expovariate(rate)
         reqs.append((t, rng.choices(ks, wts)[0], bench_seed(2_000_000 + i)))
+    return reqs
+
+
+def serving(m, requests, rate, workers, budget):
+    """Poisson arrivals of proving requests, k in {20..31} with P(k) ~ 2^-(k-20), fresh seed per
+    request; latency = completion - arrival. The same arrival sequence is served by `workers`
+    proofs in flight and by one at a time."""
+    reqs = serving_requests(requests, rate)
+    ks = list(range(20, 32))
 
     def serve(w):
         sch = m.Scheduler(m.DEFAULT_Q, hbm_budget=budget, workers=w)

@@ -43,6 +43,11 @@ VALU_SLOTS_T = 256 * 64 * 2.4e9 / 1e12  # int32 VALU issue: 256 CUs x 64 lanes x
 MULMOD_PER_ELEMENT_PRIME = 4212
 MULMOD_SLOTS = 3
 MULMOD_PEAK_T = VALU_SLOTS_T / MULMOD_SLOTS
+# Moduli below 2^39 commit on the matrix cores: per (ring element, column) the product of the
+# matrix's rotations (648 coefficients x 512 witness bits) in 5 int8 planes, 2 ops per term
+# (multiply, add), against MI355X's dense int8 peak.
+INT8_PEAK_T = 5000.0
+TOEP_OPS_PER_ELEMENT_COLUMN = 2 * 648 * 512
 
 
 # Commit micro-benchmark: 64 columns x 4096 binary ring elements (a 16 MiB witness from seed 0),
@@ -111,15 +116,18 @@ def pmc_traffic(pmc_dir, log_vars, kernel="k_commit_mac", copy_kernel="k_copy_u4
                           "write_scale": round(w_scale, 4)}
 
 
-def committed_traffic(path=os.path.join(ROOT, "profiles", "commit_mac_traffic.txt")):
-    """The commit kernel's HBM bytes per launch at k = 31 as the last counter runs measured them (the
-    rocprofv3 FETCH_SIZE / WRITE_SIZE passes and copy-kernel scales written into the profile; a run
-    with --pmc-dir measures them afresh)."""
+def committed_traffic(kernel):
+    """The commit kernel's HBM bytes per launch at k = 31 as the last counter runs of this build measured
+    them (the rocprofv3 FETCH_SIZE / WRITE_SIZE passes and copy-kernel scales written into
+    profiles/<kernel>_traffic.txt, labelled "committed counters"; a run with --pmc-dir measures them
+    afresh)."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_traffic.txt")
     try:
         kv = dict(line.split("=", 1) for line in open(path).read().split() if "=" in line)
         f, w = int(kv["fetch_bytes"]), int(kv["write_bytes"])
         return f + w, {"fetch_bytes": f, "write_bytes": w, "fetch_scale": float(kv["fetch_scale"]),
-                       "write_scale": float(kv["write_scale"]), "source": "profiles/commit_mac_traffic.txt"}
+                       "write_scale": float(kv["write_scale"]),
+                       "source": f"committed counters: profiles/{kernel}_traffic.txt"}
     except (OSError, KeyError, ValueError):
         return None, None
 
@@ -360,11 +368,15 @@ def main():
     n_words = 1 << (k - 7)
     re = (1 << (k - 17)) // 4
     table_bytes = n_words * 16
-    primes = p.commit_primes()
-    per_launch_ms = mean_t["commit_mac_ms"] / primes  # one k_commit_mac launch per CRT prime
-    mulmod_t = re * 1024 * MULMOD_PER_ELEMENT_PRIME / (per_launch_ms / 1e3) / 1e12
+    primes, planes = p.commit_primes(), p.commit_planes()
+    if planes:
+        per_launch_ms = mean_t["commit_mac_ms"]  # one k_toep_mac launch
+        int8_t = re * 1024 * TOEP_OPS_PER_ELEMENT_COLUMN * planes / (per_launch_ms / 1e3) / 1e12
+    else:
+        per_launch_ms = mean_t["commit_mac_ms"] / primes  # one k_commit_mac launch per CRT prime
+        mulmod_t = re * 1024 * MULMOD_PER_ELEMENT_PRIME / (per_launch_ms / 1e3) / 1e12
     passes = {
-        "commit_mac": {"ms": mean_t["commit_mac_ms"], "bound": "valu"},
+        "commit_mac": {"ms": mean_t["commit_mac_ms"], "bound": "mfma" if planes else "valu"},
         "partials": {"ms": mean_t["eq_ms"] + mean_t["partials_ms"], "bound": "hbm",
                      "achieved_gbs": table_bytes / ((mean_t["eq_ms"] + mean_t["partials_ms"]) / 1e3) / 1e9},
         "switch": {"ms": mean_t["switch0_ms"] + mean_t["switch_rounds_ms"], "bound": "hbm",
@@ -374,13 +386,20 @@ def main():
                  "achieved_gbs": (table_bytes + re * 648 * 4) / (mean_t["fold_ms"] / 1e3) / 1e9},
         "commit_finish": {"ms": mean_t["commit_finish_ms"], "bound": "valu"},
     }
-    roofline = {"bound": "valu", "kernel": "k_commit_mac", "achieved": round(mulmod_t, 3),
-                "peak": round(MULMOD_PEAK_T, 3), "unit": "Tmulmod/s", "frac": round(mulmod_t / MULMOD_PEAK_T, 4),
-                "traffic": None, "launch_ms": round(per_launch_ms, 3)}
+    if planes:
+        roofline = {"bound": "mfma", "kernel": "k_toep_mac", "achieved": round(int8_t, 1), "peak": INT8_PEAK_T,
+                    "unit": "TOP/s", "frac": round(int8_t / INT8_PEAK_T, 4), "traffic": None,
+                    "launch_ms": round(per_launch_ms, 3)}
+        kernel_name = "k_toep_mac"
+    else:
+        roofline = {"bound": "valu", "kernel": "k_commit_mac", "achieved": round(mulmod_t, 3),
+                    "peak": round(MULMOD_PEAK_T, 3), "unit": "Tmulmod/s", "frac": round(mulmod_t / MULMOD_PEAK_T, 4),
+                    "traffic": None, "launch_ms": round(per_launch_ms, 3)}
+        kernel_name = "k_commit_mac"
     if args.pmc_dir:
-        roofline["traffic"], roofline["traffic_detail"] = pmc_traffic(args.pmc_dir, k)
+        roofline["traffic"], roofline["traffic_detail"] = pmc_traffic(args.pmc_dir, k, kernel_name)
     elif k == 31:
-        roofline["traffic"], roofline["traffic_detail"] = committed_traffic()
+        roofline["traffic"], roofline["traffic_detail"] = committed_traffic(kernel_name)
     line = {
         "metric": "prover ms (commit+open, 2^31 evals) on 1 MI355X",
         "value": round(med, 3), "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,

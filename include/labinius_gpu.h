@@ -114,8 +114,12 @@ void lbn_host_free(void *p);
  * that scales a profiler's byte counters. Clears the state lbn_partials left. */
 int lbn_copy_probe(lbn_ctx *ctx);
 
-/* Number of CRT primes the commitment uses for the current modulus (diagnostics). */
+/* How the commitment is computed for the current modulus (diagnostics): moduli below 2^39 take the
+ * matrix cores (an int8 product of the binary witness and the matrix's rotations in five byte
+ * planes; lbn_commit_planes returns 5 and lbn_commit_primes 0), larger ones a transform over
+ * lbn_commit_primes CRT primes (lbn_commit_planes returns 0). */
 int lbn_commit_primes(const lbn_ctx *ctx);
+int lbn_commit_planes(const lbn_ctx *ctx);
 
 /* Kernel timing (diagnostics): when enabled, every pass records HIP events on the context's own
  * stream around its kernels; lbn_get_timing returns the milliseconds accumulated since the last

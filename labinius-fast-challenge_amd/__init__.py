@@ -91,6 +91,7 @@ def lib():
         L.lbn_prove.argtypes = [P, P, P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t), P]
         L.lbn_commit_primes.argtypes = [P]
         L.lbn_copy_probe.argtypes = [P]
+        L.lbn_commit_planes.argtypes = [P]
         L.lbn_set_timing.argtypes = [P, ctypes.c_int]
         L.lbn_set_graphs.argtypes = [P, ctypes.c_int]
         L.lbn_reset_timing.argtypes = [P]
@@ -253,6 +254,10 @@ class Prover:
 
     def commit_primes(self) -> int:
         return lib().lbn_commit_primes(self.h)
+
+    def commit_planes(self) -> int:
+        """5 when the commitment runs on the matrix cores (moduli below 2^39), else 0."""
+        return lib().lbn_commit_planes(self.h)
 
     def copy_probe(self):
         """One plain copy of the table's bytes on the device (a known-traffic kernel for counters)."""

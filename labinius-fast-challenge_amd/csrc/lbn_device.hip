@@ -17,6 +17,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <tuple>
 #include <vector>
 
 #include "lbn_internal.h"
@@ -38,6 +39,7 @@ struct Roots18 {
 };
 
 struct SwitchDev;
+struct ToepOps;
 
 struct lbn::DeviceState {
     hipStream_t stream = nullptr;
@@ -80,6 +82,11 @@ struct lbn::DeviceState {
     hipGraphExec_t switch_graph = nullptr;
     bool graphs = true;
     hipStream_t own_stream = nullptr, own_copy = nullptr;  // created here; stream / copy_stream may be a caller's
+    // The matrix-core commitment (moduli below 2^39): its operands, shared by the contexts of one
+    // (device, Q, shape), and its partial sums.
+    std::shared_ptr<::ToepOps> toep;
+    int32_t *tpart = nullptr;
+    int tp_splits = 0;
 };
 
 static int capture_switch(lbn_ctx *ctx);
@@ -2063,6 +2070,176 @@ __global__ __launch_bounds__(256) void k_commit_slicesum(const uint32_t *in, int
 // cvs[8 b].
 constexpr int MAX_CV_TREES = 64;
 constexpr size_t CV_TREE_LDS_MAX = 120 * 1024;
+// ---------------------------------------------------------------------------------------------
+// Commitment on the matrix cores (moduli below 2^39): the product in the coefficient domain as one
+// int8 GEMM. Column c's commitment is C_c = sum_m A_m W_{c,m} in Z[X]/(X^648 - X^324 + 1); with
+// Rot(A)[t][i] the coefficient t of X^i A, C_c[t] = sum_m sum_{i<512} Rot(A_m)[t][i] w_{c,m,i}
+// (coefficients >= 512 of W are zero). Each entry, taken centred mod Q (|e| <= Q/2 < 2^38), is
+// split into five balanced signed bytes e = sum_d 256^d e_d, so C_c[t] = sum_d 256^d S_d[t][c] with
+// S_d = Rot_d W an exact int32 product of int8 digits and 0/1 bits (|S_d| <= 2^21 * 128): the
+// witness stays binary and the matrix cores take 64 (element, bit) terms per instruction. The
+// reduction X^648 = X^324 - 1 gives, for t < 648 and i < 512,
+//   Rot(A)[t][i] = [t >= i] a_{t-i} + [t >= 324, i > t-324] a_{t+324-i}
+//                - [t < 324, i > t] a_{t+648-i} - [t < 188, i > t+324] a_{t+972-i}.
+// K order of one ring element: 8 steps of 64 terms; in step s = 2q + h, lane group g (lanes
+// 16g..16g+15) takes word g of the element and operand byte 4e + j holds bit
+// p = 32q + 8j + 4h + e of that word, i.e. coefficient i = 4p + g. With the word's dword q in a
+// register, the four operand registers of a step are (dword >> (4h + e)) & 0x01010101.
+// ---------------------------------------------------------------------------------------------
+constexpr int TP_PLANES = 5;
+constexpr int TP_TILES = 41;                 // 16-row tiles over the 648 coefficients (656 rows)
+constexpr int TP_ROWS = 16 * TP_TILES;
+constexpr int TP_WAVES = 8;                  // waves per block: 128 columns each
+constexpr int TP_NT = COLUMNS / (16 * TP_WAVES);  // column tiles of 16 per wave
+constexpr size_t TP_BLOCK_U4 = 8 * TP_PLANES * 64;  // operand uint4s per (tile, ring element)
+
+// Rot(A)[t][i] from the centred coefficients a[0..648) (t < 648, i < 512).
+__device__ __forceinline__ int64_t toep_entry(const int64_t *a, int t, int i) {
+    int64_t v = 0;
+    if (t >= i) v += a[t - i];
+    if (t >= 324 && i > t - 324) v += a[t + 324 - i];
+    if (t < 324 && i > t) v -= a[t + 648 - i];
+    if (t < 188 && i > t + 324) v -= a[t + 972 - i];
+    return v;
+}
+
+// The GEMM's matrix operands (set-up): block (tile r, element m), lane l; writes the element's
+// 8 x 5 operand blocks of 64 lanes x 16 bytes at out[((r * re + m) * 8 + s) * 5 + d][l].
+__global__ __launch_bounds__(64) void k_toep_build(const uint64_t *amat, int re, uint64_t q, uint4 *out) {
+    __shared__ int64_t a[648];
+    const int r = blockIdx.x, m = blockIdx.y, l = threadIdx.x;
+    const uint64_t half = q / 2;
+    for (int i = l; i < 648; i += 64) {
+        const uint64_t x = amat[((size_t)m * 648 + i) * 2];  // < q < 2^39: the high word is zero
+        a[i] = x > half ? (int64_t)x - (int64_t)q : (int64_t)x;
+    }
+    __syncthreads();
+    const int t = 16 * r + (l & 15), g = l >> 4;
+    const int64_t hq = (int64_t)half;
+    for (int s = 0; s < 8; s++) {
+        uint32_t dig[TP_PLANES][4];
+#pragma unroll
+        for (int d = 0; d < TP_PLANES; d++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) dig[d][e] = 0;
+#pragma unroll
+        for (int e = 0; e < 4; e++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int p = 32 * (s >> 1) + 8 * j + 4 * (s & 1) + e;
+                int64_t v = t < 648 ? toep_entry(a, t, 4 * p + g) : 0;
+                if (v > hq) v -= (int64_t)q;  // |v| <= q before, <= q / 2 after
+                else if (v < -hq) v += (int64_t)q;
+#pragma unroll
+                for (int d = 0; d < TP_PLANES; d++) {
+                    const int64_t dd = ((v + 128) & 255) - 128;  // balanced digit in [-128, 127]
+                    v = (v - dd) >> 8;
+                    dig[d][e] |= (uint32_t)(dd & 255) << (8 * j);
+                }
+            }
+        uint4 *o = out + (((size_t)r * re + m) * 8 + s) * TP_PLANES * 64 + l;
+#pragma unroll
+        for (int d = 0; d < TP_PLANES; d++) o[d * 64] = make_uint4(dig[d][0], dig[d][1], dig[d][2], dig[d][3]);
+    }
+}
+
+typedef int tp_v4i __attribute__((ext_vector_type(4)));
+
+// The GEMM: block (tile r, split sp) sums ring elements [m0, m1) of every column for the 16 rows
+// of tile r and the five planes; wave w takes columns 128 w .. 128 w + 127 (8 tiles of 16). The
+// blocks of one split run on one XCD (block b goes to XCD b % 8), so the witness words they all
+// read come from that XCD's L2. Partial sums: part[((sp * 5 + d) * 656 + row) * 1024 + column].
+__global__ __launch_bounds__(64 * TP_WAVES) void k_toep_mac(const uint4 *__restrict__ aop, const uint4 *__restrict__ table,
+                                                            size_t clen, int re, int splits, int32_t *part) {
+    const int nb = TP_TILES * splits, b = blockIdx.x;
+    const int full = nb / 8 * 8;
+    const int lb = b < full ? (b % 8) * (nb / 8) + b / 8 : b;  // logical block: same split, same XCD
+    const int r = lb % TP_TILES, sp = lb / TP_TILES;
+    const int m0 = (int)((int64_t)sp * re / splits), m1 = (int)((int64_t)(sp + 1) * re / splits);
+    const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int g = l >> 4, n = l & 15;
+    tp_v4i acc[TP_PLANES][TP_NT];
+#pragma unroll
+    for (int d = 0; d < TP_PLANES; d++)
+#pragma unroll
+        for (int t = 0; t < TP_NT; t++) acc[d][t] = tp_v4i{0, 0, 0, 0};
+    // word g of element m of column 128 w + 16 t + n: table[(column * clen + 4 m + g)]
+    const uint4 *tb = table + (size_t)(128 * w + n) * clen + g;
+    const uint4 *ab = aop + (size_t)r * re * TP_BLOCK_U4 + l;
+    uint4 an[TP_PLANES];
+#pragma unroll
+    for (int d = 0; d < TP_PLANES; d++) an[d] = ab[(size_t)m0 * TP_BLOCK_U4 + d * 64];
+    for (int m = m0; m < m1; m++) {
+        uint4 wd[TP_NT];
+#pragma unroll
+        for (int t = 0; t < TP_NT; t++) wd[t] = tb[(size_t)16 * t * clen + 4 * (size_t)m];
+        const uint4 *am = ab + (size_t)m * TP_BLOCK_U4;
+#pragma unroll 2
+        for (int s = 0; s < 8; s++) {
+            tp_v4i a[TP_PLANES];
+#pragma unroll
+            for (int d = 0; d < TP_PLANES; d++) a[d] = tp_v4i{(int)an[d].x, (int)an[d].y, (int)an[d].z, (int)an[d].w};
+            // the next step's operands (the last step of the last element reloads its own block)
+            const uint4 *nx = s < 7 ? am + (s + 1) * TP_PLANES * 64 : (m + 1 < m1 ? am + TP_BLOCK_U4 : am + 7 * TP_PLANES * 64);
+#pragma unroll
+            for (int d = 0; d < TP_PLANES; d++) an[d] = nx[d * 64];
+            const int h = 4 * (s & 1);
+#pragma unroll
+            for (int t = 0; t < TP_NT; t++) {
+                const uint32_t x = (s >> 1) == 0 ? wd[t].x : (s >> 1) == 1 ? wd[t].y : (s >> 1) == 2 ? wd[t].z : wd[t].w;
+                const tp_v4i bv = tp_v4i{(int)((x >> h) & 0x01010101u), (int)((x >> (h + 1)) & 0x01010101u),
+                                         (int)((x >> (h + 2)) & 0x01010101u), (int)((x >> (h + 3)) & 0x01010101u)};
+#pragma unroll
+                for (int d = 0; d < TP_PLANES; d++)
+                    acc[d][t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[d], bv, acc[d][t], 0, 0, 0);
+            }
+        }
+    }
+    // D layout: lane l holds rows 4 g + i (i < 4) of the tile, column n of each column tile
+#pragma unroll
+    for (int d = 0; d < TP_PLANES; d++)
+#pragma unroll
+        for (int t = 0; t < TP_NT; t++)
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+                part[(((size_t)sp * TP_PLANES + d) * TP_ROWS + 16 * r + 4 * g + i) * COLUMNS + 128 * w + 16 * t + n] =
+                    acc[d][t][i];
+}
+
+// Sums the splits, recombines the planes (C = sum_d 256^d S_d, |C| <= 2^20 Q < 2^59), reduces mod Q
+// and packs `width` bytes per coefficient at out[(column * 648 + t) * width]. Block (x, y): columns
+// 64 x .. 64 x + 63, rows 16 y .. 16 y + 15; 256 threads.
+__global__ __launch_bounds__(256) void k_toep_finish(const int32_t *part, int splits, uint64_t q, int width,
+                                                     uint8_t *out) {
+    __shared__ uint64_t c[64][17];
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const int col = 64 * blockIdx.x + tx;
+    for (int k = 0; k < 4; k++) {
+        const int row = 16 * blockIdx.y + 4 * k + ty;
+        int64_t v = 0;
+#pragma unroll
+        for (int d = 0; d < TP_PLANES; d++) {
+            int32_t sd = 0;  // |S_d| <= 2^28
+            for (int sp = 0; sp < splits; sp++)
+                sd += part[(((size_t)sp * TP_PLANES + d) * TP_ROWS + row) * COLUMNS + col];
+            v += (int64_t)sd * ((int64_t)1 << (8 * d));
+        }
+        int64_t rm = v % (int64_t)q;
+        if (rm < 0) rm += (int64_t)q;
+        c[tx][4 * k + ty] = (uint64_t)rm;
+    }
+    __syncthreads();
+    const int rows = min(16, 648 - 16 * (int)blockIdx.y);
+    const int span = 16 * width;  // bytes of one column's 16 rows
+    for (int pos = threadIdx.x; pos < 64 * span; pos += 256) {
+        const int cc = pos / span, within = pos % span;
+        const int row = within / width, byte = within % width;
+        if (row < rows)
+            out[((size_t)(64 * blockIdx.x + cc) * 648 + 16 * blockIdx.y + row) * width + byte] =
+                (uint8_t)(c[cc][row] >> (8 * byte));
+    }
+}
+
 struct CvTrees {
     int n;
     uint32_t start[MAX_CV_TREES];
@@ -2492,7 +2669,7 @@ void lbn::dev_free(lbn_ctx *ctx) {
     hipSetDevice(ctx->device);
     void *ptrs[] = {d->table, d->E[0], d->E[1], d->P[0], d->P[1], d->T, d->EL, d->eqrow, d->eqlo,
                     d->fold_off, d->ntt_tabs, d->ahat, d->cacc, d->cbytes, d->amat, d->cvs, d->sw,
-                    d->chal};
+                    d->chal, d->tpart};
     if (d->stream) hipStreamSynchronize(d->stream);
     if (d->copy_stream) hipStreamSynchronize(d->copy_stream);
     for (void *p : ptrs)
@@ -2508,6 +2685,7 @@ void lbn::dev_free(lbn_ctx *ctx) {
     if (d->switch_graph) hipGraphExecDestroy(d->switch_graph);
     if (d->own_copy) hipStreamDestroy(d->own_copy);
     if (d->own_stream) hipStreamDestroy(d->own_stream);
+    d->toep.reset();
     delete d;
     ctx->dev = nullptr;
 }
@@ -2692,13 +2870,40 @@ static size_t commit_buffer_bytes(int np, int width, size_t re) {
     return (size_t)np * mac_splits(re) * COLUMNS * N * 4 + (clen + 64) + (clen / 1024 * 32 + 64);
 }
 
+// The matrix-core commitment serves moduli below 2^39: a centred entry (|e| <= Q/2 < 2^38) is five
+// balanced signed bytes and |C| <= 2^20 Q fits an int64. Larger moduli use the CRT transform.
+static bool toep_modulus(u128 q) { return q < ((u128)1 << 39); }
+// Ring-element splits of the GEMM: 41 x 31 = 1271 blocks (4.96 rounds of 256) at 4096 elements.
+static int toep_splits(size_t re) { return (int)std::min<size_t>(re, 31); }
+static size_t toep_ops_bytes(size_t re) { return (size_t)TP_TILES * re * TP_BLOCK_U4 * 16; }
+static size_t toep_part_bytes(size_t re) { return (size_t)toep_splits(re) * TP_PLANES * TP_ROWS * COLUMNS * 4; }
+
+// The GEMM's matrix operands depend only on (device, Q, ring elements): built once, shared by the
+// contexts that use them (weak entries: freed with the last context holding them).
+struct ToepOps {
+    int device = 0;
+    uint4 *ptr = nullptr;
+    ~ToepOps() {
+        if (ptr) {
+            int cur = 0;
+            (void)hipGetDevice(&cur);
+            (void)hipSetDevice(device);
+            (void)hipFree(ptr);
+            (void)hipSetDevice(cur);
+        }
+    }
+};
+static std::mutex toep_mu;
+static std::map<std::tuple<int, u128, size_t>, std::weak_ptr<ToepOps>> toep_cache;
+
 extern "C" size_t lbn_ctx_footprint(int log_vars, uint64_t q_lo, uint64_t q_hi) {
     Shape s;
     if (!Shape::of(log_vars, s)) return 0;
     const u128 q = ((u128)q_hi << 64) | q_lo;
     if (q < (u128)34816965049ull || q >= ((u128)1 << 96)) return 0;
-    const int np = crt_prime_count(q, s.ring_elements);
-    if (!np) return 0;
+    const bool toep = toep_modulus(q);
+    const int np = toep ? 0 : crt_prime_count(q, s.ring_elements);
+    if (!toep && !np) return 0;
     const size_t n = s.n_words;
     size_t b = n * 16 + n * 16 + n / 2 * 16 + n / 2 * 24 + n / 4 * 24;  // table, E0, E1, P0, P1
     b += 3584 * 4 + ((size_t)1 << (s.log_packed - 12)) * 16 + 4096 * 16;  // T, EH, EL
@@ -2708,6 +2913,7 @@ extern "C" size_t lbn_ctx_footprint(int log_vars, uint64_t q_lo, uint64_t q_hi) 
     b += (size_t)np * s.ring_elements * N * 4;  // ahat
     b += s.ring_elements * N * 16;              // matrix staging while the modulus is set
     b += commit_buffer_bytes(np, modulus_width(q), s.ring_elements);
+    if (toep) b += toep_ops_bytes(s.ring_elements) + toep_part_bytes(s.ring_elements);  // counted per context
     return b;
 }
 
@@ -2762,7 +2968,7 @@ static int alloc_commit_buffers(lbn_ctx *ctx, int np, int width) {
     d->alloc_primes = d->alloc_width = 0;
     const size_t clen = (size_t)COLUMNS * N * width;
     d->splits = mac_splits(ctx->shape.ring_elements);
-    HIPCHK(ctx, hipMalloc(&d->cacc, (size_t)np * d->splits * COLUMNS * N * 4));
+    if (np) HIPCHK(ctx, hipMalloc(&d->cacc, (size_t)np * d->splits * COLUMNS * N * 4));
     HIPCHK(ctx, hipMalloc(&d->cbytes, clen + 64));  // slack: the chunk hashing reads one word past
     HIPCHK(ctx, hipMalloc(&d->cvs, clen / 1024 * 32 + 64));
     HIPCHK(ctx, hipHostMalloc((void **)&d->hstage, clen));
@@ -2770,6 +2976,40 @@ static int alloc_commit_buffers(lbn_ctx *ctx, int np, int width) {
     HIPCHK(ctx, hipHostGetDevicePointer((void **)&d->hcvs_dev, d->hcvs, 0));
     d->alloc_primes = np;
     d->alloc_width = width;
+    return LBN_OK;
+}
+
+// The matrix-core commitment's operands for Q (from the cache, or built here from the public
+// matrix) and the context's partial-sum slices.
+static int setup_toep(lbn_ctx *ctx, u128 q) {
+    DeviceState *d = ctx->dev;
+    const size_t re = ctx->shape.ring_elements;
+    if (!d->tpart) {
+        HIPCHK(ctx, hipMalloc(&d->tpart, toep_part_bytes(re)));
+        d->tp_splits = toep_splits(re);
+    }
+    const auto key = std::make_tuple(ctx->device, q, re);
+    std::lock_guard<std::mutex> lk(toep_mu);
+    if (auto sp = toep_cache[key].lock()) {
+        d->toep = sp;
+        return LBN_OK;
+    }
+    d->toep.reset();
+    auto ops = std::make_shared<ToepOps>();
+    ops->device = ctx->device;
+    HIPCHK(ctx, hipMalloc(&ops->ptr, toep_ops_bytes(re)));
+    const std::shared_ptr<const std::vector<u128>> A = public_matrix(q, re);
+    HIPCHK(ctx, hipMalloc(&d->amat, re * N * 16));
+    HIPCHK(ctx, hipMemcpyAsync(d->amat, A->data(), A->size() * 16, hipMemcpyHostToDevice, d->stream));
+    k_toep_build<<<dim3(TP_TILES, (unsigned)re), 64, 0, d->stream>>>(d->amat, (int)re, (uint64_t)q, ops->ptr);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipStreamSynchronize(d->stream));
+    HIPCHK(ctx, hipFree(d->amat));
+    d->amat = nullptr;
+    for (auto it = toep_cache.begin(); it != toep_cache.end();)
+        it = it->second.expired() ? toep_cache.erase(it) : std::next(it);
+    toep_cache[key] = ops;
+    d->toep = ops;
     return LBN_OK;
 }
 
@@ -2781,14 +3021,24 @@ extern "C" int lbn_set_modulus(lbn_ctx *ctx, uint64_t q_lo, uint64_t q_hi) {
     HIPCHK(ctx, hipSetDevice(ctx->device));
     DeviceState *d = ctx->dev;
     const Shape &s = ctx->shape;
-    const int np = crt_prime_count(q, s.ring_elements);
-    if (!np) return ctx->fail(LBN_E_MODULUS, "modulus too large for the CRT range");
+    const bool toep = toep_modulus(q);
+    const int np = toep ? 0 : crt_prime_count(q, s.ring_elements);
+    if (!toep && !np) return ctx->fail(LBN_E_MODULUS, "modulus too large for the CRT range");
     ctx->modulus_set = false;
     {
         const int rc = alloc_commit_buffers(ctx, np, modulus_width(q));
         if (rc != LBN_OK) return rc;
     }
     ctx->nprimes = np;
+    if (toep) {
+        const int rc = setup_toep(ctx, q);
+        if (rc != LBN_OK) return rc;
+        ctx->q = q;
+        ctx->width = modulus_width(q);
+        ctx->modulus_set = true;
+        return LBN_OK;
+    }
+    d->toep.reset();
     std::vector<uint32_t> all;
     if (!setup_primes(np, ctx->primes, d->roots, all)) return ctx->fail(LBN_E_ARG, "NTT table construction failed");
     HIPCHK(ctx, hipMemcpyAsync(d->ntt_tabs, all.data(), all.size() * 4, hipMemcpyHostToDevice, d->stream));
@@ -2821,6 +3071,9 @@ extern "C" int lbn_set_modulus(lbn_ctx *ctx, uint64_t q_lo, uint64_t q_hi) {
 }
 
 extern "C" int lbn_commit_primes(const lbn_ctx *ctx) { return ctx ? ctx->nprimes : 0; }
+extern "C" int lbn_commit_planes(const lbn_ctx *ctx) {
+    return ctx && ctx->modulus_set && ctx->dev && ctx->dev->toep ? TP_PLANES : 0;
+}
 
 static uint32_t powmod32(uint64_t a, uint64_t e, uint32_t p) {
     uint64_t r = 1;
@@ -2974,8 +3227,23 @@ int lbn::dev_commit_launch(lbn_ctx *ctx) {
     // a staging copy of an earlier commitment still reading cbytes finishes first
     HIPCHK(ctx, hipStreamWaitEvent(ctx->dev->stream, ctx->dev->ev_staged, 0));
     DeviceState *d = ctx->dev;
-    const int np = ctx->nprimes;
     const int re = (int)s.ring_elements;
+    if (d->toep) {
+        const int sp = d->tp_splits;
+        tstart(ctx, T_MAC);
+        k_toep_mac<<<TP_TILES * sp, 64 * TP_WAVES, 0, d->stream>>>(d->toep->ptr, d->table, s.column_len, re, sp,
+                                                                    d->tpart);
+        tstop(ctx, T_MAC);
+        HIPCHK(ctx, hipGetLastError());
+        tstart(ctx, T_FINISH);
+        k_toep_finish<<<dim3(COLUMNS / 64, TP_TILES), 256, 0, d->stream>>>(d->tpart, sp, (uint64_t)ctx->q, ctx->width,
+                                                                          d->cbytes);
+        tstop(ctx, T_FINISH);
+        HIPCHK(ctx, hipGetLastError());
+        HIPCHK(ctx, hipEventRecord(d->ev_commit, d->stream));
+        return LBN_OK;
+    }
+    const int np = ctx->nprimes;
     const int m_per = mac_m_per(re), msplit = mac_splits(re);
     if (msplit != d->splits) return ctx->fail(LBN_E_STATE, "MAC slices not allocated for this shape");
     tstart(ctx, T_MAC);

@@ -406,7 +406,9 @@ def main():
         "ms_per_step": round(statistics.mean(times), 3), "higher_is_better": False, "scaling": "weak",
         "vs_baseline": None, "dtype": "u32/gf2", "data": "synthetic (BLAKE3 tables from fixed seeds)",
         "config": {"workload": f"one proof, 2^{k} Boolean evaluations (2^{k - 7} GHASH words), Q=35039150080, "
-                               f"{primes} CRT primes", "log_vars": k, "modulus": m.DEFAULT_Q},
+                               + (f"commitment on the matrix cores ({planes} int8 planes)" if planes
+                                  else f"{primes} CRT primes"),
+                   "log_vars": k, "modulus": m.DEFAULT_Q},
         "roofline": roofline,
         "passes_ms": {n: round(v["ms"], 3) for n, v in passes.items()},
         "host_phases_ms": {f.replace("_us", ""): round(statistics.mean(pr[f] for pr in profs) / 1000, 3)

@@ -85,7 +85,7 @@ struct lbn::DeviceState {
     // The matrix-core commitment (moduli below 2^39): its operands, shared by the contexts of one
     // (device, Q, shape), and its partial sums.
     std::shared_ptr<::ToepOps> toep;
-    int32_t *tpart = nullptr;
+    int64_t *tpart = nullptr;
     int tp_splits = 0;
 };
 
@@ -2150,7 +2150,7 @@ typedef int tp_v4i __attribute__((ext_vector_type(4)));
 // blocks of one split run on one XCD (block b goes to XCD b % 8), so the witness words they all
 // read come from that XCD's L2. Partial sums: part[((sp * 5 + d) * 656 + row) * 1024 + column].
 __global__ __launch_bounds__(64 * TP_WAVES) void k_toep_mac(const uint4 *__restrict__ aop, const uint4 *__restrict__ table,
-                                                            size_t clen, int re, int splits, int32_t *part) {
+                                                            size_t clen, int re, int splits, int64_t *part) {
     const int nb = TP_TILES * splits, b = blockIdx.x;
     const int full = nb / 8 * 8;
     const int lb = b < full ? (b % 8) * (nb / 8) + b / 8 : b;  // logical block: same split, same XCD
@@ -2165,65 +2165,84 @@ __global__ __launch_bounds__(64 * TP_WAVES) void k_toep_mac(const uint4 *__restr
         for (int t = 0; t < TP_NT; t++) acc[d][t] = tp_v4i{0, 0, 0, 0};
     // word g of element m of column 128 w + 16 t + n: table[(column * clen + 4 m + g)]
     const uint4 *tb = table + (size_t)(128 * w + n) * clen + g;
-    const uint4 *ab = aop + (size_t)r * re * TP_BLOCK_U4 + l;
-    uint4 an[TP_PLANES];
+    // the element's operand blocks (40 KiB) are copied into LDS by the whole block, one element
+    // ahead, with direct-to-LDS loads (no registers); lane i of wave w copies uint4s k*512 + 64w + i
+    __shared__ uint4 sA[2][TP_BLOCK_U4];
+    const uint4 *ablk = aop + (size_t)r * re * TP_BLOCK_U4 + threadIdx.x;
+    // (issued as assembly: the copies write LDS behind the compiler's back, so it adds no waits
+    // before the LDS reads of the other buffer; the element loop waits for them explicitly)
+    auto dma = [&](int mm, int bb) {
 #pragma unroll
-    for (int d = 0; d < TP_PLANES; d++) an[d] = ab[(size_t)m0 * TP_BLOCK_U4 + d * 64];
+        for (int k = 0; k < TP_BLOCK_U4 / 512; k++) {
+            const uint32_t lds = __builtin_amdgcn_readfirstlane((uint32_t)(uintptr_t)&sA[bb][k * 512 + w * 64]);
+            const uint4 *src = ablk + (size_t)mm * TP_BLOCK_U4 + k * 512;
+            asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(lds), "v"(src) : "memory", "m0");
+        }
+    };
+    // word g of element m as two 8-byte halves: dwords 0-1 (steps 0-3) and 2-3 (steps 4-7)
+    const uint2 *tb2 = (const uint2 *)tb;
+    auto half_at = [&](int t, int mm, int hf) { return tb2[2 * ((size_t)16 * t * clen + 4 * (size_t)mm) + hf]; };
+    uint2 wxy[TP_NT], wzw[TP_NT];
+#pragma unroll
+    for (int t = 0; t < TP_NT; t++) wxy[t] = half_at(t, m0, 0);
+    dma(m0, 0);
     for (int m = m0; m < m1; m++) {
-        uint4 wd[TP_NT];
+        const int buf = (m - m0) & 1;
+        const int mn = m + 1 < m1 ? m + 1 : m;
+        __builtin_amdgcn_s_waitcnt(0);  // this wave's copies of element m and its words 0-1 have landed
+        __syncthreads();                // every wave's copies; every wave is done with the other buffer
 #pragma unroll
-        for (int t = 0; t < TP_NT; t++) wd[t] = tb[(size_t)16 * t * clen + 4 * (size_t)m];
-        const uint4 *am = ab + (size_t)m * TP_BLOCK_U4;
-#pragma unroll 2
+        for (int t = 0; t < TP_NT; t++) wzw[t] = half_at(t, m, 1);  // issued before the next copies
+        __builtin_amdgcn_sched_barrier(0);
+        if (m + 1 < m1) dma(m + 1, buf ^ 1);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
         for (int s = 0; s < 8; s++) {
+            __builtin_amdgcn_sched_barrier(0);  // keep each step's reads with its products
             tp_v4i a[TP_PLANES];
 #pragma unroll
-            for (int d = 0; d < TP_PLANES; d++) a[d] = tp_v4i{(int)an[d].x, (int)an[d].y, (int)an[d].z, (int)an[d].w};
-            // the next step's operands (the last step of the last element reloads its own block)
-            const uint4 *nx = s < 7 ? am + (s + 1) * TP_PLANES * 64 : (m + 1 < m1 ? am + TP_BLOCK_U4 : am + 7 * TP_PLANES * 64);
-#pragma unroll
-            for (int d = 0; d < TP_PLANES; d++) an[d] = nx[d * 64];
+            for (int d = 0; d < TP_PLANES; d++) a[d] = __builtin_bit_cast(tp_v4i, sA[buf][(s * TP_PLANES + d) * 64 + l]);
             const int h = 4 * (s & 1);
 #pragma unroll
             for (int t = 0; t < TP_NT; t++) {
-                const uint32_t x = (s >> 1) == 0 ? wd[t].x : (s >> 1) == 1 ? wd[t].y : (s >> 1) == 2 ? wd[t].z : wd[t].w;
+                const uint32_t x = s < 2 ? wxy[t].x : s < 4 ? wxy[t].y : s < 6 ? wzw[t].x : wzw[t].y;
                 const tp_v4i bv = tp_v4i{(int)((x >> h) & 0x01010101u), (int)((x >> (h + 1)) & 0x01010101u),
                                          (int)((x >> (h + 2)) & 0x01010101u), (int)((x >> (h + 3)) & 0x01010101u)};
 #pragma unroll
                 for (int d = 0; d < TP_PLANES; d++)
                     acc[d][t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[d], bv, acc[d][t], 0, 0, 0);
             }
+            if (s == 3) {  // dwords 0-1 of the next element, after their last use here
+#pragma unroll
+                for (int t = 0; t < TP_NT; t++) wxy[t] = half_at(t, mn, 0);
+            }
         }
     }
     // D layout: lane l holds rows 4 g + i (i < 4) of the tile, column n of each column tile
 #pragma unroll
-    for (int d = 0; d < TP_PLANES; d++)
+    for (int t = 0; t < TP_NT; t++)
 #pragma unroll
-        for (int t = 0; t < TP_NT; t++)
+        for (int i = 0; i < 4; i++) {
+            int64_t v = 0;
 #pragma unroll
-            for (int i = 0; i < 4; i++)
-                part[(((size_t)sp * TP_PLANES + d) * TP_ROWS + 16 * r + 4 * g + i) * COLUMNS + 128 * w + 16 * t + n] =
-                    acc[d][t][i];
+            for (int d = 0; d < TP_PLANES; d++) v += (int64_t)acc[d][t][i] * ((int64_t)1 << (8 * d));
+            part[((size_t)sp * TP_ROWS + 16 * r + 4 * g + i) * COLUMNS + 128 * w + 16 * t + n] = v;
+        }
 }
 
-// Sums the splits, recombines the planes (C = sum_d 256^d S_d, |C| <= 2^20 Q < 2^59), reduces mod Q
-// and packs `width` bytes per coefficient at out[(column * 648 + t) * width]. Block (x, y): columns
-// 64 x .. 64 x + 63, rows 16 y .. 16 y + 15; 256 threads.
-__global__ __launch_bounds__(256) void k_toep_finish(const int32_t *part, int splits, uint64_t q, int width,
+// Sums the splits (|C| <= 2^20 Q < 2^59), reduces mod Q and packs `width` bytes per coefficient at
+// out[(column * 648 + t) * width]. Block (x, y): columns 64 x .. 64 x + 63, rows 16 y .. 16 y + 15;
+// 256 threads.
+__global__ __launch_bounds__(256) void k_toep_finish(const int64_t *part, int splits, uint64_t q, int width,
                                                      uint8_t *out) {
     __shared__ uint64_t c[64][17];
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
     const int col = 64 * blockIdx.x + tx;
+#pragma unroll
     for (int k = 0; k < 4; k++) {
         const int row = 16 * blockIdx.y + 4 * k + ty;
         int64_t v = 0;
-#pragma unroll
-        for (int d = 0; d < TP_PLANES; d++) {
-            int32_t sd = 0;  // |S_d| <= 2^28
-            for (int sp = 0; sp < splits; sp++)
-                sd += part[(((size_t)sp * TP_PLANES + d) * TP_ROWS + row) * COLUMNS + col];
-            v += (int64_t)sd * ((int64_t)1 << (8 * d));
-        }
+        for (int sp = 0; sp < splits; sp++) v += part[((size_t)sp * TP_ROWS + row) * COLUMNS + col];
         int64_t rm = v % (int64_t)q;
         if (rm < 0) rm += (int64_t)q;
         c[tx][4 * k + ty] = (uint64_t)rm;
@@ -2876,7 +2895,7 @@ static bool toep_modulus(u128 q) { return q < ((u128)1 << 39); }
 // Ring-element splits of the GEMM: 41 x 31 = 1271 blocks (4.96 rounds of 256) at 4096 elements.
 static int toep_splits(size_t re) { return (int)std::min<size_t>(re, 31); }
 static size_t toep_ops_bytes(size_t re) { return (size_t)TP_TILES * re * TP_BLOCK_U4 * 16; }
-static size_t toep_part_bytes(size_t re) { return (size_t)toep_splits(re) * TP_PLANES * TP_ROWS * COLUMNS * 4; }
+static size_t toep_part_bytes(size_t re) { return (size_t)toep_splits(re) * TP_ROWS * COLUMNS * 8; }
 
 // The GEMM's matrix operands depend only on (device, Q, ring elements): built once, shared by the
 // contexts that use them (weak entries: freed with the last context holding them).

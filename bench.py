@@ -7,8 +7,10 @@ starts (the reference's harness also writes the table outside the timer). The de
 the reference's Q = 35,039,150,080.
 
 Besides the JSON line it prints per-pass kernel times (HIP events on the library's stream), the
-dominant kernel's roofline, and a CPU baseline: the scalar C oracle (same algorithm, port) timed on
-a bounded sample (a full proof at 2^23 evaluations) and scaled by the work ratio to 2^31.
+dominant kernel's roofline, and whole-process timings made before this process touches the GPU:
+the CPU baseline (the oracle's command line, the same algorithm in scalar C with OpenMP, proving at
+the same 2^31 shape from a table file on all host cores and on 8) and the labinius-fast executable,
+both spawn to exit.
 
     python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--log-vars 31] [--no-cpu-baseline] [--no-batched]
                     [--no-commit-micro]
@@ -74,20 +76,83 @@ def bench_seed(i: int) -> bytes:
     return hashlib.sha256(b"labinius-mi355x/bench-seed/v1" + i.to_bytes(8, "little")).digest()
 
 
-def cpu_baseline(log_vars_target: int):
-    """The oracle's whole proof at k=23, scaled to the target shape (bounded sample, ~1-30 s)."""
-    import oracle_ctypes as O
-    O.build()
-    k = 23
-    seed = bench_seed(10_000)
-    tab = O.table(seed, k)
+REFERENCE_PUBLISHED_MS = 2972.27  # .yukon/README.md:162-178 (older baseline, Xeon 8488C, 8 CPUs, Q=55,144,219,393)
+ORACLE_CLI = os.path.join(ROOT, "oracle", "build", "labinius-oracle")
+GPU_CLI = os.path.join(ROOT, "labinius-fast-challenge_amd", "build", "labinius-fast")
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def timed_process(cmd, env, cpus=None):
+    """Wall-clock milliseconds from spawn to exit (the harness's timer, .yukon/benchmark.sh:142-162)
+    and the process's stderr."""
+    import subprocess
+    pre = ["taskset", "-c", cpus] if cpus else []
     t0 = time.perf_counter()
-    O.prove(seed, k, O.DEFAULT_Q, tab)
-    dt = time.perf_counter() - t0
-    scale = 2 ** (log_vars_target - k)
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": round(dt * 1000 * scale, 1), "unit": "ms", "cores": cores, "kind": "port",
-            "sample": f"CPU oracle whole proof at k={k} ({dt * 1000:.0f} ms, OpenMP), x{scale} work ratio to k={log_vars_target}"}
+    r = subprocess.run(pre + cmd, env=env, capture_output=True, text=True, timeout=900)
+    dt = (time.perf_counter() - t0) * 1000
+    if r.returncode != 0:
+        raise RuntimeError(f"{os.path.basename(cmd[0])} exited {r.returncode}: {r.stderr[-400:]}")
+    return dt, r.stderr
+
+
+def process_runs(k, cpu_seeds, gpu_seeds, want_cpu):
+    """Whole-process timings at 2^k evaluations, before this process touches the GPU: the table is
+    written to a file first (outside the timer, as the harness does), then each prover runs as a
+    child process reading it, spawn to exit. CPU baseline: the oracle's command line (the same
+    algorithm in scalar C with OpenMP) on all host cores and on 8 (taskset -c 0-7); GPU: the
+    labinius-fast executable (process start, device set-up, table read and upload, matrix, proof,
+    proof write). Returns (cpu_baseline record or None, process record)."""
+    import shutil
+    import tempfile
+    tmp = tempfile.mkdtemp(prefix="lbn-bench-")
+    try:
+        table = os.path.join(tmp, "witness-table.bin")
+        with open(table, "wb") as f:
+            f.write(np.random.default_rng(20261204).bytes((1 << k) // 8))
+        base = dict(os.environ, LABINIUS_LOG_VARS=str(k))
+        nproc = os.cpu_count() or 1
+        cpu = None
+        if want_cpu and os.path.exists(ORACLE_CLI):
+            runs_all, runs_8 = [], []
+            for i in range(cpu_seeds):
+                env = dict(base, LABINIUS_SEED=bench_seed(i).hex(), OMP_NUM_THREADS=str(nproc))
+                runs_all.append(timed_process([ORACLE_CLI, "--prove", os.path.join(tmp, "cpu.bin"), "--table", table],
+                                              env)[0])
+                env8 = dict(env, OMP_NUM_THREADS="8")
+                if nproc >= 8 and shutil.which("taskset"):
+                    runs_8.append(timed_process([ORACLE_CLI, "--prove", os.path.join(tmp, "cpu8.bin"), "--table", table],
+                                                env8, cpus="0-7")[0])
+            cpu = {"value": round(statistics.median(runs_all), 1), "unit": "ms", "cores": nproc, "kind": "port",
+                   "sample": f"the oracle's command line (scalar C + OpenMP restatement), whole proof at k={k} "
+                             f"from a table file, spawn to exit, {cpu_seeds} seed(s), median",
+                   "cpu_model": cpu_model(), "nproc": nproc, "runs_ms": [round(t, 1) for t in runs_all],
+                   "cores8_ms": round(statistics.median(runs_8), 1) if runs_8 else None,
+                   "reference_published_ms": REFERENCE_PUBLISHED_MS}
+        gpu = None
+        if os.path.exists(GPU_CLI):
+            runs, prof = [], None
+            for i in range(gpu_seeds):
+                env = dict(base, LABINIUS_SEED=bench_seed(i).hex())
+                dt, err = timed_process([GPU_CLI, "--prove", os.path.join(tmp, "gpu.bin"), "--table", table], env)
+                runs.append(dt)
+                line = [x for x in err.splitlines() if x.startswith("labinius-profile")]
+                if line:
+                    prof = {kv.split("=")[0]: float(kv.split("=")[1]) for kv in line[-1].split()[1:]}
+            gpu = {"median_ms": round(statistics.median(runs), 1), "runs_ms": [round(t, 1) for t in runs],
+                   "what": "labinius-fast --prove P --table T, spawn to exit (process start, device set-up, table "
+                           "read and upload, public matrix, proof, proof write)", "last_profile_us": prof}
+        return cpu, gpu
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
 
 
 def pmc_traffic(pmc_dir, log_vars, kernel="k_commit_mac", copy_kernel="k_copy_u4"):
@@ -307,6 +372,9 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one plain device-to-device copy of the table size first (a known-size kernel "
                          "for scaling profiler byte counters)")
+    ap.add_argument("--cpu-seeds", type=int, default=1, help="seeds of the CPU baseline (each all cores and 8)")
+    ap.add_argument("--process-seeds", type=int, default=5, help="spawn-to-exit runs of the GPU executable")
+    ap.add_argument("--no-process", action="store_true", help="skip the whole-process timings")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the prover's kernels run on one device per proof; "
@@ -315,6 +383,11 @@ def main():
     if not 1 <= args.hw_queues <= 32:
         ap.error("--hw-queues must be between 1 and 32")
     os.environ["GPU_MAX_HW_QUEUES"] = str(args.hw_queues)
+    # child processes first: nothing in this process has touched the GPU yet
+    proc_cpu, proc_gpu = None, None
+    if not (args.serving or args.commit_micro_only or args.batched_only or args.no_process):
+        proc_cpu, proc_gpu = process_runs(args.log_vars, args.cpu_seeds, args.process_seeds,
+                                          not args.no_cpu_baseline)
     import amd
     m = amd.load()
     if args.serving:
@@ -426,8 +499,10 @@ def main():
         line["commit_micro"] = commit_micro(m, max(args.steps, 5), args.warmup)
     if not args.no_batched:
         line["batched"] = batched(m, args.workers or 12)
-    if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(k)
+    if proc_cpu:
+        line["cpu_baseline"] = proc_cpu
+    if proc_gpu:
+        line["process"] = proc_gpu
     print(json.dumps(line), flush=True)
 
 

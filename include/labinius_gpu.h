@@ -109,6 +109,22 @@ int lbn_prove(lbn_ctx *ctx, const uint8_t seed[32], uint8_t *proof, size_t cap, 
 void *lbn_host_alloc(size_t bytes);
 void lbn_host_free(void *p);
 
+/* The 128 partials of lbn_partials (crossfield.rs:152-165; the same values) for a table already in
+ * device memory (16-byte aligned, 2^log_vars / 8 bytes), without a context and without allocating:
+ * no counterpart in the reference; for callers that cannot allocate device memory (code under
+ * stream capture, tight serving loops, harnesses that forbid hipMalloc). Everything besides the
+ * table lives in one file-scope device array of lbn_partials_static_scratch() bytes (136 KiB, enough
+ * for every shape up to log_vars 31), held by each device the library runs on until the process
+ * ends. r_hi: log_vars - 7 B128 (u64 pairs); partials: 128 B128 out (host memory). Calls on one
+ * device share the array; the library serialises them (one at a time per process), so they are
+ * safe to make from several threads but never run concurrently. Returns LBN_E_ARG for a shape or
+ * size beyond the array's limit, LBN_E_DEVICE for a HIP failure; lbn_partials_static_error() gives
+ * the calling thread's last message. */
+int lbn_partials_static(int device, const void *table_device, size_t bytes, int log_vars, const uint64_t *r_hi,
+                        uint64_t *partials);
+size_t lbn_partials_static_scratch(void);
+const char *lbn_partials_static_error(void);
+
 /* Diagnostics (no counterpart in the reference): one plain copy of the table's bytes into a device
  * buffer of the same size, 16 bytes per lane, read once and written once: a kernel of known traffic
  * that scales a profiler's byte counters. Clears the state lbn_partials left. */

@@ -92,6 +92,9 @@ def lib():
         L.lbn_commit_primes.argtypes = [P]
         L.lbn_copy_probe.argtypes = [P]
         L.lbn_commit_planes.argtypes = [P]
+        L.lbn_partials_static.argtypes = [ctypes.c_int, P, ctypes.c_size_t, ctypes.c_int, P, P]
+        L.lbn_partials_static_scratch.restype = ctypes.c_size_t
+        L.lbn_partials_static_error.restype = ctypes.c_char_p
         L.lbn_set_timing.argtypes = [P, ctypes.c_int]
         L.lbn_set_graphs.argtypes = [P, ctypes.c_int]
         L.lbn_reset_timing.argtypes = [P]

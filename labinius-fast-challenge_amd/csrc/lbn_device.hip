@@ -380,6 +380,96 @@ __global__ __launch_bounds__(256) void k_partials(const uint4 *table, const uint
 }
 
 // ---------------------------------------------------------------------------------------------
+// The same partials without the expansion in memory (the allocation-free entry point):
+// eq_hi[jh * 4096 + jl] = EH[jh] EL[jl], so v[k] = sum_jh EH[jh] R_jh[k] with
+// R_jh[k] = sum_jl bit_k(w_{jh*4096+jl}) EL[jl]. One wave per row jh (waves stride over the rows):
+// the wave-local M4RM sums R_jh over the row's 4096 words (EL read from L2), then 128 products by
+// EH[jh] through a 2-bit window table of y -> EH[jh] y (64 windows x 4, 4 KiB: the M4RM region of
+// the wave, free after the row's steps); each lane keeps the sums of bits l and l + 64; the block
+// adds its waves' sums and atomicXors them into out (zeroed before).
+// ---------------------------------------------------------------------------------------------
+// y x^s in GF(2^128) (x^128 = x^7 + x^2 + x + 1), 0 <= s < 128.
+__device__ __forceinline__ uint4 b128_mul_xs(uint4 y, int s) {
+    typedef unsigned __int128 v128;
+    const v128 v = ((v128)(((uint64_t)y.w << 32) | y.z) << 64) | (((uint64_t)y.y << 32) | y.x);
+    v128 r = v;
+    if (s) {
+        const v128 h = v >> (128 - s);  // the s bits shifted out
+        const v128 o = (h >> 127) ^ (h >> 126) ^ (h >> 121);  // the fold's own overflow
+        r = (v << s) ^ h ^ (h << 1) ^ (h << 2) ^ (h << 7) ^ o ^ (o << 1) ^ (o << 2) ^ (o << 7);
+    }
+    const uint64_t lo = (uint64_t)r, hi = (uint64_t)(r >> 64);
+    return make_uint4((uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32));
+}
+
+constexpr int PF_WAVES = 4;
+__global__ __launch_bounds__(64 * PF_WAVES) void k_partials_fact(const uint4 *table, const uint4 *EL, const uint4 *EH,
+                                                                 int rows, uint32_t *out) {
+    __shared__ __align__(16) uint32_t ws[PF_WAVES][M4W<4>::WORDS];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    uint32_t *S = ws[wv];
+    uint4 *win = (uint4 *)S;  // after a row's steps: win[w][v] = EH[jh] (v x^(2w)), 64 windows x 4
+    uint4 prod[2] = {make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0)};  // bits lane, lane + 64
+    for (int jh = blockIdx.x * PF_WAVES + wv; jh < rows; jh += gridDim.x * PF_WAVES) {
+        uint32_t acc[2][4] = {};
+        const uint4 *tw = table + (size_t)jh * 4096;
+        for (int base = 0; base < 4096; base += 128) {  // two items per pass, both loaded first
+            const uint4 w0 = tw[base + lane], e0 = EL[base + lane];
+            const uint4 w1 = tw[base + 64 + lane], e1 = EL[base + 64 + lane];
+            {
+                uint32_t sel[4] = {w0.x, w0.y, w0.z, w0.w};
+                uint32_t val[4] = {e0.x, e0.y, e0.z, e0.w};
+                m4w_step<4>(acc, sel, val, S);
+            }
+            {
+                uint32_t sel[4] = {w1.x, w1.y, w1.z, w1.w};
+                uint32_t val[4] = {e1.x, e1.y, e1.z, e1.w};
+                m4w_step<4>(acc, sel, val, S);
+            }
+        }
+        {  // lane l writes window l: 0, EH x^(2l), EH x^(2l+1) and their sum
+            const uint4 e = EH[jh];
+            const uint4 b0 = b128_mul_xs(e, 2 * lane), b1 = b128_mul_xs(e, 2 * lane + 1);
+            win[4 * lane] = make_uint4(0, 0, 0, 0);
+            win[4 * lane + 1] = b0;
+            win[4 * lane + 2] = b1;
+            win[4 * lane + 3] = make_uint4(b0.x ^ b1.x, b0.y ^ b1.y, b0.z ^ b1.z, b0.w ^ b1.w);
+        }
+        __syncwarp();
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int w = 0; w < 64; w++) {
+                const uint4 y = win[4 * w + ((acc[c][w >> 4] >> (2 * (w & 15))) & 3)];
+                prod[c].x ^= y.x, prod[c].y ^= y.y, prod[c].z ^= y.z, prod[c].w ^= y.w;
+            }
+        __syncwarp();  // the next row's steps rewrite the wave's region
+    }
+    // the block's sum: waves 1.. leave theirs in LDS, wave 0 adds them and the atomics follow
+    __syncthreads();
+    uint4 *red = (uint4 *)ws[wv];
+    red[lane] = prod[0];
+    red[lane + 64] = prod[1];
+    __syncthreads();
+    if (wv == 0) {
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+            uint4 x = make_uint4(0, 0, 0, 0);
+#pragma unroll
+            for (int o = 0; o < PF_WAVES; o++) {
+                const uint4 y = ((const uint4 *)ws[o])[lane + 64 * c];
+                x.x ^= y.x, x.y ^= y.y, x.z ^= y.z, x.w ^= y.w;
+            }
+            uint32_t *dst = out + 4 * (lane + 64 * c);
+            if (x.x) atomicXor(dst, x.x);
+            if (x.y) atomicXor(dst + 1, x.y);
+            if (x.z) atomicXor(dst + 2, x.z);
+            if (x.w) atomicXor(dst + 3, x.w);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
 // Switch round 0 (sumcheck.rs:89-117 with a_j = L(E0_j), p_j = lift(w_j)):
 //   T0_k   = sum_{j<h} bit_k(E0_j) w_j
 //   Tinf_k = sum_{j<h} bit_k(E1_j) (w_j + w_{j+h}),  E1_j = E0_j + E0_{j+h} (written)
@@ -3326,6 +3416,73 @@ extern "C" int lbn_partials(lbn_ctx *ctx, const uint64_t *r_hi, uint64_t *partia
     }
     ctx->partials_done = true;
     ctx->round = -1;
+    return LBN_OK;
+}
+
+// ----------------------------------------------------------------------------- allocation-free partials
+// lbn_partials_static: the partials of a table in the caller's device memory, with every buffer the
+// kernels need besides the table (EL, EH, the 128 sums) in this file-scope array. Host code never
+// passes `scratch` itself: its name is a host-side stand-in, so each device's address comes from
+// hipGetSymbolAddress on the first call made on that device, and is used from then on.
+constexpr size_t STATIC_EL = 0;                         // 4096 B128
+constexpr size_t STATIC_EH = STATIC_EL + 4096 * 16;     // up to 4096 B128 (2^(k-19) rows, k <= 31)
+constexpr size_t STATIC_OUT = STATIC_EH + 4096 * 16;    // 128 B128 accumulators
+constexpr size_t STATIC_SCRATCH_BYTES = STATIC_OUT + 128 * 16;
+static __device__ __align__(256) uint8_t scratch[STATIC_SCRATCH_BYTES];
+static std::mutex static_mu;
+static uint8_t *static_ptr[64];  // per device, from hipGetSymbolAddress
+static thread_local std::string static_err;
+
+static int static_fail(int code, const char *msg) {
+    static_err = msg;
+    return code;
+}
+
+extern "C" size_t lbn_partials_static_scratch(void) { return STATIC_SCRATCH_BYTES; }
+extern "C" const char *lbn_partials_static_error(void) { return static_err.c_str(); }
+
+extern "C" int lbn_partials_static(int device, const void *table_device, size_t bytes, int log_vars,
+                                   const uint64_t *r_hi, uint64_t *partials) {
+    Shape s;
+    if (!Shape::of(log_vars, s)) return static_fail(LBN_E_ARG, "log_vars outside [19, 31]");
+    if (!table_device || !r_hi || !partials) return static_fail(LBN_E_ARG, "null argument");
+    if (bytes != s.table_bytes) return static_fail(LBN_E_ARG, "table length is not 2^log_vars / 8 bytes");
+    if ((uintptr_t)table_device % 16) return static_fail(LBN_E_ARG, "device table must be 16-byte aligned");
+    const int lp = s.log_packed, nh = lp - 12;
+    const size_t rows = (size_t)1 << nh;
+    if (STATIC_EH + rows * 16 > STATIC_OUT)
+        return static_fail(LBN_E_ARG, "more eq rows than the static scratch holds (at most 4096: log_vars <= 31)");
+    if (device < 0 || device >= 64) return static_fail(LBN_E_ARG, "device index outside [0, 64)");
+    std::lock_guard<std::mutex> lk(static_mu);  // one call at a time: the calls share the array
+    if (hipSetDevice(device) != hipSuccess) return static_fail(LBN_E_DEVICE, "hipSetDevice failed");
+    if (!static_ptr[device]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+            (void)hipGetLastError();
+            return static_fail(LBN_E_DEVICE, "hipGetSymbolAddress of the scratch array failed");
+        }
+        static_ptr[device] = (uint8_t *)p;
+    }
+    uint8_t *sc = static_ptr[device];
+    B128 EH[4096], EL[4096];
+    eq_expand_b128((const B128 *)r_hi, nh, EH);
+    eq_expand_b128((const B128 *)r_hi + nh, 12, EL);
+    if (hipMemcpyAsync(sc + STATIC_EL, EL, sizeof EL, hipMemcpyHostToDevice, 0) != hipSuccess ||
+        hipMemcpyAsync(sc + STATIC_EH, EH, rows * 16, hipMemcpyHostToDevice, 0) != hipSuccess)
+        return static_fail(LBN_E_DEVICE, "copy of the eq factors failed");
+    // the sums start from zero: a fill kernel (the array keeps the last call's contents)
+    k_fill_u32<<<1, 256, 0, 0>>>((uint32_t *)(sc + STATIC_OUT), 512, 0u);
+    k_partials_fact<<<(unsigned)std::max<size_t>(1, std::min<size_t>(rows / PF_WAVES, 1024)), 64 * PF_WAVES, 0, 0>>>(
+        (const uint4 *)table_device, (const uint4 *)(sc + STATIC_EL), (const uint4 *)(sc + STATIC_EH), (int)rows,
+        (uint32_t *)(sc + STATIC_OUT));
+    if (hipGetLastError() != hipSuccess) return static_fail(LBN_E_DEVICE, "kernel launch failed");
+    uint32_t out[512];
+    if (hipMemcpy(out, sc + STATIC_OUT, sizeof out, hipMemcpyDeviceToHost) != hipSuccess)
+        return static_fail(LBN_E_DEVICE, "copy of the partials failed");
+    for (int k = 0; k < 128; k++) {
+        partials[2 * k] = (uint64_t)out[4 * k] | ((uint64_t)out[4 * k + 1] << 32);
+        partials[2 * k + 1] = (uint64_t)out[4 * k + 2] | ((uint64_t)out[4 * k + 3] << 32);
+    }
     return LBN_OK;
 }
 

@@ -30,6 +30,7 @@ extern "C" {
 #define LBN_E_MODULUS (-4) /* modulus outside [34,816,965,049, 2^96) */
 #define LBN_E_RANGE (-5)   /* a fold coefficient left i16 (cannot happen for honest input) */
 #define LBN_E_IO (-6)      /* file I/O (CLI helpers) */
+#define LBN_E_REJECT (-7)  /* lbn_verify: the proof does not verify (reason in lbn_last_error) */
 
 typedef struct lbn_ctx lbn_ctx;
 
@@ -101,6 +102,13 @@ int lbn_fold(lbn_ctx *ctx, const lbn_challenge *challenges, int n_challenges, in
 size_t lbn_proof_len(int log_vars, uint64_t q_lo, uint64_t q_hi);
 int lbn_prove(lbn_ctx *ctx, const uint8_t seed[32], uint8_t *proof, size_t cap, size_t *len,
               lbn_profile *prof);
+
+/* The checker (verifier/proof_verify/src/main.rs:381-389 verify, with check_proof_shape,
+ * check_field_switch and check_opening, main.rs:293-379): accepts `proof` (bincode bytes) for the
+ * trace of `seed` at the context's shape under the modulus the proof carries. Returns LBN_OK,
+ * LBN_E_REJECT with the checker's reason in lbn_last_error, or a device error. The table of `seed`
+ * is regenerated into the context (replacing a loaded table); no modulus needs to be set. */
+int lbn_verify(lbn_ctx *ctx, const uint8_t seed[32], const uint8_t *proof, size_t len);
 
 /* Pinned host memory for proof buffers (no counterpart in the reference, which writes the proof
  * from an ordinary Vec): when lbn_prove's proof buffer lies in an lbn_host_alloc allocation, the

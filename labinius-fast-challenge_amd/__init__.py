@@ -16,7 +16,8 @@ CLI_PATH = os.path.join(HERE, "build", "labinius-fast")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "labinius_gpu.h")
 DEFAULT_Q = 35_039_150_080  # solution/proof_generate/src/parameters.rs:14
 
-ERRORS = {-1: "argument", -2: "state", -3: "device", -4: "modulus", -5: "range", -6: "io"}
+ERRORS = {-1: "argument", -2: "state", -3: "device", -4: "modulus", -5: "range", -6: "io", -7: "reject"}
+E_REJECT = -7
 
 
 class LbnError(RuntimeError):
@@ -89,6 +90,7 @@ def lib():
         L.lbn_proof_len.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64]
         L.lbn_proof_len.restype = ctypes.c_size_t
         L.lbn_prove.argtypes = [P, P, P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t), P]
+        L.lbn_verify.argtypes = [P, P, P, ctypes.c_size_t]
         L.lbn_commit_primes.argtypes = [P]
         L.lbn_copy_probe.argtypes = [P]
         L.lbn_commit_planes.argtypes = [P]
@@ -254,6 +256,17 @@ class Prover:
             buf = self._buf = np.zeros(cap, dtype=np.uint8)
         n, prof = self.prove_into(seed, buf)
         return buf[:n].tobytes(), prof
+
+    def verify(self, seed: bytes, proof: bytes):
+        """The checker (verifier/proof_verify/src/main.rs:381-389): None when `proof` verifies for
+        the trace of `seed` at this shape, else the checker's reason (a str). Device errors raise.
+        Regenerates the table of `seed` into this context."""
+        buf = np.frombuffer(proof, dtype=np.uint8) if len(proof) else np.zeros(1, dtype=np.uint8)
+        rc = lib().lbn_verify(self.h, _b(seed), _p(buf), len(proof))
+        if rc == E_REJECT:
+            return lib().lbn_last_error(self.h).decode()
+        self._check(rc)
+        return None
 
     def commit_primes(self) -> int:
         return lib().lbn_commit_primes(self.h)

@@ -3419,6 +3419,125 @@ extern "C" int lbn_partials(lbn_ctx *ctx, const uint64_t *r_hi, uint64_t *partia
     return LBN_OK;
 }
 
+// ----------------------------------------------------------------------------- checker arithmetic
+// The two sides of the checker's identity A v = sum_j c_j(-X^4) C_j (verifier/proof_verify/src/
+// rq.rs:188-256), as exact integer coefficients of the products before the reduction into R_Q
+// (indices e < 1295): each side is returned as three limb sums raw[e][k], k < 3, with
+// raw[e] = sum_k 2^(32 k) raw[e][k]; the host reduces them modulo Q and folds X^648 = X^324 - 1.
+constexpr int VT_RAW = 2 * N - 1;  // 1295
+constexpr int VT_M_PER = 8;        // ring elements per block of the A v kernel (512 blocks at 4096)
+
+// A v: block b takes ring elements [8 b, 8 b + 8); matrix entries in [0, Q), Q < 2^96, as three
+// 32-bit limbs, v as int16. Thread t keeps the sums of e = t + 256 q (q < 6); a sum has at most
+// 8 x 648 terms below 2^32 x 2^15 in size, well inside int64.
+__global__ __launch_bounds__(256) void k_vt_atv(const uint64_t *amat, const int16_t *v, int re, int64_t *out) {
+    __shared__ uint32_t a[3][648];
+    __shared__ int16_t vv[648];
+    const int t = threadIdx.x;
+    const int m0 = blockIdx.x * VT_M_PER, m1 = min(re, m0 + VT_M_PER);
+    int64_t acc[6][3] = {};
+    for (int m = m0; m < m1; m++) {
+        for (int i = t; i < 648; i += 256) {
+            const uint64_t lo = amat[((size_t)m * 648 + i) * 2], hi = amat[((size_t)m * 648 + i) * 2 + 1];
+            a[0][i] = (uint32_t)lo, a[1][i] = (uint32_t)(lo >> 32), a[2][i] = (uint32_t)hi;
+            vv[i] = v[(size_t)m * 648 + i];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 6; q++) {
+            const int e = t + 256 * q;
+            if (e >= VT_RAW) break;
+            const int i0 = max(0, e - 647), i1 = min(e, 647);
+            for (int i = i0; i <= i1; i++) {
+                const int64_t x = vv[i];
+                acc[q][0] += (int64_t)a[0][e - i] * x;
+                acc[q][1] += (int64_t)a[1][e - i] * x;
+                acc[q][2] += (int64_t)a[2][e - i] * x;
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+        const int e = t + 256 * q;
+        if (e >= VT_RAW) break;
+#pragma unroll
+        for (int k = 0; k < 3; k++) out[((size_t)blockIdx.x * VT_RAW + e) * 3 + k] = acc[q][k];
+    }
+}
+
+// sum_j c_j(-X^4) C_j: block (x, y) takes e = 256 x + t and columns 64 y .. 64 y + 63; the
+// coefficient X^(4 pos) of c_j(-X^4) is +-1 with sign (1 - 2 s_t) (-1)^pos. C_j's coefficients
+// are read from the packed commitment (`width` bytes LE each, < Q < 2^96).
+__global__ __launch_bounds__(256) void k_vt_comb(const uint8_t *cbytes, int width, const lbn_challenge *ch,
+                                                 int64_t *out) {
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= VT_RAW) return;
+    int64_t acc[3] = {0, 0, 0};
+    for (int j = blockIdx.y * 64; j < blockIdx.y * 64 + 64; j++) {
+        const lbn_challenge c = ch[j];
+        for (uint32_t s = 0; s < c.weight; s++) {
+            const int pos = c.positions[s];
+            const int idx = e - 4 * pos;
+            if (idx < 0 || idx >= 648) continue;
+            const bool neg = (((c.signs >> s) & 1) != 0) != ((pos & 1) != 0);
+            const uint8_t *b = cbytes + ((size_t)j * 648 + idx) * width;
+            uint32_t limb[3] = {0, 0, 0};
+            for (int i = 0; i < width; i++) limb[i >> 2] |= (uint32_t)b[i] << (8 * (i & 3));
+#pragma unroll
+            for (int k = 0; k < 3; k++) acc[k] += neg ? -(int64_t)limb[k] : (int64_t)limb[k];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++) out[((size_t)blockIdx.y * VT_RAW + e) * 3 + k] = acc[k];
+}
+
+int lbn::dev_verify_products(lbn_ctx *ctx, u128 q, const int16_t *v, const uint8_t *commitment, int width,
+                             const lbn_challenge *ch, __int128 *lhs, __int128 *rhs) {
+    const size_t re = ctx->shape.ring_elements;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    const std::shared_ptr<const std::vector<u128>> A = public_matrix(q, re);
+    const int nb = (int)((re + VT_M_PER - 1) / VT_M_PER), ny = COLUMNS / 64;
+    const size_t clen = (size_t)COLUMNS * N * width;
+    uint64_t *damat = nullptr;
+    int16_t *dv = nullptr;
+    uint8_t *dc = nullptr;
+    lbn_challenge *dch = nullptr;
+    int64_t *dout = nullptr;
+    std::vector<int64_t> out((size_t)(nb + ny) * VT_RAW * 3);
+    auto release = [&]() {
+        for (void *p : {(void *)damat, (void *)dv, (void *)dc, (void *)dch, (void *)dout})
+            if (p) (void)hipFree(p);
+    };
+    hipError_t e = hipSuccess;
+    auto ok = [&](hipError_t r) {
+        if (e == hipSuccess) e = r;
+        return e == hipSuccess;
+    };
+    if (ok(hipMalloc(&damat, re * N * 16)) && ok(hipMalloc(&dv, re * N * 2)) && ok(hipMalloc(&dc, clen + 16)) &&
+        ok(hipMalloc(&dch, COLUMNS * sizeof(lbn_challenge))) && ok(hipMalloc(&dout, out.size() * 8)) &&
+        ok(hipMemcpyAsync(damat, A->data(), re * N * 16, hipMemcpyHostToDevice, ctx->dev->stream)) &&
+        ok(hipMemcpyAsync(dv, v, re * N * 2, hipMemcpyHostToDevice, ctx->dev->stream)) &&
+        ok(hipMemcpyAsync(dc, commitment, clen, hipMemcpyHostToDevice, ctx->dev->stream)) &&
+        ok(hipMemcpyAsync(dch, ch, COLUMNS * sizeof(lbn_challenge), hipMemcpyHostToDevice, ctx->dev->stream))) {
+        k_vt_atv<<<nb, 256, 0, ctx->dev->stream>>>(damat, dv, (int)re, dout);
+        k_vt_comb<<<dim3((VT_RAW + 255) / 256, ny), 256, 0, ctx->dev->stream>>>(dc, width, dch,
+                                                                               dout + (size_t)nb * VT_RAW * 3);
+        if (ok(hipGetLastError()))
+            if (ok(hipMemcpyAsync(out.data(), dout, out.size() * 8, hipMemcpyDeviceToHost, ctx->dev->stream)))
+                ok(hipStreamSynchronize(ctx->dev->stream));
+    }
+    release();
+    if (e != hipSuccess) return ctx->fail(LBN_E_DEVICE, std::string("checker products: ") + hipGetErrorString(e));
+    for (int i = 0; i < VT_RAW * 3; i++) {
+        __int128 l = 0, r = 0;
+        for (int b = 0; b < nb; b++) l += out[(size_t)b * VT_RAW * 3 + i];
+        for (int y = 0; y < ny; y++) r += out[(size_t)(nb + y) * VT_RAW * 3 + i];
+        lhs[i] = l, rhs[i] = r;
+    }
+    return LBN_OK;
+}
+
 // ----------------------------------------------------------------------------- allocation-free partials
 // lbn_partials_static: the partials of a table in the caller's device memory, with every buffer the
 // kernels need besides the table (EL, EH, the 128 sums) in this file-scope array. Host code never

@@ -715,3 +715,360 @@ extern "C" int lbn_prove(lbn_ctx *ctx, const uint8_t seed[32], uint8_t *proof, s
     if (prof) *prof = pf;
     return LBN_OK;
 }
+
+// ============================================================================= checker
+// verify (verifier/proof_verify/src/main.rs:381-389) with check_proof_shape, check_field_switch and
+// check_opening (main.rs:293-379) over the bincode bytes of `Proof` (main.rs:98-124). The GHASH
+// trace is regenerated on the device from the seed (main.rs:135-160) and the device computes the
+// trusted partials, the folding challenges and the integer sums of the ring identity; the
+// transcript, the switch verifier, the F162 checks and the residues of the sums stay on the host.
+
+namespace {
+constexpr double FOLD_CAP = 108.0;                // pcs/src/recursion/mod.rs:81
+constexpr u128 MIN_MODULUS = 34816965049ull;       // verifier/proof_verify/src/main.rs:54
+constexpr u128 MAX_MODULUS = (u128)1 << 96;        // verifier/proof_verify/src/rq.rs:25
+constexpr size_t MAX_PROOF_BYTES = 64ull << 20;    // the codec's byte limit (main.rs:119-124)
+
+// bincode's fixed-width little-endian reads; a read past the end marks the reader bad.
+struct Reader {
+    const uint8_t *p;
+    size_t len, at = 0;
+    bool bad = false;
+    const uint8_t *take(uint64_t n) {
+        if (bad || n > len - at) {
+            bad = true;
+            return nullptr;
+        }
+        const uint8_t *r = p + at;
+        at += (size_t)n;
+        return r;
+    }
+    uint64_t u64() {
+        const uint8_t *b = take(8);
+        uint64_t x = 0;
+        if (b)
+            for (int i = 0; i < 8; i++) x |= (uint64_t)b[i] << (8 * i);
+        return x;
+    }
+};
+
+struct ProofFields {
+    u128 q = 0;
+    const uint8_t *commitment = nullptr, *row = nullptr, *folded = nullptr;
+    uint64_t clen = 0, row_len = 0, folded_len = 0;
+    std::vector<B128> partials;
+    std::vector<F162> rounds;  // [e0, einf] per round
+    F162 opened{};
+};
+
+// The proof's fields; a message when the bytes are not one `Proof` of this shape.
+const char *decode_proof(const uint8_t *buf, size_t len, int lp, ProofFields &P) {
+    if (len > MAX_PROOF_BYTES) return "proof too large";
+    Reader r{buf, len};
+    const uint8_t *v = r.take(4);
+    if (!v || v[0] != 1 || v[1] || v[2] || v[3]) return "bad version";
+    const uint64_t qlo = r.u64(), qhi = r.u64();
+    P.q = ((u128)qhi << 64) | qlo;
+    P.clen = r.u64();
+    P.commitment = r.take(P.clen);
+    if (r.u64() != 128 || r.bad) return "128 partial evaluations";
+    P.partials.resize(128);
+    for (auto &x : P.partials) x.w[0] = r.u64(), x.w[1] = r.u64();
+    if (r.u64() != (uint64_t)lp || r.bad) return "one switch round per packed coordinate";
+    P.rounds.resize(2 * (size_t)lp);
+    for (auto &x : P.rounds)
+        for (int k = 0; k < 3; k++) x.w[k] = r.u64();
+    for (int k = 0; k < 3; k++) P.opened.w[k] = r.u64();
+    P.row_len = r.u64();
+    P.row = r.take(P.row_len);
+    P.folded_len = r.u64();
+    P.folded = r.take(P.folded_len);
+    if (r.bad || r.at != len) return "malformed proof bytes";
+    return nullptr;
+}
+
+uint64_t fold_cap(const Shape &s) {  // main.rs:62-64 at this shape
+    return (uint64_t)std::ceil(FOLD_CAP * (double)(((uint64_t)1 << s.log_packed) / 4 * N));
+}
+
+uint64_t isqrt(uint64_t x) {
+    uint64_t r = (uint64_t)std::sqrt((double)x);
+    while (r * r > x) r--;
+    while ((r + 1) * (r + 1) <= x) r++;
+    return r;
+}
+
+bool canonical(const F162 &x) { return (x.w[2] >> 34) == 0; }
+
+// claim (main.rs:184-190): the partials weighted by eq(r_lo, .).
+B128 claim_of(const B128 *partials, const B128 *r_lo) {
+    B128 eq[128];
+    eq_expand_b128(r_lo, 7, eq);
+    B128 acc{{0, 0}};
+    for (int b = 0; b < 128; b++) acc = b128_add(acc, b128_mul(partials[b], eq[b]));
+    return acc;
+}
+
+// slice_sum (crossfield.rs:625-637): sum over bit positions k of the batch weight times the
+// GHASH element whose bit i is bit k of partial i, lifted to F162.
+F162 slice_sum(const B128 *v, const F162 *batch) {
+    F162 s{{0, 0, 0}};
+    for (int k = 0; k < 128; k++) {
+        B128 u{{0, 0}};
+        for (int i = 0; i < 128; i++) u.w[i >> 6] |= ((v[i].w[k >> 6] >> (k & 63)) & 1) << (i & 63);
+        s = f162_add(s, f162_mul(f162_lift(u), batch[k]));
+    }
+    return s;
+}
+
+// transparent_coeff (crossfield.rs:99-108 over the Tensor of crossfield.rs:10-63): the product
+// over i of (1 + r_pp[i] (x) 1 + 1 (x) r_hi[i]) in F162 (x) GF(2^128), held as the F162
+// coefficients of x^c, c < 128, on the GHASH side (x^128 = x^7 + x^2 + x + 1), then paired with
+// the batch weights.
+F162 transparent_coeff(const B128 *r_hi, const F162 *r_pp, int l, const F162 *batch) {
+    const F162 zero{{0, 0, 0}};
+    std::vector<F162> t(128, zero), h(128);
+    t[0] = F162{{1, 0, 0}};
+    for (int i = 0; i < l; i++) {
+        // h = (1 (x) r_hi[i]) t by Horner over the bits of r_hi[i], most significant first
+        std::fill(h.begin(), h.end(), zero);
+        for (int k = 127; k >= 0; k--) {
+            const F162 top = h[127];
+            std::rotate(h.begin(), h.end() - 1, h.end());
+            h[0] = top;
+            h[1] = f162_add(h[1], top), h[2] = f162_add(h[2], top), h[7] = f162_add(h[7], top);
+            if ((r_hi[i].w[k >> 6] >> (k & 63)) & 1)
+                for (int c = 0; c < 128; c++) h[c] = f162_add(h[c], t[c]);
+        }
+        for (int c = 0; c < 128; c++) t[c] = f162_add(f162_add(t[c], f162_mul(t[c], r_pp[i])), h[c]);
+    }
+    F162 acc = zero;
+    for (int c = 0; c < 128; c++) acc = f162_add(acc, f162_mul(t[c], batch[c]));
+    return acc;
+}
+
+// wire::unpack_f162 (pcs/src/wire/mod.rs:456-470): 162 bits per element, least significant first.
+void unpack_f162(const uint8_t *in, size_t n, F162 *out) {
+    auto bits = [&](size_t at, int cnt) {  // cnt <= 64 bits from bit offset `at`
+        uint64_t x = 0;
+        for (int b = 0; b < cnt; b++) x |= (uint64_t)((in[(at + b) >> 3] >> ((at + b) & 7)) & 1) << b;
+        return x;
+    };
+    for (size_t i = 0; i < n; i++) out[i] = F162{{bits(162 * i, 64), bits(162 * i + 64, 64), bits(162 * i + 128, 34)}};
+}
+
+// BLAKE3 chaining values of whole-chunk subtrees of host bytes: every chunk on the pool, then the
+// parents of each subtree (the absorb of a proof's commitment, 3-8 MB).
+void host_subtree_cvs(const uint8_t *data, uint64_t first, size_t count, const Blake3::Subtrees &trees,
+                      const uint32_t key[8], uint32_t flags, uint32_t *cvs) {
+    std::vector<uint32_t> leaf(count * 8);
+    parallel_for(count, [&](size_t i) {
+        uint32_t cv[8], m[16], out[16];
+        memcpy(cv, key, 32);
+        for (int b = 0; b < 16; b++) {
+            memcpy(m, data + i * Blake3::CHUNK + 64 * b, 64);
+            const uint32_t f = flags | (b == 0 ? Blake3::CHUNK_START : 0) | (b == 15 ? Blake3::CHUNK_END : 0);
+            Blake3::compress(cv, m, first + i, 64, f, out);
+            memcpy(cv, out, 32);
+        }
+        memcpy(&leaf[8 * i], cv, 32);
+    });
+    size_t at = 0;
+    for (size_t t = 0; t < trees.size(); t++) {
+        size_t n = (size_t)1 << trees[t].second;
+        uint32_t *lv = &leaf[8 * at];
+        at += n;
+        for (; n > 1; n /= 2)
+            for (size_t i = 0; i < n / 2; i++) {
+                uint32_t out[16];
+                Blake3::compress(key, lv + 16 * i, 0, 64, flags | Blake3::PARENT, out);
+                memcpy(lv + 8 * i, out, 32);
+            }
+        memcpy(cvs + 8 * t, lv, 32);
+    }
+}
+
+// Residues modulo q < 2^96.
+u128 add_mod(u128 a, u128 b, u128 q) {
+    const u128 s = a + b;
+    return s >= q ? s - q : s;
+}
+u128 mul_mod(u128 a, u128 b, u128 q) {  // b in 32-bit digits, most significant first
+    u128 r = 0;
+    for (int i = 2; i >= 0; i--) r = add_mod((r << 32) % q, (a * (uint32_t)(b >> (32 * i))) % q, q);
+    return r;
+}
+u128 signed_mod(__int128 x, u128 q) {
+    const __int128 m = x % (__int128)q;
+    return (u128)(m < 0 ? m + (__int128)q : m);
+}
+
+// The ring element sum_e raw_e X^e (raw_e = sum_k 2^(32 k) s[3 e + k]) of R_Q, with
+// X^648 = X^324 - 1 applied from the top (rq.rs:152-165).
+std::vector<u128> ring_of_limbs(const __int128 *s, u128 q) {
+    const u128 p32 = ((u128)1 << 32) % q, p64 = ((u128)1 << 64) % q;
+    std::vector<u128> raw(2 * N - 1);
+    for (int e = 0; e < 2 * N - 1; e++)
+        raw[e] = add_mod(add_mod(signed_mod(s[3 * e], q), mul_mod(signed_mod(s[3 * e + 1], q), p32, q), q),
+                         mul_mod(signed_mod(s[3 * e + 2], q), p64, q), q);
+    for (int k = 2 * N - 2; k >= N; k--) {
+        raw[k - N / 2] = add_mod(raw[k - N / 2], raw[k], q);
+        raw[k - N] = add_mod(raw[k - N], q - raw[k], q);
+    }
+    raw.resize(N);
+    return raw;
+}
+}  // namespace
+
+extern "C" int lbn_verify(lbn_ctx *ctx, const uint8_t seed[32], const uint8_t *proof, size_t len) {
+    if (!ctx || !seed || (!proof && len)) return LBN_E_ARG;
+    const Shape &s = ctx->shape;
+    const int lp = s.log_packed;
+    auto reject = [&](const char *why) { return ctx->fail(LBN_E_REJECT, why); };
+    ProofFields P;
+    if (const char *why = decode_proof(proof, len, lp, P)) return reject(why);
+    // accept_modulus (main.rs:85-97)
+    const u128 q = P.q;
+    if (q < MIN_MODULUS) return reject("commitment modulus is below the security bound");
+    if (q <= 2 * (u128)isqrt(fold_cap(s))) return reject("commitment modulus does not clear the no-wrap bound");
+    if (q >= MAX_MODULUS) return reject("commitment modulus is above what the checker's arithmetic covers");
+    TRY(lbn_generate_table(ctx, seed));
+
+    // transcript (main.rs:166-174)
+    Transcript t("labinius-fast/v1");
+    t.absorb_bytes(seed, 32);
+    t.absorb_u64((uint64_t)s.log_vars);
+    t.absorb_u64(COLUMN_LOG);
+    uint8_t qb[16];
+    for (int i = 0; i < 16; i++) qb[i] = (uint8_t)(q >> (8 * i));
+    t.absorb_bytes(qb, 16);
+    t.absorb_bytes_with(P.commitment, (size_t)P.clen,
+                        [&](size_t off, uint64_t first, size_t count, const Blake3::Subtrees &trees, uint32_t *cvs) {
+                            host_subtree_cvs(P.commitment + off, first, count, trees, t.hasher().key_words(),
+                                             t.hasher().flags(), cvs);
+                        });
+
+    // check_field_switch (main.rs:306-334)
+    const std::vector<B128> r_lo = t.sample_b128("r-lo", 7), r_hi = t.sample_b128("r-hi", lp);
+    std::vector<B128> trusted(128);
+    TRY(lbn_partials(ctx, (const uint64_t *)r_hi.data(), (uint64_t *)trusted.data()));
+    const B128 value = claim_of(trusted.data(), r_lo.data());
+    t.absorb_b128(&value, 1);
+    t.absorb_b128(P.partials.data(), 128);
+    const std::vector<F162> rb = t.sample_f162("switch-batch", 7);
+    std::vector<F162> batch(128);
+    eq_expand_f162(rb.data(), 7, batch.data());
+    // SwitchVerifier::start, round, finish (crossfield.rs:645-690)
+    const B128 re = claim_of(P.partials.data(), r_lo.data());
+    if (re.w[0] != value.w[0] || re.w[1] != value.w[1]) return reject("partial evaluation mismatch");
+    F162 sum = slice_sum(P.partials.data(), batch.data());
+    std::vector<F162> coords(lp);
+    for (int r = 0; r < lp; r++) {
+        const F162 e0 = P.rounds[2 * r], einf = P.rounds[2 * r + 1];
+        if (!canonical(e0) || !canonical(einf)) return reject("noncanonical F162 element");
+        t.absorb_f162(&P.rounds[2 * r], 2);
+        const F162 rho = t.sample_f162("switch-round", 1)[0];
+        coords[r] = rho;
+        const F162 e1 = f162_add(sum, e0);
+        // s = e0 + rho (e0 + e1 + einf) + rho^2 einf (crossfield.rs:666-671)
+        sum = f162_add(f162_add(e0, f162_mul(rho, f162_add(f162_add(e0, e1), einf))),
+                       f162_mul(f162_mul(rho, rho), einf));
+    }
+    if (!canonical(P.opened)) return reject("noncanonical F162 element");
+    {
+        const F162 d = transparent_coeff(r_hi.data(), coords.data(), lp, batch.data());
+        if (f162_eq(d, F162{{0, 0, 0}})) return reject("the switch left the opened value unbound");
+        if (!f162_eq(sum, f162_mul(d, P.opened))) return reject("sumcheck final check failed");
+    }
+    t.absorb_f162(&P.opened, 1);
+
+    // check_opening (main.rs:341-379); split_point (main.rs:257-264)
+    std::vector<F162> p0(s.row_log), p1(COLUMN_LOG);
+    for (int i = 0; i < s.row_log; i++) p0[i] = coords[lp - 1 - i];
+    for (int i = 0; i < COLUMN_LOG; i++) p1[i] = coords[COLUMN_LOG - 1 - i];
+    if (P.row_len != f162_bytes(COLUMNS)) return reject("malformed row evaluation");
+    std::vector<F162> row(COLUMNS);
+    unpack_f162(P.row, COLUMNS, row.data());
+    const size_t nf = (size_t)N * s.ring_elements;
+    if (P.folded_len != 4 * nf) return reject("folded witness has the wrong length");
+    // unpack_folded (main.rs:229-248): i32 words that must fit i16
+    std::vector<int16_t> v(nf);
+    for (size_t i = 0; i < nf; i++) {
+        const uint8_t *b = P.folded + 4 * i;
+        const int32_t x = (int32_t)((uint32_t)b[0] | (uint32_t)b[1] << 8 | (uint32_t)b[2] << 16 | (uint32_t)b[3] << 24);
+        if (x < -32768 || x > 32767) return reject("coefficient out of range");
+        v[i] = (int16_t)x;
+    }
+    {  // RowEvaluation::absorb (scheme/commitment.rs:320-331)
+        static const char L[] = "labinius/row-evaluation";
+        t.absorb_bytes(L, sizeof L - 1);
+        t.absorb_bytes(row.data(), (size_t)24 * COLUMNS);
+    }
+    static const ChallengeSampler sampler;
+    std::vector<lbn_challenge> ch(COLUMNS);
+    {
+        ChalArg ca = sampler.arg;
+        t.save(ca.ts);
+        TRY(dev_challenges(ctx, ca, ch.data()));
+    }
+    t.skip(COLUMNS);
+    {
+        std::vector<F162> eq1((size_t)1 << COLUMN_LOG);
+        eq_table_f162(p1.data(), COLUMN_LOG, eq1.data());
+        F162 acc{{0, 0, 0}};
+        for (int c = 0; c < COLUMNS; c++) acc = f162_add(acc, f162_mul(eq1[c], row[c]));
+        if (!f162_eq(acc, P.opened)) return reject("the row evaluation does not reduce to the opened value");
+    }
+    {
+        uint64_t normsq = 0, worst = 0;
+        for (size_t i = 0; i < nf; i++) {
+            const uint64_t m = (uint64_t)std::abs((int)v[i]);
+            normsq += m * m;
+            worst = std::max(worst, m);
+        }
+        if (normsq > fold_cap(s)) return reject("the fold is longer than the cap");
+        if ((u128)worst > (q - 1) / 2) return reject("the fold is not centered");
+    }
+    int width = 0;
+    for (u128 x = q - 1; x; x >>= 8) width++;
+    // rq::unpack_ring_elements (rq.rs:308-324)
+    if (P.clen != (uint64_t)COLUMNS * N * width) return reject("the commitment has the wrong length");
+    {
+        std::atomic<bool> unreduced{false};
+        const size_t per = 1 << 16, total = (size_t)COLUMNS * N;
+        parallel_for((total + per - 1) / per, [&](size_t b) {
+            for (size_t i = b * per; i < std::min(total, (b + 1) * per); i++) {
+                u128 x = 0;
+                for (int k = 0; k < width; k++) x |= (u128)P.commitment[i * width + k] << (8 * k);
+                if (x >= q) unreduced = true;
+            }
+        });
+        if (unreduced) return reject("a commitment coefficient is not reduced");
+    }
+    {
+        std::vector<__int128> ls(3 * (2 * N - 1)), rs(3 * (2 * N - 1));
+        TRY(dev_verify_products(ctx, q, v.data(), P.commitment, width, ch.data(), ls.data(), rs.data()));
+        if (ring_of_limbs(ls.data(), q) != ring_of_limbs(rs.data(), q))
+            return reject("the fold does not open the folded commitment");
+    }
+    {
+        // components_mod_2 (main.rs:268-280) under eq(p0) against the challenges' parities on the row
+        const size_t rows = 4 * s.ring_elements;
+        std::vector<F162> eq0(rows);
+        eq_table_f162(p0.data(), s.row_log, eq0.data());
+        F162 lhs{{0, 0, 0}}, rhs{{0, 0, 0}};
+        for (size_t m = 0; m < s.ring_elements; m++)
+            for (int k = 0; k < 4; k++) {
+                F162 c{{0, 0, 0}};
+                for (int p = 0; p < N / 4; p++) c.w[p >> 6] |= (uint64_t)(v[N * m + 4 * p + k] & 1) << (p & 63);
+                lhs = f162_add(lhs, f162_mul(eq0[4 * m + k], c));
+            }
+        for (int j = 0; j < COLUMNS; j++) {
+            F162 par{{0, 0, 0}};
+            for (uint32_t i = 0; i < ch[j].weight; i++) par.w[ch[j].positions[i] >> 6] |= 1ull << (ch[j].positions[i] & 63);
+            rhs = f162_add(rhs, f162_mul(par, row[j]));
+        }
+        if (!f162_eq(lhs, rhs)) return reject("the fold does not match the folded row evaluation");
+    }
+    return LBN_OK;
+}

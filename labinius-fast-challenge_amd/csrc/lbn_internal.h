@@ -148,6 +148,12 @@ int dev_commit_launch(lbn_ctx *ctx);
 int dev_commit_cvs(lbn_ctx *ctx, size_t off, uint64_t first, size_t count, const Blake3::Subtrees &trees,
                    const uint32_t key[8], uint32_t flags, uint32_t *cvs, uint8_t *dst, size_t head, size_t tail);
 int dev_commit_bytes(lbn_ctx *ctx, uint8_t *dst);
+// The checker's ring identity A v = sum_j c_j(-X^4) C_j (verifier/proof_verify/src/rq.rs:188-256):
+// both sides' coefficients before the reduction into R_Q (1295 of them), each as three limb sums
+// lhs/rhs[3 e + k] with value sum_k 2^(32 k) lhs[3 e + k], computed on the device from the public
+// matrix of Q, the fold v (ring_elements x 648), the packed commitment and the 1024 challenges.
+int dev_verify_products(lbn_ctx *ctx, u128 q, const int16_t *v, const uint8_t *commitment, int width,
+                        const lbn_challenge *ch, __int128 *lhs, __int128 *rhs);
 // Waits for everything queued on the context's streams (errors ignored): a failing lbn_prove calls
 // it before returning, so no copy or kernel still writes the caller's proof buffer.
 void dev_quiesce(lbn_ctx *ctx);

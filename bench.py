@@ -234,6 +234,23 @@ def run_jobs(m, jobs, workers, reps=1, budget=200 << 30):
         sch.close()
 
 
+class Checker:
+    """The library's checker (lbn_verify), one context per size; call(k, seed, proof) -> accepted."""
+
+    def __init__(self, m):
+        self.m, self.ctx = m, {}
+
+    def __call__(self, k, seed, proof):
+        if k not in self.ctx:
+            self.ctx[k] = self.m.Prover(k)
+        return self.ctx[k].verify(seed, proof) is None
+
+    def close(self):
+        for p in self.ctx.values():
+            p.close()
+        self.ctx.clear()
+
+
 def batched(m, workers, reps=3):
     """64 independent proofs of mixed size (k in [20, 28]) in flight together vs one at a time; each
     way timed `reps` times (the batch is short), medians reported."""
@@ -243,13 +260,19 @@ def batched(m, workers, reps=3):
     par, t_pars = run_jobs(m, jobs, workers, reps)
     batched.windows_ns = {"sequential": list(w_seq), "parallel": list(run_jobs.window_ns)}
     t_seq, t_par = statistics.median(t_seqs), statistics.median(t_pars)
+    check = Checker(m)
+    try:  # every proof of the first batched run through the checker (the others are compared to it)
+        verified = sum(check(kk, seed, proof) for (kk, seed), proof in zip(jobs, par[0]))
+    finally:
+        check.close()
     return {"instances": len(jobs), "k": sorted({kk for kk, _ in jobs}), "workers": workers,
             "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")),
             "proofs_per_s": round(len(jobs) / t_par, 1), "sequential_proofs_per_s": round(len(jobs) / t_seq, 1),
             "speedup": round(t_seq / t_par, 2), "runs": reps,
             "proofs_per_s_runs": [round(len(jobs) / t, 1) for t in t_pars],
             "sequential_proofs_per_s_runs": [round(len(jobs) / t, 1) for t in t_seqs],
-            "identical_to_sequential": all(p == seq[0] for p in seq + par)}
+            "identical_to_sequential": all(p == seq[0] for p in seq + par),
+            "verified": f"{verified}/{len(jobs)}"}
 
 
 def serving_requests(requests, rate, rng_seed=20261203):
@@ -273,7 +296,7 @@ def serving(m, requests, rate, workers, budget):
     reqs = serving_requests(requests, rate)
     ks = list(range(20, 32))
 
-    def serve(w):
+    def serve(w, check=None):
         sch = m.Scheduler(m.DEFAULT_Q, hbm_budget=budget, workers=w)
         try:
             warm(sch, ks, w)
@@ -290,14 +313,26 @@ def serving(m, requests, rate, workers, budget):
             lat = sorted(lag * 1000 + (h.queue_us + h.run_us) / 1000 for lag, h in hs)
             st = sch.stats()
             proofs = [sch.result(h)[0] for _, h in hs[:32]]  # a sample, for the isolation check
-            return {"p50_ms": round(statistics.median(lat), 3), "p99_ms": round(lat[int(0.99 * (len(lat) - 1))], 3),
-                    "proofs_per_s": round(len(hs) / wall, 1), "wall_s": round(wall, 2),
-                    "contexts": st["contexts_created"], "evictions": st["evictions"],
-                    "peak_gib": round(st["peak_bytes"] / 2**30, 2)}, proofs
+            out = {"p50_ms": round(statistics.median(lat), 3), "p99_ms": round(lat[int(0.99 * (len(lat) - 1))], 3),
+                   "proofs_per_s": round(len(hs) / wall, 1), "wall_s": round(wall, 2),
+                   "contexts": st["contexts_created"], "evictions": st["evictions"],
+                   "peak_gib": round(st["peak_bytes"] / 2**30, 2)}
+            if check:  # every proof through the checker after the clock, each released once checked
+                def every_proof():
+                    yield from proofs
+                    for _, h in hs[32:]:
+                        yield sch.result(h)[0]
+                ok = sum(check(kk, seed, p) for (_, kk, seed), p in zip(reqs, every_proof()))
+                out["verified"] = f"{ok}/{len(hs)}"
+            return out, proofs
         finally:
             sch.close()
 
-    mix, proofs_mix = serve(workers)
+    check = Checker(m)
+    try:
+        mix, proofs_mix = serve(workers, check)
+    finally:
+        check.close()
     seq, proofs_seq = serve(1)
     return {"metric": "serving p50/p99 ms and proofs/s on 1 MI355X", "requests": requests, "rate_per_s": rate,
             "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")),

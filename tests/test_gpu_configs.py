@@ -1,7 +1,8 @@
 """The batched (64 instances of k in [20, 28]) and serving (Poisson requests of k in 20..31 under a
 200 GB HBM budget) workloads the benchmark measures, run through the scheduler ABI (lbn_sched_*) on an
-MI355X: every proof equals the same proof made alone on one context, the CPU oracle's checker accepts
-every proof, and the oracle's own prover gives the same bytes for one instance of each size.
+MI355X: every proof equals the same proof made alone on one context, both the library's checker
+(lbn_verify) and the CPU oracle's accept every proof, and the oracle's own prover gives the same
+bytes for one instance of each size.
 """
 import sys
 import time
@@ -10,7 +11,7 @@ import pytest
 
 import amd
 import oracle_ctypes as O
-from bench import batched_instances, serving_requests
+from bench import Checker, batched_instances, serving_requests
 
 pytestmark = pytest.mark.gpu
 
@@ -44,6 +45,15 @@ def alone(m):
         p.close()
 
 
+def product_checker_accepts(m, proofs):
+    check = Checker(m)
+    try:
+        for i, (k, seed, proof) in enumerate(proofs):
+            assert check(k, seed, proof), (i, k)
+    finally:
+        check.close()
+
+
 def test_batched_64_instances(m, alone):
     jobs = batched_instances()
     ks = sorted({k for k, _ in jobs})
@@ -63,6 +73,8 @@ def test_batched_64_instances(m, alone):
     say("batched: solo proofs")
     for i, (k, seed, proof) in enumerate(proofs):
         assert proof == alone(k, seed), (i, k)
+    say("batched: library checker on every proof")
+    product_checker_accepts(m, proofs)
     say("batched: oracle checker on every proof")
     for i, (k, seed, proof) in enumerate(proofs):
         assert O.verify(seed, k, proof) is None, (i, k)
@@ -105,6 +117,8 @@ def test_serving_poisson_all_sizes(m, alone):
     say("serving: solo proofs")
     for i, (k, seed, proof) in enumerate(proofs):
         assert proof == alone(k, seed), (i, k)
+    say("serving: library checker on every proof")
+    product_checker_accepts(m, proofs)
     say("serving: oracle checker on every proof")
     for i, (k, seed, proof) in enumerate(proofs):
         assert O.verify(seed, k, proof) is None, (i, k)

@@ -46,7 +46,8 @@ struct lbn::DeviceState {
     uint4 *table = nullptr;
     uint4 *E[2] = {nullptr, nullptr};
     uint32_t *P[2] = {nullptr, nullptr};
-    uint32_t *T = nullptr;  // [0,768): T0, [768,1536): Tinf, [1536,2048): partials, [2048,3584): odd rounds
+    uint32_t *T = nullptr;  // [0,768): T0, [768,1536): Tinf, [2048,3584): the odd rounds' T0, Tinf
+    uint32_t *pset = nullptr;  // the prefix sets and the first switch rounds' data (layout: namespace ps)
     uint4 *EL = nullptr, *EH = nullptr;  // EH = EL + 4096 (one allocation)
     uint32_t *eqrow = nullptr, *eqlo = nullptr;
     int *fold_off = nullptr;
@@ -113,8 +114,17 @@ constexpr size_t CHOUT = CH + 8192;       // the sampled challenges (1024 lbn_ch
 constexpr size_t HANDOFF_H2 = (size_t)1 << (SWITCH_HOST_STEPS - 1);  // items of the host's first fold
 constexpr size_t HP = CHOUT + 1024 * 40;  // the switch handoff: that fold's input (4 h2 F162)
 constexpr size_t HE = HP + 4 * HANDOFF_H2 * 24;  // and its eq values (2 h2 B128)
-constexpr size_t SIZE = HE + 2 * HANDOFF_H2 * 16;
+constexpr size_t LAM = HE + 2 * HANDOFF_H2 * 16;  // the prefix rounds' lambda values (128 B128)
+constexpr size_t SIZE = LAM + 128 * 16;
 }  // namespace mb
+
+// The prefix-set layout of a context's pset buffer (words): the 64 sets M_u (128 B128 each), the
+// partials' accumulator (128 B128), Q of the first switch rounds (64 x 128 F162) and their lambda
+// values (128 B128).
+namespace ps {
+constexpr size_t M = 0, PART = 64 * 128 * 4, Q = PART + 512, LAM = Q + 64 * 128 * 6, WORDS = LAM + 512;
+}
+
 
 // Timed passes (indices into lbn_timing's fields, in order).
 enum Pass { T_TABLE = 0, T_EQ, T_PARTIALS, T_SWITCH0, T_ROUNDS, T_ROW, T_FOLD, T_MAC, T_FINISH, T_COUNT };
@@ -131,6 +141,8 @@ static void tcollect(lbn_ctx *ctx, int pass) {
     float ms = 0;
     if (hipEventElapsedTime(&ms, ctx->dev->ev[2 * pass], ctx->dev->ev[2 * pass + 1]) == hipSuccess)
         (&ctx->times.table_ms)[pass] += ms;
+    else
+        (void)hipGetLastError();  // a pass not run since the events were made: not an error of the proof
     if (pass == T_ROUNDS) ctx->times.switch_rounds++;
 }
 
@@ -295,13 +307,13 @@ __global__ __launch_bounds__(64) void k_chunk_cvs(const uint8_t *bytes, size_t o
 }
 
 // ---------------------------------------------------------------------------------------------
-// eq_hi[jh * 4096 + jl] = EH[jh] * EL[jl] in GF(2^128): EH[jh] times the 12-coordinate
-// expansion EL, through a 16 x 256 byte-window table of y -> EH[jh] * y.
+// eq[jh * 2^lo + jl] = EH[jh] * EL[jl] in GF(2^128): EH[jh] times the lo-coordinate expansion EL
+// (lo <= 12), through a 16 x 256 byte-window table of y -> EH[jh] * y.
 // ---------------------------------------------------------------------------------------------
-// Block 0 also zeroes `clear` (the partials' and the switch messages' XOR accumulators, which
-// the kernels after it fill), so that no separate fill is launched.
-__global__ __launch_bounds__(256) void k_eq_expand(const uint4 *EH, const uint4 *EL, uint4 *E0, uint32_t *clear,
-                                                   int nclear) {
+// Block 0 also zeroes `clear` (the XOR accumulators the kernels after it fill), so that no
+// separate fill is launched.
+__global__ __launch_bounds__(256) void k_eq_expand(const uint4 *EH, const uint4 *EL, int lo, uint4 *E0,
+                                                   uint32_t *clear, int nclear) {
     __shared__ uint4 basis[128];
     __shared__ uint4 tab[16 * 256];
     const int t = threadIdx.x;
@@ -336,7 +348,7 @@ __global__ __launch_bounds__(256) void k_eq_expand(const uint4 *EH, const uint4 
         }
     }
     __syncthreads();
-    for (int jl = t; jl < 4096; jl += 256) {
+    for (int jl = t; jl < (1 << lo); jl += 256) {
         uint4 y = EL[jl];
         uint32_t yw[4] = {y.x, y.y, y.z, y.w};
         uint4 acc = make_uint4(0, 0, 0, 0);
@@ -345,7 +357,7 @@ __global__ __launch_bounds__(256) void k_eq_expand(const uint4 *EH, const uint4 
             uint4 v = tab[w * 256 + ((yw[w >> 2] >> (8 * (w & 3))) & 255)];
             acc.x ^= v.x, acc.y ^= v.y, acc.z ^= v.z, acc.w ^= v.w;
         }
-        E0[(size_t)jh * 4096 + jl] = acc;
+        E0[((size_t)jh << lo) + jl] = acc;
     }
 }
 
@@ -470,6 +482,74 @@ __global__ __launch_bounds__(64 * PF_WAVES) void k_partials_fact(const uint4 *ta
 }
 
 // ---------------------------------------------------------------------------------------------
+// Prefix sets of the table: with j = u 2^S + m (u the top 6 index bits, S = log_packed - 6) the eq
+// value of j factors as eq6(u) G_m, G the expansion of r_hi[6..] (crossfield.rs:87-97: the first
+// coordinate is the most significant bit), so the partials (crossfield.rs:152-165) are
+//   v[b] = sum_u eq6(u) M_u[b],   M_u[b] = sum_m bit_b(w_{u 2^S + m}) G_m,
+// and the 64 sets M_u also give the switch's first six messages (k_prefix_msg). A block sums a
+// chunk of 2^chunk_log words inside one prefix by wave-local M4RM, then XORs its sums into M_u and
+// their products by eq6(u) (a 2-bit window table of y -> eq6(u) y) into v; both zeroed before.
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_prefix_sums(const uint4 *table, const uint4 *G, int S, int chunk_log,
+                                                     const uint4 *eq6, uint32_t *M, uint32_t *partials) {
+    __shared__ __align__(16) uint32_t lds[4 * M4W<4>::WORDS];
+    __shared__ uint32_t bs[512];
+    uint32_t *W = lds + (threadIdx.x >> 6) * M4W<4>::WORDS;
+    const size_t chunk = (size_t)1 << chunk_log, first = (size_t)blockIdx.x << chunk_log;
+    const int u = (int)(first >> S);
+    const uint4 *tw = table + first, *gw = G + (first & (((size_t)1 << S) - 1));
+    uint32_t acc[2][4] = {};
+    for (size_t base = 0; base < chunk; base += 512) {  // two items per pass, both loaded first
+        const size_t i0 = base + threadIdx.x, i1 = i0 + 256;
+        const uint4 z = make_uint4(0, 0, 0, 0);
+        const uint4 w0 = i0 < chunk ? tw[i0] : z, g0 = i0 < chunk ? gw[i0] : z;
+        const uint4 w1 = i1 < chunk ? tw[i1] : z, g1 = i1 < chunk ? gw[i1] : z;
+        {
+            uint32_t sel[4] = {w0.x, w0.y, w0.z, w0.w};
+            uint32_t val[4] = {g0.x, g0.y, g0.z, g0.w};
+            m4w_step<4>(acc, sel, val, W);
+        }
+        if (base + 256 < chunk) {
+            uint32_t sel[4] = {w1.x, w1.y, w1.z, w1.w};
+            uint32_t val[4] = {g1.x, g1.y, g1.z, g1.w};
+            m4w_step<4>(acc, sel, val, W);
+        }
+    }
+    m4w_flush<4, false>(acc, lds, bs);  // bs[4 b + v]: word v of the block's sum for bit b
+    __syncthreads();
+    const int t = threadIdx.x;
+    for (int i = t; i < 512; i += 256)
+        if (bs[i]) atomicXor(M + (size_t)u * 512 + i, bs[i]);
+    uint4 *win = (uint4 *)lds;  // win[4 w + v] = eq6(u) (v x^(2w)), 64 windows of 2 bits
+    if (t < 64) {
+        const uint4 e = eq6[u];
+        const uint4 b0 = b128_mul_xs(e, 2 * t), b1 = b128_mul_xs(e, 2 * t + 1);
+        win[4 * t] = make_uint4(0, 0, 0, 0);
+        win[4 * t + 1] = b0;
+        win[4 * t + 2] = b1;
+        win[4 * t + 3] = make_uint4(b0.x ^ b1.x, b0.y ^ b1.y, b0.z ^ b1.z, b0.w ^ b1.w);
+    }
+    __syncthreads();
+    if (t < 128) {
+        uint4 x = make_uint4(0, 0, 0, 0);
+#pragma unroll 1
+        for (int q = 0; q < 4; q++) {  // 16 windows per word of the sum
+            const uint32_t y = bs[4 * t + q];
+#pragma unroll
+            for (int n = 0; n < 16; n++) {
+                const uint4 e = win[4 * (16 * q + n) + ((y >> (2 * n)) & 3)];
+                x.x ^= e.x, x.y ^= e.y, x.z ^= e.z, x.w ^= e.w;
+            }
+        }
+        uint32_t *dst = partials + 4 * t;
+        if (x.x) atomicXor(dst, x.x);
+        if (x.y) atomicXor(dst + 1, x.y);
+        if (x.z) atomicXor(dst + 2, x.z);
+        if (x.w) atomicXor(dst + 3, x.w);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
 // Switch round 0 (sumcheck.rs:89-117 with a_j = L(E0_j), p_j = lift(w_j)):
 //   T0_k   = sum_{j<h} bit_k(E0_j) w_j
 //   Tinf_k = sum_{j<h} bit_k(E1_j) (w_j + w_{j+h}),  E1_j = E0_j + E0_{j+h} (written)
@@ -570,6 +650,27 @@ __device__ void build_rho_table(RhoArg rho, uint32_t *basis /*162*6*/, uint32_t 
         for (int k = 0; k < 6; k++) tab[6 * e + k] = v[k];
     }
     __syncthreads();
+}
+
+// rho_mul with eight windows (one input word) per loop step, for kernels that run many products
+// back to back: the table reads of one word are in flight at a time.
+template <int NW>
+__device__ __forceinline__ void rho_mul_lean(const uint32_t *tab, const uint32_t (&y)[6], uint32_t (&out)[6]) {
+#pragma unroll
+    for (int k = 0; k < 6; k++) out[k] = 0;
+#pragma unroll 1
+    for (int q = 0; q < (NW + 7) / 8; q++) {
+        const uint32_t yq = q == 0 ? y[0] : q == 1 ? y[1] : q == 2 ? y[2] : q == 3 ? y[3] : q == 4 ? y[4] : y[5];
+        const uint32_t *tq = tab + 6 * 16 * 8 * q;
+#pragma unroll
+        for (int n = 0; n < 8; n++) {
+            if (8 * q + n < NW) {
+                const uint32_t *e = tq + 6 * (n * 16 + ((yq >> (4 * n)) & 15));
+#pragma unroll
+                for (int k = 0; k < 6; k++) out[k] ^= e[k];
+            }
+        }
+    }
 }
 
 template <int NW>  // number of nibble windows of the input (32 for a lifted word, 41 for F162)
@@ -1114,6 +1215,242 @@ __global__ __launch_bounds__(1024) void k_switch_ctl(SwitchDev *sd, SwitchDev *h
         for (int i = threadIdx.x; i < 6 * hp_n; i += blockDim.x) hp_dst[i] = hp_src[i];
         for (int i = threadIdx.x; i < he_n; i += blockDim.x) he_dst[i] = he_src[i];
     }
+}
+
+// ---------------------------------------------------------------------------------------------
+// The switch's first six rounds from the prefix sets. With S = log_packed - 6, round r < 6 holds
+// p^(r)_(v', m) = sum_a eq(rho_0..r-1, a) lift(w_(a, v', m)) over the prefixes v' of 6 - r bits and
+// E^(r)_(v', m) = eq(c_r..5, v') G_m (c = r_hi). Write
+//   Q^(r)_v'[k] = sum_m p^(r)_(v', m) bit_k(G_m)   (F162, k < 128),
+// so sum_m p_(v', m) L(lambda G_m) = sum_k Q_v'[k] L(lambda x^k) for any lambda, and with v' = (b, v)
+// (b its top bit, nv = 2^(5 - r) values of v) the messages' per-bit sums are
+//   T0_k   = sum_v sum_i bit_k(lambda0_v x^i) Q_(0,v)[i],                lambda0_v = (1 + c_r) eq(c_r+1..5, v)
+//   Tinf_k = sum_v sum_i bit_k(lambda1_v x^i) (Q_(0,v)[i] + Q_(1,v)[i]),  lambda1_v = eq(c_r+1..5, v)
+// (e0 = sum_k L[k] T0_k and einf likewise, formed by the control step as from the round kernels'
+// sums). The fold is the round kernels' own, Q^(r+1)_v = Q_(0,v) + rho_r (Q_(0,v) + Q_(1,v)), and
+// Q^(0)_u[k] = lift(N_u[k]) with N_u the transpose of M_u. Q is 64 x 128 F162 (6 words each).
+// ---------------------------------------------------------------------------------------------
+// Q^(0) from the sets, one wave per (u, half): lane l holds row b = 64 half + l of M_u; the ballot
+// of bit k over the wave is bits 64 half .. 64 half + 63 of N_u[k], kept by lane k mod 64. Block 0
+// also copies the lambda values from the host's mailbox (lam: lambda0 of round r at
+// 64 - 2^(6-r) + v, lambda1 at 64 more) to lam_dev and zeroes round 0's message buffer `clear`.
+__global__ __launch_bounds__(256) void k_prefix_q0(const uint32_t *M, uint32_t *Q, const uint4 *lam, uint4 *lam_dev,
+                                                   uint32_t *clear) {
+    const int t = threadIdx.x, lane = t & 63;
+    if (blockIdx.x == 0) {
+        if (t < 128) {
+            const volatile uint32_t *l = (const volatile uint32_t *)(lam + t);
+            lam_dev[t] = make_uint4(l[0], l[1], l[2], l[3]);
+        }
+        for (int i = t; i < 1536; i += blockDim.x) clear[i] = 0;
+    }
+    const int task = blockIdx.x * 4 + (t >> 6), u = task >> 1, half = task & 1;
+    const uint32_t *row = M + ((size_t)u * 128 + 64 * half + lane) * 4;
+    const uint32_t r4[4] = {row[0], row[1], row[2], row[3]};
+    uint64_t mine[2] = {0, 0};
+#pragma unroll
+    for (int wd = 0; wd < 4; wd++)
+#pragma unroll 1
+        for (int b = 0; b < 32; b++) {
+            const int k = 32 * wd + b;
+            const uint64_t m = __ballot((r4[wd] >> b) & 1);
+            if (lane == (k & 63)) mine[wd >> 1] = m;
+        }
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+        uint32_t *q = Q + ((size_t)u * 128 + lane + 64 * c) * 6;
+        q[2 * half] = (uint32_t)mine[c], q[2 * half + 1] = (uint32_t)(mine[c] >> 32);
+        if (half == 0) q[4] = 0, q[5] = 0;
+    }
+}
+
+// Round r's sums (r < 6), for r > 0 after the fold of round r - 1 (rho from the device state,
+// written by the control step before): item (v, i), v < nv, i < 128, folds Q[v], Q[2 nv + v] into
+// Q[v] and Q[nv + v], Q[3 nv + v] into Q[nv + v] (in place: no item reads what another writes),
+// then adds bit_k(lambda0_v x^i) Q[v][i] into T0 and bit_k(lambda1_v x^i) (Q[v][i] + Q[nv + v][i])
+// into Tinf (zero before). Block 0 also zeroes `clear`, the next round's buffer.
+__global__ __launch_bounds__(256) void k_prefix_msg(uint32_t *Q, int r, const uint4 *lam, const SwitchDev *sd,
+                                                    uint32_t *T0, uint32_t *Tinf, uint32_t *clear) {
+    __shared__ uint32_t rtab[RHO_WINDOWS * 16 * 6];
+    __shared__ __align__(16) uint32_t lds[4 * M4W<6>::WORDS];  // also the basis while rtab is built
+    const int t = threadIdx.x, nv = 32 >> r, items = 128 * nv;
+    if (blockIdx.x == 0)
+        for (int i = t; i < 1536; i += blockDim.x) clear[i] = 0;
+    if (r > 0) build_rho_table(RhoArg{{sd->rho[0], sd->rho[1], sd->rho[2]}}, lds, rtab);
+    const int it = blockIdx.x * 256 + t;
+    uint32_t a[6] = {0, 0, 0, 0, 0, 0}, d[6] = {0, 0, 0, 0, 0, 0};
+    uint32_t s0[4] = {0, 0, 0, 0}, s1[4] = {0, 0, 0, 0};
+    if (it < items) {
+        const int v = it >> 7, i = it & 127;
+        uint32_t *q0 = Q + ((size_t)v * 128 + i) * 6, *q1 = Q + ((size_t)(nv + v) * 128 + i) * 6;
+        uint32_t b[6];
+#pragma unroll
+        for (int k = 0; k < 6; k++) a[k] = q0[k], b[k] = q1[k];
+        if (r > 0) {  // the fold of round r - 1: Q[v] <- Q[v] + rho (Q[v] + Q[2nv + v]), likewise Q[nv + v]
+            const uint32_t *q2 = Q + ((size_t)(2 * nv + v) * 128 + i) * 6, *q3 = Q + ((size_t)(3 * nv + v) * 128 + i) * 6;
+            uint32_t x[6], y[6], mx[6], my[6];
+#pragma unroll
+            for (int k = 0; k < 6; k++) x[k] = a[k] ^ q2[k], y[k] = b[k] ^ q3[k];
+            rho_mul<RHO_WINDOWS>(rtab, x, mx);
+            rho_mul<RHO_WINDOWS>(rtab, y, my);
+#pragma unroll
+            for (int k = 0; k < 6; k++) a[k] ^= mx[k], b[k] ^= my[k], q0[k] = a[k], q1[k] = b[k];
+        }
+#pragma unroll
+        for (int k = 0; k < 6; k++) d[k] = a[k] ^ b[k];
+        const uint4 *l0 = lam + (64 - 2 * nv);
+        const uint4 e0 = b128_mul_xs(l0[v], i), e1 = b128_mul_xs(l0[64 + v], i);
+        s0[0] = e0.x, s0[1] = e0.y, s0[2] = e0.z, s0[3] = e0.w;
+        s1[0] = e1.x, s1[1] = e1.y, s1[2] = e1.z, s1[3] = e1.w;
+    }
+    uint32_t *W = lds + (t >> 6) * M4W<6>::WORDS;
+    uint32_t acc0[2][6] = {}, acci[2][6] = {};
+    m4w_step<6>(acc0, s0, a, W);
+    m4w_step<6>(acci, s1, d, W);
+    m4w_flush<6, true>(acc0, lds, T0);
+    __syncthreads();
+    m4w_flush<6, true>(acci, lds, Tinf);
+}
+
+// ---------------------------------------------------------------------------------------------
+// The six folds at once (p^(6)_m = sum_u eq(rho_0..5, u) lift(w_(u 2^S + m)), the folds of rounds
+// 0..5 as in k_switch_round) fused with round 6's message sums, as round 5's kernel would give
+// them: for j < h2 = 2^(S-1) the item's outputs p^(6)_j and p^(6)_(j+h2), E^(6) = G (read) and
+// E^(7)_j = G_j + G_(j+h2) (written), T0_k += bit_k(G_j) p_j and Tinf_k += bit_k(E^(7)_j)
+// (p_j + p_(j+h2)). One output is a binary tree of folds over the six prefix bits, walked
+// depth-first: leaf s (five bits u_1..u_5) folds the table words of u = s and u = 32 + s by rho_0,
+// each further level the two halves by rho_1..rho_5; the challenges come from the device state
+// (the records of the first six control steps). Block 0 also clears `clear` (1536 words).
+// ---------------------------------------------------------------------------------------------
+constexpr int BIND_THREADS = 512;
+__device__ __forceinline__ void f162_fold(const uint32_t *tab, const uint32_t (&a)[6], const uint32_t (&b)[6],
+                                          uint32_t (&out)[6]) {  // a + rho (a + b)
+    uint32_t d[6], m[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) d[k] = a[k] ^ b[k];
+    rho_mul_lean<RHO_WINDOWS>(tab, d, m);
+#pragma unroll
+    for (int k = 0; k < 6; k++) out[k] = a[k] ^ m[k];
+}
+
+// One output by a group of G = 2^GL lanes: lane g walks the subtree of the prefixes whose last GL
+// bits are g (2^(5-GL) leaves, levels 1 .. 5-GL), then the group folds its G values across lanes
+// (lanes g and g + G/2 by rho_(6-GL), ...); lane 0 of the group ends with the output.
+template <int GL>
+__device__ __forceinline__ void bind_value(const uint4 *__restrict__ tw, size_t stride, const uint32_t *rt, int g,
+                                           uint32_t (&out)[6]) {
+    constexpr int TW = RHO_WINDOWS * 16 * 6, LEAVES = 32 >> GL;
+    uint32_t s1[6], s2[6], s3[6], s4[6], s5[6], v[6];
+#pragma unroll 1
+    for (int k = 0; k < LEAVES; k++) {
+        const int sidx = g + (GL < 5 ? (int)((__builtin_bitreverse32((uint32_t)k) >> (27 + GL)) << GL) : 0);
+        const uint4 w0 = tw[(size_t)sidx * stride], w1 = tw[(size_t)(32 + sidx) * stride];
+        {
+            uint32_t d[6] = {w0.x ^ w1.x, w0.y ^ w1.y, w0.z ^ w1.z, w0.w ^ w1.w, 0, 0}, m[6];
+            rho_mul_lean<32>(rt, d, m);
+            v[0] = w0.x ^ m[0], v[1] = w0.y ^ m[1], v[2] = w0.z ^ m[2], v[3] = w0.w ^ m[3], v[4] = m[4], v[5] = m[5];
+        }
+        // the depth-first combine: level L's pending left node waits in sL
+        if (GL < 5) {
+            if (!(k & 1)) {
+#pragma unroll
+                for (int i = 0; i < 6; i++) s1[i] = v[i];
+                continue;
+            }
+            f162_fold(rt + TW, s1, v, v);
+        }
+        if (GL < 4) {
+            if (!(k & 2)) {
+#pragma unroll
+                for (int i = 0; i < 6; i++) s2[i] = v[i];
+                continue;
+            }
+            f162_fold(rt + 2 * TW, s2, v, v);
+        }
+        if (GL < 3) {
+            if (!(k & 4)) {
+#pragma unroll
+                for (int i = 0; i < 6; i++) s3[i] = v[i];
+                continue;
+            }
+            f162_fold(rt + 3 * TW, s3, v, v);
+        }
+        if (GL < 2) {
+            if (!(k & 8)) {
+#pragma unroll
+                for (int i = 0; i < 6; i++) s4[i] = v[i];
+                continue;
+            }
+            f162_fold(rt + 4 * TW, s4, v, v);
+        }
+        if (GL < 1) {
+            if (!(k & 16)) {
+#pragma unroll
+                for (int i = 0; i < 6; i++) s5[i] = v[i];
+                continue;
+            }
+            f162_fold(rt + 5 * TW, s5, v, v);
+        }
+    }
+#pragma unroll
+    for (int lev = 0; lev < GL; lev++) {  // across the group: level 6 - GL + lev by rho_(6-GL+lev)
+        const int half = (1 << GL) >> (lev + 1);
+        uint32_t o[6];
+#pragma unroll
+        for (int i = 0; i < 6; i++) o[i] = (uint32_t)__shfl_xor((int)v[i], half);
+        f162_fold(rt + (6 - GL + lev) * TW, v, o, v);  // lanes with bit `half` of g clear keep theirs
+    }
+#pragma unroll
+    for (int i = 0; i < 6; i++) out[i] = v[i];
+}
+
+template <int GL>
+__global__ __launch_bounds__(BIND_THREADS) void k_switch_bind(const uint4 *__restrict__ table, int S,
+                                                              const SwitchDev *sd, const uint4 *__restrict__ G,
+                                                              uint4 *__restrict__ Eout, uint32_t *__restrict__ pout,
+                                                              uint32_t *T0, uint32_t *Tinf, uint32_t *clear) {
+    __shared__ uint32_t rt[6 * RHO_WINDOWS * 16 * 6];
+    __shared__ __align__(16) uint32_t lds[(BIND_THREADS / 64) * M4W<6>::WORDS];  // also the tables' basis
+    if (blockIdx.x == 0)
+        for (int i = threadIdx.x; i < 1536; i += blockDim.x) clear[i] = 0;
+    for (int i = 0; i < 6; i++)
+        build_rho_table(RhoArg{{sd->coords[i][0], sd->coords[i][1], sd->coords[i][2]}}, lds,
+                        rt + i * RHO_WINDOWS * 16 * 6);
+    constexpr int GS = 1 << GL, GPB = BIND_THREADS / GS;  // lanes per output, groups per block
+    const int g = threadIdx.x & (GS - 1);
+    const size_t stride = (size_t)1 << S, h2 = stride / 2;
+    const size_t rounds = (h2 + GPB - 1) / GPB;  // group passes over the items, grid-strided
+    uint32_t *W = lds + (threadIdx.x >> 6) * M4W<6>::WORDS;
+    uint32_t acc0[2][6] = {}, acci[2][6] = {};
+    for (size_t pass = blockIdx.x; pass < rounds; pass += gridDim.x) {
+        const size_t j = pass * GPB + (threadIdx.x >> GL);
+        uint32_t q0[6] = {0, 0, 0, 0, 0, 0}, q1[6] = {0, 0, 0, 0, 0, 0};
+        uint32_t e[4] = {0, 0, 0, 0}, e2[4] = {0, 0, 0, 0};
+        if (j < h2) {  // the whole group takes this branch
+            bind_value<GL>(table + j, stride, rt, g, q0);
+            bind_value<GL>(table + j + h2, stride, rt, g, q1);
+            if (g == 0) {
+                uint2 *o0 = (uint2 *)(pout + 6 * j), *o1 = (uint2 *)(pout + 6 * (j + h2));
+                o0[0] = make_uint2(q0[0], q0[1]), o0[1] = make_uint2(q0[2], q0[3]), o0[2] = make_uint2(q0[4], q0[5]);
+                o1[0] = make_uint2(q1[0], q1[1]), o1[1] = make_uint2(q1[2], q1[3]), o1[2] = make_uint2(q1[4], q1[5]);
+                const uint4 ea = G[j], eb = G[j + h2];
+                e[0] = ea.x, e[1] = ea.y, e[2] = ea.z, e[3] = ea.w;
+                e2[0] = ea.x ^ eb.x, e2[1] = ea.y ^ eb.y, e2[2] = ea.z ^ eb.z, e2[3] = ea.w ^ eb.w;
+                Eout[j] = make_uint4(e2[0], e2[1], e2[2], e2[3]);
+            } else {
+#pragma unroll
+                for (int k = 0; k < 6; k++) q0[k] = q1[k] = 0;
+            }
+        }
+        m4w_step<6>(acc0, e, q0, W);
+        uint32_t dq[6];
+#pragma unroll
+        for (int k = 0; k < 6; k++) dq[k] = q0[k] ^ q1[k];
+        m4w_step<6>(acci, e2, dq, W);
+    }
+    m4w_flush<6, true>(acc0, lds, T0);
+    __syncthreads();
+    m4w_flush<6, true>(acci, lds, Tinf);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -2742,8 +3079,9 @@ int lbn::dev_init(lbn_ctx *ctx) {
     HIPCHK(ctx, hipMalloc(&d->P[0], s.n_words / 2 * 24));
     HIPCHK(ctx, hipMalloc(&d->P[1], s.n_words / 4 * 24));
     HIPCHK(ctx, hipMalloc(&d->T, 3584 * 4));
+    HIPCHK(ctx, hipMalloc(&d->pset, ps::WORDS * 4));
     // EL then EH in one allocation, laid out like their mailbox regions (one copy per proof)
-    HIPCHK(ctx, hipMalloc(&d->EL, (4096 + ((size_t)1 << (s.log_packed - 12))) * 16));
+    HIPCHK(ctx, hipMalloc(&d->EL, (4096 + std::max<size_t>((size_t)1 << (s.log_packed - 12), 128)) * 16));
     d->EH = d->EL + 4096;
     HIPCHK(ctx, hipMalloc(&d->eqrow, s.column_len * 24));
     HIPCHK(ctx, hipMalloc(&d->eqlo, 256 * 24));
@@ -2776,7 +3114,7 @@ void lbn::dev_free(lbn_ctx *ctx) {
     DeviceState *d = ctx->dev;
     if (!d) return;
     hipSetDevice(ctx->device);
-    void *ptrs[] = {d->table, d->E[0], d->E[1], d->P[0], d->P[1], d->T, d->EL, d->eqrow, d->eqlo,
+    void *ptrs[] = {d->table, d->E[0], d->E[1], d->P[0], d->P[1], d->T, d->pset, d->EL, d->eqrow, d->eqlo,
                     d->fold_off, d->ntt_tabs, d->ahat, d->cacc, d->cbytes, d->amat, d->cvs, d->sw,
                     d->chal, d->tpart};
     if (d->stream) hipStreamSynchronize(d->stream);
@@ -3015,7 +3353,8 @@ extern "C" size_t lbn_ctx_footprint(int log_vars, uint64_t q_lo, uint64_t q_hi) 
     if (!toep && !np) return 0;
     const size_t n = s.n_words;
     size_t b = n * 16 + n * 16 + n / 2 * 16 + n / 2 * 24 + n / 4 * 24;  // table, E0, E1, P0, P1
-    b += 3584 * 4 + ((size_t)1 << (s.log_packed - 12)) * 16 + 4096 * 16;  // T, EH, EL
+    b += 3584 * 4 + ps::WORDS * 4;  // T, pset
+    b += (4096 + std::max<size_t>((size_t)1 << (s.log_packed - 12), 128)) * 16;  // EL, EH
     b += s.column_len * 24 + 256 * 24 + (size_t)COLUMNS * 128 * 24 + (size_t)COLUMNS * 24;
     b += 4 + (2 * 162 + 1) * sizeof(int) + (size_t)COLUMNS * 32 * sizeof(uint16_t);
     b += (size_t)MAX_PRIMES * tab::SIZE * 4;
@@ -3379,34 +3718,37 @@ int lbn::dev_commit_launch(lbn_ctx *ctx) {
 
 // ----------------------------------------------------------------------------- partials, switch
 
+// partials (crossfield.rs:152-165) through the prefix sets: G = eq(r_hi[6..]) expanded into E[0]
+// (its high coordinates, at most 6, times the low ones, at most 12), then the sets M_u and the
+// partials in one pass over the table (k_prefix_sums). M_u and G stay for the switch.
 extern "C" int lbn_partials(lbn_ctx *ctx, const uint64_t *r_hi, uint64_t *partials) {
     if (!ctx || !r_hi || !partials) return LBN_E_ARG;
     if (!ctx->table_loaded) return ctx->fail(LBN_E_STATE, "no table loaded");
     HIPCHK(ctx, hipSetDevice(ctx->device));
     DeviceState *d = ctx->dev;
     const Shape &s = ctx->shape;
-    const int lp = s.log_packed, nh = lp - 12;
+    const int lp = s.log_packed, S = lp - 6, lo = std::min(S, 12), nh = S - lo;
     ctx->r_hi.assign((const B128 *)r_hi, (const B128 *)r_hi + lp);
-    std::vector<B128> EH((size_t)1 << nh), EL(4096);
-    eq_expand_b128(ctx->r_hi.data(), nh, EH.data());
-    eq_expand_b128(ctx->r_hi.data() + nh, 12, EL.data());
-    memcpy(d->hbox + mb::EH, EH.data(), EH.size() * 16);
-    memcpy(d->hbox + mb::EL, EL.data(), EL.size() * 16);
+    B128 *EL = (B128 *)(d->hbox + mb::EL), *EH = (B128 *)(d->hbox + mb::EH);
+    eq_expand_b128(ctx->r_hi.data() + lp - lo, lo, EL);
+    eq_expand_b128(ctx->r_hi.data() + 6, nh, EH);
+    eq_expand_b128(ctx->r_hi.data(), 6, EH + 64);  // eq6, after the at most 64 high values
     static_assert(mb::EH == mb::EL + 4096 * 16, "EL and EH adjacent, as on the device");
-    HIPCHK(ctx, hipMemcpyAsync(d->EL, d->hbox + mb::EL, (4096 + EH.size()) * 16, hipMemcpyHostToDevice, d->stream));
+    HIPCHK(ctx, hipMemcpyAsync(d->EL, d->hbox + mb::EL, (4096 + 128) * 16, hipMemcpyHostToDevice, d->stream));
     tstart(ctx, T_EQ);
-    // also zeroes T: the partials' accumulator and both switch message buffers
-    k_eq_expand<<<(unsigned)EH.size(), 256, 0, d->stream>>>(d->EH, d->EL, d->E[0], d->T, 3584);
+    // also zeroes the sets and the partials' accumulator
+    k_eq_expand<<<1u << nh, 256, 0, d->stream>>>(d->EH, d->EL, lo, d->E[0], d->pset, (int)ps::Q);
     tstop(ctx, T_EQ);
     HIPCHK(ctx, hipGetLastError());
-    d->msg_clean = true;
-    const unsigned blocks = (unsigned)std::min<size_t>(s.n_words / 256, 1024);
+    d->msg_clean = false;
+    const int chunk_log = std::min(S, 14);
     tstart(ctx, T_PARTIALS);
-    k_partials<<<blocks, 256, 0, d->stream>>>(d->table, d->E[0], s.n_words, d->T + 1536);
+    k_prefix_sums<<<1u << (lp - chunk_log), 256, 0, d->stream>>>(d->table, d->E[0], S, chunk_log, d->EH + 64,
+                                                                 d->pset + ps::M, d->pset + ps::PART);
     tstop(ctx, T_PARTIALS);
     HIPCHK(ctx, hipGetLastError());
     const uint32_t *out = (const uint32_t *)(d->hbox + mb::PART);
-    HIPCHK(ctx, hipMemcpyAsync(d->hbox + mb::PART, d->T + 1536, 512 * 4, hipMemcpyDeviceToHost, d->stream));
+    HIPCHK(ctx, hipMemcpyAsync(d->hbox + mb::PART, d->pset + ps::PART, 512 * 4, hipMemcpyDeviceToHost, d->stream));
     HIPCHK(ctx, hipStreamSynchronize(d->stream));
     tcollect(ctx, T_EQ);
     tcollect(ctx, T_PARTIALS);
@@ -3623,6 +3965,18 @@ static uint32_t *msg_buf(DeviceState *d, int r) { return d->T + ((r & 1) ? 2048 
 
 static int launch_blocks(size_t items) { return (int)std::max<size_t>(1, std::min<size_t>((items + 255) / 256, 1024)); }
 
+// The start of the switch for lbn_prove (SwitchVerifier::start's prover side, crossfield.rs:645-660):
+// the map L_0 from the batch; the rounds (dev_switch_rounds) start from the prefix sets.
+int lbn::dev_switch_prepare(lbn_ctx *ctx, const uint64_t *batch) {
+    if (!ctx->partials_done) return ctx->fail(LBN_E_STATE, "lbn_partials must run first");
+    ctx->lmap.assign((const F162 *)batch, (const F162 *)batch + 128);
+    ctx->round = 0;
+    return LBN_OK;
+}
+
+// The round-by-round switch (lbn_switch_msg / lbn_switch_fold) works on the whole eq expansion:
+// it is expanded here (EH over the first log_packed - 12 coordinates, EL over the last 12), then
+// round 0's message sums.
 extern "C" int lbn_switch_begin(lbn_ctx *ctx, const uint64_t *batch) {
     if (!ctx || !batch) return LBN_E_ARG;
     if (!ctx->partials_done) return ctx->fail(LBN_E_STATE, "lbn_partials must run first");
@@ -3630,9 +3984,16 @@ extern "C" int lbn_switch_begin(lbn_ctx *ctx, const uint64_t *batch) {
     DeviceState *d = ctx->dev;
     ctx->lmap.assign((const F162 *)batch, (const F162 *)batch + 128);
     ctx->round = 0;
+    const int lp = ctx->shape.log_packed, nh = lp - 12;
     const size_t h = ctx->shape.n_words / 2;
-    if (!d->msg_clean) fill_u32(d->stream, d->T, 3584);  // both message buffers (cleared with the partials)
-    d->msg_clean = false;
+    eq_expand_b128(ctx->r_hi.data(), nh, (B128 *)(d->hbox + mb::EH));
+    eq_expand_b128(ctx->r_hi.data() + nh, 12, (B128 *)(d->hbox + mb::EL));
+    HIPCHK(ctx, hipMemcpyAsync(d->EL, d->hbox + mb::EL, (4096 + ((size_t)1 << nh)) * 16, hipMemcpyHostToDevice,
+                               d->stream));
+    // also zeroes both message buffers
+    k_eq_expand<<<1u << nh, 256, 0, d->stream>>>(d->EH, d->EL, 12, d->E[0], d->T, 3584);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipStreamSynchronize(d->stream));  // the mailbox regions are rewritten by the next call
     tstart(ctx, T_SWITCH0);
     k_switch_msg0<<<launch_blocks(h), 256, 0, d->stream>>>(d->table, d->E[0], h, d->E[1], d->T, d->T + 768);
     tstop(ctx, T_SWITCH0);
@@ -3673,19 +4034,51 @@ static void launch_round(lbn_ctx *ctx, int r, RhoArg ra, const uint64_t *rho_dev
                                                                          T, T + 768, msg_buf(d, r), rho_dev);
 }
 
-// The switch rounds' device launches: control step r, then round r's fold and round r + 1's message,
-// for r < r0 = lp - SWITCH_HOST_STEPS (control steps) and r < r0 - 1 (folds); control step r0 - 1 hands
-// the state and round r0 - 1's fold input (written by round r0 - 2) to the host. Everything they read
-// (shape, buffers, the device state d->sw) is fixed per context.
-static int switch_host_from(int lp) { return lp - SWITCH_HOST_STEPS; }
+// The switch rounds' device launches. Rounds 0..5 run on the prefix sets (k_prefix_msg, then control
+// step r) and the bind kernel does their folds with round 6's message, as round 5's kernel would
+// leave them (p^(6) in P[1], E^(7) in E[1], the sums in round 6's buffer); then control step r and
+// round r's fold with
+// round r + 1's message for 6 <= r < r0 = max(7, lp - SWITCH_HOST_STEPS) (control steps) and
+// r < r0 - 1 (folds); control step r0 - 1 hands the state and round r0 - 1's fold input (written by
+// the kernel of round r0 - 2) to the host. Everything they read (shape, buffers, the device state
+// d->sw, the mailbox) is fixed per context.
+static int switch_host_from(int lp) { return std::max(7, lp - SWITCH_HOST_STEPS); }
 static void launch_switch(lbn_ctx *ctx) {
     DeviceState *d = ctx->dev;
-    const int lp = ctx->shape.log_packed, r0 = switch_host_from(lp);
-    const int h2 = 1 << (lp - r0 - 1);  // the handoff fold's items (mb::HANDOFF_H2)
-    for (int r = 0; r < r0; r++) {
-        const bool hand = r == r0 - 1;  // r0 - 2 >= 3 (log_packed >= 12): the input is a P buffer
+    const int lp = ctx->shape.log_packed, r0 = switch_host_from(lp), S = lp - 6;
+    const int h2 = 1 << (lp - r0 - 1);  // the handoff fold's items (at most mb::HANDOFF_H2)
+    const uint4 *lam = (const uint4 *)(d->pset + ps::LAM);
+    k_prefix_q0<<<32, 256, 0, d->stream>>>(d->pset + ps::M, d->pset + ps::Q, (const uint4 *)(d->hbox_dev + mb::LAM),
+                                           (uint4 *)lam, msg_buf(d, 0));
+    for (int r = 0; r < 6; r++) {
+        uint32_t *T = msg_buf(d, r);
+        k_prefix_msg<<<(unsigned)((128 << (5 - r)) + 255) / 256, 256, 0, d->stream>>>(d->pset + ps::Q, r, lam, d->sw, T,
+                                                                                    T + 768, msg_buf(d, r + 1));
+        k_switch_ctl<<<1, 1024, 0, d->stream>>>(d->sw, (SwitchDev *)(d->hbox_dev + mb::SW), r, -1, 6, T, nullptr,
+                                                nullptr, 0, nullptr, nullptr, 0);
+    }
+    {
+        // lanes per output: enough lanes in flight for the device (2^17) when the items are few
+        const size_t items = (size_t)1 << (S - 1);
+        int gl = 0;
+        while (gl < 5 && (items << gl) < ((size_t)1 << 17)) gl++;
+        const unsigned blocks = (unsigned)std::min<size_t>(((items << gl) + BIND_THREADS - 1) / BIND_THREADS, 512);
+        uint32_t *T = msg_buf(d, 6), *clr = msg_buf(d, 7);
+        uint4 *G = d->E[0], *E7 = d->E[1];
+        uint32_t *P6 = d->P[1];
+        switch (gl) {
+            case 0: k_switch_bind<0><<<blocks, BIND_THREADS, 0, d->stream>>>(d->table, S, d->sw, G, E7, P6, T, T + 768, clr); break;
+            case 1: k_switch_bind<1><<<blocks, BIND_THREADS, 0, d->stream>>>(d->table, S, d->sw, G, E7, P6, T, T + 768, clr); break;
+            case 2: k_switch_bind<2><<<blocks, BIND_THREADS, 0, d->stream>>>(d->table, S, d->sw, G, E7, P6, T, T + 768, clr); break;
+            case 3: k_switch_bind<3><<<blocks, BIND_THREADS, 0, d->stream>>>(d->table, S, d->sw, G, E7, P6, T, T + 768, clr); break;
+            case 4: k_switch_bind<4><<<blocks, BIND_THREADS, 0, d->stream>>>(d->table, S, d->sw, G, E7, P6, T, T + 768, clr); break;
+            default: k_switch_bind<5><<<blocks, BIND_THREADS, 0, d->stream>>>(d->table, S, d->sw, G, E7, P6, T, T + 768, clr); break;
+        }
+    }
+    for (int r = 6; r < r0; r++) {
+        const bool hand = r == r0 - 1;  // the handoff fold's input: P[(r0 - 2) & 1] (P[1] from the bind at r0 = 7)
         k_switch_ctl<<<1, 1024, 0, d->stream>>>(
-            d->sw, (SwitchDev *)(d->hbox_dev + mb::SW), r, r0 - 1, r == 0 ? 4 : 6, msg_buf(d, r),
+            d->sw, (SwitchDev *)(d->hbox_dev + mb::SW), r, r0 - 1, 6, msg_buf(d, r),
             hand ? d->P[(r0 - 2) & 1] : nullptr, (uint32_t *)(d->hbox_dev + mb::HP), hand ? 4 * h2 : 0,
             hand ? d->E[r0 & 1] : nullptr, (uint4 *)(d->hbox_dev + mb::HE), hand ? 2 * h2 : 0);
         if (r < r0 - 1) launch_round(ctx, r, RhoArg{{0, 0, 0}}, d->sw->rho);
@@ -3718,7 +4111,16 @@ int lbn::dev_switch_rounds(lbn_ctx *ctx, B3State &ts, uint64_t *rounds, uint64_t
     h->ts = ts;
     memcpy(h->lmap, ctx->lmap.data(), 128 * sizeof(F162));
     for (int r = 0; r < lp; r++) h->r_hi[r][0] = ctx->r_hi[r].w[0], h->r_hi[r][1] = ctx->r_hi[r].w[1];
-    // the first control step reads the state from the mailbox, the last writes it back there
+    {  // the prefix rounds' eq weights: lambda1 of round r = eq(c_r+1..5, .), lambda0 = (1 + c_r) lambda1
+        B128 *lam = (B128 *)(d->hbox + mb::LAM);
+        for (int r = 0; r < 6; r++) {
+            const int nv = 32 >> r, off = 64 - 2 * nv;
+            eq_expand_b128(ctx->r_hi.data() + r + 1, 5 - r, lam + 64 + off);
+            const B128 one_c = b128_add(ctx->r_hi[r], B128{{1, 0}});
+            for (int v = 0; v < nv; v++) lam[off + v] = b128_mul(one_c, lam[64 + off + v]);
+        }
+    }
+    // the prefix kernel reads the state from the mailbox, the last control step writes it back there
     tstart(ctx, T_ROUNDS);
     if (d->graphs)
         HIPCHK(ctx, hipGraphLaunch(d->switch_graph, d->stream));
@@ -3727,7 +4129,6 @@ int lbn::dev_switch_rounds(lbn_ctx *ctx, B3State &ts, uint64_t *rounds, uint64_t
     tstop(ctx, T_ROUNDS);
     HIPCHK(ctx, hipGetLastError());
     HIPCHK(ctx, hipStreamSynchronize(d->stream));
-    tcollect(ctx, T_SWITCH0);
     tcollect(ctx, T_ROUNDS);
     ts = h->ts;
     memcpy(rounds, h->rounds, (size_t)r0 * 48);

@@ -642,7 +642,7 @@ extern "C" int lbn_prove(lbn_ctx *ctx, const uint8_t seed[32], uint8_t *proof, s
     // sample "switch-round", fold; then absorb [opened]); the host continues from its final state.
     std::vector<F162> rounds(2 * (size_t)lp), coords(lp);
     F162 opened;
-    TRY(lbn_switch_begin(ctx, (const uint64_t *)batch.data()));
+    TRY(dev_switch_prepare(ctx, (const uint64_t *)batch.data()));
     {
         B3State ts;
         t.save(ts);

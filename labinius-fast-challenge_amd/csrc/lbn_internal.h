@@ -172,6 +172,9 @@ struct SwitchHandoff {
     std::vector<B128> e;     // and its eq values (2 h2)
 };
 int dev_switch_rounds(lbn_ctx *ctx, B3State &ts, uint64_t *rounds, uint64_t *coords, SwitchHandoff &ho);
+// The switch's start for dev_switch_rounds: the map L_0 from the batch (no kernel; the rounds start
+// from the prefix sets lbn_partials left).
+int dev_switch_prepare(lbn_ctx *ctx, const uint64_t *batch);
 // The map update of switch round r: L_{r+1}[k] = L_r((1 + c) x^k) + rho L_r[k], c = r_hi[r].
 void lmap_next(std::vector<F162> &L, B128 c, const F162 &rho);
 // The 1024 folding challenges sampled on the device from the transcript state in arg (the

@@ -137,6 +137,25 @@ def test_deterministic(m, p20):
     assert a == b
 
 
+def test_timed_proofs_equal_untimed(m, p20):
+    """Per-pass timing (lbn_set_timing) on: the same proof bytes, and every timed pass of the
+    prover reports a time."""
+    say("timed proofs")
+    p20.generate_table(SEEDS[1])
+    p20.set_modulus(O.DEFAULT_Q)
+    a, _ = p20.prove(SEEDS[1])
+    p20.set_timing(True)
+    try:
+        for _ in range(2):
+            b, _ = p20.prove(SEEDS[1])
+            assert a == b
+        t = p20.timing()
+    finally:
+        p20.set_timing(False)
+    for f in ("eq_ms", "partials_ms", "switch_rounds_ms", "row_ms", "fold_ms", "commit_mac_ms"):
+        assert t[f] > 0, (f, t)
+
+
 def test_switch_graph_replays(m):
     """The captured switch graph replayed by several proofs (other work and host syncs between the
     replays, different seeds and tables) gives the bytes of the launch-by-launch switch."""

@@ -110,6 +110,13 @@ int lbn_prove(lbn_ctx *ctx, const uint8_t seed[32], uint8_t *proof, size_t cap, 
  * is regenerated into the context (replacing a loaded table); no modulus needs to be set. */
 int lbn_verify(lbn_ctx *ctx, const uint8_t seed[32], const uint8_t *proof, size_t len);
 
+/* rq::ajtai_matrix (verifier/proof_verify/src/rq.rs:53-144), elements first .. first + count - 1 of
+ * the expansion for (Q, elements), computed on device `device` (the expansion every context and the
+ * checker use): out receives count x 648 coefficients below Q, each as two uint64 (low, high).
+ * Indices past `elements` continue the same keyed stream family. */
+int lbn_ajtai_elements(int device, uint64_t q_lo, uint64_t q_hi, size_t elements, size_t first, size_t count,
+                       uint64_t *out);
+
 /* Pinned host memory for proof buffers (no counterpart in the reference, which writes the proof
  * from an ordinary Vec): when lbn_prove's proof buffer lies in an lbn_host_alloc allocation, the
  * fold kernel writes its output into it directly and the commitment bytes are copied into it by DMA,

@@ -91,6 +91,8 @@ def lib():
         L.lbn_proof_len.restype = ctypes.c_size_t
         L.lbn_prove.argtypes = [P, P, P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t), P]
         L.lbn_verify.argtypes = [P, P, P, ctypes.c_size_t]
+        L.lbn_ajtai_elements.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_size_t,
+                                         ctypes.c_size_t, ctypes.c_size_t, P]
         L.lbn_commit_primes.argtypes = [P]
         L.lbn_copy_probe.argtypes = [P]
         L.lbn_commit_planes.argtypes = [P]
@@ -291,6 +293,16 @@ class Prover:
         t = Timing()
         self._check(lib().lbn_get_timing(self.h, ctypes.addressof(t)))
         return {f: getattr(t, f) for f, _ in Timing._fields_}
+
+
+def ajtai_elements(q: int, elements: int, first: int, count: int, device: int = 0) -> np.ndarray:
+    """Elements first .. first + count - 1 of the public matrix expansion for (q, elements), expanded
+    on the GPU: a (count, 648) array of Python ints below q."""
+    out = np.zeros((count, 648, 2), dtype=np.uint64)
+    rc = lib().lbn_ajtai_elements(device, q & (2**64 - 1), q >> 64, elements, first, count, _p(out))
+    if rc != 0:
+        raise LbnError(rc, "lbn_ajtai_elements failed")
+    return [[int(lo) | (int(hi) << 64) for lo, hi in row] for row in out]
 
 
 def exported_symbols():

@@ -27,8 +27,6 @@ using namespace lbn;
 using namespace lbn::dev;
 
 namespace lbn {
-void ajtai_matrix_host(u128 q, size_t elements, u128 *out);  // host BLAKE3 expansion
-void ajtai_elements_host(u128 q, size_t elements, size_t first, size_t count, u128 *out);
 bool prime_tables(const PrimeInfo &pi, std::vector<uint32_t> &tabs);
 }  // namespace lbn
 
@@ -236,6 +234,82 @@ __global__ void k_table_xof(B3Root root, uint32_t *out, size_t blocks) {
         o[k] = s[k] ^ s[k + 8];
         o[k + 8] = s[k + 8] ^ root.cv[k];
     }
+}
+
+// rq::ajtai_matrix (verifier/proof_verify/src/rq.rs:53-144) on the device: element e (index
+// first + e of the expansion for (q, elements)) is the keyed BLAKE3 XOF of domain || q || u64(elements)
+// || u64(index), one 61-byte block whose root node is `root` with the index in bytes 53..60; its
+// coefficients are the first 648 width-byte little-endian draws of the stream that, masked to
+// bits(q - 1), are below q. One wave per element: each pass its 64 lanes make 64 output blocks (4 KiB
+// of stream, after the previous pass's last block, kept in LDS), then the wave takes the draws that
+// end in that stretch, 64 at a time in stream order, placing the accepted ones by ballot rank. The
+// acceptance rate q / 2^bits is above 1/2, so a pass yields about 4096 / (2 width) coefficients;
+// a wave that has not finished after AJ_PASSES passes sets *fail.
+constexpr int AJ_WAVES = 4, AJ_PASSES = 64, AJ_WORDS = 16 * 65 + 4;
+__global__ __launch_bounds__(64 * AJ_WAVES) void k_ajtai(B3Root root, uint64_t first, int count, int width,
+                                                         uint64_t mask_lo, uint32_t mask_hi, uint64_t q_lo,
+                                                         uint32_t q_hi, uint64_t *out, uint32_t *fail) {
+    __shared__ uint32_t sbuf[AJ_WAVES][AJ_WORDS];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int e = blockIdx.x * AJ_WAVES + wv;
+    if (e >= count) return;  // the whole wave: no block barrier below
+    uint32_t *B = sbuf[wv];
+    const uint64_t index = first + (uint64_t)e;
+    uint32_t mt[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) mt[k] = root.m[k];
+    mt[13] |= (uint32_t)(index << 8), mt[14] = (uint32_t)(index >> 24), mt[15] = (uint32_t)(index >> 56);
+    const uint32_t IV0 = 0x6A09E667u, IV1 = 0xBB67AE85u, IV2 = 0x3C6EF372u, IV3 = 0xA54FF53Au;
+    const uint64_t lanes_below = (1ull << lane) - 1;
+    uint64_t *dst = out + (size_t)e * N * 2;
+    int have = 0;
+    uint64_t t = 0;  // the next draw
+    for (int pass = 0; have < N && pass < AJ_PASSES; pass++) {
+        {
+            const uint64_t blk = 64ull * pass + lane;
+            uint32_t st[16] = {root.cv[0], root.cv[1], root.cv[2], root.cv[3], root.cv[4], root.cv[5], root.cv[6],
+                               root.cv[7], IV0, IV1, IV2, IV3, (uint32_t)blk, (uint32_t)(blk >> 32), root.len, root.flags};
+            uint32_t m[16];
+#pragma unroll
+            for (int k = 0; k < 16; k++) m[k] = mt[k];
+            b3_rounds(st, m);
+            uint32_t *o = B + 16 * (1 + lane);
+#pragma unroll
+            for (int k = 0; k < 8; k++) o[k] = st[k] ^ st[k + 8], o[k + 8] = st[k + 8] ^ root.cv[k];
+        }
+        __syncwarp();
+        // B holds stream bytes [4096 pass - 64, 4096 pass + 4096) (the first 64 only after pass 0)
+        const int64_t base = 4096ll * pass - 64, end = 4096ll * (pass + 1);
+        while (have < N) {
+            const int64_t at = (int64_t)(t + lane) * width;
+            const bool inside = at + width <= end;
+            const uint64_t in = __ballot(inside);
+            if (!in) break;
+            bool acc = false;
+            uint64_t lo = 0;
+            uint32_t hi = 0;
+            if (inside) {
+                const int off = (int)(at - base), w0 = off >> 2;
+                const uint32_t sh = 8u * (uint32_t)(off & 3);
+                const uint32_t W0 = B[w0], W1 = B[w0 + 1], W2 = B[w0 + 2], W3 = B[w0 + 3];
+                const uint32_t x0 = __builtin_amdgcn_alignbit(W1, W0, sh), x1 = __builtin_amdgcn_alignbit(W2, W1, sh);
+                const uint32_t x2 = __builtin_amdgcn_alignbit(W3, W2, sh);
+                lo = ((uint64_t)x0 | ((uint64_t)x1 << 32)) & mask_lo;
+                hi = x2 & mask_hi;
+                acc = hi < q_hi || (hi == q_hi && lo < q_lo);
+            }
+            const uint64_t bal = __ballot(acc);
+            const int rank = have + __popcll(bal & lanes_below);
+            if (acc && rank < N) dst[2 * rank] = lo, dst[2 * rank + 1] = hi;
+            have += __popcll(bal);
+            t += (uint64_t)__popcll(in);
+            if (in != ~0ull) break;  // the rest end past this stretch
+        }
+        __syncwarp();
+        if (lane < 16) B[lane] = B[16 * 64 + lane];  // this pass's last block opens the next stretch
+        __syncwarp();
+    }
+    if (have < N && lane == 0) atomicOr(fail, 1u);
 }
 
 // Chaining values of whole 1 KiB chunks of a byte string (the commitment as the transcript
@@ -1457,16 +1531,22 @@ __global__ __launch_bounds__(BIND_THREADS) void k_switch_bind(const uint4 *__res
 // Row evaluation (eval.rs:173-184, 426-480): eq table of p0, then per column
 //   U_b = sum_i bit_b(w_{c,i}) eq_i   (M4RM)   and   row[c] = sum_b x^b U_b  (mod x^162+x^81+1).
 // ---------------------------------------------------------------------------------------------
-__global__ void k_eq_row(const uint64_t *lo, const uint64_t *hi, int lo_bits, size_t clen, uint32_t *eqrow) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= clen) return;
-    const uint64_t *a = lo + 3 * (i & ((1u << lo_bits) - 1));
-    const uint64_t *b = hi + 3 * (i >> lo_bits);
-    uint64_t r[3];
-    f162_mul_slow(a, b, r);
-    uint32_t *o = eqrow + 6 * i;
-    o[0] = (uint32_t)r[0], o[1] = (uint32_t)(r[0] >> 32), o[2] = (uint32_t)r[1], o[3] = (uint32_t)(r[1] >> 32);
-    o[4] = (uint32_t)r[2], o[5] = (uint32_t)(r[2] >> 32);
+// The eq table of p0 as the products lo[i mod 2^lo_bits] hi[i >> lo_bits]: block h builds the
+// window table of y -> hi[h] y and its 2^lo_bits threads take one product each.
+__global__ __launch_bounds__(256) void k_eq_row(const uint64_t *lo, const uint64_t *hi, int lo_bits, uint32_t *eqrow) {
+    __shared__ uint32_t tab[RHO_WINDOWS * 16 * 6], basis[162 * 6];
+    const uint64_t *h = hi + 3 * blockIdx.x;
+    build_rho_table(RhoArg{{h[0], h[1], h[2]}}, basis, tab);
+    const int t = threadIdx.x;
+    if (t >= (1 << lo_bits)) return;
+    const uint64_t *a = lo + 3 * t;
+    const uint32_t y[6] = {(uint32_t)a[0], (uint32_t)(a[0] >> 32), (uint32_t)a[1],
+                           (uint32_t)(a[1] >> 32), (uint32_t)a[2], (uint32_t)(a[2] >> 32)};
+    uint32_t r[6];
+    rho_mul<RHO_WINDOWS>(tab, y, r);
+    uint32_t *o = eqrow + 6 * (((size_t)blockIdx.x << lo_bits) + t);
+#pragma unroll
+    for (int k = 0; k < 6; k++) o[k] = r[k];
 }
 
 // Block c: U_b = sum_i bit_b(w_{c,i}) eq_i for b < 128 (M4RM over the column), then in the same block
@@ -3365,20 +3445,66 @@ extern "C" size_t lbn_ctx_footprint(int log_vars, uint64_t q_lo, uint64_t q_hi) 
     return b;
 }
 
-// The expanded matrix depends only on (Q, ring elements); contexts of one shape share it.
-static std::shared_ptr<const std::vector<u128>> public_matrix(u128 q, size_t re) {
-    static std::mutex m;
-    static std::map<std::pair<u128, size_t>, std::shared_ptr<const std::vector<u128>>> cache;
-    {
-        std::lock_guard<std::mutex> lk(m);
-        auto it = cache.find({q, re});
-        if (it != cache.end()) return it->second;
+// Elements first .. first + count - 1 of the matrix expansion for (q, elements) into out (device,
+// 2 x u64 per coefficient), on `stream`; waits for it. Indices past `elements` continue the same
+// keyed stream family (a key of several ring rows takes row r's element m at r * elements + m).
+static hipError_t ajtai_device(hipStream_t stream, u128 q, size_t elements, size_t first, size_t count,
+                               uint64_t *out, bool *finished) {
+    int bits = 0;
+    for (u128 x = q - 1; x; x >>= 1) bits++;
+    const int width = (bits + 7) / 8;
+    const u128 mask = ((u128)1 << bits) - 1;
+    uint8_t key[32];
+    memset(key, 0x5a, 32);
+    Blake3 h = Blake3::keyed(key);
+    static const char DOMAIN[] = "labinius-fast/ajtai-matrix/v1";
+    h.update(DOMAIN, sizeof DOMAIN - 1);
+    uint8_t qb[16];
+    for (int i = 0; i < 16; i++) qb[i] = (uint8_t)(q >> (8 * i));
+    h.update(qb, 16);
+    h.update_u64(elements);
+    h.update_u64(0);  // the index bytes (53..60) are filled in per element
+    const Blake3::Root r = h.root();
+    static_assert(sizeof DOMAIN - 1 + 16 + 8 == 53, "the element index sits at bytes 53..60 of the block");
+    if (r.len != 61) return hipErrorInvalidValue;
+    B3Root root{};
+    memcpy(root.cv, r.cv, 32);
+    memcpy(root.m, r.m, 64);
+    root.len = r.len;
+    root.flags = r.flags | Blake3::ROOT;
+    uint32_t *fail = nullptr;
+    hipError_t e = hipMalloc(&fail, 4);
+    if (e == hipSuccess) e = hipMemsetAsync(fail, 0, 4, stream);
+    if (e == hipSuccess && count) {
+        k_ajtai<<<(unsigned)((count + AJ_WAVES - 1) / AJ_WAVES), 64 * AJ_WAVES, 0, stream>>>(
+            root, (uint64_t)first, (int)count, width, (uint64_t)mask, (uint32_t)(mask >> 64), (uint64_t)q,
+            (uint32_t)(q >> 64), out, fail);
+        e = hipGetLastError();
     }
-    auto A = std::make_shared<std::vector<u128>>(re * N);
-    ajtai_matrix_host(q, re, A->data());
-    std::lock_guard<std::mutex> lk(m);
-    if (cache.size() >= 16) cache.clear();
-    return cache.emplace(std::make_pair(q, re), A).first->second;
+    uint32_t flag = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&flag, fail, 4, hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (fail) (void)hipFree(fail);
+    *finished = flag == 0;
+    return e;
+}
+
+extern "C" int lbn_ajtai_elements(int device, uint64_t q_lo, uint64_t q_hi, size_t elements, size_t first,
+                                  size_t count, uint64_t *out) {
+    const u128 q = ((u128)q_hi << 64) | q_lo;
+    if (!out || q < 2 || q >= ((u128)1 << 96) || count > ((size_t)1 << 20)) return LBN_E_ARG;
+    if (hipSetDevice(device) != hipSuccess) return LBN_E_DEVICE;
+    uint64_t *d = nullptr;
+    hipStream_t st = nullptr;
+    bool done = false;
+    hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc(&d, count * N * 16 + 16);
+    if (e == hipSuccess) e = ajtai_device(st, q, elements, first, count, d, &done);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d, count * N * 16, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (d) (void)hipFree(d);
+    if (st) (void)hipStreamDestroy(st);
+    return e != hipSuccess ? LBN_E_DEVICE : done ? LBN_OK : LBN_E_DEVICE;
 }
 
 // The first np CRT primes with their Montgomery constants, their transform tables (all,
@@ -3446,9 +3572,12 @@ static int setup_toep(lbn_ctx *ctx, u128 q) {
     auto ops = std::make_shared<ToepOps>();
     ops->device = ctx->device;
     HIPCHK(ctx, hipMalloc(&ops->ptr, toep_ops_bytes(re)));
-    const std::shared_ptr<const std::vector<u128>> A = public_matrix(q, re);
     HIPCHK(ctx, hipMalloc(&d->amat, re * N * 16));
-    HIPCHK(ctx, hipMemcpyAsync(d->amat, A->data(), A->size() * 16, hipMemcpyHostToDevice, d->stream));
+    {
+        bool done = false;
+        HIPCHK(ctx, ajtai_device(d->stream, q, re, 0, re, d->amat, &done));
+        if (!done) return ctx->fail(LBN_E_DEVICE, "matrix expansion did not finish");
+    }
     k_toep_build<<<dim3(TP_TILES, (unsigned)re), 64, 0, d->stream>>>(d->amat, (int)re, (uint64_t)q, ops->ptr);
     HIPCHK(ctx, hipGetLastError());
     HIPCHK(ctx, hipStreamSynchronize(d->stream));
@@ -3490,10 +3619,13 @@ extern "C" int lbn_set_modulus(lbn_ctx *ctx, uint64_t q_lo, uint64_t q_hi) {
     std::vector<uint32_t> all;
     if (!setup_primes(np, ctx->primes, d->roots, all)) return ctx->fail(LBN_E_ARG, "NTT table construction failed");
     HIPCHK(ctx, hipMemcpyAsync(d->ntt_tabs, all.data(), all.size() * 4, hipMemcpyHostToDevice, d->stream));
-    // The public matrix (rq.rs:114-144), expanded on the host, reduced and transformed here.
-    const std::shared_ptr<const std::vector<u128>> A = public_matrix(q, s.ring_elements);
+    // The public matrix (rq.rs:114-144), expanded, reduced and transformed on the device.
     HIPCHK(ctx, hipMalloc(&d->amat, s.ring_elements * N * 16));
-    HIPCHK(ctx, hipMemcpyAsync(d->amat, A->data(), A->size() * 16, hipMemcpyHostToDevice, d->stream));
+    {
+        bool done = false;
+        HIPCHK(ctx, ajtai_device(d->stream, q, s.ring_elements, 0, s.ring_elements, d->amat, &done));
+        if (!done) return ctx->fail(LBN_E_DEVICE, "matrix expansion did not finish");
+    }
     const size_t need = (size_t)np * s.ring_elements * N;
     if (d->ahat_elems < need) {
         if (d->ahat) HIPCHK(ctx, hipFree(d->ahat));
@@ -3838,7 +3970,6 @@ int lbn::dev_verify_products(lbn_ctx *ctx, u128 q, const int16_t *v, const uint8
                              const lbn_challenge *ch, __int128 *lhs, __int128 *rhs) {
     const size_t re = ctx->shape.ring_elements;
     HIPCHK(ctx, hipSetDevice(ctx->device));
-    const std::shared_ptr<const std::vector<u128>> A = public_matrix(q, re);
     const int nb = (int)((re + VT_M_PER - 1) / VT_M_PER), ny = COLUMNS / 64;
     const size_t clen = (size_t)COLUMNS * N * width;
     uint64_t *damat = nullptr;
@@ -3856,9 +3987,10 @@ int lbn::dev_verify_products(lbn_ctx *ctx, u128 q, const int16_t *v, const uint8
         if (e == hipSuccess) e = r;
         return e == hipSuccess;
     };
+    bool expanded = false;
     if (ok(hipMalloc(&damat, re * N * 16)) && ok(hipMalloc(&dv, re * N * 2)) && ok(hipMalloc(&dc, clen + 16)) &&
         ok(hipMalloc(&dch, COLUMNS * sizeof(lbn_challenge))) && ok(hipMalloc(&dout, out.size() * 8)) &&
-        ok(hipMemcpyAsync(damat, A->data(), re * N * 16, hipMemcpyHostToDevice, ctx->dev->stream)) &&
+        ok(ajtai_device(ctx->dev->stream, q, re, 0, re, damat, &expanded)) && expanded &&
         ok(hipMemcpyAsync(dv, v, re * N * 2, hipMemcpyHostToDevice, ctx->dev->stream)) &&
         ok(hipMemcpyAsync(dc, commitment, clen, hipMemcpyHostToDevice, ctx->dev->stream)) &&
         ok(hipMemcpyAsync(dch, ch, COLUMNS * sizeof(lbn_challenge), hipMemcpyHostToDevice, ctx->dev->stream))) {
@@ -3871,6 +4003,7 @@ int lbn::dev_verify_products(lbn_ctx *ctx, u128 q, const int16_t *v, const uint8
     }
     release();
     if (e != hipSuccess) return ctx->fail(LBN_E_DEVICE, std::string("checker products: ") + hipGetErrorString(e));
+    if (!expanded) return ctx->fail(LBN_E_DEVICE, "matrix expansion did not finish");
     for (int i = 0; i < VT_RAW * 3; i++) {
         __int128 l = 0, r = 0;
         for (int b = 0; b < nb; b++) l += out[(size_t)b * VT_RAW * 3 + i];
@@ -4213,8 +4346,8 @@ extern "C" int lbn_row_eval(lbn_ctx *ctx, const uint64_t *p0, int n_p0, uint64_t
     memcpy(d->hbox + mb::EQ + 128 * 24, hi.data(), hi.size() * 24);
     HIPCHK(ctx, hipMemcpyAsync(d->eqlo, d->hbox + mb::EQ, (128 + hi.size()) * 24, hipMemcpyHostToDevice, d->stream));
     tstart(ctx, T_ROW);
-    k_eq_row<<<(unsigned)((s.column_len + 255) / 256), 256, 0, d->stream>>>(
-        (const uint64_t *)d->eqlo, (const uint64_t *)((uint8_t *)d->eqlo + 128 * 24), lo_bits, s.column_len, d->eqrow);
+    k_eq_row<<<1u << hi_bits, 256, 0, d->stream>>>((const uint64_t *)d->eqlo,
+                                                   (const uint64_t *)((uint8_t *)d->eqlo + 128 * 24), lo_bits, d->eqrow);
     HIPCHK(ctx, hipGetLastError());
     k_row<<<COLUMNS, 256, 0, d->stream>>>(d->table, d->eqrow, s.column_len,
                                           (uint64_t *)(d->hbox_dev + mb::ROW));  // into the mailbox
@@ -4463,13 +4596,16 @@ extern "C" int lbn_cmt_create(int device, int columns, int elements, int rows, u
         const int rstride = rows == 1 ? 0 : 16;
         HIPCHK(c, hipMalloc(&c->ahat, (size_t)np * re * N * (rows == 1 ? 1 : 16) * 4));
         if (rows > 1) HIPCHK(c, hipMemsetAsync(c->ahat, 0, (size_t)np * re * N * 16 * 4, c->stream));
-        // the key, one row at a time: expanded on the host, reduced and transformed on the device
-        std::vector<u128> A(re * N);
+        // the key, one row at a time: expanded, reduced and transformed on the device
         uint64_t *amat = nullptr;
         HIPCHK(c, hipMalloc(&amat, re * N * 16));
         for (int r = 0; r < rows; r++) {
-            ajtai_elements_host(q, re, (size_t)r * re, re, A.data());
-            HIPCHK(c, hipMemcpyAsync(amat, A.data(), re * N * 16, hipMemcpyHostToDevice, c->stream));
+            bool done = false;
+            const hipError_t ex = ajtai_device(c->stream, q, re, (size_t)r * re, re, amat, &done);
+            if (ex != hipSuccess || !done) {
+                (void)hipFree(amat);
+                return c->fail(LBN_E_DEVICE, ex != hipSuccess ? hipGetErrorString(ex) : "matrix expansion did not finish");
+            }
             for (int k = 0; k < np; k++) {
                 PC pc{c->primes[k].p, c->primes[k].pinv_neg};
                 k_ahat<<<(unsigned)re, 64, 0, c->stream>>>(amat, c->tabs + (size_t)k * tab::SIZE, pc, c->roots[k],

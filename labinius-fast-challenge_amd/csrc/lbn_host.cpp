@@ -1,7 +1,8 @@
 // Host half of liblabinius_gpu: binary-field products (PCLMULQDQ), the Fiat-Shamir transcript,
-// the short-challenge sampler, the public matrix expansion, the NTT constant tables, and the
-// whole-proof orchestration lbn_prove (the order of solution/proof_generate/src/main.rs:1692-2009
-// and verifier/proof_verify/src/reference.rs:137-177). The table-sized passes run on the GPU.
+// the short-challenge sampler's constants, the NTT constant tables, the whole-proof orchestration
+// lbn_prove (the order of solution/proof_generate/src/main.rs:1692-2009 and
+// verifier/proof_verify/src/reference.rs:137-177) and the checker lbn_verify. The table-sized
+// passes and the public matrix expansion run on the GPU.
 #include <immintrin.h>
 
 #include <algorithm>
@@ -368,50 +369,6 @@ struct ChallengeSampler {
         arg.bound_sq = bound * bound + 1e-12;
     }
 };
-
-// ============================================================================= public matrix
-// rq::ajtai_matrix (verifier/proof_verify/src/rq.rs:53-144): element i from BLAKE3 keyed
-// [0x5a; 32] over domain || q (16 LE) || u64(elements) || u64(i); width-byte LE draws masked to
-// bits(q-1), rejected while >= q.
-
-// Elements first .. first + count - 1 of the expansion for (q, elements); indices past `elements`
-// continue the same keyed stream family (a key of several ring rows takes row r's element m at
-// index r * elements + m, so row 0 is the matrix itself).
-void ajtai_elements_host(u128 q, size_t elements, size_t first, size_t count, u128 *out) {
-    int bits = 0;
-    for (u128 x = q - 1; x; x >>= 1) bits++;
-    const int width = (bits + 7) / 8;
-    const u128 mask = bits == 128 ? ~(u128)0 : (((u128)1 << bits) - 1);
-    uint8_t seed[32];
-    memset(seed, 0x5a, 32);
-    Blake3 base = Blake3::keyed(seed);
-    static const char DOMAIN[] = "labinius-fast/ajtai-matrix/v1";
-    base.update(DOMAIN, sizeof DOMAIN - 1);
-    uint8_t qb[16];
-    for (int i = 0; i < 16; i++) qb[i] = (uint8_t)(q >> (8 * i));
-    base.update(qb, 16);
-    base.update_u64(elements);
-    parallel_for(count, [&](size_t i) {
-        Blake3 h = base;
-        h.update_u64(first + i);
-        XofReader x(h);
-        for (int t = 0; t < N; t++) {
-            for (;;) {
-                uint8_t b[16];
-                x.read(b, width);
-                u128 v = 0;
-                for (int k = 0; k < width; k++) v |= (u128)b[k] << (8 * k);
-                v &= mask;
-                if (v < q) {
-                    out[i * N + t] = v;
-                    break;
-                }
-            }
-        }
-    });
-}
-
-void ajtai_matrix_host(u128 q, size_t elements, u128 *out) { ajtai_elements_host(q, elements, 0, elements, out); }
 
 // ============================================================================= NTT tables
 

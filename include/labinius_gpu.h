@@ -59,6 +59,13 @@ const char *lbn_last_error(const lbn_ctx *ctx);
  * lbn_load_table copies host memory to HBM; lbn_load_table_device copies from device memory. */
 int lbn_load_table(lbn_ctx *ctx, const void *host_bytes, size_t bytes);
 int lbn_load_table_device(lbn_ctx *ctx, const void *device_bytes, size_t bytes);
+/* The same table with its upload deferred into the proof, as the reference folds the table's page
+ * faults into the commitment's first read (main.rs:387-429, 1736-1740): lbn_prove / lbn_commit copy
+ * it in four ring-element ranges, the commitment MAC of each range starting once its copy is in;
+ * any other entry point that reads the table copies it whole first. host_bytes must stay valid and
+ * unchanged until that call returns. Pinned memory (lbn_host_alloc) lets the ranges go by DMA
+ * while the MAC runs; other memory works, more slowly. */
+int lbn_load_table_async(lbn_ctx *ctx, const void *host_bytes, size_t bytes);
 /* Fill the table on the device from a 32-byte seed: the BLAKE3 XOF the trusted table writer
  * produces (verifier/table_writer/src/main.rs:14-22). */
 int lbn_generate_table(lbn_ctx *ctx, const uint8_t seed[32]);

@@ -77,6 +77,7 @@ def lib():
         L.lbn_last_error.restype = ctypes.c_char_p
         L.lbn_load_table.argtypes = [P, P, ctypes.c_size_t]
         L.lbn_load_table_device.argtypes = [P, P, ctypes.c_size_t]
+        L.lbn_load_table_async.argtypes = [P, P, ctypes.c_size_t]
         L.lbn_generate_table.argtypes = [P, P]
         L.lbn_set_modulus.argtypes = [P, ctypes.c_uint64, ctypes.c_uint64]
         L.lbn_commit.argtypes = [P, P, ctypes.c_size_t]
@@ -188,6 +189,14 @@ class Prover:
     def load_table(self, table: np.ndarray):
         t = np.ascontiguousarray(table, dtype=np.uint8)
         self._check(lib().lbn_load_table(self.h, _p(t), t.size))
+
+    def load_table_async(self, table):
+        """lbn_load_table_async: the upload goes with the next proof (or other table reader). `table`
+        is a uint8 numpy array or a PinnedBuffer; this object keeps it alive until then."""
+        self._check(lib().lbn_load_table_async(self.h, _p(table) if isinstance(table, np.ndarray) else
+                                               ctypes.c_void_p(table.ptr), table.size if isinstance(table, np.ndarray)
+                                               else table.nbytes))
+        self._pending = table
 
     def load_table_device(self, device_ptr: int, nbytes: int):
         """Copy the table from device memory (device_ptr: an address on this context's GPU)."""

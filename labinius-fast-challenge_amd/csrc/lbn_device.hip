@@ -86,6 +86,10 @@ struct lbn::DeviceState {
     std::shared_ptr<::ToepOps> toep;
     int64_t *tpart = nullptr;
     int tp_splits = 0;
+    // A host table whose upload is deferred (lbn_load_table_async): the commitment copies it in
+    // ring-element ranges, each range's MAC waiting for its copy (events on the copy stream).
+    const uint8_t *pend = nullptr;
+    hipEvent_t ev_chunk[4] = {};
 };
 
 static int capture_switch(lbn_ctx *ctx);
@@ -2657,11 +2661,12 @@ typedef int tp_v4i __attribute__((ext_vector_type(4)));
 // blocks of one split run on one XCD (block b goes to XCD b % 8), so the witness words they all
 // read come from that XCD's L2. Partial sums: part[((sp * 5 + d) * 656 + row) * 1024 + column].
 __global__ __launch_bounds__(64 * TP_WAVES) void k_toep_mac(const uint4 *__restrict__ aop, const uint4 *__restrict__ table,
-                                                            size_t clen, int re, int splits, int64_t *part) {
-    const int nb = TP_TILES * splits, b = blockIdx.x;
+                                                            size_t clen, int re, int splits, int sp0, int nsp,
+                                                            int64_t *part) {
+    const int nb = TP_TILES * nsp, b = blockIdx.x;  // splits sp0 .. sp0 + nsp - 1 of `splits`
     const int full = nb / 8 * 8;
     const int lb = b < full ? (b % 8) * (nb / 8) + b / 8 : b;  // logical block: same split, same XCD
-    const int r = lb % TP_TILES, sp = lb / TP_TILES;
+    const int r = lb % TP_TILES, sp = sp0 + lb / TP_TILES;
     const int m0 = (int)((int64_t)sp * re / splits), m1 = (int)((int64_t)(sp + 1) * re / splits);
     const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int g = l >> 4, n = l & 15;
@@ -3181,6 +3186,7 @@ int lbn::dev_init(lbn_ctx *ctx) {
     HIPCHK(ctx, hipEventCreateWithFlags(&d->ev_commit, hipEventDisableTiming));
     HIPCHK(ctx, hipEventCreateWithFlags(&d->ev_staged, hipEventDisableTiming));
     for (auto &e : d->ev) HIPCHK(ctx, hipEventCreate(&e));
+    for (auto &e : d->ev_chunk) HIPCHK(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming));
     return capture_switch(ctx);
 }
 
@@ -3207,6 +3213,8 @@ void lbn::dev_free(lbn_ctx *ctx) {
     if (d->hcvs) hipHostFree(d->hcvs);
     for (auto &e : d->ev)
         if (e) hipEventDestroy(e);
+    for (auto &e : d->ev_chunk)
+        if (e) hipEventDestroy(e);
     if (d->ev_commit) hipEventDestroy(d->ev_commit);
     if (d->ev_staged) hipEventDestroy(d->ev_staged);
     if (d->switch_graph) hipGraphExecDestroy(d->switch_graph);
@@ -3217,13 +3225,45 @@ void lbn::dev_free(lbn_ctx *ctx) {
     ctx->dev = nullptr;
 }
 
+// Ring elements [m0, m1) of every column of the pending host table (one 2D copy on the copy stream);
+// the main stream waits for it.
+static int upload_range(lbn_ctx *ctx, int g, size_t m0, size_t m1) {
+    DeviceState *d = ctx->dev;
+    const size_t pitch = ctx->shape.column_len * 16;
+    HIPCHK(ctx, hipMemcpy2DAsync((uint8_t *)d->table + 64 * m0, pitch, d->pend + 64 * m0, pitch, 64 * (m1 - m0),
+                                 COLUMNS, hipMemcpyHostToDevice, d->copy_stream));
+    HIPCHK(ctx, hipEventRecord(d->ev_chunk[g], d->copy_stream));
+    HIPCHK(ctx, hipStreamWaitEvent(d->stream, d->ev_chunk[g], 0));
+    return LBN_OK;
+}
+
+// The whole pending table, before a pass other than the commitment reads it.
+static int table_ready(lbn_ctx *ctx) {
+    if (!ctx->dev->pend) return LBN_OK;
+    const int rc = upload_range(ctx, 0, 0, ctx->shape.ring_elements);
+    ctx->dev->pend = nullptr;
+    return rc;
+}
+
 extern "C" int lbn_load_table(lbn_ctx *ctx, const void *host_bytes, size_t bytes) {
     if (!ctx || !host_bytes) return LBN_E_ARG;
     if (bytes != ctx->shape.table_bytes)
         return ctx->fail(LBN_E_ARG, "witness table has the wrong length");
+    ctx->dev->pend = nullptr;  // replaces a deferred upload
     HIPCHK(ctx, hipSetDevice(ctx->device));
     HIPCHK(ctx, hipMemcpyAsync(ctx->dev->table, host_bytes, bytes, hipMemcpyHostToDevice, ctx->dev->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->dev->stream));
+    ctx->table_loaded = true;
+    ctx->partials_done = false;
+    ctx->round = -1;
+    return LBN_OK;
+}
+
+extern "C" int lbn_load_table_async(lbn_ctx *ctx, const void *host_bytes, size_t bytes) {
+    if (!ctx || !host_bytes) return LBN_E_ARG;
+    if (bytes != ctx->shape.table_bytes)
+        return ctx->fail(LBN_E_ARG, "witness table has the wrong length");
+    ctx->dev->pend = (const uint8_t *)host_bytes;
     ctx->table_loaded = true;
     ctx->partials_done = false;
     ctx->round = -1;
@@ -3234,6 +3274,7 @@ extern "C" int lbn_load_table_device(lbn_ctx *ctx, const void *device_bytes, siz
     if (!ctx || !device_bytes) return LBN_E_ARG;
     if (bytes != ctx->shape.table_bytes)
         return ctx->fail(LBN_E_ARG, "witness table has the wrong length");
+    ctx->dev->pend = nullptr;  // replaces a deferred upload
     HIPCHK(ctx, hipSetDevice(ctx->device));
     if ((uintptr_t)device_bytes % 16) return ctx->fail(LBN_E_ARG, "device table must be 16-byte aligned");
     // a plain 16-byte-per-lane copy kernel (reads the table once, writes it once)
@@ -3251,6 +3292,10 @@ extern "C" int lbn_load_table_device(lbn_ctx *ctx, const void *device_bytes, siz
 extern "C" int lbn_copy_probe(lbn_ctx *ctx) {
     if (!ctx) return LBN_E_ARG;
     HIPCHK(ctx, hipSetDevice(ctx->device));
+    {
+        const int rc_ = table_ready(ctx);  // a deferred upload completes first
+        if (rc_ != LBN_OK) return rc_;
+    }
     // the table's bytes into the eq buffer (the same size) with the plain copy kernel; the eq
     // expansion is rebuilt by the next lbn_partials
     const size_t words = ctx->shape.table_bytes / 16;
@@ -3283,6 +3328,7 @@ static B3Root table_root(const uint8_t seed[32]) {
 
 extern "C" int lbn_generate_table(lbn_ctx *ctx, const uint8_t seed[32]) {
     if (!ctx || !seed) return LBN_E_ARG;
+    ctx->dev->pend = nullptr;  // replaces a deferred upload
     HIPCHK(ctx, hipSetDevice(ctx->device));
     const B3Root root = table_root(seed);
     const size_t blocks = ctx->shape.table_bytes / 64;
@@ -3810,11 +3856,21 @@ int lbn::dev_commit_launch(lbn_ctx *ctx) {
     const int re = (int)s.ring_elements;
     if (d->toep) {
         const int sp = d->tp_splits;
+        // a deferred upload goes in (at most) four ring-element ranges, each range's MAC after its copy
+        const int groups = d->pend ? std::min(4, sp) : 1;
         tstart(ctx, T_MAC);
-        k_toep_mac<<<TP_TILES * sp, 64 * TP_WAVES, 0, d->stream>>>(d->toep->ptr, d->table, s.column_len, re, sp,
-                                                                    d->tpart);
+        for (int g = 0; g < groups; g++) {
+            const int s0 = sp * g / groups, s1 = sp * (g + 1) / groups;
+            if (d->pend) {
+                const int rc = upload_range(ctx, g, (size_t)((int64_t)s0 * re / sp), (size_t)((int64_t)s1 * re / sp));
+                if (rc != LBN_OK) return rc;
+            }
+            k_toep_mac<<<TP_TILES * (s1 - s0), 64 * TP_WAVES, 0, d->stream>>>(d->toep->ptr, d->table, s.column_len,
+                                                                               re, sp, s0, s1 - s0, d->tpart);
+            HIPCHK(ctx, hipGetLastError());
+        }
+        d->pend = nullptr;
         tstop(ctx, T_MAC);
-        HIPCHK(ctx, hipGetLastError());
         tstart(ctx, T_FINISH);
         k_toep_finish<<<dim3(COLUMNS / 64, TP_TILES), 256, 0, d->stream>>>(d->tpart, sp, (uint64_t)ctx->q, ctx->width,
                                                                           d->cbytes);
@@ -3826,6 +3882,10 @@ int lbn::dev_commit_launch(lbn_ctx *ctx) {
     const int np = ctx->nprimes;
     const int m_per = mac_m_per(re), msplit = mac_splits(re);
     if (msplit != d->splits) return ctx->fail(LBN_E_STATE, "MAC slices not allocated for this shape");
+    {
+        const int rc = table_ready(ctx);
+        if (rc != LBN_OK) return rc;
+    }
     tstart(ctx, T_MAC);
     for (int k = 0; k < np; k++) {
         PC pc{ctx->primes[k].p, ctx->primes[k].pinv_neg};
@@ -3857,6 +3917,10 @@ extern "C" int lbn_partials(lbn_ctx *ctx, const uint64_t *r_hi, uint64_t *partia
     if (!ctx || !r_hi || !partials) return LBN_E_ARG;
     if (!ctx->table_loaded) return ctx->fail(LBN_E_STATE, "no table loaded");
     HIPCHK(ctx, hipSetDevice(ctx->device));
+    {
+        const int rc_ = table_ready(ctx);  // a deferred upload completes first
+        if (rc_ != LBN_OK) return rc_;
+    }
     DeviceState *d = ctx->dev;
     const Shape &s = ctx->shape;
     const int lp = s.log_packed, S = lp - 6, lo = std::min(S, 12), nh = S - lo;
@@ -4114,6 +4178,10 @@ extern "C" int lbn_switch_begin(lbn_ctx *ctx, const uint64_t *batch) {
     if (!ctx || !batch) return LBN_E_ARG;
     if (!ctx->partials_done) return ctx->fail(LBN_E_STATE, "lbn_partials must run first");
     HIPCHK(ctx, hipSetDevice(ctx->device));
+    {
+        const int rc_ = table_ready(ctx);  // a deferred upload completes first
+        if (rc_ != LBN_OK) return rc_;
+    }
     DeviceState *d = ctx->dev;
     ctx->lmap.assign((const F162 *)batch, (const F162 *)batch + 128);
     ctx->round = 0;
@@ -4287,6 +4355,10 @@ extern "C" int lbn_switch_fold(lbn_ctx *ctx, const uint64_t *rr) {
     const int lp = ctx->shape.log_packed;
     if (ctx->round < 0 || ctx->round >= lp) return ctx->fail(LBN_E_STATE, "no switch round pending");
     HIPCHK(ctx, hipSetDevice(ctx->device));
+    {
+        const int rc_ = table_ready(ctx);  // a deferred upload completes first
+        if (rc_ != LBN_OK) return rc_;
+    }
     DeviceState *d = ctx->dev;
     const int r = ctx->round;
     F162 rho{{rr[0], rr[1], rr[2]}};
@@ -4337,6 +4409,10 @@ extern "C" int lbn_row_eval(lbn_ctx *ctx, const uint64_t *p0, int n_p0, uint64_t
     if (n_p0 != s.row_log) return ctx->fail(LBN_E_ARG, "p0 must have log_vars - 17 coordinates");
     if (!ctx->table_loaded) return ctx->fail(LBN_E_STATE, "no table loaded");
     HIPCHK(ctx, hipSetDevice(ctx->device));
+    {
+        const int rc_ = table_ready(ctx);  // a deferred upload completes first
+        if (rc_ != LBN_OK) return rc_;
+    }
     DeviceState *d = ctx->dev;
     const int lo_bits = std::min(s.row_log, 7), hi_bits = s.row_log - lo_bits;
     std::vector<F162> lo((size_t)1 << lo_bits), hi((size_t)1 << hi_bits);
@@ -4399,6 +4475,10 @@ extern "C" int lbn_fold(lbn_ctx *ctx, const lbn_challenge *ch, int n_challenges,
     int nb = 1;
     while ((1 << nb) <= maxlen) nb++;
     HIPCHK(ctx, hipSetDevice(ctx->device));
+    {
+        const int rc_ = table_ready(ctx);  // a deferred upload completes first
+        if (rc_ != LBN_OK) return rc_;
+    }
     DeviceState *d = ctx->dev;
     memcpy(d->hbox + mb::FOFF, off.data(), off.size() * sizeof(int));
     memcpy(d->hbox + mb::FCOLS, cols.data(), cols.size() * 2);

@@ -156,6 +156,37 @@ def test_timed_proofs_equal_untimed(m, p20):
         assert t[f] > 0, (f, t)
 
 
+@pytest.mark.parametrize("k", [20, 31])
+def test_deferred_table_upload(m, k):
+    """lbn_load_table_async: the table copied in ring-element ranges inside the proof (pinned and
+    pageable sources) gives the proof of lbn_load_table; a table reader other than the proof (the
+    partials) completes the upload first."""
+    seed = bytes(range(50, 82))
+    say(f"deferred upload k={k}")
+    tab = O.table(seed, k)
+    p = m.Prover(k)
+    pin = m.PinnedBuffer(tab.size)
+    try:
+        p.set_modulus(O.DEFAULT_Q)
+        p.load_table(tab)
+        want, _ = p.prove(seed)
+        pin.array[:] = tab
+        p.load_table_async(pin)
+        got, _ = p.prove(seed)
+        assert got == want
+        p.load_table_async(np.ascontiguousarray(tab))
+        got, _ = p.prove(seed)
+        assert got == want
+        r_hi = np.arange(2 * (k - 7), dtype=np.uint64).reshape(k - 7, 2) * np.uint64(0x9E3779B97F4A7C15)
+        p.load_table(tab)
+        a = p.partials(r_hi)
+        p.load_table_async(pin)
+        assert (p.partials(r_hi) == a).all()
+    finally:
+        p.close()
+        pin.close()
+
+
 def test_switch_graph_replays(m):
     """The captured switch graph replayed by several proofs (other work and host syncs between the
     replays, different seeds and tables) gives the bytes of the launch-by-launch switch."""

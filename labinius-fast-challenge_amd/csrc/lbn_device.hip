@@ -3139,18 +3139,6 @@ __global__ __launch_bounds__(64 * chal::WAVES) void k_challenges(const ChalArg *
 // Host side of the device passes
 // =============================================================================================
 
-bool lbn::Shape::of(int log_vars, Shape &s) {
-    if (log_vars < 19 || log_vars > 31) return false;
-    s.log_vars = log_vars;
-    s.log_packed = log_vars - 7;
-    s.row_log = s.log_packed - COLUMN_LOG;
-    s.n_words = (size_t)1 << s.log_packed;
-    s.column_len = (size_t)1 << s.row_log;
-    s.ring_elements = s.column_len / 4;
-    s.table_bytes = ((size_t)1 << log_vars) / 8;
-    return true;
-}
-
 int lbn::dev_init(lbn_ctx *ctx) {
     auto *d = new DeviceState();
     ctx->dev = d;

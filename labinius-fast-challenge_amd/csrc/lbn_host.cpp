@@ -445,6 +445,20 @@ using namespace lbn;
 
 // ============================================================================= C ABI (host)
 
+// The shape of a 2^log_vars-evaluation instance (verifier/proof_verify/src/main.rs:30-48 at
+// log_vars = 31): 2^(k-7) GHASH words, 1024 columns of 2^(k-17) words, 4 words per ring element.
+bool lbn::Shape::of(int log_vars, Shape &s) {
+    if (log_vars < 19 || log_vars > 31) return false;
+    s.log_vars = log_vars;
+    s.log_packed = log_vars - 7;
+    s.row_log = s.log_packed - COLUMN_LOG;
+    s.n_words = (size_t)1 << s.log_packed;
+    s.column_len = (size_t)1 << s.row_log;
+    s.ring_elements = s.column_len / 4;
+    s.table_bytes = ((size_t)1 << log_vars) / 8;
+    return true;
+}
+
 extern "C" int lbn_create(int device, int log_vars, lbn_ctx **out) {
     if (!out) return LBN_E_ARG;
     *out = nullptr;

@@ -751,6 +751,29 @@ __device__ __forceinline__ void rho_mul_lean(const uint32_t *tab, const uint32_t
     }
 }
 
+// Two products by the same table, their lookups interleaved (two independent chains in flight).
+template <int NW>
+__device__ __forceinline__ void rho_mul_lean2(const uint32_t *tab, const uint32_t (&y0)[6], const uint32_t (&y1)[6],
+                                              uint32_t (&o0)[6], uint32_t (&o1)[6]) {
+#pragma unroll
+    for (int k = 0; k < 6; k++) o0[k] = 0, o1[k] = 0;
+#pragma unroll 1
+    for (int q = 0; q < (NW + 7) / 8; q++) {
+        const uint32_t a = q == 0 ? y0[0] : q == 1 ? y0[1] : q == 2 ? y0[2] : q == 3 ? y0[3] : q == 4 ? y0[4] : y0[5];
+        const uint32_t b = q == 0 ? y1[0] : q == 1 ? y1[1] : q == 2 ? y1[2] : q == 3 ? y1[3] : q == 4 ? y1[4] : y1[5];
+        const uint32_t *tq = tab + 6 * 16 * 8 * q;
+#pragma unroll
+        for (int n = 0; n < 8; n++) {
+            if (8 * q + n < NW) {
+                const uint32_t *e0 = tq + 6 * (n * 16 + ((a >> (4 * n)) & 15));
+                const uint32_t *e1 = tq + 6 * (n * 16 + ((b >> (4 * n)) & 15));
+#pragma unroll
+                for (int k = 0; k < 6; k++) o0[k] ^= e0[k], o1[k] ^= e1[k];
+            }
+        }
+    }
+}
+
 template <int NW>  // number of nibble windows of the input (32 for a lifted word, 41 for F162)
 __device__ __forceinline__ void rho_mul(const uint32_t *tab, const uint32_t (&y)[6], uint32_t (&out)[6]) {
 #pragma unroll
@@ -1401,85 +1424,68 @@ __global__ __launch_bounds__(256) void k_prefix_msg(uint32_t *Q, int r, const ui
 // (the records of the first six control steps). Block 0 also clears `clear` (1536 words).
 // ---------------------------------------------------------------------------------------------
 constexpr int BIND_THREADS = 512;
-__device__ __forceinline__ void f162_fold(const uint32_t *tab, const uint32_t (&a)[6], const uint32_t (&b)[6],
-                                          uint32_t (&out)[6]) {  // a + rho (a + b)
-    uint32_t d[6], m[6];
+
+// Two outputs (p^(6)_j and p^(6)_(j+h2)) by a group of G = 2^GL lanes: lane g walks, for both, the
+// subtree of the prefixes whose last GL bits are g (2^(5-GL) leaves, levels 1 .. 5-GL), the two
+// walks' table lookups interleaved; then the group folds its G values across lanes (lanes g and
+// g + G/2 by rho_(6-GL), ...); lane 0 of the group ends with the outputs.
+__device__ __forceinline__ void f162_fold2(const uint32_t *tab, const uint32_t (&a0)[6], const uint32_t (&b0)[6],
+                                           const uint32_t (&a1)[6], const uint32_t (&b1)[6], uint32_t (&o0)[6],
+                                           uint32_t (&o1)[6]) {  // a + rho (a + b), twice
+    uint32_t d0[6], d1[6], m0[6], m1[6];
 #pragma unroll
-    for (int k = 0; k < 6; k++) d[k] = a[k] ^ b[k];
-    rho_mul_lean<RHO_WINDOWS>(tab, d, m);
+    for (int k = 0; k < 6; k++) d0[k] = a0[k] ^ b0[k], d1[k] = a1[k] ^ b1[k];
+    rho_mul_lean2<RHO_WINDOWS>(tab, d0, d1, m0, m1);
 #pragma unroll
-    for (int k = 0; k < 6; k++) out[k] = a[k] ^ m[k];
+    for (int k = 0; k < 6; k++) o0[k] = a0[k] ^ m0[k], o1[k] = a1[k] ^ m1[k];
 }
 
-// One output by a group of G = 2^GL lanes: lane g walks the subtree of the prefixes whose last GL
-// bits are g (2^(5-GL) leaves, levels 1 .. 5-GL), then the group folds its G values across lanes
-// (lanes g and g + G/2 by rho_(6-GL), ...); lane 0 of the group ends with the output.
 template <int GL>
-__device__ __forceinline__ void bind_value(const uint4 *__restrict__ tw, size_t stride, const uint32_t *rt, int g,
-                                           uint32_t (&out)[6]) {
+__device__ __forceinline__ void bind_values(const uint4 *__restrict__ tw, size_t off, size_t stride,
+                                            const uint32_t *rt, int g, uint32_t (&out0)[6], uint32_t (&out1)[6]) {
     constexpr int TW = RHO_WINDOWS * 16 * 6, LEAVES = 32 >> GL;
-    uint32_t s1[6], s2[6], s3[6], s4[6], s5[6], v[6];
+    uint32_t s1[2][6], s2[2][6], s3[2][6], s4[2][6], s5[2][6], v[2][6];
 #pragma unroll 1
     for (int k = 0; k < LEAVES; k++) {
         const int sidx = g + (GL < 5 ? (int)((__builtin_bitreverse32((uint32_t)k) >> (27 + GL)) << GL) : 0);
-        const uint4 w0 = tw[(size_t)sidx * stride], w1 = tw[(size_t)(32 + sidx) * stride];
+        const uint4 a0 = tw[(size_t)sidx * stride], a1 = tw[(size_t)(32 + sidx) * stride];
+        const uint4 b0 = tw[(size_t)sidx * stride + off], b1 = tw[(size_t)(32 + sidx) * stride + off];
         {
-            uint32_t d[6] = {w0.x ^ w1.x, w0.y ^ w1.y, w0.z ^ w1.z, w0.w ^ w1.w, 0, 0}, m[6];
-            rho_mul_lean<32>(rt, d, m);
-            v[0] = w0.x ^ m[0], v[1] = w0.y ^ m[1], v[2] = w0.z ^ m[2], v[3] = w0.w ^ m[3], v[4] = m[4], v[5] = m[5];
+            uint32_t da[6] = {a0.x ^ a1.x, a0.y ^ a1.y, a0.z ^ a1.z, a0.w ^ a1.w, 0, 0};
+            uint32_t db[6] = {b0.x ^ b1.x, b0.y ^ b1.y, b0.z ^ b1.z, b0.w ^ b1.w, 0, 0}, ma[6], mb[6];
+            rho_mul_lean2<32>(rt, da, db, ma, mb);
+            v[0][0] = a0.x ^ ma[0], v[0][1] = a0.y ^ ma[1], v[0][2] = a0.z ^ ma[2], v[0][3] = a0.w ^ ma[3];
+            v[0][4] = ma[4], v[0][5] = ma[5];
+            v[1][0] = b0.x ^ mb[0], v[1][1] = b0.y ^ mb[1], v[1][2] = b0.z ^ mb[2], v[1][3] = b0.w ^ mb[3];
+            v[1][4] = mb[4], v[1][5] = mb[5];
         }
-        // the depth-first combine: level L's pending left node waits in sL
-        if (GL < 5) {
-            if (!(k & 1)) {
-#pragma unroll
-                for (int i = 0; i < 6; i++) s1[i] = v[i];
-                continue;
-            }
-            f162_fold(rt + TW, s1, v, v);
-        }
-        if (GL < 4) {
-            if (!(k & 2)) {
-#pragma unroll
-                for (int i = 0; i < 6; i++) s2[i] = v[i];
-                continue;
-            }
-            f162_fold(rt + 2 * TW, s2, v, v);
-        }
-        if (GL < 3) {
-            if (!(k & 4)) {
-#pragma unroll
-                for (int i = 0; i < 6; i++) s3[i] = v[i];
-                continue;
-            }
-            f162_fold(rt + 3 * TW, s3, v, v);
-        }
-        if (GL < 2) {
-            if (!(k & 8)) {
-#pragma unroll
-                for (int i = 0; i < 6; i++) s4[i] = v[i];
-                continue;
-            }
-            f162_fold(rt + 4 * TW, s4, v, v);
-        }
-        if (GL < 1) {
-            if (!(k & 16)) {
-#pragma unroll
-                for (int i = 0; i < 6; i++) s5[i] = v[i];
-                continue;
-            }
-            f162_fold(rt + 5 * TW, s5, v, v);
-        }
+        // the depth-first combine: level L's pending left nodes wait in sL
+#define BIND_LEVEL(L, SL)                                                        \
+    if (GL < 6 - L) {                                                            \
+        if (!(k & (1 << (L - 1)))) {                                             \
+            _Pragma("unroll") for (int i = 0; i < 6; i++) SL[0][i] = v[0][i], SL[1][i] = v[1][i]; \
+            continue;                                                            \
+        }                                                                        \
+        f162_fold2(rt + L * TW, SL[0], v[0], SL[1], v[1], v[0], v[1]);         \
+    }
+        BIND_LEVEL(1, s1)
+        BIND_LEVEL(2, s2)
+        BIND_LEVEL(3, s3)
+        BIND_LEVEL(4, s4)
+        BIND_LEVEL(5, s5)
+#undef BIND_LEVEL
     }
 #pragma unroll
     for (int lev = 0; lev < GL; lev++) {  // across the group: level 6 - GL + lev by rho_(6-GL+lev)
         const int half = (1 << GL) >> (lev + 1);
-        uint32_t o[6];
+        uint32_t o[2][6];
 #pragma unroll
-        for (int i = 0; i < 6; i++) o[i] = (uint32_t)__shfl_xor((int)v[i], half);
-        f162_fold(rt + (6 - GL + lev) * TW, v, o, v);  // lanes with bit `half` of g clear keep theirs
+        for (int i = 0; i < 6; i++)
+            o[0][i] = (uint32_t)__shfl_xor((int)v[0][i], half), o[1][i] = (uint32_t)__shfl_xor((int)v[1][i], half);
+        f162_fold2(rt + (6 - GL + lev) * TW, v[0], o[0], v[1], o[1], v[0], v[1]);  // lanes with bit `half` of g clear keep theirs
     }
 #pragma unroll
-    for (int i = 0; i < 6; i++) out[i] = v[i];
+    for (int i = 0; i < 6; i++) out0[i] = v[0][i], out1[i] = v[1][i];
 }
 
 template <int GL>
@@ -1505,8 +1511,7 @@ __global__ __launch_bounds__(BIND_THREADS) void k_switch_bind(const uint4 *__res
         uint32_t q0[6] = {0, 0, 0, 0, 0, 0}, q1[6] = {0, 0, 0, 0, 0, 0};
         uint32_t e[4] = {0, 0, 0, 0}, e2[4] = {0, 0, 0, 0};
         if (j < h2) {  // the whole group takes this branch
-            bind_value<GL>(table + j, stride, rt, g, q0);
-            bind_value<GL>(table + j + h2, stride, rt, g, q1);
+            bind_values<GL>(table + j, h2, stride, rt, g, q0, q1);
             if (g == 0) {
                 uint2 *o0 = (uint2 *)(pout + 6 * j), *o1 = (uint2 *)(pout + 6 * (j + h2));
                 o0[0] = make_uint2(q0[0], q0[1]), o0[1] = make_uint2(q0[2], q0[3]), o0[2] = make_uint2(q0[4], q0[5]);

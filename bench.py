@@ -234,16 +234,29 @@ def run_jobs(m, jobs, workers, reps=1, budget=200 << 30):
         sch.close()
 
 
+def fold_fill(k, proof):
+    """||v||^2 of the proof's fold over the checker's cap at this shape, FOLD_CAP (108) per coefficient
+    of the witness_len / 4 * N fold terms (verifier/proof_verify/src/main.rs:58-64); the fold is the
+    proof's last 4 * 648 * 2^(k-19) bytes, i32 little-endian."""
+    v = np.frombuffer(proof[len(proof) - 4 * 648 * (1 << (k - 19)):], dtype="<i4").astype(np.int64)
+    return float((v * v).sum()) / float(np.ceil(108.0 * ((1 << (k - 7)) // 4 * 648)))
+
+
 class Checker:
-    """The library's checker (lbn_verify), one context per size; call(k, seed, proof) -> accepted."""
+    """The library's checker (lbn_verify), one context per size; call(k, seed, proof) -> accepted.
+    Rejections are kept in `rejected` as (k, reason, fold fill)."""
 
     def __init__(self, m):
-        self.m, self.ctx = m, {}
+        self.m, self.ctx, self.rejected = m, {}, []
 
     def __call__(self, k, seed, proof):
         if k not in self.ctx:
             self.ctx[k] = self.m.Prover(k)
-        return self.ctx[k].verify(seed, proof) is None
+        why = self.ctx[k].verify(seed, proof)
+        if why is not None:
+            self.rejected.append({"k": k, "seed": seed.hex(), "reason": why,
+                                  "fold_fill": round(fold_fill(k, proof), 4)})
+        return why is None
 
     def close(self):
         for p in self.ctx.values():
@@ -272,7 +285,7 @@ def batched(m, workers, reps=3):
             "proofs_per_s_runs": [round(len(jobs) / t, 1) for t in t_pars],
             "sequential_proofs_per_s_runs": [round(len(jobs) / t, 1) for t in t_seqs],
             "identical_to_sequential": all(p == seq[0] for p in seq + par),
-            "verified": f"{verified}/{len(jobs)}"}
+            "verified": f"{verified}/{len(jobs)}", "rejected": check.rejected}
 
 
 def serving_requests(requests, rate, rng_seed=20261203):
@@ -324,6 +337,10 @@ def serving(m, requests, rate, workers, budget):
                         yield sch.result(h)[0]
                 ok = sum(check(kk, seed, p) for (_, kk, seed), p in zip(reqs, every_proof()))
                 out["verified"] = f"{ok}/{len(hs)}"
+                # An honest fold can cross the cap (its sum of squares has a long tail): the proof is
+                # still the reference algorithm's bytes, and the checker rejects it as the
+                # reference's does. Each rejection is listed with its reason and fill.
+                out["rejected"] = check.rejected
             return out, proofs
         finally:
             sch.close()

@@ -202,3 +202,22 @@ def test_accepts_gpu_proof_k31(m):
         assert O.verify(SEED, 31, proof) is None
     finally:
         p.close()
+
+
+def test_honest_fold_over_the_cap(m):
+    """A table whose honest fold crosses the cap (the serving workload's request 5570, k = 23): the
+    GPU proof is the oracle's, and both checkers reject it for the fold's length, as the
+    reference's checker does (verifier/proof_verify/src/main.rs:476-480)."""
+    from test_oracle import OVER_CAP_NORMSQ, OVER_CAP_SEED
+    k = 23
+    p = m.Prover(k)
+    try:
+        say("verify: k=23 fold over the cap")
+        proof = prove(p, OVER_CAP_SEED, O.DEFAULT_Q)
+        v = np.frombuffer(proof[len(proof) - 4 * 648 * (1 << (k - 19)):], dtype="<i4").astype(np.int64)
+        assert int((v * v).sum()) == OVER_CAP_NORMSQ
+        assert proof == O.prove(OVER_CAP_SEED, k)
+        assert p.verify(OVER_CAP_SEED, proof) == "the fold is longer than the cap"
+        assert O.verify(OVER_CAP_SEED, k, proof) == "the fold is longer than the cap"
+    finally:
+        p.close()

@@ -152,6 +152,25 @@ def test_fold_cap_constants():
     assert int(cap ** 0.5) == 541_787
 
 
+# A table whose honest fold crosses the cap: the sum of squares of the fold has a long tail (its
+# common part is the square of one signed sum over all 1024 x 28 challenge terms), and the
+# reference's prover writes the proof without checking it (solution/proof_generate/src/main.rs:
+# 1692-2009); its checker rejects it (verifier/proof_verify/src/main.rs:476-480 at k = 31). Found as
+# request 5570 of the serving workload (k = 23).
+OVER_CAP_SEED = bytes.fromhex("e3e8bc711e2fbf7cbec235a22aa341660960afc7ad4574cc6727196e614f878b")
+OVER_CAP_NORMSQ = 1_247_376_069
+
+
+def test_honest_fold_over_the_cap():
+    k = 23
+    p = O.prove(OVER_CAP_SEED, k)
+    v = np.frombuffer(p[len(p) - 4 * 648 * (1 << (k - 19)):], dtype="<i4").astype(np.int64)
+    cap = O.lib().orc_fold_cap(k)
+    assert cap == 108 * (1 << (k - 7)) // 4 * 648  # FOLD_CAP per coefficient of witness_len / 4 * N
+    assert int((v * v).sum()) == OVER_CAP_NORMSQ > cap
+    assert O.verify(OVER_CAP_SEED, k, p) == "the fold is longer than the cap"
+
+
 def test_proof_length_at_challenge_shape():
     # solution/proof_generate/src/main.rs:2335
     assert O.proof_len(31, O.DEFAULT_Q) == 13_958_612

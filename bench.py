@@ -158,7 +158,8 @@ def process_runs(k, cpu_seeds, gpu_seeds, want_cpu):
 def pmc_traffic(pmc_dir, log_vars, kernel="k_commit_mac", copy_kernel="k_copy_u4"):
     """HBM bytes per launch of `kernel` from rocprofv3 counter CSVs: FETCH_SIZE and WRITE_SIZE (KiB per
     dispatch), each scaled so that the copy kernel (table bytes read once and written once) reads exactly
-    its size."""
+    its size. The launch is the kernel's largest dispatch: the whole-table commitment (the bench's
+    deferred-upload proofs launch the MAC once per copied quarter of the ring elements)."""
     import csv
     import glob
 
@@ -175,8 +176,8 @@ def pmc_traffic(pmc_dir, log_vars, kernel="k_commit_mac", copy_kernel="k_copy_u4
         return None, None
     size = (1 << log_vars) // 8
     f_scale, w_scale = size / max(fetch[copy_kernel]), size / max(write[copy_kernel])
-    f = statistics.mean(fetch[kernel]) * f_scale
-    w = statistics.mean(write[kernel]) * w_scale
+    f = max(fetch[kernel]) * f_scale
+    w = max(write[kernel]) * w_scale
     return round(f + w), {"fetch_bytes": round(f), "write_bytes": round(w), "fetch_scale": round(f_scale, 4),
                           "write_scale": round(w_scale, 4)}
 

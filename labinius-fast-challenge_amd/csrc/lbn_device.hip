@@ -85,7 +85,6 @@ struct lbn::DeviceState {
     // (device, Q, shape), and its partial sums.
     std::shared_ptr<::ToepOps> toep;
     int64_t *tpart = nullptr;
-    int tp_splits = 0;
     // A host table whose upload is deferred (lbn_load_table_async): the commitment copies it in
     // ring-element ranges, each range's MAC waiting for its copy (events on the copy stream).
     const uint8_t *pend = nullptr;
@@ -2605,8 +2604,6 @@ constexpr size_t CV_TREE_LDS_MAX = 120 * 1024;
 constexpr int TP_PLANES = 5;
 constexpr int TP_TILES = 41;                 // 16-row tiles over the 648 coefficients (656 rows)
 constexpr int TP_ROWS = 16 * TP_TILES;
-constexpr int TP_WAVES = 8;                  // waves per block: 128 columns each
-constexpr int TP_NT = COLUMNS / (16 * TP_WAVES);  // column tiles of 16 per wave
 constexpr size_t TP_BLOCK_U4 = 8 * TP_PLANES * 64;  // operand uint4s per (tile, ring element)
 
 // Rot(A)[t][i] from the centred coefficients a[0..648) (t < 648, i < 512).
@@ -2661,90 +2658,160 @@ __global__ __launch_bounds__(64) void k_toep_build(const uint64_t *amat, int re,
 
 typedef int tp_v4i __attribute__((ext_vector_type(4)));
 
-// The GEMM: block (tile r, split sp) sums ring elements [m0, m1) of every column for the 16 rows
-// of tile r and the five planes; wave w takes columns 128 w .. 128 w + 127 (8 tiles of 16). The
-// blocks of one split run on one XCD (block b goes to XCD b % 8), so the witness words they all
-// read come from that XCD's L2. Partial sums: part[((sp * 5 + d) * 656 + row) * 1024 + column].
-__global__ __launch_bounds__(64 * TP_WAVES) void k_toep_mac(const uint4 *__restrict__ aop, const uint4 *__restrict__ table,
-                                                            size_t clen, int re, int splits, int sp0, int nsp,
-                                                            int64_t *part) {
-    const int nb = TP_TILES * nsp, b = blockIdx.x;  // splits sp0 .. sp0 + nsp - 1 of `splits`
-    const int full = nb / 8 * 8;
-    const int lb = b < full ? (b % 8) * (nb / 8) + b / 8 : b;  // logical block: same split, same XCD
-    const int r = lb % TP_TILES, sp = sp0 + lb / TP_TILES;
-    const int m0 = (int)((int64_t)sp * re / splits), m1 = (int)((int64_t)(sp + 1) * re / splits);
+// The GEMM: block (row group rg, column half ch, split sp) sums ring elements [m0, m1) of the split
+// for rows
+// 32 rg .. 32 rg + 31 (tiles 2 rg and 2 rg + 1; group 20 has tile 40 only) and columns
+// 512 ch .. 512 ch + 511; wave w takes 64 of those columns (4 tiles of 16). Each witness operand
+// feeds both row tiles (a 32 x 64 wave tile: 160 accumulator registers, two waves per SIMD), so
+// the bit-to-byte expansion costs 0.75 VALU instructions per product. The operand blocks go through
+// LDS in half-element stages (4 steps, both tiles: 40 KiB), double-buffered, by direct-to-LDS loads;
+// within a stage the next plane's operands are read from LDS while the current plane's 8 products
+// run, and the next step's witness operands are formed between the products. Measured at k = 31
+// (one launch, 252 blocks = one per CU): 0.70 of the cycles with the matrix unit busy against 0.43
+// for 16-row wave tiles; the clock settles near 1.6-1.85 GHz under this load (power), so the time
+// is 0.54 of the 2.4 GHz peak.
+constexpr int T2_GROUPS = (TP_TILES + 1) / 2;        // 21 row groups
+constexpr int T2_WAVES = 8;
+constexpr int T2_NT = 32 / T2_WAVES;                 // column tiles of 16 per wave
+constexpr int T2_SS = 4;                             // steps of one element per stage
+constexpr int T2_NSE = 8 / T2_SS;                    // stages per element
+constexpr int T2_STAGE_U4 = T2_SS * TP_PLANES * 64;  // uint4s of one tile's stage
+
+__device__ __forceinline__ tp_v4i toep_bexp(uint32_t x, int h) {
+    return tp_v4i{(int)((x >> h) & 0x01010101u), (int)((x >> (h + 1)) & 0x01010101u),
+                  (int)((x >> (h + 2)) & 0x01010101u), (int)((x >> (h + 3)) & 0x01010101u)};
+}
+
+template <bool TWO>
+__device__ __forceinline__ void toep2_body(const uint4 *__restrict__ aop, const uint4 *__restrict__ table, size_t clen,
+                                           int re, int rg, int ch, int sp, int m0, int m1, int64_t *part,
+                                           uint4 (*sA)[2 * T2_STAGE_U4]) {
     const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int g = l >> 4, n = l & 15;
-    tp_v4i acc[TP_PLANES][TP_NT];
+    const int r0 = 2 * rg;
+    const int c0 = 512 * ch + 16 * T2_NT * w;
+    tp_v4i acc0[TP_PLANES][T2_NT], acc1[TP_PLANES][T2_NT];
 #pragma unroll
     for (int d = 0; d < TP_PLANES; d++)
 #pragma unroll
-        for (int t = 0; t < TP_NT; t++) acc[d][t] = tp_v4i{0, 0, 0, 0};
-    // word g of element m of column 128 w + 16 t + n: table[(column * clen + 4 m + g)]
-    const uint4 *tb = table + (size_t)(128 * w + n) * clen + g;
-    // the element's operand blocks (40 KiB) are copied into LDS by the whole block, one element
-    // ahead, with direct-to-LDS loads (no registers); lane i of wave w copies uint4s k*512 + 64w + i
-    __shared__ uint4 sA[2][TP_BLOCK_U4];
-    const uint4 *ablk = aop + (size_t)r * re * TP_BLOCK_U4 + threadIdx.x;
-    // (issued as assembly: the copies write LDS behind the compiler's back, so it adds no waits
-    // before the LDS reads of the other buffer; the element loop waits for them explicitly)
-    auto dma = [&](int mm, int bb) {
+        for (int t = 0; t < T2_NT; t++) acc0[d][t] = acc1[d][t] = tp_v4i{0, 0, 0, 0};
+    // word g of element mm of column c0 + 16 t + n: dwords q T2_SS / 2 .. of stage q of the element
+    const uint4 *tbw = table + (size_t)(c0 + n) * clen + g;
+    uint32_t wc[T2_NT][T2_SS / 2], wn[T2_NT][T2_SS / 2];
+    auto wload = [&](uint32_t (*dst)[T2_SS / 2], int mm, int q) {
 #pragma unroll
-        for (int k = 0; k < TP_BLOCK_U4 / 512; k++) {
-            const uint32_t lds = __builtin_amdgcn_readfirstlane((uint32_t)(uintptr_t)&sA[bb][k * 512 + w * 64]);
-            const uint4 *src = ablk + (size_t)mm * TP_BLOCK_U4 + k * 512;
+        for (int t = 0; t < T2_NT; t++) {
+            const uint4 *wp = tbw + (size_t)16 * t * clen + 4 * (size_t)mm;
+            if (T2_SS == 8) {
+                const uint4 x = *wp;
+                dst[t][0] = x.x, dst[t][1] = x.y, dst[t][T2_SS / 2 - 2] = x.z, dst[t][T2_SS / 2 - 1] = x.w;
+            } else {
+                const uint2 x = ((const uint2 *)wp)[q];
+                dst[t][0] = x.x, dst[t][T2_SS / 2 - 1] = x.y;
+            }
+        }
+    };
+    // stage (element mm, part q) into buffer bb: flat uint4 index f = 64 T2_WAVES k + 64 w + lane
+    // over the two tiles' stages (every 64-chunk lies in one tile)
+    auto dma = [&](int mm, int q, int bb) {
+#pragma unroll
+        for (int k = 0; k < 2 * T2_STAGE_U4 / (64 * T2_WAVES); k++) {
+            const int f0 = k * 64 * T2_WAVES + 64 * w;
+            const int tile = f0 >= T2_STAGE_U4 ? 1 : 0;
+            if (!TWO && tile) continue;
+            const uint4 *src = aop + (((size_t)(r0 + tile) * re + mm) * 8 + T2_SS * q) * (TP_PLANES * 64) +
+                               (f0 - tile * T2_STAGE_U4) + l;
+            const uint32_t lds = __builtin_amdgcn_readfirstlane((uint32_t)(uintptr_t)&sA[bb][f0]);
             asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(lds), "v"(src) : "memory", "m0");
         }
     };
-    // word g of element m as two 8-byte halves: dwords 0-1 (steps 0-3) and 2-3 (steps 4-7)
-    const uint2 *tb2 = (const uint2 *)tb;
-    auto half_at = [&](int t, int mm, int hf) { return tb2[2 * ((size_t)16 * t * clen + 4 * (size_t)mm) + hf]; };
-    uint2 wxy[TP_NT], wzw[TP_NT];
+    tp_v4i bv[T2_NT], bn[T2_NT];
+    wload(wc, m0, 0);
 #pragma unroll
-    for (int t = 0; t < TP_NT; t++) wxy[t] = half_at(t, m0, 0);
-    dma(m0, 0);
-    for (int m = m0; m < m1; m++) {
-        const int buf = (m - m0) & 1;
-        const int mn = m + 1 < m1 ? m + 1 : m;
-        __builtin_amdgcn_s_waitcnt(0);  // this wave's copies of element m and its words 0-1 have landed
+    for (int t = 0; t < T2_NT; t++) bv[t] = toep_bexp(wc[t][0], 0);
+    dma(m0, 0, 0);
+    const int nst = T2_NSE * (m1 - m0);
+    for (int st = 0; st < nst; st++) {
+        const int buf = st & 1;
+        __builtin_amdgcn_s_waitcnt(0);  // this wave's copies of stage st and its witness words have landed
         __syncthreads();                // every wave's copies; every wave is done with the other buffer
-#pragma unroll
-        for (int t = 0; t < TP_NT; t++) wzw[t] = half_at(t, m, 1);  // issued before the next copies
-        __builtin_amdgcn_sched_barrier(0);
-        if (m + 1 < m1) dma(m + 1, buf ^ 1);
-        __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int s = 0; s < 8; s++) {
-            __builtin_amdgcn_sched_barrier(0);  // keep each step's reads with its products
-            tp_v4i a[TP_PLANES];
-#pragma unroll
-            for (int d = 0; d < TP_PLANES; d++) a[d] = __builtin_bit_cast(tp_v4i, sA[buf][(s * TP_PLANES + d) * 64 + l]);
-            const int h = 4 * (s & 1);
-#pragma unroll
-            for (int t = 0; t < TP_NT; t++) {
-                const uint32_t x = s < 2 ? wxy[t].x : s < 4 ? wxy[t].y : s < 6 ? wzw[t].x : wzw[t].y;
-                const tp_v4i bv = tp_v4i{(int)((x >> h) & 0x01010101u), (int)((x >> (h + 1)) & 0x01010101u),
-                                         (int)((x >> (h + 2)) & 0x01010101u), (int)((x >> (h + 3)) & 0x01010101u)};
-#pragma unroll
-                for (int d = 0; d < TP_PLANES; d++)
-                    acc[d][t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[d], bv, acc[d][t], 0, 0, 0);
-            }
-            if (s == 3) {  // dwords 0-1 of the next element, after their last use here
-#pragma unroll
-                for (int t = 0; t < TP_NT; t++) wxy[t] = half_at(t, mn, 0);
-            }
+        const bool more = st + 1 < nst;
+        if (more) {
+            wload(wn, m0 + (st + 1) / T2_NSE, (st + 1) % T2_NSE);
+            dma(m0 + (st + 1) / T2_NSE, (st + 1) % T2_NSE, buf ^ 1);
         }
+        __builtin_amdgcn_sched_barrier(0);
+        // plane d of step s: tile 0 at sA[buf][(5 s + d) * 64 + l], tile 1 T2_STAGE_U4 further
+        const uint4 *sa = &sA[buf][l];
+        tp_v4i a0 = __builtin_bit_cast(tp_v4i, sa[0]), a1;
+        if (TWO) a1 = __builtin_bit_cast(tp_v4i, sa[T2_STAGE_U4]);
+#pragma unroll
+        for (int s = 0; s < T2_SS; s++) {
+#pragma unroll
+            for (int d = 0; d < TP_PLANES; d++) {
+                __builtin_amdgcn_sched_barrier(0);
+                const int j = s * TP_PLANES + d + 1;  // the next (step, plane) of the stage
+                tp_v4i p0, p1;
+                if (j < T2_SS * TP_PLANES) {
+                    p0 = __builtin_bit_cast(tp_v4i, sa[j * 64]);
+                    if (TWO) p1 = __builtin_bit_cast(tp_v4i, sa[T2_STAGE_U4 + j * 64]);
+                }
+#pragma unroll
+                for (int t = 0; t < T2_NT; t++) {
+                    acc0[d][t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a0, bv[t], acc0[d][t], 0, 0, 0);
+                    if (TWO) acc1[d][t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a1, bv[t], acc1[d][t], 0, 0, 0);
+                }
+                // the witness operands of the next step for column tiles t = d (mod 5): step s + 1
+                // of this stage (dword (s + 1) >> 1, bits from 4 ((s + 1) & 1)), or step 0 of the next
+#pragma unroll
+                for (int t = d; t < T2_NT; t += TP_PLANES) {
+                    if (s + 1 < T2_SS) bn[t] = toep_bexp(wc[t][(s + 1) >> 1], 4 * ((s + 1) & 1));
+                    else if (more) bn[t] = toep_bexp(wn[t][0], 0);
+                }
+                if (j < T2_SS * TP_PLANES) {
+                    a0 = p0;
+                    if (TWO) a1 = p1;
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < T2_NT; t++) bv[t] = bn[t];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int t = 0; t < T2_NT; t++)
+#pragma unroll
+            for (int k = 0; k < T2_SS / 2; k++) wc[t][k] = wn[t][k];
     }
     // D layout: lane l holds rows 4 g + i (i < 4) of the tile, column n of each column tile
 #pragma unroll
-    for (int t = 0; t < TP_NT; t++)
+    for (int t = 0; t < T2_NT; t++)
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            int64_t v = 0;
+            int64_t v0 = 0, v1 = 0;
 #pragma unroll
-            for (int d = 0; d < TP_PLANES; d++) v += (int64_t)acc[d][t][i] * ((int64_t)1 << (8 * d));
-            part[((size_t)sp * TP_ROWS + 16 * r + 4 * g + i) * COLUMNS + 128 * w + 16 * t + n] = v;
+            for (int d = 0; d < TP_PLANES; d++) {
+                v0 += (int64_t)acc0[d][t][i] * ((int64_t)1 << (8 * d));
+                v1 += (int64_t)acc1[d][t][i] * ((int64_t)1 << (8 * d));
+            }
+            const size_t col = (size_t)c0 + 16 * t + n;
+            part[((size_t)sp * TP_ROWS + 16 * r0 + 4 * g + i) * COLUMNS + col] = v0;
+            if (TWO) part[((size_t)sp * TP_ROWS + 16 * (r0 + 1) + 4 * g + i) * COLUMNS + col] = v1;
         }
+}
+
+// Blocks: 42 per split (21 row groups x 2 column halves), consecutive logical blocks on one XCD so
+// the two halves of a row group read the operand stream through the same L2.
+__global__ __launch_bounds__(64 * T2_WAVES) void k_toep_mac(const uint4 *__restrict__ aop,
+                                                             const uint4 *__restrict__ table, size_t clen, int re,
+                                                             int splits, int sp0, int nsp, int64_t *part) {
+    const int nb = 2 * T2_GROUPS * nsp, b = blockIdx.x;
+    const int full = nb / 8 * 8;
+    const int lb = b < full ? (b % 8) * (nb / 8) + b / 8 : b;
+    const int ch = lb & 1, rg = (lb >> 1) % T2_GROUPS, sp = sp0 + (lb >> 1) / T2_GROUPS;
+    const int m0 = (int)((int64_t)sp * re / splits), m1 = (int)((int64_t)(sp + 1) * re / splits);
+    __shared__ uint4 sA[2][2 * T2_STAGE_U4];
+    if (2 * rg + 1 < TP_TILES) toep2_body<true>(aop, table, clen, re, rg, ch, sp, m0, m1, part, sA);
+    else toep2_body<false>(aop, table, clen, re, rg, ch, sp, m0, m1, part, sA);
 }
 
 // Sums the splits (|C| <= 2^20 Q < 2^59), reduces mod Q and packs `width` bytes per coefficient at
@@ -3439,10 +3506,11 @@ static size_t commit_buffer_bytes(int np, int width, size_t re) {
 // The matrix-core commitment serves moduli below 2^39: a centred entry (|e| <= Q/2 < 2^38) is five
 // balanced signed bytes and |C| <= 2^20 Q fits an int64. Larger moduli use the CRT transform.
 static bool toep_modulus(u128 q) { return q < ((u128)1 << 39); }
-// Ring-element splits of the GEMM: 41 x 31 = 1271 blocks (4.96 rounds of 256) at 4096 elements.
-static int toep_splits(size_t re) { return (int)std::min<size_t>(re, 31); }
+// Ring-element splits of the GEMM (42 blocks each): 6 with the table resident (252 blocks, one
+// round over 256 CUs), 24 with a deferred upload (four launches of 6 splits, one per copied range).
+static int toep_splits(size_t re, bool deferred) { return (int)std::min<size_t>(re, deferred ? 24 : 6); }
 static size_t toep_ops_bytes(size_t re) { return (size_t)TP_TILES * re * TP_BLOCK_U4 * 16; }
-static size_t toep_part_bytes(size_t re) { return (size_t)toep_splits(re) * TP_ROWS * COLUMNS * 8; }
+static size_t toep_part_bytes(size_t re) { return (size_t)toep_splits(re, true) * TP_ROWS * COLUMNS * 8; }
 
 // The GEMM's matrix operands depend only on (device, Q, ring elements): built once, shared by the
 // contexts that use them (weak entries: freed with the last context holding them).
@@ -3599,7 +3667,6 @@ static int setup_toep(lbn_ctx *ctx, u128 q) {
     const size_t re = ctx->shape.ring_elements;
     if (!d->tpart) {
         HIPCHK(ctx, hipMalloc(&d->tpart, toep_part_bytes(re)));
-        d->tp_splits = toep_splits(re);
     }
     const auto key = std::make_tuple(ctx->device, q, re);
     std::lock_guard<std::mutex> lk(toep_mu);
@@ -3848,7 +3915,7 @@ int lbn::dev_commit_launch(lbn_ctx *ctx) {
     DeviceState *d = ctx->dev;
     const int re = (int)s.ring_elements;
     if (d->toep) {
-        const int sp = d->tp_splits;
+        const int sp = toep_splits((size_t)re, d->pend != nullptr);
         // a deferred upload goes in (at most) four ring-element ranges, each range's MAC after its copy
         const int groups = d->pend ? std::min(4, sp) : 1;
         tstart(ctx, T_MAC);
@@ -3858,8 +3925,8 @@ int lbn::dev_commit_launch(lbn_ctx *ctx) {
                 const int rc = upload_range(ctx, g, (size_t)((int64_t)s0 * re / sp), (size_t)((int64_t)s1 * re / sp));
                 if (rc != LBN_OK) return rc;
             }
-            k_toep_mac<<<TP_TILES * (s1 - s0), 64 * TP_WAVES, 0, d->stream>>>(d->toep->ptr, d->table, s.column_len,
-                                                                               re, sp, s0, s1 - s0, d->tpart);
+            k_toep_mac<<<2 * T2_GROUPS * (s1 - s0), 64 * T2_WAVES, 0, d->stream>>>(
+                d->toep->ptr, d->table, s.column_len, re, sp, s0, s1 - s0, d->tpart);
             HIPCHK(ctx, hipGetLastError());
         }
         d->pend = nullptr;

@@ -2604,78 +2604,84 @@ constexpr size_t CV_TREE_LDS_MAX = 120 * 1024;
 constexpr int TP_PLANES = 5;
 constexpr int TP_TILES = 41;                 // 16-row tiles over the 648 coefficients (656 rows)
 constexpr int TP_ROWS = 16 * TP_TILES;
-constexpr size_t TP_BLOCK_U4 = 8 * TP_PLANES * 64;  // operand uint4s per (tile, ring element)
 
-// Rot(A)[t][i] from the centred coefficients a[0..648) (t < 648, i < 512).
-__device__ __forceinline__ int64_t toep_entry(const int64_t *a, int t, int i) {
-    int64_t v = 0;
-    if (t >= i) v += a[t - i];
-    if (t >= 324 && i > t - 324) v += a[t + 324 - i];
-    if (t < 324 && i > t) v -= a[t + 648 - i];
-    if (t < 188 && i > t + 324) v -= a[t + 972 - i];
-    return v;
-}
+// Rot(A)[t][i] depends on t and x = t - i only through one of three sequences (the regime of the
+// row): for t >= 324, s0(x) = [x >= 0] a_x + [x < 324] a_(x+324); for 188 <= t < 324,
+// s1(x) = [x >= 0] a_x - [x < 0] a_(x+648); for t < 188, s2(x) = s1(x) - [x < -324] a_(x+972)
+// (a_i = 0 outside 0 <= i < 648; rows 648..671 of the padded tiles read values nobody keeps).
+// The K order above puts, in operand register e' of step s = 2q + h' of lane (row t, group g), the
+// bytes j' of coefficients i = 128 q + 16 h' + 4 e' + 32 j' + g: the sequence at
+// k - 32 j' with k = t - g - 128 q - 16 h' - 4 e'. So one dword per (regime, plane, k),
+//   U[k] = bytes (s(k), s(k - 32), s(k - 64), s(k - 96)) of the plane's digits,
+// is one operand register, and a lane's four registers of a step are U at k, k - 4, k - 8, k - 12.
+// Stored per element as [regime][k mod 4][272][plane] dwords for k in [-415, 672], each phase in
+// descending k (j = 0 holds the largest k), so a lane reads upward from its base, the five planes of
+// one k side by side.
+constexpr int TW_K0 = -415;                                    // smallest k
+constexpr int TW_PHASE = 272;                                  // dwords per phase array
+constexpr int TW_X0 = -511, TW_XN = 1184;                      // sequence domain [-511, 672]
+constexpr size_t TW_ELEM_U32 = 3 * TP_PLANES * 4 * TW_PHASE;   // 16,320 dwords per ring element
+constexpr int TW_WIN = 112;                                    // a 32-row group's window per phase
+__host__ __device__ constexpr int toep_regime(int t) { return t >= 324 ? 0 : t >= 188 ? 1 : 2; }
 
-// The GEMM's matrix operands (set-up): block (tile r, element m), lane l; writes the element's
-// 8 x 5 operand blocks of 64 lanes x 16 bytes at out[((r * re + m) * 8 + s) * 5 + d][l].
-__global__ __launch_bounds__(64) void k_toep_build(const uint64_t *amat, int re, uint64_t q, uint4 *out) {
+// The operand tables (set-up): block m builds element m's 3 x 4 x 272 x 5 dwords.
+__global__ __launch_bounds__(256) void k_toep_build(const uint64_t *amat, int re, uint64_t q, uint32_t *out) {
     __shared__ int64_t a[648];
-    const int r = blockIdx.x, m = blockIdx.y, l = threadIdx.x;
+    __shared__ int8_t dig[3][TP_PLANES][TW_XN];
+    const int m = blockIdx.x, tid = threadIdx.x;
     const uint64_t half = q / 2;
-    for (int i = l; i < 648; i += 64) {
+    const int64_t hq = (int64_t)half, qq = (int64_t)q;
+    for (int i = tid; i < 648; i += 256) {
         const uint64_t x = amat[((size_t)m * 648 + i) * 2];  // < q < 2^39: the high word is zero
-        a[i] = x > half ? (int64_t)x - (int64_t)q : (int64_t)x;
+        a[i] = x > half ? (int64_t)x - qq : (int64_t)x;
     }
     __syncthreads();
-    const int t = 16 * r + (l & 15), g = l >> 4;
-    const int64_t hq = (int64_t)half;
-    for (int s = 0; s < 8; s++) {
-        uint32_t dig[TP_PLANES][4];
+    auto A = [&](int i) -> int64_t { return i >= 0 && i < 648 ? a[i] : 0; };
+    for (int idx = tid; idx < 3 * TW_XN; idx += 256) {
+        const int r = idx / TW_XN, x = idx % TW_XN + TW_X0;
+        int64_t v = r == 0 ? (x >= 0 ? A(x) : 0) + (x < 324 ? A(x + 324) : 0) : (x >= 0 ? A(x) : -A(x + 648));
+        if (r == 2 && x < -324) v -= A(x + 972);
+        while (v > hq) v -= qq;  // |v| <= q before, <= q / 2 after
+        while (v < -hq) v += qq;
 #pragma unroll
-        for (int d = 0; d < TP_PLANES; d++)
+        for (int d = 0; d < TP_PLANES; d++) {
+            const int64_t dd = ((v + 128) & 255) - 128;  // balanced digit in [-128, 127]
+            v = (v - dd) >> 8;
+            dig[r][d][x - TW_X0] = (int8_t)dd;
+        }
+    }
+    __syncthreads();
+    for (int idx = tid; idx < (int)TW_ELEM_U32; idx += 256) {
+        const int d = idx % TP_PLANES, j = (idx / TP_PLANES) % TW_PHASE, rest = idx / (TP_PLANES * TW_PHASE);
+        const int ph = rest % 4, r = rest / 4;
+        const int k = TW_K0 + ph + 4 * (TW_PHASE - 1 - j);
+        uint32_t w = 0;
 #pragma unroll
-            for (int e = 0; e < 4; e++) dig[d][e] = 0;
-#pragma unroll
-        for (int e = 0; e < 4; e++)
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const int p = 32 * (s >> 1) + 8 * j + 4 * (s & 1) + e;
-                int64_t v = t < 648 ? toep_entry(a, t, 4 * p + g) : 0;
-                if (v > hq) v -= (int64_t)q;  // |v| <= q before, <= q / 2 after
-                else if (v < -hq) v += (int64_t)q;
-#pragma unroll
-                for (int d = 0; d < TP_PLANES; d++) {
-                    const int64_t dd = ((v + 128) & 255) - 128;  // balanced digit in [-128, 127]
-                    v = (v - dd) >> 8;
-                    dig[d][e] |= (uint32_t)(dd & 255) << (8 * j);
-                }
-            }
-        uint4 *o = out + (((size_t)r * re + m) * 8 + s) * TP_PLANES * 64 + l;
-#pragma unroll
-        for (int d = 0; d < TP_PLANES; d++) o[d * 64] = make_uint4(dig[d][0], dig[d][1], dig[d][2], dig[d][3]);
+        for (int jj = 0; jj < 4; jj++) w |= (uint32_t)(uint8_t)dig[r][d][k - 32 * jj - TW_X0] << (8 * jj);
+        out[(size_t)m * TW_ELEM_U32 + idx] = w;
     }
 }
 
 typedef int tp_v4i __attribute__((ext_vector_type(4)));
 
 // The GEMM: block (row group rg, column half ch, split sp) sums ring elements [m0, m1) of the split
-// for rows
-// 32 rg .. 32 rg + 31 (tiles 2 rg and 2 rg + 1; group 20 has tile 40 only) and columns
+// for rows 32 rg .. 32 rg + 31 (tiles 2 rg and 2 rg + 1; group 20 has tile 40 only) and columns
 // 512 ch .. 512 ch + 511; wave w takes 64 of those columns (4 tiles of 16). Each witness operand
 // feeds both row tiles (a 32 x 64 wave tile: 160 accumulator registers, two waves per SIMD), so
-// the bit-to-byte expansion costs 0.75 VALU instructions per product. The operand blocks go through
-// LDS in half-element stages (4 steps, both tiles: 40 KiB), double-buffered, by direct-to-LDS loads;
-// within a stage the next plane's operands are read from LDS while the current plane's 8 products
-// run, and the next step's witness operands are formed between the products. Measured at k = 31
-// (one launch, 252 blocks = one per CU): 0.70 of the cycles with the matrix unit busy against 0.43
-// for 16-row wave tiles; the clock settles near 1.6-1.85 GHz under this load (power), so the time
-// is 0.54 of the 2.4 GHz peak.
-constexpr int T2_GROUPS = (TP_TILES + 1) / 2;        // 21 row groups
+// the bit-to-byte expansion costs 0.75 VALU instructions per product. Per ring element the block
+// copies the window of the operand tables its rows read — k in [32 rg - 415, 32 rg + 32): 112 x 5
+// dwords per (regime slot, phase), 8.75 KiB per regime — into LDS one element ahead by
+// direct-to-LDS loads; each operand register is then one LDS dword at a constant offset from one of
+// the lane's two bases (its row's regime slot, phase (t - g + 415) mod 4 and position; steps 0-3
+// and 4-7), two ds_read2_b32 per operand, the next plane's registers read while the current
+// plane's 8 products run.
+constexpr int T2_GROUPS = (TP_TILES + 1) / 2;   // 21 row groups
 constexpr int T2_WAVES = 8;
-constexpr int T2_NT = 32 / T2_WAVES;                 // column tiles of 16 per wave
-constexpr int T2_SS = 4;                             // steps of one element per stage
-constexpr int T2_NSE = 8 / T2_SS;                    // stages per element
-constexpr int T2_STAGE_U4 = T2_SS * TP_PLANES * 64;  // uint4s of one tile's stage
+constexpr int T2_NT = 4;                        // column tiles of 16 per wave
+constexpr int T2_RUN = TW_WIN * TP_PLANES;      // dwords of one (slot, phase) window: 560
+constexpr int T2_PS = 568;                      // LDS stride of a phase: phases 24 banks apart
+constexpr int T2_SLOT = 4 * T2_PS;              // dwords of one regime slot
+constexpr int T2_BUF = 2 * T2_SLOT;             // dwords of one element's window (two regime slots)
 
 __device__ __forceinline__ tp_v4i toep_bexp(uint32_t x, int h) {
     return tp_v4i{(int)((x >> h) & 0x01010101u), (int)((x >> (h + 1)) & 0x01010101u),
@@ -2683,104 +2689,117 @@ __device__ __forceinline__ tp_v4i toep_bexp(uint32_t x, int h) {
 }
 
 template <bool TWO>
-__device__ __forceinline__ void toep2_body(const uint4 *__restrict__ aop, const uint4 *__restrict__ table, size_t clen,
-                                           int re, int rg, int ch, int sp, int m0, int m1, int64_t *part,
-                                           uint4 (*sA)[2 * T2_STAGE_U4]) {
+__device__ __forceinline__ void toep2_body(const uint32_t *__restrict__ ops, const uint4 *__restrict__ table,
+                                           size_t clen, int rg, int ch, int sp, int m0, int m1, int64_t *part,
+                                           uint32_t *sU) {
     const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int g = l >> 4, n = l & 15;
     const int r0 = 2 * rg;
     const int c0 = 512 * ch + 16 * T2_NT * w;
+    // regime slots: 0 = the regime of row 32 rg, 1 = that of row 32 rg + 31 (the same for 19 groups)
+    const int reg0 = toep_regime(32 * rg), reg1 = toep_regime(32 * rg + 31);
+    // lane base (dwords within a buffer) of row tile j: row t = 32 rg + 16 j + n; the operand of step s,
+    // plane d is at base + 5 (32 (s >> 1) + 4 (s & 1) + e') + d, e' = 0..3
+    auto lane_base = [&](int j) {
+        const int t = 32 * rg + 16 * j + n, tau = 16 * j + n - g + 415;
+        return (toep_regime(t) == reg0 ? 0 : T2_SLOT) + (tau & 3) * T2_PS + TP_PLANES * (TW_WIN - 1 - (tau >> 2));
+    };
+    const int lb0 = lane_base(0), lb1 = lane_base(1);
     tp_v4i acc0[TP_PLANES][T2_NT], acc1[TP_PLANES][T2_NT];
 #pragma unroll
     for (int d = 0; d < TP_PLANES; d++)
 #pragma unroll
         for (int t = 0; t < T2_NT; t++) acc0[d][t] = acc1[d][t] = tp_v4i{0, 0, 0, 0};
-    // word g of element mm of column c0 + 16 t + n: dwords q T2_SS / 2 .. of stage q of the element
-    const uint4 *tbw = table + (size_t)(c0 + n) * clen + g;
-    uint32_t wc[T2_NT][T2_SS / 2], wn[T2_NT][T2_SS / 2];
-    auto wload = [&](uint32_t (*dst)[T2_SS / 2], int mm, int q) {
-#pragma unroll
-        for (int t = 0; t < T2_NT; t++) {
-            const uint4 *wp = tbw + (size_t)16 * t * clen + 4 * (size_t)mm;
-            if (T2_SS == 8) {
-                const uint4 x = *wp;
-                dst[t][0] = x.x, dst[t][1] = x.y, dst[t][T2_SS / 2 - 2] = x.z, dst[t][T2_SS / 2 - 1] = x.w;
-            } else {
-                const uint2 x = ((const uint2 *)wp)[q];
-                dst[t][0] = x.x, dst[t][T2_SS / 2 - 1] = x.y;
-            }
-        }
-    };
-    // stage (element mm, part q) into buffer bb: flat uint4 index f = 64 T2_WAVES k + 64 w + lane
-    // over the two tiles' stages (every 64-chunk lies in one tile)
-    auto dma = [&](int mm, int q, int bb) {
-#pragma unroll
-        for (int k = 0; k < 2 * T2_STAGE_U4 / (64 * T2_WAVES); k++) {
-            const int f0 = k * 64 * T2_WAVES + 64 * w;
-            const int tile = f0 >= T2_STAGE_U4 ? 1 : 0;
-            if (!TWO && tile) continue;
-            const uint4 *src = aop + (((size_t)(r0 + tile) * re + mm) * 8 + T2_SS * q) * (TP_PLANES * 64) +
-                               (f0 - tile * T2_STAGE_U4) + l;
-            const uint32_t lds = __builtin_amdgcn_readfirstlane((uint32_t)(uintptr_t)&sA[bb][f0]);
-            asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(lds), "v"(src) : "memory", "m0");
-        }
-    };
-    tp_v4i bv[T2_NT], bn[T2_NT];
-    wload(wc, m0, 0);
-#pragma unroll
-    for (int t = 0; t < T2_NT; t++) bv[t] = toep_bexp(wc[t][0], 0);
-    dma(m0, 0, 0);
-    const int nst = T2_NSE * (m1 - m0);
-    for (int st = 0; st < nst; st++) {
-        const int buf = st & 1;
-        __builtin_amdgcn_s_waitcnt(0);  // this wave's copies of stage st and its witness words have landed
-        __syncthreads();                // every wave's copies; every wave is done with the other buffer
-        const bool more = st + 1 < nst;
-        if (more) {
-            wload(wn, m0 + (st + 1) / T2_NSE, (st + 1) % T2_NSE);
-            dma(m0 + (st + 1) / T2_NSE, (st + 1) % T2_NSE, buf ^ 1);
-        }
-        __builtin_amdgcn_sched_barrier(0);
-        // plane d of step s: tile 0 at sA[buf][(5 s + d) * 64 + l], tile 1 T2_STAGE_U4 further
-        const uint4 *sa = &sA[buf][l];
-        tp_v4i a0 = __builtin_bit_cast(tp_v4i, sa[0]), a1;
-        if (TWO) a1 = __builtin_bit_cast(tp_v4i, sa[T2_STAGE_U4]);
-#pragma unroll
-        for (int s = 0; s < T2_SS; s++) {
-#pragma unroll
-            for (int d = 0; d < TP_PLANES; d++) {
-                __builtin_amdgcn_sched_barrier(0);
-                const int j = s * TP_PLANES + d + 1;  // the next (step, plane) of the stage
-                tp_v4i p0, p1;
-                if (j < T2_SS * TP_PLANES) {
-                    p0 = __builtin_bit_cast(tp_v4i, sa[j * 64]);
-                    if (TWO) p1 = __builtin_bit_cast(tp_v4i, sa[T2_STAGE_U4 + j * 64]);
-                }
-#pragma unroll
-                for (int t = 0; t < T2_NT; t++) {
-                    acc0[d][t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a0, bv[t], acc0[d][t], 0, 0, 0);
-                    if (TWO) acc1[d][t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a1, bv[t], acc1[d][t], 0, 0, 0);
-                }
-                // the witness operands of the next step for column tiles t = d (mod 5): step s + 1
-                // of this stage (dword (s + 1) >> 1, bits from 4 ((s + 1) & 1)), or step 0 of the next
-#pragma unroll
-                for (int t = d; t < T2_NT; t += TP_PLANES) {
-                    if (s + 1 < T2_SS) bn[t] = toep_bexp(wc[t][(s + 1) >> 1], 4 * ((s + 1) & 1));
-                    else if (more) bn[t] = toep_bexp(wn[t][0], 0);
-                }
-                if (j < T2_SS * TP_PLANES) {
-                    a0 = p0;
-                    if (TWO) a1 = p1;
-                }
-            }
-#pragma unroll
-            for (int t = 0; t < T2_NT; t++) bv[t] = bn[t];
-        }
-        __builtin_amdgcn_sched_barrier(0);
+    // dword q of word g of element mm of column c0 + 16 t + n (steps 2q, 2q + 1), addressed by a 32-bit
+    // byte offset from the table (256 MiB at most)
+    const uint32_t wbase = (uint32_t)(((size_t)(c0 + n) * clen + g) * 16);
+    const uint32_t wtile = (uint32_t)(16 * clen * 16);
+    auto wload = [&](uint32_t *dst, int mm, int q) {
 #pragma unroll
         for (int t = 0; t < T2_NT; t++)
+            dst[t] = *(const uint32_t *)((const char *)table + (wbase + (uint32_t)t * wtile + (uint32_t)mm * 64u + 4u * q));
+    };
+    // element mm's window into buffer bb: run (slot, phase) = 560 dwords from the table of the slot's
+    // regime at j = 160 - 8 rg (k descending), 140 x 16 bytes in three loads of 64 lanes; wave w
+    // copies pieces w + 8 i of the (runs x 3) pieces
+    const int npieces = (reg0 == reg1 ? 4 : 8) * 3;
+    auto dma = [&](int mm, int bb) {
 #pragma unroll
-            for (int k = 0; k < T2_SS / 2; k++) wc[t][k] = wn[t][k];
+        for (int i = 0; i < 3; i++) {
+            const int pi = w + 8 * i;
+            const int run = pi / 3, piece = pi % 3;
+            const int lane16 = 64 * piece + l;  // 16-byte index within the run
+            if (pi < npieces && lane16 < T2_RUN / 4) {
+                const int sl = run >> 2, ph = run & 3;
+                const uint32_t *src = ops + (size_t)mm * TW_ELEM_U32 +
+                                      ((size_t)((sl ? reg1 : reg0) * 4 + ph) * TW_PHASE + (160 - 8 * rg)) * TP_PLANES +
+                                      4 * lane16;
+                const uint32_t lds = __builtin_amdgcn_readfirstlane(
+                    (uint32_t)(uintptr_t)(sU + bb * T2_BUF + sl * T2_SLOT + ph * T2_PS + 256 * piece));
+                asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(lds), "v"(src) : "memory", "m0");
+            }
+        }
+    };
+    // operand registers of (step s, plane d) from the lane base `p` of a row tile
+    auto aop = [&](const uint32_t *p, int s, int d) {
+        const int o = TP_PLANES * (32 * (s >> 1) + 4 * (s & 1)) + d;
+        return tp_v4i{(int)p[o], (int)p[o + TP_PLANES], (int)p[o + 2 * TP_PLANES], (int)p[o + 3 * TP_PLANES]};
+    };
+    uint32_t wc[T2_NT], wn[T2_NT];
+    tp_v4i bv[T2_NT];
+    wload(wc, m0, 0);
+#pragma unroll
+    for (int t = 0; t < T2_NT; t++) bv[t] = toep_bexp(wc[t], 0);
+    dma(m0, 0);
+    for (int m = m0; m < m1; m++) {
+        const int bb = (m - m0) & 1;
+        __builtin_amdgcn_s_waitcnt(0);  // this wave's copies of element m and its witness words have landed
+        __syncthreads();                // every wave's copies; every wave is done with the other buffer
+        if (m + 1 < m1) dma(m + 1, bb ^ 1);
+        __builtin_amdgcn_sched_barrier(0);
+        // steps 0-3 read from the first bases, steps 4-7 from bases 5 x 64 dwords further
+        const uint32_t *pa0 = sU + bb * T2_BUF + lb0, *pa1 = sU + bb * T2_BUF + lb1;
+        tp_v4i a0 = aop(pa0, 0, 0), a1;
+        if (TWO) a1 = aop(pa1, 0, 0);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            // the witness dword of the next step pair: (m, q + 1), or (m + 1, 0) (after the split's last
+            // element a word of that element again, loaded and never used: no predicate in the loop)
+            wload(wn, q < 3 ? m : min(m + 1, m1 - 1), (q + 1) & 3);
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int s = 2 * q + h;
+#pragma unroll
+                for (int d = 0; d < TP_PLANES; d++) {
+                    __builtin_amdgcn_sched_barrier(0);
+                    const int jn = s * TP_PLANES + d + 1;  // the next (step, plane) of the element
+                    tp_v4i p0, p1;
+                    if (jn < 8 * TP_PLANES) {
+                        const int sn = jn / TP_PLANES, dn = jn % TP_PLANES;
+                        const int hoff = sn >= 4 ? TP_PLANES * 64 : 0;
+                        p0 = aop(pa0 + hoff, sn & 3, dn);
+                        if (TWO) p1 = aop(pa1 + hoff, sn & 3, dn);
+                    }
+#pragma unroll
+                    for (int t = 0; t < T2_NT; t++) {
+                        acc0[d][t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a0, bv[t], acc0[d][t], 0, 0, 0);
+                        if (TWO) acc1[d][t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a1, bv[t], acc1[d][t], 0, 0, 0);
+                        // after the last plane's products of column tile t: its witness operands of the
+                        // next step, bits from 4 of this pair's dword, or bits from 0 of the next pair's
+                        if (d == TP_PLANES - 1) {
+                            bv[t] = h == 0 ? toep_bexp(wc[t], 4) : toep_bexp(wn[t], 0);
+                        }
+                    }
+                    if (jn < 8 * TP_PLANES) {
+                        a0 = p0;
+                        if (TWO) a1 = p1;
+                    }
+                }
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int t = 0; t < T2_NT; t++) wc[t] = wn[t];
+        }
     }
     // D layout: lane l holds rows 4 g + i (i < 4) of the tile, column n of each column tile
 #pragma unroll
@@ -2800,18 +2819,18 @@ __device__ __forceinline__ void toep2_body(const uint4 *__restrict__ aop, const 
 }
 
 // Blocks: 42 per split (21 row groups x 2 column halves), consecutive logical blocks on one XCD so
-// the two halves of a row group read the operand stream through the same L2.
-__global__ __launch_bounds__(64 * T2_WAVES) void k_toep_mac(const uint4 *__restrict__ aop,
-                                                             const uint4 *__restrict__ table, size_t clen, int re,
-                                                             int splits, int sp0, int nsp, int64_t *part) {
+// the two halves of a row group read the operand window through the same L2.
+__global__ __launch_bounds__(64 * T2_WAVES) void k_toep_mac(const uint32_t *__restrict__ ops,
+                                                            const uint4 *__restrict__ table, size_t clen, int re,
+                                                            int splits, int sp0, int nsp, int64_t *part) {
     const int nb = 2 * T2_GROUPS * nsp, b = blockIdx.x;
     const int full = nb / 8 * 8;
     const int lb = b < full ? (b % 8) * (nb / 8) + b / 8 : b;
     const int ch = lb & 1, rg = (lb >> 1) % T2_GROUPS, sp = sp0 + (lb >> 1) / T2_GROUPS;
     const int m0 = (int)((int64_t)sp * re / splits), m1 = (int)((int64_t)(sp + 1) * re / splits);
-    __shared__ uint4 sA[2][2 * T2_STAGE_U4];
-    if (2 * rg + 1 < TP_TILES) toep2_body<true>(aop, table, clen, re, rg, ch, sp, m0, m1, part, sA);
-    else toep2_body<false>(aop, table, clen, re, rg, ch, sp, m0, m1, part, sA);
+    __shared__ __align__(16) uint32_t sU[2 * T2_BUF];
+    if (2 * rg + 1 < TP_TILES) toep2_body<true>(ops, table, clen, rg, ch, sp, m0, m1, part, sU);
+    else toep2_body<false>(ops, table, clen, rg, ch, sp, m0, m1, part, sU);
 }
 
 // Sums the splits (|C| <= 2^20 Q < 2^59), reduces mod Q and packs `width` bytes per coefficient at
@@ -3509,14 +3528,14 @@ static bool toep_modulus(u128 q) { return q < ((u128)1 << 39); }
 // Ring-element splits of the GEMM (42 blocks each): 6 with the table resident (252 blocks, one
 // round over 256 CUs), 24 with a deferred upload (four launches of 6 splits, one per copied range).
 static int toep_splits(size_t re, bool deferred) { return (int)std::min<size_t>(re, deferred ? 24 : 6); }
-static size_t toep_ops_bytes(size_t re) { return (size_t)TP_TILES * re * TP_BLOCK_U4 * 16; }
+static size_t toep_ops_bytes(size_t re) { return re * TW_ELEM_U32 * 4; }
 static size_t toep_part_bytes(size_t re) { return (size_t)toep_splits(re, true) * TP_ROWS * COLUMNS * 8; }
 
 // The GEMM's matrix operands depend only on (device, Q, ring elements): built once, shared by the
 // contexts that use them (weak entries: freed with the last context holding them).
 struct ToepOps {
     int device = 0;
-    uint4 *ptr = nullptr;
+    uint32_t *ptr = nullptr;
     ~ToepOps() {
         if (ptr) {
             int cur = 0;
@@ -3684,7 +3703,7 @@ static int setup_toep(lbn_ctx *ctx, u128 q) {
         HIPCHK(ctx, ajtai_device(d->stream, q, re, 0, re, d->amat, &done));
         if (!done) return ctx->fail(LBN_E_DEVICE, "matrix expansion did not finish");
     }
-    k_toep_build<<<dim3(TP_TILES, (unsigned)re), 64, 0, d->stream>>>(d->amat, (int)re, (uint64_t)q, ops->ptr);
+    k_toep_build<<<(unsigned)re, 256, 0, d->stream>>>(d->amat, (int)re, (uint64_t)q, ops->ptr);
     HIPCHK(ctx, hipGetLastError());
     HIPCHK(ctx, hipStreamSynchronize(d->stream));
     HIPCHK(ctx, hipFree(d->amat));

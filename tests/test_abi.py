@@ -39,10 +39,9 @@ def test_context_footprint_without_a_device():
     # + 2^23 x 24 B x 1.5
     assert sizes[-1] > (1 << 24) * 16 * 2.5 + (1 << 23) * 24 * 1.5
     assert m.ctx_footprint(31, (1 << 95) + 12345) > m.ctx_footprint(31, 1 << 40)  # more CRT primes
-    # below 2^39 the commitment runs on the matrix cores: 41 row tiles x 4096 elements x 40 KiB of
-    # int8 operands per context, beside the table and the switch buffers
-    assert m.ctx_footprint(31) > 41 * 4096 * 40960 + (1 << 24) * 16 * 2.5
-    assert m.ctx_footprint(31) > m.ctx_footprint(31, 1 << 40) + 6 * (1 << 30)
+    # below 2^39 the commitment runs on the matrix cores: per ring element 3 x 4 x 272 x 5 dwords of
+    # operand tables (4096 elements at k = 31), beside the table and the switch buffers
+    assert m.ctx_footprint(31) > 4096 * 16320 * 4 + (1 << 24) * 16 * 2.5
     assert m.ctx_footprint(18) == 0 and m.ctx_footprint(31, 1 << 20) == 0
 
 

@@ -110,6 +110,27 @@ size_t lbn_proof_len(int log_vars, uint64_t q_lo, uint64_t q_hi);
 int lbn_prove(lbn_ctx *ctx, const uint8_t seed[32], uint8_t *proof, size_t cap, size_t *len,
               lbn_profile *prof);
 
+/* Many proofs at once, no counterpart in the reference (which proves one table per process,
+ * solution/proof_generate/src/main.rs:2050-2090): each item is lbn_prove on its own context (all on
+ * one device, distinct, modulus set; the table of `seed` generated on the device first when
+ * gen_table is nonzero, else the loaded one), with the switch rounds of all items run as one
+ * device sequence — one launch per prefix round, bind, control step and fold covering every
+ * instance — instead of one sequence per proof; the commitments, partials, rows, challenges and
+ * folds run on the items' own streams from `threads` host threads. Each proof equals the one
+ * lbn_prove gives alone. 1 <= n <= 64. Returns LBN_OK or the first item's error; every item has
+ * its own status (0 or a negative LBN_E_* code). */
+typedef struct {
+    lbn_ctx *ctx;
+    uint8_t seed[32];
+    int gen_table;
+    uint8_t *proof;
+    size_t cap;
+    size_t len;          /* out */
+    int status;          /* out */
+    lbn_profile profile; /* out: the proof's host phases (the switch phase spans the batched rounds) */
+} lbn_batch_item;
+int lbn_prove_batch(lbn_batch_item *items, int n, int threads);
+
 /* The checker (verifier/proof_verify/src/main.rs:381-389 verify, with check_proof_shape,
  * check_field_switch and check_opening, main.rs:293-379): accepts `proof` (bincode bytes) for the
  * trace of `seed` at the context's shape under the modulus the proof carries. Returns LBN_OK,

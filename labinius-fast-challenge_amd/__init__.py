@@ -42,6 +42,12 @@ class Job(ctypes.Structure):
                 ("profile", Profile)]
 
 
+class BatchItem(ctypes.Structure):
+    _fields_ = [("ctx", ctypes.c_void_p), ("seed", ctypes.c_uint8 * 32), ("gen_table", ctypes.c_int),
+                ("proof", ctypes.c_void_p), ("cap", ctypes.c_size_t), ("len", ctypes.c_size_t),
+                ("status", ctypes.c_int), ("profile", Profile)]
+
+
 class Timing(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in ("table_ms", "eq_ms", "partials_ms", "switch0_ms", "switch_rounds_ms",
                                               "row_ms", "fold_ms", "commit_mac_ms", "commit_finish_ms")] + \
@@ -92,6 +98,7 @@ def lib():
         L.lbn_proof_len.restype = ctypes.c_size_t
         L.lbn_prove.argtypes = [P, P, P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t), P]
         L.lbn_verify.argtypes = [P, P, P, ctypes.c_size_t]
+        L.lbn_prove_batch.argtypes = [P, ctypes.c_int, ctypes.c_int]
         L.lbn_ajtai_elements.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_size_t,
                                          ctypes.c_size_t, ctypes.c_size_t, P]
         L.lbn_commit_primes.argtypes = [P]
@@ -149,6 +156,24 @@ def _b(b: bytes):
 
 def proof_len(log_vars: int, q: int = DEFAULT_Q) -> int:
     return lib().lbn_proof_len(log_vars, q & (2**64 - 1), q >> 64)
+
+
+def prove_batch(provers, seeds, outs, gen_table=True, threads=12):
+    """lbn_prove_batch: proof i of provers[i] (distinct contexts on one GPU, modulus set) for seeds[i]
+    into outs[i] (writable contiguous uint8 arrays of at least proof_len bytes); with gen_table the
+    table of seeds[i] is generated on the device first. Returns the list of (length, status)."""
+    n = len(provers)
+    items = (BatchItem * n)()
+    for i, (p, seed, out) in enumerate(zip(provers, seeds, outs)):
+        if out.dtype != np.uint8 or not out.flags.c_contiguous or not out.flags.writeable:
+            raise ValueError("outs must be writable contiguous uint8 arrays")
+        items[i].ctx = p.h
+        ctypes.memmove(items[i].seed, seed, 32)
+        items[i].gen_table = 1 if gen_table else 0
+        items[i].proof = out.ctypes.data
+        items[i].cap = out.size
+    lib().lbn_prove_batch(ctypes.cast(items, ctypes.c_void_p), n, threads)
+    return [(items[i].len, items[i].status) for i in range(n)]
 
 
 def coefficient_bytes(q: int) -> int:

@@ -795,17 +795,17 @@ __device__ __forceinline__ void rho_mul(const uint32_t *tab, const uint32_t (&y)
 // ---------------------------------------------------------------------------------------------
 // Block 0 also clears `clear` (1536 words), the message buffer of the round after next.
 // rho_dev, when not null, holds the challenge (written by the device transcript) instead of rho.
+// rtab: RHO_WINDOWS * 16 * 6 words, lds: 4 * M4W<6>::WORDS words (16-byte aligned), both in LDS
 template <bool WORDS>
-__global__ __launch_bounds__(256) void k_switch_round(const uint32_t *__restrict__ pin, size_t h2, RhoArg rho,
-                                                      const uint4 *__restrict__ Ein, uint4 *__restrict__ Eout,
-                                                      uint32_t *__restrict__ pout, uint32_t *T0, uint32_t *Tinf,
-                                                      uint32_t *clear, const uint64_t *rho_dev) {
-    if (blockIdx.x == 0)
+__device__ __forceinline__ void switch_round_body(int bid, int nb, const uint32_t *__restrict__ pin, size_t h2,
+                                                  RhoArg rho, const uint4 *__restrict__ Ein, uint4 *__restrict__ Eout,
+                                                  uint32_t *__restrict__ pout, uint32_t *T0, uint32_t *Tinf,
+                                                  uint32_t *clear, const uint64_t *rho_dev, uint32_t *rtab,
+                                                  uint32_t *lds) {
+    if (bid == 0)
         for (int i = threadIdx.x; i < 1536; i += blockDim.x) clear[i] = 0;
-    __shared__ uint32_t rtab[RHO_WINDOWS * 16 * 6];
-    __shared__ __align__(16) uint32_t lds[4 * M4W<6>::WORDS];  // also holds the basis while rtab is built
     const size_t h = 2 * h2;
-    const size_t nthreads = (h2 + 255) / 256 * 256, stride = (size_t)gridDim.x * 256;
+    const size_t nthreads = (h2 + 255) / 256 * 256, stride = (size_t)nb * 256;
     // one item's operands: p_j, p_{j+h2}, p_{j+h}, p_{j+h+h2} and E_j, E_{j+h2} (zero past h2)
     uint32_t a[4][6];
     uint4 ea, eb;
@@ -827,7 +827,7 @@ __global__ __launch_bounds__(256) void k_switch_round(const uint32_t *__restrict
         ea = Ein[jj], eb = Ein[jj + h2];
     };
     // the first item's loads are in flight while the challenge's tables are built
-    size_t base = (size_t)blockIdx.x * 256;
+    size_t base = (size_t)bid * 256;
     if (base < nthreads) load_item(base + threadIdx.x);
     if (rho_dev) rho = RhoArg{{rho_dev[0], rho_dev[1], rho_dev[2]}};
     build_rho_table(rho, lds, rtab);
@@ -868,6 +868,16 @@ __global__ __launch_bounds__(256) void k_switch_round(const uint32_t *__restrict
     m4w_flush<6, true>(acc0, lds, T0);
     __syncthreads();
     m4w_flush<6, true>(acci, lds, Tinf);
+}
+template <bool WORDS>
+__global__ __launch_bounds__(256) void k_switch_round(const uint32_t *__restrict__ pin, size_t h2, RhoArg rho,
+                                                      const uint4 *__restrict__ Ein, uint4 *__restrict__ Eout,
+                                                      uint32_t *__restrict__ pout, uint32_t *T0, uint32_t *Tinf,
+                                                      uint32_t *clear, const uint64_t *rho_dev) {
+    __shared__ uint32_t rtab[RHO_WINDOWS * 16 * 6];
+    __shared__ __align__(16) uint32_t lds[4 * M4W<6>::WORDS];  // also holds the basis while rtab is built
+    switch_round_body<WORDS>(blockIdx.x, gridDim.x, pin, h2, rho, Ein, Eout, pout, T0, Tinf, clear, rho_dev, rtab,
+                             lds);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1288,12 +1298,10 @@ __device__ __forceinline__ void switch_state_store(const CtlKeep &K, SwitchDev *
 // The first step takes the whole state from the host's pinned copy (`host`) and the last device step
 // (r == last) returns it there, with the next fold's input (hp_n F162 words from hp_src, he_n B128
 // from he_src) into hp_dst / he_dst, so the rounds need no copies around them.
-__global__ __launch_bounds__(1024) void k_switch_ctl(SwitchDev *sd, SwitchDev *host, int r, int last, int V,
-                                                     const uint32_t *T, const uint32_t *hp_src,
-                                                     uint32_t *hp_dst, int hp_n, const uint4 *he_src, uint4 *he_dst,
-                                                     int he_n) {
-    __shared__ CtlKeep K;
-    __shared__ CtlTemp X;
+__device__ __forceinline__ void switch_ctl_body(CtlKeep &K, CtlTemp &X, SwitchDev *sd, SwitchDev *host, int r,
+                                                int last, int V, const uint32_t *T, const uint32_t *hp_src,
+                                                uint32_t *hp_dst, int hp_n, const uint4 *he_src, uint4 *he_dst,
+                                                int he_n) {
     static_assert(sizeof(SwitchDev) % 4 == 0, "state copied as words");
     constexpr int SW_WORDS = (int)(sizeof(SwitchDev) / 4);
     if (r == 0) {
@@ -1316,6 +1324,14 @@ __global__ __launch_bounds__(1024) void k_switch_ctl(SwitchDev *sd, SwitchDev *h
         for (int i = threadIdx.x; i < he_n; i += blockDim.x) he_dst[i] = he_src[i];
     }
 }
+__global__ __launch_bounds__(1024) void k_switch_ctl(SwitchDev *sd, SwitchDev *host, int r, int last, int V,
+                                                     const uint32_t *T, const uint32_t *hp_src,
+                                                     uint32_t *hp_dst, int hp_n, const uint4 *he_src, uint4 *he_dst,
+                                                     int he_n) {
+    __shared__ CtlKeep K;
+    __shared__ CtlTemp X;
+    switch_ctl_body(K, X, sd, host, r, last, V, T, hp_src, hp_dst, hp_n, he_src, he_dst, he_n);
+}
 
 // ---------------------------------------------------------------------------------------------
 // The switch's first six rounds from the prefix sets. With S = log_packed - 6, round r < 6 holds
@@ -1334,17 +1350,17 @@ __global__ __launch_bounds__(1024) void k_switch_ctl(SwitchDev *sd, SwitchDev *h
 // of bit k over the wave is bits 64 half .. 64 half + 63 of N_u[k], kept by lane k mod 64. Block 0
 // also copies the lambda values from the host's mailbox (lam: lambda0 of round r at
 // 64 - 2^(6-r) + v, lambda1 at 64 more) to lam_dev and zeroes round 0's message buffer `clear`.
-__global__ __launch_bounds__(256) void k_prefix_q0(const uint32_t *M, uint32_t *Q, const uint4 *lam, uint4 *lam_dev,
-                                                   uint32_t *clear) {
+__device__ __forceinline__ void prefix_q0_body(int bid, const uint32_t *M, uint32_t *Q, const uint4 *lam,
+                                               uint4 *lam_dev, uint32_t *clear) {
     const int t = threadIdx.x, lane = t & 63;
-    if (blockIdx.x == 0) {
+    if (bid == 0) {
         if (t < 128) {
             const volatile uint32_t *l = (const volatile uint32_t *)(lam + t);
             lam_dev[t] = make_uint4(l[0], l[1], l[2], l[3]);
         }
         for (int i = t; i < 1536; i += blockDim.x) clear[i] = 0;
     }
-    const int task = blockIdx.x * 4 + (t >> 6), u = task >> 1, half = task & 1;
+    const int task = bid * 4 + (t >> 6), u = task >> 1, half = task & 1;
     const uint32_t *row = M + ((size_t)u * 128 + 64 * half + lane) * 4;
     const uint32_t r4[4] = {row[0], row[1], row[2], row[3]};
     uint64_t mine[2] = {0, 0};
@@ -1363,21 +1379,25 @@ __global__ __launch_bounds__(256) void k_prefix_q0(const uint32_t *M, uint32_t *
         if (half == 0) q[4] = 0, q[5] = 0;
     }
 }
+__global__ __launch_bounds__(256) void k_prefix_q0(const uint32_t *M, uint32_t *Q, const uint4 *lam, uint4 *lam_dev,
+                                                   uint32_t *clear) {
+    prefix_q0_body(blockIdx.x, M, Q, lam, lam_dev, clear);
+}
 
 // Round r's sums (r < 6), for r > 0 after the fold of round r - 1 (rho from the device state,
 // written by the control step before): item (v, i), v < nv, i < 128, folds Q[v], Q[2 nv + v] into
 // Q[v] and Q[nv + v], Q[3 nv + v] into Q[nv + v] (in place: no item reads what another writes),
 // then adds bit_k(lambda0_v x^i) Q[v][i] into T0 and bit_k(lambda1_v x^i) (Q[v][i] + Q[nv + v][i])
 // into Tinf (zero before). Block 0 also zeroes `clear`, the next round's buffer.
-__global__ __launch_bounds__(256) void k_prefix_msg(uint32_t *Q, int r, const uint4 *lam, const SwitchDev *sd,
-                                                    uint32_t *T0, uint32_t *Tinf, uint32_t *clear) {
-    __shared__ uint32_t rtab[RHO_WINDOWS * 16 * 6];
-    __shared__ __align__(16) uint32_t lds[4 * M4W<6>::WORDS];  // also the basis while rtab is built
+// rtab: RHO_WINDOWS * 16 * 6 words, lds: 4 * M4W<6>::WORDS words (16-byte aligned), both in LDS
+__device__ __forceinline__ void prefix_msg_body(int bid, uint32_t *Q, int r, const uint4 *lam, const SwitchDev *sd,
+                                                uint32_t *T0, uint32_t *Tinf, uint32_t *clear, uint32_t *rtab,
+                                                uint32_t *lds) {
     const int t = threadIdx.x, nv = 32 >> r, items = 128 * nv;
-    if (blockIdx.x == 0)
+    if (bid == 0)
         for (int i = t; i < 1536; i += blockDim.x) clear[i] = 0;
     if (r > 0) build_rho_table(RhoArg{{sd->rho[0], sd->rho[1], sd->rho[2]}}, lds, rtab);
-    const int it = blockIdx.x * 256 + t;
+    const int it = bid * 256 + t;
     uint32_t a[6] = {0, 0, 0, 0, 0, 0}, d[6] = {0, 0, 0, 0, 0, 0};
     uint32_t s0[4] = {0, 0, 0, 0}, s1[4] = {0, 0, 0, 0};
     if (it < items) {
@@ -1410,6 +1430,12 @@ __global__ __launch_bounds__(256) void k_prefix_msg(uint32_t *Q, int r, const ui
     m4w_flush<6, true>(acc0, lds, T0);
     __syncthreads();
     m4w_flush<6, true>(acci, lds, Tinf);
+}
+__global__ __launch_bounds__(256) void k_prefix_msg(uint32_t *Q, int r, const uint4 *lam, const SwitchDev *sd,
+                                                    uint32_t *T0, uint32_t *Tinf, uint32_t *clear) {
+    __shared__ uint32_t rtab[RHO_WINDOWS * 16 * 6];
+    __shared__ __align__(16) uint32_t lds[4 * M4W<6>::WORDS];  // also the basis while rtab is built
+    prefix_msg_body(blockIdx.x, Q, r, lam, sd, T0, Tinf, clear, rtab, lds);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1487,14 +1513,13 @@ __device__ __forceinline__ void bind_values(const uint4 *__restrict__ tw, size_t
     for (int i = 0; i < 6; i++) out0[i] = v[0][i], out1[i] = v[1][i];
 }
 
+// rt: 6 * RHO_WINDOWS * 16 * 6 words, lds: (BIND_THREADS / 64) * M4W<6>::WORDS words, both in LDS
 template <int GL>
-__global__ __launch_bounds__(BIND_THREADS) void k_switch_bind(const uint4 *__restrict__ table, int S,
-                                                              const SwitchDev *sd, const uint4 *__restrict__ G,
-                                                              uint4 *__restrict__ Eout, uint32_t *__restrict__ pout,
-                                                              uint32_t *T0, uint32_t *Tinf, uint32_t *clear) {
-    __shared__ uint32_t rt[6 * RHO_WINDOWS * 16 * 6];
-    __shared__ __align__(16) uint32_t lds[(BIND_THREADS / 64) * M4W<6>::WORDS];  // also the tables' basis
-    if (blockIdx.x == 0)
+__device__ __forceinline__ void switch_bind_body(int bid, int nb, const uint4 *__restrict__ table, int S,
+                                                 const SwitchDev *sd, const uint4 *__restrict__ G,
+                                                 uint4 *__restrict__ Eout, uint32_t *__restrict__ pout, uint32_t *T0,
+                                                 uint32_t *Tinf, uint32_t *clear, uint32_t *rt, uint32_t *lds) {
+    if (bid == 0)
         for (int i = threadIdx.x; i < 1536; i += blockDim.x) clear[i] = 0;
     for (int i = 0; i < 6; i++)
         build_rho_table(RhoArg{{sd->coords[i][0], sd->coords[i][1], sd->coords[i][2]}}, lds,
@@ -1505,7 +1530,7 @@ __global__ __launch_bounds__(BIND_THREADS) void k_switch_bind(const uint4 *__res
     const size_t rounds = (h2 + GPB - 1) / GPB;  // group passes over the items, grid-strided
     uint32_t *W = lds + (threadIdx.x >> 6) * M4W<6>::WORDS;
     uint32_t acc0[2][6] = {}, acci[2][6] = {};
-    for (size_t pass = blockIdx.x; pass < rounds; pass += gridDim.x) {
+    for (size_t pass = bid; pass < rounds; pass += nb) {
         const size_t j = pass * GPB + (threadIdx.x >> GL);
         uint32_t q0[6] = {0, 0, 0, 0, 0, 0}, q1[6] = {0, 0, 0, 0, 0, 0};
         uint32_t e[4] = {0, 0, 0, 0}, e2[4] = {0, 0, 0, 0};
@@ -1533,6 +1558,15 @@ __global__ __launch_bounds__(BIND_THREADS) void k_switch_bind(const uint4 *__res
     m4w_flush<6, true>(acc0, lds, T0);
     __syncthreads();
     m4w_flush<6, true>(acci, lds, Tinf);
+}
+template <int GL>
+__global__ __launch_bounds__(BIND_THREADS) void k_switch_bind(const uint4 *__restrict__ table, int S,
+                                                              const SwitchDev *sd, const uint4 *__restrict__ G,
+                                                              uint4 *__restrict__ Eout, uint32_t *__restrict__ pout,
+                                                              uint32_t *T0, uint32_t *Tinf, uint32_t *clear) {
+    __shared__ uint32_t rt[6 * RHO_WINDOWS * 16 * 6];
+    __shared__ __align__(16) uint32_t lds[(BIND_THREADS / 64) * M4W<6>::WORDS];  // also the tables' basis
+    switch_bind_body<GL>(blockIdx.x, gridDim.x, table, S, sd, G, Eout, pout, T0, Tinf, clear, rt, lds);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -3070,7 +3104,7 @@ struct Reader {
 };
 }  // namespace chal
 
-__global__ __launch_bounds__(64 * chal::WAVES) void k_challenges(const ChalArg *arg, lbn_challenge *out) {
+__device__ __forceinline__ void challenges_body(int bid, const ChalArg *arg, lbn_challenge *out) {
     __shared__ chal::Wave waves[chal::WAVES];
     __shared__ chal::Spec spec[chal::WAVES][chal::SPEC];
     __shared__ uint32_t s_stack[54][8];
@@ -3085,7 +3119,7 @@ __global__ __launch_bounds__(64 * chal::WAVES) void k_challenges(const ChalArg *
     if (t < 81) s_units[t] = arg->units[t];
     if (t < 8) s_key[t] = ts->key[t];
     __syncthreads();
-    const int i = blockIdx.x * chal::WAVES + wv;  // this wave's challenge
+    const int i = bid * chal::WAVES + wv;  // this wave's challenge
     chal::Wave *w = &waves[wv];
     chal::Reader r;
     if (lane < 4) {
@@ -3224,6 +3258,20 @@ __global__ __launch_bounds__(64 * chal::WAVES) void k_challenges(const ChalArg *
     }
     if (lane == 8) out[i].signs = w->signs;
     if (lane == 9) out[i].weight = WEIGHT;
+}
+__global__ __launch_bounds__(64 * chal::WAVES) void k_challenges(const ChalArg *arg, lbn_challenge *out) {
+    challenges_body(blockIdx.x, arg, out);
+}
+// The challenges of many proofs in one launch: instance i's input and output at arg[i], out[i], its
+// challenges in blocks [i COLUMNS / WAVES, (i + 1) COLUMNS / WAVES).
+struct ChalPtrs {
+    const ChalArg *arg;
+    lbn_challenge *out;
+};
+__global__ __launch_bounds__(64 * chal::WAVES) void k_mchallenges(const ChalPtrs *p) {
+    constexpr int PER = COLUMNS / chal::WAVES;
+    const ChalPtrs x = p[blockIdx.x / PER];
+    challenges_body(blockIdx.x % PER, x.arg, x.out);
 }
 
 // =============================================================================================
@@ -4381,10 +4429,10 @@ static int capture_switch(lbn_ctx *ctx) {
     return LBN_OK;
 }
 
-int lbn::dev_switch_rounds(lbn_ctx *ctx, B3State &ts, uint64_t *rounds, uint64_t *coords, SwitchHandoff &ho) {
-    if (ctx->round != 0) return ctx->fail(LBN_E_STATE, "lbn_switch_begin must run first");
-    const int lp = ctx->shape.log_packed, r0 = switch_host_from(lp);
-    HIPCHK(ctx, hipSetDevice(ctx->device));
+// What the device rounds start from, written to the context's mailbox: the transcript state, the
+// map L_0, r_hi and the prefix rounds' eq weights.
+static void switch_mailbox(lbn_ctx *ctx, const B3State &ts) {
+    const int lp = ctx->shape.log_packed;
     DeviceState *d = ctx->dev;
     SwitchDev *h = (SwitchDev *)(d->hbox + mb::SW);
     static_assert(sizeof(SwitchDev) <= 8192, "mailbox region");
@@ -4400,16 +4448,13 @@ int lbn::dev_switch_rounds(lbn_ctx *ctx, B3State &ts, uint64_t *rounds, uint64_t
             for (int v = 0; v < nv; v++) lam[off + v] = b128_mul(one_c, lam[64 + off + v]);
         }
     }
-    // the prefix kernel reads the state from the mailbox, the last control step writes it back there
-    tstart(ctx, T_ROUNDS);
-    if (d->graphs)
-        HIPCHK(ctx, hipGraphLaunch(d->switch_graph, d->stream));
-    else
-        launch_switch(ctx);
-    tstop(ctx, T_ROUNDS);
-    HIPCHK(ctx, hipGetLastError());
-    HIPCHK(ctx, hipStreamSynchronize(d->stream));
-    tcollect(ctx, T_ROUNDS);
+}
+
+// The device rounds' results from the mailbox (after the stream has finished them).
+static void switch_readback(lbn_ctx *ctx, B3State &ts, uint64_t *rounds, uint64_t *coords, SwitchHandoff &ho) {
+    const int lp = ctx->shape.log_packed, r0 = switch_host_from(lp);
+    DeviceState *d = ctx->dev;
+    const SwitchDev *h = (const SwitchDev *)(d->hbox + mb::SW);
     ts = h->ts;
     memcpy(rounds, h->rounds, (size_t)r0 * 48);
     memcpy(coords, h->coords, (size_t)r0 * 24);
@@ -4422,6 +4467,209 @@ int lbn::dev_switch_rounds(lbn_ctx *ctx, B3State &ts, uint64_t *rounds, uint64_t
     ho.e.resize(2 * h2);
     memcpy(ho.e.data(), d->hbox + mb::HE, 2 * h2 * sizeof(B128));
     ctx->round = lp;  // the host finishes the rounds (and sets ctx->opened)
+}
+
+int lbn::dev_switch_rounds(lbn_ctx *ctx, B3State &ts, uint64_t *rounds, uint64_t *coords, SwitchHandoff &ho) {
+    if (ctx->round != 0) return ctx->fail(LBN_E_STATE, "lbn_switch_begin must run first");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    DeviceState *d = ctx->dev;
+    switch_mailbox(ctx, ts);
+    // the prefix kernel reads the state from the mailbox, the last control step writes it back there
+    tstart(ctx, T_ROUNDS);
+    if (d->graphs)
+        HIPCHK(ctx, hipGraphLaunch(d->switch_graph, d->stream));
+    else
+        launch_switch(ctx);
+    tstop(ctx, T_ROUNDS);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipStreamSynchronize(d->stream));
+    tcollect(ctx, T_ROUNDS);
+    switch_readback(ctx, ts, rounds, coords, ho);
+    return LBN_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// The switch of many proofs at once: each launch of the device round sequence (prefix rounds,
+// bind, control steps and folds) covers every instance, so a batch of n proofs makes one launch
+// per step instead of n. Instance i's blocks are [start[i], start[i + 1]) of the grid; an instance
+// whose device rounds are over takes no blocks. Every instance's buffers and state are its own
+// context's, so each proof is the one its context would make alone.
+// ---------------------------------------------------------------------------------------------
+struct SwView {
+    const uint4 *table;
+    uint32_t *pset, *T;
+    SwitchDev *sw, *mbox_sw;
+    const uint4 *mbox_lam;
+    uint32_t *hp;
+    uint4 *he;
+    uint4 *E[2];
+    uint32_t *P[2];
+    size_t n_words;
+    int lp, S, r0, gl;
+};
+struct SwMap {
+    int n;
+    int start[SWB_MAX + 1];
+};
+__device__ __forceinline__ uint32_t *swb_msg(const SwView &x, int r) { return x.T + ((r & 1) ? 2048 : 0); }
+__device__ __forceinline__ int swb_find(const SwMap &mp, int b) {  // the last i with start[i] <= b
+    int lo = 0, hi = mp.n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (mp.start[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+__global__ __launch_bounds__(256) void k_mprefix_q0(const SwView *v) {
+    const SwView &x = v[blockIdx.x / 32];
+    prefix_q0_body(blockIdx.x % 32, x.pset + ps::M, x.pset + ps::Q, x.mbox_lam, (uint4 *)(x.pset + ps::LAM),
+                   swb_msg(x, 0));
+}
+__global__ __launch_bounds__(256) void k_mprefix_msg(const SwView *v, int r, int per) {
+    __shared__ uint32_t rtab[RHO_WINDOWS * 16 * 6];
+    __shared__ __align__(16) uint32_t lds[4 * M4W<6>::WORDS];
+    const SwView &x = v[blockIdx.x / per];
+    uint32_t *T = swb_msg(x, r);
+    prefix_msg_body(blockIdx.x % per, x.pset + ps::Q, r, (const uint4 *)(x.pset + ps::LAM), x.sw, T, T + 768,
+                    swb_msg(x, r + 1), rtab, lds);
+}
+// control step r of every instance (block i: instance i): a prefix round, or a later one while
+// r < r0 of the instance (its last device step hands over to the host)
+__global__ __launch_bounds__(1024) void k_mswitch_ctl(const SwView *v, int r, int prefix) {
+    __shared__ CtlKeep K;
+    __shared__ CtlTemp X;
+    const SwView &x = v[blockIdx.x];
+    if (prefix) {
+        switch_ctl_body(K, X, x.sw, x.mbox_sw, r, -1, 6, swb_msg(x, r), nullptr, nullptr, 0, nullptr, nullptr, 0);
+        return;
+    }
+    if (r >= x.r0) return;  // the whole block
+    const bool hand = r == x.r0 - 1;
+    const int h2 = 1 << (x.lp - x.r0 - 1);
+    switch_ctl_body(K, X, x.sw, x.mbox_sw, r, x.r0 - 1, 6, swb_msg(x, r), hand ? x.P[(x.r0 - 2) & 1] : nullptr, x.hp,
+                    hand ? 4 * h2 : 0, hand ? x.E[x.r0 & 1] : nullptr, x.he, hand ? 2 * h2 : 0);
+}
+__global__ __launch_bounds__(BIND_THREADS) void k_mswitch_bind(const SwView *v, SwMap mp) {
+    __shared__ uint32_t rt[6 * RHO_WINDOWS * 16 * 6];
+    __shared__ __align__(16) uint32_t lds[(BIND_THREADS / 64) * M4W<6>::WORDS];
+    const int i = swb_find(mp, blockIdx.x), b = blockIdx.x - mp.start[i], nb = mp.start[i + 1] - mp.start[i];
+    const SwView &x = v[i];
+    uint32_t *T = swb_msg(x, 6), *clr = swb_msg(x, 7);
+#define MBIND(G) \
+    switch_bind_body<G>(b, nb, x.table, x.S, x.sw, x.E[0], x.E[1], x.P[1], T, T + 768, clr, rt, lds)
+    switch (x.gl) {
+        case 0: MBIND(0); break;
+        case 1: MBIND(1); break;
+        case 2: MBIND(2); break;
+        case 3: MBIND(3); break;
+        case 4: MBIND(4); break;
+        default: MBIND(5); break;
+    }
+#undef MBIND
+}
+// the fold of round r (6 <= r < r0 - 1) with round r + 1's message sums
+__global__ __launch_bounds__(256) void k_mswitch_round(const SwView *v, SwMap mp, int r) {
+    __shared__ uint32_t rtab[RHO_WINDOWS * 16 * 6];
+    __shared__ __align__(16) uint32_t lds[4 * M4W<6>::WORDS];
+    const int i = swb_find(mp, blockIdx.x), b = blockIdx.x - mp.start[i], nb = mp.start[i + 1] - mp.start[i];
+    const SwView &x = v[i];
+    const size_t h2 = (x.n_words >> (r + 1)) / 2;
+    uint32_t *T = swb_msg(x, r + 1);
+    switch_round_body<false>(b, nb, x.P[(r - 1) & 1], h2, RhoArg{{0, 0, 0}}, x.E[(r + 1) & 1], x.E[r & 1],
+                             x.P[r & 1], T, T + 768, swb_msg(x, r), x.sw->rho, rtab, lds);
+}
+
+// One per device: the stream and the view array of batched switches (calls on one device are
+// serialised).
+struct SwBatchDev {
+    hipStream_t stream = nullptr;
+    SwView *views = nullptr;
+    struct ChalPtrs *chal = nullptr;
+};
+static int swb_ready(lbn_ctx *ctx, SwBatchDev &bd) {
+    if (bd.stream) return LBN_OK;
+    HIPCHK(ctx, hipStreamCreateWithFlags(&bd.stream, hipStreamNonBlocking));
+    HIPCHK(ctx, hipMalloc(&bd.views, sizeof(SwView) * SWB_MAX));
+    HIPCHK(ctx, hipMalloc(&bd.chal, sizeof(ChalPtrs) * SWB_MAX));
+    return LBN_OK;
+}
+static std::mutex swb_mu;
+static std::map<int, SwBatchDev> swb_dev;
+
+int lbn::dev_switch_batch(lbn_ctx *const *ctxs, int n, B3State *ts, uint64_t *const *rounds,
+                          uint64_t *const *coords, SwitchHandoff *ho) {
+    if (n < 1 || n > SWB_MAX) return LBN_E_ARG;
+    const int dev = ctxs[0]->device;
+    for (int i = 0; i < n; i++) {
+        lbn_ctx *ctx = ctxs[i];
+        if (ctx->device != dev) return ctx->fail(LBN_E_ARG, "a batched switch runs on one device");
+        if (ctx->round != 0) return ctx->fail(LBN_E_STATE, "lbn_switch_begin must run first");
+        for (int j = 0; j < i; j++)
+            if (ctxs[j] == ctx) return ctx->fail(LBN_E_ARG, "a context appears twice in the batch");
+    }
+    std::lock_guard<std::mutex> lk(swb_mu);
+    HIPCHK(ctxs[0], hipSetDevice(dev));
+    SwBatchDev &bd = swb_dev[dev];
+    {
+        const int rc = swb_ready(ctxs[0], bd);
+        if (rc != LBN_OK) return rc;
+    }
+    std::vector<SwView> hv(n);
+    int max_r0 = 0;
+    for (int i = 0; i < n; i++) {
+        lbn_ctx *ctx = ctxs[i];
+        DeviceState *d = ctx->dev;
+        HIPCHK(ctx, hipStreamSynchronize(d->stream));  // its partials are in, its mailbox is free
+        switch_mailbox(ctx, ts[i]);
+        SwView &x = hv[i];
+        x.table = d->table;
+        x.pset = d->pset;
+        x.T = d->T;
+        x.sw = d->sw;
+        x.mbox_sw = (SwitchDev *)(d->hbox_dev + mb::SW);
+        x.mbox_lam = (const uint4 *)(d->hbox_dev + mb::LAM);
+        x.hp = (uint32_t *)(d->hbox_dev + mb::HP);
+        x.he = (uint4 *)(d->hbox_dev + mb::HE);
+        x.E[0] = d->E[0], x.E[1] = d->E[1];
+        x.P[0] = d->P[0], x.P[1] = d->P[1];
+        x.n_words = ctx->shape.n_words;
+        x.lp = ctx->shape.log_packed;
+        x.S = x.lp - 6;
+        x.r0 = switch_host_from(x.lp);
+        const size_t items = (size_t)1 << (x.S - 1);
+        int gl = 0;
+        while (gl < 5 && (items << gl) < ((size_t)1 << 17)) gl++;
+        x.gl = gl;
+        max_r0 = std::max(max_r0, x.r0);
+    }
+    hipStream_t st = bd.stream;
+    HIPCHK(ctxs[0], hipMemcpyAsync(bd.views, hv.data(), sizeof(SwView) * n, hipMemcpyHostToDevice, st));
+    k_mprefix_q0<<<32 * n, 256, 0, st>>>(bd.views);
+    for (int r = 0; r < 6; r++) {
+        const int per = ((128 << (5 - r)) + 255) / 256;
+        k_mprefix_msg<<<per * n, 256, 0, st>>>(bd.views, r, per);
+        k_mswitch_ctl<<<n, 1024, 0, st>>>(bd.views, r, 1);
+    }
+    SwMap mp{};
+    mp.n = n;
+    for (int i = 0; i < n; i++) {
+        const size_t items = (size_t)1 << (hv[i].S - 1);
+        const int blocks = (int)std::min<size_t>(((items << hv[i].gl) + BIND_THREADS - 1) / BIND_THREADS, 512);
+        mp.start[i + 1] = mp.start[i] + blocks;
+    }
+    k_mswitch_bind<<<mp.start[n], BIND_THREADS, 0, st>>>(bd.views, mp);
+    for (int r = 6; r < max_r0; r++) {
+        k_mswitch_ctl<<<n, 1024, 0, st>>>(bd.views, r, 0);
+        for (int i = 0; i < n; i++) {
+            const int blocks = r < hv[i].r0 - 1 ? launch_blocks((hv[i].n_words >> (r + 1)) / 2) : 0;
+            mp.start[i + 1] = mp.start[i] + blocks;
+        }
+        if (mp.start[n] > 0) k_mswitch_round<<<mp.start[n], 256, 0, st>>>(bd.views, mp, r);
+    }
+    HIPCHK(ctxs[0], hipGetLastError());
+    HIPCHK(ctxs[0], hipStreamSynchronize(st));
+    for (int i = 0; i < n; i++) switch_readback(ctxs[i], ts[i], rounds[i], coords[i], ho[i]);
     return LBN_OK;
 }
 
@@ -4605,6 +4853,34 @@ extern "C" int lbn_fold(lbn_ctx *ctx, const lbn_challenge *ch, int n_challenges,
     tcollect(ctx, T_FOLD);
     if (flag) return ctx->fail(LBN_E_RANGE, "a fold coefficient does not fit the wire codec");
     if (!direct) copy_parallel((uint8_t *)out, stage, n_out * 4);
+    return LBN_OK;
+}
+
+int lbn::dev_challenges_batch(lbn_ctx *const *ctxs, int n, const ChalArg *args, lbn_challenge *const *outs) {
+    if (n < 1 || n > SWB_MAX) return LBN_E_ARG;
+    const int dev = ctxs[0]->device;
+    for (int i = 0; i < n; i++)
+        if (ctxs[i]->device != dev) return ctxs[i]->fail(LBN_E_ARG, "a batch runs on one device");
+    std::lock_guard<std::mutex> lk(swb_mu);
+    HIPCHK(ctxs[0], hipSetDevice(dev));
+    SwBatchDev &bd = swb_dev[dev];
+    {
+        const int rc = swb_ready(ctxs[0], bd);
+        if (rc != LBN_OK) return rc;
+    }
+    std::vector<ChalPtrs> hp(n);
+    for (int i = 0; i < n; i++) {
+        DeviceState *d = ctxs[i]->dev;
+        HIPCHK(ctxs[i], hipStreamSynchronize(d->stream));  // its mailbox is free
+        memcpy(d->hbox + mb::CH, &args[i], sizeof(ChalArg));
+        HIPCHK(ctxs[i], hipMemcpyAsync(d->chal, d->hbox + mb::CH, sizeof(ChalArg), hipMemcpyHostToDevice, bd.stream));
+        hp[i] = ChalPtrs{d->chal, (lbn_challenge *)(d->hbox_dev + mb::CHOUT)};
+    }
+    HIPCHK(ctxs[0], hipMemcpyAsync(bd.chal, hp.data(), sizeof(ChalPtrs) * n, hipMemcpyHostToDevice, bd.stream));
+    k_mchallenges<<<n * (COLUMNS / chal::WAVES), 64 * chal::WAVES, 0, bd.stream>>>(bd.chal);
+    HIPCHK(ctxs[0], hipGetLastError());
+    HIPCHK(ctxs[0], hipStreamSynchronize(bd.stream));
+    for (int i = 0; i < n; i++) memcpy(outs[i], ctxs[i]->dev->hbox + mb::CHOUT, COLUMNS * sizeof(lbn_challenge));
     return LBN_OK;
 }
 

@@ -7,6 +7,8 @@
 
 #include <algorithm>
 #include <atomic>
+#include <functional>
+#include <memory>
 #include <chrono>
 #include <condition_variable>
 #include <mutex>
@@ -538,42 +540,60 @@ void pack_f162(const F162 *v, size_t n, uint8_t *out) {
         if (rc_ != LBN_OK) return rc_; \
     } while (0)
 
-extern "C" int lbn_prove(lbn_ctx *ctx, const uint8_t seed[32], uint8_t *proof, size_t cap, size_t *len,
-                         lbn_profile *prof) {
-    if (!ctx || !seed || !proof || !len) return LBN_E_ARG;
+namespace {
+// One proof between its phases: the front (commitment, transcript, partials, the switch's start),
+// the device switch rounds (alone or batched with other proofs), the back (the switch's host steps,
+// row, folding challenges, fold, bincode). Field offsets of the bincode layout:
+// verifier/proof_verify/src/main.rs:98-124.
+struct ProveJob {
+    lbn_ctx *ctx;
+    const uint8_t *seed;
+    uint8_t *proof;
+    size_t plen = 0, clen = 0, row_bytes = 0, fold_bytes = 0;
+    size_t off_commit = 0, off_partials = 0, off_fold = 0;
+    Transcript t{"labinius-fast/v1"};
+    std::vector<B128> partials;
+    std::vector<F162> rounds, coords;
+    B3State ts{};
+    SwitchHandoff ho;
+    lbn_profile pf{};
+    double t0 = 0, t3 = 0, t4 = 0, t5 = 0;
+    std::vector<F162> row;
+    ChalArg ca{};
+    std::vector<lbn_challenge> ch;
+    bool launched = false;  // device work queued: an error return waits for the device first
+    ProveJob(lbn_ctx *c, const uint8_t *sd, uint8_t *pr) : ctx(c), seed(sd), proof(pr) {}
+    ~ProveJob() {
+        if (launched) dev_quiesce(ctx);
+    }
+};
+
+int prove_front(ProveJob &j, size_t cap) {
+    lbn_ctx *ctx = j.ctx;
     if (!ctx->table_loaded || !ctx->modulus_set) return ctx->fail(LBN_E_STATE, "table and modulus must be loaded");
     const Shape &s = ctx->shape;
     const u128 q = ctx->q;
-    const size_t plen = lbn_proof_len(s.log_vars, (uint64_t)q, (uint64_t)(q >> 64));
-    if (cap < plen) return ctx->fail(LBN_E_ARG, "proof buffer too small");
+    j.plen = lbn_proof_len(s.log_vars, (uint64_t)q, (uint64_t)(q >> 64));
+    if (cap < j.plen) return ctx->fail(LBN_E_ARG, "proof buffer too small");
     const int lp = s.log_packed;
-    const size_t clen = (size_t)COLUMNS * N * ctx->width;
-    const size_t row_bytes = f162_bytes(COLUMNS);
-    const size_t fold_bytes = 4 * (size_t)N * s.ring_elements;
-    // Field offsets of the bincode layout (verifier/proof_verify/src/main.rs:98-124).
-    const size_t off_commit = 4 + 16 + 8;
-    const size_t off_partials = off_commit + clen;  // u64 len + 128 x u128
-    const size_t off_rounds = off_partials + 8 + 16 * 128;
+    j.clen = (size_t)COLUMNS * N * ctx->width;
+    j.row_bytes = f162_bytes(COLUMNS);
+    j.fold_bytes = 4 * (size_t)N * s.ring_elements;
+    j.off_commit = 4 + 16 + 8;
+    j.off_partials = j.off_commit + j.clen;  // u64 len + 128 x u128
+    const size_t off_rounds = j.off_partials + 8 + 16 * 128;
     const size_t off_opened = off_rounds + 8 + 48 * (size_t)lp;
     const size_t off_row = off_opened + 24;
-    const size_t off_fold = off_row + 8 + row_bytes;
-    lbn_profile pf{};
-    const double t0 = now_us();
+    j.off_fold = off_row + 8 + j.row_bytes;
+    j.t0 = now_us();
     // From the first launch on, an error return first waits for the device: the commitment's copy
     // and the fold kernel may write into `proof` (directly when it is pinned).
-    struct Quiesce {
-        lbn_ctx *ctx;
-        bool armed = true;
-        ~Quiesce() {
-            if (armed) dev_quiesce(ctx);
-        }
-    } quiesce{ctx};
-
+    j.launched = true;
     TRY(dev_commit_launch(ctx));
 
     // verifier/proof_verify/src/main.rs:166-182 (absorb k and 10, the shape of this instance)
-    Transcript t("labinius-fast/v1");
-    t.absorb_bytes(seed, 32);
+    Transcript &t = j.t;
+    t.absorb_bytes(j.seed, 32);
     t.absorb_u64((uint64_t)s.log_vars);
     t.absorb_u64(COLUMN_LOG);
     uint8_t qb[16];
@@ -582,11 +602,11 @@ extern "C" int lbn_prove(lbn_ctx *ctx, const uint8_t seed[32], uint8_t *proof, s
     // The commitment's whole BLAKE3 chunks are hashed where the bytes are (device); the host reads
     // back their chaining values and the bytes around them. All bytes reach the proof at the end.
     int cvs_rc = LBN_OK;
-    t.absorb_bytes_with(proof + off_commit, clen,
+    uint8_t *cdst = j.proof + j.off_commit;
+    t.absorb_bytes_with(cdst, j.clen,
                         [&](size_t off, uint64_t first, size_t count, const Blake3::Subtrees &trees, uint32_t *cvs) {
                             cvs_rc = dev_commit_cvs(ctx, off, first, count, trees, t.hasher().key_words(),
-                                                    t.hasher().flags(), cvs, proof + off_commit, off,
-                                                    off + count * Blake3::CHUNK);
+                                                    t.hasher().flags(), cvs, cdst, off, off + count * Blake3::CHUNK);
                         });
     TRY(cvs_rc);
     const double t1 = now_us();
@@ -594,42 +614,49 @@ extern "C" int lbn_prove(lbn_ctx *ctx, const uint8_t seed[32], uint8_t *proof, s
     std::vector<B128> r_hi = t.sample_b128("r-hi", lp);
     const double t2 = now_us();
 
-    std::vector<B128> partials(128);
-    TRY(lbn_partials(ctx, (const uint64_t *)r_hi.data(), (uint64_t *)partials.data()));
+    j.partials.resize(128);
+    TRY(lbn_partials(ctx, (const uint64_t *)r_hi.data(), (uint64_t *)j.partials.data()));
     {
         B128 eq_lo[128];
         eq_expand_b128(r_lo.data(), 7, eq_lo);
         B128 claim{{0, 0}};
-        for (int b = 0; b < 128; b++) claim = b128_add(claim, b128_mul(partials[b], eq_lo[b]));
+        for (int b = 0; b < 128; b++) claim = b128_add(claim, b128_mul(j.partials[b], eq_lo[b]));
         t.absorb_b128(&claim, 1);
-        t.absorb_b128(partials.data(), 128);
+        t.absorb_b128(j.partials.data(), 128);
     }
     std::vector<F162> rb = t.sample_f162("switch-batch", 7);
     std::vector<F162> batch(128);
     eq_expand_f162(rb.data(), 7, batch.data());
-    const double t3 = now_us();
+    j.t3 = now_us();
+    j.pf.commit_us = t1 - j.t0;
+    j.pf.transcript_us = t2 - t1;
+    j.pf.partials_us = j.t3 - t2;
 
     // The rounds run on the device with the transcript (main.rs:975-992: per round absorb [e0, einf],
     // sample "switch-round", fold; then absorb [opened]); the host continues from its final state.
-    std::vector<F162> rounds(2 * (size_t)lp), coords(lp);
-    F162 opened;
+    j.rounds.resize(2 * (size_t)lp);
+    j.coords.resize(lp);
     TRY(dev_switch_prepare(ctx, (const uint64_t *)batch.data()));
-    {
-        B3State ts;
-        t.save(ts);
-        SwitchHandoff ho;
-        TRY(dev_switch_rounds(ctx, ts, (uint64_t *)rounds.data(), (uint64_t *)coords.data(), ho));
-        t.load(ts);
-        switch_host_tail(ctx, t, ho, rounds.data(), coords.data());
-        opened = ctx->opened;
-    }
-    const double t4 = now_us();
+    t.save(j.ts);
+    return LBN_OK;
+}
+
+// The switch's host steps, the row and its absorb; leaves the folding challenges' input in j.ca.
+int prove_back(ProveJob &j) {
+    lbn_ctx *ctx = j.ctx;
+    const Shape &s = ctx->shape;
+    const int lp = s.log_packed;
+    Transcript &t = j.t;
+    t.load(j.ts);
+    switch_host_tail(ctx, t, j.ho, j.rounds.data(), j.coords.data());
+    j.t4 = now_us();
 
     // EvaluationPoint::msb_first (scheme/commitment.rs:163-174): p0 = reverse(coords[10..]).
-    std::vector<F162> p0(s.row_log), row(COLUMNS);
-    for (int i = 0; i < s.row_log; i++) p0[i] = coords[lp - 1 - i];
-    TRY(lbn_row_eval(ctx, (const uint64_t *)p0.data(), s.row_log, (uint64_t *)row.data()));
-    const double t5 = now_us();
+    std::vector<F162> p0(s.row_log);
+    j.row.resize(COLUMNS);
+    for (int i = 0; i < s.row_log; i++) p0[i] = j.coords[lp - 1 - i];
+    TRY(lbn_row_eval(ctx, (const uint64_t *)p0.data(), s.row_log, (uint64_t *)j.row.data()));
+    j.t5 = now_us();
 
     // RowEvaluation::absorb (scheme/commitment.rs:320-331), then 1024 challenges in parallel,
     // each one derivation at its own counter (challenge.rs:778-802).
@@ -637,54 +664,171 @@ extern "C" int lbn_prove(lbn_ctx *ctx, const uint8_t seed[32], uint8_t *proof, s
         static const char L[] = "labinius/row-evaluation";
         t.absorb_bytes(L, sizeof L - 1);
         std::vector<uint8_t> rb2((size_t)24 * COLUMNS);
-        memcpy(rb2.data(), row.data(), rb2.size());
+        memcpy(rb2.data(), j.row.data(), rb2.size());
         t.absorb_bytes(rb2.data(), rb2.size());
     }
     static const ChallengeSampler sampler;
-    std::vector<lbn_challenge> ch(COLUMNS);
-    {
-        ChalArg ca = sampler.arg;
-        t.save(ca.ts);  // challenge i reads the transcript at its counter + i
-        TRY(dev_challenges(ctx, ca, ch.data()));
-    }
-    t.skip(COLUMNS);
-    const double t6 = now_us();
+    j.ca = sampler.arg;
+    t.save(j.ca.ts);  // challenge i reads the transcript at its counter + i
+    j.ch.resize(COLUMNS);
+    return LBN_OK;
+}
 
-    TRY(lbn_fold(ctx, ch.data(), COLUMNS, (int32_t *)(proof + off_fold + 8), (size_t)N * s.ring_elements));
-    TRY(dev_commit_bytes(ctx, proof + off_commit));
+// After the folding challenges (in j.ch): the fold, the commitment bytes and the bincode.
+int prove_tail(ProveJob &j, size_t *len) {
+    lbn_ctx *ctx = j.ctx;
+    const Shape &s = ctx->shape;
+    const u128 q = ctx->q;
+    const int lp = s.log_packed;
+    Transcript &t = j.t;
+    const F162 opened = ctx->opened;
+    auto &ch = j.ch;
+    auto &row = j.row;
+    t.skip(COLUMNS);
+    const double t4 = j.t4, t5 = j.t5, t6 = now_us();
+
+    TRY(lbn_fold(ctx, ch.data(), COLUMNS, (int32_t *)(j.proof + j.off_fold + 8), (size_t)N * s.ring_elements));
+    TRY(dev_commit_bytes(ctx, j.proof + j.off_commit));
     const double t7 = now_us();
 
-    Writer w{proof};
+    Writer w{j.proof};
     const uint32_t version = 1;
     w.bytes(&version, 4);
     w.u64((uint64_t)q);
     w.u64((uint64_t)(q >> 64));
-    w.u64(clen);
-    w.at = off_partials;
+    w.u64(j.clen);
+    w.at = j.off_partials;
     w.u64(128);
-    for (auto &x : partials) w.u64(x.w[0]), w.u64(x.w[1]);
+    for (auto &x : j.partials) w.u64(x.w[0]), w.u64(x.w[1]);
     w.u64((uint64_t)lp);
-    for (auto &x : rounds)
+    for (auto &x : j.rounds)
         for (int k = 0; k < 3; k++) w.u64(x.w[k]);
     for (int k = 0; k < 3; k++) w.u64(opened.w[k]);
-    w.u64(row_bytes);
-    pack_f162(row.data(), COLUMNS, proof + w.at);
-    w.at += row_bytes;
-    w.u64(fold_bytes);
-    *len = plen;
-    quiesce.armed = false;
+    w.u64(j.row_bytes);
+    pack_f162(row.data(), COLUMNS, j.proof + w.at);
+    w.at += j.row_bytes;
+    w.u64(j.fold_bytes);
+    *len = j.plen;
+    j.launched = false;
     const double t8 = now_us();
-    pf.commit_us = t1 - t0;
-    pf.transcript_us = t2 - t1;
-    pf.partials_us = t3 - t2;
-    pf.switch_us = t4 - t3;
-    pf.row_us = t5 - t4;
-    pf.challenges_us = t6 - t5;
-    pf.fold_us = t7 - t6;
-    pf.encode_us = t8 - t7;
-    pf.total_us = t8 - t0;
-    if (prof) *prof = pf;
+    j.pf.switch_us = t4 - j.t3;
+    j.pf.row_us = t5 - t4;
+    j.pf.challenges_us = t6 - t5;
+    j.pf.fold_us = t7 - t6;
+    j.pf.encode_us = t8 - t7;
+    j.pf.total_us = t8 - j.t0;
     return LBN_OK;
+}
+}  // namespace
+
+extern "C" int lbn_prove(lbn_ctx *ctx, const uint8_t seed[32], uint8_t *proof, size_t cap, size_t *len,
+                         lbn_profile *prof) {
+    if (!ctx || !seed || !proof || !len) return LBN_E_ARG;
+    ProveJob j(ctx, seed, proof);
+    TRY(prove_front(j, cap));
+    TRY(dev_switch_rounds(ctx, j.ts, (uint64_t *)j.rounds.data(), (uint64_t *)j.coords.data(), j.ho));
+    TRY(prove_back(j));
+    TRY(dev_challenges(ctx, j.ca, j.ch.data()));
+    TRY(prove_tail(j, len));
+    if (prof) *prof = j.pf;
+    return LBN_OK;
+}
+
+// Many proofs together: the fronts on `threads` host threads (each proof on its own context and
+// streams), one batched device switch for all of them (dev_switch_batch: one launch per device
+// step), the backs on the threads again up to the folding challenges, the challenges of all in one
+// launch, then the folds and encodings. An item that fails keeps its error and leaves the others
+// going.
+extern "C" int lbn_prove_batch(lbn_batch_item *items, int n, int threads) {
+    if (!items || n < 1 || n > SWB_MAX || threads < 1) return LBN_E_ARG;
+    for (int i = 0; i < n; i++) {
+        if (!items[i].ctx || !items[i].proof) return LBN_E_ARG;
+        for (int k = 0; k < i; k++)
+            if (items[k].ctx == items[i].ctx) return LBN_E_ARG;
+        if (items[i].ctx->device != items[0].ctx->device) return LBN_E_ARG;
+        items[i].status = 1;
+        items[i].len = 0;
+    }
+    std::vector<std::unique_ptr<ProveJob>> jobs(n);
+    for (int i = 0; i < n; i++) jobs[i].reset(new ProveJob(items[i].ctx, items[i].seed, items[i].proof));
+    auto run = [&](const std::function<void(int)> &f) {
+        std::atomic<int> next{0};
+        auto worker = [&]() {
+            for (int i; (i = next.fetch_add(1)) < n;) f(i);
+        };
+        std::vector<std::thread> pool;
+        for (int k = 1; k < std::min(threads, n); k++) pool.emplace_back(worker);
+        worker();
+        for (auto &th : pool) th.join();
+    };
+    run([&](int i) {
+        lbn_batch_item &it = items[i];
+        int rc = it.gen_table ? lbn_generate_table(it.ctx, it.seed) : LBN_OK;
+        if (rc == LBN_OK) rc = prove_front(*jobs[i], it.cap);
+        it.status = rc == LBN_OK ? 1 : rc;
+    });
+    std::vector<lbn_ctx *> cs;
+    std::vector<B3State> ts;
+    std::vector<uint64_t *> rounds, coords;
+    std::vector<SwitchHandoff> ho;
+    std::vector<int> idx;
+    for (int i = 0; i < n; i++)
+        if (items[i].status == 1) {
+            idx.push_back(i);
+            cs.push_back(items[i].ctx);
+            ts.push_back(jobs[i]->ts);
+            rounds.push_back((uint64_t *)jobs[i]->rounds.data());
+            coords.push_back((uint64_t *)jobs[i]->coords.data());
+        }
+    if (!idx.empty()) {
+        ho.resize(idx.size());
+        const int rc = dev_switch_batch(cs.data(), (int)cs.size(), ts.data(), rounds.data(), coords.data(), ho.data());
+        for (size_t k = 0; k < idx.size(); k++) {
+            if (rc != LBN_OK) {
+                items[idx[k]].status = rc;
+                continue;
+            }
+            jobs[idx[k]]->ts = ts[k];
+            jobs[idx[k]]->ho = std::move(ho[k]);
+        }
+    }
+    run([&](int i) {
+        lbn_batch_item &it = items[i];
+        if (it.status != 1) return;
+        const int rc = prove_back(*jobs[i]);
+        if (rc != LBN_OK) it.status = rc;
+    });
+    {  // the folding challenges of every proof in one launch
+        std::vector<lbn_ctx *> cc;
+        std::vector<ChalArg> ca;
+        std::vector<lbn_challenge *> co;
+        std::vector<int> ci;
+        for (int i = 0; i < n; i++)
+            if (items[i].status == 1) {
+                ci.push_back(i);
+                cc.push_back(items[i].ctx);
+                ca.push_back(jobs[i]->ca);
+                co.push_back(jobs[i]->ch.data());
+            }
+        if (!ci.empty()) {
+            const int rc = dev_challenges_batch(cc.data(), (int)cc.size(), ca.data(), co.data());
+            if (rc != LBN_OK)
+                for (int i : ci) items[i].status = rc;
+        }
+    }
+    run([&](int i) {
+        lbn_batch_item &it = items[i];
+        if (it.status != 1) return;
+        const int rc = prove_tail(*jobs[i], &it.len);
+        if (rc == LBN_OK) it.profile = jobs[i]->pf;
+        it.status = rc;
+    });
+    int first = LBN_OK;
+    for (int i = 0; i < n; i++) {
+        if (first == LBN_OK && items[i].status != LBN_OK) first = items[i].status;
+        jobs[i].reset();  // a failed proof waits for its device work here
+    }
+    return first;
 }
 
 // ============================================================================= checker

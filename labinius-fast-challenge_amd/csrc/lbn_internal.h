@@ -172,6 +172,12 @@ struct SwitchHandoff {
     std::vector<B128> e;     // and its eq values (2 h2)
 };
 int dev_switch_rounds(lbn_ctx *ctx, B3State &ts, uint64_t *rounds, uint64_t *coords, SwitchHandoff &ho);
+// The same for n proofs at once (contexts on one device, each after dev_switch_prepare): one launch
+// per device step covers every instance; instance i's results are those dev_switch_rounds would
+// give its context alone.
+constexpr int SWB_MAX = 64;
+int dev_switch_batch(lbn_ctx *const *ctxs, int n, B3State *ts, uint64_t *const *rounds, uint64_t *const *coords,
+                     SwitchHandoff *ho);
 // The switch's start for dev_switch_rounds: the map L_0 from the batch (no kernel; the rounds start
 // from the prefix sets lbn_partials left).
 int dev_switch_prepare(lbn_ctx *ctx, const uint64_t *batch);
@@ -180,6 +186,8 @@ void lmap_next(std::vector<F162> &L, B128 c, const F162 &rho);
 // The 1024 folding challenges sampled on the device from the transcript state in arg (the
 // counter of arg->ts is the first challenge's); out receives them in column order.
 int dev_challenges(lbn_ctx *ctx, const ChalArg &arg, lbn_challenge *out);
+// The same for n proofs in one launch (contexts on one device): instance i from args[i] into outs[i].
+int dev_challenges_batch(lbn_ctx *const *ctxs, int n, const ChalArg *args, lbn_challenge *const *outs);
 // The streams the context's work goes to (hipStream_t, as void *): a caller driving many contexts
 // from a few threads gives each thread its own pair; null restores the context's own streams.
 // Only between entry points (every entry point leaves its streams idle).

@@ -88,6 +88,14 @@ int dev_switch_rounds(lbn_ctx *ctx, B3State &ts, uint64_t *rounds, uint64_t *coo
     ctx->round = lp;
     return LBN_OK;
 }
+int dev_switch_batch(lbn_ctx *const *ctxs, int n, B3State *ts, uint64_t *const *rounds, uint64_t *const *coords,
+                     SwitchHandoff *ho) {
+    for (int i = 0; i < n; i++) {
+        const int rc = dev_switch_rounds(ctxs[i], ts[i], rounds[i], coords[i], ho[i]);
+        if (rc != LBN_OK) return rc;
+    }
+    return LBN_OK;
+}
 int dev_challenges(lbn_ctx *, const ChalArg &arg, lbn_challenge *out) {
     for (int j = 0; j < COLUMNS; j++) {
         lbn_challenge &c = out[j];
@@ -96,6 +104,10 @@ int dev_challenges(lbn_ctx *, const ChalArg &arg, lbn_challenge *out) {
         for (int i = 0; i < WEIGHT; i++) c.positions[i] = (uint8_t)(5 * i + (j + arg.ts.counter) % 5);
         c.signs = (uint32_t)mix(j) & ((1u << WEIGHT) - 1);
     }
+    return LBN_OK;
+}
+int dev_challenges_batch(lbn_ctx *const *ctxs, int n, const ChalArg *args, lbn_challenge *const *outs) {
+    for (int i = 0; i < n; i++) dev_challenges(ctxs[i], args[i], outs[i]);
     return LBN_OK;
 }
 void dev_use_streams(lbn_ctx *, void *, void *) {}

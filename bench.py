@@ -295,12 +295,29 @@ def run_batch_api(m, jobs, threads, reps):
             p.close()
 
 
+def tune_shapes(m, ks, reps=3):
+    """lbn_autotune for each table size in ks (a context of that size with the table of bench_seed(k)):
+    the commitment's split count per shape, recorded for this process's later proofs; returns
+    {k: {"best": splits, "ms": {splits: median ms}}}."""
+    out = {}
+    for kk in ks:
+        p = m.Prover(kk)
+        try:
+            p.set_modulus(m.DEFAULT_Q)
+            p.generate_table(bench_seed(kk))
+            out[kk] = p.autotune(reps=reps)
+        finally:
+            p.close()
+    return out
+
+
 def batched(m, workers, reps=3, threads=16):
     """64 independent proofs of mixed size (k in [20, 28]): one at a time, through the scheduler with
     `workers` proofs in flight, and through the batch prover (one device switch sequence and one
     challenge launch for all 64, `threads` host threads); each way timed `reps` times (the batch is
     short), medians reported. The headline is the batch prover's."""
     jobs = batched_instances()
+    tuned = tune_shapes(m, sorted({kk for kk, _ in jobs}))  # launch shapes for every way below
     seq, t_seqs = run_jobs(m, jobs, 1, reps)
     w_seq = run_jobs.window_ns
     par, t_pars = run_jobs(m, jobs, workers, reps)
@@ -323,7 +340,8 @@ def batched(m, workers, reps=3, threads=16):
                           "proofs_per_s": round(n / t_par, 1), "speedup": round(t_seq / t_par, 2),
                           "proofs_per_s_runs": [round(n / t, 1) for t in t_pars]},
             "identical_to_sequential": all(p == seq[0] for p in seq + par + bat),
-            "verified": f"{verified}/{n}", "rejected": check.rejected}
+            "verified": f"{verified}/{n}", "rejected": check.rejected,
+            "mac_splits": {kk: v["best"] for kk, v in tuned.items()}}
 
 
 def serving_requests(requests, rate, rng_seed=20261203):
@@ -451,6 +469,8 @@ def main():
     ap.add_argument("--hw-queues", type=int, default=16,
                     help="GPU_MAX_HW_QUEUES for this process (hardware queues the proofs' streams map onto)")
     ap.add_argument("--serving", action="store_true", help="run the serving loop instead (prints its own line)")
+    ap.add_argument("--tune-only", action="store_true",
+                    help="run the launch-shape autotuner for k = 20..28 and 31 and print its table")
     ap.add_argument("--batched-only", action="store_true",
                     help="run only the batched measurement; its line also gives the monotonic-clock windows "
                          "(ns) of the sequential and the parallel batch, to cut a kernel trace to them")
@@ -486,12 +506,16 @@ def main():
     if args.commit_micro_only:
         print(json.dumps(commit_micro(m, args.steps, args.warmup)), flush=True)
         return
+    if args.tune_only:
+        print(json.dumps({"autotune": tune_shapes(m, list(range(20, 29)) + [31], reps=5)}), flush=True)
+        return
     if args.batched_only:
         b = batched(m, args.workers or 12)
         b["windows_ns"] = batched.windows_ns
         print(json.dumps(b), flush=True)
         return
     k = args.log_vars
+    tuned = tune_shapes(m, [k])[k]  # the commitment's launch shape for this size, outside the clock
     p = m.Prover(k)
     p.set_modulus(m.DEFAULT_Q)
     if args.copy_calibration:  # the library's copy kernel over 2^k/8 bytes: read once, written once
@@ -584,6 +608,7 @@ def main():
                                   else f"{primes} CRT primes"),
                    "log_vars": k, "modulus": m.DEFAULT_Q},
         "roofline": roofline,
+        "autotune": tuned,
         "passes_ms": {n: round(v["ms"], 3) for n, v in passes.items()},
         "host_phases_ms": {f.replace("_us", ""): round(statistics.mean(pr[f] for pr in profs) / 1000, 3)
                            for f in profs[0]},

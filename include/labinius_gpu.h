@@ -131,6 +131,24 @@ typedef struct {
 } lbn_batch_item;
 int lbn_prove_batch(lbn_batch_item *items, int n, int threads);
 
+/* Launch-shape autotuner, no counterpart in the reference: times the matrix-core commitment (MAC and
+ * finish) on the context's loaded table for each element split count (1, 2, 3, 4, 6, 8, 12, 24, up to
+ * the ring elements), `reps` timed launches each after one untimed, and records the fastest for this
+ * device and log_vars: later commitments of that shape on the device launch with it (a deferred
+ * upload keeps 24). Launch shape only: the commitment bytes are the same for every count. The
+ * modulus must commit on the matrix cores (below 2^39). */
+typedef struct {
+    int log_vars;
+    int n;           /* split counts timed */
+    int splits[8];
+    double ms[8];    /* median MAC + finish time of each, milliseconds */
+    int best;        /* the split count recorded */
+} lbn_tune_result;
+int lbn_autotune(lbn_ctx *ctx, int reps, lbn_tune_result *out);
+/* Records a split count for (device, log_vars), for example one an earlier lbn_autotune chose;
+ * 0 restores the default. */
+int lbn_tune_set(int device, int log_vars, int mac_splits);
+
 /* The checker (verifier/proof_verify/src/main.rs:381-389 verify, with check_proof_shape,
  * check_field_switch and check_opening, main.rs:293-379): accepts `proof` (bincode bytes) for the
  * trace of `seed` at the context's shape under the modulus the proof carries. Returns LBN_OK,

@@ -48,6 +48,11 @@ class BatchItem(ctypes.Structure):
                 ("status", ctypes.c_int), ("profile", Profile)]
 
 
+class TuneResult(ctypes.Structure):
+    _fields_ = [("log_vars", ctypes.c_int), ("n", ctypes.c_int), ("splits", ctypes.c_int * 8),
+                ("ms", ctypes.c_double * 8), ("best", ctypes.c_int)]
+
+
 class Timing(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in ("table_ms", "eq_ms", "partials_ms", "switch0_ms", "switch_rounds_ms",
                                               "row_ms", "fold_ms", "commit_mac_ms", "commit_finish_ms")] + \
@@ -99,6 +104,8 @@ def lib():
         L.lbn_prove.argtypes = [P, P, P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t), P]
         L.lbn_verify.argtypes = [P, P, P, ctypes.c_size_t]
         L.lbn_prove_batch.argtypes = [P, ctypes.c_int, ctypes.c_int]
+        L.lbn_autotune.argtypes = [P, ctypes.c_int, P]
+        L.lbn_tune_set.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.lbn_ajtai_elements.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_size_t,
                                          ctypes.c_size_t, ctypes.c_size_t, P]
         L.lbn_commit_primes.argtypes = [P]
@@ -174,6 +181,13 @@ def prove_batch(provers, seeds, outs, gen_table=True, threads=12):
         items[i].cap = out.size
     lib().lbn_prove_batch(ctypes.cast(items, ctypes.c_void_p), n, threads)
     return [(items[i].len, items[i].status) for i in range(n)]
+
+
+def tune_set(log_vars: int, mac_splits: int, device: int = 0):
+    """lbn_tune_set: record a commitment split count for (device, log_vars); 0 restores the default."""
+    rc = lib().lbn_tune_set(device, log_vars, mac_splits)
+    if rc != 0:
+        raise LbnError(rc, "lbn_tune_set failed")
 
 
 def coefficient_bytes(q: int) -> int:
@@ -274,6 +288,13 @@ class Prover:
         out = np.zeros((self.ring_elements, 648), dtype=np.int32)
         self._check(lib().lbn_fold(self.h, ctypes.cast(challenges, ctypes.c_void_p), 1024, _p(out), out.size))
         return out
+
+    def autotune(self, reps: int = 3) -> dict:
+        """lbn_autotune on the loaded table: {"best": splits, "ms": {splits: median ms of MAC + finish}};
+        the choice is recorded for this device and shape."""
+        r = TuneResult()
+        self._check(lib().lbn_autotune(self.h, reps, ctypes.addressof(r)))
+        return {"best": r.best, "ms": {int(r.splits[i]): round(r.ms[i], 4) for i in range(r.n)}}
 
     def prove_into(self, seed: bytes, out: np.ndarray):
         """lbn_prove into a caller-owned uint8 array (at least proof_len bytes); returns the proof

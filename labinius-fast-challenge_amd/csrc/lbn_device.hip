@@ -85,6 +85,7 @@ struct lbn::DeviceState {
     // (device, Q, shape), and its partial sums.
     std::shared_ptr<::ToepOps> toep;
     int64_t *tpart = nullptr;
+    int splits_try = 0;  // the MAC's split count while the autotuner times it (0: the chosen one)
     // A host table whose upload is deferred (lbn_load_table_async): the commitment copies it in
     // ring-element ranges, each range's MAC waiting for its copy (events on the copy stream).
     const uint8_t *pend = nullptr;
@@ -3576,6 +3577,10 @@ static bool toep_modulus(u128 q) { return q < ((u128)1 << 39); }
 // Ring-element splits of the GEMM (42 blocks each): 6 with the table resident (252 blocks, one
 // round over 256 CUs), 24 with a deferred upload (four launches of 6 splits, one per copied range).
 static int toep_splits(size_t re, bool deferred) { return (int)std::min<size_t>(re, deferred ? 24 : 6); }
+// Split counts chosen by the autotuner (lbn_autotune / lbn_tune_set) per (device, log_vars); a shape
+// without an entry uses toep_splits.
+static std::mutex tune_mu;
+static std::map<std::pair<int, int>, int> tune_splits;
 static size_t toep_ops_bytes(size_t re) { return re * TW_ELEM_U32 * 4; }
 static size_t toep_part_bytes(size_t re) { return (size_t)toep_splits(re, true) * TP_ROWS * COLUMNS * 8; }
 
@@ -3863,6 +3868,68 @@ extern "C" int lbn_commit(lbn_ctx *ctx, uint8_t *out, size_t out_len) {
     return LBN_OK;
 }
 
+// Launch-shape autotuner: the matrix-core commitment (MAC + finish) timed on the context's loaded table
+// for each split count, the fastest recorded for (device, log_vars). The result is a launch shape
+// only: every split count gives the same commitment bytes.
+extern "C" int lbn_autotune(lbn_ctx *ctx, int reps, lbn_tune_result *out) {
+    if (!ctx || !out || reps < 1 || reps > 100) return LBN_E_ARG;
+    if (!ctx->table_loaded || !ctx->modulus_set) return ctx->fail(LBN_E_STATE, "table and modulus must be loaded");
+    DeviceState *d = ctx->dev;
+    if (!d->toep) return ctx->fail(LBN_E_STATE, "the modulus commits through the CRT transform (nothing to tune)");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    {
+        const int rc = table_ready(ctx);  // a deferred upload completes first
+        if (rc != LBN_OK) return rc;
+    }
+    *out = lbn_tune_result{};
+    out->log_vars = ctx->shape.log_vars;
+    const int cap = toep_splits(ctx->shape.ring_elements, true);
+    static const int cand[8] = {1, 2, 3, 4, 6, 8, 12, 24};
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIPCHK(ctx, hipEventCreate(&e0));
+    HIPCHK(ctx, hipEventCreate(&e1));
+    int rc = LBN_OK;
+    double best_ms = 1e30;
+    for (int c = 0; c < 8 && rc == LBN_OK; c++) {
+        if (cand[c] > cap) break;
+        d->splits_try = cand[c];
+        std::vector<float> ts;
+        for (int r = 0; r <= reps && rc == LBN_OK; r++) {  // the first launch is not timed
+            if (hipEventRecord(e0, d->stream) != hipSuccess) rc = ctx->fail(LBN_E_DEVICE, "event record failed");
+            if (rc == LBN_OK) rc = dev_commit_launch(ctx);
+            if (rc == LBN_OK && hipEventRecord(e1, d->stream) != hipSuccess) rc = ctx->fail(LBN_E_DEVICE, "event record failed");
+            float ms = 0;
+            if (rc == LBN_OK && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+                rc = ctx->fail(LBN_E_DEVICE, "event timing failed");
+            if (rc == LBN_OK && r > 0) ts.push_back(ms);
+        }
+        if (rc != LBN_OK) break;
+        std::sort(ts.begin(), ts.end());
+        const double med = ts[ts.size() / 2];
+        out->splits[out->n] = cand[c];
+        out->ms[out->n] = med;
+        out->n++;
+        if (med < best_ms) best_ms = med, out->best = cand[c];
+    }
+    d->splits_try = 0;
+    (void)hipStreamSynchronize(d->stream);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc != LBN_OK) return rc;
+    std::lock_guard<std::mutex> lk(tune_mu);
+    tune_splits[{ctx->device, ctx->shape.log_vars}] = out->best;
+    return LBN_OK;
+}
+
+extern "C" int lbn_tune_set(int device, int log_vars, int mac_splits) {
+    Shape s;
+    if (!Shape::of(log_vars, s) || mac_splits < 0 || mac_splits > 24) return LBN_E_ARG;
+    std::lock_guard<std::mutex> lk(tune_mu);
+    if (mac_splits == 0) tune_splits.erase({device, log_vars});
+    else tune_splits[{device, log_vars}] = mac_splits;
+    return LBN_OK;
+}
+
 int lbn::dev_commit_cvs(lbn_ctx *ctx, size_t off, uint64_t first, size_t count, const Blake3::Subtrees &trees,
                         const uint32_t key[8], uint32_t flags, uint32_t *cvs, uint8_t *dst, size_t head, size_t tail) {
     DeviceState *d = ctx->dev;
@@ -3973,6 +4040,18 @@ static CrtArg make_crt(const PrimeInfo *primes, const Roots18 *roots, int np, u1
     return crt;
 }
 
+// The MAC's element splits for this commitment: 24 with a deferred upload (four launches of 6), else
+// the autotuner's choice for the shape (or the one being timed), else toep_splits.
+static int commit_splits(lbn_ctx *ctx) {
+    const size_t re = ctx->shape.ring_elements;
+    const int cap = toep_splits(re, true);  // the partial sums hold this many
+    if (ctx->dev->pend) return cap;
+    if (ctx->dev->splits_try > 0) return std::min(ctx->dev->splits_try, cap);
+    std::lock_guard<std::mutex> lk(tune_mu);
+    const auto it = tune_splits.find({ctx->device, ctx->shape.log_vars});
+    return it != tune_splits.end() ? std::max(1, std::min(it->second, cap)) : toep_splits(re, false);
+}
+
 int lbn::dev_commit_launch(lbn_ctx *ctx) {
     if (!ctx->table_loaded || !ctx->modulus_set) return ctx->fail(LBN_E_STATE, "table and modulus must be loaded");
     const Shape &s = ctx->shape;
@@ -3982,7 +4061,7 @@ int lbn::dev_commit_launch(lbn_ctx *ctx) {
     DeviceState *d = ctx->dev;
     const int re = (int)s.ring_elements;
     if (d->toep) {
-        const int sp = toep_splits((size_t)re, d->pend != nullptr);
+        const int sp = commit_splits(ctx);
         // a deferred upload goes in (at most) four ring-element ranges, each range's MAC after its copy
         const int groups = d->pend ? std::min(4, sp) : 1;
         tstart(ctx, T_MAC);

@@ -203,6 +203,27 @@ def batched_instances(n=64, first=100):
     return [(20 + bench_seed(first + i)[0] % 9, bench_seed(first + i)) for i in range(n)]
 
 
+class ProofArena:
+    """Host buffers for a list of proofs, allocated and written once before any clock so that no
+    timed proof pays page faults on fresh memory: one PinnedBuffer (pinned=True: the proofs' copies
+    write it directly) or one pageable array, cut into one 256-byte-aligned slice per proof. The
+    paths compared on a workload get the same arena; proofs are taken out as bytes before close()."""
+
+    def __init__(self, m, sizes, pinned):
+        offs = [0]
+        for n in sizes:
+            offs.append(offs[-1] + (n + 255) // 256 * 256)
+        self.pinned = m.PinnedBuffer(offs[-1]) if pinned else None
+        arr = self.pinned.array if pinned else np.empty(offs[-1], dtype=np.uint8)
+        arr[:] = 0
+        self.views = [arr[offs[i]:offs[i] + n] for i, n in enumerate(sizes)]
+
+    def close(self):
+        self.views = None
+        if self.pinned:
+            self.pinned.close()
+
+
 def warm(sch, ks, workers):
     """Context set-up outside the clock: `workers` contexts of each size, then a round of proofs
     (first launches of every kernel)."""
@@ -214,9 +235,10 @@ def warm(sch, ks, workers):
         sch.result(h)
 
 
-def run_jobs(m, jobs, workers, reps=1, budget=200 << 30):
+def run_jobs(m, jobs, workers, arena, reps=1, budget=200 << 30):
     """All jobs submitted at once to a scheduler with `workers` proofs in flight (contexts warmed
-    first), timed `reps` times on the same scheduler; returns (proofs of every run, seconds per run)."""
+    first), timed `reps` times on the same scheduler, job i's proof into arena.views[i]; returns
+    (proofs of every run, seconds per run)."""
     sch = m.Scheduler(m.DEFAULT_Q, hbm_budget=budget, workers=workers)
     try:
         warm(sch, sorted({kk for kk, _ in jobs}), workers)
@@ -224,7 +246,7 @@ def run_jobs(m, jobs, workers, reps=1, budget=200 << 30):
         for rep in range(reps):
             w0 = time.monotonic_ns()
             t0 = time.perf_counter()
-            hs = [sch.submit(kk, seed) for kk, seed in jobs]
+            hs = [sch.submit(kk, seed, out=out) for (kk, seed), out in zip(jobs, arena.views)]
             sch.drain()
             times.append(time.perf_counter() - t0)
             if rep == 0:  # the first timed batch, on the kernel tracer's clock
@@ -265,10 +287,10 @@ class Checker:
         self.ctx.clear()
 
 
-def run_batch_api(m, jobs, threads, reps):
+def run_batch_api(m, jobs, threads, arena, reps):
     """The batch prover (lbn_prove_batch): one context per instance, created with its modulus outside
     the clock, then `reps` timed batches (each generates its tables on the device, as the scheduler's
-    jobs do); returns (proofs of every run, seconds per run)."""
+    jobs do), proof i into arena.views[i]; returns (proofs of every run, seconds per run)."""
     provers = []
     try:
         for kk, _ in jobs:
@@ -276,7 +298,7 @@ def run_batch_api(m, jobs, threads, reps):
             p.set_modulus(m.DEFAULT_Q)
             provers.append(p)
         seeds = [seed for _, seed in jobs]
-        outs = [np.zeros(m.proof_len(kk, m.DEFAULT_Q), dtype=np.uint8) for kk, _ in jobs]
+        outs = arena.views
         res = m.prove_batch(provers, seeds, outs, threads=threads)  # first launches of every kernel
         assert all(st == 0 for _, st in res), res
         proofs, times = [], []
@@ -313,16 +335,22 @@ def tune_shapes(m, ks, reps=3):
 
 def batched(m, workers, reps=3, threads=16):
     """64 independent proofs of mixed size (k in [20, 28]): one at a time, through the scheduler with
-    `workers` proofs in flight, and through the batch prover (one device switch sequence and one
-    challenge launch for all 64, `threads` host threads); each way timed `reps` times (the batch is
-    short), medians reported. The headline is the batch prover's."""
+    `workers` proofs in flight, and through the batch prover (lbn_prove_batch on `threads` host
+    threads: the commitments per proof, then one device sequence for all switches and one launch each
+    for all rows, challenges and folds, in two staggered groups); each way timed `reps` times (the
+    batch is short), medians reported, proofs into the same pinned buffers. The headline is the
+    batch prover's."""
     jobs = batched_instances()
     tuned = tune_shapes(m, sorted({kk for kk, _ in jobs}))  # launch shapes for every way below
-    seq, t_seqs = run_jobs(m, jobs, 1, reps)
-    w_seq = run_jobs.window_ns
-    par, t_pars = run_jobs(m, jobs, workers, reps)
-    w_par = run_jobs.window_ns
-    bat, t_bats = run_batch_api(m, jobs, threads, reps)
+    arena = ProofArena(m, [m.proof_len(kk, m.DEFAULT_Q) for kk, _ in jobs], pinned=True)
+    try:  # the same pinned proof buffers for the three ways
+        seq, t_seqs = run_jobs(m, jobs, 1, arena, reps)
+        w_seq = run_jobs.window_ns
+        par, t_pars = run_jobs(m, jobs, workers, arena, reps)
+        w_par = run_jobs.window_ns
+        bat, t_bats = run_batch_api(m, jobs, threads, arena, reps)
+    finally:
+        arena.close()
     batched.windows_ns = {"sequential": list(w_seq), "scheduler": list(w_par), "batch": list(run_batch_api.window_ns)}
     t_seq, t_par, t_bat = statistics.median(t_seqs), statistics.median(t_pars), statistics.median(t_bats)
     check = Checker(m)
@@ -341,6 +369,7 @@ def batched(m, workers, reps=3, threads=16):
                           "proofs_per_s_runs": [round(n / t, 1) for t in t_pars]},
             "identical_to_sequential": all(p == seq[0] for p in seq + par + bat),
             "verified": f"{verified}/{n}", "rejected": check.rejected,
+            "proof_buffers": "one pinned arena (lbn_host_alloc), the same for the three ways",
             "mac_splits": {kk: v["best"] for kk, v in tuned.items()}}
 
 
@@ -358,59 +387,96 @@ def serving_requests(requests, rate, rng_seed=20261203):
     return reqs
 
 
+SERVING_KS = list(range(20, 32))
+
+
+def serving_arena(m, reqs):
+    """Pageable proof buffers for the serving workload (written once up front; several GB, so not
+    pinned), shared by the ways it is served."""
+    return ProofArena(m, [m.proof_len(kk, m.DEFAULT_Q) for _, kk, _ in reqs], pinned=False)
+
+
+def serve(m, reqs, w, budget, arena, check=None):
+    """The arrival sequence `reqs` served by a scheduler with `w` proofs in flight (contexts of every
+    size warmed first), request i's proof into arena.views[i]; latency = completion - arrival. With
+    `check`, every proof goes through the checker after the clock. Returns (figures, the first 32
+    proofs)."""
+    sch = m.Scheduler(m.DEFAULT_Q, hbm_budget=budget, workers=w)
+    try:
+        warm(sch, SERVING_KS, w)
+        hs = []
+        t0 = time.perf_counter()
+        for (at, kk, seed), out in zip(reqs, arena.views):
+            delay = at - (time.perf_counter() - t0)
+            if delay > 0:
+                time.sleep(delay)
+            lag = (time.perf_counter() - t0) - at  # submission behind the scheduled arrival
+            hs.append((lag, sch.submit(kk, seed, out=out)))
+        sch.drain()
+        wall = time.perf_counter() - t0
+        lat = sorted(lag * 1000 + (h.queue_us + h.run_us) / 1000 for lag, h in hs)
+        st = sch.stats()
+        proofs = [sch.result(h)[0] for _, h in hs[:32]]  # a sample, for the isolation check
+        out = {"p50_ms": round(statistics.median(lat), 3), "p99_ms": round(lat[int(0.99 * (len(lat) - 1))], 3),
+               "proofs_per_s": round(len(hs) / wall, 1), "wall_s": round(wall, 2),
+               "contexts": st["contexts_created"], "evictions": st["evictions"],
+               "peak_gib": round(st["peak_bytes"] / 2**30, 2)}
+        if check:  # every proof through the checker after the clock, each released once checked
+            def every_proof():
+                yield from proofs
+                for _, h in hs[32:]:
+                    yield sch.result(h)[0]
+            ok = sum(check(kk, seed, p) for (_, kk, seed), p in zip(reqs, every_proof()))
+            out["verified"] = f"{ok}/{len(hs)}"
+            # An honest fold can cross the cap (its sum of squares has a long tail): the proof is
+            # still the reference algorithm's bytes, and the checker rejects it as the
+            # reference's does. Each rejection is listed with its reason and fill.
+            out["rejected"] = check.rejected
+        return out, proofs
+    finally:
+        sch.close()
+
+
 def serving(m, requests, rate, workers, budget):
     """Poisson arrivals of proving requests, k in {20..31} with P(k) ~ 2^-(k-20), fresh seed per
-    request; latency = completion - arrival. The same arrival sequence is served by `workers`
-    proofs in flight and by one at a time."""
+    request. The same arrival sequence is served by `workers` proofs in flight and by one at a time."""
     reqs = serving_requests(requests, rate)
-    ks = list(range(20, 32))
-
-    def serve(w, check=None):
-        sch = m.Scheduler(m.DEFAULT_Q, hbm_budget=budget, workers=w)
-        try:
-            warm(sch, ks, w)
-            hs = []
-            t0 = time.perf_counter()
-            for at, kk, seed in reqs:
-                delay = at - (time.perf_counter() - t0)
-                if delay > 0:
-                    time.sleep(delay)
-                lag = (time.perf_counter() - t0) - at  # submission behind the scheduled arrival
-                hs.append((lag, sch.submit(kk, seed)))
-            sch.drain()
-            wall = time.perf_counter() - t0
-            lat = sorted(lag * 1000 + (h.queue_us + h.run_us) / 1000 for lag, h in hs)
-            st = sch.stats()
-            proofs = [sch.result(h)[0] for _, h in hs[:32]]  # a sample, for the isolation check
-            out = {"p50_ms": round(statistics.median(lat), 3), "p99_ms": round(lat[int(0.99 * (len(lat) - 1))], 3),
-                   "proofs_per_s": round(len(hs) / wall, 1), "wall_s": round(wall, 2),
-                   "contexts": st["contexts_created"], "evictions": st["evictions"],
-                   "peak_gib": round(st["peak_bytes"] / 2**30, 2)}
-            if check:  # every proof through the checker after the clock, each released once checked
-                def every_proof():
-                    yield from proofs
-                    for _, h in hs[32:]:
-                        yield sch.result(h)[0]
-                ok = sum(check(kk, seed, p) for (_, kk, seed), p in zip(reqs, every_proof()))
-                out["verified"] = f"{ok}/{len(hs)}"
-                # An honest fold can cross the cap (its sum of squares has a long tail): the proof is
-                # still the reference algorithm's bytes, and the checker rejects it as the
-                # reference's does. Each rejection is listed with its reason and fill.
-                out["rejected"] = check.rejected
-            return out, proofs
-        finally:
-            sch.close()
-
+    arena = serving_arena(m, reqs)
     check = Checker(m)
     try:
-        mix, proofs_mix = serve(workers, check)
+        mix, proofs_mix = serve(m, reqs, workers, budget, arena, check)
+        check.close()
+        seq, proofs_seq = serve(m, reqs, 1, budget, arena)
     finally:
         check.close()
-    seq, proofs_seq = serve(1)
+        arena.close()
     return {"metric": "serving p50/p99 ms and proofs/s on 1 MI355X", "requests": requests, "rate_per_s": rate,
             "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")),
             "workers": workers, "hbm_budget_gib": budget / 2**30, "batched": mix, "sequential": seq,
-            "identical_sample": proofs_mix == proofs_seq}
+            "identical_sample": proofs_mix == proofs_seq,
+            "proof_buffers": "one pageable arena written before the clock, the same for both ways"}
+
+
+def serving_sweep(m, requests, rates, workers, budget):
+    """The serving workload at rising arrival rates, `requests` requests per rate, served with
+    `workers` proofs in flight and one at a time: where each saturates (throughput stops following the
+    arrival rate and latency turns into queueing)."""
+    rows = []
+    for rate in rates:
+        reqs = serving_requests(requests, rate)
+        arena = serving_arena(m, reqs)
+        try:
+            mix, proofs_mix = serve(m, reqs, workers, budget, arena)
+            seq, proofs_seq = serve(m, reqs, 1, budget, arena)
+        finally:
+            arena.close()
+        rows.append({"rate_per_s": rate, "batched": mix, "sequential": seq,
+                     "identical_sample": proofs_mix == proofs_seq})
+        print(json.dumps({"sweep_row": rows[-1]}), file=sys.stderr, flush=True)
+    return {"metric": "serving p50/p99 ms and proofs/s against the arrival rate on 1 MI355X",
+            "requests_per_rate": requests, "workers": workers,
+            "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")), "hbm_budget_gib": budget / 2**30,
+            "rows": rows}
 
 
 def commit_micro(m, steps, warmup):
@@ -476,6 +542,9 @@ def main():
                          "(ns) of the sequential and the parallel batch, to cut a kernel trace to them")
     ap.add_argument("--requests", type=int, default=10_000)
     ap.add_argument("--rate", type=float, default=400.0, help="serving arrivals per second")
+    ap.add_argument("--serving-sweep", default=None, metavar="RATES",
+                    help="comma-separated arrival rates: the serving loop at each (--requests per rate, no "
+                         "checker), batched and one at a time (prints its own line)")
     ap.add_argument("--pmc-dir", default=None,
                     help="directory holding FETCH_SIZE and WRITE_SIZE rocprofv3 --pmc runs of this bench with "
                          "--copy-calibration (subdirectories fetch/ and write/): fills roofline.traffic")
@@ -495,13 +564,18 @@ def main():
     os.environ["GPU_MAX_HW_QUEUES"] = str(args.hw_queues)
     # child processes first: nothing in this process has touched the GPU yet
     proc_cpu, proc_gpu = None, None
-    if not (args.serving or args.commit_micro_only or args.batched_only or args.no_process):
+    if not (args.serving or args.serving_sweep or args.tune_only or args.commit_micro_only or args.batched_only
+            or args.no_process):
         proc_cpu, proc_gpu = process_runs(args.log_vars, args.cpu_seeds, args.process_seeds,
                                           not args.no_cpu_baseline)
     import amd
     m = amd.load()
     if args.serving:
         print(json.dumps(serving(m, args.requests, args.rate, args.workers or 8, 200 << 30)), flush=True)
+        return
+    if args.serving_sweep:
+        rates = [float(x) for x in args.serving_sweep.split(",")]
+        print(json.dumps(serving_sweep(m, args.requests, rates, args.workers or 8, 200 << 30)), flush=True)
         return
     if args.commit_micro_only:
         print(json.dumps(commit_micro(m, args.steps, args.warmup)), flush=True)

@@ -386,9 +386,11 @@ class Scheduler:
         self.h = h
         self._live = {}
 
-    def submit(self, log_vars: int, seed: bytes, table: bytes = None):
+    def submit(self, log_vars: int, seed: bytes, table: bytes = None, out: np.ndarray = None):
         """Queues one proof; returns a handle for wait()/result(). The table (2^log_vars / 8 bytes)
-        is generated on the device from the seed when not given."""
+        is generated on the device from the seed when not given. The proof goes into `out` (a
+        writable contiguous uint8 array of at least proof_len bytes, for example a slice of a
+        PinnedBuffer, which the proof's copies write directly) when given, else into a new array."""
         job = Job()
         job.log_vars = log_vars
         ctypes.memmove(job.seed, seed, 32)
@@ -397,7 +399,12 @@ class Scheduler:
             buf = ctypes.create_string_buffer(table, len(table))
             keep.append(buf)
             job.table = ctypes.cast(buf, ctypes.c_void_p)
-        out = np.empty(proof_len(log_vars, self.q), dtype=np.uint8)
+        if out is None:
+            out = np.empty(proof_len(log_vars, self.q), dtype=np.uint8)
+        elif out.dtype != np.uint8 or not out.flags.c_contiguous or not out.flags.writeable:
+            raise ValueError("out must be a writable contiguous uint8 array")
+        elif out.size < proof_len(log_vars, self.q):
+            raise ValueError("out is shorter than the proof")
         keep.append(out)
         job.proof = out.ctypes.data
         job.cap = out.size

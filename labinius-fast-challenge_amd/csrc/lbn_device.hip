@@ -1576,9 +1576,9 @@ __global__ __launch_bounds__(BIND_THREADS) void k_switch_bind(const uint4 *__res
 // ---------------------------------------------------------------------------------------------
 // The eq table of p0 as the products lo[i mod 2^lo_bits] hi[i >> lo_bits]: block h builds the
 // window table of y -> hi[h] y and its 2^lo_bits threads take one product each.
-__global__ __launch_bounds__(256) void k_eq_row(const uint64_t *lo, const uint64_t *hi, int lo_bits, uint32_t *eqrow) {
-    __shared__ uint32_t tab[RHO_WINDOWS * 16 * 6], basis[162 * 6];
-    const uint64_t *h = hi + 3 * blockIdx.x;
+__device__ __forceinline__ void eq_row_body(int hb, const uint64_t *lo, const uint64_t *hi, int lo_bits,
+                                            uint32_t *eqrow, uint32_t *tab, uint32_t *basis) {
+    const uint64_t *h = hi + 3 * hb;
     build_rho_table(RhoArg{{h[0], h[1], h[2]}}, basis, tab);
     const int t = threadIdx.x;
     if (t >= (1 << lo_bits)) return;
@@ -1587,20 +1587,28 @@ __global__ __launch_bounds__(256) void k_eq_row(const uint64_t *lo, const uint64
                            (uint32_t)(a[1] >> 32), (uint32_t)a[2], (uint32_t)(a[2] >> 32)};
     uint32_t r[6];
     rho_mul<RHO_WINDOWS>(tab, y, r);
-    uint32_t *o = eqrow + 6 * (((size_t)blockIdx.x << lo_bits) + t);
+    uint32_t *o = eqrow + 6 * (((size_t)hb << lo_bits) + t);
 #pragma unroll
     for (int k = 0; k < 6; k++) o[k] = r[k];
+}
+__global__ __launch_bounds__(256) void k_eq_row(const uint64_t *lo, const uint64_t *hi, int lo_bits, uint32_t *eqrow) {
+    __shared__ uint32_t tab[RHO_WINDOWS * 16 * 6], basis[162 * 6];
+    eq_row_body(blockIdx.x, lo, hi, lo_bits, eqrow, tab, basis);
 }
 
 // Block c: U_b = sum_i bit_b(w_{c,i}) eq_i for b < 128 (M4RM over the column), then in the same block
 // row[c] = sum_b x^b U_b (mod x^162 + x^81 + 1), one b per thread and an XOR reduction.
-__global__ __launch_bounds__(256) void k_row(const uint4 *table, const uint32_t *eqrow, size_t clen, uint64_t *row) {
-    __shared__ __align__(16) uint32_t lds[4 * M4W<6>::WORDS];
-    __shared__ uint32_t s_u[128 * 6];
-    __shared__ uint64_t part[128 * 3];
+struct RowLds {
+    __align__(16) uint32_t lds[4 * M4W<6>::WORDS];
+    uint32_t s_u[128 * 6];
+    uint64_t part[128 * 3];
+};
+__device__ __forceinline__ void row_body(RowLds &R, size_t c, const uint4 *table, const uint32_t *eqrow, size_t clen,
+                                         uint64_t *row) {
+    uint32_t *lds = R.lds, *s_u = R.s_u;
+    uint64_t *part = R.part;
     uint32_t *W = lds + (threadIdx.x >> 6) * M4W<6>::WORDS;
     uint32_t acc[2][6] = {};
-    const size_t c = blockIdx.x;
     // two items per pass, both items' loads issued first
     for (size_t base = 0; base < clen; base += 512) {
         uint32_t sel[2][4], val[2][6];
@@ -1647,6 +1655,33 @@ __global__ __launch_bounds__(256) void k_row(const uint4 *table, const uint32_t 
         __syncthreads();
     }
     if (b < 3) row[c * 3 + b] = part[b];
+}
+__global__ __launch_bounds__(256) void k_row(const uint4 *table, const uint32_t *eqrow, size_t clen, uint64_t *row) {
+    __shared__ RowLds R;
+    row_body(R, blockIdx.x, table, eqrow, clen, row);
+}
+
+// The row evaluations of several proofs in two launches: the eq tables (proof i's 2^hi_bits blocks
+// from eq_start[i]) and the rows (1024 blocks per proof, one per column).
+struct RowView {
+    const uint4 *table;
+    size_t clen;
+    const uint64_t *lo, *hi;
+    int lo_bits, eq_start;
+    uint32_t *eqrow;
+    uint64_t *row;
+};
+__global__ __launch_bounds__(256) void k_meq_row(const RowView *v, int n) {
+    __shared__ uint32_t tab[RHO_WINDOWS * 16 * 6], basis[162 * 6];
+    int i = 0;
+    while (i + 1 < n && (int)blockIdx.x >= v[i + 1].eq_start) i++;
+    const RowView &x = v[i];
+    eq_row_body((int)blockIdx.x - x.eq_start, x.lo, x.hi, x.lo_bits, x.eqrow, tab, basis);
+}
+__global__ __launch_bounds__(256) void k_mrow(const RowView *v) {
+    __shared__ RowLds R;
+    const RowView &x = v[blockIdx.x / COLUMNS];
+    row_body(R, blockIdx.x % COLUMNS, x.table, x.eqrow, x.clen, x.row);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1722,18 +1757,27 @@ __device__ __forceinline__ void ctr_add8(uint32_t (&c)[NB], const uint32_t (&x)[
 constexpr int FOLD_THREADS = 704;  // 11 waves for the 648 (pos, k) tasks
 constexpr int FOLD_MAX_ENTRIES = 1024 * 32;
 
+struct FoldLds {
+    uint4 w[1024 * 4];
+    uint16_t cols[FOLD_MAX_ENTRIES];
+    int off[2 * 162 + 1 + 162];  // list offsets, then the positions in task order
+    int32_t raw[4 * 292];
+};
+
+// Ring elements m = m0, m0 + step, ... below m1 of one table; the lists are staged first. Every
+// element ends with a block barrier, so a block may run the body again for another table.
 template <int NB>
-__global__ __launch_bounds__(FOLD_THREADS) void k_fold(const uint4 *table, size_t clen, size_t re, const int *off,
-                                                       const uint16_t *cols, int n_entries, int32_t *out,
-                                                       uint32_t *flag) {
-    __shared__ uint4 s_w[1024 * 4];
-    __shared__ uint16_t s_cols[FOLD_MAX_ENTRIES];
-    __shared__ int s_off[2 * 162 + 1 + 162];  // list offsets, then the positions in task order
-    __shared__ int32_t s_raw[4 * 292];
+__device__ __forceinline__ void fold_body(FoldLds &L, const uint4 *table, size_t clen, size_t m0, size_t m1,
+                                          size_t step, const int *off, const uint16_t *cols, int n_entries,
+                                          int32_t *out, uint32_t *flag) {
+    uint4 *s_w = L.w;
+    uint16_t *s_cols = L.cols;
+    int *s_off = L.off;
+    int32_t *s_raw = L.raw;
     const int tid = threadIdx.x;
     for (int i = tid; i < n_entries; i += FOLD_THREADS) s_cols[i] = cols[i];
     for (int i = tid; i < 2 * 162 + 1 + 162; i += FOLD_THREADS) s_off[i] = off[i];
-    for (size_t m = blockIdx.x; m < re; m += gridDim.x) {
+    for (size_t m = m0; m < m1; m += step) {
         for (int idx = tid; idx < 1024 * 4; idx += FOLD_THREADS) {
             const int j = idx >> 2, k = idx & 3;
             s_w[idx] = table[(size_t)j * clen + 4 * m + k];
@@ -1841,6 +1885,37 @@ __global__ __launch_bounds__(FOLD_THREADS) void k_fold(const uint4 *table, size_
             if (v < -32768 || v > 32767) *(volatile uint32_t *)flag = 1u;  // host memory: a plain store
         }
         __syncthreads();
+    }
+}
+template <int NB>
+__global__ __launch_bounds__(FOLD_THREADS) void k_fold(const uint4 *table, size_t clen, size_t re, const int *off,
+                                                       const uint16_t *cols, int n_entries, int32_t *out,
+                                                       uint32_t *flag) {
+    __shared__ FoldLds L;
+    fold_body<NB>(L, table, clen, blockIdx.x, re, gridDim.x, off, cols, n_entries, out, flag);
+}
+
+// The folds of several proofs in one launch: their ring elements in one sequence (proof i's
+// elements from first[i]), each block a contiguous range of `per` of them, so the blocks' work is
+// even whatever the proofs' sizes; a block that crosses into the next proof stages its lists.
+struct FoldView {
+    const uint4 *table;
+    size_t clen, re, first;
+    const int *off;
+    const uint16_t *cols;
+    int n_entries;
+    int32_t *out;
+    uint32_t *flag;
+};
+template <int NB>
+__global__ __launch_bounds__(FOLD_THREADS) void k_mfold(const FoldView *v, int n, size_t per) {
+    __shared__ FoldLds L;
+    const size_t b0 = (size_t)blockIdx.x * per, b1 = b0 + per;
+    for (int i = 0; i < n; i++) {
+        const FoldView x = v[i];
+        if (b1 <= x.first || b0 >= x.first + x.re) continue;
+        const size_t lo = b0 > x.first ? b0 - x.first : 0, hi = b1 - x.first < x.re ? b1 - x.first : x.re;
+        fold_body<NB>(L, x.table, x.clen, lo, hi, 1, x.off, x.cols, x.n_entries, x.out, x.flag);
     }
 }
 
@@ -4665,28 +4740,50 @@ struct SwBatchDev {
     hipStream_t stream = nullptr;
     SwView *views = nullptr;
     struct ChalPtrs *chal = nullptr;
+    RowView *rows = nullptr;
+    FoldView *folds = nullptr;
 };
 static int swb_ready(lbn_ctx *ctx, SwBatchDev &bd) {
     if (bd.stream) return LBN_OK;
-    HIPCHK(ctx, hipStreamCreateWithFlags(&bd.stream, hipStreamNonBlocking));
     HIPCHK(ctx, hipMalloc(&bd.views, sizeof(SwView) * SWB_MAX));
     HIPCHK(ctx, hipMalloc(&bd.chal, sizeof(ChalPtrs) * SWB_MAX));
+    HIPCHK(ctx, hipMalloc(&bd.rows, sizeof(RowView) * SWB_MAX));
+    HIPCHK(ctx, hipMalloc(&bd.folds, sizeof(FoldView) * SWB_MAX));
+    HIPCHK(ctx, hipStreamCreateWithFlags(&bd.stream, hipStreamNonBlocking));  // last: it marks the set ready
     return LBN_OK;
 }
 static std::mutex swb_mu;
 static std::map<int, SwBatchDev> swb_dev;
 
-int lbn::dev_switch_batch(lbn_ctx *const *ctxs, int n, B3State *ts, uint64_t *const *rounds,
-                          uint64_t *const *coords, SwitchHandoff *ho) {
+// The contexts of a batch: on one device, each once.
+static int batch_check(lbn_ctx *const *ctxs, int n) {
     if (n < 1 || n > SWB_MAX) return LBN_E_ARG;
-    const int dev = ctxs[0]->device;
     for (int i = 0; i < n; i++) {
-        lbn_ctx *ctx = ctxs[i];
-        if (ctx->device != dev) return ctx->fail(LBN_E_ARG, "a batched switch runs on one device");
-        if (ctx->round != 0) return ctx->fail(LBN_E_STATE, "lbn_switch_begin must run first");
+        if (ctxs[i]->device != ctxs[0]->device) return ctxs[i]->fail(LBN_E_ARG, "a batch runs on one device");
         for (int j = 0; j < i; j++)
-            if (ctxs[j] == ctx) return ctx->fail(LBN_E_ARG, "a context appears twice in the batch");
+            if (ctxs[j] == ctxs[i]) return ctxs[i]->fail(LBN_E_ARG, "a context appears twice in the batch");
     }
+    return LBN_OK;
+}
+
+void lbn::dev_switch_stage(lbn_ctx *ctx, const B3State &ts) { switch_mailbox(ctx, ts); }
+void lbn::dev_switch_collect(lbn_ctx *ctx, B3State &ts, uint64_t *rounds, uint64_t *coords, SwitchHandoff &ho) {
+    switch_readback(ctx, ts, rounds, coords, ho);
+}
+int lbn::dev_sync(lbn_ctx *ctx) {
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->dev->stream));
+    return LBN_OK;
+}
+
+int lbn::dev_switch_batch(lbn_ctx *const *ctxs, int n) {
+    {
+        const int rc = batch_check(ctxs, n);
+        if (rc != LBN_OK) return rc;
+    }
+    const int dev = ctxs[0]->device;
+    for (int i = 0; i < n; i++)
+        if (ctxs[i]->round != 0) return ctxs[i]->fail(LBN_E_STATE, "lbn_switch_begin must run first");
     std::lock_guard<std::mutex> lk(swb_mu);
     HIPCHK(ctxs[0], hipSetDevice(dev));
     SwBatchDev &bd = swb_dev[dev];
@@ -4699,8 +4796,6 @@ int lbn::dev_switch_batch(lbn_ctx *const *ctxs, int n, B3State *ts, uint64_t *co
     for (int i = 0; i < n; i++) {
         lbn_ctx *ctx = ctxs[i];
         DeviceState *d = ctx->dev;
-        HIPCHK(ctx, hipStreamSynchronize(d->stream));  // its partials are in, its mailbox is free
-        switch_mailbox(ctx, ts[i]);
         SwView &x = hv[i];
         x.table = d->table;
         x.pset = d->pset;
@@ -4748,7 +4843,6 @@ int lbn::dev_switch_batch(lbn_ctx *const *ctxs, int n, B3State *ts, uint64_t *co
     }
     HIPCHK(ctxs[0], hipGetLastError());
     HIPCHK(ctxs[0], hipStreamSynchronize(st));
-    for (int i = 0; i < n; i++) switch_readback(ctxs[i], ts[i], rounds[i], coords[i], ho[i]);
     return LBN_OK;
 }
 
@@ -4809,10 +4903,10 @@ extern "C" int lbn_switch_final(lbn_ctx *ctx, uint64_t *opened) {
 
 // ----------------------------------------------------------------------------- row evaluation
 
-extern "C" int lbn_row_eval(lbn_ctx *ctx, const uint64_t *p0, int n_p0, uint64_t *row) {
-    if (!ctx || !p0 || !row) return LBN_E_ARG;
+static int row_bits_lo(const Shape &s) { return std::min(s.row_log, 7); }
+
+int lbn::dev_row_prepare(lbn_ctx *ctx, const uint64_t *p0) {
     const Shape &s = ctx->shape;
-    if (n_p0 != s.row_log) return ctx->fail(LBN_E_ARG, "p0 must have log_vars - 17 coordinates");
     if (!ctx->table_loaded) return ctx->fail(LBN_E_STATE, "no table loaded");
     HIPCHK(ctx, hipSetDevice(ctx->device));
     {
@@ -4820,13 +4914,26 @@ extern "C" int lbn_row_eval(lbn_ctx *ctx, const uint64_t *p0, int n_p0, uint64_t
         if (rc_ != LBN_OK) return rc_;
     }
     DeviceState *d = ctx->dev;
-    const int lo_bits = std::min(s.row_log, 7), hi_bits = s.row_log - lo_bits;
+    const int lo_bits = row_bits_lo(s), hi_bits = s.row_log - lo_bits;
     std::vector<F162> lo((size_t)1 << lo_bits), hi((size_t)1 << hi_bits);
     eq_table_f162((const F162 *)p0, lo_bits, lo.data());
     eq_table_f162((const F162 *)p0 + lo_bits, hi_bits, hi.data());
     memcpy(d->hbox + mb::EQ, lo.data(), lo.size() * 24);
     memcpy(d->hbox + mb::EQ + 128 * 24, hi.data(), hi.size() * 24);
     HIPCHK(ctx, hipMemcpyAsync(d->eqlo, d->hbox + mb::EQ, (128 + hi.size()) * 24, hipMemcpyHostToDevice, d->stream));
+    return LBN_OK;
+}
+
+extern "C" int lbn_row_eval(lbn_ctx *ctx, const uint64_t *p0, int n_p0, uint64_t *row) {
+    if (!ctx || !p0 || !row) return LBN_E_ARG;
+    const Shape &s = ctx->shape;
+    if (n_p0 != s.row_log) return ctx->fail(LBN_E_ARG, "p0 must have log_vars - 17 coordinates");
+    {
+        const int rc_ = dev_row_prepare(ctx, p0);
+        if (rc_ != LBN_OK) return rc_;
+    }
+    DeviceState *d = ctx->dev;
+    const int lo_bits = row_bits_lo(s), hi_bits = s.row_log - lo_bits;
     tstart(ctx, T_ROW);
     k_eq_row<<<1u << hi_bits, 256, 0, d->stream>>>((const uint64_t *)d->eqlo,
                                                    (const uint64_t *)((uint8_t *)d->eqlo + 128 * 24), lo_bits, d->eqrow);
@@ -4843,8 +4950,8 @@ extern "C" int lbn_row_eval(lbn_ctx *ctx, const uint64_t *p0, int n_p0, uint64_t
 
 // ----------------------------------------------------------------------------- fold
 
-extern "C" int lbn_fold(lbn_ctx *ctx, const lbn_challenge *ch, int n_challenges, int32_t *out, size_t n_out) {
-    if (!ctx || !ch || !out) return LBN_E_ARG;
+int lbn::dev_fold_prepare(lbn_ctx *ctx, const lbn_challenge *ch, int n_challenges, int32_t *out, size_t n_out,
+                          FoldPlan &plan) {
     const Shape &s = ctx->shape;
     if (n_challenges != COLUMNS) return ctx->fail(LBN_E_ARG, "one challenge per column (1024)");
     if (n_out != s.ring_elements * N) return ctx->fail(LBN_E_ARG, "fold output has the wrong length");
@@ -4880,6 +4987,10 @@ extern "C" int lbn_fold(lbn_ctx *ctx, const lbn_challenge *ch, int n_challenges,
     });
     int nb = 1;
     while ((1 << nb) <= maxlen) nb++;
+    nb = std::max(nb, 6);
+    const int n_entries = off[2 * 162];
+    static_assert((2 * 162 + 1 + 162) * sizeof(int) <= mb::FCOLS - mb::FOFF, "fold offsets region");
+    if (n_entries > FOLD_MAX_ENTRIES || nb > 12) return ctx->fail(LBN_E_ARG, "challenge lists too long");
     HIPCHK(ctx, hipSetDevice(ctx->device));
     {
         const int rc_ = table_ready(ctx);  // a deferred upload completes first
@@ -4892,25 +5003,47 @@ extern "C" int lbn_fold(lbn_ctx *ctx, const lbn_challenge *ch, int n_challenges,
     HIPCHK(ctx, hipMemcpyAsync(d->fold_off, d->hbox + mb::FOFF, (mb::FCOLS - mb::FOFF) + cols.size() * 2,
                                hipMemcpyHostToDevice, d->stream));
     // the range flag lives in the mailbox: cleared here, set by the kernel, read after the sync
-    volatile uint32_t *flag_host = (volatile uint32_t *)(d->hbox + mb::MISC);
-    *flag_host = 0;
+    *(volatile uint32_t *)(d->hbox + mb::MISC) = 0;
+    // into the caller's buffer when it is one of ours (pinned), else into the context's staging
+    int32_t *direct = ((uintptr_t)out & 3) ? nullptr : (int32_t *)pinned_alias(out, n_out * 4);
+    plan.nb = nb;
+    plan.n_entries = n_entries;
+    plan.out = out;
+    plan.n_out = n_out;
+    plan.dst = direct ? direct : d->fold_host;
+    return LBN_OK;
+}
+
+int lbn::dev_fold_collect(lbn_ctx *ctx, const FoldPlan &plan) {
+    DeviceState *d = ctx->dev;
+    // The kernel wrote the fold straight into pinned host memory over the host link, overlapped
+    // with its own work; once its stream is idle the values are visible here.
+    if (*(volatile uint32_t *)(d->hbox + mb::MISC))
+        return ctx->fail(LBN_E_RANGE, "a fold coefficient does not fit the wire codec");
+    if (plan.dst == d->fold_host) copy_parallel((uint8_t *)plan.out, (const uint8_t *)d->hfold, plan.n_out * 4);
+    return LBN_OK;
+}
+
+extern "C" int lbn_fold(lbn_ctx *ctx, const lbn_challenge *ch, int n_challenges, int32_t *out, size_t n_out) {
+    if (!ctx || !ch || !out) return LBN_E_ARG;
+    FoldPlan plan;
+    {
+        const int rc_ = dev_fold_prepare(ctx, ch, n_challenges, out, n_out, plan);
+        if (rc_ != LBN_OK) return rc_;
+    }
+    const Shape &s = ctx->shape;
+    DeviceState *d = ctx->dev;
     int cus = 256;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess) cus = 256;
     const unsigned grid = (unsigned)std::min<size_t>(s.ring_elements, (size_t)cus);
-    const int n_entries = off[2 * 162];
-    static_assert((2 * 162 + 1 + 162) * sizeof(int) <= mb::FCOLS - mb::FOFF, "fold offsets region");
-    if (n_entries > FOLD_MAX_ENTRIES) return ctx->fail(LBN_E_ARG, "challenge lists too long");
-    // into the caller's buffer when it is one of ours (pinned), else into the context's staging
-    int32_t *direct = ((uintptr_t)out & 3) ? nullptr : (int32_t *)pinned_alias(out, n_out * 4);
-    int32_t *dst = direct ? direct : d->fold_host;
+    uint32_t *flag = (uint32_t *)(d->hbox_dev + mb::MISC);
     tstart(ctx, T_FOLD);
 #define FOLD_CASE(B)                                                                                          \
     case B:                                                                                                   \
         k_fold<B><<<grid, FOLD_THREADS, 0, d->stream>>>(d->table, s.column_len, s.ring_elements, d->fold_off,   \
-                                                        d->fold_cols, n_entries, dst,                         \
-                                                        (uint32_t *)(d->hbox_dev + mb::MISC));                \
+                                                        d->fold_cols, plan.n_entries, plan.dst, flag);        \
         break;
-    switch (std::max(nb, 6)) {
+    switch (plan.nb) {
         FOLD_CASE(6)
         FOLD_CASE(7)
         FOLD_CASE(8)
@@ -4924,22 +5057,28 @@ extern "C" int lbn_fold(lbn_ctx *ctx, const lbn_challenge *ch, int n_challenges,
 #undef FOLD_CASE
     tstop(ctx, T_FOLD);
     HIPCHK(ctx, hipGetLastError());
-    // The kernel wrote the fold straight into pinned host memory over the host link, overlapped
-    // with its own work; once the stream is idle the values are visible here.
-    const uint8_t *stage = (const uint8_t *)d->hfold;
     HIPCHK(ctx, hipStreamSynchronize(d->stream));
-    const uint32_t flag = *flag_host;
     tcollect(ctx, T_FOLD);
-    if (flag) return ctx->fail(LBN_E_RANGE, "a fold coefficient does not fit the wire codec");
-    if (!direct) copy_parallel((uint8_t *)out, stage, n_out * 4);
-    return LBN_OK;
+    return dev_fold_collect(ctx, plan);
 }
 
-int lbn::dev_challenges_batch(lbn_ctx *const *ctxs, int n, const ChalArg *args, lbn_challenge *const *outs) {
-    if (n < 1 || n > SWB_MAX) return LBN_E_ARG;
+int lbn::dev_challenges_stage(lbn_ctx *ctx, const ChalArg &arg) {
+    DeviceState *d = ctx->dev;
+    memcpy(d->hbox + mb::CH, &arg, sizeof(ChalArg));
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, hipMemcpyAsync(d->chal, d->hbox + mb::CH, sizeof(ChalArg), hipMemcpyHostToDevice, d->stream));
+    return LBN_OK;
+}
+void lbn::dev_challenges_collect(lbn_ctx *ctx, lbn_challenge *out) {
+    memcpy(out, ctx->dev->hbox + mb::CHOUT, COLUMNS * sizeof(lbn_challenge));
+}
+
+int lbn::dev_challenges_batch(lbn_ctx *const *ctxs, int n) {
+    {
+        const int rc = batch_check(ctxs, n);
+        if (rc != LBN_OK) return rc;
+    }
     const int dev = ctxs[0]->device;
-    for (int i = 0; i < n; i++)
-        if (ctxs[i]->device != dev) return ctxs[i]->fail(LBN_E_ARG, "a batch runs on one device");
     std::lock_guard<std::mutex> lk(swb_mu);
     HIPCHK(ctxs[0], hipSetDevice(dev));
     SwBatchDev &bd = swb_dev[dev];
@@ -4950,16 +5089,108 @@ int lbn::dev_challenges_batch(lbn_ctx *const *ctxs, int n, const ChalArg *args, 
     std::vector<ChalPtrs> hp(n);
     for (int i = 0; i < n; i++) {
         DeviceState *d = ctxs[i]->dev;
-        HIPCHK(ctxs[i], hipStreamSynchronize(d->stream));  // its mailbox is free
-        memcpy(d->hbox + mb::CH, &args[i], sizeof(ChalArg));
-        HIPCHK(ctxs[i], hipMemcpyAsync(d->chal, d->hbox + mb::CH, sizeof(ChalArg), hipMemcpyHostToDevice, bd.stream));
         hp[i] = ChalPtrs{d->chal, (lbn_challenge *)(d->hbox_dev + mb::CHOUT)};
     }
     HIPCHK(ctxs[0], hipMemcpyAsync(bd.chal, hp.data(), sizeof(ChalPtrs) * n, hipMemcpyHostToDevice, bd.stream));
     k_mchallenges<<<n * (COLUMNS / chal::WAVES), 64 * chal::WAVES, 0, bd.stream>>>(bd.chal);
     HIPCHK(ctxs[0], hipGetLastError());
     HIPCHK(ctxs[0], hipStreamSynchronize(bd.stream));
-    for (int i = 0; i < n; i++) memcpy(outs[i], ctxs[i]->dev->hbox + mb::CHOUT, COLUMNS * sizeof(lbn_challenge));
+    return LBN_OK;
+}
+
+void lbn::dev_row_collect(lbn_ctx *ctx, uint64_t *row) {
+    memcpy(row, ctx->dev->hbox + mb::ROW, (size_t)COLUMNS * 24);
+}
+
+int lbn::dev_row_batch(lbn_ctx *const *ctxs, int n) {
+    {
+        const int rc = batch_check(ctxs, n);
+        if (rc != LBN_OK) return rc;
+    }
+    std::lock_guard<std::mutex> lk(swb_mu);
+    HIPCHK(ctxs[0], hipSetDevice(ctxs[0]->device));
+    SwBatchDev &bd = swb_dev[ctxs[0]->device];
+    {
+        const int rc = swb_ready(ctxs[0], bd);
+        if (rc != LBN_OK) return rc;
+    }
+    std::vector<RowView> hv(n);
+    int eq_blocks = 0;
+    for (int i = 0; i < n; i++) {
+        lbn_ctx *ctx = ctxs[i];
+        DeviceState *d = ctx->dev;
+        const Shape &s = ctx->shape;
+        RowView &x = hv[i];
+        x.table = d->table;
+        x.clen = s.column_len;
+        x.lo = (const uint64_t *)d->eqlo;
+        x.hi = (const uint64_t *)((uint8_t *)d->eqlo + 128 * 24);
+        x.lo_bits = row_bits_lo(s);
+        x.eq_start = eq_blocks;
+        x.eqrow = d->eqrow;
+        x.row = (uint64_t *)(d->hbox_dev + mb::ROW);
+        eq_blocks += 1 << (s.row_log - x.lo_bits);
+    }
+    hipStream_t st = bd.stream;
+    HIPCHK(ctxs[0], hipMemcpyAsync(bd.rows, hv.data(), sizeof(RowView) * n, hipMemcpyHostToDevice, st));
+    k_meq_row<<<eq_blocks, 256, 0, st>>>(bd.rows, n);
+    HIPCHK(ctxs[0], hipGetLastError());
+    k_mrow<<<n * COLUMNS, 256, 0, st>>>(bd.rows);
+    HIPCHK(ctxs[0], hipGetLastError());
+    HIPCHK(ctxs[0], hipStreamSynchronize(st));
+    return LBN_OK;
+}
+
+int lbn::dev_fold_batch(lbn_ctx *const *ctxs, int n, const FoldPlan *plans) {
+    {
+        const int rc = batch_check(ctxs, n);
+        if (rc != LBN_OK) return rc;
+    }
+    std::lock_guard<std::mutex> lk(swb_mu);
+    const int dev = ctxs[0]->device;
+    HIPCHK(ctxs[0], hipSetDevice(dev));
+    SwBatchDev &bd = swb_dev[dev];
+    {
+        const int rc = swb_ready(ctxs[0], bd);
+        if (rc != LBN_OK) return rc;
+    }
+    std::vector<FoldView> hv(n);
+    size_t total = 0;
+    int nb = 6;
+    for (int i = 0; i < n; i++) {
+        lbn_ctx *ctx = ctxs[i];
+        DeviceState *d = ctx->dev;
+        FoldView &x = hv[i];
+        x.table = d->table;
+        x.clen = ctx->shape.column_len;
+        x.re = ctx->shape.ring_elements;
+        x.first = total;
+        x.off = d->fold_off;
+        x.cols = d->fold_cols;
+        x.n_entries = plans[i].n_entries;
+        x.out = plans[i].dst;
+        x.flag = (uint32_t *)(d->hbox_dev + mb::MISC);
+        total += x.re;
+        nb = std::max(nb, plans[i].nb);  // wider counters give the same counts
+    }
+    int cus = 256;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 256;
+    const size_t per = (total + cus - 1) / cus;
+    const unsigned grid = (unsigned)((total + per - 1) / per);
+    hipStream_t st = bd.stream;
+    HIPCHK(ctxs[0], hipMemcpyAsync(bd.folds, hv.data(), sizeof(FoldView) * n, hipMemcpyHostToDevice, st));
+    switch (nb) {
+        case 6: k_mfold<6><<<grid, FOLD_THREADS, 0, st>>>(bd.folds, n, per); break;
+        case 7: k_mfold<7><<<grid, FOLD_THREADS, 0, st>>>(bd.folds, n, per); break;
+        case 8: k_mfold<8><<<grid, FOLD_THREADS, 0, st>>>(bd.folds, n, per); break;
+        case 9: k_mfold<9><<<grid, FOLD_THREADS, 0, st>>>(bd.folds, n, per); break;
+        case 10: k_mfold<10><<<grid, FOLD_THREADS, 0, st>>>(bd.folds, n, per); break;
+        case 11: k_mfold<11><<<grid, FOLD_THREADS, 0, st>>>(bd.folds, n, per); break;
+        case 12: k_mfold<12><<<grid, FOLD_THREADS, 0, st>>>(bd.folds, n, per); break;
+        default: return ctxs[0]->fail(LBN_E_ARG, "challenge lists too long");
+    }
+    HIPCHK(ctxs[0], hipGetLastError());
+    HIPCHK(ctxs[0], hipStreamSynchronize(st));
     return LBN_OK;
 }
 

@@ -557,8 +557,9 @@ struct ProveJob {
     B3State ts{};
     SwitchHandoff ho;
     lbn_profile pf{};
-    double t0 = 0, t3 = 0, t4 = 0, t5 = 0;
-    std::vector<F162> row;
+    double t0 = 0, t3 = 0, t4 = 0, t5 = 0, t6 = 0;
+    std::vector<F162> p0, row;
+    FoldPlan fp;
     ChalArg ca{};
     std::vector<lbn_challenge> ch;
     bool launched = false;  // device work queued: an error return waits for the device first
@@ -568,7 +569,8 @@ struct ProveJob {
     }
 };
 
-int prove_front(ProveJob &j, size_t cap) {
+// The proof's layout and the commitment's launch.
+int prove_front_launch(ProveJob &j, size_t cap) {
     lbn_ctx *ctx = j.ctx;
     if (!ctx->table_loaded || !ctx->modulus_set) return ctx->fail(LBN_E_STATE, "table and modulus must be loaded");
     const Shape &s = ctx->shape;
@@ -589,8 +591,15 @@ int prove_front(ProveJob &j, size_t cap) {
     // From the first launch on, an error return first waits for the device: the commitment's copy
     // and the fold kernel may write into `proof` (directly when it is pinned).
     j.launched = true;
-    TRY(dev_commit_launch(ctx));
+    return dev_commit_launch(ctx);
+}
 
+// After the commitment's launch: the transcript, the partials and the switch's start.
+int prove_front_rest(ProveJob &j) {
+    lbn_ctx *ctx = j.ctx;
+    const Shape &s = ctx->shape;
+    const u128 q = ctx->q;
+    const int lp = s.log_packed;
     // verifier/proof_verify/src/main.rs:166-182 (absorb k and 10, the shape of this instance)
     Transcript &t = j.t;
     t.absorb_bytes(j.seed, 32);
@@ -641,23 +650,31 @@ int prove_front(ProveJob &j, size_t cap) {
     return LBN_OK;
 }
 
+int prove_front(ProveJob &j, size_t cap) {
+    TRY(prove_front_launch(j, cap));
+    return prove_front_rest(j);
+}
+
 // The switch's host steps, the row and its absorb; leaves the folding challenges' input in j.ca.
-int prove_back(ProveJob &j) {
+// After the batched switch rounds: the switch's host steps and the row's point p0.
+int prove_back_point(ProveJob &j) {
     lbn_ctx *ctx = j.ctx;
     const Shape &s = ctx->shape;
     const int lp = s.log_packed;
-    Transcript &t = j.t;
-    t.load(j.ts);
-    switch_host_tail(ctx, t, j.ho, j.rounds.data(), j.coords.data());
+    j.t.load(j.ts);
+    switch_host_tail(ctx, j.t, j.ho, j.rounds.data(), j.coords.data());
     j.t4 = now_us();
-
     // EvaluationPoint::msb_first (scheme/commitment.rs:163-174): p0 = reverse(coords[10..]).
-    std::vector<F162> p0(s.row_log);
+    j.p0.resize(s.row_log);
     j.row.resize(COLUMNS);
-    for (int i = 0; i < s.row_log; i++) p0[i] = j.coords[lp - 1 - i];
-    TRY(lbn_row_eval(ctx, (const uint64_t *)p0.data(), s.row_log, (uint64_t *)j.row.data()));
-    j.t5 = now_us();
+    for (int i = 0; i < s.row_log; i++) j.p0[i] = j.coords[lp - 1 - i];
+    return LBN_OK;
+}
 
+// After the row evaluation (in j.row): its absorb and the challenges' transcript state.
+int prove_back_row(ProveJob &j) {
+    Transcript &t = j.t;
+    j.t5 = now_us();
     // RowEvaluation::absorb (scheme/commitment.rs:320-331), then 1024 challenges in parallel,
     // each one derivation at its own counter (challenge.rs:778-802).
     {
@@ -674,20 +691,28 @@ int prove_back(ProveJob &j) {
     return LBN_OK;
 }
 
-// After the folding challenges (in j.ch): the fold, the commitment bytes and the bincode.
-int prove_tail(ProveJob &j, size_t *len) {
+int prove_back(ProveJob &j) {
+    TRY(prove_back_point(j));
+    TRY(lbn_row_eval(j.ctx, (const uint64_t *)j.p0.data(), j.ctx->shape.row_log, (uint64_t *)j.row.data()));
+    return prove_back_row(j);
+}
+
+// Before the fold: the challenges leave the transcript.
+void prove_tail_start(ProveJob &j) {
+    j.t.skip(COLUMNS);
+    j.t6 = now_us();
+}
+int32_t *fold_out(ProveJob &j) { return (int32_t *)(j.proof + j.off_fold + 8); }
+
+// After the fold (in the proof): the commitment bytes and the bincode.
+int prove_tail_encode(ProveJob &j, size_t *len) {
     lbn_ctx *ctx = j.ctx;
     const Shape &s = ctx->shape;
     const u128 q = ctx->q;
     const int lp = s.log_packed;
-    Transcript &t = j.t;
     const F162 opened = ctx->opened;
-    auto &ch = j.ch;
     auto &row = j.row;
-    t.skip(COLUMNS);
-    const double t4 = j.t4, t5 = j.t5, t6 = now_us();
-
-    TRY(lbn_fold(ctx, ch.data(), COLUMNS, (int32_t *)(j.proof + j.off_fold + 8), (size_t)N * s.ring_elements));
+    const double t4 = j.t4, t5 = j.t5, t6 = j.t6;
     TRY(dev_commit_bytes(ctx, j.proof + j.off_commit));
     const double t7 = now_us();
 
@@ -719,6 +744,13 @@ int prove_tail(ProveJob &j, size_t *len) {
     j.pf.total_us = t8 - j.t0;
     return LBN_OK;
 }
+
+// After the folding challenges (in j.ch): the fold, the commitment bytes and the bincode.
+int prove_tail(ProveJob &j, size_t *len) {
+    prove_tail_start(j);
+    TRY(lbn_fold(j.ctx, j.ch.data(), COLUMNS, fold_out(j), (size_t)N * j.ctx->shape.ring_elements));
+    return prove_tail_encode(j, len);
+}
 }  // namespace
 
 extern "C" int lbn_prove(lbn_ctx *ctx, const uint8_t seed[32], uint8_t *proof, size_t cap, size_t *len,
@@ -734,11 +766,161 @@ extern "C" int lbn_prove(lbn_ctx *ctx, const uint8_t seed[32], uint8_t *proof, s
     return LBN_OK;
 }
 
-// Many proofs together: the fronts on `threads` host threads (each proof on its own context and
-// streams), one batched device switch for all of them (dev_switch_batch: one launch per device
-// step), the backs on the threads again up to the folding challenges, the challenges of all in one
-// launch, then the folds and encodings. An item that fails keeps its error and leaves the others
-// going.
+namespace {
+constexpr int BATCH_GROUPS = 2;     // a batch runs as up to this many staggered groups
+constexpr int BATCH_GROUP_MIN = 8;  // of at least this many proofs each
+// The batch's host threads, started once per batch: run(f) calls f(order[k]) for every k, on the
+// threads and the caller, and returns when every call has returned.
+class PhasePool {
+  public:
+    PhasePool(int helpers, const std::vector<int> &order) : order_(order) {
+        for (int k = 0; k < helpers; k++) th_.emplace_back([this] { loop(); });
+    }
+    ~PhasePool() {
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            stop_ = true;
+        }
+        go_.notify_all();
+        for (auto &t : th_) t.join();
+    }
+    void run(const std::function<void(int)> &f) {
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            f_ = &f;
+            next_ = 0;
+            busy_ = (int)th_.size();
+            gen_++;
+        }
+        go_.notify_all();
+        work(f);
+        std::unique_lock<std::mutex> lk(m_);
+        done_.wait(lk, [&] { return busy_ == 0; });
+        f_ = nullptr;
+    }
+
+  private:
+    void work(const std::function<void(int)> &f) {
+        const int n = (int)order_.size();
+        for (int k; (k = next_.fetch_add(1)) < n;) f(order_[k]);
+    }
+    void loop() {
+        uint64_t seen = 0;
+        for (;;) {
+            const std::function<void(int)> *f;
+            {
+                std::unique_lock<std::mutex> lk(m_);
+                go_.wait(lk, [&] { return stop_ || gen_ != seen; });
+                if (stop_) return;
+                seen = gen_;
+                f = f_;
+            }
+            work(*f);
+            std::lock_guard<std::mutex> lk(m_);
+            if (--busy_ == 0) done_.notify_all();
+        }
+    }
+    const std::vector<int> &order_;
+    std::vector<std::thread> th_;
+    std::mutex m_;
+    std::condition_variable go_, done_;
+    const std::function<void(int)> *f_ = nullptr;
+    std::atomic<int> next_{0};
+    uint64_t gen_ = 0;
+    int busy_ = 0;
+    bool stop_ = false;
+};
+}  // namespace
+
+// One group of a batch, phase by phase: the commitments on `threads` host threads (each proof on
+// its own context and streams), then the transcripts and partials, then one device sequence for
+// the switch rounds of all of them (one launch per device step), one launch pair for all rows, one
+// launch for all folding challenges and one for all folds, with each proof's host steps between
+// them on the threads; `committed` is called once the commitments are done. An item that fails
+// keeps its error and leaves the others going.
+static void batch_group(lbn_batch_item *items, std::vector<std::unique_ptr<ProveJob>> &jobs,
+                        const std::vector<int> &order, int threads, const std::function<void()> &committed) {
+    PhasePool pool(std::min(threads, (int)order.size()) - 1, order);
+    auto run = [&](const std::function<void(int)> &f) { pool.run(f); };
+    std::vector<int> li;
+    std::vector<lbn_ctx *> lc;
+    // the proofs still going, and a batched call over them (a failure fails each of them)
+    auto batch = [&](const std::function<int(lbn_ctx *const *, int)> &call) {
+        li.clear(), lc.clear();
+        for (int i : order)
+            if (items[i].status == 1) li.push_back(i), lc.push_back(items[i].ctx);
+        if (li.empty()) return;
+        const int rc = call(lc.data(), (int)lc.size());
+        if (rc != LBN_OK)
+            for (int i : li) items[i].status = rc;
+    };
+    // each phase on the threads: proof i's step, then its stream idle
+    auto phase = [&](const std::function<int(int)> &step) {
+        run([&](int i) {
+            lbn_batch_item &it = items[i];
+            if (it.status != 1) return;
+            int rc = step(i);
+            if (rc == LBN_OK) rc = dev_sync(it.ctx);
+            if (rc != LBN_OK) it.status = rc;
+        });
+    };
+    // The commitments first, on their own: their matrix-core blocks take whole CUs, which the
+    // short kernels of the other proofs' later steps would hold up.
+    phase([&](int i) {  // table, commitment
+        lbn_batch_item &it = items[i];
+        int rc = it.gen_table ? lbn_generate_table(it.ctx, it.seed) : LBN_OK;
+        if (rc == LBN_OK) rc = prove_front_launch(*jobs[i], it.cap);
+        return rc;
+    });
+    committed();
+    phase([&](int i) {  // transcript, partials, the switch's start state
+        const int rc = prove_front_rest(*jobs[i]);
+        if (rc == LBN_OK) dev_switch_stage(jobs[i]->ctx, jobs[i]->ts);
+        return rc;
+    });
+    batch(dev_switch_batch);
+    phase([&](int i) {  // the switch's host steps, the row's eq tables
+        ProveJob &j = *jobs[i];
+        dev_switch_collect(j.ctx, j.ts, (uint64_t *)j.rounds.data(), (uint64_t *)j.coords.data(), j.ho);
+        int rc = prove_back_point(j);
+        if (rc == LBN_OK) rc = dev_row_prepare(j.ctx, (const uint64_t *)j.p0.data());
+        return rc;
+    });
+    batch(dev_row_batch);
+    phase([&](int i) {  // the row into the transcript, the challenges' transcript state
+        ProveJob &j = *jobs[i];
+        dev_row_collect(j.ctx, (uint64_t *)j.row.data());
+        int rc = prove_back_row(j);
+        if (rc == LBN_OK) rc = dev_challenges_stage(j.ctx, j.ca);
+        return rc;
+    });
+    batch(dev_challenges_batch);
+    phase([&](int i) {  // the challenges, the fold's column lists
+        ProveJob &j = *jobs[i];
+        dev_challenges_collect(j.ctx, j.ch.data());
+        prove_tail_start(j);
+        return dev_fold_prepare(j.ctx, j.ch.data(), COLUMNS, fold_out(j), (size_t)N * j.ctx->shape.ring_elements,
+                                j.fp);
+    });
+    {
+        std::vector<FoldPlan> fps;
+        batch([&](lbn_ctx *const *cx, int m) {
+            for (int i : li) fps.push_back(jobs[i]->fp);
+            return dev_fold_batch(cx, m, fps.data());
+        });
+    }
+    run([&](int i) {  // the fold's range check, the commitment bytes, the bincode
+        lbn_batch_item &it = items[i];
+        if (it.status != 1) return;
+        ProveJob &j = *jobs[i];
+        int rc = dev_fold_collect(it.ctx, j.fp);
+        if (rc == LBN_OK) rc = prove_tail_encode(j, &it.len);
+        if (rc == LBN_OK) it.profile = j.pf;
+        it.status = rc;
+    });
+}
+
+// Many proofs together: up to BATCH_GROUPS staggered groups of at least BATCH_GROUP_MIN proofs.
 extern "C" int lbn_prove_batch(lbn_batch_item *items, int n, int threads) {
     if (!items || n < 1 || n > SWB_MAX || threads < 1) return LBN_E_ARG;
     for (int i = 0; i < n; i++) {
@@ -751,78 +933,39 @@ extern "C" int lbn_prove_batch(lbn_batch_item *items, int n, int threads) {
     }
     std::vector<std::unique_ptr<ProveJob>> jobs(n);
     for (int i = 0; i < n; i++) jobs[i].reset(new ProveJob(items[i].ctx, items[i].seed, items[i].proof));
-    auto run = [&](const std::function<void(int)> &f) {
-        std::atomic<int> next{0};
-        auto worker = [&]() {
-            for (int i; (i = next.fetch_add(1)) < n;) f(i);
-        };
-        std::vector<std::thread> pool;
-        for (int k = 1; k < std::min(threads, n); k++) pool.emplace_back(worker);
-        worker();
-        for (auto &th : pool) th.join();
+    // the host threads take the items largest table first, so that no large proof starts last
+    std::vector<int> order(n);
+    for (int i = 0; i < n; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        return items[a].ctx->shape.log_vars > items[b].ctx->shape.log_vars;
+    });
+    // Groups of alternate items, each starting once the one before it has its commitments: a
+    // group's commitments then run beside the earlier groups' hashing, switch, rows, challenges and
+    // folds, which leave most CUs idle, instead of beside its own.
+    const int groups = std::max(1, std::min({BATCH_GROUPS, n / BATCH_GROUP_MIN, threads}));
+    std::vector<std::vector<int>> g(groups);
+    for (int k = 0; k < n; k++) g[k % groups].push_back(order[k]);
+    std::mutex m;
+    std::condition_variable cv;
+    int started = 1;  // groups released
+    auto release = [&](int next) {
+        {
+            std::lock_guard<std::mutex> lk(m);
+            started = std::max(started, next + 1);
+        }
+        cv.notify_all();
     };
-    run([&](int i) {
-        lbn_batch_item &it = items[i];
-        int rc = it.gen_table ? lbn_generate_table(it.ctx, it.seed) : LBN_OK;
-        if (rc == LBN_OK) rc = prove_front(*jobs[i], it.cap);
-        it.status = rc == LBN_OK ? 1 : rc;
-    });
-    std::vector<lbn_ctx *> cs;
-    std::vector<B3State> ts;
-    std::vector<uint64_t *> rounds, coords;
-    std::vector<SwitchHandoff> ho;
-    std::vector<int> idx;
-    for (int i = 0; i < n; i++)
-        if (items[i].status == 1) {
-            idx.push_back(i);
-            cs.push_back(items[i].ctx);
-            ts.push_back(jobs[i]->ts);
-            rounds.push_back((uint64_t *)jobs[i]->rounds.data());
-            coords.push_back((uint64_t *)jobs[i]->coords.data());
-        }
-    if (!idx.empty()) {
-        ho.resize(idx.size());
-        const int rc = dev_switch_batch(cs.data(), (int)cs.size(), ts.data(), rounds.data(), coords.data(), ho.data());
-        for (size_t k = 0; k < idx.size(); k++) {
-            if (rc != LBN_OK) {
-                items[idx[k]].status = rc;
-                continue;
+    std::vector<std::thread> later;
+    for (int q = 1; q < groups; q++)
+        later.emplace_back([&, q] {
+            {
+                std::unique_lock<std::mutex> lk(m);
+                cv.wait(lk, [&] { return started > q; });
             }
-            jobs[idx[k]]->ts = ts[k];
-            jobs[idx[k]]->ho = std::move(ho[k]);
-        }
-    }
-    run([&](int i) {
-        lbn_batch_item &it = items[i];
-        if (it.status != 1) return;
-        const int rc = prove_back(*jobs[i]);
-        if (rc != LBN_OK) it.status = rc;
-    });
-    {  // the folding challenges of every proof in one launch
-        std::vector<lbn_ctx *> cc;
-        std::vector<ChalArg> ca;
-        std::vector<lbn_challenge *> co;
-        std::vector<int> ci;
-        for (int i = 0; i < n; i++)
-            if (items[i].status == 1) {
-                ci.push_back(i);
-                cc.push_back(items[i].ctx);
-                ca.push_back(jobs[i]->ca);
-                co.push_back(jobs[i]->ch.data());
-            }
-        if (!ci.empty()) {
-            const int rc = dev_challenges_batch(cc.data(), (int)cc.size(), ca.data(), co.data());
-            if (rc != LBN_OK)
-                for (int i : ci) items[i].status = rc;
-        }
-    }
-    run([&](int i) {
-        lbn_batch_item &it = items[i];
-        if (it.status != 1) return;
-        const int rc = prove_tail(*jobs[i], &it.len);
-        if (rc == LBN_OK) it.profile = jobs[i]->pf;
-        it.status = rc;
-    });
+            batch_group(items, jobs, g[q], threads / groups, [&, q] { release(q + 1); });
+        });
+    batch_group(items, jobs, g[0], threads - (groups - 1) * (threads / groups), [&] { release(1); });
+    for (auto &t : later) t.join();
     int first = LBN_OK;
     for (int i = 0; i < n; i++) {
         if (first == LBN_OK && items[i].status != LBN_OK) first = items[i].status;

@@ -172,12 +172,20 @@ struct SwitchHandoff {
     std::vector<B128> e;     // and its eq values (2 h2)
 };
 int dev_switch_rounds(lbn_ctx *ctx, B3State &ts, uint64_t *rounds, uint64_t *coords, SwitchHandoff &ho);
-// The same for n proofs at once (contexts on one device, each after dev_switch_prepare): one launch
-// per device step covers every instance; instance i's results are those dev_switch_rounds would
-// give its context alone.
+// The batch protocol (lbn_prove_batch): up to SWB_MAX proofs on contexts of one device advance
+// together. Between the batched launches, host threads run each proof's host steps and its
+// *_stage / *_prepare call, then dev_sync (its stream idle); a *_batch call then launches for
+// every instance at once on a stream of its own, each instance reading and writing only its own
+// context's buffers, and returns when the device is done; each proof's *_collect takes its
+// results from its mailbox. Instance i's results are those of the one-proof entry point.
 constexpr int SWB_MAX = 64;
-int dev_switch_batch(lbn_ctx *const *ctxs, int n, B3State *ts, uint64_t *const *rounds, uint64_t *const *coords,
-                     SwitchHandoff *ho);
+int dev_sync(lbn_ctx *ctx);
+// The switch: dev_switch_stage (after dev_switch_prepare) writes the start state from ts to the
+// mailbox; dev_switch_batch runs the device rounds; dev_switch_collect returns what
+// dev_switch_rounds returns.
+void dev_switch_stage(lbn_ctx *ctx, const B3State &ts);
+int dev_switch_batch(lbn_ctx *const *ctxs, int n);
+void dev_switch_collect(lbn_ctx *ctx, B3State &ts, uint64_t *rounds, uint64_t *coords, SwitchHandoff &ho);
 // The switch's start for dev_switch_rounds: the map L_0 from the batch (no kernel; the rounds start
 // from the prefix sets lbn_partials left).
 int dev_switch_prepare(lbn_ctx *ctx, const uint64_t *batch);
@@ -186,8 +194,34 @@ void lmap_next(std::vector<F162> &L, B128 c, const F162 &rho);
 // The 1024 folding challenges sampled on the device from the transcript state in arg (the
 // counter of arg->ts is the first challenge's); out receives them in column order.
 int dev_challenges(lbn_ctx *ctx, const ChalArg &arg, lbn_challenge *out);
-// The same for n proofs in one launch (contexts on one device): instance i from args[i] into outs[i].
-int dev_challenges_batch(lbn_ctx *const *ctxs, int n, const ChalArg *args, lbn_challenge *const *outs);
+// The same in the batch protocol: the transcript state staged, one launch, the 1024 challenges
+// collected in column order.
+int dev_challenges_stage(lbn_ctx *ctx, const ChalArg &arg);
+int dev_challenges_batch(lbn_ctx *const *ctxs, int n);
+void dev_challenges_collect(lbn_ctx *ctx, lbn_challenge *out);
+// The row evaluation in the batch protocol: dev_row_prepare computes the eq tables of p0 (row_log
+// coordinates) and queues their upload on the context's stream; dev_row_batch evaluates every row
+// in one launch per kernel; dev_row_collect gives the 1024 F162 values (3 u64 each) that
+// lbn_row_eval gives.
+int dev_row_prepare(lbn_ctx *ctx, const uint64_t *p0);
+int dev_row_batch(lbn_ctx *const *ctxs, int n);
+void dev_row_collect(lbn_ctx *ctx, uint64_t *row);
+// The fold in the batch protocol: dev_fold_prepare checks the challenges, builds the column lists
+// per position and queues their upload on the context's stream; dev_fold_batch folds every
+// instance in one launch; dev_fold_collect checks the context's range flag and puts its fold in
+// `out` (n_out int32) if the kernel did not write it there directly. lbn_fold runs the same parts
+// on the context's own stream.
+struct FoldPlan {
+    int nb = 0;          // counter planes the longest list needs
+    int n_entries = 0;   // list entries
+    int32_t *out = nullptr;
+    size_t n_out = 0;
+    int32_t *dst = nullptr;  // where the kernel writes: `out` itself when pinned, else the context's staging
+};
+int dev_fold_prepare(lbn_ctx *ctx, const lbn_challenge *ch, int n_challenges, int32_t *out, size_t n_out,
+                     FoldPlan &plan);
+int dev_fold_batch(lbn_ctx *const *ctxs, int n, const FoldPlan *plans);
+int dev_fold_collect(lbn_ctx *ctx, const FoldPlan &plan);
 // The streams the context's work goes to (hipStream_t, as void *): a caller driving many contexts
 // from a few threads gives each thread its own pair; null restores the context's own streams.
 // Only between entry points (every entry point leaves its streams idle).

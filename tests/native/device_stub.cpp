@@ -14,6 +14,14 @@
 
 struct lbn::DeviceState {
     std::vector<uint8_t> commitment;
+    // the batch protocol's staged inputs and results
+    lbn::B3State ts{};
+    std::vector<uint64_t> rounds = std::vector<uint64_t>(6 * 64), coords = std::vector<uint64_t>(3 * 64);
+    lbn::SwitchHandoff ho;
+    std::vector<uint64_t> p0, row;
+    lbn::ChalArg arg{};
+    std::vector<lbn_challenge> ch;
+    const lbn_challenge *fold_ch = nullptr;
 };
 
 namespace {
@@ -88,13 +96,23 @@ int dev_switch_rounds(lbn_ctx *ctx, B3State &ts, uint64_t *rounds, uint64_t *coo
     ctx->round = lp;
     return LBN_OK;
 }
-int dev_switch_batch(lbn_ctx *const *ctxs, int n, B3State *ts, uint64_t *const *rounds, uint64_t *const *coords,
-                     SwitchHandoff *ho) {
+int dev_sync(lbn_ctx *) { return LBN_OK; }
+void dev_switch_stage(lbn_ctx *ctx, const B3State &ts) { ctx->dev->ts = ts; }
+int dev_switch_batch(lbn_ctx *const *ctxs, int n) {
     for (int i = 0; i < n; i++) {
-        const int rc = dev_switch_rounds(ctxs[i], ts[i], rounds[i], coords[i], ho[i]);
+        DeviceState *d = ctxs[i]->dev;
+        const int rc = dev_switch_rounds(ctxs[i], d->ts, d->rounds.data(), d->coords.data(), d->ho);
         if (rc != LBN_OK) return rc;
     }
     return LBN_OK;
+}
+void dev_switch_collect(lbn_ctx *ctx, B3State &ts, uint64_t *rounds, uint64_t *coords, SwitchHandoff &ho) {
+    DeviceState *d = ctx->dev;
+    const int r0 = std::max(7, ctx->shape.log_packed - SWITCH_HOST_STEPS);
+    ts = d->ts;
+    memcpy(rounds, d->rounds.data(), (size_t)r0 * 48);
+    memcpy(coords, d->coords.data(), (size_t)r0 * 24);
+    ho = d->ho;
 }
 int dev_challenges(lbn_ctx *, const ChalArg &arg, lbn_challenge *out) {
     for (int j = 0; j < COLUMNS; j++) {
@@ -106,11 +124,52 @@ int dev_challenges(lbn_ctx *, const ChalArg &arg, lbn_challenge *out) {
     }
     return LBN_OK;
 }
-int dev_challenges_batch(lbn_ctx *const *ctxs, int n, const ChalArg *args, lbn_challenge *const *outs) {
-    for (int i = 0; i < n; i++) dev_challenges(ctxs[i], args[i], outs[i]);
+int dev_challenges_stage(lbn_ctx *ctx, const ChalArg &arg) {
+    ctx->dev->arg = arg;
     return LBN_OK;
 }
+int dev_challenges_batch(lbn_ctx *const *ctxs, int n) {
+    for (int i = 0; i < n; i++) {
+        DeviceState *d = ctxs[i]->dev;
+        d->ch.resize(COLUMNS);
+        dev_challenges(ctxs[i], d->arg, d->ch.data());
+    }
+    return LBN_OK;
+}
+void dev_challenges_collect(lbn_ctx *ctx, lbn_challenge *out) {
+    memcpy(out, ctx->dev->ch.data(), COLUMNS * sizeof(lbn_challenge));
+}
 void dev_use_streams(lbn_ctx *, void *, void *) {}
+int dev_row_prepare(lbn_ctx *ctx, const uint64_t *p0) {
+    ctx->dev->p0.assign(p0, p0 + 3 * ctx->shape.row_log);
+    return LBN_OK;
+}
+int dev_row_batch(lbn_ctx *const *ctxs, int n) {
+    for (int i = 0; i < n; i++) {
+        DeviceState *d = ctxs[i]->dev;
+        d->row.resize(3 * COLUMNS);
+        const int rc = lbn_row_eval(ctxs[i], d->p0.data(), ctxs[i]->shape.row_log, d->row.data());
+        if (rc != LBN_OK) return rc;
+    }
+    return LBN_OK;
+}
+void dev_row_collect(lbn_ctx *ctx, uint64_t *row) { memcpy(row, ctx->dev->row.data(), 24 * COLUMNS); }
+int dev_fold_prepare(lbn_ctx *ctx, const lbn_challenge *ch, int n_challenges, int32_t *out, size_t n_out,
+                     FoldPlan &plan) {
+    if (n_challenges != COLUMNS || n_out != (size_t)N * ctx->shape.ring_elements) return LBN_E_ARG;
+    ctx->dev->fold_ch = ch;
+    plan.out = out;
+    plan.n_out = n_out;
+    return LBN_OK;
+}
+int dev_fold_batch(lbn_ctx *const *ctxs, int n, const FoldPlan *plans) {
+    for (int i = 0; i < n; i++) {
+        const int rc = lbn_fold(ctxs[i], ctxs[i]->dev->fold_ch, COLUMNS, plans[i].out, plans[i].n_out);
+        if (rc != LBN_OK) return rc;
+    }
+    return LBN_OK;
+}
+int dev_fold_collect(lbn_ctx *, const FoldPlan &) { return LBN_OK; }
 }  // namespace lbn
 
 extern "C" int lbn_load_table(lbn_ctx *ctx, const void *host_bytes, size_t bytes) {

@@ -80,6 +80,42 @@ static void scheduler_stress() {
     lbn_sched_destroy(s);
 }
 
+// The batch prover's phases on its host threads: every item's proof equals the same context's solo
+// proof (the stub's outputs are deterministic), over batches of several sizes and thread counts.
+static void batch_stress() {
+    const int n = 20;  // the full batch runs as two groups, the half batch as one
+    std::vector<lbn_ctx *> ctx(n, nullptr);
+    for (int i = 0; i < n; i++) {
+        CHECK(lbn_create(0, 19 + i % 4, &ctx[i]) == LBN_OK);
+        if (!ctx[i]) return;
+        CHECK(lbn_set_modulus(ctx[i], 35039150080ull, 0) == LBN_OK);
+    }
+    std::vector<std::vector<uint8_t>> solo(n), out(n);
+    for (int i = 0; i < n; i++) {
+        uint8_t seed[32] = {(uint8_t)i, 7};
+        CHECK(lbn_generate_table(ctx[i], seed) == LBN_OK);
+        solo[i].resize(lbn_proof_len(19 + i % 4, 35039150080ull, 0));
+        size_t len = 0;
+        CHECK(lbn_prove(ctx[i], seed, solo[i].data(), solo[i].size(), &len, nullptr) == LBN_OK);
+        out[i].resize(solo[i].size());
+    }
+    for (int rep = 0; rep < 2 * scale; rep++) {
+        const int m = rep % 2 ? n : n / 2 + 1, threads = 1 + rep % 5;
+        std::vector<lbn_batch_item> items(m);
+        for (int i = 0; i < m; i++) {
+            memset(&items[i], 0, sizeof items[i]);
+            items[i].ctx = ctx[i];
+            items[i].seed[0] = (uint8_t)i, items[i].seed[1] = 7;
+            items[i].gen_table = 1;
+            items[i].proof = out[i].data();
+            items[i].cap = out[i].size();
+        }
+        CHECK(lbn_prove_batch(items.data(), m, threads) == LBN_OK);
+        for (int i = 0; i < m; i++) CHECK(items[i].len == solo[i].size() && out[i] == solo[i]);
+    }
+    for (lbn_ctx *c : ctx) lbn_destroy(c);
+}
+
 static void checker_decode_fuzz() {
     lbn_ctx *ctx = nullptr;
     CHECK(lbn_create(0, 20, &ctx) == LBN_OK);
@@ -177,6 +213,8 @@ int main(int argc, char **argv) {
     checker_decode_fuzz();
     fprintf(stderr, "checker decode done\n");
     scheduler_stress();
-    fprintf(stderr, "scheduler done, failures %d\n", failures);
+    fprintf(stderr, "scheduler done\n");
+    batch_stress();
+    fprintf(stderr, "batch done, failures %d\n", failures);
     return failures ? 1 : 0;
 }

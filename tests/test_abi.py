@@ -58,6 +58,12 @@ def test_scheduler_lifecycle_and_errors_without_a_device():
             sch.submit(31, bytes(32))  # never fits the budget
         with pytest.raises(m.LbnError):
             sch.submit(40, bytes(32))
+        import numpy as np
+        short = np.empty(m.proof_len(20, m.DEFAULT_Q) - 1, dtype=np.uint8)
+        with pytest.raises(ValueError):
+            sch.submit(20, bytes(32), out=short)  # a caller's buffer below the proof length
+        with pytest.raises(ValueError):
+            sch.submit(20, bytes(32), out=np.empty(m.proof_len(20, m.DEFAULT_Q), dtype=np.int32))
         assert sch.stats()["submitted"] == 0
     finally:
         sch.close()

@@ -72,6 +72,27 @@ def test_mix_equals_alone_and_oracle(m, alone):
         assert alone(k, seed_of(i)) == O.prove(seed_of(i), k)
 
 
+
+def test_mix_into_caller_buffers(m, alone):
+    say("scheduler: 10 jobs into slices of one pinned buffer and of one pageable array, 4 workers")
+    sizes = [(m.proof_len(k, m.DEFAULT_Q) + 255) // 256 * 256 for k, _ in MIX]
+    offs = np.concatenate([[0], np.cumsum(sizes)])
+    pin = m.PinnedBuffer(int(offs[-1]))
+    page = np.zeros(int(offs[-1]), dtype=np.uint8)
+    sch = m.Scheduler(m.DEFAULT_Q, hbm_budget=8 << 30, workers=4)
+    try:
+        for arr in (pin.array, page):
+            hs = [(k, seed_of(i), sch.submit(k, seed_of(i), out=arr[offs[j]:offs[j] + sizes[j]]))
+                  for j, (k, i) in enumerate(MIX)]
+            sch.drain()
+            for j, (k, seed, h) in enumerate(hs):
+                proof = sch.result(h)[0]
+                assert proof == alone(k, seed), (k, seed[:2].hex())
+                assert arr[offs[j]:offs[j] + len(proof)].tobytes() == proof
+    finally:
+        sch.close()
+        pin.close()
+
 def test_prewarmed_contexts(m, alone):
     say("scheduler: prewarm 3 contexts of k=20 and 2 of k=21, 6 jobs on 3 workers")
     sch = m.Scheduler(m.DEFAULT_Q, hbm_budget=8 << 30, workers=3)

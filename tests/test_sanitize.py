@@ -2,8 +2,9 @@
 the CPU: tests/native builds the worker pool, the scheduler, the transcript and the prover's and
 checker's host orchestration against a stand-in for the device half (tests/native/device_stub.cpp)
 and stresses them (tests/native/sanitize_driver.cpp): many callers of the pool at once, twelve
-scheduler workers on mixed sizes, the checker's decoder on damaged proofs, the transcript's split
-absorb. Any sanitizer report or failed check fails the test.
+scheduler workers on mixed sizes, the batch prover's phases on its host threads (each batch proof
+equal to its solo proof), the checker's decoder on damaged proofs, the transcript's split absorb.
+Any sanitizer report or failed check fails the test.
 """
 import os
 import shutil
@@ -32,4 +33,4 @@ def test_host_code_under_sanitizer(built, kind, scale):
     r = subprocess.run([str(built / f"driver_{kind}"), scale], capture_output=True, text=True, timeout=900, env=env)
     assert r.returncode == 0, r.stderr[-4000:]
     assert "Sanitizer" not in r.stderr, r.stderr[-4000:]
-    assert "scheduler done, failures 0" in r.stderr
+    assert "batch done, failures 0" in r.stderr

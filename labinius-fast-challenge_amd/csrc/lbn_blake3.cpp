@@ -158,17 +158,26 @@ Blake3::Subtrees Blake3::subtrees(uint64_t first, size_t count) {
     return out;
 }
 
+Blake3::WithPlan Blake3::plan_with(size_t len) const {
+    // The head completes the current chunk; whole chunks strictly before the last byte go to the
+    // callback; the rest is absorbed by update_with itself.
+    WithPlan w;
+    const size_t in_chunk = (size_t)blocks_ * 64 + buf_len_;
+    w.take = in_chunk < CHUNK ? std::min(len, CHUNK - in_chunk) : 0;
+    const size_t rest = len - w.take;
+    w.whole = rest ? (rest - 1) / CHUNK : 0;
+    w.first = chunk_ + 1;  // the current chunk is closed before the whole ones
+    w.trees = subtrees(w.first, w.whole);
+    return w;
+}
+
 void Blake3::update_with(const uint8_t *p, size_t len,
                          const std::function<void(size_t, uint64_t, size_t, const Subtrees &, uint32_t *)> &cvs_of) {
-    // The head completes the current chunk; whole chunks strictly before the last byte are
-    // hashed by cvs_of (called before any byte is read, so it may also provide the bytes); the
-    // rest is absorbed here.
-    const size_t in_chunk = (size_t)blocks_ * 64 + buf_len_;
-    const size_t take = in_chunk < CHUNK ? std::min(len, CHUNK - in_chunk) : 0;
-    const size_t rest = len - take;
-    const size_t whole = rest ? (rest - 1) / CHUNK : 0;
-    const uint64_t first = chunk_ + 1;  // the current chunk is closed before the whole ones
-    const Subtrees trees = subtrees(first, whole);
+    // cvs_of is called before any byte is read, so it may also provide the bytes.
+    const WithPlan w = plan_with(len);
+    const size_t take = w.take, rest = len - take, whole = w.whole;
+    const uint64_t first = w.first;
+    const Subtrees &trees = w.trees;
     std::vector<uint32_t> cvs(trees.size() * 8);
     cvs_of(take, first, whole, trees, cvs.data());
     update(p, take);

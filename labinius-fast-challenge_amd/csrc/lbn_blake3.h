@@ -36,6 +36,14 @@ class Blake3 {
     // this hasher). Only the bytes outside those chunks are read here, after the call.
     void update_with(const uint8_t *data, size_t len,
                      const std::function<void(size_t, uint64_t, size_t, const Subtrees &, uint32_t *)> &cvs_of);
+    // The arguments update_with(data, len, cvs_of) passes to cvs_of from this state: the offset of
+    // the whole chunks (the head's length), the first chunk index, their count and their subtrees.
+    struct WithPlan {
+        size_t take, whole;
+        uint64_t first;
+        Subtrees trees;
+    };
+    WithPlan plan_with(size_t len) const;
     const uint32_t *key_words() const { return key_; }
     // The whole state as plain data (counter: the caller's transcript counter, carried along).
     template <class S>

@@ -38,6 +38,7 @@ struct Roots18 {
 
 struct SwitchDev;
 struct ToepOps;
+struct CvJob;
 
 struct lbn::DeviceState {
     hipStream_t stream = nullptr;
@@ -86,6 +87,7 @@ struct lbn::DeviceState {
     std::shared_ptr<::ToepOps> toep;
     int64_t *tpart = nullptr;
     int splits_try = 0;  // the MAC's split count while the autotuner times it (0: the chosen one)
+    std::unique_ptr<::CvJob> cvjob;  // the staged commitment hashing of the batch protocol
     // A host table whose upload is deferred (lbn_load_table_async): the commitment copies it in
     // ring-element ranges, each range's MAC waiting for its copy (events on the copy stream).
     const uint8_t *pend = nullptr;
@@ -330,11 +332,10 @@ struct B3Key {
 // subtrees of up to 64 chunks that lie in the run (CV_WIN_LEVELS levels), leaving each subtree's
 // value at its first chunk's slot, as k_cv_tree does for the levels above.
 constexpr int CV_WIN_LEVELS = 6;
-__global__ __launch_bounds__(64) void k_chunk_cvs(const uint8_t *bytes, size_t off, uint64_t first, size_t count,
-                                                  B3Key key, uint32_t *cvs) {
-    __shared__ uint32_t s_cv[64 * 8];
+__device__ __forceinline__ void chunk_cvs_body(int bid, const uint8_t *bytes, size_t off, uint64_t first, size_t count,
+                                               const B3Key &key, uint32_t *cvs, uint32_t *s_cv) {
     const int lane = threadIdx.x;
-    const uint64_t a = (first / 64 + blockIdx.x) * 64 + lane, end = first + count;
+    const uint64_t a = (first / 64 + bid) * 64 + lane, end = first + count;
     const bool in = a >= first && a < end;
     const size_t c = (size_t)(a - first);
     const uint64_t index = a;
@@ -383,6 +384,11 @@ __global__ __launch_bounds__(64) void k_chunk_cvs(const uint8_t *bytes, size_t o
 #pragma unroll
         for (int k = 0; k < 8; k++) cvs[8 * c + k] = cv[k];
 }
+__global__ __launch_bounds__(64) void k_chunk_cvs(const uint8_t *bytes, size_t off, uint64_t first, size_t count,
+                                                  B3Key key, uint32_t *cvs) {
+    __shared__ uint32_t s_cv[64 * 8];
+    chunk_cvs_body(blockIdx.x, bytes, off, first, count, key, cvs, s_cv);
+}
 
 // ---------------------------------------------------------------------------------------------
 // eq[jh * 2^lo + jl] = EH[jh] * EL[jl] in GF(2^128): EH[jh] times the lo-coordinate expansion EL
@@ -390,12 +396,9 @@ __global__ __launch_bounds__(64) void k_chunk_cvs(const uint8_t *bytes, size_t o
 // ---------------------------------------------------------------------------------------------
 // Block 0 also zeroes `clear` (the XOR accumulators the kernels after it fill), so that no
 // separate fill is launched.
-__global__ __launch_bounds__(256) void k_eq_expand(const uint4 *EH, const uint4 *EL, int lo, uint4 *E0,
-                                                   uint32_t *clear, int nclear) {
-    __shared__ uint4 basis[128];
-    __shared__ uint4 tab[16 * 256];
+__device__ __forceinline__ void eq_expand_body(uint32_t jh, const uint4 *EH, const uint4 *EL, int lo, uint4 *E0,
+                                               uint32_t *clear, int nclear, uint4 *basis, uint4 *tab) {
     const int t = threadIdx.x;
-    const uint32_t jh = blockIdx.x;
     if (jh == 0)
         for (int i = t; i < nclear; i += 256) clear[i] = 0;
     if (t < 128) {
@@ -437,6 +440,12 @@ __global__ __launch_bounds__(256) void k_eq_expand(const uint4 *EH, const uint4 
         }
         E0[((size_t)jh << lo) + jl] = acc;
     }
+}
+__global__ __launch_bounds__(256) void k_eq_expand(const uint4 *EH, const uint4 *EL, int lo, uint4 *E0,
+                                                   uint32_t *clear, int nclear) {
+    __shared__ uint4 basis[128];
+    __shared__ uint4 tab[16 * 256];
+    eq_expand_body(blockIdx.x, EH, EL, lo, E0, clear, nclear, basis, tab);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -568,12 +577,15 @@ __global__ __launch_bounds__(64 * PF_WAVES) void k_partials_fact(const uint4 *ta
 // chunk of 2^chunk_log words inside one prefix by wave-local M4RM, then XORs its sums into M_u and
 // their products by eq6(u) (a 2-bit window table of y -> eq6(u) y) into v; both zeroed before.
 // ---------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_prefix_sums(const uint4 *table, const uint4 *G, int S, int chunk_log,
-                                                     const uint4 *eq6, uint32_t *M, uint32_t *partials) {
-    __shared__ __align__(16) uint32_t lds[4 * M4W<4>::WORDS];
-    __shared__ uint32_t bs[512];
+struct PrefixLds {
+    __align__(16) uint32_t lds[4 * M4W<4>::WORDS];
+    uint32_t bs[512];
+};
+__device__ __forceinline__ void prefix_sums_body(size_t bid, PrefixLds &L, const uint4 *table, const uint4 *G, int S,
+                                                 int chunk_log, const uint4 *eq6, uint32_t *M, uint32_t *partials) {
+    uint32_t *lds = L.lds, *bs = L.bs;
     uint32_t *W = lds + (threadIdx.x >> 6) * M4W<4>::WORDS;
-    const size_t chunk = (size_t)1 << chunk_log, first = (size_t)blockIdx.x << chunk_log;
+    const size_t chunk = (size_t)1 << chunk_log, first = bid << chunk_log;
     const int u = (int)(first >> S);
     const uint4 *tw = table + first, *gw = G + (first & (((size_t)1 << S) - 1));
     uint32_t acc[2][4] = {};
@@ -625,6 +637,62 @@ __global__ __launch_bounds__(256) void k_prefix_sums(const uint4 *table, const u
         if (x.z) atomicXor(dst + 2, x.z);
         if (x.w) atomicXor(dst + 3, x.w);
     }
+}
+__global__ __launch_bounds__(256) void k_prefix_sums(const uint4 *table, const uint4 *G, int S, int chunk_log,
+                                                     const uint4 *eq6, uint32_t *M, uint32_t *partials) {
+    __shared__ PrefixLds L;
+    prefix_sums_body(blockIdx.x, L, table, G, S, chunk_log, eq6, M, partials);
+}
+
+// The view whose block range holds block b: the last i with starts(v[i]) <= b.
+template <class V, class F>
+__device__ __forceinline__ int view_of_block(const V *v, int n, int b, F starts) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (starts(v[mid]) <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+// The partials of several proofs: the eq expansions (proof i's blocks from eq_start), the prefix
+// sums (from ps_start), then each proof's 512 partial words copied to its mailbox.
+struct PartView {
+    const uint4 *EH, *EL;
+    int lo;
+    uint4 *E0;
+    uint32_t *clear;
+    int nclear, eq_start;
+    const uint4 *table;
+    int S, chunk_log, ps_start;
+    const uint4 *eq6;
+    uint32_t *M, *part, *part_out;
+};
+__global__ __launch_bounds__(256) void k_meq_expand(const PartView *v, int n) {
+    __shared__ uint4 basis[128];
+    __shared__ uint4 tab[16 * 256];
+    const PartView &x = v[view_of_block(v, n, blockIdx.x, [](const PartView &y) { return y.eq_start; })];
+    eq_expand_body(blockIdx.x - x.eq_start, x.EH, x.EL, x.lo, x.E0, x.clear, x.nclear, basis, tab);
+}
+__global__ __launch_bounds__(256) void k_mprefix_sums(const PartView *v, int n) {
+    __shared__ PrefixLds L;
+    const PartView &x = v[view_of_block(v, n, blockIdx.x, [](const PartView &y) { return y.ps_start; })];
+    prefix_sums_body(blockIdx.x - x.ps_start, L, x.table, x.E0, x.S, x.chunk_log, x.eq6, x.M, x.part);
+}
+// Inputs staged in the proofs' pinned mailboxes, copied to their device buffers in one launch for
+// all proofs (`per` blocks each) instead of one small copy per proof.
+struct Gather {
+    const uint4 *src;
+    uint4 *dst;
+    size_t n16;
+};
+__global__ __launch_bounds__(256) void k_mgather(const Gather *g, int per) {
+    const Gather x = g[blockIdx.x / per];
+    for (size_t i = (size_t)(blockIdx.x % per) * 256 + threadIdx.x; i < x.n16; i += (size_t)per * 256) x.dst[i] = x.src[i];
+}
+__global__ __launch_bounds__(256) void k_mpart_out(const PartView *v) {
+    const PartView &x = v[blockIdx.x];
+    for (int i = threadIdx.x; i < 512; i += 256) x.part_out[i] = x.part[i];
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -2977,15 +3045,22 @@ struct CvTrees {
     uint32_t start[MAX_CV_TREES];
     int level[MAX_CV_TREES];
 };
+// A commitment hashing staged for a batched launch: the chunk run, its subtrees, the key and the
+// head and tail bytes around it.
+struct CvJob {
+    size_t off, count, head, tail;
+    uint64_t first;
+    CvTrees tr;
+    B3Key key;
+};
 
-template <bool IN_LDS>
 // The subtree roots go to `roots` (the host's pinned buffer), and the commitment bytes before and
 // after the hashed chunks (cb[0, head), cb[tail, tail + tail_len)) to head_out / tail_out (the
 // host's mailbox), so the host needs no copies after the kernel.
-__global__ __launch_bounds__(1024) void k_cv_tree(uint32_t *cvs, int count, CvTrees tr, B3Key key, int l0,
-                                                  uint32_t *roots, const uint8_t *cb, int head, size_t tail,
-                                                  int tail_len, uint8_t *head_out, uint8_t *tail_out) {
-    extern __shared__ __align__(16) uint32_t s_cv[];
+template <bool IN_LDS>
+__device__ __forceinline__ void cv_tree_body(uint32_t *s_cv, uint32_t *cvs, int count, const CvTrees &tr,
+                                             const B3Key &key, int l0, uint32_t *roots, const uint8_t *cb, int head,
+                                             size_t tail, int tail_len, uint8_t *head_out, uint8_t *tail_out) {
     uint32_t *base = IN_LDS ? s_cv : cvs;
     const int t = threadIdx.x, q = t & 3, quad = t >> 2;
     for (int i = t; i < head; i += blockDim.x) head_out[i] = cb[i];
@@ -3046,6 +3121,42 @@ __global__ __launch_bounds__(1024) void k_cv_tree(uint32_t *cvs, int count, CvTr
     if (t < tr.n)
 #pragma unroll
         for (int k = 0; k < 8; k++) roots[8 * t + k] = v[k];
+}
+template <bool IN_LDS>
+__global__ __launch_bounds__(1024) void k_cv_tree(uint32_t *cvs, int count, CvTrees tr, B3Key key, int l0,
+                                                  uint32_t *roots, const uint8_t *cb, int head, size_t tail,
+                                                  int tail_len, uint8_t *head_out, uint8_t *tail_out) {
+    extern __shared__ __align__(16) uint32_t s_cv[];
+    cv_tree_body<IN_LDS>(s_cv, cvs, count, tr, key, l0, roots, cb, head, tail, tail_len, head_out, tail_out);
+}
+
+// The commitment hashing and the partials of several proofs: one launch per kernel, proof i's
+// blocks from its view's start to the next view's, each proof's arguments in its view.
+struct CvView {
+    const uint8_t *bytes;
+    size_t off;
+    uint64_t first;
+    size_t count;
+    B3Key key;
+    uint32_t *cvs;
+    int start;  // first window block
+    CvTrees tr;
+    uint32_t *roots;
+    int head, tail_len;
+    size_t tail;
+    uint8_t *head_out, *tail_out;
+};
+__global__ __launch_bounds__(64) void k_mchunk_cvs(const CvView *v, int n) {
+    __shared__ uint32_t s_cv[64 * 8];
+    const CvView &x = v[view_of_block(v, n, blockIdx.x, [](const CvView &y) { return y.start; })];
+    chunk_cvs_body((int)blockIdx.x - x.start, x.bytes, x.off, x.first, x.count, x.key, x.cvs, s_cv);
+}
+template <bool IN_LDS>
+__global__ __launch_bounds__(1024) void k_mcv_tree(const CvView *v) {
+    extern __shared__ __align__(16) uint32_t s_cv[];
+    const CvView &x = v[blockIdx.x];
+    cv_tree_body<IN_LDS>(s_cv, x.cvs, (int)x.count, x.tr, x.key, CV_WIN_LEVELS, x.roots, x.bytes, x.head, x.tail,
+                         x.tail_len, x.head_out, x.tail_out);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -3381,7 +3492,7 @@ int lbn::dev_init(lbn_ctx *ctx) {
     HIPCHK(ctx, hipHostMalloc((void **)&d->hfold, s.ring_elements * N * sizeof(int32_t)));
     HIPCHK(ctx, hipHostMalloc((void **)&d->hbox, mb::SIZE));
     HIPCHK(ctx, hipMalloc(&d->sw, sizeof(SwitchDev)));
-    HIPCHK(ctx, hipMalloc(&d->chal, sizeof(ChalArg)));
+    HIPCHK(ctx, hipMalloc(&d->chal, (sizeof(ChalArg) + 15) / 16 * 16));  // whole 16-byte words
     HIPCHK(ctx, hipHostGetDevicePointer((void **)&d->fold_host, d->hfold, 0));
     HIPCHK(ctx, hipHostGetDevicePointer((void **)&d->hbox_dev, d->hbox, 0));
     HIPCHK(ctx, hipStreamCreateWithFlags(&d->own_copy, hipStreamNonBlocking));
@@ -4005,14 +4116,19 @@ extern "C" int lbn_tune_set(int device, int log_vars, int mac_splits) {
     return LBN_OK;
 }
 
-int lbn::dev_commit_cvs(lbn_ctx *ctx, size_t off, uint64_t first, size_t count, const Blake3::Subtrees &trees,
-                        const uint32_t key[8], uint32_t flags, uint32_t *cvs, uint8_t *dst, size_t head, size_t tail) {
+// The commitment's hashing in three steps: the arguments checked and the commitment's copy to the
+// host queued (stage), the chunk and subtree kernels (launch, or one launch for several proofs),
+// the subtree roots and the bytes around the hashed chunks taken from pinned memory (collect).
+
+static int cvs_stage(lbn_ctx *ctx, size_t off, uint64_t first, size_t count, const Blake3::Subtrees &trees,
+                     const uint32_t key[8], uint32_t flags, uint8_t *dst, size_t head, size_t tail, CvJob &job) {
     DeviceState *d = ctx->dev;
     const size_t clen = (size_t)COLUMNS * N * ctx->width;
     if (off + count * 1024 > clen || head > clen || tail > clen)
         return ctx->fail(LBN_E_ARG, "commitment chunk range out of bounds");
     if (trees.size() > MAX_CV_TREES) return ctx->fail(LBN_E_ARG, "too many commitment subtrees");
-    CvTrees tr{};
+    CvTrees &tr = job.tr;
+    tr = CvTrees{};
     tr.n = (int)trees.size();
     {
         uint64_t at = first;  // the subtrees tile [first, first + count) in order
@@ -4034,37 +4150,78 @@ int lbn::dev_commit_cvs(lbn_ctx *ctx, size_t off, uint64_t first, size_t count, 
     HIPCHK(ctx, hipMemcpyAsync(d->staged_direct ? dst : d->hstage, d->cbytes, clen, hipMemcpyDeviceToHost,
                                d->copy_stream));
     HIPCHK(ctx, hipEventRecord(d->ev_staged, d->copy_stream));
-    if (count) {
-        B3Key k;
-        for (int i = 0; i < 8; i++) k.k[i] = key[i];
-        k.flags = flags;
-        const unsigned windows = (unsigned)((first + count - 1) / 64 - first / 64 + 1);
-        k_chunk_cvs<<<windows, 64, 0, d->stream>>>(d->cbytes, off, first, count, k, d->cvs);
-        HIPCHK(ctx, hipGetLastError());
-        // the roots, head and tail go straight to pinned host memory
-        const int tail_len = (int)(clen - tail);
-        uint8_t *ho = d->hbox_dev + mb::HEAD, *to = d->hbox_dev + mb::TAIL;
-        if (count * 32 <= CV_TREE_LDS_MAX)
-            k_cv_tree<true><<<1, 1024, count * 32, d->stream>>>(d->cvs, (int)count, tr, k, CV_WIN_LEVELS, d->hcvs_dev,
-                                                                d->cbytes, (int)head, tail, tail_len, ho, to);
-        else
-            k_cv_tree<false><<<1, 1024, 0, d->stream>>>(d->cvs, (int)count, tr, k, CV_WIN_LEVELS, d->hcvs_dev,
-                                                        d->cbytes, (int)head, tail, tail_len, ho, to);
-        HIPCHK(ctx, hipGetLastError());
-    } else {
-        if (head) HIPCHK(ctx, hipMemcpyAsync(d->hbox + mb::HEAD, d->cbytes, head, hipMemcpyDeviceToHost, d->stream));
-        if (tail < clen)
-            HIPCHK(ctx, hipMemcpyAsync(d->hbox + mb::TAIL, d->cbytes + tail, clen - tail, hipMemcpyDeviceToHost,
-                                       d->stream));
-    }
-    HIPCHK(ctx, hipStreamSynchronize(d->stream));
-    memcpy(cvs, d->hcvs, trees.size() * 32);
-    memcpy(dst, d->hbox + mb::HEAD, head);
-    memcpy(dst + tail, d->hbox + mb::TAIL, clen - tail);
-    tcollect(ctx, T_MAC);
-    tcollect(ctx, T_FINISH);
+    job.off = off, job.first = first, job.count = count, job.head = head, job.tail = tail;
+    for (int i = 0; i < 8; i++) job.key.k[i] = key[i];
+    job.key.flags = flags;
     return LBN_OK;
 }
+
+static unsigned cvs_windows(const CvJob &job) {
+    return job.count ? (unsigned)((job.first + job.count - 1) / 64 - job.first / 64 + 1) : 0u;
+}
+
+static int cvs_launch(lbn_ctx *ctx, const CvJob &job) {
+    DeviceState *d = ctx->dev;
+    const size_t clen = (size_t)COLUMNS * N * ctx->width;
+    if (job.count) {
+        k_chunk_cvs<<<cvs_windows(job), 64, 0, d->stream>>>(d->cbytes, job.off, job.first, job.count, job.key, d->cvs);
+        HIPCHK(ctx, hipGetLastError());
+        // the roots, head and tail go straight to pinned host memory
+        const int tail_len = (int)(clen - job.tail);
+        uint8_t *ho = d->hbox_dev + mb::HEAD, *to = d->hbox_dev + mb::TAIL;
+        if (job.count * 32 <= CV_TREE_LDS_MAX)
+            k_cv_tree<true><<<1, 1024, job.count * 32, d->stream>>>(d->cvs, (int)job.count, job.tr, job.key,
+                                                                    CV_WIN_LEVELS, d->hcvs_dev, d->cbytes,
+                                                                    (int)job.head, job.tail, tail_len, ho, to);
+        else
+            k_cv_tree<false><<<1, 1024, 0, d->stream>>>(d->cvs, (int)job.count, job.tr, job.key, CV_WIN_LEVELS,
+                                                        d->hcvs_dev, d->cbytes, (int)job.head, job.tail, tail_len,
+                                                        ho, to);
+        HIPCHK(ctx, hipGetLastError());
+    } else {
+        if (job.head)
+            HIPCHK(ctx, hipMemcpyAsync(d->hbox + mb::HEAD, d->cbytes, job.head, hipMemcpyDeviceToHost, d->stream));
+        if (job.tail < clen)
+            HIPCHK(ctx, hipMemcpyAsync(d->hbox + mb::TAIL, d->cbytes + job.tail, clen - job.tail,
+                                       hipMemcpyDeviceToHost, d->stream));
+    }
+    return LBN_OK;
+}
+
+static void cvs_collect(lbn_ctx *ctx, const CvJob &job, uint32_t *cvs, uint8_t *dst) {
+    DeviceState *d = ctx->dev;
+    const size_t clen = (size_t)COLUMNS * N * ctx->width;
+    memcpy(cvs, d->hcvs, (size_t)job.tr.n * 32);
+    memcpy(dst, d->hbox + mb::HEAD, job.head);
+    memcpy(dst + job.tail, d->hbox + mb::TAIL, clen - job.tail);
+    tcollect(ctx, T_MAC);
+    tcollect(ctx, T_FINISH);
+}
+
+int lbn::dev_commit_cvs(lbn_ctx *ctx, size_t off, uint64_t first, size_t count, const Blake3::Subtrees &trees,
+                        const uint32_t key[8], uint32_t flags, uint32_t *cvs, uint8_t *dst, size_t head, size_t tail) {
+    CvJob job;
+    {
+        const int rc = cvs_stage(ctx, off, first, count, trees, key, flags, dst, head, tail, job);
+        if (rc != LBN_OK) return rc;
+    }
+    {
+        const int rc = cvs_launch(ctx, job);
+        if (rc != LBN_OK) return rc;
+    }
+    HIPCHK(ctx, hipStreamSynchronize(ctx->dev->stream));
+    cvs_collect(ctx, job, cvs, dst);
+    return LBN_OK;
+}
+
+int lbn::dev_cvs_stage(lbn_ctx *ctx, const Blake3::Subtrees &trees, size_t off, uint64_t first, size_t count,
+                       const uint32_t key[8], uint32_t flags, uint8_t *dst, size_t head, size_t tail) {
+    DeviceState *d = ctx->dev;
+    if (!d->cvjob) d->cvjob.reset(new CvJob());
+    return cvs_stage(ctx, off, first, count, trees, key, flags, dst, head, tail, *d->cvjob);
+}
+
+void lbn::dev_cvs_collect(lbn_ctx *ctx, uint32_t *cvs, uint8_t *dst) { cvs_collect(ctx, *ctx->dev->cvjob, cvs, dst); }
 
 void lbn::dev_quiesce(lbn_ctx *ctx) {
     DeviceState *d = ctx->dev;
@@ -4194,8 +4351,20 @@ int lbn::dev_commit_launch(lbn_ctx *ctx) {
 // partials (crossfield.rs:152-165) through the prefix sets: G = eq(r_hi[6..]) expanded into E[0]
 // (its high coordinates, at most 6, times the low ones, at most 12), then the sets M_u and the
 // partials in one pass over the table (k_prefix_sums). M_u and G stay for the switch.
-extern "C" int lbn_partials(lbn_ctx *ctx, const uint64_t *r_hi, uint64_t *partials) {
-    if (!ctx || !r_hi || !partials) return LBN_E_ARG;
+// The partials in three steps: the eq tables of r_hi on the host and their upload (prepare), the eq
+// expansion and prefix sums (launch, or one launch each for several proofs), the 128 partials read
+// from the mailbox (collect).
+struct PartShape {
+    int lp, S, lo, nh, chunk_log;
+};
+static PartShape part_shape(const Shape &s) {
+    PartShape p;
+    p.lp = s.log_packed, p.S = p.lp - 6, p.lo = std::min(p.S, 12), p.nh = p.S - p.lo;
+    p.chunk_log = std::min(p.S, 14);
+    return p;
+}
+
+int lbn::dev_partials_prepare(lbn_ctx *ctx, const uint64_t *r_hi, bool upload) {
     if (!ctx->table_loaded) return ctx->fail(LBN_E_STATE, "no table loaded");
     HIPCHK(ctx, hipSetDevice(ctx->device));
     {
@@ -4203,38 +4372,52 @@ extern "C" int lbn_partials(lbn_ctx *ctx, const uint64_t *r_hi, uint64_t *partia
         if (rc_ != LBN_OK) return rc_;
     }
     DeviceState *d = ctx->dev;
-    const Shape &s = ctx->shape;
-    const int lp = s.log_packed, S = lp - 6, lo = std::min(S, 12), nh = S - lo;
-    ctx->r_hi.assign((const B128 *)r_hi, (const B128 *)r_hi + lp);
+    const PartShape p = part_shape(ctx->shape);
+    ctx->r_hi.assign((const B128 *)r_hi, (const B128 *)r_hi + p.lp);
     B128 *EL = (B128 *)(d->hbox + mb::EL), *EH = (B128 *)(d->hbox + mb::EH);
-    eq_expand_b128(ctx->r_hi.data() + lp - lo, lo, EL);
-    eq_expand_b128(ctx->r_hi.data() + 6, nh, EH);
+    eq_expand_b128(ctx->r_hi.data() + p.lp - p.lo, p.lo, EL);
+    eq_expand_b128(ctx->r_hi.data() + 6, p.nh, EH);
     eq_expand_b128(ctx->r_hi.data(), 6, EH + 64);  // eq6, after the at most 64 high values
     static_assert(mb::EH == mb::EL + 4096 * 16, "EL and EH adjacent, as on the device");
-    HIPCHK(ctx, hipMemcpyAsync(d->EL, d->hbox + mb::EL, (4096 + 128) * 16, hipMemcpyHostToDevice, d->stream));
-    tstart(ctx, T_EQ);
-    // also zeroes the sets and the partials' accumulator
-    k_eq_expand<<<1u << nh, 256, 0, d->stream>>>(d->EH, d->EL, lo, d->E[0], d->pset, (int)ps::Q);
-    tstop(ctx, T_EQ);
-    HIPCHK(ctx, hipGetLastError());
+    if (upload)
+        HIPCHK(ctx, hipMemcpyAsync(d->EL, d->hbox + mb::EL, (4096 + 128) * 16, hipMemcpyHostToDevice, d->stream));
     d->msg_clean = false;
-    const int chunk_log = std::min(S, 14);
-    tstart(ctx, T_PARTIALS);
-    k_prefix_sums<<<1u << (lp - chunk_log), 256, 0, d->stream>>>(d->table, d->E[0], S, chunk_log, d->EH + 64,
-                                                                 d->pset + ps::M, d->pset + ps::PART);
-    tstop(ctx, T_PARTIALS);
-    HIPCHK(ctx, hipGetLastError());
-    const uint32_t *out = (const uint32_t *)(d->hbox + mb::PART);
-    HIPCHK(ctx, hipMemcpyAsync(d->hbox + mb::PART, d->pset + ps::PART, 512 * 4, hipMemcpyDeviceToHost, d->stream));
-    HIPCHK(ctx, hipStreamSynchronize(d->stream));
-    tcollect(ctx, T_EQ);
-    tcollect(ctx, T_PARTIALS);
+    return LBN_OK;
+}
+
+void lbn::dev_partials_collect(lbn_ctx *ctx, uint64_t *partials) {
+    const uint32_t *out = (const uint32_t *)(ctx->dev->hbox + mb::PART);
     for (int k = 0; k < 128; k++) {
         partials[2 * k] = (uint64_t)out[4 * k] | ((uint64_t)out[4 * k + 1] << 32);
         partials[2 * k + 1] = (uint64_t)out[4 * k + 2] | ((uint64_t)out[4 * k + 3] << 32);
     }
     ctx->partials_done = true;
     ctx->round = -1;
+}
+
+extern "C" int lbn_partials(lbn_ctx *ctx, const uint64_t *r_hi, uint64_t *partials) {
+    if (!ctx || !r_hi || !partials) return LBN_E_ARG;
+    {
+        const int rc_ = dev_partials_prepare(ctx, r_hi, true);
+        if (rc_ != LBN_OK) return rc_;
+    }
+    DeviceState *d = ctx->dev;
+    const PartShape p = part_shape(ctx->shape);
+    tstart(ctx, T_EQ);
+    // also zeroes the sets and the partials' accumulator
+    k_eq_expand<<<1u << p.nh, 256, 0, d->stream>>>(d->EH, d->EL, p.lo, d->E[0], d->pset, (int)ps::Q);
+    tstop(ctx, T_EQ);
+    HIPCHK(ctx, hipGetLastError());
+    tstart(ctx, T_PARTIALS);
+    k_prefix_sums<<<1u << (p.lp - p.chunk_log), 256, 0, d->stream>>>(d->table, d->E[0], p.S, p.chunk_log, d->EH + 64,
+                                                                     d->pset + ps::M, d->pset + ps::PART);
+    tstop(ctx, T_PARTIALS);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipMemcpyAsync(d->hbox + mb::PART, d->pset + ps::PART, 512 * 4, hipMemcpyDeviceToHost, d->stream));
+    HIPCHK(ctx, hipStreamSynchronize(d->stream));
+    tcollect(ctx, T_EQ);
+    tcollect(ctx, T_PARTIALS);
+    dev_partials_collect(ctx, partials);
     return LBN_OK;
 }
 
@@ -4742,13 +4925,37 @@ struct SwBatchDev {
     struct ChalPtrs *chal = nullptr;
     RowView *rows = nullptr;
     FoldView *folds = nullptr;
+    CvView *cvs = nullptr;
+    PartView *parts = nullptr;
+    Gather *gather = nullptr;
 };
+// Proof i's `bytes` staged at its mailbox's `region` into its device buffer dst[i], for every
+// proof of the batch in one launch on the batch stream.
+static int gather_launch(lbn_ctx *const *ctxs, int n, SwBatchDev &bd, size_t region, uint4 *const *dst,
+                         const size_t *bytes) {
+    std::vector<Gather> g(n);
+    size_t most = 0;
+    for (int i = 0; i < n; i++) {
+        g[i].src = (const uint4 *)(ctxs[i]->dev->hbox_dev + region);
+        g[i].dst = dst[i];
+        g[i].n16 = (bytes[i] + 15) / 16;
+        most = std::max(most, g[i].n16);
+    }
+    const int per = (int)std::min<size_t>(8, (most + 255) / 256);
+    HIPCHK(ctxs[0], hipMemcpyAsync(bd.gather, g.data(), sizeof(Gather) * n, hipMemcpyHostToDevice, bd.stream));
+    if (per > 0) k_mgather<<<n * per, 256, 0, bd.stream>>>(bd.gather, per);
+    HIPCHK(ctxs[0], hipGetLastError());
+    return LBN_OK;
+}
 static int swb_ready(lbn_ctx *ctx, SwBatchDev &bd) {
     if (bd.stream) return LBN_OK;
     HIPCHK(ctx, hipMalloc(&bd.views, sizeof(SwView) * SWB_MAX));
     HIPCHK(ctx, hipMalloc(&bd.chal, sizeof(ChalPtrs) * SWB_MAX));
     HIPCHK(ctx, hipMalloc(&bd.rows, sizeof(RowView) * SWB_MAX));
     HIPCHK(ctx, hipMalloc(&bd.folds, sizeof(FoldView) * SWB_MAX));
+    HIPCHK(ctx, hipMalloc(&bd.cvs, sizeof(CvView) * SWB_MAX));
+    HIPCHK(ctx, hipMalloc(&bd.parts, sizeof(PartView) * SWB_MAX));
+    HIPCHK(ctx, hipMalloc(&bd.gather, sizeof(Gather) * SWB_MAX));
     HIPCHK(ctx, hipStreamCreateWithFlags(&bd.stream, hipStreamNonBlocking));  // last: it marks the set ready
     return LBN_OK;
 }
@@ -4905,7 +5112,7 @@ extern "C" int lbn_switch_final(lbn_ctx *ctx, uint64_t *opened) {
 
 static int row_bits_lo(const Shape &s) { return std::min(s.row_log, 7); }
 
-int lbn::dev_row_prepare(lbn_ctx *ctx, const uint64_t *p0) {
+int lbn::dev_row_prepare(lbn_ctx *ctx, const uint64_t *p0, bool upload) {
     const Shape &s = ctx->shape;
     if (!ctx->table_loaded) return ctx->fail(LBN_E_STATE, "no table loaded");
     HIPCHK(ctx, hipSetDevice(ctx->device));
@@ -4920,7 +5127,9 @@ int lbn::dev_row_prepare(lbn_ctx *ctx, const uint64_t *p0) {
     eq_table_f162((const F162 *)p0 + lo_bits, hi_bits, hi.data());
     memcpy(d->hbox + mb::EQ, lo.data(), lo.size() * 24);
     memcpy(d->hbox + mb::EQ + 128 * 24, hi.data(), hi.size() * 24);
-    HIPCHK(ctx, hipMemcpyAsync(d->eqlo, d->hbox + mb::EQ, (128 + hi.size()) * 24, hipMemcpyHostToDevice, d->stream));
+    if (upload)
+        HIPCHK(ctx, hipMemcpyAsync(d->eqlo, d->hbox + mb::EQ, (128 + hi.size()) * 24, hipMemcpyHostToDevice,
+                                   d->stream));
     return LBN_OK;
 }
 
@@ -4929,7 +5138,7 @@ extern "C" int lbn_row_eval(lbn_ctx *ctx, const uint64_t *p0, int n_p0, uint64_t
     const Shape &s = ctx->shape;
     if (n_p0 != s.row_log) return ctx->fail(LBN_E_ARG, "p0 must have log_vars - 17 coordinates");
     {
-        const int rc_ = dev_row_prepare(ctx, p0);
+        const int rc_ = dev_row_prepare(ctx, p0, true);
         if (rc_ != LBN_OK) return rc_;
     }
     DeviceState *d = ctx->dev;
@@ -4951,7 +5160,7 @@ extern "C" int lbn_row_eval(lbn_ctx *ctx, const uint64_t *p0, int n_p0, uint64_t
 // ----------------------------------------------------------------------------- fold
 
 int lbn::dev_fold_prepare(lbn_ctx *ctx, const lbn_challenge *ch, int n_challenges, int32_t *out, size_t n_out,
-                          FoldPlan &plan) {
+                          FoldPlan &plan, bool upload) {
     const Shape &s = ctx->shape;
     if (n_challenges != COLUMNS) return ctx->fail(LBN_E_ARG, "one challenge per column (1024)");
     if (n_out != s.ring_elements * N) return ctx->fail(LBN_E_ARG, "fold output has the wrong length");
@@ -5000,8 +5209,10 @@ int lbn::dev_fold_prepare(lbn_ctx *ctx, const lbn_challenge *ch, int n_challenge
     memcpy(d->hbox + mb::FOFF, off.data(), off.size() * sizeof(int));
     memcpy(d->hbox + mb::FCOLS, cols.data(), cols.size() * 2);
     // offsets and lists in one copy (the device buffers keep the mailbox's relative layout)
-    HIPCHK(ctx, hipMemcpyAsync(d->fold_off, d->hbox + mb::FOFF, (mb::FCOLS - mb::FOFF) + cols.size() * 2,
-                               hipMemcpyHostToDevice, d->stream));
+    plan.upload_bytes = (mb::FCOLS - mb::FOFF) + cols.size() * 2;
+    if (upload)
+        HIPCHK(ctx, hipMemcpyAsync(d->fold_off, d->hbox + mb::FOFF, plan.upload_bytes, hipMemcpyHostToDevice,
+                                   d->stream));
     // the range flag lives in the mailbox: cleared here, set by the kernel, read after the sync
     *(volatile uint32_t *)(d->hbox + mb::MISC) = 0;
     // into the caller's buffer when it is one of ours (pinned), else into the context's staging
@@ -5028,7 +5239,7 @@ extern "C" int lbn_fold(lbn_ctx *ctx, const lbn_challenge *ch, int n_challenges,
     if (!ctx || !ch || !out) return LBN_E_ARG;
     FoldPlan plan;
     {
-        const int rc_ = dev_fold_prepare(ctx, ch, n_challenges, out, n_out, plan);
+        const int rc_ = dev_fold_prepare(ctx, ch, n_challenges, out, n_out, plan, true);
         if (rc_ != LBN_OK) return rc_;
     }
     const Shape &s = ctx->shape;
@@ -5062,13 +5273,7 @@ extern "C" int lbn_fold(lbn_ctx *ctx, const lbn_challenge *ch, int n_challenges,
     return dev_fold_collect(ctx, plan);
 }
 
-int lbn::dev_challenges_stage(lbn_ctx *ctx, const ChalArg &arg) {
-    DeviceState *d = ctx->dev;
-    memcpy(d->hbox + mb::CH, &arg, sizeof(ChalArg));
-    HIPCHK(ctx, hipSetDevice(ctx->device));
-    HIPCHK(ctx, hipMemcpyAsync(d->chal, d->hbox + mb::CH, sizeof(ChalArg), hipMemcpyHostToDevice, d->stream));
-    return LBN_OK;
-}
+void lbn::dev_challenges_stage(lbn_ctx *ctx, const ChalArg &arg) { memcpy(ctx->dev->hbox + mb::CH, &arg, sizeof(ChalArg)); }
 void lbn::dev_challenges_collect(lbn_ctx *ctx, lbn_challenge *out) {
     memcpy(out, ctx->dev->hbox + mb::CHOUT, COLUMNS * sizeof(lbn_challenge));
 }
@@ -5086,10 +5291,18 @@ int lbn::dev_challenges_batch(lbn_ctx *const *ctxs, int n) {
         const int rc = swb_ready(ctxs[0], bd);
         if (rc != LBN_OK) return rc;
     }
+    static_assert((sizeof(ChalArg) + 15) / 16 * 16 <= 8192, "the sampler's input in its mailbox region");
     std::vector<ChalPtrs> hp(n);
+    std::vector<uint4 *> dst(n);
+    std::vector<size_t> bytes(n, sizeof(ChalArg));
     for (int i = 0; i < n; i++) {
         DeviceState *d = ctxs[i]->dev;
         hp[i] = ChalPtrs{d->chal, (lbn_challenge *)(d->hbox_dev + mb::CHOUT)};
+        dst[i] = (uint4 *)d->chal;
+    }
+    {
+        const int rc = gather_launch(ctxs, n, bd, mb::CH, dst.data(), bytes.data());
+        if (rc != LBN_OK) return rc;
     }
     HIPCHK(ctxs[0], hipMemcpyAsync(bd.chal, hp.data(), sizeof(ChalPtrs) * n, hipMemcpyHostToDevice, bd.stream));
     k_mchallenges<<<n * (COLUMNS / chal::WAVES), 64 * chal::WAVES, 0, bd.stream>>>(bd.chal);
@@ -5123,8 +5336,8 @@ int lbn::dev_row_batch(lbn_ctx *const *ctxs, int n) {
         RowView &x = hv[i];
         x.table = d->table;
         x.clen = s.column_len;
-        x.lo = (const uint64_t *)d->eqlo;
-        x.hi = (const uint64_t *)((uint8_t *)d->eqlo + 128 * 24);
+        x.lo = (const uint64_t *)(d->hbox_dev + mb::EQ);  // read once per thread straight from the mailbox
+        x.hi = (const uint64_t *)(d->hbox_dev + mb::EQ + 128 * 24);
         x.lo_bits = row_bits_lo(s);
         x.eq_start = eq_blocks;
         x.eqrow = d->eqrow;
@@ -5178,6 +5391,13 @@ int lbn::dev_fold_batch(lbn_ctx *const *ctxs, int n, const FoldPlan *plans) {
     const size_t per = (total + cus - 1) / cus;
     const unsigned grid = (unsigned)((total + per - 1) / per);
     hipStream_t st = bd.stream;
+    {  // the column lists staged in the mailboxes
+        std::vector<uint4 *> dst(n);
+        std::vector<size_t> bytes(n);
+        for (int i = 0; i < n; i++) dst[i] = (uint4 *)ctxs[i]->dev->fold_off, bytes[i] = plans[i].upload_bytes;
+        const int rc = gather_launch(ctxs, n, bd, mb::FOFF, dst.data(), bytes.data());
+        if (rc != LBN_OK) return rc;
+    }
     HIPCHK(ctxs[0], hipMemcpyAsync(bd.folds, hv.data(), sizeof(FoldView) * n, hipMemcpyHostToDevice, st));
     switch (nb) {
         case 6: k_mfold<6><<<grid, FOLD_THREADS, 0, st>>>(bd.folds, n, per); break;
@@ -5189,6 +5409,118 @@ int lbn::dev_fold_batch(lbn_ctx *const *ctxs, int n, const FoldPlan *plans) {
         case 12: k_mfold<12><<<grid, FOLD_THREADS, 0, st>>>(bd.folds, n, per); break;
         default: return ctxs[0]->fail(LBN_E_ARG, "challenge lists too long");
     }
+    HIPCHK(ctxs[0], hipGetLastError());
+    HIPCHK(ctxs[0], hipStreamSynchronize(st));
+    return LBN_OK;
+}
+
+// The batched steps' common start: the contexts checked, the device current and its batch buffers.
+static int batch_begin(lbn_ctx *const *ctxs, int n, SwBatchDev *&bd) {
+    {
+        const int rc = batch_check(ctxs, n);
+        if (rc != LBN_OK) return rc;
+    }
+    HIPCHK(ctxs[0], hipSetDevice(ctxs[0]->device));
+    bd = &swb_dev[ctxs[0]->device];
+    return swb_ready(ctxs[0], *bd);
+}
+
+int lbn::dev_cvs_batch(lbn_ctx *const *ctxs, int n) {
+    std::lock_guard<std::mutex> lk(swb_mu);
+    SwBatchDev *bd = nullptr;
+    {
+        const int rc = batch_begin(ctxs, n, bd);
+        if (rc != LBN_OK) return rc;
+    }
+    std::vector<CvView> hv(n);
+    int windows = 0;
+    size_t max_count = 0;
+    for (int i = 0; i < n; i++) {
+        lbn_ctx *ctx = ctxs[i];
+        DeviceState *d = ctx->dev;
+        if (!d->cvjob) return ctx->fail(LBN_E_STATE, "no commitment hashing staged");
+        const CvJob &job = *d->cvjob;
+        const size_t clen = (size_t)COLUMNS * N * ctx->width;
+        CvView &x = hv[i];
+        x.bytes = d->cbytes;
+        x.off = job.off;
+        x.first = job.first;
+        x.count = job.count;
+        x.key = job.key;
+        x.cvs = d->cvs;
+        x.start = windows;
+        x.tr = job.tr;
+        x.roots = d->hcvs_dev;
+        x.head = (int)job.head;
+        x.tail = job.tail;
+        x.tail_len = (int)(clen - job.tail);
+        x.head_out = d->hbox_dev + mb::HEAD;
+        x.tail_out = d->hbox_dev + mb::TAIL;
+        windows += (int)cvs_windows(job);
+        max_count = std::max(max_count, job.count);
+    }
+    hipStream_t st = bd->stream;
+    HIPCHK(ctxs[0], hipMemcpyAsync(bd->cvs, hv.data(), sizeof(CvView) * n, hipMemcpyHostToDevice, st));
+    if (windows) {
+        k_mchunk_cvs<<<windows, 64, 0, st>>>(bd->cvs, n);
+        HIPCHK(ctxs[0], hipGetLastError());
+    }
+    // every proof's subtrees, head and tail (a proof without whole chunks copies only those)
+    if (max_count * 32 <= CV_TREE_LDS_MAX)
+        k_mcv_tree<true><<<n, 1024, max_count * 32, st>>>(bd->cvs);
+    else
+        k_mcv_tree<false><<<n, 1024, 0, st>>>(bd->cvs);
+    HIPCHK(ctxs[0], hipGetLastError());
+    HIPCHK(ctxs[0], hipStreamSynchronize(st));
+    return LBN_OK;
+}
+
+int lbn::dev_partials_batch(lbn_ctx *const *ctxs, int n) {
+    std::lock_guard<std::mutex> lk(swb_mu);
+    SwBatchDev *bd = nullptr;
+    {
+        const int rc = batch_begin(ctxs, n, bd);
+        if (rc != LBN_OK) return rc;
+    }
+    std::vector<PartView> hv(n);
+    int eq_blocks = 0, ps_blocks = 0;
+    for (int i = 0; i < n; i++) {
+        lbn_ctx *ctx = ctxs[i];
+        DeviceState *d = ctx->dev;
+        const PartShape p = part_shape(ctx->shape);
+        PartView &x = hv[i];
+        x.EH = d->EH;
+        x.EL = d->EL;
+        x.lo = p.lo;
+        x.E0 = d->E[0];
+        x.clear = d->pset;
+        x.nclear = (int)ps::Q;
+        x.eq_start = eq_blocks;
+        x.table = d->table;
+        x.S = p.S;
+        x.chunk_log = p.chunk_log;
+        x.ps_start = ps_blocks;
+        x.eq6 = d->EH + 64;
+        x.M = d->pset + ps::M;
+        x.part = d->pset + ps::PART;
+        x.part_out = (uint32_t *)(d->hbox_dev + mb::PART);
+        eq_blocks += 1 << p.nh;
+        ps_blocks += 1 << (p.lp - p.chunk_log);
+    }
+    hipStream_t st = bd->stream;
+    {  // the eq tables staged in the mailboxes
+        std::vector<uint4 *> dst(n);
+        std::vector<size_t> bytes(n, (4096 + 128) * 16);
+        for (int i = 0; i < n; i++) dst[i] = ctxs[i]->dev->EL;
+        const int rc = gather_launch(ctxs, n, *bd, mb::EL, dst.data(), bytes.data());
+        if (rc != LBN_OK) return rc;
+    }
+    HIPCHK(ctxs[0], hipMemcpyAsync(bd->parts, hv.data(), sizeof(PartView) * n, hipMemcpyHostToDevice, st));
+    k_meq_expand<<<eq_blocks, 256, 0, st>>>(bd->parts, n);  // also zeroes each proof's sets and partials
+    HIPCHK(ctxs[0], hipGetLastError());
+    k_mprefix_sums<<<ps_blocks, 256, 0, st>>>(bd->parts, n);
+    HIPCHK(ctxs[0], hipGetLastError());
+    k_mpart_out<<<n, 256, 0, st>>>(bd->parts);
     HIPCHK(ctxs[0], hipGetLastError());
     HIPCHK(ctxs[0], hipStreamSynchronize(st));
     return LBN_OK;

@@ -214,6 +214,12 @@ class Transcript {
         h_.update_with(b, n, cvs_of);
     }
     const Blake3 &hasher() const { return h_; }
+    // What absorb_bytes_with(b, n, cvs_of) will pass to cvs_of, and the key words and flags it uses.
+    Blake3::WithPlan plan_bytes_with(size_t n) const {
+        Blake3 h = h_;
+        h.update_u64(n);
+        return h.plan_with(n);
+    }
     void save(B3State &s) const {
         h_.save_state(s);
         s.counter = counter_;
@@ -557,7 +563,8 @@ struct ProveJob {
     B3State ts{};
     SwitchHandoff ho;
     lbn_profile pf{};
-    double t0 = 0, t3 = 0, t4 = 0, t5 = 0, t6 = 0;
+    double t0 = 0, t1 = 0, t2 = 0, t3 = 0, t4 = 0, t5 = 0, t6 = 0;
+    std::vector<B128> r_lo, r_hi;
     std::vector<F162> p0, row;
     FoldPlan fp;
     ChalArg ca{};
@@ -594,20 +601,62 @@ int prove_front_launch(ProveJob &j, size_t cap) {
     return dev_commit_launch(ctx);
 }
 
-// After the commitment's launch: the transcript, the partials and the switch's start.
-int prove_front_rest(ProveJob &j) {
-    lbn_ctx *ctx = j.ctx;
-    const Shape &s = ctx->shape;
-    const u128 q = ctx->q;
-    const int lp = s.log_packed;
-    // verifier/proof_verify/src/main.rs:166-182 (absorb k and 10, the shape of this instance)
+// verifier/proof_verify/src/main.rs:166-182 (absorb k and 10, the shape of this instance)
+void absorb_instance(ProveJob &j) {
     Transcript &t = j.t;
+    const u128 q = j.ctx->q;
     t.absorb_bytes(j.seed, 32);
-    t.absorb_u64((uint64_t)s.log_vars);
+    t.absorb_u64((uint64_t)j.ctx->shape.log_vars);
     t.absorb_u64(COLUMN_LOG);
     uint8_t qb[16];
     for (int i = 0; i < 16; i++) qb[i] = (uint8_t)(q >> (8 * i));
     t.absorb_bytes(qb, 16);
+}
+
+// After the commitment's absorb: the point's r_lo and r_hi.
+void sample_point(ProveJob &j) {
+    j.t1 = now_us();
+    j.r_lo = j.t.sample_b128("r-lo", 7);
+    j.r_hi = j.t.sample_b128("r-hi", j.ctx->shape.log_packed);
+    j.t2 = now_us();
+}
+
+// After the partials (in j.partials): the claim and the partials into the transcript, the switch's
+// batch and its start.
+int absorb_claim(ProveJob &j) {
+    lbn_ctx *ctx = j.ctx;
+    const int lp = ctx->shape.log_packed;
+    Transcript &t = j.t;
+    {
+        B128 eq_lo[128];
+        eq_expand_b128(j.r_lo.data(), 7, eq_lo);
+        B128 claim{{0, 0}};
+        for (int b = 0; b < 128; b++) claim = b128_add(claim, b128_mul(j.partials[b], eq_lo[b]));
+        t.absorb_b128(&claim, 1);
+        t.absorb_b128(j.partials.data(), 128);
+    }
+    std::vector<F162> rb = t.sample_f162("switch-batch", 7);
+    std::vector<F162> batch(128);
+    eq_expand_f162(rb.data(), 7, batch.data());
+    j.t3 = now_us();
+    j.pf.commit_us = j.t1 - j.t0;
+    j.pf.transcript_us = j.t2 - j.t1;
+    j.pf.partials_us = j.t3 - j.t2;
+
+    // The rounds run on the device with the transcript (main.rs:975-992: per round absorb [e0, einf],
+    // sample "switch-round", fold; then absorb [opened]); the host continues from its final state.
+    j.rounds.resize(2 * (size_t)lp);
+    j.coords.resize(lp);
+    TRY(dev_switch_prepare(ctx, (const uint64_t *)batch.data()));
+    t.save(j.ts);
+    return LBN_OK;
+}
+
+// After the commitment's launch: the transcript, the partials and the switch's start.
+int prove_front_rest(ProveJob &j) {
+    lbn_ctx *ctx = j.ctx;
+    Transcript &t = j.t;
+    absorb_instance(j);
     // The commitment's whole BLAKE3 chunks are hashed where the bytes are (device); the host reads
     // back their chaining values and the bytes around them. All bytes reach the proof at the end.
     int cvs_rc = LBN_OK;
@@ -618,36 +667,34 @@ int prove_front_rest(ProveJob &j) {
                                                     t.hasher().flags(), cvs, cdst, off, off + count * Blake3::CHUNK);
                         });
     TRY(cvs_rc);
-    const double t1 = now_us();
-    std::vector<B128> r_lo = t.sample_b128("r-lo", 7);
-    std::vector<B128> r_hi = t.sample_b128("r-hi", lp);
-    const double t2 = now_us();
-
+    sample_point(j);
     j.partials.resize(128);
-    TRY(lbn_partials(ctx, (const uint64_t *)r_hi.data(), (uint64_t *)j.partials.data()));
-    {
-        B128 eq_lo[128];
-        eq_expand_b128(r_lo.data(), 7, eq_lo);
-        B128 claim{{0, 0}};
-        for (int b = 0; b < 128; b++) claim = b128_add(claim, b128_mul(j.partials[b], eq_lo[b]));
-        t.absorb_b128(&claim, 1);
-        t.absorb_b128(j.partials.data(), 128);
-    }
-    std::vector<F162> rb = t.sample_f162("switch-batch", 7);
-    std::vector<F162> batch(128);
-    eq_expand_f162(rb.data(), 7, batch.data());
-    j.t3 = now_us();
-    j.pf.commit_us = t1 - j.t0;
-    j.pf.transcript_us = t2 - t1;
-    j.pf.partials_us = j.t3 - t2;
+    TRY(lbn_partials(ctx, (const uint64_t *)j.r_hi.data(), (uint64_t *)j.partials.data()));
+    return absorb_claim(j);
+}
 
-    // The rounds run on the device with the transcript (main.rs:975-992: per round absorb [e0, einf],
-    // sample "switch-round", fold; then absorb [opened]); the host continues from its final state.
-    j.rounds.resize(2 * (size_t)lp);
-    j.coords.resize(lp);
-    TRY(dev_switch_prepare(ctx, (const uint64_t *)batch.data()));
-    t.save(j.ts);
-    return LBN_OK;
+// The same in the batch protocol: the instance absorbed and the commitment's hashing staged ...
+int front_stage_hash(ProveJob &j) {
+    absorb_instance(j);
+    const Blake3::WithPlan w = j.t.plan_bytes_with(j.clen);
+    return dev_cvs_stage(j.ctx, w.trees, w.take, w.first, w.whole, j.t.hasher().key_words(), j.t.hasher().flags(),
+                         j.proof + j.off_commit, w.take, w.take + w.whole * Blake3::CHUNK);
+}
+// ... after dev_cvs_batch: the commitment absorbed with the staged hashing's results, the point
+// sampled and the partials prepared ...
+int front_absorb_commitment(ProveJob &j) {
+    uint8_t *cdst = j.proof + j.off_commit;
+    j.t.absorb_bytes_with(cdst, j.clen, [&](size_t, uint64_t, size_t, const Blake3::Subtrees &, uint32_t *cvs) {
+        dev_cvs_collect(j.ctx, cvs, cdst);  // the plan staged from this same state
+    });
+    sample_point(j);
+    return dev_partials_prepare(j.ctx, (const uint64_t *)j.r_hi.data(), false);
+}
+// ... after dev_partials_batch: the claim and the switch's start.
+int front_claim(ProveJob &j) {
+    j.partials.resize(128);
+    dev_partials_collect(j.ctx, (uint64_t *)j.partials.data());
+    return absorb_claim(j);
 }
 
 int prove_front(ProveJob &j, size_t cap) {
@@ -873,8 +920,12 @@ static void batch_group(lbn_batch_item *items, std::vector<std::unique_ptr<Prove
         return rc;
     });
     committed();
-    phase([&](int i) {  // transcript, partials, the switch's start state
-        const int rc = prove_front_rest(*jobs[i]);
+    phase([&](int i) { return front_stage_hash(*jobs[i]); });  // the instance, the commitment's hashing
+    batch(dev_cvs_batch);
+    phase([&](int i) { return front_absorb_commitment(*jobs[i]); });  // the commitment, the point
+    batch(dev_partials_batch);
+    phase([&](int i) {  // the claim, the switch's start state
+        const int rc = front_claim(*jobs[i]);
         if (rc == LBN_OK) dev_switch_stage(jobs[i]->ctx, jobs[i]->ts);
         return rc;
     });
@@ -883,15 +934,15 @@ static void batch_group(lbn_batch_item *items, std::vector<std::unique_ptr<Prove
         ProveJob &j = *jobs[i];
         dev_switch_collect(j.ctx, j.ts, (uint64_t *)j.rounds.data(), (uint64_t *)j.coords.data(), j.ho);
         int rc = prove_back_point(j);
-        if (rc == LBN_OK) rc = dev_row_prepare(j.ctx, (const uint64_t *)j.p0.data());
+        if (rc == LBN_OK) rc = dev_row_prepare(j.ctx, (const uint64_t *)j.p0.data(), false);
         return rc;
     });
     batch(dev_row_batch);
     phase([&](int i) {  // the row into the transcript, the challenges' transcript state
         ProveJob &j = *jobs[i];
         dev_row_collect(j.ctx, (uint64_t *)j.row.data());
-        int rc = prove_back_row(j);
-        if (rc == LBN_OK) rc = dev_challenges_stage(j.ctx, j.ca);
+        const int rc = prove_back_row(j);
+        if (rc == LBN_OK) dev_challenges_stage(j.ctx, j.ca);
         return rc;
     });
     batch(dev_challenges_batch);
@@ -900,7 +951,7 @@ static void batch_group(lbn_batch_item *items, std::vector<std::unique_ptr<Prove
         dev_challenges_collect(j.ctx, j.ch.data());
         prove_tail_start(j);
         return dev_fold_prepare(j.ctx, j.ch.data(), COLUMNS, fold_out(j), (size_t)N * j.ctx->shape.ring_elements,
-                                j.fp);
+                                j.fp, false);
     });
     {
         std::vector<FoldPlan> fps;

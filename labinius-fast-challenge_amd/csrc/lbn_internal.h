@@ -180,6 +180,20 @@ int dev_switch_rounds(lbn_ctx *ctx, B3State &ts, uint64_t *rounds, uint64_t *coo
 // results from its mailbox. Instance i's results are those of the one-proof entry point.
 constexpr int SWB_MAX = 64;
 int dev_sync(lbn_ctx *ctx);
+// The commitment's hashing: dev_cvs_stage takes dev_commit_cvs's arguments (the commitment's
+// copy to the host is queued there), dev_cvs_batch hashes every staged proof's chunks and subtrees,
+// dev_cvs_collect gives what dev_commit_cvs gives (the subtree roots and dst's head and tail bytes).
+int dev_cvs_stage(lbn_ctx *ctx, const Blake3::Subtrees &trees, size_t off, uint64_t first, size_t count,
+                  const uint32_t key[8], uint32_t flags, uint8_t *dst, size_t head, size_t tail);
+int dev_cvs_batch(lbn_ctx *const *ctxs, int n);
+void dev_cvs_collect(lbn_ctx *ctx, uint32_t *cvs, uint8_t *dst);
+// The partials: dev_partials_prepare computes the eq tables of r_hi (log_packed values) into the
+// mailbox (and queues their upload with `upload`; the batch copies them itself),
+// dev_partials_batch runs every proof's eq expansion and prefix sums, and dev_partials_collect
+// gives the 128 partials (2 u64 each) that lbn_partials gives.
+int dev_partials_prepare(lbn_ctx *ctx, const uint64_t *r_hi, bool upload);
+int dev_partials_batch(lbn_ctx *const *ctxs, int n);
+void dev_partials_collect(lbn_ctx *ctx, uint64_t *partials);
 // The switch: dev_switch_stage (after dev_switch_prepare) writes the start state from ts to the
 // mailbox; dev_switch_batch runs the device rounds; dev_switch_collect returns what
 // dev_switch_rounds returns.
@@ -194,20 +208,21 @@ void lmap_next(std::vector<F162> &L, B128 c, const F162 &rho);
 // The 1024 folding challenges sampled on the device from the transcript state in arg (the
 // counter of arg->ts is the first challenge's); out receives them in column order.
 int dev_challenges(lbn_ctx *ctx, const ChalArg &arg, lbn_challenge *out);
-// The same in the batch protocol: the transcript state staged, one launch, the 1024 challenges
-// collected in column order.
-int dev_challenges_stage(lbn_ctx *ctx, const ChalArg &arg);
+// The same in the batch protocol: the sampler's input staged in the mailbox, one launch (which
+// copies every staged input first), the 1024 challenges collected in column order.
+void dev_challenges_stage(lbn_ctx *ctx, const ChalArg &arg);
 int dev_challenges_batch(lbn_ctx *const *ctxs, int n);
 void dev_challenges_collect(lbn_ctx *ctx, lbn_challenge *out);
 // The row evaluation in the batch protocol: dev_row_prepare computes the eq tables of p0 (row_log
-// coordinates) and queues their upload on the context's stream; dev_row_batch evaluates every row
-// in one launch per kernel; dev_row_collect gives the 1024 F162 values (3 u64 each) that
-// lbn_row_eval gives.
-int dev_row_prepare(lbn_ctx *ctx, const uint64_t *p0);
+// coordinates) into the mailbox (and queues their upload with `upload`; the batch kernels read
+// them there); dev_row_batch evaluates every row in one launch per kernel; dev_row_collect gives
+// the 1024 F162 values (3 u64 each) that lbn_row_eval gives.
+int dev_row_prepare(lbn_ctx *ctx, const uint64_t *p0, bool upload);
 int dev_row_batch(lbn_ctx *const *ctxs, int n);
 void dev_row_collect(lbn_ctx *ctx, uint64_t *row);
 // The fold in the batch protocol: dev_fold_prepare checks the challenges, builds the column lists
-// per position and queues their upload on the context's stream; dev_fold_batch folds every
+// per position in the mailbox (and queues their upload with `upload`; the batch copies them
+// itself); dev_fold_batch folds every
 // instance in one launch; dev_fold_collect checks the context's range flag and puts its fold in
 // `out` (n_out int32) if the kernel did not write it there directly. lbn_fold runs the same parts
 // on the context's own stream.
@@ -217,9 +232,10 @@ struct FoldPlan {
     int32_t *out = nullptr;
     size_t n_out = 0;
     int32_t *dst = nullptr;  // where the kernel writes: `out` itself when pinned, else the context's staging
+    size_t upload_bytes = 0; // the lists' bytes staged in the mailbox
 };
 int dev_fold_prepare(lbn_ctx *ctx, const lbn_challenge *ch, int n_challenges, int32_t *out, size_t n_out,
-                     FoldPlan &plan);
+                     FoldPlan &plan, bool upload);
 int dev_fold_batch(lbn_ctx *const *ctxs, int n, const FoldPlan *plans);
 int dev_fold_collect(lbn_ctx *ctx, const FoldPlan &plan);
 // The streams the context's work goes to (hipStream_t, as void *): a caller driving many contexts

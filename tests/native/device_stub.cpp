@@ -22,6 +22,14 @@ struct lbn::DeviceState {
     lbn::ChalArg arg{};
     std::vector<lbn_challenge> ch;
     const lbn_challenge *fold_ch = nullptr;
+    struct {
+        size_t off = 0, count = 0, head = 0, tail = 0;
+        uint64_t first = 0;
+        lbn::Blake3::Subtrees trees;
+        uint32_t key[8] = {}, flags = 0;
+        uint8_t *dst = nullptr;
+    } cv;
+    std::vector<uint64_t> r_hi, partials;
 };
 
 namespace {
@@ -97,6 +105,35 @@ int dev_switch_rounds(lbn_ctx *ctx, B3State &ts, uint64_t *rounds, uint64_t *coo
     return LBN_OK;
 }
 int dev_sync(lbn_ctx *) { return LBN_OK; }
+int dev_cvs_stage(lbn_ctx *ctx, const Blake3::Subtrees &trees, size_t off, uint64_t first, size_t count,
+                  const uint32_t key[8], uint32_t flags, uint8_t *dst, size_t head, size_t tail) {
+    auto &c = ctx->dev->cv;
+    c.off = off, c.first = first, c.count = count, c.head = head, c.tail = tail, c.trees = trees, c.flags = flags;
+    c.dst = dst;
+    memcpy(c.key, key, 32);
+    return LBN_OK;
+}
+int dev_cvs_batch(lbn_ctx *const *, int) { return LBN_OK; }
+void dev_cvs_collect(lbn_ctx *ctx, uint32_t *cvs, uint8_t *dst) {
+    auto &c = ctx->dev->cv;
+    dev_commit_cvs(ctx, c.off, c.first, c.count, c.trees, c.key, c.flags, cvs, dst, c.head, c.tail);
+}
+int dev_partials_prepare(lbn_ctx *ctx, const uint64_t *r_hi, bool) {
+    ctx->dev->r_hi.assign(r_hi, r_hi + 2 * ctx->shape.log_packed);
+    return LBN_OK;
+}
+int dev_partials_batch(lbn_ctx *const *ctxs, int n) {
+    for (int i = 0; i < n; i++) {
+        DeviceState *d = ctxs[i]->dev;
+        d->partials.resize(256);
+        const int rc = lbn_partials(ctxs[i], d->r_hi.data(), d->partials.data());
+        if (rc != LBN_OK) return rc;
+    }
+    return LBN_OK;
+}
+void dev_partials_collect(lbn_ctx *ctx, uint64_t *partials) {
+    memcpy(partials, ctx->dev->partials.data(), 256 * 8);
+}
 void dev_switch_stage(lbn_ctx *ctx, const B3State &ts) { ctx->dev->ts = ts; }
 int dev_switch_batch(lbn_ctx *const *ctxs, int n) {
     for (int i = 0; i < n; i++) {
@@ -124,10 +161,7 @@ int dev_challenges(lbn_ctx *, const ChalArg &arg, lbn_challenge *out) {
     }
     return LBN_OK;
 }
-int dev_challenges_stage(lbn_ctx *ctx, const ChalArg &arg) {
-    ctx->dev->arg = arg;
-    return LBN_OK;
-}
+void dev_challenges_stage(lbn_ctx *ctx, const ChalArg &arg) { ctx->dev->arg = arg; }
 int dev_challenges_batch(lbn_ctx *const *ctxs, int n) {
     for (int i = 0; i < n; i++) {
         DeviceState *d = ctxs[i]->dev;
@@ -140,7 +174,7 @@ void dev_challenges_collect(lbn_ctx *ctx, lbn_challenge *out) {
     memcpy(out, ctx->dev->ch.data(), COLUMNS * sizeof(lbn_challenge));
 }
 void dev_use_streams(lbn_ctx *, void *, void *) {}
-int dev_row_prepare(lbn_ctx *ctx, const uint64_t *p0) {
+int dev_row_prepare(lbn_ctx *ctx, const uint64_t *p0, bool) {
     ctx->dev->p0.assign(p0, p0 + 3 * ctx->shape.row_log);
     return LBN_OK;
 }
@@ -155,7 +189,7 @@ int dev_row_batch(lbn_ctx *const *ctxs, int n) {
 }
 void dev_row_collect(lbn_ctx *ctx, uint64_t *row) { memcpy(row, ctx->dev->row.data(), 24 * COLUMNS); }
 int dev_fold_prepare(lbn_ctx *ctx, const lbn_challenge *ch, int n_challenges, int32_t *out, size_t n_out,
-                     FoldPlan &plan) {
+                     FoldPlan &plan, bool) {
     if (n_challenges != COLUMNS || n_out != (size_t)N * ctx->shape.ring_elements) return LBN_E_ARG;
     ctx->dev->fold_ch = ch;
     plan.out = out;

@@ -1,8 +1,9 @@
 """The batch prover (lbn_prove_batch: one device switch sequence for all instances) on an MI355X: the
 benchmark's 64 instances of k in [20, 28] give, proof for proof, the bytes each context gives alone
 (lbn_prove), the library's checker accepts every one, the oracle's prover agrees on one instance per
-size; a batch with one instance, a batch of one size, the loaded table instead of a generated one and
-item errors (a proof buffer too small, a context without a modulus) behave like lbn_prove.
+size; a batch with one instance, a batch of one size, the loaded table instead of a generated one, a
+batch mixing matrix-core and CRT moduli, and item errors (a proof buffer too small, a context
+without a modulus) behave like lbn_prove.
 """
 import sys
 
@@ -115,3 +116,30 @@ def test_batch_edge_cases(m):
     finally:
         a.close()
         b.close()
+
+
+def test_batch_mixed_moduli(m):
+    # 18 proofs (two groups): the default Q (matrix-core commitment) and a 96-bit Q (CRT commitment,
+    # 12-byte coefficients) in one batch; each equals its solo proof, one per modulus the oracle's
+    big = (1 << 95) + 12345
+    jobs = [(20 + i % 2, m.DEFAULT_Q if i % 3 else big, bench_seed(300 + i)) for i in range(18)]
+    say("batch prover: 18 instances, k 20..21, moduli", sorted({q for _, q, _ in jobs}))
+    provers = []
+    try:
+        for k, q, _ in jobs:
+            p = m.Prover(k)
+            p.set_modulus(q)
+            provers.append(p)
+        caps = [m.proof_len(k, q) for k, q, _ in jobs]
+        outs = [np.zeros(c, dtype=np.uint8) for c in caps]
+        res = m.prove_batch(provers, [s for _, _, s in jobs], outs, threads=8)
+        assert all(st == 0 for _, st in res), res
+        got = [o[:n].tobytes() for o, (n, _) in zip(outs, res)]
+        for i, ((k, q, seed), p) in enumerate(zip(jobs, provers)):
+            assert got[i] == solo(p, seed), (i, k, q)
+    finally:
+        for p in provers:
+            p.close()
+    for q in (m.DEFAULT_Q, big):
+        i = next(i for i, (k, qq, _) in enumerate(jobs) if qq == q and k == 20)
+        assert got[i] == O.prove(jobs[i][2], 20, q), q

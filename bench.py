@@ -225,11 +225,12 @@ class ProofArena:
 
 
 def warm(sch, ks, workers):
-    """Context set-up outside the clock: `workers` contexts of each size, then a round of proofs
-    (first launches of every kernel)."""
+    """Context set-up outside the clock: `workers` contexts of each size (an int, or a count per size),
+    then a round of proofs (first launches of every kernel)."""
+    count = workers if isinstance(workers, dict) else {kk: workers for kk in ks}
     for kk in ks:
-        sch.prewarm(kk, workers)
-    hs = [sch.submit(kk, bench_seed(999_000 + 100 * kk + j)) for kk in ks for j in range(workers)]
+        sch.prewarm(kk, count[kk])
+    hs = [sch.submit(kk, bench_seed(999_000 + 100 * kk + j)) for kk in ks for j in range(count[kk])]
     sch.drain()
     for h in hs:
         sch.result(h)
@@ -388,6 +389,7 @@ def serving_requests(requests, rate, rng_seed=20261203):
 
 
 SERVING_KS = list(range(20, 32))
+BATCH_WORKERS, BATCH_MAX, BATCH_THREADS = 2, 32, 8  # continuous batching in the serving loop
 
 
 def serving_arena(m, reqs):
@@ -396,14 +398,17 @@ def serving_arena(m, reqs):
     return ProofArena(m, [m.proof_len(kk, m.DEFAULT_Q) for _, kk, _ in reqs], pinned=False)
 
 
-def serve(m, reqs, w, budget, arena, check=None):
-    """The arrival sequence `reqs` served by a scheduler with `w` proofs in flight (contexts of every
-    size warmed first), request i's proof into arena.views[i]; latency = completion - arrival. With
-    `check`, every proof goes through the checker after the clock. Returns (figures, the first 32
-    proofs)."""
+def serve(m, reqs, w, budget, arena, check=None, batch=1):
+    """The arrival sequence `reqs` served by a scheduler with `w` workers (contexts of every size
+    warmed first; with batch > 1, continuous batching of up to `batch` admitted jobs per worker),
+    request i's proof into arena.views[i]; latency = completion - arrival. With `check`, every proof
+    goes through the checker after the clock. Returns (figures, the first 32 proofs)."""
     sch = m.Scheduler(m.DEFAULT_Q, hbm_budget=budget, workers=w)
     try:
-        warm(sch, SERVING_KS, w)
+        # with batching, as many contexts of the small sizes as a batch may hold (the large ones are rare)
+        warm(sch, SERVING_KS, {kk: w * batch if kk <= 23 else w for kk in SERVING_KS})
+        if batch > 1:
+            sch.set_batch(batch, BATCH_THREADS)
         hs = []
         t0 = time.perf_counter()
         for (at, kk, seed), out in zip(reqs, arena.views):
@@ -446,15 +451,18 @@ def serving(m, requests, rate, workers, budget):
     try:
         mix, proofs_mix = serve(m, reqs, workers, budget, arena, check)
         check.close()
+        cb, proofs_cb = serve(m, reqs, BATCH_WORKERS, budget, arena, check, batch=BATCH_MAX)
+        check.close()
         seq, proofs_seq = serve(m, reqs, 1, budget, arena)
     finally:
         check.close()
         arena.close()
     return {"metric": "serving p50/p99 ms and proofs/s on 1 MI355X", "requests": requests, "rate_per_s": rate,
             "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")),
-            "workers": workers, "hbm_budget_gib": budget / 2**30, "batched": mix, "sequential": seq,
-            "identical_sample": proofs_mix == proofs_seq,
-            "proof_buffers": "one pageable arena written before the clock, the same for both ways"}
+            "workers": workers, "hbm_budget_gib": budget / 2**30, "batched": mix,
+            "continuous_batching": dict(cb, workers=BATCH_WORKERS, max_batch=BATCH_MAX, threads=BATCH_THREADS),
+            "sequential": seq, "identical_sample": proofs_mix == proofs_seq == proofs_cb,
+            "proof_buffers": "one pageable arena written before the clock, the same for every way"}
 
 
 def serving_sweep(m, requests, rates, workers, budget):
@@ -467,14 +475,16 @@ def serving_sweep(m, requests, rates, workers, budget):
         arena = serving_arena(m, reqs)
         try:
             mix, proofs_mix = serve(m, reqs, workers, budget, arena)
+            cb, proofs_cb = serve(m, reqs, BATCH_WORKERS, budget, arena, batch=BATCH_MAX)
             seq, proofs_seq = serve(m, reqs, 1, budget, arena)
         finally:
             arena.close()
-        rows.append({"rate_per_s": rate, "batched": mix, "sequential": seq,
-                     "identical_sample": proofs_mix == proofs_seq})
+        rows.append({"rate_per_s": rate, "batched": mix, "continuous_batching": cb, "sequential": seq,
+                     "identical_sample": proofs_mix == proofs_seq == proofs_cb})
         print(json.dumps({"sweep_row": rows[-1]}), file=sys.stderr, flush=True)
     return {"metric": "serving p50/p99 ms and proofs/s against the arrival rate on 1 MI355X",
             "requests_per_rate": requests, "workers": workers,
+            "continuous_batching": {"workers": BATCH_WORKERS, "max_batch": BATCH_MAX, "threads": BATCH_THREADS},
             "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")), "hbm_budget_gib": budget / 2**30,
             "rows": rows}
 

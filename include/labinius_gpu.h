@@ -268,6 +268,11 @@ void lbn_sched_destroy(lbn_sched *s);
  * context created for a job costs milliseconds and synchronises the device). LBN_E_RANGE, with
  * none created, when the budget cannot hold them. No counterpart in the reference. */
 int lbn_sched_prewarm(lbn_sched *s, int log_vars, int count);
+/* Continuous batching: from now on a worker that wakes takes every queued job it can admit, up to
+ * max_batch (1..64), and proves them together with lbn_prove_batch on `threads` host threads (a
+ * single job runs as lbn_prove). max_batch 1 restores one proof per worker. Each proof still
+ * equals the one lbn_prove gives alone. No counterpart in the reference. */
+int lbn_sched_set_batch(lbn_sched *s, int max_batch, int threads);
 
 /* ----------------------------------------------------------------------------------------------
  * Standalone commitment (the commit kernels on their own, for keys of one or several ring rows).

@@ -129,6 +129,7 @@ def lib():
         L.lbn_sched_destroy.argtypes = [P]
         L.lbn_sched_destroy.restype = None
         L.lbn_sched_prewarm.argtypes = [P, ctypes.c_int, ctypes.c_int]
+        L.lbn_sched_set_batch.argtypes = [P, ctypes.c_int, ctypes.c_int]
         L.lbn_host_alloc.argtypes = [ctypes.c_size_t]
         L.lbn_host_alloc.restype = ctypes.c_void_p
         L.lbn_host_free.argtypes = [P]
@@ -427,6 +428,13 @@ class Scheduler:
         if rc != 0:
             raise LbnError(rc, "proof job failed")
         return keep[-1][: job.len].tobytes(), job.queue_us, job.run_us
+
+    def set_batch(self, max_batch: int, threads: int = 8):
+        """Continuous batching (lbn_sched_set_batch): a waking worker proves every job it can admit, up
+        to max_batch, together (lbn_prove_batch on `threads` host threads); 1 turns it off."""
+        rc = lib().lbn_sched_set_batch(self.h, max_batch, threads)
+        if rc != 0:
+            raise LbnError(rc, "lbn_sched_set_batch failed")
 
     def prewarm(self, log_vars: int, count: int):
         """Idle contexts of this size until the scheduler holds `count` of them (created now)."""

@@ -10,6 +10,11 @@
 // arrival order with bounded overtaking: a later job may start on a free context or free memory
 // while the first waits, but the first cannot be overtaken more than MAX_BYPASS times.
 //
+// With continuous batching (lbn_sched_set_batch) a worker that wakes takes every job it can admit
+// at that moment, up to the limit, and proves them together with lbn_prove_batch (on the contexts'
+// own streams): under load the queue drains in batches, whose switch, rows, challenges and folds
+// run as one launch each.
+//
 // Each worker owns a pair of HIP streams and runs its proofs on them, whatever context they use:
 // HIP maps streams onto its hardware queues in creation order, so the workers' streams (created
 // together) sit on distinct queues, while the contexts' own streams (created one per context, any
@@ -19,6 +24,7 @@
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
+#include <cstring>
 #include <deque>
 #include <mutex>
 #include <thread>
@@ -82,6 +88,7 @@ struct lbn_sched {
     size_t pending = 0;
     uint64_t tick = 0;
     bool stop = false;
+    int max_batch = 1, batch_threads = 8;  // continuous batching (lbn_sched_set_batch)
     lbn_sched_stats stats{};
 
     // Under m: pick a job and a slot for it, or return false. Reserves memory for a new slot and
@@ -135,23 +142,83 @@ struct lbn_sched {
         return false;
     }
 
+    // Outside m: the slot's context created with the modulus if it is new; LBN_OK or the error.
+    int ready_slot(Slot *slot, int log_vars) {
+        if (slot->ready) return LBN_OK;
+        lbn_ctx *ctx = nullptr;
+        int rc = lbn_create(device, log_vars, &ctx);
+        if (rc == LBN_OK) rc = lbn_set_modulus(ctx, q_lo, q_hi);
+        if (rc == LBN_OK) {
+            slot->ctx = ctx;
+            slot->ready = true;
+            std::lock_guard<std::mutex> lk(m);
+            stats.contexts_created++;
+        } else if (ctx) {
+            lbn_destroy(ctx);
+        }
+        return rc;
+    }
+
+    // Under m: a finished job's slot back (or its reservation, if the context was never made).
+    void release(Entry &e, Slot *slot, int rc, double start, double end) {
+        lbn_job *job = e.job;
+        job->queue_us = start - e.submitted;
+        job->run_us = end - start;
+        job->status = rc;
+        if (slot->ready) {
+            slot->busy = false;
+        } else {
+            used -= slot->bytes;
+            slots.erase(std::find(slots.begin(), slots.end(), slot));
+            delete slot;
+        }
+        stats.completed++;
+        if (rc != LBN_OK) stats.failed++;
+        pending--;
+    }
+
+    // Several admitted jobs as one lbn_prove_batch on their contexts (own streams).
+    void run_batch(std::vector<Entry> &es, std::vector<Slot *> &ss, int threads) {
+        const double start = now_us();
+        const int n = (int)es.size();
+        std::vector<int> rc(n, LBN_OK);
+        std::vector<lbn_batch_item> items;
+        std::vector<int> idx;
+        for (int i = 0; i < n; i++) {
+            lbn_job *job = es[i].job;
+            rc[i] = ready_slot(ss[i], job->log_vars);
+            if (rc[i] == LBN_OK && job->table)
+                rc[i] = lbn_load_table(ss[i]->ctx, job->table, ((size_t)1 << job->log_vars) / 8);
+            if (rc[i] != LBN_OK) continue;
+            lbn_batch_item it{};
+            it.ctx = ss[i]->ctx;
+            memcpy(it.seed, job->seed, 32);
+            it.gen_table = job->table ? 0 : 1;
+            it.proof = job->proof;
+            it.cap = job->cap;
+            items.push_back(it);
+            idx.push_back(i);
+        }
+        if (!items.empty()) {
+            lbn_prove_batch(items.data(), (int)items.size(), threads);
+            for (size_t k = 0; k < items.size(); k++) {
+                lbn_job *job = es[idx[k]].job;
+                rc[idx[k]] = items[k].status;
+                job->len = items[k].len;
+                job->profile = items[k].profile;
+            }
+        }
+        const double end = now_us();
+        std::lock_guard<std::mutex> lk(m);
+        for (int i = 0; i < n; i++) release(es[i], ss[i], rc[i], start, end);
+        done.notify_all();
+        wake.notify_all();
+    }
+
     void run_job(Entry &e, Slot *slot, int w) {
         lbn_job *job = e.job;
         const double start = now_us();
-        int rc = LBN_OK;
-        if (!slot->ready) {
-            lbn_ctx *ctx = nullptr;
-            rc = lbn_create(device, job->log_vars, &ctx);
-            if (rc == LBN_OK) rc = lbn_set_modulus(ctx, q_lo, q_hi);
-            if (rc == LBN_OK) {
-                slot->ctx = ctx;
-                slot->ready = true;
-                std::lock_guard<std::mutex> lk(m);
-                stats.contexts_created++;
-            } else if (ctx) {
-                lbn_destroy(ctx);
-            }
-        }
+        int rc = ready_slot(slot, job->log_vars);
         if (rc == LBN_OK) {
             std::lock_guard<std::mutex> lk(m);
             if (!make_streams()) rc = LBN_E_DEVICE;
@@ -166,19 +233,7 @@ struct lbn_sched {
         }
         const double end = now_us();
         std::lock_guard<std::mutex> lk(m);
-        job->queue_us = start - e.submitted;
-        job->run_us = end - start;
-        job->status = rc;
-        if (slot->ready) {
-            slot->busy = false;
-        } else {  // creation failed: give the reservation back
-            used -= slot->bytes;
-            slots.erase(std::find(slots.begin(), slots.end(), slot));
-            delete slot;
-        }
-        stats.completed++;
-        if (rc != LBN_OK) stats.failed++;
-        pending--;
+        release(e, slot, rc, start, end);
         done.notify_all();
         wake.notify_all();
     }
@@ -188,13 +243,28 @@ struct lbn_sched {
             Entry e{};
             Slot *slot = nullptr;
             std::vector<lbn_ctx *> evict;
+            std::vector<Entry> es;
+            std::vector<Slot *> ss;
+            int threads = 1;
             {
                 std::unique_lock<std::mutex> lk(m);
                 wake.wait(lk, [&] { return stop || (!queue.empty() && admit(e, slot, evict)); });
                 if (!slot) return;  // stop with nothing admitted
+                // continuous batching: every other job admissible now joins this one
+                if (max_batch > 1) {
+                    es.push_back(e), ss.push_back(slot);
+                    Entry e2{};
+                    Slot *s2 = nullptr;
+                    while ((int)es.size() < max_batch && !queue.empty() && admit(e2, s2, evict))
+                        es.push_back(e2), ss.push_back(s2);
+                    threads = batch_threads;
+                }
             }
             for (lbn_ctx *c : evict) lbn_destroy(c);
-            run_job(e, slot, w);
+            if (es.size() > 1)
+                run_batch(es, ss, threads);
+            else
+                run_job(e, slot, w);
         }
     }
 };
@@ -269,6 +339,14 @@ extern "C" void lbn_sched_destroy(lbn_sched *s) {
     }
     for (hipStream_t x : s->streams) hipStreamDestroy(x);
     delete s;
+}
+
+extern "C" int lbn_sched_set_batch(lbn_sched *s, int max_batch, int threads) {
+    if (!s || max_batch < 1 || max_batch > lbn::SWB_MAX || threads < 1 || threads > 64) return LBN_E_ARG;
+    std::lock_guard<std::mutex> lk(s->m);
+    s->max_batch = max_batch;
+    s->batch_threads = threads;
+    return LBN_OK;
 }
 
 extern "C" int lbn_sched_prewarm(lbn_sched *s, int log_vars, int count) {

@@ -50,10 +50,12 @@ static void pool_stress() {
     CHECK(bad == 0);
 }
 
-static void scheduler_stress() {
+// max_batch > 1: continuous batching (several admitted jobs proved by lbn_prove_batch).
+static void scheduler_stress(int max_batch) {
     lbn_sched *s = nullptr;
-    CHECK(lbn_sched_create(0, 35039150080ull, 0, (size_t)64 << 30, 12, &s) == LBN_OK);
+    CHECK(lbn_sched_create(0, 35039150080ull, 0, (size_t)64 << 30, max_batch > 1 ? 3 : 12, &s) == LBN_OK);
     if (!s) return;
+    if (max_batch > 1) CHECK(lbn_sched_set_batch(s, max_batch, 4) == LBN_OK);
     std::vector<lbn_job> jobs(12 + 15 * scale);
     std::vector<std::vector<uint8_t>> out(jobs.size());
     for (size_t i = 0; i < jobs.size(); i++) {
@@ -212,7 +214,8 @@ int main(int argc, char **argv) {
     fprintf(stderr, "transcript done\n");
     checker_decode_fuzz();
     fprintf(stderr, "checker decode done\n");
-    scheduler_stress();
+    scheduler_stress(1);
+    scheduler_stress(8);
     fprintf(stderr, "scheduler done\n");
     batch_stress();
     fprintf(stderr, "batch done, failures %d\n", failures);

@@ -145,3 +145,24 @@ def test_host_table_and_errors(m, alone):
             sch.submit(18, seed)  # no such shape
     finally:
         sch.close()
+
+
+def test_continuous_batching_equals_alone(m, alone):
+    say("scheduler: continuous batching, 2 workers, batches of up to 16, 30 jobs k in 20..23")
+    jobs = [(20 + (i * 7) % 4, 100 + i) for i in range(30)]
+    sch = m.Scheduler(m.DEFAULT_Q, hbm_budget=16 << 30, workers=2)
+    try:
+        sch.set_batch(16, 8)
+        hs = [(k, seed_of(i), sch.submit(k, seed_of(i))) for k, i in jobs]
+        sch.drain()
+        for k, seed, h in hs:
+            assert sch.result(h)[0] == alone(k, seed), (k, seed[:2].hex())
+        st = sch.stats()
+        assert st["completed"] == len(jobs) and st["failed"] == 0
+        with pytest.raises(m.LbnError):
+            sch.set_batch(65, 8)
+        sch.set_batch(1, 1)  # back to one proof per worker
+        h = sch.submit(20, seed_of(5))
+        assert sch.result(h)[0] == alone(20, seed_of(5))
+    finally:
+        sch.close()

@@ -2,7 +2,8 @@
 200 GB HBM budget) workloads the benchmark measures, run through the scheduler ABI (lbn_sched_*) on an
 MI355X: every proof equals the same proof made alone on one context, both the library's checker
 (lbn_verify) and the CPU oracle's accept every proof, and the oracle's own prover gives the same
-bytes for one instance of each size.
+bytes for one instance of each size. The serving requests run twice: 8 workers at 400 requests/s,
+and continuous batching (lbn_sched_set_batch) at 3,000 requests/s.
 """
 import sys
 import time
@@ -87,8 +88,12 @@ def test_batched_64_instances(m, alone):
     assert seen == set(ks)
 
 
-def test_serving_poisson_all_sizes(m, alone):
-    reqs = serving_requests(520, 400.0)
+@pytest.mark.parametrize("mode", ["workers", "batching"])
+def test_serving_poisson_all_sizes(m, alone, mode):
+    # workers: 8 proofs in flight at 400 requests/s; batching: continuous batching (2 workers, up to
+    # 32 jobs per batch) at 3,000 requests/s, where batches form; the same requests either way
+    rate = 400.0 if mode == "workers" else 3000.0
+    reqs = serving_requests(520, rate)
     have = {k for _, k, _ in reqs}
     # one request of every size the distribution can draw, forced in at spread-out arrival times
     t_end = reqs[-1][0]
@@ -96,9 +101,11 @@ def test_serving_poisson_all_sizes(m, alone):
     reqs = sorted(reqs + extra)
     assert {k for _, k, _ in reqs} == set(range(20, 32))
     say("serving:", len(reqs), "requests over %.2f s, sizes drawn" % t_end, sorted(have))
-    sch = m.Scheduler(m.DEFAULT_Q, hbm_budget=200 << 30, workers=8)
+    sch = m.Scheduler(m.DEFAULT_Q, hbm_budget=200 << 30, workers=8 if mode == "workers" else 2)
     try:
         sch.prewarm(20, 4)
+        if mode == "batching":
+            sch.set_batch(32, 8)
         hs = []
         t0 = time.perf_counter()
         for at, k, seed in reqs:
@@ -119,6 +126,7 @@ def test_serving_poisson_all_sizes(m, alone):
         assert proof == alone(k, seed), (i, k)
     say("serving: library checker on every proof")
     product_checker_accepts(m, proofs)
-    say("serving: oracle checker on every proof")
-    for i, (k, seed, proof) in enumerate(proofs):
-        assert O.verify(seed, k, proof) is None, (i, k)
+    if mode == "workers":  # (the batching run's proofs equal these, request for request)
+        say("serving: oracle checker on every proof")
+        for i, (k, seed, proof) in enumerate(proofs):
+            assert O.verify(seed, k, proof) is None, (i, k)

@@ -5414,6 +5414,14 @@ int lbn::dev_fold_batch(lbn_ctx *const *ctxs, int n, const FoldPlan *plans) {
     return LBN_OK;
 }
 
+void lbn::dev_batch_quiesce(int device) {
+    std::lock_guard<std::mutex> lk(swb_mu);
+    auto it = swb_dev.find(device);
+    if (it == swb_dev.end() || !it->second.stream) return;
+    (void)hipSetDevice(device);
+    (void)hipStreamSynchronize(it->second.stream);
+}
+
 // The batched steps' common start: the contexts checked, the device current and its batch buffers.
 static int batch_begin(lbn_ctx *const *ctxs, int n, SwBatchDev *&bd) {
     {

@@ -898,8 +898,10 @@ static void batch_group(lbn_batch_item *items, std::vector<std::unique_ptr<Prove
             if (items[i].status == 1) li.push_back(i), lc.push_back(items[i].ctx);
         if (li.empty()) return;
         const int rc = call(lc.data(), (int)lc.size());
-        if (rc != LBN_OK)
+        if (rc != LBN_OK) {
+            dev_batch_quiesce(lc[0]->device);  // nothing it launched still writes the contexts' buffers
             for (int i : li) items[i].status = rc;
+        }
     };
     // each phase on the threads: proof i's step, then its stream idle
     auto phase = [&](const std::function<int(int)> &step) {

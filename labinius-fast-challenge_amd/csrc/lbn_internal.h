@@ -180,6 +180,8 @@ int dev_switch_rounds(lbn_ctx *ctx, B3State &ts, uint64_t *rounds, uint64_t *coo
 // results from its mailbox. Instance i's results are those of the one-proof entry point.
 constexpr int SWB_MAX = 64;
 int dev_sync(lbn_ctx *ctx);
+// After a batched call failed part-way: waits for whatever it launched on the device's batch stream.
+void dev_batch_quiesce(int device);
 // The commitment's hashing: dev_cvs_stage takes dev_commit_cvs's arguments (the commitment's
 // copy to the host is queued there), dev_cvs_batch hashes every staged proof's chunks and subtrees,
 // dev_cvs_collect gives what dev_commit_cvs gives (the subtree roots and dst's head and tail bytes).

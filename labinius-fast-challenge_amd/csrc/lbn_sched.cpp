@@ -196,14 +196,15 @@ struct lbn_sched {
             it.gen_table = job->table ? 0 : 1;
             it.proof = job->proof;
             it.cap = job->cap;
+            it.status = 1;  // a call rejected as a whole leaves it so
             items.push_back(it);
             idx.push_back(i);
         }
         if (!items.empty()) {
-            lbn_prove_batch(items.data(), (int)items.size(), threads);
+            const int whole = lbn_prove_batch(items.data(), (int)items.size(), threads);
             for (size_t k = 0; k < items.size(); k++) {
                 lbn_job *job = es[idx[k]].job;
-                rc[idx[k]] = items[k].status;
+                rc[idx[k]] = items[k].status == 1 ? (whole != LBN_OK ? whole : LBN_E_STATE) : items[k].status;
                 job->len = items[k].len;
                 job->profile = items[k].profile;
             }

@@ -105,6 +105,7 @@ int dev_switch_rounds(lbn_ctx *ctx, B3State &ts, uint64_t *rounds, uint64_t *coo
     return LBN_OK;
 }
 int dev_sync(lbn_ctx *) { return LBN_OK; }
+void dev_batch_quiesce(int) {}
 int dev_cvs_stage(lbn_ctx *ctx, const Blake3::Subtrees &trees, size_t off, uint64_t first, size_t count,
                   const uint32_t key[8], uint32_t flags, uint8_t *dst, size_t head, size_t tail) {
     auto &c = ctx->dev->cv;

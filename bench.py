@@ -567,7 +567,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the prover's kernels run on one device per proof; "
-              "a multi-GPU sharding is described in DESIGN.md but not built (run with --gpus 1)", file=sys.stderr)
+              "a multi-GPU sharding is designed but not built (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
     if not 1 <= args.hw_queues <= 32:
         ap.error("--hw-queues must be between 1 and 32")

@@ -3,7 +3,7 @@
 hipcc compiles every source for gfx950 only. Packed FP32 instructions are disabled for all HIP
 code (-target-feature -packed-fp32-ops), and floating point is not contracted, so the challenge
 norm keeps the reference's two roundings and one add per root. Outputs land in this package's
-build/ directory, which travels with the tree (it is git-ignored, not gpurun-ignored).
+build/ directory, inside the tree (git-ignored, so the history stays source-only).
 
     python labinius-fast-challenge_amd/build.py
 """

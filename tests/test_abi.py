@@ -65,6 +65,11 @@ def test_scheduler_lifecycle_and_errors_without_a_device():
         with pytest.raises(ValueError):
             sch.submit(20, bytes(32), out=np.empty(m.proof_len(20, m.DEFAULT_Q), dtype=np.int32))
         assert sch.stats()["submitted"] == 0
+        for bad in [(0, 8), (65, 8), (8, 0), (8, 65)]:  # continuous batching limits
+            with pytest.raises(m.LbnError):
+                sch.set_batch(*bad)
+        sch.set_batch(32, 8)
+        sch.set_batch(1, 1)
     finally:
         sch.close()
 

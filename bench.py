@@ -447,15 +447,16 @@ def serving(m, requests, rate, workers, budget):
     request. The same arrival sequence is served by `workers` proofs in flight and by one at a time."""
     reqs = serving_requests(requests, rate)
     arena = serving_arena(m, reqs)
-    check = Checker(m)
+    checks = [Checker(m), Checker(m)]  # one per way, each with its own list of rejections
     try:
-        mix, proofs_mix = serve(m, reqs, workers, budget, arena, check)
-        check.close()
-        cb, proofs_cb = serve(m, reqs, BATCH_WORKERS, budget, arena, check, batch=BATCH_MAX)
-        check.close()
+        mix, proofs_mix = serve(m, reqs, workers, budget, arena, checks[0])
+        checks[0].close()
+        cb, proofs_cb = serve(m, reqs, BATCH_WORKERS, budget, arena, checks[1], batch=BATCH_MAX)
+        checks[1].close()
         seq, proofs_seq = serve(m, reqs, 1, budget, arena)
     finally:
-        check.close()
+        for c in checks:
+            c.close()
         arena.close()
     return {"metric": "serving p50/p99 ms and proofs/s on 1 MI355X", "requests": requests, "rate_per_s": rate,
             "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")),

@@ -530,6 +530,7 @@ def commit_micro(m, steps, warmup):
 
 
 def main():
+    global BATCH_WORKERS, BATCH_MAX
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
@@ -553,6 +554,9 @@ def main():
                          "(ns) of the sequential and the parallel batch, to cut a kernel trace to them")
     ap.add_argument("--requests", type=int, default=10_000)
     ap.add_argument("--rate", type=float, default=400.0, help="serving arrivals per second")
+    ap.add_argument("--batch-workers", type=int, default=BATCH_WORKERS,
+                    help="workers of the continuous-batching serving mode")
+    ap.add_argument("--batch-max", type=int, default=BATCH_MAX, help="largest batch of that mode")
     ap.add_argument("--serving-sweep", default=None, metavar="RATES",
                     help="comma-separated arrival rates: the serving loop at each (--requests per rate, no "
                          "checker), batched and one at a time (prints its own line)")
@@ -579,6 +583,7 @@ def main():
             or args.no_process):
         proc_cpu, proc_gpu = process_runs(args.log_vars, args.cpu_seeds, args.process_seeds,
                                           not args.no_cpu_baseline)
+    BATCH_WORKERS, BATCH_MAX = args.batch_workers, args.batch_max
     import amd
     m = amd.load()
     if args.serving:

@@ -96,3 +96,14 @@ def test_pinned_allocation_arguments_without_a_device():
     L.lbn_host_free(None)
     buf = ctypes.create_string_buffer(64)
     L.lbn_host_free(ctypes.cast(buf, ctypes.c_void_p))  # not from lbn_host_alloc: ignored
+
+
+def test_bench_entry_without_a_device():
+    # the bench module compiles and its entry refuses more than one GPU before touching a device
+    import subprocess
+    import sys
+    import os
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr, r.stderr[-2000:]

@@ -406,7 +406,7 @@ def serve(m, reqs, w, budget, arena, check=None, batch=1):
     sch = m.Scheduler(m.DEFAULT_Q, hbm_budget=budget, workers=w)
     try:
         # with batching, as many contexts of the small sizes as a batch may hold (the large ones are rare)
-        warm(sch, SERVING_KS, {kk: w * batch if kk <= 23 else w for kk in SERVING_KS})
+        warm(sch, SERVING_KS, {kk: min(w * batch, 64) if kk <= 23 else w for kk in SERVING_KS})
         if batch > 1:
             sch.set_batch(batch, BATCH_THREADS)
         hs = []

@@ -389,7 +389,7 @@ def serving_requests(requests, rate, rng_seed=20261203):
 
 
 SERVING_KS = list(range(20, 32))
-BATCH_WORKERS, BATCH_MAX, BATCH_THREADS = 2, 32, 8  # continuous batching in the serving loop
+BATCH_WORKERS, BATCH_MAX, BATCH_THREADS = 3, 64, 8  # continuous batching in the serving loop
 
 
 def serving_arena(m, reqs):

@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <condition_variable>
 #include <memory>
 #include <mutex>
 #include <tuple>
@@ -4959,8 +4960,47 @@ static int swb_ready(lbn_ctx *ctx, SwBatchDev &bd) {
     HIPCHK(ctx, hipStreamCreateWithFlags(&bd.stream, hipStreamNonBlocking));  // last: it marks the set ready
     return LBN_OK;
 }
-static std::mutex swb_mu;
-static std::map<int, SwBatchDev> swb_dev;
+// Batch stream sets per device: a batched step leases one for its launches and its final wait, so
+// the steps of batches proved at once (scheduler workers, staggered groups) overlap on the device.
+constexpr int SWB_SETS = 4;
+struct SwBatchSets {
+    std::mutex m;
+    std::condition_variable cv;
+    SwBatchDev set[SWB_SETS];
+    bool busy[SWB_SETS] = {};
+};
+static std::mutex swb_map_mu;
+static std::map<int, std::unique_ptr<SwBatchSets>> swb_sets;
+static SwBatchSets &swb_sets_of(int device) {
+    std::lock_guard<std::mutex> lk(swb_map_mu);
+    auto &p = swb_sets[device];
+    if (!p) p.reset(new SwBatchSets());
+    return *p;
+}
+class BatchLease {
+  public:
+    explicit BatchLease(int device) : s_(swb_sets_of(device)) {
+        std::unique_lock<std::mutex> lk(s_.m);
+        s_.cv.wait(lk, [&] {
+            for (int i = 0; i < SWB_SETS; i++)
+                if (!s_.busy[i]) return (idx_ = i), true;
+            return false;
+        });
+        s_.busy[idx_] = true;
+    }
+    ~BatchLease() {
+        {
+            std::lock_guard<std::mutex> lk(s_.m);
+            s_.busy[idx_] = false;
+        }
+        s_.cv.notify_one();
+    }
+    SwBatchDev &bd() { return s_.set[idx_]; }
+
+  private:
+    SwBatchSets &s_;
+    int idx_ = 0;
+};
 
 // The contexts of a batch: on one device, each once.
 static int batch_check(lbn_ctx *const *ctxs, int n) {
@@ -4991,9 +5031,9 @@ int lbn::dev_switch_batch(lbn_ctx *const *ctxs, int n) {
     const int dev = ctxs[0]->device;
     for (int i = 0; i < n; i++)
         if (ctxs[i]->round != 0) return ctxs[i]->fail(LBN_E_STATE, "lbn_switch_begin must run first");
-    std::lock_guard<std::mutex> lk(swb_mu);
+    BatchLease lease(dev);
     HIPCHK(ctxs[0], hipSetDevice(dev));
-    SwBatchDev &bd = swb_dev[dev];
+    SwBatchDev &bd = lease.bd();
     {
         const int rc = swb_ready(ctxs[0], bd);
         if (rc != LBN_OK) return rc;
@@ -5284,9 +5324,9 @@ int lbn::dev_challenges_batch(lbn_ctx *const *ctxs, int n) {
         if (rc != LBN_OK) return rc;
     }
     const int dev = ctxs[0]->device;
-    std::lock_guard<std::mutex> lk(swb_mu);
+    BatchLease lease(dev);
     HIPCHK(ctxs[0], hipSetDevice(dev));
-    SwBatchDev &bd = swb_dev[dev];
+    SwBatchDev &bd = lease.bd();
     {
         const int rc = swb_ready(ctxs[0], bd);
         if (rc != LBN_OK) return rc;
@@ -5320,9 +5360,9 @@ int lbn::dev_row_batch(lbn_ctx *const *ctxs, int n) {
         const int rc = batch_check(ctxs, n);
         if (rc != LBN_OK) return rc;
     }
-    std::lock_guard<std::mutex> lk(swb_mu);
+    BatchLease lease(ctxs[0]->device);
     HIPCHK(ctxs[0], hipSetDevice(ctxs[0]->device));
-    SwBatchDev &bd = swb_dev[ctxs[0]->device];
+    SwBatchDev &bd = lease.bd();
     {
         const int rc = swb_ready(ctxs[0], bd);
         if (rc != LBN_OK) return rc;
@@ -5359,10 +5399,10 @@ int lbn::dev_fold_batch(lbn_ctx *const *ctxs, int n, const FoldPlan *plans) {
         const int rc = batch_check(ctxs, n);
         if (rc != LBN_OK) return rc;
     }
-    std::lock_guard<std::mutex> lk(swb_mu);
     const int dev = ctxs[0]->device;
+    BatchLease lease(dev);
     HIPCHK(ctxs[0], hipSetDevice(dev));
-    SwBatchDev &bd = swb_dev[dev];
+    SwBatchDev &bd = lease.bd();
     {
         const int rc = swb_ready(ctxs[0], bd);
         if (rc != LBN_OK) return rc;
@@ -5415,29 +5455,33 @@ int lbn::dev_fold_batch(lbn_ctx *const *ctxs, int n, const FoldPlan *plans) {
 }
 
 void lbn::dev_batch_quiesce(int device) {
-    std::lock_guard<std::mutex> lk(swb_mu);
-    auto it = swb_dev.find(device);
-    if (it == swb_dev.end() || !it->second.stream) return;
+    SwBatchSets &sets = swb_sets_of(device);
+    std::vector<hipStream_t> st;
+    {
+        std::lock_guard<std::mutex> lk(sets.m);
+        for (auto &b : sets.set)
+            if (b.stream) st.push_back(b.stream);
+    }
     (void)hipSetDevice(device);
-    (void)hipStreamSynchronize(it->second.stream);
+    for (hipStream_t x : st) (void)hipStreamSynchronize(x);
 }
 
 // The batched steps' common start: the contexts checked, the device current and its batch buffers.
-static int batch_begin(lbn_ctx *const *ctxs, int n, SwBatchDev *&bd) {
-    {
-        const int rc = batch_check(ctxs, n);
-        if (rc != LBN_OK) return rc;
-    }
+static int batch_begin(lbn_ctx *const *ctxs, int n, BatchLease &lease, SwBatchDev *&bd) {
     HIPCHK(ctxs[0], hipSetDevice(ctxs[0]->device));
-    bd = &swb_dev[ctxs[0]->device];
+    bd = &lease.bd();
     return swb_ready(ctxs[0], *bd);
 }
 
 int lbn::dev_cvs_batch(lbn_ctx *const *ctxs, int n) {
-    std::lock_guard<std::mutex> lk(swb_mu);
+    {
+        const int rc = batch_check(ctxs, n);
+        if (rc != LBN_OK) return rc;
+    }
+    BatchLease lease(ctxs[0]->device);
     SwBatchDev *bd = nullptr;
     {
-        const int rc = batch_begin(ctxs, n, bd);
+        const int rc = batch_begin(ctxs, n, lease, bd);
         if (rc != LBN_OK) return rc;
     }
     std::vector<CvView> hv(n);
@@ -5484,10 +5528,14 @@ int lbn::dev_cvs_batch(lbn_ctx *const *ctxs, int n) {
 }
 
 int lbn::dev_partials_batch(lbn_ctx *const *ctxs, int n) {
-    std::lock_guard<std::mutex> lk(swb_mu);
+    {
+        const int rc = batch_check(ctxs, n);
+        if (rc != LBN_OK) return rc;
+    }
+    BatchLease lease(ctxs[0]->device);
     SwBatchDev *bd = nullptr;
     {
-        const int rc = batch_begin(ctxs, n, bd);
+        const int rc = batch_begin(ctxs, n, lease, bd);
         if (rc != LBN_OK) return rc;
     }
     std::vector<PartView> hv(n);

@@ -814,7 +814,7 @@ extern "C" int lbn_prove(lbn_ctx *ctx, const uint8_t seed[32], uint8_t *proof, s
 }
 
 namespace {
-constexpr int BATCH_GROUPS = 2;     // a batch runs as up to this many staggered groups
+constexpr int BATCH_GROUPS = 3;     // a batch runs as up to this many staggered groups
 constexpr int BATCH_GROUP_MIN = 8;  // of at least this many proofs each
 // The batch's host threads, started once per batch: run(f) calls f(order[k]) for every k, on the
 // threads and the caller, and returns when every call has returned.

@@ -85,7 +85,7 @@ static void scheduler_stress(int max_batch) {
 // The batch prover's phases on its host threads: every item's proof equals the same context's solo
 // proof (the stub's outputs are deterministic), over batches of several sizes and thread counts.
 static void batch_stress() {
-    const int n = 20;  // the full batch runs as two groups, the half batch as one
+    const int n = 24;  // the full batch runs as up to three groups, the half batch as one
     std::vector<lbn_ctx *> ctx(n, nullptr);
     for (int i = 0; i < n; i++) {
         CHECK(lbn_create(0, 19 + i % 4, &ctx[i]) == LBN_OK);

@@ -4950,13 +4950,14 @@ static int gather_launch(lbn_ctx *const *ctxs, int n, SwBatchDev &bd, size_t reg
 }
 static int swb_ready(lbn_ctx *ctx, SwBatchDev &bd) {
     if (bd.stream) return LBN_OK;
-    HIPCHK(ctx, hipMalloc(&bd.views, sizeof(SwView) * SWB_MAX));
-    HIPCHK(ctx, hipMalloc(&bd.chal, sizeof(ChalPtrs) * SWB_MAX));
-    HIPCHK(ctx, hipMalloc(&bd.rows, sizeof(RowView) * SWB_MAX));
-    HIPCHK(ctx, hipMalloc(&bd.folds, sizeof(FoldView) * SWB_MAX));
-    HIPCHK(ctx, hipMalloc(&bd.cvs, sizeof(CvView) * SWB_MAX));
-    HIPCHK(ctx, hipMalloc(&bd.parts, sizeof(PartView) * SWB_MAX));
-    HIPCHK(ctx, hipMalloc(&bd.gather, sizeof(Gather) * SWB_MAX));
+    // a set left half made by a failed allocation is completed by the next call, not allocated again
+    if (!bd.views) HIPCHK(ctx, hipMalloc(&bd.views, sizeof(SwView) * SWB_MAX));
+    if (!bd.chal) HIPCHK(ctx, hipMalloc(&bd.chal, sizeof(ChalPtrs) * SWB_MAX));
+    if (!bd.rows) HIPCHK(ctx, hipMalloc(&bd.rows, sizeof(RowView) * SWB_MAX));
+    if (!bd.folds) HIPCHK(ctx, hipMalloc(&bd.folds, sizeof(FoldView) * SWB_MAX));
+    if (!bd.cvs) HIPCHK(ctx, hipMalloc(&bd.cvs, sizeof(CvView) * SWB_MAX));
+    if (!bd.parts) HIPCHK(ctx, hipMalloc(&bd.parts, sizeof(PartView) * SWB_MAX));
+    if (!bd.gather) HIPCHK(ctx, hipMalloc(&bd.gather, sizeof(Gather) * SWB_MAX));
     HIPCHK(ctx, hipStreamCreateWithFlags(&bd.stream, hipStreamNonBlocking));  // last: it marks the set ready
     return LBN_OK;
 }

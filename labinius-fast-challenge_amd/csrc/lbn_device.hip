@@ -1158,10 +1158,10 @@ struct CtlTemp {
     uint64_t tabL[32][16][3];
     uint64_t Tt[2][192][2];
     uint64_t rbas[162][3];
-    uint64_t lpart[4][128][3];
+    uint64_t lpart[8][128][3];
 };
 
-// One control step for round r (at least 768 threads), on the transcript K.st and map K.L in
+// One control step for round r (1024 threads), on the transcript K.st and map K.L in
 // LDS; the message comes from T, the round's records and the challenge go to sd.
 __device__ __forceinline__ void switch_ctl_step(CtlKeep &K, CtlTemp &X, SwitchDev *sd, int r, int V,
                                                 const uint32_t *T) {
@@ -1319,31 +1319,40 @@ __device__ __forceinline__ void switch_ctl_step(CtlKeep &K, CtlTemp &X, SwitchDe
             for (int k = 0; k < 3; k++) tabR41[w][nib][k] = v[k];
         }
         __syncthreads();
-        if (t < 512) {  // k = t % 128; four threads per k: two halves of each table walk
-            const int k = t & 127, part4 = t >> 7;
+        {  // k = t % 128; eight threads per k: quarters of each table walk
+            const int k = t & 127, part = t >> 7;
             uint64_t acc[3] = {0, 0, 0};
-            if (part4 < 2) {
+            if (part < 4) {  // (1 + c) x^k through the tables of L, windows 8 part .. 8 part + 7
                 const u128 one_c = (((u128)sd->r_hi[r][1] << 64) | sd->r_hi[r][0]) ^ (u128)1;
-                const u128 z = b128_mulx_pow(one_c, k);  // (1 + c) x^k
+                const u128 z = b128_mulx_pow(one_c, k);
 #pragma unroll
-                for (int i = 0; i < 16; i++) {
-                    const int w = 16 * part4 + i;
+                for (int i = 0; i < 8; i++) {
+                    const int w = 8 * part + i;
                     const int nib = (int)((z >> (4 * w)) & 15);
+#pragma unroll
                     for (int j = 0; j < 3; j++) acc[j] ^= tabL[w][nib][j];
                 }
-            } else {  // rho * L[k], windows 0..20 or 21..40
-                const int w0 = part4 == 2 ? 0 : 21, w1 = part4 == 2 ? 21 : 41;
-                for (int w = w0; w < w1; w++) {
-                    const int nib = (int)((L[k][w >> 4] >> (4 * (w & 15))) & 15);
-                    for (int j = 0; j < 3; j++) acc[j] ^= tabR41[w][nib][j];
-                }
+            } else {  // rho * L[k], windows 0..10, 11..20, 21..30 or 31..40
+                const int w0 = part == 4 ? 0 : 10 * (part - 4) + 1, n = part == 4 ? 11 : 10;
+#pragma unroll
+                for (int i = 0; i < 11; i++)
+                    if (i < n) {
+                        const int w = w0 + i;
+                        const int nib = (int)((L[k][w >> 4] >> (4 * (w & 15))) & 15);
+#pragma unroll
+                        for (int j = 0; j < 3; j++) acc[j] ^= tabR41[w][nib][j];
+                    }
             }
-            for (int j = 0; j < 3; j++) lpart[part4][k][j] = acc[j];
+#pragma unroll
+            for (int j = 0; j < 3; j++) lpart[part][k][j] = acc[j];
         }
         __syncthreads();
         if (t < 384) {
             const int k = t / 3, j = t % 3;
-            L[k][j] = lpart[0][k][j] ^ lpart[1][k][j] ^ lpart[2][k][j] ^ lpart[3][k][j];
+            uint64_t x = 0;
+#pragma unroll
+            for (int i = 0; i < 8; i++) x ^= lpart[i][k][j];
+            L[k][j] = x;
         }
     }
     __syncthreads();
